@@ -1,0 +1,374 @@
+"""ctypes wrapper of the C oracle (oracle/build/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, always as the checker.  The product library
+never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "build", "liborc.so")
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def _load():
+    if not os.path.exists(_LIB):
+        build()
+    return C.CDLL(_LIB)
+
+
+lib = _load()
+
+
+class Peclet(C.Structure):
+    _fields_ = [("form", C.c_int), ("hybrid", C.c_double), ("c1", C.c_double),
+                ("c2", C.c_double)]
+
+
+class ScalarOpts(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("alpha_upw", C.c_double),
+                ("upw", C.c_double), ("relax", C.c_double),
+                ("use_limiter", C.c_int), ("peclet", Peclet)]
+
+
+class MomentumOpts(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("alpha_upw", C.c_double),
+                ("upw", C.c_double), ("relax", C.c_double),
+                ("include_divU", C.c_double), ("use_limiter", C.c_int)]
+
+
+class ContinuityOpts(C.Structure):
+    _fields_ = [("dt", C.c_double), ("gamma1", C.c_double), ("noc", C.c_double),
+                ("interp_together", C.c_double),
+                ("incompressible", C.c_double)]
+
+
+class Csr(C.Structure):
+    _fields_ = [("nnode", C.c_int64), ("ndof", C.c_int), ("nrows", C.c_int64),
+                ("nnz", C.c_int64), ("row_ptr", C.POINTER(C.c_int64)),
+                ("cols", C.POINTER(C.c_int32)), ("vals", C.POINTER(C.c_double)),
+                ("rhs", C.POINTER(C.c_double))]
+
+
+class GmresOpts(C.Structure):
+    _fields_ = [("kspace", C.c_int), ("max_iters", C.c_int), ("tol", C.c_double),
+                ("precond", C.c_int), ("cheb_degree", C.c_int),
+                ("cheb_ratio", C.c_double)]
+
+
+class GmresStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("implicit_rel", C.c_double),
+                ("final_abs", C.c_double), ("bnorm", C.c_double),
+                ("converged", C.c_int)]
+
+
+PREC_NONE, PREC_JACOBI, PREC_CHEBYSHEV = 0, 1, 2
+
+lib.orc_hex_scs_areav.argtypes = [_f64p, _f64p]
+lib.orc_hex_scv_volume.argtypes = [_f64p, _f64p]
+lib.orc_create_edges_hex.argtypes = [C.c_int64, _i32p, C.c_int64, _i32p,
+                                     C.c_int64, _i32p]
+lib.orc_create_edges_hex.restype = C.c_int64
+lib.orc_edge_area_vec_hex.argtypes = [C.c_int64, _i32p, _f64p, _i32p, _i32p,
+                                      C.c_int64, _f64p]
+lib.orc_dual_volume_hex.argtypes = [C.c_int64, _i32p, _f64p, C.c_int64, _f64p]
+lib.orc_nodal_grad_edge.argtypes = [C.c_int64, C.c_int64, _i32p, _f64p, _f64p,
+                                    C.c_int, _f64p, _f64p]
+lib.orc_mdot_edge.argtypes = [C.c_int64, _i32p, _f64p, _f64p, _f64p, _f64p]
+lib.orc_peclet_classic.argtypes = [C.c_double, C.c_double]
+lib.orc_peclet_classic.restype = C.c_double
+lib.orc_peclet_tanh.argtypes = [C.c_double, C.c_double, C.c_double]
+lib.orc_peclet_tanh.restype = C.c_double
+lib.orc_van_leer.argtypes = [C.c_double, C.c_double, C.c_double]
+lib.orc_van_leer.restype = C.c_double
+lib.orc_momentum_edge_peclet.argtypes = [C.c_int64, _i32p, _f64p, _f64p, _f64p,
+                                         _f64p, C.POINTER(Peclet), C.c_void_p,
+                                         _f64p]
+lib.orc_scalar_edge.argtypes = [C.POINTER(ScalarOpts), C.c_int64, C.c_int32,
+                                C.c_int32] + [_f64p] * 8 + [_f64p, _f64p]
+lib.orc_momentum_edge.argtypes = [C.POINTER(MomentumOpts), C.c_int64,
+                                  C.c_int32, C.c_int32] + [_f64p] * 9 + [_f64p,
+                                                                         _f64p]
+lib.orc_continuity_edge.argtypes = [C.POINTER(ContinuityOpts), C.c_int64,
+                                    C.c_int32, C.c_int32] + [_f64p] * 7 + [
+                                        _f64p, _f64p]
+lib.orc_csr_build_edge_graph.argtypes = [C.c_int64, C.c_int64, _i32p, C.c_int,
+                                         C.POINTER(Csr)]
+lib.orc_csr_free.argtypes = [C.POINTER(Csr)]
+lib.orc_csr_sum_into.argtypes = [C.POINTER(Csr), C.c_int, _i32p, _f64p, _f64p]
+lib.orc_csr_reset_rows.argtypes = [C.POINTER(Csr), C.c_int64, _i32p, C.c_int,
+                                   C.c_int, C.c_double, C.c_double]
+lib.orc_csr_apply_dirichlet.argtypes = [C.POINTER(Csr), C.c_int64, _i32p,
+                                        C.c_int, C.c_int, _f64p, _f64p, C.c_int]
+lib.orc_csr_fix_node.argtypes = [C.POINTER(Csr), C.c_int32, C.c_double,
+                                 C.c_double]
+_dense = C.c_void_p
+lib.orc_assemble_scalar.argtypes = [C.POINTER(ScalarOpts), C.c_int64,
+                                    C.c_int64, _i32p] + [_f64p] * 8 + [
+                                        C.c_void_p, _dense, _dense]
+lib.orc_assemble_momentum.argtypes = [C.POINTER(MomentumOpts), C.c_int64,
+                                      C.c_int64, _i32p] + [_f64p] * 9 + [
+                                          C.c_void_p, _dense, _dense]
+lib.orc_assemble_continuity.argtypes = [C.POINTER(ContinuityOpts), C.c_int64,
+                                        C.c_int64, _i32p] + [_f64p] * 7 + [
+                                            C.c_void_p, _dense, _dense]
+lib.orc_spmv.argtypes = [C.POINTER(Csr), _f64p, _f64p]
+lib.orc_gmres.argtypes = [C.POINTER(Csr), _f64p, _f64p, C.POINTER(GmresOpts),
+                          C.POINTER(GmresStats)]
+lib.orc_gmres.restype = C.c_int
+lib.orc_dense_solve.argtypes = [C.c_int, _f64p, _f64p]
+lib.orc_dense_solve.restype = C.c_int
+lib.orc_jacobi_lambda_max.argtypes = [C.POINTER(Csr), C.c_int, C.c_uint64]
+lib.orc_jacobi_lambda_max.restype = C.c_double
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- geometry
+
+def hex_scs_areav(coords8):
+    out = np.zeros(36)
+    lib.orc_hex_scs_areav(f64(coords8).ravel(), out)
+    return out.reshape(12, 3)
+
+
+def hex_scv_volume(coords8):
+    out = np.zeros(8)
+    lib.orc_hex_scv_volume(f64(coords8).ravel(), out)
+    return out
+
+
+def create_edges_hex(conn, nnode):
+    conn = i32(conn)
+    nelem = conn.size // 8
+    cap = 3 * nnode + 12 * nelem
+    edges = np.zeros(2 * cap, np.int32)
+    elem_edges = np.zeros(12 * nelem, np.int32)
+    ne = lib.orc_create_edges_hex(nelem, conn.ravel(), nnode, edges, cap,
+                                  elem_edges)
+    assert ne >= 0
+    return edges[:2 * ne].reshape(ne, 2).copy(), elem_edges.reshape(nelem, 12)
+
+
+def edge_area_vec_hex(conn, coords, elem_edges, edges):
+    conn = i32(conn)
+    out = np.zeros((edges.shape[0], 3))
+    lib.orc_edge_area_vec_hex(conn.size // 8, conn.ravel(), f64(coords).ravel(),
+                              i32(elem_edges).ravel(), i32(edges).ravel(),
+                              edges.shape[0], out)
+    return out
+
+
+def dual_volume_hex(conn, coords):
+    conn = i32(conn)
+    coords = f64(coords)
+    out = np.zeros(coords.shape[0])
+    lib.orc_dual_volume_hex(conn.size // 8, conn.ravel(), coords.ravel(),
+                            coords.shape[0], out)
+    return out
+
+
+def nodal_grad_edge(edges, areav, dualvol, phi):
+    phi = f64(phi)
+    n = dualvol.shape[0]
+    dim1 = 1 if phi.ndim == 1 else phi.shape[1]
+    out = np.zeros((n, dim1 * 3))
+    lib.orc_nodal_grad_edge(n, edges.shape[0], i32(edges).ravel(),
+                            f64(areav).ravel(), f64(dualvol), dim1,
+                            phi.ravel(), out)
+    return out
+
+
+def mdot_edge(edges, areav, rho, vel):
+    out = np.zeros(edges.shape[0])
+    lib.orc_mdot_edge(edges.shape[0], i32(edges).ravel(), f64(areav).ravel(),
+                      f64(rho), f64(vel).ravel(), out)
+    return out
+
+
+def momentum_edge_peclet(edges, coords, vel, rho, visc, peclet):
+    out = np.zeros(edges.shape[0])
+    lib.orc_momentum_edge_peclet(edges.shape[0], i32(edges).ravel(),
+                                 f64(coords).ravel(), f64(vel).ravel(),
+                                 f64(rho), f64(visc), C.byref(peclet), None,
+                                 out)
+    return out
+
+
+# ---------------------------------------------------------------- CSR
+
+class CsrSystem:
+    """Owns an orc_csr built from an edge graph."""
+
+    def __init__(self, nnode, edges, ndof):
+        self.c = Csr()
+        rc = lib.orc_csr_build_edge_graph(nnode, edges.shape[0],
+                                          i32(edges).ravel(), ndof,
+                                          C.byref(self.c))
+        assert rc == 0
+
+    def __del__(self):
+        try:
+            lib.orc_csr_free(C.byref(self.c))
+        except Exception:
+            pass
+
+    @property
+    def ref(self):
+        return C.byref(self.c)
+
+    def _view(self, ptr, n, dt):
+        return np.ctypeslib.as_array(ptr, shape=(n,)).view(dt)
+
+    @property
+    def row_ptr(self):
+        return np.ctypeslib.as_array(self.c.row_ptr, shape=(self.c.nrows + 1,))
+
+    @property
+    def cols(self):
+        return np.ctypeslib.as_array(self.c.cols, shape=(self.c.nnz,))
+
+    @property
+    def vals(self):
+        return np.ctypeslib.as_array(self.c.vals, shape=(self.c.nnz,))
+
+    @property
+    def rhs(self):
+        return np.ctypeslib.as_array(self.c.rhs, shape=(self.c.nrows,))
+
+    def dense(self):
+        n = self.c.nrows
+        D = np.zeros((n, n))
+        rp, cols, vals = self.row_ptr, self.cols, self.vals
+        for r in range(n):
+            for k in range(rp[r], rp[r + 1]):
+                D[r, cols[k]] += vals[k]
+        return D
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        n = self.c.nrows
+        return sp.csr_matrix((self.vals.copy(), self.cols.copy(),
+                              self.row_ptr.copy()), shape=(n, n))
+
+    def sum_into(self, nodes, lhs, rhs):
+        lib.orc_csr_sum_into(self.ref, len(nodes), i32(nodes), f64(lhs).ravel(),
+                             f64(rhs))
+
+    def reset_rows(self, nodes, begin, end, diag=0.0, rhs_res=0.0):
+        lib.orc_csr_reset_rows(self.ref, len(nodes), i32(nodes), begin, end,
+                               diag, rhs_res)
+
+    def apply_dirichlet(self, nodes, begin, end, sol, bc):
+        sol = f64(sol)
+        ncomp = 1 if sol.ndim == 1 else sol.shape[1]
+        lib.orc_csr_apply_dirichlet(self.ref, len(nodes), i32(nodes), begin,
+                                    end, sol.ravel(), f64(bc).ravel(), ncomp)
+
+    def fix_node(self, node, pref, pnode):
+        lib.orc_csr_fix_node(self.ref, int(node), pref, pnode)
+
+    def spmv(self, x):
+        y = np.zeros(self.c.nrows)
+        lib.orc_spmv(self.ref, f64(x), y)
+        return y
+
+    def gmres(self, b, x0=None, kspace=50, max_iters=1000, tol=1e-8,
+              precond=PREC_JACOBI, cheb_degree=4, cheb_ratio=30.0):
+        x = np.zeros(self.c.nrows) if x0 is None else f64(x0).copy()
+        o = GmresOpts(kspace, max_iters, tol, precond, cheb_degree, cheb_ratio)
+        st = GmresStats()
+        lib.orc_gmres(self.ref, f64(b), x, C.byref(o), C.byref(st))
+        return x, st
+
+    def jacobi_lambda_max(self, iters=30, seed=12345):
+        return lib.orc_jacobi_lambda_max(self.ref, iters, seed)
+
+
+def dense_solve(A, b):
+    A = f64(A).copy()
+    x = f64(b).copy()
+    rc = lib.orc_dense_solve(A.shape[0], A.ravel(), x)
+    assert rc == 0
+    return x
+
+
+# ---------------------------------------------------------------- assembly
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def assemble_scalar(opts, nnode, edges, coords, vrtm, q, dqdx, rho, dflux,
+                    areav, mdot, csr=None):
+    """Returns csr (if given) or a dense (lhs, rhs) from the unit-test mock."""
+    ld = rd = None
+    if csr is None:
+        ld = np.zeros((nnode, nnode))
+        rd = np.zeros(nnode)
+    lib.orc_assemble_scalar(C.byref(opts), nnode, edges.shape[0],
+                            i32(edges).ravel(), f64(coords).ravel(),
+                            f64(vrtm).ravel(), f64(q), f64(dqdx).ravel(),
+                            f64(rho), f64(dflux), f64(areav).ravel(), f64(mdot),
+                            csr.ref if csr is not None else None, _ptr(ld),
+                            _ptr(rd))
+    return csr if csr is not None else (ld, rd)
+
+
+def assemble_momentum(opts, nnode, edges, coords, vel, dudx, visc, rho, areav,
+                      mdot, pecfac, wallmask, csr=None):
+    ld = rd = None
+    if csr is None:
+        ld = np.zeros((3 * nnode, 3 * nnode))
+        rd = np.zeros(3 * nnode)
+    lib.orc_assemble_momentum(C.byref(opts), nnode, edges.shape[0],
+                              i32(edges).ravel(), f64(coords).ravel(),
+                              f64(vel).ravel(), f64(dudx).ravel(), f64(visc),
+                              f64(rho), f64(areav).ravel(), f64(mdot),
+                              f64(pecfac), f64(wallmask),
+                              csr.ref if csr is not None else None, _ptr(ld),
+                              _ptr(rd))
+    return csr if csr is not None else (ld, rd)
+
+
+def assemble_continuity(opts, nnode, edges, coords, vel, dpdx, rho, p, udiag,
+                        areav, csr=None):
+    ld = rd = None
+    if csr is None:
+        ld = np.zeros((nnode, nnode))
+        rd = np.zeros(nnode)
+    lib.orc_assemble_continuity(C.byref(opts), nnode, edges.shape[0],
+                                i32(edges).ravel(), f64(coords).ravel(),
+                                f64(vel).ravel(), f64(dpdx).ravel(), f64(rho),
+                                f64(p), f64(udiag), f64(areav).ravel(),
+                                csr.ref if csr is not None else None, _ptr(ld),
+                                _ptr(rd))
+    return csr if csr is not None else (ld, rd)
+
+
+def peclet_classic(hf, pe):
+    return lib.orc_peclet_classic(hf, pe)
+
+
+def peclet_tanh(c1, c2, x):
+    return lib.orc_peclet_tanh(c1, c2, x)
+
+
+def van_leer(a, b, eps=1e-16):
+    return lib.orc_van_leer(a, b, eps)

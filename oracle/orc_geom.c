@@ -1,0 +1,261 @@
+/*
+ * orc_geom.c -- oracle restatement of the Hex8 CVFEM geometry and of the
+ * edge/node input producers (edge area vectors, dual volumes, edge nodal
+ * gradient, edge mass flow rate).  Test infrastructure only; see orc.h.
+ */
+#include "orc.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* include/master_element/Hex8CVFEM.h:292-293 */
+const int orc_hex_lrscv[24] = {0, 1, 1, 2, 2, 3, 0, 3, 4, 5, 5, 6,
+                               6, 7, 4, 7, 0, 4, 1, 5, 2, 6, 3, 7};
+
+/* STK Hex8 edge node ordinals (edge k joins these element-local nodes). */
+static const int hex_edge_nodes[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0},
+                                          {4, 5}, {5, 6}, {6, 7}, {7, 4},
+                                          {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+/* subdivide_hex_8, include/master_element/Hex8GeometryFunctions.h:258-330:
+ * the 8 corners, 12 edge midpoints, 6 face centroids and the centroid, in
+ * the face-by-face order that the SCS/SCV tables below index. */
+static void subdivide_hex_8(const double c[24], double v[27][3])
+{
+  for (int n = 0; n < 8; ++n)
+    for (int d = 0; d < 3; ++d) v[n][d] = c[3 * n + d];
+#define C(n, d) c[3 * (n) + (d)]
+  for (int d = 0; d < 3; ++d) {
+    v[8][d] = 0.5 * (C(0, d) + C(1, d));
+    v[9][d] = 0.5 * (C(1, d) + C(2, d));
+    v[10][d] = 0.5 * (C(2, d) + C(3, d));
+    v[11][d] = 0.5 * (C(3, d) + C(0, d));
+    v[12][d] = 0.25 * (C(0, d) + C(1, d) + C(2, d) + C(3, d));
+    v[13][d] = 0.5 * (C(4, d) + C(5, d));
+    v[14][d] = 0.5 * (C(5, d) + C(6, d));
+    v[15][d] = 0.5 * (C(6, d) + C(7, d));
+    v[16][d] = 0.5 * (C(7, d) + C(4, d));
+    v[17][d] = 0.25 * (C(4, d) + C(5, d) + C(6, d) + C(7, d));
+    v[18][d] = 0.5 * (C(1, d) + C(5, d));
+    v[19][d] = 0.5 * (C(0, d) + C(4, d));
+    v[20][d] = 0.25 * (C(0, d) + C(1, d) + C(4, d) + C(5, d));
+    v[21][d] = 0.5 * (C(3, d) + C(7, d));
+    v[22][d] = 0.5 * (C(2, d) + C(6, d));
+    v[23][d] = 0.25 * (C(2, d) + C(3, d) + C(6, d) + C(7, d));
+    v[24][d] = 0.25 * (C(1, d) + C(2, d) + C(5, d) + C(6, d));
+    v[25][d] = 0.25 * (C(0, d) + C(3, d) + C(4, d) + C(7, d));
+    double s = 0.0;
+    for (int n = 0; n < 8; ++n) s += C(n, d);
+    v[26][d] = s * 0.125;
+  }
+#undef C
+}
+
+/* quad_area_by_triangulation, Hex8GeometryFunctions.h:36-83: four
+ * triangles about the quad centroid (Grandy). */
+static void quad_area(const double q[4][3], double area[3])
+{
+  double xm[3];
+  for (int d = 0; d < 3; ++d)
+    xm[d] = 0.25 * (q[0][d] + q[1][d] + q[2][d] + q[3][d]);
+  area[0] = area[1] = area[2] = 0.0;
+  double r1[3] = {q[0][0] - xm[0], q[0][1] - xm[1], q[0][2] - xm[2]};
+  for (int t = 0; t < 4; ++t) {
+    const int k = (t + 1) % 4;
+    const double r2[3] = {q[k][0] - xm[0], q[k][1] - xm[1], q[k][2] - xm[2]};
+    area[0] += r1[1] * r2[2] - r2[1] * r1[2];
+    area[1] += r1[2] * r2[0] - r2[2] * r1[0];
+    area[2] += r1[0] * r2[1] - r2[0] * r1[1];
+    r1[0] = r2[0];
+    r1[1] = r2[1];
+    r1[2] = r2[2];
+  }
+  area[0] *= 0.5;
+  area[1] *= 0.5;
+  area[2] *= 0.5;
+}
+
+/* HexSCS::determinant_scs, src/master_element/Hex8CVFEM.C:567-591 */
+void orc_hex_scs_areav(const double coords[24], double areav[36])
+{
+  static const int facet[12][4] = {
+    {20, 8, 12, 26},  {24, 9, 12, 26},  {10, 12, 26, 23}, {11, 25, 26, 12},
+    {13, 20, 26, 17}, {17, 14, 24, 26}, {17, 15, 23, 26}, {16, 17, 26, 25},
+    {19, 20, 26, 25}, {20, 18, 24, 26}, {22, 23, 26, 24}, {21, 25, 26, 23}};
+  double v[27][3];
+  subdivide_hex_8(coords, v);
+  for (int ics = 0; ics < 12; ++ics) {
+    double q[4][3];
+    for (int i = 0; i < 4; ++i)
+      for (int d = 0; d < 3; ++d) q[i][d] = v[facet[ics][i]][d];
+    quad_area(q, &areav[3 * ics]);
+  }
+}
+
+/* hex_volume_grandy, Hex8GeometryFunctions.h:85-160 */
+static double hex_volume_grandy(const double h[8][3])
+{
+  static const int face_nodes[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7},
+                                       {0, 1, 5, 4}, {2, 3, 7, 6},
+                                       {1, 2, 6, 5}, {0, 4, 3, 7}};
+  static const int tri[24][3] = {
+    {0, 8, 1},  {8, 2, 1},  {3, 2, 8},  {3, 8, 0},  {6, 9, 5},  {7, 9, 6},
+    {4, 9, 7},  {4, 5, 9},  {10, 0, 1}, {5, 10, 1}, {4, 10, 5}, {4, 0, 10},
+    {7, 6, 11}, {6, 2, 11}, {2, 3, 11}, {3, 7, 11}, {6, 12, 2}, {5, 12, 6},
+    {5, 1, 12}, {1, 2, 12}, {0, 4, 13}, {4, 7, 13}, {7, 3, 13}, {3, 0, 13}};
+  double cv[14][3];
+  for (int n = 0; n < 8; ++n)
+    for (int d = 0; d < 3; ++d) cv[n][d] = h[n][d];
+  for (int k = 0; k < 6; ++k)
+    for (int d = 0; d < 3; ++d)
+      cv[8 + k][d] = 0.25 * (cv[face_nodes[k][0]][d] + cv[face_nodes[k][1]][d] +
+                             cv[face_nodes[k][2]][d] + cv[face_nodes[k][3]][d]);
+  double vol = 0.0;
+  for (int k = 0; k < 24; ++k) {
+    const int p = tri[k][0], q = tri[k][1], r = tri[k][2];
+    const double m[3] = {cv[p][0] + cv[q][0] + cv[r][0],
+                         cv[p][1] + cv[q][1] + cv[r][1],
+                         cv[p][2] + cv[q][2] + cv[r][2]};
+    double dx[3];
+    dx[0] = (cv[q][1] - cv[p][1]) * (cv[r][2] - cv[p][2]) -
+            (cv[r][1] - cv[p][1]) * (cv[q][2] - cv[p][2]);
+    dx[1] = (cv[r][0] - cv[p][0]) * (cv[q][2] - cv[p][2]) -
+            (cv[q][0] - cv[p][0]) * (cv[r][2] - cv[p][2]);
+    dx[2] = (cv[q][0] - cv[p][0]) * (cv[r][1] - cv[p][1]) -
+            (cv[r][0] - cv[p][0]) * (cv[q][1] - cv[p][1]);
+    vol += m[0] * dx[0] + m[1] * dx[1] + m[2] * dx[2];
+  }
+  return vol / 18.0;
+}
+
+/* HexSCV::determinant_scv, src/master_element/Hex8CVFEM.C:365-389 */
+void orc_hex_scv_volume(const double coords[24], double vol[8])
+{
+  static const int sub[8][8] = {
+    {0, 8, 12, 11, 19, 20, 26, 25},  {8, 1, 9, 12, 20, 18, 24, 26},
+    {12, 9, 2, 10, 26, 24, 22, 23},  {11, 12, 10, 3, 25, 26, 23, 21},
+    {19, 20, 26, 25, 4, 13, 17, 16}, {20, 18, 24, 26, 13, 5, 14, 17},
+    {26, 24, 22, 23, 17, 14, 6, 15}, {25, 26, 23, 21, 16, 17, 15, 7}};
+  double v[27][3];
+  subdivide_hex_8(coords, v);
+  for (int ip = 0; ip < 8; ++ip) {
+    double h[8][3];
+    for (int n = 0; n < 8; ++n)
+      for (int d = 0; d < 3; ++d) h[n][d] = v[sub[ip][n]][d];
+    vol[ip] = hex_volume_grandy(h);
+  }
+}
+
+int64_t orc_create_edges_hex(int64_t nelem, const int32_t* conn,
+                             int64_t nnode, int32_t* edges, int64_t edges_cap,
+                             int32_t* elem_edges)
+{
+  /* per-node singly linked list of edges whose node[0] is that node */
+  int64_t* head = (int64_t*)malloc(sizeof(int64_t) * (size_t)nnode);
+  int64_t* next = (int64_t*)malloc(sizeof(int64_t) * (size_t)edges_cap);
+  for (int64_t i = 0; i < nnode; ++i) head[i] = -1;
+  int64_t ne = 0;
+  for (int64_t el = 0; el < nelem; ++el) {
+    for (int k = 0; k < 12; ++k) {
+      int32_t a = conn[8 * el + hex_edge_nodes[k][0]];
+      int32_t b = conn[8 * el + hex_edge_nodes[k][1]];
+      if (b < a) {
+        int32_t t = a;
+        a = b;
+        b = t;
+      }
+      int64_t found = -1;
+      for (int64_t it = head[a]; it >= 0; it = next[it])
+        if (edges[2 * it + 1] == b) {
+          found = it;
+          break;
+        }
+      if (found < 0) {
+        if (ne >= edges_cap) {
+          free(head);
+          free(next);
+          return -1;
+        }
+        edges[2 * ne] = a;
+        edges[2 * ne + 1] = b;
+        next[ne] = head[a];
+        head[a] = ne;
+        found = ne++;
+      }
+      elem_edges[12 * el + k] = (int32_t)found;
+    }
+  }
+  free(head);
+  free(next);
+  return ne;
+}
+
+void orc_edge_area_vec_hex(int64_t nelem, const int32_t* conn,
+                           const double* coords, const int32_t* elem_edges,
+                           const int32_t* edges, int64_t nedge, double* areav)
+{
+  memset(areav, 0, sizeof(double) * 3 * (size_t)nedge);
+  for (int64_t el = 0; el < nelem; ++el) {
+    double c[24], a[36];
+    for (int n = 0; n < 8; ++n)
+      for (int d = 0; d < 3; ++d) c[3 * n + d] = coords[3 * conn[8 * el + n] + d];
+    orc_hex_scs_areav(c, a);
+    for (int ip = 0; ip < 12; ++ip) {
+      const int32_t e = elem_edges[12 * el + ip]; /* scsIpEdgeOrd = identity */
+      const int32_t lnElem = conn[8 * el + orc_hex_lrscv[2 * ip]];
+      const double sgn = (lnElem == edges[2 * e]) ? 1.0 : -1.0;
+      for (int d = 0; d < 3; ++d) areav[3 * e + d] += a[3 * ip + d] * sgn;
+    }
+  }
+}
+
+void orc_dual_volume_hex(int64_t nelem, const int32_t* conn,
+                         const double* coords, int64_t nnode, double* dualvol)
+{
+  memset(dualvol, 0, sizeof(double) * (size_t)nnode);
+  for (int64_t el = 0; el < nelem; ++el) {
+    double c[24], v[8];
+    for (int n = 0; n < 8; ++n)
+      for (int d = 0; d < 3; ++d) c[3 * n + d] = coords[3 * conn[8 * el + n] + d];
+    orc_hex_scv_volume(c, v);
+    /* HexSCV ipNodeMap is the identity: sub-volume ip belongs to node ip */
+    for (int ip = 0; ip < 8; ++ip) dualvol[conn[8 * el + ip]] += v[ip];
+  }
+}
+
+void orc_nodal_grad_edge(int64_t nnode, int64_t nedge, const int32_t* edges,
+                         const double* areav, const double* dualvol, int dim1,
+                         const double* phi, double* grad)
+{
+  memset(grad, 0, sizeof(double) * (size_t)nnode * (size_t)dim1 * 3);
+  for (int64_t e = 0; e < nedge; ++e) {
+    const int32_t L = edges[2 * e], R = edges[2 * e + 1];
+    const double invL = 1.0 / dualvol[L];
+    const double invR = 1.0 / dualvol[R];
+    int counter = 0;
+    for (int i = 0; i < dim1; ++i) {
+      const double phiIp = 0.5 * (phi[(int64_t)dim1 * L + i] +
+                                  phi[(int64_t)dim1 * R + i]);
+      for (int j = 0; j < 3; ++j) {
+        const double a = areav[3 * e + j] * phiIp;
+        grad[(int64_t)dim1 * 3 * L + counter] += a * invL;
+        grad[(int64_t)dim1 * 3 * R + counter] -= a * invR;
+        counter++;
+      }
+    }
+  }
+}
+
+void orc_mdot_edge(int64_t nedge, const int32_t* edges, const double* areav,
+                   const double* rho, const double* vel, double* mdot)
+{
+  for (int64_t e = 0; e < nedge; ++e) {
+    const int32_t L = edges[2 * e], R = edges[2 * e + 1];
+    double t = 0.0;
+    for (int d = 0; d < 3; ++d)
+      t += 0.5 * (rho[L] * vel[3 * L + d] + rho[R] * vel[3 * R + d]) *
+           areav[3 * e + d];
+    mdot[e] = t;
+  }
+}

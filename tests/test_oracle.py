@@ -1,0 +1,353 @@
+"""CPU checks that pin the oracle: the reference's own golden values, then
+closed forms and invariants for what the golds leave open (non-orthogonal
+correction, cross-component momentum couplings, geometry closure, GMRES).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from tests import fixtures as fx
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_golds.json")))
+
+CONT_OPTS = orc.ContinuityOpts(1.0, 1.0, 1.0, 1.0, 0.0)   # dt, gamma1, noc, interp, incompressible
+MOM_DEFAULT = orc.MomentumOpts(0.0, 1.0, 1.0, 1.0, 0.0, 0)
+SCAL_DEFAULT = orc.ScalarOpts(0.0, 1.0, 1.0, 1.0, 0, orc.Peclet(0, 0.0, 2.0, 4.0))
+
+
+# ------------------------------------------------------------ golden values
+
+def test_continuity_golds():
+    """UnitTestContinuityAdvEdge.C:109-141 (dense mock applier)."""
+    f = fx.continuity_fixture()
+    g = GOLD["continuity_adv_edge"]
+    L, R = orc.assemble_continuity(CONT_OPTS, 8, f["edges"], f["coords"],
+                                   f["vel"], f["dpdx"], f["rho"], f["p"],
+                                   f["udiag"], f["areav"])
+    np.testing.assert_allclose(R, g["rhs"], rtol=0, atol=g["rhs_tol"])
+    np.testing.assert_allclose(L.ravel(), g["lhs"], rtol=0, atol=g["lhs_tol"])
+
+
+def test_momentum_golds():
+    """UnitTestMomentumAdvDiffEdge.C:232-266 (mock keeps same-component
+    couplings only; 12 sumInto calls = one per edge)."""
+    f = fx.momentum_fixture()
+    g = GOLD["momentum_adv_diff_edge"]
+    assert f["edges"].shape[0] == g["num_sum_into_calls"]
+    L, R = orc.assemble_momentum(MOM_DEFAULT, 8, f["edges"], f["coords"],
+                                 f["vel"], f["dudx"], f["visc"], f["rho"],
+                                 f["areav"], f["mdot"], f["pecfac"],
+                                 f["wallmask"])
+    np.testing.assert_allclose(R, g["rhs"], rtol=0, atol=g["tol"])
+    np.testing.assert_allclose(L.ravel(), g["lhs"], rtol=0, atol=g["tol"])
+
+
+def _scalar_csr():
+    f = fx.mixture_fraction_fixture()
+    A = orc.CsrSystem(8, f["edges"], 1)
+    orc.assemble_scalar(SCAL_DEFAULT, 8, f["edges"], f["coords"], f["vel"],
+                        f["q"], f["dqdx"], f["rho"], f["visc"], f["areav"],
+                        f["mdot"], csr=A)
+    return f, A
+
+
+def test_scalar_tpetra_golds():
+    """UnitTestScalarAdvDiffEdge.C:159-227, serial CSR golds."""
+    g = GOLD["scalar_adv_diff_edge_tpetra"]
+    _, A = _scalar_csr()
+    assert list(A.row_ptr) == g["rowOffsets_serial"]
+    assert list(A.cols) == g["cols_serial"]
+    np.testing.assert_allclose(A.vals, g["vals_serial"], rtol=0, atol=g["tol"])
+    np.testing.assert_allclose(A.rhs, g["rhs_serial"], rtol=0, atol=g["tol"])
+
+
+def test_scalar_fix_pressure_golds():
+    """UnitTestScalarAdvDiffEdge.C:229-293: node id 1, p_ref 1, p = density."""
+    g = GOLD["scalar_adv_diff_edge_tpetra"]
+    f, A = _scalar_csr()
+    A.fix_node(g["fix_node_id"] - 1, g["fix_ref_pressure"], f["rho"][0])
+    np.testing.assert_allclose(A.vals, g["fixed_vals_serial"], rtol=0,
+                               atol=g["tol"])
+    np.testing.assert_allclose(A.rhs, g["fixed_rhs_serial"], rtol=0,
+                               atol=g["tol"])
+
+
+def test_scalar_dirichlet_golds():
+    """UnitTestScalarAdvDiffEdge.C:295-381: solution 2, bc 0 except node 1."""
+    g = GOLD["scalar_adv_diff_edge_tpetra"]
+    _, A = _scalar_csr()
+    sol = np.full(8, 2.0)
+    bc = np.zeros(8)
+    bc[0] = 1.0
+    A.apply_dirichlet(np.arange(8), 0, 1, sol, bc)
+    np.testing.assert_allclose(A.vals, g["dirichlet_vals_serial"], rtol=0,
+                               atol=g["tol"])
+    np.testing.assert_allclose(A.rhs, g["dirichlet_rhs_serial"], rtol=0,
+                               atol=g["tol"])
+
+
+def test_peclet_golds():
+    """UnitTestPecletFunction.C:36-110."""
+    g = GOLD["peclet_classic"]
+    for pe, fac in zip(g["pe"], g["fac"]):
+        assert abs(orc.peclet_classic(g["hybrid"], pe) - fac) < g["tol"]
+    g = GOLD["peclet_tanh"]
+    for x, fac in zip(g["x"], g["fac"]):
+        assert abs(orc.peclet_tanh(g["c1"], g["c2"], x) - fac) < g["tol"]
+
+
+def test_van_leer_limits():
+    # smooth data (equal slopes) -> 1; opposite slopes -> 0; symmetric
+    assert orc.van_leer(1.0, 1.0) == pytest.approx(1.0)
+    assert orc.van_leer(1.0, -1.0) == 0.0
+    assert orc.van_leer(2.0, 1.0) == pytest.approx(orc.van_leer(1.0, 2.0))
+    assert orc.van_leer(3.0, 1.0) == pytest.approx(2 * 2 * 3 / 16.0)
+
+
+def test_geometry_golds():
+    """UnitTestGeometryAlg.C:20-100: unit hex dual volume 0.125 per node,
+    element volume 1, |A|^2 = 0.0625 on all 12 edges."""
+    g = GOLD["geometry_unit_hex"]
+    coords, conn, edges, elem_edges, areav = fx.unit_hex()
+    dv = orc.dual_volume_hex(conn, coords)
+    np.testing.assert_allclose(dv, g["dual_volume"], atol=g["tol"])
+    assert abs(dv.sum() - g["element_volume"]) < 1e-15
+    assert edges.shape[0] == g["num_edges"]
+    np.testing.assert_allclose((areav ** 2).sum(1), g["area_mag_sq"],
+                               atol=g["tol"])
+
+
+# ------------------------------------------------------------ invariants
+
+def _skewed_box(n=3, seed=3):
+    coords, conn = fx.hex_box(n, n, n, stretch_z=1.1, jitter=0.2, seed=seed)
+    edges, elem_edges = orc.create_edges_hex(conn, coords.shape[0])
+    areav = orc.edge_area_vec_hex(conn, coords, elem_edges, edges)
+    dv = orc.dual_volume_hex(conn, coords)
+    return coords, conn, edges, areav, dv
+
+
+def test_geometry_closure_and_volume():
+    coords, conn, edges, areav, dv = _skewed_box()
+    n = coords.shape[0]
+    # element volumes sum to the box volume (planar box faces)
+    assert abs(dv.sum() - 1.0) < 1e-13
+    # sum of outward area vectors vanishes around interior control volumes
+    S = np.zeros((n, 3))
+    np.add.at(S, edges[:, 0], areav)
+    np.add.at(S, edges[:, 1], -areav)
+    boundary = np.zeros(n, bool)
+    for d in range(3):
+        lo, hi = coords[:, d].min(), coords[:, d].max()
+        boundary |= (np.abs(coords[:, d] - lo) < 1e-12) | (
+            np.abs(coords[:, d] - hi) < 1e-12)
+    assert np.abs(S[~boundary]).max() < 1e-14
+    # uniform box: |A| = h^2 on interior edges, dual volume h^3 inside
+    c2, k2 = fx.hex_box(4, 4, 4)
+    e2, ee2 = orc.create_edges_hex(k2, c2.shape[0])
+    a2 = orc.edge_area_vec_hex(k2, c2, ee2, e2)
+    d2 = orc.dual_volume_hex(k2, c2)
+    h = 0.25
+    assert np.isclose(np.abs(a2).sum(1).max(), h * h)
+    assert np.isclose(d2.max(), h ** 3)
+
+
+def test_nodal_grad_exact_for_linear_interior():
+    # exact where each sub-control-surface centroid lies at its edge
+    # midpoint in the field's gradient direction: the uniform box
+    coords, conn = fx.hex_box(4, 4, 4)
+    edges, elem_edges = orc.create_edges_hex(conn, coords.shape[0])
+    areav = orc.edge_area_vec_hex(conn, coords, elem_edges, edges)
+    dv = orc.dual_volume_hex(conn, coords)
+    g = np.array([0.3, -1.2, 2.5])
+    phi = coords @ g + 0.7
+    G = orc.nodal_grad_edge(edges, areav, dv, phi)
+    boundary = np.zeros(coords.shape[0], bool)
+    for d in range(3):
+        boundary |= (np.abs(coords[:, d] - coords[:, d].min()) < 1e-12) | (
+            np.abs(coords[:, d] - coords[:, d].max()) < 1e-12)
+    np.testing.assert_allclose(G[~boundary], np.tile(g, ((~boundary).sum(), 1)),
+                               rtol=0, atol=1e-12)
+
+
+def test_continuity_noc_exact_for_linear_pressure():
+    """With noc = 1, a linear pressure with its exact gradient and zero
+    velocity gives tmdot = 0 on any (skewed) mesh; noc = 0 does not."""
+    coords, conn, edges, areav, dv = _skewed_box()
+    n = coords.shape[0]
+    g = np.array([1.0, -2.0, 0.5])
+    p = coords @ g
+    args = (coords, np.zeros((n, 3)), np.tile(g, (n, 1)), np.ones(n), p,
+            np.full(n, 2.0), areav)
+    L, R = orc.assemble_continuity(CONT_OPTS, n, edges, *args)
+    assert np.abs(R).max() < 1e-13
+    L0, R0 = orc.assemble_continuity(orc.ContinuityOpts(1.0, 1.0, 0.0, 1.0, 0.0),
+                                     n, edges, *args)
+    assert np.abs(R0).max() > 1e-4
+    # LHS is a symmetric graph Laplacian with zero row sums
+    assert np.abs(L - L.T).max() < 1e-15
+    assert np.abs(L.sum(1)).max() < 1e-13
+
+
+def test_scalar_diffusion_exact_for_linear_field():
+    """Diffusive flux with the non-orthogonal correction equals -mu A.g for a
+    linear field with exact gradient; pure-diffusion rows sum to zero."""
+    coords, conn, edges, areav, dv = _skewed_box()
+    n = coords.shape[0]
+    g = np.array([0.4, 1.0, -0.3])
+    q = coords @ g
+    mu = 0.02
+    A = orc.CsrSystem(n, edges, 1)
+    orc.assemble_scalar(SCAL_DEFAULT, n, edges, coords, np.zeros((n, 3)), q,
+                        np.tile(g, (n, 1)), np.ones(n), np.full(n, mu), areav,
+                        np.zeros(edges.shape[0]), csr=A)
+    expect = np.zeros(n)
+    flux = -mu * (areav @ g)          # diffFlux per edge
+    np.add.at(expect, edges[:, 0], -flux)
+    np.add.at(expect, edges[:, 1], flux)
+    np.testing.assert_allclose(A.rhs, expect, rtol=0, atol=1e-14)
+    D = A.dense()
+    assert np.abs(D.sum(1)).max() < 1e-15
+
+
+def _momentum_rhs(o, n, edges, coords, vel, dudx, visc, areav, mdot, pf, m):
+    _, R = orc.assemble_momentum(o, n, edges, coords, vel, dudx, visc,
+                                 np.ones(n), areav, mdot, pf, m)
+    return R
+
+
+def test_momentum_lhs_is_jacobian_of_rhs():
+    """For frozen gradients/mdot, central advection and no limiter, the
+    momentum LHS is -d(RHS)/du (Picard = Newton for this linear flux); this
+    pins the cross-component -mu A_i A_j/(A.dx) couplings and the
+    relaxation placement that the reference mock does not capture."""
+    coords, conn, edges, areav, dv = _skewed_box(2)
+    n = coords.shape[0]
+    rng = np.random.default_rng(5)
+    vel = rng.normal(size=(n, 3))
+    dudx = rng.normal(size=(n, 9))
+    visc = rng.uniform(0.01, 0.1, n)
+    mdot = rng.normal(size=edges.shape[0]) * 0.1
+    for pecfac, alpha, aupw in [(0.0, 0.0, 1.0), (1.0, 0.0, 1.0), (0.5, 0.5, 0.3)]:
+        o = orc.MomentumOpts(alpha, aupw, 0.0, 1.0, 0.0, 0)
+        pf = np.full(edges.shape[0], pecfac)
+        mask = np.ones(n)
+        A = orc.CsrSystem(n, edges, 3)
+        orc.assemble_momentum(o, n, edges, coords, vel, dudx, visc, np.ones(n),
+                              areav, mdot, pf, mask, csr=A)
+        D = A.dense()
+        R0 = A.rhs.copy()
+        J = np.zeros((3 * n, 3 * n))
+        h = 1e-6
+        for k in range(3 * n):
+            v2 = vel.copy().ravel()
+            v2[k] += h
+            A2 = orc.CsrSystem(n, edges, 3)
+            orc.assemble_momentum(o, n, edges, coords, v2.reshape(n, 3), dudx,
+                                  visc, np.ones(n), areav, mdot, pf, mask,
+                                  csr=A2)
+            J[:, k] = -(A2.rhs - R0) / h
+        np.testing.assert_allclose(D, J, rtol=0, atol=2e-8 * np.abs(D).max())
+
+
+def test_scalar_lhs_is_jacobian_of_rhs():
+    coords, conn, edges, areav, dv = _skewed_box(2)
+    n = coords.shape[0]
+    rng = np.random.default_rng(7)
+    q = rng.normal(size=n)
+    dqdx = rng.normal(size=(n, 3))
+    mdot = rng.normal(size=edges.shape[0]) * 0.1
+    visc = rng.uniform(0.01, 0.1, n)
+    for hf in [0.0, 1.0]:
+        o = orc.ScalarOpts(0.3, 1.0, 0.0, 1.0, 0, orc.Peclet(0, hf, 2.0, 4.0))
+        vel = rng.normal(size=(n, 3)) * 3
+
+        def run(qq):
+            A = orc.CsrSystem(n, edges, 1)
+            orc.assemble_scalar(o, n, edges, coords, vel, qq, dqdx, np.ones(n),
+                                visc, areav, mdot, csr=A)
+            return A
+        A = run(q)
+        D = A.dense()
+        J = np.zeros((n, n))
+        h = 1e-6
+        for k in range(n):
+            q2 = q.copy()
+            q2[k] += h
+            A2 = run(q2)
+            J[:, k] = -(A2.rhs - A.rhs) / h
+        np.testing.assert_allclose(D, J, rtol=0, atol=2e-8 * np.abs(D).max())
+
+
+def test_orientation_invariance():
+    """Swapping edge node order (with the area vector sign) leaves the CSR
+    unchanged when no mdot is exactly zero."""
+    coords, conn, edges, areav, dv = _skewed_box()
+    n = coords.shape[0]
+    rng = np.random.default_rng(11)
+    vel = rng.normal(size=(n, 3))
+    dudx = rng.normal(size=(n, 9))
+    visc = rng.uniform(0.01, 0.1, n)
+    mdot = orc.mdot_edge(edges, areav, np.ones(n), vel)
+    pf = rng.uniform(0, 1, edges.shape[0])
+    o = orc.MomentumOpts(0.2, 0.7, 1.0, 0.8, 1.0, 1)
+    A = orc.CsrSystem(n, edges, 3)
+    orc.assemble_momentum(o, n, edges, coords, vel, dudx, visc, np.ones(n),
+                          areav, mdot, pf, np.ones(n), csr=A)
+    flip = rng.uniform(size=edges.shape[0]) < 0.5
+    e2 = edges.copy()
+    e2[flip] = e2[flip][:, ::-1]
+    a2 = areav.copy()
+    a2[flip] *= -1
+    m2 = mdot.copy()
+    m2[flip] *= -1
+    B = orc.CsrSystem(n, e2, 3)
+    orc.assemble_momentum(o, n, e2, coords, vel, dudx, visc, np.ones(n), a2,
+                          m2, pf, np.ones(n), csr=B)
+    np.testing.assert_allclose(B.vals, A.vals, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(B.rhs, A.rhs, rtol=1e-12, atol=1e-14)
+
+
+def test_dense_and_csr_agree_same_component():
+    """The dense mock and the CSR scatter agree on the couplings the mock
+    keeps."""
+    f = fx.momentum_fixture()
+    L, R = orc.assemble_momentum(MOM_DEFAULT, 8, f["edges"], f["coords"],
+                                 f["vel"], f["dudx"], f["visc"], f["rho"],
+                                 f["areav"], f["mdot"], f["pecfac"],
+                                 f["wallmask"])
+    A = orc.CsrSystem(8, f["edges"], 3)
+    orc.assemble_momentum(MOM_DEFAULT, 8, f["edges"], f["coords"], f["vel"],
+                          f["dudx"], f["visc"], f["rho"], f["areav"], f["mdot"],
+                          f["pecfac"], f["wallmask"], csr=A)
+    D = A.dense()
+    same = np.zeros_like(D, bool)
+    for d in range(3):
+        same[d::3, d::3] = True
+    np.testing.assert_allclose(D[same], L[same], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(A.rhs, R, rtol=0, atol=1e-15)
+
+
+def test_gmres_matches_dense_solve():
+    coords, conn, edges, areav, dv = _skewed_box(3)
+    n = coords.shape[0]
+    rng = np.random.default_rng(2)
+    A = orc.CsrSystem(n, edges, 1)
+    orc.assemble_continuity(CONT_OPTS, n, edges, coords, rng.normal(size=(n, 3)),
+                            np.zeros((n, 3)), np.ones(n), np.zeros(n),
+                            np.ones(n), areav, csr=A)
+    top = np.where(np.abs(coords[:, 2] - 1.0) < 1e-12)[0].astype(np.int32)
+    A.apply_dirichlet(top, 0, 1, np.zeros(n), np.zeros(n))
+    b = A.rhs.copy()
+    xd = orc.dense_solve(A.dense(), b)
+    for prec in (orc.PREC_NONE, orc.PREC_JACOBI, orc.PREC_CHEBYSHEV):
+        x, st = A.gmres(b, kspace=10, max_iters=2000, tol=1e-10, precond=prec)
+        assert st.converged
+        assert st.final_abs <= 1e-10 * st.bnorm * 1.0001
+        np.testing.assert_allclose(x, xd, rtol=0,
+                                   atol=1e-7 * np.abs(xd).max())
