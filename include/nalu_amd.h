@@ -1,0 +1,235 @@
+/*
+ * nalu_amd.h -- C ABI of the MI355X edge-assembly + pressure-solve library.
+ *
+ * One context = one GPU, one HIP stream, one mesh (edge graph), its node and
+ * edge fields, one linear system and its solver workspace.  Every call is
+ * stream-ordered on the context's stream, returns 0 on success or a negative
+ * NW_E_* code, and never throws; nw_last_error() gives the message of the
+ * last failure on the calling thread.  Arrays are plain pointers with sizes;
+ * "host" arrays are copied, device arrays are the caller's to keep alive.
+ *
+ * Each entry point names the reference interface it replaces
+ * (Exawind/nalu-wind, paths relative to its source root).
+ */
+#ifndef NALU_AMD_H
+#define NALU_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nw_ctx nw_ctx;
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  NW_OK = 0,
+  NW_E_ARG = -1,       /* bad argument / size */
+  NW_E_STATE = -2,     /* call out of order (no mesh, no graph, ...) */
+  NW_E_HIP = -3,       /* HIP runtime error */
+  NW_E_NOGRAPHCOL = -4,/* sumInto column absent from the graph */
+  NW_E_NONFINITE = -5, /* NaN/Inf found where the call checks */
+  NW_E_NOCONV = -6,    /* solver did not reach the tolerance */
+  NW_E_ALLOC = -7      /* device allocation failed */
+};
+
+/* ---- node and edge fields ------------------------------------------------
+ * Node fields are [nnode][ncomp] doubles (components of a node adjacent, as
+ * in STK field data); edge fields are [nedge][ncomp].  Field names follow
+ * the reference's registered field names. */
+enum {
+  NW_F_COORDINATES = 0, /* 3  "coordinates" */
+  NW_F_VELOCITY,        /* 3  "velocity" (== velocity_rtm on static mesh) */
+  NW_F_DENSITY,         /* 1  "density" (state NP1) */
+  NW_F_PRESSURE,        /* 1  "pressure" */
+  NW_F_DPDX,            /* 3  "dpdx" */
+  NW_F_UDIAG,           /* 1  "momentum_diag" */
+  NW_F_VISCOSITY,       /* 1  "viscosity" / "effective_viscosity_u" */
+  NW_F_DUDX,            /* 9  "dudx" */
+  NW_F_WALL_MASK,       /* 1  "abl_wall_no_slip_wall_func_node_mask" */
+  NW_F_SCALAR_Q,        /* 1  transported scalar (e.g. "mixture_fraction") */
+  NW_F_DQDX,            /* 3  its gradient */
+  NW_F_DIFF_FLUX_COEFF, /* 1  scalar diffusion coefficient */
+  NW_F_DUAL_VOLUME,     /* 1  "dual_nodal_volume" */
+  NW_F_BC_VALUE,        /* 3  Dirichlet values ("*_bc") */
+  NW_E_AREA_VECTOR,     /* 3  "edge_area_vector" (edge field) */
+  NW_E_MASS_FLOW_RATE,  /* 1  "mass_flow_rate" (edge field) */
+  NW_E_PECLET_FACTOR,   /* 1  "peclet_factor" (edge field) */
+  NW_E_PECLET_NUMBER,   /* 1  "peclet_number" (edge field) */
+  NW_NUM_FIELDS
+};
+
+/* ---- scheme options (SolutionOptions values the edge algs read) -------- */
+typedef struct {
+  int form;      /* 0 classic (hybrid factor), 1 tanh */
+  double hybrid; /* classic: hf (default 0) */
+  double c1;     /* tanh transition (default 2) */
+  double c2;     /* tanh width (default 4) */
+} nw_peclet;
+
+typedef struct {
+  double alpha;     /* default 0 */
+  double alpha_upw; /* default 1 */
+  double upw;       /* default 1 */
+  double relax;     /* relaxation factor, default 1 */
+  int use_limiter;  /* van Leer limiter on the upwind extrapolation */
+  nw_peclet peclet;
+  int accumulate;   /* 0: rows overwritten (fused zeroSystem), 1: sumInto */
+} nw_scalar_opts;
+
+typedef struct {
+  double alpha, alpha_upw, upw, relax;
+  double include_divU; /* default 0 */
+  int use_limiter;
+  int accumulate;
+} nw_momentum_opts;
+
+typedef struct {
+  double dt, gamma1;
+  double noc;             /* non-orthogonal correction on (1) / off (0) */
+  double interp_together; /* mdot interpolates rho*U together (default 1) */
+  double incompressible;  /* incompressible continuity (default 0) */
+  int accumulate;
+} nw_continuity_opts;
+
+/* ---- lifecycle ----------------------------------------------------------- */
+
+/* Context on `device` with its own HIP stream. */
+int nw_ctx_create(int device, nw_ctx** ctx);
+int nw_ctx_destroy(nw_ctx* ctx);
+/* The context's hipStream_t, for callers that time or order work on it. */
+int nw_ctx_stream(nw_ctx* ctx, void** stream);
+int nw_ctx_synchronize(nw_ctx* ctx);
+const char* nw_last_error(void);
+
+/* ---- mesh ---------------------------------------------------------------
+ * Replaces the STK edge/node connectivity the edge algorithms loop over
+ * (stk::mesh::create_edges, src/Realm.C:2304-2321; edge node[0] = L,
+ * node[1] = R as in include/AssembleEdgeSolverAlgorithm.h:86-96).
+ * edge_nodes is host [nedge][2] with 0 <= node < nnode, L != R, no
+ * duplicate edges. */
+int nw_mesh_set(nw_ctx* ctx, int64_t nnode, int64_t nedge,
+                const int32_t* edge_nodes);
+
+/* ---- fields (STK NgpField sync_to_device / sync_to_host) ----------------- */
+int nw_field_upload(nw_ctx* ctx, int field, const double* host, int64_t count);
+int nw_field_download(nw_ctx* ctx, int field, double* host, int64_t count);
+int nw_field_fill(nw_ctx* ctx, int field, double value);
+/* Device pointer of the field (allocated on first use), [n][ncomp]. */
+int nw_field_device_ptr(nw_ctx* ctx, int field, double** dptr);
+int nw_field_components(int field);
+
+/* ---- linear system (LinearSystem / TpetraLinearSystem) -------------------
+ * nw_linsys_build_edge_graph: buildEdgeToNodeGraph + finalizeLinearSystem
+ *   (src/TpetraLinearSystem.C:489-494, 420-440, 1137-1272): rows = nodes,
+ *   columns = self + edge neighbours, sorted; ndof 1 (scalar/continuity)
+ *   or 3 (momentum, 3x3 blocks). */
+int nw_linsys_build_edge_graph(nw_ctx* ctx, int ndof);
+/* zeroSystem (src/TpetraLinearSystem.C:1274-1291): values, rhs, solution. */
+int nw_linsys_zero(nw_ctx* ctx);
+/* loadComplete (src/TpetraLinearSystem.C:1788-1809): fillComplete + export
+ * of shared rows; on one GPU every row is owned, so this only orders. */
+int nw_linsys_load_complete(nw_ctx* ctx);
+/* Sizes of the point-CSR view (Tpetra layout: nrows = nnode*ndof). */
+int nw_linsys_sizes(nw_ctx* ctx, int64_t* nrows, int64_t* nnz);
+/* Download in the Tpetra point-CSR layout: row r = node*ndof + d, the
+ * columns of each connected node as ndof consecutive entries, sorted.
+ * Any pointer may be NULL. row_ptr has nrows+1 entries. */
+int nw_linsys_download(nw_ctx* ctx, int64_t* row_ptr, int32_t* cols,
+                       double* vals, double* rhs);
+/* Device pointers of the block-CSR arrays: block row_ptr [nnode+1] and
+ * cols [nnzb] (int32), values [nnzb][ndof][ndof] and rhs [nnode][ndof]. */
+int nw_linsys_device_arrays(nw_ctx* ctx, int32_t** row_ptr, int32_t** cols,
+                            double** vals, double** rhs);
+
+/* CoeffApplier::operator() / LinearSystem::sumInto
+ * (include/LinearSystem.h:53-70, src/TpetraLinearSystem.C:1381-1459):
+ * nblocks blocks of nent nodes each, lhs [nblocks][(nent*ndof)^2] row-major
+ * (row/col = node_slot*ndof + dof), rhs [nblocks][nent*ndof]; additive, in
+ * block order (deterministic).  A column absent from the graph is an error
+ * (NW_E_NOGRAPHCOL) and nothing of that call is applied. Host arrays. */
+int nw_linsys_sum_into(nw_ctx* ctx, int nblocks, int nent,
+                       const int32_t* nodes, const double* lhs,
+                       const double* rhs);
+/* CoeffApplier::resetRows / LinearSystem::resetRows
+ * (src/TpetraLinearSystem.C:1461-1520, 1759-1786). Host node list. */
+int nw_linsys_reset_rows(nw_ctx* ctx, int64_t nnodes, const int32_t* nodes,
+                         int begin, int end, double diag_value,
+                         double rhs_residual);
+/* applyDirichletBCs (src/TpetraLinearSystem.C:1686-1757): rows -> identity,
+ * rhs = bc_field - sol_field for dofs [begin, end). Host node list. */
+int nw_linsys_apply_dirichlet(nw_ctx* ctx, int64_t nnodes,
+                              const int32_t* nodes, int sol_field,
+                              int bc_field, int begin, int end);
+/* FixPressureAtNodeAlgorithm::execute (src/FixPressureAtNodeAlgorithm.C:
+ * 57-124): reset the node's row, then sumInto lhs = 1, rhs = pref - p. */
+int nw_fix_pressure_at_node(nw_ctx* ctx, int32_t node, double pref);
+
+/* ---- edge solver algorithms ---------------------------------------------
+ * Each replaces one AssembleEdgeSolverAlgorithm subclass execute(): all
+ * edges in one launch, deterministic (no atomics), rows written whole. */
+/* ScalarEdgeSolverAlg::execute (src/edge_kernels/ScalarEdgeSolverAlg.C:
+ * 56-206): reads NW_F_COORDINATES, VELOCITY, SCALAR_Q, DQDX, DENSITY,
+ * DIFF_FLUX_COEFF and NW_E_AREA_VECTOR, NW_E_MASS_FLOW_RATE. */
+int nw_assemble_scalar(nw_ctx* ctx, const nw_scalar_opts* opts);
+/* MomentumEdgeSolverAlg::execute (src/edge_kernels/MomentumEdgeSolverAlg.C:
+ * 71-313): reads COORDINATES, VELOCITY, DUDX, VISCOSITY, WALL_MASK and
+ * AREA_VECTOR, MASS_FLOW_RATE, PECLET_FACTOR; needs an ndof = 3 graph. */
+int nw_assemble_momentum(nw_ctx* ctx, const nw_momentum_opts* opts);
+/* ContinuityEdgeSolverAlg::execute (src/edge_kernels/ContinuityEdgeSolverAlg.C:
+ * 38-195): reads COORDINATES, VELOCITY, DPDX, DENSITY, PRESSURE, UDIAG and
+ * AREA_VECTOR. */
+int nw_assemble_continuity(nw_ctx* ctx, const nw_continuity_opts* opts);
+/* MomentumEdgePecletAlg::execute (src/edge_kernels/MomentumEdgePecletAlg.C:
+ * 49-102): writes NW_E_PECLET_NUMBER and NW_E_PECLET_FACTOR. */
+int nw_momentum_edge_peclet(nw_ctx* ctx, const nw_peclet* peclet);
+
+/* ---- linear solve (TpetraLinearSystem::solve -> Belos GMRES) ------------ */
+enum { NW_PREC_NONE = 0, NW_PREC_JACOBI = 1, NW_PREC_CHEBYSHEV = 2 };
+enum {
+  NW_SOLVE_FP64 = 0,       /* plain FP64 GMRES(m) */
+  NW_SOLVE_FP32_REFINE = 1 /* FP32 inner GMRES + FP64 iterative refinement */
+};
+enum {
+  NW_X0_ZERO = 0,     /* x0 = 0, the reference's choice */
+  NW_X0_SOLUTION = 1, /* current contents of the solution vector */
+  NW_X0_LEARNED = 2   /* message-passing network prediction (nw_mpnn_*) */
+};
+
+typedef struct {
+  int kspace;         /* restart length ("Num Blocks", default 50) */
+  int max_iters;      /* total Krylov iterations cap */
+  double tol;         /* ||b - A x||_2 / ||b||_2 target */
+  int precond;        /* NW_PREC_* (right preconditioning) */
+  int cheb_degree;    /* Chebyshev polynomial degree */
+  double cheb_ratio;  /* lambda_max / lambda_min of the Chebyshev interval */
+  int mode;           /* NW_SOLVE_* */
+  double inner_tol;   /* FP32 inner relative tolerance per refinement step */
+  int x0;             /* NW_X0_* */
+} nw_solver_cfg;
+
+typedef struct {
+  int iters;            /* Krylov iterations (all refinement steps) */
+  int refinements;      /* outer FP64 refinement steps (mode 1) */
+  int converged;
+  double final_rel;     /* explicit ||b - A x||_2 / ||b||_2 in FP64 */
+  double final_abs;     /* explicit ||b - A x||_2 (the reference's report) */
+  double bnorm;
+  double ms;            /* device time of the solve on the context stream */
+} nw_solve_stats;
+
+/* Solve A x = rhs into the solution vector (the pressure increment). */
+int nw_linsys_solve(nw_ctx* ctx, const nw_solver_cfg* cfg,
+                    nw_solve_stats* stats);
+int nw_linsys_solution_upload(nw_ctx* ctx, const double* host);
+int nw_linsys_solution_download(nw_ctx* ctx, double* host);
+int nw_linsys_solution_device_ptr(nw_ctx* ctx, double** dptr);
+/* y = A x on device arrays of nrows doubles (FP64 residual checks). */
+int nw_linsys_spmv(nw_ctx* ctx, const double* d_x, double* d_y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

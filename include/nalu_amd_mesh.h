@@ -1,0 +1,65 @@
+/*
+ * nalu_amd_mesh.h -- host-side mesh and geometry producers for the edge
+ * path: a structured Hex8 box generator (uniform / stretched / jittered),
+ * edge extraction, CVFEM edge area vectors and dual volumes, the edge
+ * nodal gradient and the edge mass flow rate.  These produce the inputs the
+ * edge solver algorithms consume; they run multithreaded on the host.
+ * All arrays are host arrays; return 0 or a negative NW_E_* code.
+ */
+#ifndef NALU_AMD_MESH_H
+#define NALU_AMD_MESH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Node and element counts of an nx x ny x nz cell box. */
+int nw_hex_box_sizes(int nx, int ny, int nz, int64_t* nnode, int64_t* nelem);
+
+/* Box [0,1]^3: nodes numbered x fastest; z spacing stretched geometrically
+ * toward z = 0 by `stretch_z` per cell (1 = uniform); interior nodes moved
+ * by a seeded uniform offset of +-jitter times the local spacing.
+ * coords [nnode][3], conn [nelem][8] in exodus Hex8 node order. */
+int nw_hex_box(int nx, int ny, int nz, double stretch_z, double jitter,
+               uint64_t seed, double* coords, int32_t* conn);
+
+/* Unique edges of a Hex8 mesh, node[0] = smaller node id (the polarity of
+ * stk::mesh::create_edges), sorted by (node[0], node[1]); elem_edges
+ * [nelem][12] gives the edge of each Hex8 edge ordinal. edges must hold
+ * 2*cap ints; *nedge receives the count. */
+int nw_hex_edges(int64_t nelem, const int32_t* conn, int64_t nnode,
+                 int32_t* edges, int64_t cap, int64_t* nedge,
+                 int32_t* elem_edges);
+
+/* GeometryInteriorAlg::impl_compute_edge_area_vector
+ * (src/ngp_algorithms/GeometryInteriorAlg.C:163-225) with HexSCS areas
+ * (src/master_element/Hex8CVFEM.C:567-591): areav [nedge][3]. */
+int nw_hex_edge_area_vector(int64_t nelem, const int32_t* conn,
+                            const double* coords, const int32_t* elem_edges,
+                            const int32_t* edges, int64_t nedge,
+                            double* areav);
+
+/* GeometryInteriorAlg::impl_compute_dual_nodal_volume
+ * (src/ngp_algorithms/GeometryInteriorAlg.C:70-112) with HexSCV volumes
+ * (src/master_element/Hex8CVFEM.C:365-389). */
+int nw_hex_dual_volume(int64_t nelem, const int32_t* conn,
+                       const double* coords, int64_t nnode, double* dualvol);
+
+/* NodalGradEdgeAlg::execute (src/ngp_algorithms/NodalGradEdgeAlg.C:58-111),
+ * interior edges: grad [nnode][dim1*3] is overwritten. */
+int nw_nodal_grad_edge(int64_t nnode, int64_t nedge, const int32_t* edges,
+                       const double* areav, const double* dualvol, int dim1,
+                       const double* phi, double* grad);
+
+/* Edge mass flow rate 0.5*(rho_L u_L + rho_R u_R).A (the form of
+ * unit_tests/kernels/UnitTestKernelUtils.C:725-769). */
+int nw_mdot_edge(int64_t nedge, const int32_t* edges, const double* areav,
+                 const double* rho, const double* vel, double* mdot);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,97 @@
+// nw_internal.h -- context layout and helpers shared by the HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "nalu_amd.h"
+
+namespace nw {
+
+void set_error(const char* fmt, ...);
+
+#define NW_HIP(call)                                                      \
+  do {                                                                    \
+    hipError_t nw_e_ = (call);                                            \
+    if (nw_e_ != hipSuccess) {                                            \
+      ::nw::set_error("%s failed: %s (%s:%d)", #call,                     \
+                      hipGetErrorString(nw_e_), __FILE__, __LINE__);      \
+      return NW_E_HIP;                                                    \
+    }                                                                     \
+  } while (0)
+
+#define NW_LAUNCH_CHECK()                                                 \
+  do {                                                                    \
+    hipError_t nw_e_ = hipGetLastError();                                 \
+    if (nw_e_ != hipSuccess) {                                            \
+      ::nw::set_error("kernel launch failed: %s (%s:%d)",                 \
+                      hipGetErrorString(nw_e_), __FILE__, __LINE__);      \
+      return NW_E_HIP;                                                    \
+    }                                                                     \
+  } while (0)
+
+#define NW_REQUIRE(cond, code, ...)                                       \
+  do {                                                                    \
+    if (!(cond)) {                                                        \
+      ::nw::set_error(__VA_ARGS__);                                       \
+      return code;                                                        \
+    }                                                                     \
+  } while (0)
+
+// Rows handled by one wavefront in the slot-parallel row kernels.
+constexpr int kRowsPerWave = 64;
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+// Slot code of a CSR (block) slot: diagonal slot of row n -> -(n+1);
+// off-diagonal slot reached through edge e from its node[side] -> 2e+side.
+__host__ __device__ inline int32_t diag_code(int32_t n) { return -(n + 1); }
+
+struct Graph {
+  int ndof = 0;
+  int64_t nnode = 0;
+  int64_t nnzb = 0;           // block slots (one per node pair)
+  int32_t* row_ptr = nullptr; // [nnode+1]
+  int32_t* cols = nullptr;    // [nnzb]
+  int32_t* code = nullptr;    // [nnzb]
+  int32_t* diag = nullptr;    // [nnode] slot of the diagonal block
+  double* vals = nullptr;     // [nnzb][ndof][ndof]
+  double* rhs = nullptr;      // [nnode][ndof]
+  double* sln = nullptr;      // [nnode][ndof]
+  std::vector<int32_t> h_row_ptr, h_cols;
+};
+
+struct Krylov;  // solver workspace (nw_krylov.hip)
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t nnode = -1;
+  int64_t nedge = -1;
+  int32_t* edges = nullptr;  // [nedge][2] device
+  std::vector<int32_t> h_edges;
+  double* field[NW_NUM_FIELDS] = {};
+  Graph g;
+  Krylov* kry = nullptr;
+};
+
+int field_components(int f);
+bool field_is_edge(int f);
+int64_t field_count(const Ctx* c, int f);  // doubles in the field
+// Device pointer of a field, allocated (zero-filled) on first use.
+int ensure_field(Ctx* c, int f, double** out);
+int require_field(Ctx* c, int f, const double** out);
+void free_krylov(Krylov* k);
+
+inline unsigned grid_for(int64_t n, int block) {
+  return (unsigned)((n + block - 1) / block);
+}
+
+}  // namespace nw
+
+struct nw_ctx : nw::Ctx {};

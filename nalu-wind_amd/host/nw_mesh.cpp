@@ -1,0 +1,369 @@
+// nw_mesh.cpp -- host-side producers of the edge path's inputs: structured
+// Hex8 boxes, edge extraction, CVFEM Hex8 sub-control-surface area vectors
+// and sub-control volumes, edge area vectors, dual nodal volumes, the edge
+// nodal gradient and the edge mass flow rate.
+//
+// Element-level geometry is computed in parallel in element chunks and then
+// accumulated onto edges/nodes serially in element order, so the result is
+// bitwise independent of the thread count.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "nalu_amd.h"
+#include "nalu_amd_mesh.h"
+
+namespace {
+
+// Hex8 edge ordinals (element-local node pairs) and the L/R node pair of each
+// sub-control surface (include/master_element/Hex8CVFEM.h:292-293).
+const int kHexEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                             {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+const int kLrscv[24] = {0, 1, 1, 2, 2, 3, 0, 3, 4, 5, 5, 6,
+                        6, 7, 4, 7, 0, 4, 1, 5, 2, 6, 3, 7};
+
+// The 27 points of the subdivided hex: corners, edge midpoints, face
+// centroids, centroid (include/master_element/Hex8GeometryFunctions.h:
+// 258-330 orders them face by face).
+void subdivide(const double c[8][3], double v[27][3])
+{
+  static const int mid[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                 {6, 7}, {7, 4}, {1, 5}, {0, 4}, {3, 7}, {2, 6}};
+  static const int mid_at[12] = {8, 9, 10, 11, 13, 14, 15, 16, 18, 19, 21, 22};
+  static const int face[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 4, 5},
+                                 {2, 3, 6, 7}, {1, 2, 5, 6}, {0, 3, 4, 7}};
+  static const int face_at[6] = {12, 17, 20, 23, 24, 25};
+  for (int n = 0; n < 8; ++n)
+    for (int d = 0; d < 3; ++d) v[n][d] = c[n][d];
+  for (int d = 0; d < 3; ++d) {
+    for (int k = 0; k < 12; ++k)
+      v[mid_at[k]][d] = 0.5 * (c[mid[k][0]][d] + c[mid[k][1]][d]);
+    for (int f = 0; f < 6; ++f)
+      v[face_at[f]][d] = 0.25 * (c[face[f][0]][d] + c[face[f][1]][d] +
+                                 c[face[f][2]][d] + c[face[f][3]][d]);
+    double s = 0.0;
+    for (int n = 0; n < 8; ++n) s += c[n][d];
+    v[26][d] = s * 0.125;
+  }
+}
+
+// Area vector of a quad by four triangles about its centroid (Grandy).
+void quad_area(const double q[4][3], double a[3])
+{
+  double xm[3];
+  for (int d = 0; d < 3; ++d) xm[d] = 0.25 * (q[0][d] + q[1][d] + q[2][d] + q[3][d]);
+  a[0] = a[1] = a[2] = 0.0;
+  double r1[3] = {q[0][0] - xm[0], q[0][1] - xm[1], q[0][2] - xm[2]};
+  for (int t = 0; t < 4; ++t) {
+    const int k = (t + 1) & 3;
+    const double r2[3] = {q[k][0] - xm[0], q[k][1] - xm[1], q[k][2] - xm[2]};
+    a[0] += r1[1] * r2[2] - r2[1] * r1[2];
+    a[1] += r1[2] * r2[0] - r2[2] * r1[0];
+    a[2] += r1[0] * r2[1] - r2[0] * r1[1];
+    std::memcpy(r1, r2, sizeof(r1));
+  }
+  for (int d = 0; d < 3; ++d) a[d] *= 0.5;
+}
+
+// HexSCS::determinant_scs (src/master_element/Hex8CVFEM.C:567-591)
+void hex_scs(const double c[8][3], double a[12][3])
+{
+  static const int facet[12][4] = {
+    {20, 8, 12, 26},  {24, 9, 12, 26},  {10, 12, 26, 23}, {11, 25, 26, 12},
+    {13, 20, 26, 17}, {17, 14, 24, 26}, {17, 15, 23, 26}, {16, 17, 26, 25},
+    {19, 20, 26, 25}, {20, 18, 24, 26}, {22, 23, 26, 24}, {21, 25, 26, 23}};
+  double v[27][3];
+  subdivide(c, v);
+  for (int s = 0; s < 12; ++s) {
+    double q[4][3];
+    for (int i = 0; i < 4; ++i)
+      for (int d = 0; d < 3; ++d) q[i][d] = v[facet[s][i]][d];
+    quad_area(q, a[s]);
+  }
+}
+
+// Volume of a trilinear hex by 24 facet triangles about the face centroids
+// (Grandy, "Efficient computation of volume of hexahedral cells", 1997).
+double grandy_volume(const double h[8][3])
+{
+  static const int face[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                 {2, 3, 7, 6}, {1, 2, 6, 5}, {0, 4, 3, 7}};
+  static const int tri[24][3] = {
+    {0, 8, 1},  {8, 2, 1},  {3, 2, 8},  {3, 8, 0},  {6, 9, 5},  {7, 9, 6},
+    {4, 9, 7},  {4, 5, 9},  {10, 0, 1}, {5, 10, 1}, {4, 10, 5}, {4, 0, 10},
+    {7, 6, 11}, {6, 2, 11}, {2, 3, 11}, {3, 7, 11}, {6, 12, 2}, {5, 12, 6},
+    {5, 1, 12}, {1, 2, 12}, {0, 4, 13}, {4, 7, 13}, {7, 3, 13}, {3, 0, 13}};
+  double p[14][3];
+  for (int n = 0; n < 8; ++n)
+    for (int d = 0; d < 3; ++d) p[n][d] = h[n][d];
+  for (int f = 0; f < 6; ++f)
+    for (int d = 0; d < 3; ++d)
+      p[8 + f][d] = 0.25 * (p[face[f][0]][d] + p[face[f][1]][d] +
+                            p[face[f][2]][d] + p[face[f][3]][d]);
+  double vol = 0.0;
+  for (int t = 0; t < 24; ++t) {
+    const double* a = p[tri[t][0]];
+    const double* b = p[tri[t][1]];
+    const double* c = p[tri[t][2]];
+    const double m[3] = {a[0] + b[0] + c[0], a[1] + b[1] + c[1], a[2] + b[2] + c[2]};
+    const double n0 = (b[1] - a[1]) * (c[2] - a[2]) - (c[1] - a[1]) * (b[2] - a[2]);
+    const double n1 = (c[0] - a[0]) * (b[2] - a[2]) - (b[0] - a[0]) * (c[2] - a[2]);
+    const double n2 = (b[0] - a[0]) * (c[1] - a[1]) - (c[0] - a[0]) * (b[1] - a[1]);
+    vol += m[0] * n0 + m[1] * n1 + m[2] * n2;
+  }
+  return vol / 18.0;
+}
+
+// HexSCV::determinant_scv (src/master_element/Hex8CVFEM.C:365-389)
+void hex_scv(const double c[8][3], double vol[8])
+{
+  static const int sub[8][8] = {
+    {0, 8, 12, 11, 19, 20, 26, 25},  {8, 1, 9, 12, 20, 18, 24, 26},
+    {12, 9, 2, 10, 26, 24, 22, 23},  {11, 12, 10, 3, 25, 26, 23, 21},
+    {19, 20, 26, 25, 4, 13, 17, 16}, {20, 18, 24, 26, 13, 5, 14, 17},
+    {26, 24, 22, 23, 17, 14, 6, 15}, {25, 26, 23, 21, 16, 17, 15, 7}};
+  double v[27][3];
+  subdivide(c, v);
+  for (int s = 0; s < 8; ++s) {
+    double h[8][3];
+    for (int n = 0; n < 8; ++n)
+      for (int d = 0; d < 3; ++d) h[n][d] = v[sub[s][n]][d];
+    vol[s] = grandy_volume(h);
+  }
+}
+
+void gather(const int32_t* conn, const double* coords, int64_t el,
+            double c[8][3])
+{
+  for (int n = 0; n < 8; ++n)
+    for (int d = 0; d < 3; ++d) c[n][d] = coords[3 * (int64_t)conn[8 * el + n] + d];
+}
+
+uint64_t mix(uint64_t z)
+{
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+double unit(uint64_t seed, uint64_t i)
+{
+  return (double)(mix(seed + 0x9e3779b97f4a7c15ULL * (i + 1)) >> 11) *
+         (1.0 / 9007199254740992.0);
+}
+
+constexpr int64_t kChunk = 1 << 18;
+
+}  // namespace
+
+extern "C" {
+
+int nw_hex_box_sizes(int nx, int ny, int nz, int64_t* nnode, int64_t* nelem)
+{
+  if (nx < 1 || ny < 1 || nz < 1) return NW_E_ARG;
+  if (nnode) *nnode = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  if (nelem) *nelem = (int64_t)nx * ny * nz;
+  return NW_OK;
+}
+
+int nw_hex_box(int nx, int ny, int nz, double stretch_z, double jitter,
+               uint64_t seed, double* coords, int32_t* conn)
+{
+  if (nx < 1 || ny < 1 || nz < 1 || !(stretch_z > 0) || jitter < 0 ||
+      jitter >= 0.5 || !coords || !conn)
+    return NW_E_ARG;
+  const int64_t px = nx + 1, py = ny + 1, pz = nz + 1;
+  if (px * py * pz >= INT32_MAX) return NW_E_ARG;
+  std::vector<double> xs(px), ys(py), zs(pz);
+  for (int i = 0; i < px; ++i) xs[i] = (double)i / nx;
+  for (int j = 0; j < py; ++j) ys[j] = (double)j / ny;
+  if (stretch_z == 1.0) {
+    for (int k = 0; k < pz; ++k) zs[k] = (double)k / nz;
+  } else {
+    double tot = 0.0;
+    std::vector<double> h(nz);
+    for (int k = 0; k < nz; ++k) {
+      h[k] = std::pow(stretch_z, (double)k);
+      tot += h[k];
+    }
+    zs[0] = 0.0;
+    for (int k = 0; k < nz; ++k) zs[k + 1] = zs[k] + h[k];
+    for (int k = 0; k <= nz; ++k) zs[k] /= tot;
+    zs[nz] = 1.0;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < pz; ++k)
+    for (int64_t j = 0; j < py; ++j)
+      for (int64_t i = 0; i < px; ++i) {
+        const int64_t n = i + px * (j + py * k);
+        double x = xs[i], y = ys[j], z = zs[k];
+        if (jitter > 0 && i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz) {
+          const double hx = 1.0 / nx, hy = 1.0 / ny;
+          const double hz = std::min(zs[k] - zs[k - 1], zs[k + 1] - zs[k]);
+          x += (2.0 * unit(seed, 3 * n) - 1.0) * jitter * hx;
+          y += (2.0 * unit(seed, 3 * n + 1) - 1.0) * jitter * hy;
+          z += (2.0 * unit(seed, 3 * n + 2) - 1.0) * jitter * hz;
+        }
+        coords[3 * n] = x;
+        coords[3 * n + 1] = y;
+        coords[3 * n + 2] = z;
+      }
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        const int64_t e = i + (int64_t)nx * (j + (int64_t)ny * k);
+        const int64_t n0 = i + px * (j + py * k);
+        const int64_t q[8] = {n0,           n0 + 1,           n0 + 1 + px,
+                              n0 + px,      n0 + px * py,     n0 + 1 + px * py,
+                              n0 + 1 + px + px * py, n0 + px + px * py};
+        for (int a = 0; a < 8; ++a) conn[8 * e + a] = (int32_t)q[a];
+      }
+  return NW_OK;
+}
+
+int nw_hex_edges(int64_t nelem, const int32_t* conn, int64_t nnode,
+                 int32_t* edges, int64_t cap, int64_t* nedge,
+                 int32_t* elem_edges)
+{
+  if (nelem < 0 || !conn || nnode <= 0 || !edges || !nedge) return NW_E_ARG;
+  // neighbours above each node, bucketed by the smaller id
+  std::vector<int64_t> cnt(nnode + 1, 0);
+  for (int64_t el = 0; el < nelem; ++el)
+    for (int k = 0; k < 12; ++k) {
+      const int32_t a = conn[8 * el + kHexEdge[k][0]];
+      const int32_t b = conn[8 * el + kHexEdge[k][1]];
+      if (a < 0 || b < 0 || a >= nnode || b >= nnode || a == b) return NW_E_ARG;
+      cnt[std::min(a, b) + 1]++;
+    }
+  for (int64_t n = 0; n < nnode; ++n) cnt[n + 1] += cnt[n];
+  std::vector<int32_t> nb(cnt[nnode]);
+  {
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t el = 0; el < nelem; ++el)
+      for (int k = 0; k < 12; ++k) {
+        const int32_t a = conn[8 * el + kHexEdge[k][0]];
+        const int32_t b = conn[8 * el + kHexEdge[k][1]];
+        nb[pos[std::min(a, b)]++] = std::max(a, b);
+      }
+  }
+  // sort + unique each bucket; edge ids in (L, R) order
+  std::vector<int64_t> ucnt(nnode + 1, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t n = 0; n < nnode; ++n) {
+    auto b = nb.begin() + cnt[n], e = nb.begin() + cnt[n + 1];
+    std::sort(b, e);
+    ucnt[n + 1] = std::unique(b, e) - b;
+  }
+  for (int64_t n = 0; n < nnode; ++n) ucnt[n + 1] += ucnt[n];
+  const int64_t ne = ucnt[nnode];
+  *nedge = ne;
+  if (ne > cap) return NW_E_ARG;
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t n = 0; n < nnode; ++n)
+    for (int64_t i = 0; i < ucnt[n + 1] - ucnt[n]; ++i) {
+      edges[2 * (ucnt[n] + i)] = (int32_t)n;
+      edges[2 * (ucnt[n] + i) + 1] = nb[cnt[n] + i];
+    }
+  if (elem_edges) {
+#pragma omp parallel for schedule(static)
+    for (int64_t el = 0; el < nelem; ++el)
+      for (int k = 0; k < 12; ++k) {
+        const int32_t a = conn[8 * el + kHexEdge[k][0]];
+        const int32_t b = conn[8 * el + kHexEdge[k][1]];
+        const int32_t lo = std::min(a, b), hi = std::max(a, b);
+        auto bb = nb.begin() + cnt[lo];
+        auto ee = bb + (ucnt[lo + 1] - ucnt[lo]);
+        elem_edges[12 * el + k] = (int32_t)(ucnt[lo] + (std::lower_bound(bb, ee, hi) - bb));
+      }
+  }
+  return NW_OK;
+}
+
+int nw_hex_edge_area_vector(int64_t nelem, const int32_t* conn,
+                            const double* coords, const int32_t* elem_edges,
+                            const int32_t* edges, int64_t nedge, double* areav)
+{
+  if (!conn || !coords || !elem_edges || !edges || !areav) return NW_E_ARG;
+  std::memset(areav, 0, sizeof(double) * 3 * nedge);
+  std::vector<double> buf((size_t)std::min(nelem, kChunk) * 36);
+  for (int64_t e0 = 0; e0 < nelem; e0 += kChunk) {
+    const int64_t e1 = std::min(nelem, e0 + kChunk);
+#pragma omp parallel for schedule(static)
+    for (int64_t el = e0; el < e1; ++el) {
+      double c[8][3];
+      gather(conn, coords, el, c);
+      hex_scs(c, reinterpret_cast<double(*)[3]>(&buf[(el - e0) * 36]));
+    }
+    // element-ordered accumulation: sign +1 when the element-local left
+    // node is the edge's node[0] (GeometryInteriorAlg.C:209-219)
+    for (int64_t el = e0; el < e1; ++el)
+      for (int ip = 0; ip < 12; ++ip) {
+        const int32_t e = elem_edges[12 * el + ip];
+        const double sgn = conn[8 * el + kLrscv[2 * ip]] == edges[2 * e] ? 1.0 : -1.0;
+        const double* a = &buf[(el - e0) * 36 + 3 * ip];
+        for (int d = 0; d < 3; ++d) areav[3 * (int64_t)e + d] += a[d] * sgn;
+      }
+  }
+  return NW_OK;
+}
+
+int nw_hex_dual_volume(int64_t nelem, const int32_t* conn,
+                       const double* coords, int64_t nnode, double* dualvol)
+{
+  if (!conn || !coords || !dualvol) return NW_E_ARG;
+  std::memset(dualvol, 0, sizeof(double) * nnode);
+  std::vector<double> buf((size_t)std::min(nelem, kChunk) * 8);
+  for (int64_t e0 = 0; e0 < nelem; e0 += kChunk) {
+    const int64_t e1 = std::min(nelem, e0 + kChunk);
+#pragma omp parallel for schedule(static)
+    for (int64_t el = e0; el < e1; ++el) {
+      double c[8][3];
+      gather(conn, coords, el, c);
+      hex_scv(c, &buf[(el - e0) * 8]);
+    }
+    for (int64_t el = e0; el < e1; ++el)
+      for (int n = 0; n < 8; ++n) dualvol[conn[8 * el + n]] += buf[(el - e0) * 8 + n];
+  }
+  return NW_OK;
+}
+
+int nw_nodal_grad_edge(int64_t nnode, int64_t nedge, const int32_t* edges,
+                       const double* areav, const double* dualvol, int dim1,
+                       const double* phi, double* grad)
+{
+  if (!edges || !areav || !dualvol || !phi || !grad || dim1 < 1) return NW_E_ARG;
+  std::memset(grad, 0, sizeof(double) * nnode * dim1 * 3);
+  for (int64_t e = 0; e < nedge; ++e) {
+    const int64_t L = edges[2 * e], R = edges[2 * e + 1];
+    const double iL = 1.0 / dualvol[L], iR = 1.0 / dualvol[R];
+    for (int i = 0; i < dim1; ++i) {
+      const double pip = 0.5 * (phi[dim1 * L + i] + phi[dim1 * R + i]);
+      for (int j = 0; j < 3; ++j) {
+        const double a = areav[3 * e + j] * pip;
+        grad[dim1 * 3 * L + 3 * i + j] += a * iL;
+        grad[dim1 * 3 * R + 3 * i + j] -= a * iR;
+      }
+    }
+  }
+  return NW_OK;
+}
+
+int nw_mdot_edge(int64_t nedge, const int32_t* edges, const double* areav,
+                 const double* rho, const double* vel, double* mdot)
+{
+  if (!edges || !areav || !rho || !vel || !mdot) return NW_E_ARG;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < nedge; ++e) {
+    const int64_t L = edges[2 * e], R = edges[2 * e + 1];
+    double t = 0.0;
+    for (int d = 0; d < 3; ++d)
+      t += 0.5 * (rho[L] * vel[3 * L + d] + rho[R] * vel[3 * R + d]) * areav[3 * e + d];
+    mdot[e] = t;
+  }
+  return NW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,390 @@
+"""Python binding (ctypes) of libnaluwind_amd.so, the MI355X edge-assembly +
+pressure-solve library.  Mirrors the C ABI in include/nalu_amd.h and
+include/nalu_amd_mesh.h one to one; numpy arrays in, numpy arrays out.
+
+The library must be built (nalu-wind_amd/build.py).  Importing this module
+without it raises ImportError: there is no fallback implementation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnaluwind_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libnaluwind_amd.so is not built (run nalu-wind_amd/build.py)")
+lib = C.CDLL(LIB_PATH)
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+# status codes
+OK, E_ARG, E_STATE, E_HIP, E_NOGRAPHCOL, E_NONFINITE, E_NOCONV, E_ALLOC = (
+    0, -1, -2, -3, -4, -5, -6, -7)
+
+# fields
+(F_COORDINATES, F_VELOCITY, F_DENSITY, F_PRESSURE, F_DPDX, F_UDIAG,
+ F_VISCOSITY, F_DUDX, F_WALL_MASK, F_SCALAR_Q, F_DQDX, F_DIFF_FLUX_COEFF,
+ F_DUAL_VOLUME, F_BC_VALUE, E_AREA_VECTOR, E_MASS_FLOW_RATE, E_PECLET_FACTOR,
+ E_PECLET_NUMBER) = range(18)
+
+PREC_NONE, PREC_JACOBI, PREC_CHEBYSHEV = 0, 1, 2
+SOLVE_FP64, SOLVE_FP32_REFINE = 0, 1
+X0_ZERO, X0_SOLUTION, X0_LEARNED = 0, 1, 2
+
+
+class NwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("nalu_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Peclet(C.Structure):
+    _fields_ = [("form", C.c_int), ("hybrid", C.c_double), ("c1", C.c_double),
+                ("c2", C.c_double)]
+
+
+class ScalarOpts(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("alpha_upw", C.c_double),
+                ("upw", C.c_double), ("relax", C.c_double),
+                ("use_limiter", C.c_int), ("peclet", Peclet),
+                ("accumulate", C.c_int)]
+
+
+class MomentumOpts(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("alpha_upw", C.c_double),
+                ("upw", C.c_double), ("relax", C.c_double),
+                ("include_divU", C.c_double), ("use_limiter", C.c_int),
+                ("accumulate", C.c_int)]
+
+
+class ContinuityOpts(C.Structure):
+    _fields_ = [("dt", C.c_double), ("gamma1", C.c_double), ("noc", C.c_double),
+                ("interp_together", C.c_double),
+                ("incompressible", C.c_double), ("accumulate", C.c_int)]
+
+
+class SolverCfg(C.Structure):
+    _fields_ = [("kspace", C.c_int), ("max_iters", C.c_int), ("tol", C.c_double),
+                ("precond", C.c_int), ("cheb_degree", C.c_int),
+                ("cheb_ratio", C.c_double), ("mode", C.c_int),
+                ("inner_tol", C.c_double), ("x0", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("refinements", C.c_int),
+                ("converged", C.c_int), ("final_rel", C.c_double),
+                ("final_abs", C.c_double), ("bnorm", C.c_double),
+                ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def scalar_defaults(**kw):
+    """SolutionOptions defaults (src/SolutionOptions.C:32-52)."""
+    o = ScalarOpts(0.0, 1.0, 1.0, 1.0, 0, Peclet(0, 0.0, 2.0, 4.0), 0)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def momentum_defaults(**kw):
+    o = MomentumOpts(0.0, 1.0, 1.0, 1.0, 0.0, 0, 0)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def continuity_defaults(**kw):
+    o = ContinuityOpts(1.0, 1.0, 1.0, 1.0, 0.0, 0)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def solver_cfg(**kw):
+    o = SolverCfg(50, 1000, 1e-8, PREC_JACOBI, 4, 30.0, SOLVE_FP64, 0.0, X0_ZERO)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+_P = C.c_void_p
+_sig = {
+    "nw_ctx_create": ([C.c_int, C.POINTER(_P)], C.c_int),
+    "nw_ctx_destroy": ([_P], C.c_int),
+    "nw_ctx_stream": ([_P, C.POINTER(_P)], C.c_int),
+    "nw_ctx_synchronize": ([_P], C.c_int),
+    "nw_last_error": ([], C.c_char_p),
+    "nw_mesh_set": ([_P, C.c_int64, C.c_int64, _i32p], C.c_int),
+    "nw_field_upload": ([_P, C.c_int, _f64p, C.c_int64], C.c_int),
+    "nw_field_download": ([_P, C.c_int, _f64p, C.c_int64], C.c_int),
+    "nw_field_fill": ([_P, C.c_int, C.c_double], C.c_int),
+    "nw_field_device_ptr": ([_P, C.c_int, C.POINTER(_P)], C.c_int),
+    "nw_field_components": ([C.c_int], C.c_int),
+    "nw_linsys_build_edge_graph": ([_P, C.c_int], C.c_int),
+    "nw_linsys_zero": ([_P], C.c_int),
+    "nw_linsys_load_complete": ([_P], C.c_int),
+    "nw_linsys_sizes": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "nw_linsys_download": ([_P, _P, _P, _P, _P], C.c_int),
+    "nw_linsys_device_arrays": ([_P, C.POINTER(_P), C.POINTER(_P),
+                                 C.POINTER(_P), C.POINTER(_P)], C.c_int),
+    "nw_linsys_sum_into": ([_P, C.c_int, C.c_int, _i32p, _f64p, _f64p], C.c_int),
+    "nw_linsys_reset_rows": ([_P, C.c_int64, _i32p, C.c_int, C.c_int,
+                              C.c_double, C.c_double], C.c_int),
+    "nw_linsys_apply_dirichlet": ([_P, C.c_int64, _i32p, C.c_int, C.c_int,
+                                   C.c_int, C.c_int], C.c_int),
+    "nw_fix_pressure_at_node": ([_P, C.c_int32, C.c_double], C.c_int),
+    "nw_assemble_scalar": ([_P, C.POINTER(ScalarOpts)], C.c_int),
+    "nw_assemble_momentum": ([_P, C.POINTER(MomentumOpts)], C.c_int),
+    "nw_assemble_continuity": ([_P, C.POINTER(ContinuityOpts)], C.c_int),
+    "nw_momentum_edge_peclet": ([_P, C.POINTER(Peclet)], C.c_int),
+    "nw_linsys_solve": ([_P, C.POINTER(SolverCfg), C.POINTER(SolveStats)], C.c_int),
+    "nw_linsys_solution_upload": ([_P, _f64p], C.c_int),
+    "nw_linsys_solution_download": ([_P, _f64p], C.c_int),
+    "nw_linsys_solution_device_ptr": ([_P, C.POINTER(_P)], C.c_int),
+    "nw_linsys_spmv": ([_P, _P, _P], C.c_int),
+    "nw_hex_box_sizes": ([C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64),
+                          C.POINTER(C.c_int64)], C.c_int),
+    "nw_hex_box": ([C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                    C.c_uint64, _f64p, _i32p], C.c_int),
+    "nw_hex_edges": ([C.c_int64, _i32p, C.c_int64, _i32p, C.c_int64,
+                      C.POINTER(C.c_int64), _i32p], C.c_int),
+    "nw_hex_edge_area_vector": ([C.c_int64, _i32p, _f64p, _i32p, _i32p,
+                                 C.c_int64, _f64p], C.c_int),
+    "nw_hex_dual_volume": ([C.c_int64, _i32p, _f64p, C.c_int64, _f64p], C.c_int),
+    "nw_nodal_grad_edge": ([C.c_int64, C.c_int64, _i32p, _f64p, _f64p, C.c_int,
+                            _f64p, _f64p], C.c_int),
+    "nw_mdot_edge": ([C.c_int64, _i32p, _f64p, _f64p, _f64p, _f64p], C.c_int),
+}
+for _name, (_a, _r) in _sig.items():
+    _f = getattr(lib, _name)
+    _f.argtypes = _a
+    _f.restype = _r
+
+EXPORTED = sorted(_sig)
+
+
+def last_error():
+    return lib.nw_last_error().decode()
+
+
+def check(rc):
+    if rc != OK:
+        raise NwError(rc, last_error())
+    return rc
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ------------------------------------------------------------------ mesh
+
+def hex_box(nx, ny, nz, stretch_z=1.0, jitter=0.0, seed=0):
+    nn, ne = C.c_int64(), C.c_int64()
+    check(lib.nw_hex_box_sizes(nx, ny, nz, C.byref(nn), C.byref(ne)))
+    coords = np.zeros((nn.value, 3))
+    conn = np.zeros((ne.value, 8), np.int32)
+    check(lib.nw_hex_box(nx, ny, nz, stretch_z, jitter, seed, coords, conn))
+    return coords, conn
+
+
+def hex_edges(conn, nnode):
+    conn = _i32(conn)
+    nelem = conn.shape[0]
+    cap = 3 * nnode + 12 * nelem
+    edges = np.zeros((cap, 2), np.int32)
+    ee = np.zeros((nelem, 12), np.int32)
+    ne = C.c_int64()
+    check(lib.nw_hex_edges(nelem, conn, nnode, edges, cap, C.byref(ne), ee))
+    return edges[:ne.value].copy(), ee
+
+
+def hex_edge_area_vector(conn, coords, elem_edges, edges):
+    out = np.zeros((edges.shape[0], 3))
+    check(lib.nw_hex_edge_area_vector(conn.shape[0], _i32(conn), _f64(coords),
+                                      _i32(elem_edges), _i32(edges),
+                                      edges.shape[0], out))
+    return out
+
+
+def hex_dual_volume(conn, coords):
+    out = np.zeros(coords.shape[0])
+    check(lib.nw_hex_dual_volume(conn.shape[0], _i32(conn), _f64(coords),
+                                 coords.shape[0], out))
+    return out
+
+
+def nodal_grad_edge(edges, areav, dualvol, phi):
+    phi = _f64(phi)
+    dim1 = 1 if phi.ndim == 1 else phi.shape[1]
+    n = dualvol.shape[0]
+    out = np.zeros((n, 3 * dim1))
+    check(lib.nw_nodal_grad_edge(n, edges.shape[0], _i32(edges), _f64(areav),
+                                 _f64(dualvol), dim1, phi, out))
+    return out
+
+
+def mdot_edge(edges, areav, rho, vel):
+    out = np.zeros(edges.shape[0])
+    check(lib.nw_mdot_edge(edges.shape[0], _i32(edges), _f64(areav), _f64(rho),
+                           _f64(vel), out))
+    return out
+
+
+# ------------------------------------------------------------------ context
+
+class Context:
+    """One GPU context: mesh, fields, one linear system, solver workspace."""
+
+    def __init__(self, device=0):
+        self.h = _P()
+        check(lib.nw_ctx_create(device, C.byref(self.h)))
+        self.nnode = 0
+        self.nedge = 0
+
+    def close(self):
+        if self.h:
+            lib.nw_ctx_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def stream(self):
+        s = _P()
+        check(lib.nw_ctx_stream(self.h, C.byref(s)))
+        return s.value
+
+    def synchronize(self):
+        check(lib.nw_ctx_synchronize(self.h))
+
+    def mesh_set(self, nnode, edges):
+        edges = _i32(edges).reshape(-1, 2)
+        check(lib.nw_mesh_set(self.h, nnode, edges.shape[0], edges))
+        self.nnode, self.nedge = nnode, edges.shape[0]
+
+    def field_upload(self, fid, arr):
+        a = _f64(arr).ravel()
+        check(lib.nw_field_upload(self.h, fid, a, a.size))
+
+    def field_download(self, fid):
+        nc = lib.nw_field_components(fid)
+        edge = fid >= E_AREA_VECTOR
+        n = (self.nedge if edge else self.nnode) * nc
+        out = np.zeros(n)
+        check(lib.nw_field_download(self.h, fid, out, n))
+        return out.reshape(-1, nc) if nc > 1 else out
+
+    def field_fill(self, fid, value):
+        check(lib.nw_field_fill(self.h, fid, value))
+
+    def field_ptr(self, fid):
+        p = _P()
+        check(lib.nw_field_device_ptr(self.h, fid, C.byref(p)))
+        return p.value
+
+    def build_graph(self, ndof):
+        check(lib.nw_linsys_build_edge_graph(self.h, ndof))
+        self.ndof = ndof
+
+    def zero(self):
+        check(lib.nw_linsys_zero(self.h))
+
+    def load_complete(self):
+        check(lib.nw_linsys_load_complete(self.h))
+
+    def sizes(self):
+        a, b = C.c_int64(), C.c_int64()
+        check(lib.nw_linsys_sizes(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def download(self):
+        """(row_ptr, cols, vals, rhs) in the Tpetra point-CSR layout."""
+        nrows, nnz = self.sizes()
+        rp = np.zeros(nrows + 1, np.int64)
+        cols = np.zeros(nnz, np.int32)
+        vals = np.zeros(nnz)
+        rhs = np.zeros(nrows)
+        check(lib.nw_linsys_download(self.h, rp.ctypes.data, cols.ctypes.data,
+                                     vals.ctypes.data, rhs.ctypes.data))
+        return rp, cols, vals, rhs
+
+    def device_arrays(self):
+        ps = [_P() for _ in range(4)]
+        check(lib.nw_linsys_device_arrays(self.h, *[C.byref(p) for p in ps]))
+        return [p.value for p in ps]
+
+    def sum_into(self, nodes, lhs, rhs):
+        nodes = _i32(nodes)
+        nb = nodes.shape[0] if nodes.ndim == 2 else 1
+        nent = nodes.shape[-1]
+        check(lib.nw_linsys_sum_into(self.h, nb, nent, nodes.ravel(),
+                                     _f64(lhs).ravel(), _f64(rhs).ravel()))
+
+    def reset_rows(self, nodes, begin, end, diag=0.0, rhs_residual=0.0):
+        nodes = _i32(nodes)
+        check(lib.nw_linsys_reset_rows(self.h, nodes.size, nodes, begin, end,
+                                       diag, rhs_residual))
+
+    def apply_dirichlet(self, nodes, sol_field, bc_field, begin, end):
+        nodes = _i32(nodes)
+        check(lib.nw_linsys_apply_dirichlet(self.h, nodes.size, nodes, sol_field,
+                                            bc_field, begin, end))
+
+    def fix_pressure_at_node(self, node, pref):
+        check(lib.nw_fix_pressure_at_node(self.h, int(node), pref))
+
+    def assemble_scalar(self, opts):
+        check(lib.nw_assemble_scalar(self.h, C.byref(opts)))
+
+    def assemble_momentum(self, opts):
+        check(lib.nw_assemble_momentum(self.h, C.byref(opts)))
+
+    def assemble_continuity(self, opts):
+        check(lib.nw_assemble_continuity(self.h, C.byref(opts)))
+
+    def momentum_edge_peclet(self, peclet):
+        check(lib.nw_momentum_edge_peclet(self.h, C.byref(peclet)))
+
+    def solve(self, cfg, raise_on_noconv=True):
+        st = SolveStats()
+        rc = lib.nw_linsys_solve(self.h, C.byref(cfg), C.byref(st))
+        if rc != OK and not (rc == E_NOCONV and not raise_on_noconv):
+            raise NwError(rc, last_error())
+        return st
+
+    def solution(self):
+        nrows, _ = self.sizes()
+        out = np.zeros(nrows)
+        check(lib.nw_linsys_solution_download(self.h, out))
+        return out
+
+    def set_solution(self, x):
+        check(lib.nw_linsys_solution_upload(self.h, _f64(x)))
+
+    def solution_ptr(self):
+        p = _P()
+        check(lib.nw_linsys_solution_device_ptr(self.h, C.byref(p)))
+        return p.value
+
+    def spmv_device(self, d_x, d_y):
+        check(lib.nw_linsys_spmv(self.h, d_x, d_y))

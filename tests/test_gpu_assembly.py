@@ -1,0 +1,311 @@
+"""GPU parity of the edge solver algorithms through the C ABI: the reference
+unit-test golds, then the oracle on seeded skewed meshes over the scheme
+options, determinism, the sumInto / resetRows / Dirichlet / fix-node row
+operations and error paths."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from tests import fixtures as fx
+from tests import problems as pb
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_golds.json")))
+
+
+@pytest.fixture(scope="module")
+def nw():
+    return pb.nw()
+
+
+@pytest.fixture
+def ctx(nw):
+    c = nw.Context(0)
+    yield c
+    c.close()
+
+
+def _dense(got, n):
+    rp, cols, vals, rhs = got
+    D = np.zeros((n, n))
+    for r in range(n):
+        for k in range(rp[r], rp[r + 1]):
+            D[r, cols[k]] += vals[k]
+    return D, rhs
+
+
+def _load_fixture(nw, ctx, f, extra):
+    ctx.mesh_set(f["coords"].shape[0], f["edges"])
+    ctx.field_upload(nw.F_COORDINATES, f["coords"])
+    ctx.field_upload(nw.E_AREA_VECTOR, f["areav"])
+    for fid, key in extra:
+        ctx.field_upload(fid, f[key])
+
+
+# ------------------------------------------------------------ golds
+
+def test_continuity_golds(nw, ctx):
+    f = fx.continuity_fixture()
+    _load_fixture(nw, ctx, f, [(nw.F_VELOCITY, "vel"), (nw.F_DPDX, "dpdx"),
+                               (nw.F_DENSITY, "rho"), (nw.F_PRESSURE, "p"),
+                               (nw.F_UDIAG, "udiag")])
+    ctx.build_graph(1)
+    ctx.assemble_continuity(nw.continuity_defaults())
+    D, rhs = _dense(ctx.download(), 8)
+    g = GOLD["continuity_adv_edge"]
+    np.testing.assert_allclose(rhs, g["rhs"], rtol=0, atol=g["rhs_tol"])
+    np.testing.assert_allclose(D.ravel(), g["lhs"], rtol=0, atol=g["lhs_tol"])
+
+
+def test_momentum_golds(nw, ctx):
+    f = fx.momentum_fixture()
+    _load_fixture(nw, ctx, f, [(nw.F_VELOCITY, "vel"), (nw.F_DUDX, "dudx"),
+                               (nw.F_VISCOSITY, "visc"), (nw.F_DENSITY, "rho"),
+                               (nw.E_MASS_FLOW_RATE, "mdot"),
+                               (nw.E_PECLET_FACTOR, "pecfac"),
+                               (nw.F_WALL_MASK, "wallmask")])
+    ctx.build_graph(3)
+    ctx.assemble_momentum(nw.momentum_defaults())
+    D, rhs = _dense(ctx.download(), 24)
+    g = GOLD["momentum_adv_diff_edge"]
+    np.testing.assert_allclose(rhs, g["rhs"], rtol=0, atol=g["tol"])
+    # the reference mock keeps same-component couplings only
+    G = np.array(g["lhs"]).reshape(24, 24)
+    same = np.zeros((24, 24), bool)
+    for d in range(3):
+        same[d::3, d::3] = True
+    np.testing.assert_allclose(D[same], G[same], rtol=0, atol=g["tol"])
+
+
+def _scalar_fixture(nw, ctx):
+    f = fx.mixture_fraction_fixture()
+    _load_fixture(nw, ctx, f, [(nw.F_VELOCITY, "vel"), (nw.F_SCALAR_Q, "q"),
+                               (nw.F_DQDX, "dqdx"), (nw.F_DENSITY, "rho"),
+                               (nw.F_DIFF_FLUX_COEFF, "visc"),
+                               (nw.E_MASS_FLOW_RATE, "mdot")])
+    ctx.build_graph(1)
+    ctx.assemble_scalar(nw.scalar_defaults())
+    return f
+
+
+def test_scalar_tpetra_golds(nw, ctx):
+    _scalar_fixture(nw, ctx)
+    rp, cols, vals, rhs = ctx.download()
+    g = GOLD["scalar_adv_diff_edge_tpetra"]
+    assert list(rp) == g["rowOffsets_serial"]
+    assert list(cols) == g["cols_serial"]
+    np.testing.assert_allclose(vals, g["vals_serial"], rtol=0, atol=g["tol"])
+    np.testing.assert_allclose(rhs, g["rhs_serial"], rtol=0, atol=g["tol"])
+
+
+def test_scalar_fix_pressure_golds(nw, ctx):
+    f = _scalar_fixture(nw, ctx)
+    ctx.field_upload(nw.F_PRESSURE, f["rho"])  # the test fixes "density"
+    g = GOLD["scalar_adv_diff_edge_tpetra"]
+    ctx.fix_pressure_at_node(g["fix_node_id"] - 1, g["fix_ref_pressure"])
+    rp, cols, vals, rhs = ctx.download()
+    np.testing.assert_allclose(vals, g["fixed_vals_serial"], rtol=0, atol=g["tol"])
+    np.testing.assert_allclose(rhs, g["fixed_rhs_serial"], rtol=0, atol=g["tol"])
+
+
+def test_scalar_dirichlet_golds(nw, ctx):
+    _scalar_fixture(nw, ctx)
+    ctx.field_upload(nw.F_SCALAR_Q, np.full(8, 2.0))
+    bc = np.zeros(8)
+    bc[0] = 1.0
+    # any 1-wide node field can hold the bc values (the reference test uses
+    # its viscosity field the same way)
+    ctx.field_upload(nw.F_DIFF_FLUX_COEFF, bc)
+    ctx.apply_dirichlet(np.arange(8), nw.F_SCALAR_Q, nw.F_DIFF_FLUX_COEFF, 0, 1)
+    rp, cols, vals, rhs = ctx.download()
+    g = GOLD["scalar_adv_diff_edge_tpetra"]
+    np.testing.assert_allclose(vals, g["dirichlet_vals_serial"], rtol=0, atol=g["tol"])
+    np.testing.assert_allclose(rhs, g["dirichlet_rhs_serial"], rtol=0, atol=g["tol"])
+
+
+# ------------------------------------------------------------ oracle parity
+
+@pytest.fixture(scope="module")
+def skewed():
+    return pb.Problem(9, 7, 8, stretch=1.15, jitter=0.25, seed=4)
+
+
+CONT_CASES = [dict(), dict(noc=0.0), dict(interp_together=0.0),
+              dict(incompressible=1.0, dt=0.3, gamma1=1.5)]
+
+
+@pytest.mark.parametrize("kw", CONT_CASES)
+def test_continuity_matches_oracle(nw, ctx, skewed, kw):
+    o = nw.continuity_defaults(**kw)
+    skewed.upload(ctx)
+    ctx.build_graph(1)
+    ctx.assemble_continuity(o)
+    pb.assert_csr_close(ctx.download(), skewed.oracle_continuity(pb.to_orc_continuity(o)))
+
+
+SCAL_CASES = [dict(), dict(alpha=0.5, alpha_upw=0.3, upw=0.7, relax=0.8),
+              dict(use_limiter=1, alpha=1.0),
+              dict(peclet=("classic", 1.0)), dict(peclet=("tanh", 0.0))]
+
+
+def _scal_opts(nw, kw):
+    kw = dict(kw)
+    o = nw.scalar_defaults()
+    if "peclet" in kw:
+        form, hf = kw.pop("peclet")
+        o.peclet = nw.Peclet(0, hf, 2.0, 4.0) if form == "classic" else nw.Peclet(1, 0.0, 2.0, 4.0)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+@pytest.mark.parametrize("kw", SCAL_CASES)
+def test_scalar_matches_oracle(nw, ctx, skewed, kw):
+    o = _scal_opts(nw, kw)
+    skewed.upload(ctx)
+    ctx.build_graph(1)
+    ctx.assemble_scalar(o)
+    pb.assert_csr_close(ctx.download(), skewed.oracle_scalar(pb.to_orc_scalar(o)))
+
+
+MOM_CASES = [dict(), dict(use_limiter=1, include_divU=1.0),
+             dict(alpha=0.3, alpha_upw=0.6, upw=0.5, relax=0.7)]
+
+
+@pytest.mark.parametrize("kw", MOM_CASES)
+def test_momentum_matches_oracle(nw, ctx, skewed, kw):
+    o = nw.momentum_defaults(**kw)
+    skewed.upload(ctx)
+    ctx.build_graph(3)
+    ctx.assemble_momentum(o)
+    pb.assert_csr_close(ctx.download(), skewed.oracle_momentum(pb.to_orc_momentum(o)))
+
+
+def test_uniform_32_cubed_scalar_config(nw, ctx):
+    """32^3 uniform box, Taylor-Green velocity, mu = 1e-3, seeded phi with
+    its edge nodal gradient: the scalar workload, against the oracle."""
+    P = pb.Problem(32, 32, 32, seed=7)
+    x, y, z = P.coords.T
+    P.vel = np.stack([np.sin(x) * np.cos(y) * np.cos(z),
+                      -np.cos(x) * np.sin(y) * np.cos(z), 0 * z], 1)
+    P.rho = np.ones(P.n)
+    P.dflux = np.full(P.n, 1e-3)
+    P.q = np.random.default_rng(7).uniform(0, 1, P.n)
+    P.dqdx = pb.nw().nodal_grad_edge(P.edges, P.areav, P.dualvol, P.q)
+    P.mdot = pb.nw().mdot_edge(P.edges, P.areav, P.rho, P.vel)
+    P.upload(ctx)
+    ctx.build_graph(1)
+    o = nw.scalar_defaults()
+    ctx.assemble_scalar(o)
+    pb.assert_csr_close(ctx.download(), P.oracle_scalar(pb.to_orc_scalar(o)))
+
+
+def test_rows_spanning_chunks(nw, ctx):
+    """Rows longer than a 64-lane chunk: a star graph whose hub has 150
+    neighbours, so the hub row spans three chunks."""
+    rng = np.random.default_rng(3)
+    n = 151
+    edges = np.stack([np.zeros(150, np.int32), np.arange(1, 151, dtype=np.int32)], 1)
+    coords = rng.normal(size=(n, 3))
+    coords[0] = 0
+    areav = coords[1:] * rng.uniform(0.5, 1.5, (150, 1))
+    fields = dict(vel=rng.normal(size=(n, 3)), dpdx=rng.normal(size=(n, 3)),
+                  rho=rng.uniform(0.5, 2, n), p=rng.normal(size=n),
+                  udiag=rng.uniform(0.5, 2, n))
+    ctx.mesh_set(n, edges)
+    ctx.field_upload(nw.F_COORDINATES, coords)
+    ctx.field_upload(nw.E_AREA_VECTOR, areav)
+    for fid, k in [(nw.F_VELOCITY, "vel"), (nw.F_DPDX, "dpdx"), (nw.F_DENSITY, "rho"),
+                   (nw.F_PRESSURE, "p"), (nw.F_UDIAG, "udiag")]:
+        ctx.field_upload(fid, fields[k])
+    ctx.build_graph(1)
+    o = nw.continuity_defaults()
+    ctx.assemble_continuity(o)
+    A = orc.CsrSystem(n, edges, 1)
+    orc.assemble_continuity(pb.to_orc_continuity(o), n, edges, coords, fields["vel"],
+                            fields["dpdx"], fields["rho"], fields["p"],
+                            fields["udiag"], areav, csr=A)
+    pb.assert_csr_close(ctx.download(), A)
+
+
+def test_deterministic_and_accumulate(nw, ctx, skewed):
+    skewed.upload(ctx)
+    ctx.build_graph(3)
+    o = nw.momentum_defaults(use_limiter=1)
+    ctx.assemble_momentum(o)
+    a = ctx.download()
+    ctx.assemble_momentum(o)
+    b = ctx.download()
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    o.accumulate = 1
+    ctx.assemble_momentum(o)
+    c = ctx.download()
+    assert np.array_equal(c[2], 2 * a[2]) and np.array_equal(c[3], 2 * a[3])
+
+
+def test_momentum_peclet_matches_oracle(nw, ctx, skewed):
+    skewed.upload(ctx)
+    for pf in [nw.Peclet(0, 1.0, 2.0, 4.0), nw.Peclet(1, 0.0, 5.0, 3.0)]:
+        ctx.momentum_edge_peclet(pf)
+        got = ctx.field_download(nw.E_PECLET_FACTOR)
+        ref = orc.momentum_edge_peclet(skewed.edges, skewed.coords, skewed.vel,
+                                       skewed.rho, skewed.visc,
+                                       orc.Peclet(pf.form, pf.hybrid, pf.c1, pf.c2))
+        np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-15)
+
+
+# ------------------------------------------------------------ row operations
+
+def test_sum_into_reset_and_dirichlet_match_oracle(nw, ctx, skewed):
+    skewed.upload(ctx)
+    ctx.build_graph(3)
+    o = nw.momentum_defaults()
+    ctx.assemble_momentum(o)
+    A = skewed.oracle_momentum(pb.to_orc_momentum(o))
+    rng = np.random.default_rng(0)
+    e = skewed.edges[5]
+    lhs = rng.normal(size=(6, 6))
+    rhs = rng.normal(size=6)
+    ctx.sum_into(e, lhs, rhs)
+    A.sum_into(e, lhs, rhs)
+    nodes = np.array([3, 10, 11], np.int32)
+    ctx.reset_rows(nodes, 1, 3, 2.5, -1.0)
+    A.reset_rows(nodes, 1, 3, 2.5, -1.0)
+    sol = rng.normal(size=(skewed.n, 3))
+    bc = rng.normal(size=(skewed.n, 3))
+    ctx.field_upload(nw.F_VELOCITY, sol)
+    ctx.field_upload(nw.F_BC_VALUE, bc)
+    dn = np.array([0, 1, 40], np.int32)
+    ctx.apply_dirichlet(dn, nw.F_VELOCITY, nw.F_BC_VALUE, 0, 3)
+    A.apply_dirichlet(dn, 0, 3, sol, bc)
+    pb.assert_csr_close(ctx.download(), A)
+
+
+def test_sum_into_missing_column_is_an_error(nw, ctx, skewed):
+    skewed.upload(ctx)
+    ctx.build_graph(1)
+    before = ctx.download()
+    far = np.array([0, skewed.n - 1], np.int32)   # not an edge
+    with pytest.raises(nw.NwError) as ei:
+        ctx.sum_into(far, np.ones((2, 2)), np.ones(2))
+    assert ei.value.code == nw.E_NOGRAPHCOL
+    after = ctx.download()
+    assert np.array_equal(before[2], after[2])
+
+
+def test_bad_calls_are_errors(nw, ctx, skewed):
+    with pytest.raises(nw.NwError):
+        ctx.assemble_continuity(nw.continuity_defaults())  # no mesh/graph
+    skewed.upload(ctx)
+    ctx.build_graph(1)
+    with pytest.raises(nw.NwError):
+        ctx.assemble_momentum(nw.momentum_defaults())  # 1-dof graph
+    with pytest.raises(nw.NwError):
+        ctx.field_upload(nw.F_VELOCITY, np.zeros(5))
+    with pytest.raises(nw.NwError):
+        ctx.mesh_set(4, np.array([[0, 0]], np.int32))
