@@ -1,0 +1,309 @@
+#!/usr/bin/env python3
+"""Benchmark of the edge-assembly + pressure-solve path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+                    [--solve-size N] [--no-cpu]
+
+Headline (one JSON line on stdout): momentum edge assembly (FP64, 3 dof,
+3x3 block CSR) on a 128^3-cell box stretched 1.05 toward z = 0 with +-10 %
+node jitter, log-law velocity + 5 % noise, nu_t ~ U(0, 1e-2), alpha = 0.5.
+A step is MomentumEdgePecletAlg + MomentumEdgeSolverAlg over all edges
+(inputs resident in HBM), value = edges / step time in Medges/s.  The line
+also carries the continuity (256^3) and scalar (32^3) assembly rates, the
+pressure-Poisson solves, the roofline of the assembly kernel measured with
+HIP events on the library's stream, and the CPU oracle timed on this host.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nalu-wind_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Medges/s assembled + %HBM peak; pressure solve ms to 1e-8 rel. residual"
+
+
+class Hip:
+    """hipEvent timing on a given stream through the HIP runtime the
+    library already loaded (same process-wide runtime)."""
+
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so.7")
+        self.lib.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.lib.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.lib.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.lib.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p,
+                                                 C.c_void_p]
+        self.lib.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.lib.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.lib.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed(self, a, b):
+        assert self.lib.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.lib.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+    def sync(self):
+        assert self.lib.hipDeviceSynchronize() == 0
+
+
+def momentum_bytes(W, nnzb):
+    # node fields read once: coords 3, velocity 3, dudx 9, viscosity 1,
+    # wall mask 1 (17 f64); edge: area vector 3 f64, mdot, peclet factor,
+    # two int32 node ids; block values written once (9 f64 per block), rhs
+    # (3 f64 per node)
+    return 17 * 8 * W.n + (5 * 8 + 8) * W.ne + 72 * nnzb + 24 * W.n
+
+
+def continuity_bytes(W, nnz):
+    # coords 3, velocity 3, dpdx 3, density, pressure, Udiag (12 f64);
+    # edge: area vector + node ids; values + rhs written once
+    return 12 * 8 * W.n + (24 + 8) * W.ne + 8 * nnz + 8 * W.n
+
+
+def scalar_bytes(W, nnz):
+    # coords 3, velocity 3, q, dqdx 3, density, diffusion coeff (12 f64);
+    # edge: area vector, mdot, node ids
+    return 12 * 8 * W.n + (32 + 8) * W.ne + 8 * nnz + 8 * W.n
+
+
+def time_launches(hip, ctx, fn, steps, warmup):
+    """Times `steps` calls of fn with events around each call (kernel
+    durations) and around the whole loop; returns (avg_ms_per_call,
+    loop_ms)."""
+    s = ctx.stream()
+    for _ in range(warmup):
+        fn()
+    hip.sync()
+    ev = [hip.event() for _ in range(2 * steps + 2)]
+    hip.record(ev[0], s)
+    for i in range(steps):
+        hip.record(ev[1 + 2 * i], s)
+        fn()
+        hip.record(ev[2 + 2 * i], s)
+    hip.record(ev[-1], s)
+    total = hip.elapsed(ev[0], ev[-1])
+    per = [hip.elapsed(ev[1 + 2 * i], ev[2 + 2 * i]) for i in range(steps)]
+    return sum(per) / steps, total
+
+
+def cpu_momentum_baseline(seconds=12.0, n=48):
+    """The oracle (single-threaded C restatement of MomentumEdgeSolverAlg +
+    sum_into) on a bounded sample: a 48^3 box of the same configuration,
+    repeated for about `seconds`."""
+    sys.path.insert(0, ROOT)
+    from oracle import orc
+    import workloads as wl
+    import naluwind_amd as nw
+    W = wl.momentum_abl(n)
+    f = W.fields
+    pecfac = orc.momentum_edge_peclet(W.edges, W.coords, f[nw.F_VELOCITY],
+                                      f[nw.F_DENSITY], f[nw.F_VISCOSITY],
+                                      orc.Peclet(0, 1.0, 2.0, 4.0))
+    o = orc.MomentumOpts(0.5, 1.0, 1.0, 1.0, 0.0, 0)
+    A = orc.CsrSystem(W.n, W.edges, 3)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        orc.assemble_momentum(o, W.n, W.edges, W.coords, f[nw.F_VELOCITY],
+                              f[nw.F_DUDX], f[nw.F_VISCOSITY], f[nw.F_DENSITY],
+                              W.areav, f[nw.E_MASS_FLOW_RATE], pecfac,
+                              f[nw.F_WALL_MASK], csr=A)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > seconds:
+            break
+    return {"value": round(reps * W.ne / el / 1e6, 3), "unit": "Medges/s",
+            "cores": 1, "kind": "port",
+            "sample": "oracle momentum assembly (C, serial, CSR sum_into), "
+                      "%d^3-cell box of the same configuration (%d edges) x %d "
+                      "repetitions in %.1f s" % (n, W.ne, reps, el)}
+
+
+def read_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if one
+    exists for this kernel/workload (line 'traffic_bytes_per_launch: X')."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--quick", action="store_true", help="small sizes (testing)")
+    ap.add_argument("--solve-size", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the multi-GPU path (node "
+              "partition + shared-row export over RCCL) is a design, not a "
+              "build; use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import naluwind_amd as nw
+    import workloads as wl
+    hip = Hip()
+    log = lambda *x: print(*x, file=sys.stderr, flush=True)
+    n2 = 32 if a.quick else 128
+    n3 = 48 if a.quick else 256
+    n1 = 16 if a.quick else 32
+
+    out = {"metric": METRIC, "unit": "Medges/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+
+    # ---------------------------------------------------------- momentum
+    t0 = time.perf_counter()
+    W = wl.momentum_abl(n2)
+    log("momentum workload %d^3: %d nodes %d edges (%.1fs)" % (n2, W.n, W.ne,
+                                                               time.perf_counter() - t0))
+    with nw.Context(0) as ctx:
+        W.upload(ctx)
+        ctx.build_graph(3)
+        _, nnz = ctx.sizes()
+        nnzb = nnz // 9
+
+        def step():
+            ctx.momentum_edge_peclet(W.peclet)
+            ctx.assemble_momentum(W.opts)
+
+        for _ in range(a.warmup):
+            step()
+        hip.sync()
+        s = ctx.stream()
+        ev = [hip.event() for _ in range(2 * a.steps + 2)]
+        wall0 = time.perf_counter()
+        hip.record(ev[0], s)
+        for i in range(a.steps):
+            ctx.momentum_edge_peclet(W.peclet)
+            hip.record(ev[1 + 2 * i], s)
+            ctx.assemble_momentum(W.opts)
+            hip.record(ev[2 + 2 * i], s)
+        hip.record(ev[-1], s)
+        hip.sync()
+        wall = time.perf_counter() - wall0
+        total_ms = hip.elapsed(ev[0], ev[-1])
+        kern_ms = sum(hip.elapsed(ev[1 + 2 * i], ev[2 + 2 * i])
+                      for i in range(a.steps)) / a.steps
+        ms_step = total_ms / a.steps
+        mbytes = momentum_bytes(W, nnzb)
+        achieved = mbytes / (kern_ms * 1e-3) / 1e9
+        out["value"] = round(W.ne / (ms_step * 1e-3) / 1e6, 2)
+        out["ms_per_step"] = round(ms_step, 4)
+        out["config"] = {
+            "workload": "momentum edge assembly, %d^3-cell Hex8 box stretched "
+                        "1.05 toward z=0, +-10%% jitter, log-law u + 5%% noise, "
+                        "nu_t~U(0,1e-2), alpha=0.5, hybrid 1" % n2,
+            "nodes": W.n, "edges": W.ne, "block_nnz": nnzb, "ndof": 3,
+            "step": "MomentumEdgePecletAlg + MomentumEdgeSolverAlg (rows "
+                    "overwritten: fused zeroSystem)",
+            "wall_s_timed_region": round(wall, 4)}
+        traffic = read_traffic("momentum_abl_128_assembly.txt")
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4),
+                           "traffic": traffic,
+                           "kernel": "k_assemble_rows<MomentumPhys>",
+                           "kernel_avg_ms": round(kern_ms, 4),
+                           "algorithmic_bytes_per_launch": mbytes}
+        log("momentum: %.3f ms/step, assembly kernel %.3f ms, %.0f GB/s" %
+            (ms_step, kern_ms, achieved))
+    del W
+
+    # ---------------------------------------------------------- scalar
+    W = wl.scalar_tg(n1)
+    with nw.Context(0) as ctx:
+        W.upload(ctx)
+        ctx.build_graph(1)
+        _, nnz = ctx.sizes()
+        kms, tot = time_launches(hip, ctx, lambda: ctx.assemble_scalar(W.opts),
+                                 a.steps, a.warmup)
+        out["scalar_assembly"] = {
+            "workload": "scalar adv-diff %d^3 uniform, Taylor-Green u, mu=1e-3" % n1,
+            "nodes": W.n, "edges": W.ne, "Medges_per_s": round(W.ne / (kms * 1e-3) / 1e6, 1),
+            "kernel_avg_ms": round(kms, 5),
+            "GBps_algorithmic": round(scalar_bytes(W, nnz) / (kms * 1e-3) / 1e9, 1),
+            "note": "~10 MB working set: cache-resident, no HBM claim"}
+    del W
+
+    # ---------------------------------------------------------- continuity
+    t0 = time.perf_counter()
+    W = wl.continuity_rf(n3)
+    log("continuity workload %d^3: %d nodes %d edges (%.1fs)" % (n3, W.n, W.ne,
+                                                                 time.perf_counter() - t0))
+    with nw.Context(0) as ctx:
+        W.upload(ctx)
+        ctx.build_graph(1)
+        _, nnz = ctx.sizes()
+        kms, tot = time_launches(hip, ctx, lambda: ctx.assemble_continuity(W.opts),
+                                 a.steps, a.warmup)
+        cb = continuity_bytes(W, nnz)
+        out["continuity_assembly"] = {
+            "workload": "continuity %d^3-cell box stretched 1.05, random-Fourier u (8 modes)" % n3,
+            "nodes": W.n, "edges": W.ne, "nnz": nnz,
+            "Medges_per_s": round(W.ne / (kms * 1e-3) / 1e6, 1),
+            "kernel_avg_ms": round(kms, 4),
+            "GBps_algorithmic": round(cb / (kms * 1e-3) / 1e9, 1),
+            "frac_hbm_peak": round(cb / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes_per_launch": cb}
+        log("continuity: %.3f ms, %.0f GB/s" % (kms, cb / (kms * 1e-3) / 1e9))
+    del W
+
+    # ---------------------------------------------------------- solve
+    if not a.no_solve:
+        ns = 24 if a.quick else a.solve_size
+        W = wl.continuity_rf(ns)
+        res = {"workload": "pressure Poisson from the continuity assembly, %d^3 "
+                           "cells stretched 1.05, Dirichlet p=0 on z=1, tol 1e-8 "
+                           "on ||b-Ax||/||b||, GMRES(50), x0=0" % ns,
+               "nodes": W.n}
+        with nw.Context(0) as ctx:
+            W.upload(ctx)
+            ctx.build_graph(1)
+            ctx.assemble_continuity(W.opts)
+            wl.apply_pressure_bc(ctx, W)
+            for name, mode in [("fp64", nw.SOLVE_FP64),
+                               ("fp32_refined", nw.SOLVE_FP32_REFINE)]:
+                cfg = nw.solver_cfg(kspace=50, max_iters=4000, tol=1e-8,
+                                    precond=nw.PREC_CHEBYSHEV, cheb_degree=8,
+                                    cheb_ratio=30.0, mode=mode)
+                st = ctx.solve(cfg, raise_on_noconv=False)
+                res[name] = {"precond": "chebyshev(8) on D^-1 A", "ms": round(st.ms, 1),
+                             "iters": st.iters, "cycles": st.refinements,
+                             "final_rel_residual": st.final_rel,
+                             "converged": bool(st.converged)}
+                log("solve %s: %s" % (name, res[name]))
+        if res["fp64"]["converged"] and res["fp32_refined"]["converged"]:
+            res["speedup_fp32_refined_vs_fp64"] = round(res["fp64"]["ms"] / res["fp32_refined"]["ms"], 3)
+        out["pressure_solve"] = res
+        del W
+
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_momentum_baseline(3.0 if a.quick else 12.0,
+                                                    16 if a.quick else 48)
+        log("cpu baseline: %s" % out["cpu_baseline"])
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -211,7 +211,8 @@ typedef struct {
 
 typedef struct {
   int iters;            /* Krylov iterations (all refinement steps) */
-  int refinements;      /* outer FP64 refinement steps (mode 1) */
+  int refinements;      /* restart cycles = FP64 refinement steps */
+  int reorthogonalizations; /* second Gram-Schmidt passes taken */
   int converged;
   double final_rel;     /* explicit ||b - A x||_2 / ||b||_2 in FP64 */
   double final_abs;     /* explicit ||b - A x||_2 (the reference's report) */

@@ -76,7 +76,7 @@ class SolverCfg(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("refinements", C.c_int),
-                ("converged", C.c_int), ("final_rel", C.c_double),
+                ("reorthogonalizations", C.c_int), ("converged", C.c_int), ("final_rel", C.c_double),
                 ("final_abs", C.c_double), ("bnorm", C.c_double),
                 ("ms", C.c_double)]
 

@@ -1,0 +1,128 @@
+"""Synthetic workloads of the edge-assembly / pressure-solve benchmarks.
+
+Each builder returns a Workload: a Hex8 box mesh (library generator), its
+edges and geometry, and the seeded node/edge fields of one configuration:
+
+  scalar_tg      uniform box, Taylor-Green velocity u = sin x cos y cos z,
+                 v = -cos x sin y cos z, w = 0; rho = 1, mu = 1e-3,
+                 phi ~ U(0,1) seeded, dphi/dx from the edge nodal gradient,
+                 mdot = 0.5 (rho u_L + rho u_R).A
+  momentum_abl   box stretched 1.05 per cell toward z = 0, +-10 % node
+                 jitter, log-law u(z) plus 5 % Gaussian noise, nu_t ~
+                 U(0, 1e-2), mu_eff = mu + rho nu_t, du/dx from the edge
+                 nodal gradient, scheme alpha = 0.5
+  continuity_rf  stretched box, divergence-free random Fourier velocity
+                 (8 modes), rho = 1, p = 0, dp/dx = 0, Udiag = gamma1/dt;
+                 Dirichlet p = 0 on the top face z = 1
+"""
+import math
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import naluwind_amd as nw  # noqa: E402
+
+
+class Workload:
+    def __init__(self, name, nx, ny, nz, stretch, jitter, seed):
+        self.name = name
+        self.cells = (nx, ny, nz)
+        self.coords, self.conn = nw.hex_box(nx, ny, nz, stretch, jitter, seed)
+        self.n = self.coords.shape[0]
+        self.edges, elem_edges = nw.hex_edges(self.conn, self.n)
+        self.ne = self.edges.shape[0]
+        self.areav = nw.hex_edge_area_vector(self.conn, self.coords, elem_edges,
+                                             self.edges)
+        del elem_edges
+        self.dualvol = nw.hex_dual_volume(self.conn, self.coords)
+        self.fields = {}   # field id -> array
+        self.top = None    # Dirichlet nodes (continuity)
+
+    def upload(self, ctx):
+        ctx.mesh_set(self.n, self.edges)
+        ctx.field_upload(nw.F_COORDINATES, self.coords)
+        ctx.field_upload(nw.E_AREA_VECTOR, self.areav)
+        ctx.field_upload(nw.F_DUAL_VOLUME, self.dualvol)
+        for fid, a in self.fields.items():
+            ctx.field_upload(fid, a)
+
+    def drop_elements(self):
+        self.conn = None
+
+
+def scalar_tg(n=32, seed=1234):
+    W = Workload("scalar_tg", n, n, n, 1.0, 0.0, seed)
+    x, y, z = W.coords.T
+    vel = np.stack([np.sin(x) * np.cos(y) * np.cos(z),
+                    -np.cos(x) * np.sin(y) * np.cos(z), np.zeros_like(z)], 1)
+    rho = np.ones(W.n)
+    phi = np.random.default_rng(seed).uniform(0.0, 1.0, W.n)
+    W.fields = {
+        nw.F_VELOCITY: vel, nw.F_DENSITY: rho, nw.F_SCALAR_Q: phi,
+        nw.F_DIFF_FLUX_COEFF: np.full(W.n, 1.0e-3),
+        nw.F_DQDX: nw.nodal_grad_edge(W.edges, W.areav, W.dualvol, phi),
+        nw.E_MASS_FLOW_RATE: nw.mdot_edge(W.edges, W.areav, rho, vel),
+    }
+    W.opts = nw.scalar_defaults()
+    W.drop_elements()
+    return W
+
+
+def momentum_abl(n=128, seed=1234, mu=1.8e-5):
+    W = Workload("momentum_abl", n, n, n, 1.05, 0.10, seed)
+    rng = np.random.default_rng(seed)
+    z = W.coords[:, 2]
+    uref, z0 = 10.0, 1.0e-3
+    vel = np.zeros((W.n, 3))
+    vel[:, 0] = uref * np.log1p(z / z0) / math.log1p(1.0 / z0)
+    vel += 0.05 * uref * rng.standard_normal((W.n, 3))
+    rho = np.ones(W.n)
+    nut = rng.uniform(0.0, 1.0e-2, W.n)
+    W.fields = {
+        nw.F_VELOCITY: vel, nw.F_DENSITY: rho, nw.F_VISCOSITY: mu + rho * nut,
+        nw.F_DUDX: nw.nodal_grad_edge(W.edges, W.areav, W.dualvol, vel),
+        nw.F_WALL_MASK: np.ones(W.n),
+        nw.E_MASS_FLOW_RATE: nw.mdot_edge(W.edges, W.areav, rho, vel),
+    }
+    W.opts = nw.momentum_defaults(alpha=0.5)
+    W.peclet = nw.Peclet(0, 1.0, 2.0, 4.0)   # hybrid factor 1: blend active
+    W.drop_elements()
+    return W
+
+
+def fourier_velocity(coords, seed, modes=8, t=0.0):
+    """Divergence-free sum of 8 seeded Fourier modes b_m sin(k_m.x + phi_m +
+    w_m t) with b_m orthogonal to k_m."""
+    rng = np.random.default_rng(seed)
+    u = np.zeros_like(coords)
+    for _ in range(modes):
+        k = 2.0 * math.pi * rng.integers(1, 4, size=3) * rng.choice([-1, 1], 3)
+        b = rng.standard_normal(3)
+        b -= (b @ k) / (k @ k) * k
+        ph = rng.uniform(0, 2 * math.pi)
+        om = rng.uniform(0.5, 2.0)
+        u += np.sin(coords @ k + ph + om * t)[:, None] * b[None, :]
+    return u
+
+
+def continuity_rf(n=256, seed=1234, stretch=1.05, dt=1.0e-3, t=0.0):
+    W = Workload("continuity_rf", n, n, n, stretch, 0.0, seed)
+    vel = fourier_velocity(W.coords, seed, t=t)
+    W.fields = {
+        nw.F_VELOCITY: vel, nw.F_DENSITY: np.ones(W.n),
+        nw.F_PRESSURE: np.zeros(W.n), nw.F_DPDX: np.zeros((W.n, 3)),
+        nw.F_UDIAG: np.full(W.n, 1.0 / dt),
+        nw.F_DIFF_FLUX_COEFF: np.zeros(W.n),
+    }
+    W.opts = nw.continuity_defaults(dt=dt, gamma1=1.0)
+    W.top = np.where(W.coords[:, 2] >= 1.0 - 1e-12)[0].astype(np.int32)
+    W.seed = seed
+    W.drop_elements()
+    return W
+
+
+def apply_pressure_bc(ctx, W):
+    """Dirichlet p = 0 on the top face (solution field = bc field = 0)."""
+    ctx.apply_dirichlet(W.top, nw.F_PRESSURE, nw.F_DIFF_FLUX_COEFF, 0, 1)
