@@ -35,9 +35,12 @@ enum {
 };
 
 /* ---- node and edge fields ------------------------------------------------
- * Node fields are [nnode][ncomp] doubles (components of a node adjacent, as
- * in STK field data); edge fields are [nedge][ncomp].  Field names follow
- * the reference's registered field names. */
+ * Upload/download arrays are [n][ncomp] doubles (components of a node
+ * adjacent, as in STK field data), n = nnode for node fields and nedge for
+ * edge fields.  On the device the library keeps every field component-major
+ * ([ncomp][n], SoA) so that lanes reading consecutive nodes or edges read
+ * consecutive addresses; nw_field_device_ptr exposes that layout.  Field
+ * names follow the reference's registered field names. */
 enum {
   NW_F_COORDINATES = 0, /* 3  "coordinates" */
   NW_F_VELOCITY,        /* 3  "velocity" (== velocity_rtm on static mesh) */
@@ -116,7 +119,8 @@ int nw_mesh_set(nw_ctx* ctx, int64_t nnode, int64_t nedge,
 int nw_field_upload(nw_ctx* ctx, int field, const double* host, int64_t count);
 int nw_field_download(nw_ctx* ctx, int field, double* host, int64_t count);
 int nw_field_fill(nw_ctx* ctx, int field, double value);
-/* Device pointer of the field (allocated on first use), [n][ncomp]. */
+/* Device pointer of the field (allocated on first use), component-major
+ * [ncomp][n]. */
 int nw_field_device_ptr(nw_ctx* ctx, int field, double** dptr);
 int nw_field_components(int field);
 
@@ -158,7 +162,8 @@ int nw_linsys_reset_rows(nw_ctx* ctx, int64_t nnodes, const int32_t* nodes,
                          int begin, int end, double diag_value,
                          double rhs_residual);
 /* applyDirichletBCs (src/TpetraLinearSystem.C:1686-1757): rows -> identity,
- * rhs = bc_field - sol_field for dofs [begin, end). Host node list. */
+ * rhs = bc_field - sol_field for dofs [begin, end). Host node list; the two
+ * node fields have the same component count. */
 int nw_linsys_apply_dirichlet(nw_ctx* ctx, int64_t nnodes,
                               const int32_t* nodes, int sol_field,
                               int bc_field, int begin, int end);
@@ -186,7 +191,13 @@ int nw_assemble_continuity(nw_ctx* ctx, const nw_continuity_opts* opts);
 int nw_momentum_edge_peclet(nw_ctx* ctx, const nw_peclet* peclet);
 
 /* ---- linear solve (TpetraLinearSystem::solve -> Belos GMRES) ------------ */
-enum { NW_PREC_NONE = 0, NW_PREC_JACOBI = 1, NW_PREC_CHEBYSHEV = 2 };
+enum {
+  NW_PREC_NONE = 0,
+  NW_PREC_JACOBI = 1,    /* Ifpack2 RELAXATION Jacobi, 1 sweep */
+  NW_PREC_CHEBYSHEV = 2, /* Jacobi-scaled Chebyshev polynomial */
+  NW_PREC_AMG = 3        /* smoothed-aggregation AMG V-cycle (the role of
+                            MueLu with reg_tests/xml/milestone.xml) */
+};
 enum {
   NW_SOLVE_FP64 = 0,       /* plain FP64 GMRES(m) */
   NW_SOLVE_FP32_REFINE = 1 /* FP32 inner GMRES + FP64 iterative refinement */
@@ -207,6 +218,8 @@ typedef struct {
   int mode;           /* NW_SOLVE_* */
   double inner_tol;   /* FP32 inner relative tolerance per refinement step */
   int x0;             /* NW_X0_* */
+  int reuse_preconditioner; /* AMG: keep an existing hierarchy (the
+                               reference's reuse_preconditioner option) */
 } nw_solver_cfg;
 
 typedef struct {
@@ -217,7 +230,15 @@ typedef struct {
   double final_rel;     /* explicit ||b - A x||_2 / ||b||_2 in FP64 */
   double final_abs;     /* explicit ||b - A x||_2 (the reference's report) */
   double bnorm;
-  double ms;            /* device time of the solve on the context stream */
+  double ms;            /* device time of the solve on the context stream,
+                           preconditioner setup excluded */
+  double setup_ms;      /* preconditioner setup of this call (0 if reused) */
+  int precond_levels;   /* AMG levels (0 for other preconditioners) */
+  int guess;            /* NW_X0_* of the guess GMRES started from (a learned
+                           guess that did not lower the residual falls back
+                           to NW_X0_SOLUTION) */
+  double x0_rel;        /* ||b - A x0||_2 / ||b||_2 of that guess */
+  double guess_ms;      /* wall time of the learned prediction (in ms) */
 } nw_solve_stats;
 
 /* Solve A x = rhs into the solution vector (the pressure increment). */
@@ -228,6 +249,56 @@ int nw_linsys_solution_download(nw_ctx* ctx, double* host);
 int nw_linsys_solution_device_ptr(nw_ctx* ctx, double** dptr);
 /* y = A x on device arrays of nrows doubles (FP64 residual checks). */
 int nw_linsys_spmv(nw_ctx* ctx, const double* d_x, double* d_y);
+
+/* ------------------------------------------------------------------ */
+/* Learned initial guess for the pressure solve                        */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference, whose pressure solve always starts
+ * from x0 = 0 (src/LinearSolver.C:209-254, Belos setProblem with a zero
+ * LHS).  A message-passing network on the graph of the context's 1-dof
+ * system predicts x = x0 + s_e * delta from the features of (A, b, x0),
+ * where x0 is the previous-step solution.  GMRES to the requested tolerance
+ * still decides the answer; a prediction whose residual is not below that
+ * of x0 is discarded (nw_solve_stats.guess reports which guess was used).
+ * A network belongs to one context and must be destroyed before it. */
+typedef struct nw_mpnn nw_mpnn;
+
+typedef struct {
+  int epochs;       /* passes over the recorded samples */
+  int batch;        /* samples per Adam step */
+  double lr;        /* Adam step size */
+  double beta1, beta2, eps; /* Adam moment decay and denominator guard */
+  uint64_t seed;    /* sample order shuffling */
+} nw_mpnn_train_cfg;
+
+/* Parameter count of a network with the given hidden width and layers. */
+int64_t nw_mpnn_nparams(int hidden, int layers);
+/* hidden in {32, 64}, layers in [1, 8]; seeded He-uniform initialisation. */
+int nw_mpnn_create(nw_ctx* ctx, int hidden, int layers, uint64_t seed,
+                   nw_mpnn** out);
+void nw_mpnn_destroy(nw_mpnn* net);
+/* Flat FP64 parameters: W_in[H][5], b_in[H], per layer W_self[H][H],
+ * W_nbr[H][H], b[H], then w_out[H], b_out. */
+int nw_mpnn_set_params(nw_mpnn* net, const double* params, int64_t n);
+int nw_mpnn_get_params(const nw_mpnn* net, double* params, int64_t n);
+/* Records one training pair from the context's last solve: the features
+ * of (A, b, x0) with x0 the guess that solve was given, and the target
+ * (x - x0) / s_e with x the current solution vector. */
+int nw_mpnn_record(nw_mpnn* net);
+int nw_mpnn_samples(const nw_mpnn* net);
+int nw_mpnn_clear_samples(nw_mpnn* net);
+/* Mean-squared loss of recorded sample s and its gradient (nparams
+ * doubles, may be NULL). */
+int nw_mpnn_loss_grad(nw_mpnn* net, int sample, double* loss, double* grad);
+/* Adam over the recorded samples; *last_loss = mean loss of the last
+ * epoch. */
+int nw_mpnn_train(nw_mpnn* net, const nw_mpnn_train_cfg* cfg,
+                  double* last_loss);
+/* Prediction from the current system with x0 = the solution vector:
+ * writes x0 + s_e * delta to x_host (nrows doubles) and s_e to *scale. */
+int nw_mpnn_predict(nw_mpnn* net, double* x_host, double* scale);
+/* Network used by solves with x0 = NW_X0_LEARNED (NULL detaches). */
+int nw_linsys_set_learned_guess(nw_ctx* ctx, nw_mpnn* net);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,7 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-I" + INC, "-I" + os.path.join(HERE, "csrc"),
+    "-I" + os.path.join(HERE, "host"),
     "-Wall", "-Wno-unused-result",
 ]
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-march=x86-64-v2",
@@ -60,6 +61,13 @@ def build(jobs=None, verbose=False):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] +
          objs + ["-lgomp"])
     os.replace(tmp, LIB)
+    # C++ adapter test driver (mirrors the reference unit tests)
+    test_src = os.path.join(ROOT, "tests", "cpp", "unit_edge_kernels.cpp")
+    if os.path.exists(test_src):
+        _run(["g++", "-O2", "-std=c++17", "-I" + INC,
+              "-I" + os.path.join(HERE, "host"), test_src, "-o",
+              os.path.join(OBJ, "unit_edge_kernels"), "-L" + os.path.dirname(LIB),
+              "-lnaluwind_amd", "-Wl,-rpath," + os.path.dirname(LIB)])
     return LIB
 
 

@@ -1,0 +1,25 @@
+// nw_amg.h -- smoothed-aggregation AMG preconditioner (device side).
+#pragma once
+
+#include "nw_amg_setup.h"
+#include "nw_internal.h"
+
+namespace nw {
+
+struct AmgDevice;
+
+// Builds the hierarchy from the context's current 1-dof matrix (host setup).
+int amg_setup(Ctx* c, AmgDevice** out);
+void amg_free(AmgDevice* a);
+int amg_levels(const AmgDevice* a);
+double amg_setup_seconds(const AmgDevice* a);
+int64_t amg_level_rows(const AmgDevice* a, int level);
+// Uploads the hierarchy in type T on first use and refreshes the fine-level
+// values from the current matrix.
+template <typename T>
+int amg_prepare(Ctx* c, AmgDevice* a);
+// x = M^-1 b (one V-cycle); x and b are fine-level vectors, x != b.
+template <typename T>
+int amg_apply(AmgDevice* a, const T* b, T* x, hipStream_t st);
+
+}  // namespace nw

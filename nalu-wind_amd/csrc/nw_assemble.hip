@@ -4,27 +4,34 @@
 // slots of n and of its edge neighbours, sorted by column (the sorted Tpetra
 // graph).  Every slot carries a code: the diagonal slot of row n is
 // -(n+1); the slot (n, m) reached through edge e = (L, R) is 2e + side with
-// side = 0 when n == L and 1 when n == R.
+// side = 0 when n == L and 1 when n == R.  Node and edge fields are stored
+// component-major (SoA): component d of node n is f[d * nnode + n].
 //
-// Kernel shape ("row owner, slot parallel").  A wavefront owns 64
-// consecutive rows and walks their slots 64 at a time, one slot per lane:
-//   * an off-diagonal slot (n, m) recomputes its edge's flux in registers
-//     from the edge's area vector and both nodes' fields, exactly as the
-//     reference lambda does for that edge, and stores the (n, m) block of
-//     the edge's LHS straight into its own slot (coalesced: slots are
-//     consecutive in memory);
-//   * the edge's contributions to the diagonal block of n and to rhs(n)
-//     are parked in LDS, and the last lane of each row sums the row's slots
-//     in column order and writes the diagonal block and the rhs once.
-// Each edge is therefore evaluated twice (once from each end), the matrix
-// and rhs are written exactly once, and no atomics are used: results are
-// bitwise reproducible run to run.  Rows may span chunks; the partial sums
-// are carried in registers (wave-uniform) to the next chunk.
+// Kernel shape ("row owner").  Each lane owns one row.  It loads its own
+// node's fields once, then walks the row's slots in column order: for each
+// off-diagonal slot it gathers the neighbour's fields and the edge's area
+// vector, evaluates the edge flux in registers exactly as the reference
+// lambda does for that edge, stores the (row, neighbour) block of the edge
+// LHS into the slot, and adds the edge's (row, row) block and rhs(row)
+// contributions to register accumulators in column order.  The diagonal
+// block and the rhs are written once at the end.  Every edge is therefore
+// evaluated twice (once from each end), each matrix entry is written once,
+// no atomics are used, and the result is bitwise reproducible.
 //
-// The physics of each slot restates the reference edge kernels:
+// Wavefronts take chunks of 64 consecutive rows from a schedule sorted by
+// the Morton code of the chunk centroids, and workgroups are renumbered so
+// that each XCD works on one contiguous stretch of that schedule: rows
+// processed together are neighbours in space and their shared node data
+// stays in the XCD's L2.
+//
+// The physics restates the reference edge kernels:
 //   ScalarEdgeSolverAlg.C:87-205, MomentumEdgeSolverAlg.C:107-312,
 //   ContinuityEdgeSolverAlg.C:111-194 (src/edge_kernels/).
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
+#include <vector>
 
 #include "nw_internal.h"
 
@@ -54,43 +61,58 @@ __device__ inline double van_leer(double dqm, double dqp, double eps)
 }
 
 // ------------------------------------------------------------------ physics
-// Each physics type computes, for slot (row n = node[side] of edge e):
+// Each physics type has a Node record of the fields one node contributes,
+// load(n, rec), and edge(e, side, L, R, off, acc) which computes, for the
+// row n = node[side] of edge e:
 //   off[NB]  the (n, other) block of the edge LHS, row-major ndof x ndof
 //   acc[NV]  the (n, n) block contribution (NB values) then rhs(n) (NDOF)
 
+struct ContNode {
+  double x[3], u[3], g[3], rho, p, ud;
+};
+
 struct ContinuityPhys {
   static constexpr int NDOF = 1;
+  using Node = ContNode;
   const double *coords, *vel, *dpdx, *rho, *p, *udiag, *areav;
+  int64_t N, E;
   double nocFac, tauScale, interpTogether, incompressible;
 
-  __device__ void operator()(int e, int side, int nL, int nR, double* off,
-                             double* acc) const
+  __device__ void load(int n, Node& r) const
+  {
+    for (int d = 0; d < 3; ++d) {
+      r.x[d] = coords[d * N + n];
+      r.u[d] = vel[d * N + n];
+      r.g[d] = dpdx[d * N + n];
+    }
+    r.rho = rho[n];
+    r.p = p[n];
+    r.ud = udiag[n];
+  }
+
+  __device__ void edge(int e, int side, const Node& L, const Node& R,
+                       double* off, double* acc) const
   {
     const double om_interpTogether = 1.0 - interpTogether;
     const double om_incomp = 1.0 - incompressible;
-    const double av[3] = {areav[3 * e], areav[3 * e + 1], areav[3 * e + 2]};
-    const double pressureL = p[nL], pressureR = p[nR];
-    const double densityL = rho[nL], densityR = rho[nR];
-    const double udiagL = udiag[nL], udiagR = udiag[nR];
-    const double projTimeScale = 0.5 * (1.0 / udiagL + 1.0 / udiagR);
-    const double rhoIp = 0.5 * (densityL + densityR);
+    const double av[3] = {areav[e], areav[E + e], areav[2 * E + e]};
+    const double projTimeScale = 0.5 * (1.0 / L.ud + 1.0 / R.ud);
+    const double rhoIp = 0.5 * (L.rho + R.rho);
     const double denScale = (1.0 / rhoIp) * incompressible + om_incomp;
     double dx[3];
     double axdx = 0.0, asq = 0.0;
     for (int d = 0; d < 3; ++d) {
-      dx[d] = coords[3 * nR + d] - coords[3 * nL + d];
+      dx[d] = R.x[d] - L.x[d];
       asq += av[d] * av[d];
       axdx += av[d] * dx[d];
     }
     const double inv_axdx = 1.0 / axdx;
-    double tmdot = -projTimeScale * (pressureR - pressureL) * asq * inv_axdx;
+    double tmdot = -projTimeScale * (R.p - L.p) * asq * inv_axdx;
     for (int d = 0; d < 3; ++d) {
       const double kxj = av[d] - asq * inv_axdx * dx[d];
-      const double uR = vel[3 * nR + d], uL = vel[3 * nL + d];
-      const double rhoUjIp = 0.5 * (densityR * uR + densityL * uL);
-      const double ujIp = 0.5 * (uR + uL);
-      const double GjIp =
-        0.5 * (dpdx[3 * nR + d] / udiagR + dpdx[3 * nL + d] / udiagL);
+      const double rhoUjIp = 0.5 * (R.rho * R.u[d] + L.rho * L.u[d]);
+      const double ujIp = 0.5 * (R.u[d] + L.u[d]);
+      const double GjIp = 0.5 * (R.g[d] / R.ud + L.g[d] / L.ud);
       tmdot += (interpTogether * rhoUjIp + om_interpTogether * rhoIp * ujIp +
                 GjIp) * av[d] -
                kxj * GjIp * nocFac;
@@ -104,41 +126,56 @@ struct ContinuityPhys {
   }
 };
 
+struct ScalNode {
+  double x[3], u[3], g[3], q, rho, dflux;
+};
+
 struct ScalarPhys {
   static constexpr int NDOF = 1;
+  using Node = ScalNode;
   const double *coords, *vrtm, *q, *dqdx, *rho, *dflux, *areav, *mdotf;
+  int64_t N, E;
   double alpha, alphaUpw, hoUpwind, relaxFac;
   int useLimiter;
   PecletDev pec;
 
-  __device__ void operator()(int e, int side, int nL, int nR, double* off,
-                             double* acc) const
+  __device__ void load(int n, Node& r) const
+  {
+    for (int d = 0; d < 3; ++d) {
+      r.x[d] = coords[d * N + n];
+      r.u[d] = vrtm[d * N + n];
+      r.g[d] = dqdx[d * N + n];
+    }
+    r.q = q[n];
+    r.rho = rho[n];
+    r.dflux = dflux[n];
+  }
+
+  __device__ void edge(int e, int side, const Node& L, const Node& R,
+                       double* off, double* acc) const
   {
     const double eps = 1.0e-16;
     const double om_alpha = 1.0 - alpha, om_alphaUpw = 1.0 - alphaUpw;
-    const double av[3] = {areav[3 * e], areav[3 * e + 1], areav[3 * e + 2]};
+    const double av[3] = {areav[e], areav[E + e], areav[2 * E + e]};
     const double mdot = mdotf[e];
-    const double densityL = rho[nL], densityR = rho[nR];
-    const double qNp1L = q[nL], qNp1R = q[nR];
-    const double viscosityL = dflux[nL], viscosityR = dflux[nR];
-    const double viscIp = 0.5 * (viscosityL + viscosityR);
-    const double diffIp = 0.5 * (viscosityL / densityL + viscosityR / densityR);
+    const double qNp1L = L.q, qNp1R = R.q;
+    const double viscIp = 0.5 * (L.dflux + R.dflux);
+    const double diffIp = 0.5 * (L.dflux / L.rho + R.dflux / R.rho);
     double dx[3];
     double axdx = 0.0, asq = 0.0, udotx = 0.0;
     for (int d = 0; d < 3; ++d) {
-      dx[d] = coords[3 * nR + d] - coords[3 * nL + d];
+      dx[d] = R.x[d] - L.x[d];
       asq += av[d] * av[d];
       axdx += av[d] * dx[d];
-      udotx += 0.5 * dx[d] * (vrtm[3 * nR + d] + vrtm[3 * nL + d]);
+      udotx += 0.5 * dx[d] * (R.u[d] + L.u[d]);
     }
     const double inv_axdx = 1.0 / axdx;
     double dqL = 0.0, dqR = 0.0, nonOrth = 0.0;
     for (int d = 0; d < 3; ++d) {
-      const double gL = dqdx[3 * nL + d], gR = dqdx[3 * nR + d];
-      dqL += 0.5 * dx[d] * gL;
-      dqR += 0.5 * dx[d] * gR;
+      dqL += 0.5 * dx[d] * L.g[d];
+      dqR += 0.5 * dx[d] * R.g[d];
       const double kxj = av[d] - asq * inv_axdx * dx[d];
-      nonOrth += -viscIp * kxj * 0.5 * (gR + gL);
+      nonOrth += -viscIp * kxj * 0.5 * (R.g[d] + L.g[d]);
     }
     const double pecnum = fabs(udotx) / (diffIp + eps);
     const double pecfac = pec(pecnum);
@@ -184,49 +221,59 @@ struct ScalarPhys {
   }
 };
 
+struct MomNode {
+  double x[3], u[3], g[9], visc, wall;
+};
+
 struct MomentumPhys {
   static constexpr int NDOF = 3;
+  using Node = MomNode;
   const double *coords, *vel, *dudx, *visc, *areav, *mdotf, *pecfacf, *wall;
+  int64_t N, E;
   double alpha, alphaUpw, hoUpwind, relaxFacU, includeDivU;
   int useLimiter;
 
-  __device__ void operator()(int e, int side, int nL, int nR, double* off,
-                             double* acc) const
+  __device__ void load(int n, Node& r) const
+  {
+    for (int d = 0; d < 3; ++d) {
+      r.x[d] = coords[d * N + n];
+      r.u[d] = vel[d * N + n];
+    }
+    for (int k = 0; k < 9; ++k) r.g[k] = dudx[k * N + n];
+    r.visc = visc[n];
+    r.wall = wall[n];
+  }
+
+  __device__ void edge(int e, int side, const Node& L, const Node& R,
+                       double* off, double* acc) const
   {
     const double eps = 1.0e-16;
     const double om_alpha = 1.0 - alpha, om_alphaUpw = 1.0 - alphaUpw;
-    const double av[3] = {areav[3 * e], areav[3 * e + 1], areav[3 * e + 2]};
+    const double av[3] = {areav[e], areav[E + e], areav[2 * E + e]};
     const double mdot = mdotf[e];
-    const double viscIp = 0.5 * (visc[nL] + visc[nR]);
-    double dx[3], uL[3], uR[3];
+    const double viscIp = 0.5 * (L.visc + R.visc);
+    double dx[3];
     double axdx = 0.0, asq = 0.0;
     for (int d = 0; d < 3; ++d) {
-      dx[d] = coords[3 * nR + d] - coords[3 * nL + d];
-      uL[d] = vel[3 * nL + d];
-      uR[d] = vel[3 * nR + d];
+      dx[d] = R.x[d] - L.x[d];
       asq += av[d] * av[d];
       axdx += av[d] * dx[d];
     }
     const double inv_axdx = 1.0 / axdx;
-    double gL[9], gR[9];
-    for (int k = 0; k < 9; ++k) {
-      gL[k] = dudx[9 * (int64_t)nL + k];
-      gR[k] = dudx[9 * (int64_t)nR + k];
-    }
     double duL[3], duR[3];
     for (int i = 0; i < 3; ++i) {
       duL[i] = 0.0;
       duR[i] = 0.0;
       for (int j = 0; j < 3; ++j) {
         const double dxj = 0.5 * dx[j];
-        duL[i] += dxj * gL[3 * i + j];
-        duR[i] += dxj * gR[3 * i + j];
+        duL[i] += dxj * L.g[3 * i + j];
+        duR[i] += dxj * R.g[3 * i + j];
       }
     }
     double limitL[3] = {1.0, 1.0, 1.0}, limitR[3] = {1.0, 1.0, 1.0};
     if (useLimiter) {
       for (int d = 0; d < 3; ++d) {
-        const double du = uR[d] - uL[d];
+        const double du = R.u[d] - L.u[d];
         limitL[d] = van_leer(4.0 * duL[d] - du, du, eps);
         limitR[d] = van_leer(4.0 * duR[d] - du, du, eps);
       }
@@ -235,22 +282,22 @@ struct MomentumPhys {
     const double om_pecfac = 1.0 - pecfac;
     double uIpL[3], uIpR[3];
     for (int d = 0; d < 3; ++d) {
-      uIpL[d] = uL[d] + duL[d] * hoUpwind * limitL[d] * 1.0;
-      uIpR[d] = uR[d] - duR[d] * hoUpwind * limitR[d] * 1.0;
+      uIpL[d] = L.u[d] + duL[d] * hoUpwind * limitL[d] * 1.0;
+      uIpR[d] = R.u[d] - duR[d] * hoUpwind * limitR[d] * 1.0;
     }
     double duidxj[3][3];
     for (int i = 0; i < 3; ++i) {
-      const double dui = uR[i] - uL[i];
+      const double dui = R.u[i] - L.u[i];
       double gjuidx = 0.0;
       for (int j = 0; j < 3; ++j)
-        gjuidx += 0.5 * (gR[3 * i + j] + gL[3 * i + j]) * dx[j];
+        gjuidx += 0.5 * (R.g[3 * i + j] + L.g[3 * i + j]) * dx[j];
       for (int j = 0; j < 3; ++j) {
-        const double gjui = 0.5 * (gR[3 * i + j] + gL[3 * i + j]);
+        const double gjui = 0.5 * (R.g[3 * i + j] + L.g[3 * i + j]);
         duidxj[i][j] = gjui + (dui - gjuidx) * av[j] * inv_axdx;
       }
     }
     const double dlhsfac = -viscIp * asq * inv_axdx;
-    const double mask = fmin(wall[nL], wall[nR]);
+    const double mask = fmin(L.wall, R.wall);
     const double alhsL = 0.5 * (mdot + fabs(mdot)) * pecfac * alphaUpw +
                          0.5 * alpha * om_pecfac * mdot;
     const double alhsR = 0.5 * (mdot - fabs(mdot)) * pecfac * alphaUpw +
@@ -261,7 +308,7 @@ struct MomentumPhys {
     double divU = 0.0;
     for (int j = 0; j < 3; ++j) divU += duidxj[j][j];
     for (int i = 0; i < 3; ++i) {
-      const double uiIp = 0.5 * (uR[i] + uL[i]);
+      const double uiIp = 0.5 * (R.u[i] + L.u[i]);
       const double uiUpw = (mdot > 0.0)
                              ? (alphaUpw * uIpL[i] + om_alphaUpw * uiIp)
                              : (alphaUpw * uIpR[i] + om_alphaUpw * uiIp);
@@ -274,7 +321,7 @@ struct MomentumPhys {
       for (int j = 0; j < 3; ++j)
         diff_flux += -viscIp * (duidxj[i][j] + duidxj[j][i]) * av[j];
       const double total_flux = adv_flux + diff_flux * mask;
-      // rows of this side: (rowL,rowL)/(rowL,rowR) or (rowR,rowR)/(rowR,rowL)
+      // this side's rows: (rowL,rowL)/(rowL,rowR) or (rowR,rowR)/(rowR,rowL)
       double dii, oii;
       if (side == 0) {
         dii = alhsL / r;
@@ -309,112 +356,75 @@ struct MomentumPhys {
 
 // ------------------------------------------------------------------ kernel
 
+// XCD-aware block order: the hardware deals workgroups round-robin to the
+// 8 XCDs; renumbering them as below hands each XCD one contiguous stretch
+// of the (locality-ordered) chunk list, so neighbouring rows share an L2.
+__device__ inline int xcd_block(int nblocks_padded)
+{
+  const int per = nblocks_padded / 8;
+  return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+}
+
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_assemble_rows(
   P phys, const int32_t* __restrict__ row_ptr,
   const int32_t* __restrict__ code, const int2* __restrict__ enodes,
-  double* __restrict__ vals, double* __restrict__ rhs, int nrows,
-  int accumulate)
+  const int2* __restrict__ sched, int nchunks, double* __restrict__ vals,
+  double* __restrict__ rhs, int accumulate)
 {
   constexpr int ND = P::NDOF, NB = ND * ND, NV = NB + ND;
-  __shared__ double sh[kBlock / kWave][NV][kWave];
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = threadIdx.x / kWave;
-  const int gw = blockIdx.x * (kBlock / kWave) + wv;
-  const int r0 = gw * kRowsPerWave;
-  if (r0 >= nrows) return;
-  const int r1 = min(r0 + kRowsPerWave, nrows);
-  const int s0 = row_ptr[r0], s1 = row_ptr[r1];
-
-  double carry[NV];
-  for (int v = 0; v < NV; ++v) carry[v] = 0.0;
-  int crow = -1, cdslot = -1;
-
-  for (int base = s0; base < s1; base += kWave) {
-    const int s = base + lane;
-    const bool act = s < s1;
-    int row = INT_MAX;
-    bool isdiag = false;
-    double acc[NV];
-    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-    if (act) {
-      const int c = code[s];
-      if (c < 0) {
-        row = -c - 1;
-        isdiag = true;
-      } else {
-        const int e = c >> 1, side = c & 1;
-        const int2 lr = enodes[e];
-        row = side ? lr.y : lr.x;
-        double off[NB];
-        phys(e, side, lr.x, lr.y, off, acc);
-        double* vp = vals + (int64_t)s * NB;
-        if (accumulate) {
-          for (int k = 0; k < NB; ++k) vp[k] += off[k];
-        } else {
-          for (int k = 0; k < NB; ++k) vp[k] = off[k];
-        }
-      }
+  const int gw = xcd_block((int)gridDim.x) * (kBlock / kWave) +
+                 (int)(threadIdx.x / kWave);
+  if (gw >= nchunks) return;
+  const int2 chunk = sched[gw];
+  const int row = chunk.x + lane;
+  if (row >= chunk.y) return;
+  typename P::Node me, ot;
+  phys.load(row, me);
+  double acc[NV];
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  int dslot = -1;
+  const int s0 = row_ptr[row], s1 = row_ptr[row + 1];
+  for (int s = s0; s < s1; ++s) {
+    const int c = code[s];
+    if (c < 0) {
+      dslot = s;
+      continue;
     }
-    for (int v = 0; v < NV; ++v) sh[wv][v][lane] = acc[v];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-    const int prow = __shfl_up(row, 1);
-    const int nrow = __shfl_down(row, 1);
-    const bool head = act && (lane == 0 || prow != row);
-    const bool tail = act && (lane == kWave - 1 || nrow != row);
-    const uint64_t heads = __ballot(head);
-    const uint64_t diags = __ballot(isdiag);
-
-    double sum[NV];
-    for (int v = 0; v < NV; ++v) sum[v] = 0.0;
-    int dslot = -1;
-    bool open = false;  // row continues into the next chunk
-    if (tail) {
-      const uint64_t upto =
-        (lane == kWave - 1) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-      const uint64_t hm = heads & upto;
-      const int start = 63 - __builtin_clzll(hm);
-      const bool cont = (start == 0) && (row == crow);
-      if (cont)
-        for (int v = 0; v < NV; ++v) sum[v] = carry[v];
-      for (int t = start; t <= lane; ++t)
-        for (int v = 0; v < NV; ++v) sum[v] += sh[wv][v][t];
-      const uint64_t dm = diags & upto & ~((1ull << start) - 1ull);
-      dslot = dm ? base + __builtin_ctzll(dm) : (cont ? cdslot : -1);
-      open = (lane == kWave - 1) && (s + 1 < s1) && (row_ptr[row + 1] > s + 1);
-      if (!open && dslot >= 0) {
-        double* dp = vals + (int64_t)dslot * NB;
-        double* rp = rhs + (int64_t)row * ND;
-        if (accumulate) {
-          for (int k = 0; k < NB; ++k) dp[k] += sum[k];
-          for (int d = 0; d < ND; ++d) rp[d] += sum[NB + d];
-        } else {
-          for (int k = 0; k < NB; ++k) dp[k] = sum[k];
-          for (int d = 0; d < ND; ++d) rp[d] = sum[NB + d];
-        }
-      }
-    }
-    // carry the open row of lane 63 (wave-uniform)
-    const int open63 = __shfl((int)open, kWave - 1);
-    if (open63) {
-      crow = __shfl(row, kWave - 1);
-      cdslot = __shfl(dslot, kWave - 1);
-      for (int v = 0; v < NV; ++v) carry[v] = __shfl(sum[v], kWave - 1);
+    const int e = c >> 1, side = c & 1;
+    const int2 lr = enodes[e];
+    phys.load(side ? lr.x : lr.y, ot);
+    double off[NB], con[NV];
+    if (side)
+      phys.edge(e, 1, ot, me, off, con);
+    else
+      phys.edge(e, 0, me, ot, off, con);
+    double* vp = vals + (int64_t)s * NB;
+    if (accumulate) {
+      for (int k = 0; k < NB; ++k) vp[k] += off[k];
     } else {
-      crow = -1;
+      for (int k = 0; k < NB; ++k) vp[k] = off[k];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    for (int v = 0; v < NV; ++v) acc[v] += con[v];
+  }
+  if (dslot >= 0) {
+    double* dp = vals + (int64_t)dslot * NB;
+    double* rp = rhs + (int64_t)row * ND;
+    if (accumulate) {
+      for (int k = 0; k < NB; ++k) dp[k] += acc[k];
+      for (int d = 0; d < ND; ++d) rp[d] += acc[NB + d];
+    } else {
+      for (int k = 0; k < NB; ++k) dp[k] = acc[k];
+      for (int d = 0; d < ND; ++d) rp[d] = acc[NB + d];
+    }
   }
 }
 
 // MomentumEdgePecletAlg::execute (src/edge_kernels/MomentumEdgePecletAlg.C:
 // 74-101): one edge per lane, edge-ordered (coalesced) writes.
 __global__ __launch_bounds__(kBlock) void k_momentum_peclet(
-  int64_t nedge, const int2* __restrict__ enodes,
+  int64_t nedge, int64_t N, const int2* __restrict__ enodes,
   const double* __restrict__ coords, const double* __restrict__ vel,
   const double* __restrict__ rho, const double* __restrict__ visc,
   PecletDev pec, double* __restrict__ pecnum, double* __restrict__ pecfac)
@@ -427,24 +437,92 @@ __global__ __launch_bounds__(kBlock) void k_momentum_peclet(
   const double diffIp = 0.5 * (visc[nL] / rho[nL] + visc[nR] / rho[nR]);
   double udotx = 0.0;
   for (int d = 0; d < 3; ++d) {
-    const double dxj = coords[3 * nR + d] - coords[3 * nL + d];
-    udotx += 0.5 * dxj * (vel[3 * nR + d] + vel[3 * nL + d]);
+    const double dxj = coords[d * N + nR] - coords[d * N + nL];
+    udotx += 0.5 * dxj * (vel[d * N + nR] + vel[d * N + nL]);
   }
   const double pn = fabs(udotx) / (diffIp + eps);
   pecnum[e] = pn;
   pecfac[e] = pec(pn);
 }
 
+uint64_t spread3(uint64_t v)
+{
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+// Chunks of <= kRowsPerWave consecutive rows, ordered by the Morton code of
+// their coordinate centroid (NW_ROW_ORDER=natural keeps the row order).
+int build_schedule(Ctx* c)
+{
+  Graph& g = c->g;
+  const int64_t N = g.nnode;
+  const int64_t nch = (N + kRowsPerWave - 1) / kRowsPerWave;
+  std::vector<int2> ch(nch);
+  for (int64_t i = 0; i < nch; ++i)
+    ch[i] = make_int2((int)(i * kRowsPerWave),
+                      (int)std::min<int64_t>(N, (i + 1) * kRowsPerWave));
+  const char* env = getenv("NW_ROW_ORDER");
+  const bool natural = env && std::string(env) == "natural";
+  if (!natural && c->field[NW_F_COORDINATES]) {
+    std::vector<double> x(3 * N);  // SoA: x[d * N + n]
+    NW_HIP(hipMemcpyAsync(x.data(), c->field[NW_F_COORDINATES],
+                          sizeof(double) * 3 * N, hipMemcpyDeviceToHost,
+                          c->stream));
+    NW_HIP(hipStreamSynchronize(c->stream));
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      const auto mm = std::minmax_element(x.begin() + d * N, x.begin() + (d + 1) * N);
+      lo[d] = *mm.first;
+      hi[d] = *mm.second;
+    }
+    std::vector<std::pair<uint64_t, int64_t>> key(nch);
+    for (int64_t i = 0; i < nch; ++i) {
+      uint64_t k = 0;
+      for (int d = 0; d < 3; ++d) {
+        double m = 0.0;
+        for (int r = ch[i].x; r < ch[i].y; ++r) m += x[d * N + r];
+        const double span = hi[d] > lo[d] ? hi[d] - lo[d] : 1.0;
+        const double t = (m / (ch[i].y - ch[i].x) - lo[d]) / span;
+        const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
+        k |= spread3(q) << d;
+      }
+      key[i] = {k, i};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int2> o(nch);
+    for (int64_t i = 0; i < nch; ++i) o[i] = ch[key[i].second];
+    ch.swap(o);
+  }
+  if (!g.sched) NW_HIP(hipMalloc(&g.sched, sizeof(int2) * std::max<int64_t>(nch, 1)));
+  NW_HIP(hipMemcpyAsync(g.sched, ch.data(), sizeof(int2) * nch,
+                        hipMemcpyHostToDevice, c->stream));
+  NW_HIP(hipStreamSynchronize(c->stream));
+  g.nchunks = nch;
+  g.sched_valid = true;
+  return NW_OK;
+}
+
 template <class P>
 int launch_rows(Ctx* c, const P& phys, int accumulate)
 {
   Graph& g = c->g;
-  const int64_t nwaves = (g.nnode + kRowsPerWave - 1) / kRowsPerWave;
-  const unsigned blocks = (unsigned)((nwaves + 3) / 4);
-  if (blocks == 0) return NW_OK;
-  k_assemble_rows<P><<<blocks, kBlock, 0, c->stream>>>(
-    phys, g.row_ptr, g.code, reinterpret_cast<const int2*>(c->edges), g.vals,
-    g.rhs, (int)g.nnode, accumulate);
+  if (!g.sched_valid) {
+    int rc = build_schedule(c);
+    if (rc) return rc;
+  }
+  if (g.nchunks == 0) return NW_OK;
+  const int64_t wpb = kBlock / kWave;
+  int64_t blocks = (g.nchunks + wpb - 1) / wpb;
+  blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
+  k_assemble_rows<P><<<(unsigned)blocks, kBlock, 0, c->stream>>>(
+    phys, g.row_ptr, g.code, reinterpret_cast<const int2*>(c->edges), g.sched,
+    (int)g.nchunks, g.vals, g.rhs, accumulate);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }
@@ -476,8 +554,9 @@ int nw_assemble_continuity(nw_ctx* c, const nw_continuity_opts* o)
   NW_GET(NW_F_PRESSURE, p);
   NW_GET(NW_F_UDIAG, udiag);
   NW_GET(NW_E_AREA_VECTOR, areav);
-  ContinuityPhys P{coords, vel, dpdx, rho, p, udiag, areav, o->noc,
-                   o->dt / o->gamma1, o->interp_together, o->incompressible};
+  ContinuityPhys P{coords, vel, dpdx, rho, p, udiag, areav, c->nnode, c->nedge,
+                   o->noc, o->dt / o->gamma1, o->interp_together,
+                   o->incompressible};
   return launch_rows(c, P, o->accumulate);
 }
 
@@ -493,7 +572,7 @@ int nw_assemble_scalar(nw_ctx* c, const nw_scalar_opts* o)
   NW_GET(NW_F_DIFF_FLUX_COEFF, dflux);
   NW_GET(NW_E_AREA_VECTOR, areav);
   NW_GET(NW_E_MASS_FLOW_RATE, mdot);
-  ScalarPhys P{coords, vel, q, dqdx, rho, dflux, areav, mdot,
+  ScalarPhys P{coords, vel, q, dqdx, rho, dflux, areav, mdot, c->nnode, c->nedge,
                o->alpha, o->alpha_upw, o->upw, o->relax, o->use_limiter,
                PecletDev{o->peclet.form, o->peclet.hybrid, o->peclet.c1,
                          o->peclet.c2}};
@@ -513,8 +592,8 @@ int nw_assemble_momentum(nw_ctx* c, const nw_momentum_opts* o)
   NW_GET(NW_E_MASS_FLOW_RATE, mdot);
   NW_GET(NW_E_PECLET_FACTOR, pecfac);
   MomentumPhys P{coords, vel, dudx, visc, areav, mdot, pecfac, wall,
-                 o->alpha, o->alpha_upw, o->upw, o->relax, o->include_divU,
-                 o->use_limiter};
+                 c->nnode, c->nedge, o->alpha, o->alpha_upw, o->upw, o->relax,
+                 o->include_divU, o->use_limiter};
   return launch_rows(c, P, o->accumulate);
 }
 
@@ -533,8 +612,8 @@ int nw_momentum_edge_peclet(nw_ctx* c, const nw_peclet* pf)
   if (rc) return rc;
   if (c->nedge == 0) return NW_OK;
   k_momentum_peclet<<<grid_for(c->nedge, kBlock), kBlock, 0, c->stream>>>(
-    c->nedge, reinterpret_cast<const int2*>(c->edges), coords, vel, rho, visc,
-    PecletDev{pf->form, pf->hybrid, pf->c1, pf->c2}, pn, pfac);
+    c->nedge, c->nnode, reinterpret_cast<const int2*>(c->edges), coords, vel,
+    rho, visc, PecletDev{pf->form, pf->hybrid, pf->c1, pf->c2}, pn, pfac);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }

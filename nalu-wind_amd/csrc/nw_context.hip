@@ -81,10 +81,32 @@ static void free_graph(Graph& g)
   (void)hipFree(g.vals);
   (void)hipFree(g.rhs);
   (void)hipFree(g.sln);
+  (void)hipFree(g.sched);
   g = Graph();
 }
 
 // ------------------------------------------------------------- kernels
+
+// [n][nc] (components of a node adjacent) <-> [nc][n] (component-major)
+__global__ void k_aos_to_soa(const double* __restrict__ a, double* __restrict__ s,
+                             int64_t n, int nc)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nc) return;
+  const int64_t i = t / nc;
+  const int d = (int)(t % nc);
+  s[d * n + i] = a[t];
+}
+
+__global__ void k_soa_to_aos(const double* __restrict__ s, double* __restrict__ a,
+                             int64_t n, int nc)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nc) return;
+  const int64_t i = t / nc;
+  const int d = (int)(t % nc);
+  a[t] = s[d * n + i];
+}
 
 __global__ void k_fill(double* __restrict__ p, double v, int64_t n)
 {
@@ -102,7 +124,7 @@ __global__ void k_reset_rows(const int32_t* __restrict__ nodes, int64_t nn,
                              double* __restrict__ vals,
                              double* __restrict__ rhs, double diag_value,
                              double rhs_value, const double* __restrict__ bc,
-                             const double* __restrict__ sol, int ncomp)
+                             const double* __restrict__ sol, int64_t nnode)
 {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nd = end - begin;
@@ -116,7 +138,7 @@ __global__ void k_reset_rows(const int32_t* __restrict__ nodes, int64_t nn,
       vals[(int64_t)s * nb + d * ndof + c] = (self && c == d) ? diag_value : 0.0;
   }
   rhs[(int64_t)n * ndof + d] =
-    bc ? (bc[(int64_t)n * ncomp + d] - sol[(int64_t)n * ncomp + d]) : rhs_value;
+    bc ? (bc[d * nnode + n] - sol[d * nnode + n]) : rhs_value;
 }
 
 // Segments of sorted (target, value) contributions, each summed in input
@@ -175,6 +197,8 @@ int nw_ctx_destroy(nw_ctx* c)
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c->kry);
+  amg_free(c->amg);
+  (void)hipFree(c->x_init);
   free_graph(c->g);
   for (auto& f : c->field) (void)hipFree(f);
   (void)hipFree(c->edges);
@@ -214,6 +238,10 @@ int nw_mesh_set(nw_ctx* c, int64_t nnode, int64_t nedge,
   NW_HIP(hipStreamSynchronize(c->stream));
   free_krylov(c->kry);
   c->kry = nullptr;
+  amg_free(c->amg);
+  c->amg = nullptr;
+  (void)hipFree(c->x_init);
+  c->x_init = nullptr;
   free_graph(c->g);
   for (auto& f : c->field) {
     (void)hipFree(f);
@@ -234,6 +262,7 @@ int nw_mesh_set(nw_ctx* c, int64_t nnode, int64_t nedge,
 int nw_field_device_ptr(nw_ctx* c, int field, double** dptr)
 {
   NW_REQUIRE(c && dptr, NW_E_ARG, "null argument");
+  if (field == NW_F_COORDINATES) c->g.sched_valid = false;
   return ensure_field(c, field, dptr);
 }
 
@@ -246,9 +275,23 @@ int nw_field_upload(nw_ctx* c, int field, const double* host, int64_t count)
   NW_REQUIRE(count == field_count(c, field), NW_E_ARG,
              "field %d expects %lld doubles, got %lld", field,
              (long long)field_count(c, field), (long long)count);
-  NW_HIP(hipMemcpyAsync(d, host, sizeof(double) * count, hipMemcpyHostToDevice,
-                        c->stream));
+  const int nc = field_components(field);
+  if (nc == 1) {
+    NW_HIP(hipMemcpyAsync(d, host, sizeof(double) * count,
+                          hipMemcpyHostToDevice, c->stream));
+  } else {
+    double* tmp = nullptr;
+    NW_HIP(hipMalloc(&tmp, sizeof(double) * count));
+    NW_HIP(hipMemcpyAsync(tmp, host, sizeof(double) * count,
+                          hipMemcpyHostToDevice, c->stream));
+    k_aos_to_soa<<<grid_for(count, 256), 256, 0, c->stream>>>(tmp, d, count / nc, nc);
+    hipError_t le = hipGetLastError();
+    NW_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(tmp);
+    NW_REQUIRE(le == hipSuccess, NW_E_HIP, "transpose: %s", hipGetErrorString(le));
+  }
   NW_HIP(hipStreamSynchronize(c->stream));
+  if (field == NW_F_COORDINATES) c->g.sched_valid = false;
   return NW_OK;
 }
 
@@ -261,9 +304,22 @@ int nw_field_download(nw_ctx* c, int field, double* host, int64_t count)
   NW_REQUIRE(count == field_count(c, field), NW_E_ARG,
              "field %d holds %lld doubles, asked %lld", field,
              (long long)field_count(c, field), (long long)count);
-  NW_HIP(hipMemcpyAsync(host, d, sizeof(double) * count, hipMemcpyDeviceToHost,
+  const int nc = field_components(field);
+  if (nc == 1) {
+    NW_HIP(hipMemcpyAsync(host, d, sizeof(double) * count,
+                          hipMemcpyDeviceToHost, c->stream));
+    NW_HIP(hipStreamSynchronize(c->stream));
+    return NW_OK;
+  }
+  double* tmp = nullptr;
+  NW_HIP(hipMalloc(&tmp, sizeof(double) * count));
+  k_soa_to_aos<<<grid_for(count, 256), 256, 0, c->stream>>>(d, tmp, count / nc, nc);
+  hipError_t le = hipGetLastError();
+  NW_HIP(hipMemcpyAsync(host, tmp, sizeof(double) * count, hipMemcpyDeviceToHost,
                         c->stream));
   NW_HIP(hipStreamSynchronize(c->stream));
+  (void)hipFree(tmp);
+  NW_REQUIRE(le == hipSuccess, NW_E_HIP, "transpose: %s", hipGetErrorString(le));
   return NW_OK;
 }
 
@@ -289,6 +345,10 @@ int nw_linsys_build_edge_graph(nw_ctx* c, int ndof)
   NW_HIP(hipSetDevice(c->device));
   free_krylov(c->kry);
   c->kry = nullptr;
+  amg_free(c->amg);
+  c->amg = nullptr;
+  (void)hipFree(c->x_init);
+  c->x_init = nullptr;
   free_graph(c->g);
   Graph& g = c->g;
   const int64_t N = c->nnode, E = c->nedge;
@@ -508,7 +568,7 @@ int nw_linsys_sum_into(nw_ctx* c, int nblocks, int nent, const int32_t* nodes,
 static int reset_rows_impl(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
                            int begin, int end, double diag_value,
                            double rhs_value, const double* bc,
-                           const double* sol, int ncomp)
+                           const double* sol)
 {
   NW_REQUIRE(c && c->g.ndof, NW_E_STATE, "linear system not built");
   NW_REQUIRE(nnodes >= 0 && (nodes || nnodes == 0), NW_E_ARG, "bad node list");
@@ -525,7 +585,7 @@ static int reset_rows_impl(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
   const int64_t nt = nnodes * (end - begin);
   k_reset_rows<<<grid_for(nt, 256), 256, 0, c->stream>>>(
     d_nodes, nnodes, begin, end, c->g.ndof, c->g.row_ptr, c->g.cols, c->g.vals,
-    c->g.rhs, diag_value, rhs_value, bc, sol, ncomp);
+    c->g.rhs, diag_value, rhs_value, bc, sol, c->g.nnode);
   hipError_t le = hipGetLastError();
   NW_HIP(hipStreamSynchronize(c->stream));
   (void)hipFree(d_nodes);
@@ -539,7 +599,7 @@ int nw_linsys_reset_rows(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
                          double rhs_residual)
 {
   return reset_rows_impl(c, nnodes, nodes, begin, end, diag_value,
-                         rhs_residual, nullptr, nullptr, 0);
+                         rhs_residual, nullptr, nullptr);
 }
 
 int nw_linsys_apply_dirichlet(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
@@ -555,8 +615,7 @@ int nw_linsys_apply_dirichlet(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
   if (rc) return rc;
   rc = require_field(c, bc_field, &bc);
   if (rc) return rc;
-  return reset_rows_impl(c, nnodes, nodes, begin, end, 1.0, 0.0, bc, sol,
-                         field_components(sol_field));
+  return reset_rows_impl(c, nnodes, nodes, begin, end, 1.0, 0.0, bc, sol);
 }
 
 int nw_fix_pressure_at_node(nw_ctx* c, int32_t node, double pref)

@@ -64,9 +64,17 @@ struct Graph {
   double* rhs = nullptr;      // [nnode][ndof]
   double* sln = nullptr;      // [nnode][ndof]
   std::vector<int32_t> h_row_ptr, h_cols;
+  // Row chunks (<= kRowsPerWave consecutive rows each) in the order the
+  // assembly visits them: a Morton order of the chunk centroids, so that
+  // chunks processed together are neighbours in space.
+  int2* sched = nullptr;
+  int64_t nchunks = 0;
+  bool sched_valid = false;
 };
 
-struct Krylov;  // solver workspace (nw_krylov.hip)
+struct Krylov;     // solver workspace (nw_krylov.hip)
+struct AmgDevice;  // AMG hierarchy (nw_amg.hip)
+struct Mpnn;       // learned initial guess (nw_mpnn.hip)
 
 struct Ctx {
   int device = 0;
@@ -78,6 +86,9 @@ struct Ctx {
   double* field[NW_NUM_FIELDS] = {};
   Graph g;
   Krylov* kry = nullptr;
+  AmgDevice* amg = nullptr;
+  double* x_init = nullptr;  // [nnode] guess the last solve started from
+  Mpnn* mpnn = nullptr;      // network for NW_X0_LEARNED (not owned)
 };
 
 int field_components(int f);
@@ -87,6 +98,11 @@ int64_t field_count(const Ctx* c, int f);  // doubles in the field
 int ensure_field(Ctx* c, int f, double** out);
 int require_field(Ctx* c, int f, const double** out);
 void free_krylov(Krylov* k);
+void amg_free(AmgDevice* a);
+// x = x0 + s_e * delta predicted from the context's system and x0 (device
+// vectors of nnode doubles); wall time in mpnn_last_ms.
+int mpnn_predict_device(Mpnn* m, const double* x0, double* x);
+double mpnn_last_ms(const Mpnn* m);
 
 inline unsigned grid_for(int64_t n, int block) {
   return (unsigned)((n + block - 1) / block);

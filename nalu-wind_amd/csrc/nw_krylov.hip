@@ -34,6 +34,8 @@
 #include <vector>
 
 #include "nw_internal.h"
+#include "nw_sparse.h"
+#include "nw_amg.h"
 
 namespace nw {
 
@@ -84,60 +86,6 @@ __global__ __launch_bounds__(kBlock) void k_reduce(
   __shared__ double tot[1];
   block_reduce_store<1>(a, tot);
   if (threadIdx.x == 0) out[v] = tot[0];
-}
-
-// y = alpha * (A x); G lanes per row.
-template <typename T, int G>
-__global__ __launch_bounds__(kBlock) void k_spmv(
-  const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-  const T* __restrict__ va, const T* __restrict__ x, T* __restrict__ y, int n,
-  T alpha)
-{
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int row = (int)(t / G);
-  const int g = threadIdx.x & (G - 1);
-  T s = 0;
-  if (row < n) {
-    for (int k = rp[row] + g; k < rp[row + 1]; k += G) s += va[k] * x[ci[k]];
-  }
-  for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
-  if (row < n && g == 0) y[row] = alpha * s;
-}
-
-// Chebyshev step: r = v - A z; d = c1 d + c2 dinv r; zo = z + d.
-template <typename T, int G>
-__global__ __launch_bounds__(kBlock) void k_cheb_step(
-  const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-  const T* __restrict__ va, const T* __restrict__ z, const T* __restrict__ v,
-  const T* __restrict__ dinv, T* __restrict__ d, T* __restrict__ zo, int n,
-  T c1, T c2)
-{
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int row = (int)(t / G);
-  const int g = threadIdx.x & (G - 1);
-  T s = 0;
-  if (row < n) {
-    for (int k = rp[row] + g; k < rp[row + 1]; k += G) s += va[k] * z[ci[k]];
-  }
-  for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
-  if (row < n && g == 0) {
-    const T dn = c1 * d[row] + c2 * dinv[row] * (v[row] - s);
-    d[row] = dn;
-    zo[row] = z[row] + dn;
-  }
-}
-
-// d = dinv * v * inv_theta; z = d
-template <typename T>
-__global__ void k_cheb_init(const T* __restrict__ v, const T* __restrict__ dinv,
-                            T* __restrict__ d, T* __restrict__ z, int n,
-                            T inv_theta)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const T di = dinv[i] * v[i] * inv_theta;
-  d[i] = di;
-  z[i] = di;
 }
 
 // Fused Arnoldi dots.  Block b = rowblock * ntiles + tile (tiles of one row
@@ -328,10 +276,6 @@ __global__ void k_vmul(double* __restrict__ y, const double* __restrict__ d,
   if (i < n) y[i] *= d[i];
 }
 
-int lanes_per_row(double avg)
-{
-  return avg <= 4.5 ? 4 : (avg <= 9.0 ? 8 : 16);
-}
 
 unsigned stream_grid(int64_t n)
 {
@@ -359,6 +303,7 @@ struct Krylov {
   double* red = nullptr;   // reduced results (device)
   double* coef = nullptr;  // coefficients (device)
   double* pe = nullptr;    // power iteration work
+  double* xg = nullptr;    // learned guess
   int* bad = nullptr;
   double* h_pinned = nullptr;  // host pinned scratch
   Space<double> d;
@@ -391,6 +336,7 @@ void free_krylov(Krylov* k)
   (void)hipFree(k->red);
   (void)hipFree(k->coef);
   (void)hipFree(k->pe);
+  (void)hipFree(k->xg);
   (void)hipFree(k->bad);
   (void)hipHostFree(k->h_pinned);
   free_space(k->d);
@@ -441,17 +387,7 @@ struct Solve {
   int spmv(const T* va, const T* x, T* y, T alpha)
   {
     const int n = (int)k->n;
-    const unsigned blocks = grid_for((int64_t)n * G, kBlock);
-    switch (G) {
-    case 4:
-      k_spmv<T, 4><<<blocks, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, va, x, y, n, alpha);
-      break;
-    case 8:
-      k_spmv<T, 8><<<blocks, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, va, x, y, n, alpha);
-      break;
-    default:
-      k_spmv<T, 16><<<blocks, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, va, x, y, n, alpha);
-    }
+    NW_ROW_LAUNCH(k_spmv, T, G, n, st, c->g.row_ptr, c->g.cols, va, x, y, n, alpha);
     NW_LAUNCH_CHECK();
     return NW_OK;
   }
@@ -461,17 +397,8 @@ struct Solve {
                 T c1, T c2)
   {
     const int n = (int)k->n;
-    const unsigned blocks = grid_for((int64_t)n * G, kBlock);
-    switch (G) {
-    case 4:
-      k_cheb_step<T, 4><<<blocks, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, va, z, v, dinv, d, zo, n, c1, c2);
-      break;
-    case 8:
-      k_cheb_step<T, 8><<<blocks, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, va, z, v, dinv, d, zo, n, c1, c2);
-      break;
-    default:
-      k_cheb_step<T, 16><<<blocks, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, va, z, v, dinv, d, zo, n, c1, c2);
-    }
+    NW_ROW_LAUNCH(k_cheb_step, T, G, n, st, c->g.row_ptr, c->g.cols, va, z, v,
+                  dinv, d, zo, n, c1, c2);
     NW_LAUNCH_CHECK();
     return NW_OK;
   }
@@ -506,6 +433,11 @@ struct Solve {
   template <typename T>
   int apply_op(Space<T>& s, const T* v, T* W)
   {
+    if (cfg->precond == NW_PREC_AMG) {
+      int rc = amg_apply<T>(c->amg, v, s.cz, st);
+      if (rc) return rc;
+      return spmv<T>(s.mat, s.cz, W, (T)1);
+    }
     if (cfg->precond == NW_PREC_CHEBYSHEV) {
       T* z = nullptr;
       int rc = cheb_apply<T>(s, v, &z);
@@ -578,8 +510,11 @@ struct Solve {
     case 8:
       k_residual<8><<<nb, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, c->g.vals, x, c->g.rhs, k->r, n, k->partial);
       break;
-    default:
+    case 16:
       k_residual<16><<<nb, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, c->g.vals, x, c->g.rhs, k->r, n, k->partial);
+      break;
+    default:
+      k_residual<32><<<nb, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, c->g.vals, x, c->g.rhs, k->r, n, k->partial);
     }
     NW_LAUNCH_CHECK();
     double r2 = 0;
@@ -711,6 +646,10 @@ struct Solve {
       rc = cheb_apply<T>(s, s.u, &z);
       if (rc) return rc;
       u = z;
+    } else if (cfg->precond == NW_PREC_AMG) {
+      rc = amg_apply<T>(c->amg, s.u, s.cz, st);
+      if (rc) return rc;
+      u = s.cz;
     }
     k_axpy_prec<T><<<grid_for(n, kBlock), kBlock, 0, st>>>(
       x, u, cfg->precond == NW_PREC_JACOBI ? k->dinv : nullptr, n);
@@ -759,13 +698,8 @@ int nw_linsys_spmv(nw_ctx* c, const double* d_x, double* d_y)
              "spmv needs a built 1-dof system");
   const int n = (int)c->g.nnode;
   const int G = lanes_per_row((double)c->g.nnzb / std::max<int64_t>(n, 1));
-  const unsigned blocks = grid_for((int64_t)n * G, kBlock);
-  if (G == 4)
-    k_spmv<double, 4><<<blocks, kBlock, 0, c->stream>>>(c->g.row_ptr, c->g.cols, c->g.vals, d_x, d_y, n, 1.0);
-  else if (G == 8)
-    k_spmv<double, 8><<<blocks, kBlock, 0, c->stream>>>(c->g.row_ptr, c->g.cols, c->g.vals, d_x, d_y, n, 1.0);
-  else
-    k_spmv<double, 16><<<blocks, kBlock, 0, c->stream>>>(c->g.row_ptr, c->g.cols, c->g.vals, d_x, d_y, n, 1.0);
+  NW_ROW_LAUNCH(k_spmv, double, G, n, c->stream, c->g.row_ptr, c->g.cols,
+                c->g.vals, d_x, d_y, n, 1.0);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }
@@ -777,19 +711,23 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
   NW_REQUIRE(cfg->kspace >= 1 && cfg->kspace < kMaxBasis, NW_E_ARG,
              "kspace must be in [1, %d)", kMaxBasis);
   NW_REQUIRE(cfg->tol > 0 && cfg->max_iters >= 0, NW_E_ARG, "bad tolerance");
-  NW_REQUIRE(cfg->precond >= NW_PREC_NONE && cfg->precond <= NW_PREC_CHEBYSHEV,
+  NW_REQUIRE(cfg->precond >= NW_PREC_NONE && cfg->precond <= NW_PREC_AMG,
              NW_E_ARG, "bad preconditioner");
   NW_REQUIRE(cfg->precond != NW_PREC_CHEBYSHEV ||
                (cfg->cheb_degree >= 1 && cfg->cheb_ratio > 1.0),
              NW_E_ARG, "bad Chebyshev parameters");
   NW_REQUIRE(cfg->mode == NW_SOLVE_FP64 || cfg->mode == NW_SOLVE_FP32_REFINE,
              NW_E_ARG, "bad solve mode");
-  NW_REQUIRE(cfg->x0 == NW_X0_ZERO || cfg->x0 == NW_X0_SOLUTION, NW_E_ARG,
-             "x0 mode %d not available in this call", cfg->x0);
+  NW_REQUIRE(cfg->x0 == NW_X0_ZERO || cfg->x0 == NW_X0_SOLUTION ||
+               cfg->x0 == NW_X0_LEARNED,
+             NW_E_ARG, "bad x0 mode %d", cfg->x0);
+  NW_REQUIRE(cfg->x0 != NW_X0_LEARNED || c->mpnn, NW_E_STATE,
+             "x0 = learned needs a network (nw_linsys_set_learned_guess)");
   std::memset(out, 0, sizeof(*out));
   NW_HIP(hipSetDevice(c->device));
   const int64_t n = c->g.nnode, nnz = c->g.nnzb;
   const bool cheb = cfg->precond == NW_PREC_CHEBYSHEV;
+  const bool amgp = cfg->precond == NW_PREC_AMG;
   const bool f32 = cfg->mode == NW_SOLVE_FP32_REFINE;
   if (c->kry && (c->kry->n != n || c->kry->m < cfg->kspace)) {
     free_krylov(c->kry);
@@ -813,12 +751,24 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
     NW_HIP(hipEventCreate(&k->ev0));
     NW_HIP(hipEventCreate(&k->ev1));
   }
-  int rc = f32 ? alloc_space(k->f, n, nnz, k->m, cheb)
-               : alloc_space(k->d, n, nnz, k->m, cheb);
+  int rc = f32 ? alloc_space(k->f, n, nnz, k->m, cheb || amgp)
+               : alloc_space(k->d, n, nnz, k->m, cheb || amgp);
   if (rc) return rc;
   if (cheb && !k->pe) NW_HIP(hipMalloc(&k->pe, sizeof(double) * 2 * n));
 
   hipStream_t st = c->stream;
+  if (amgp && (!c->amg || !cfg->reuse_preconditioner)) {
+    amg_free(c->amg);
+    c->amg = nullptr;
+    rc = amg_setup(c, &c->amg);
+    if (rc) return rc;
+    out->setup_ms = 1e3 * amg_setup_seconds(c->amg);
+  }
+  if (amgp) {
+    rc = f32 ? amg_prepare<float>(c, c->amg) : amg_prepare<double>(c, c->amg);
+    if (rc) return rc;
+    out->precond_levels = amg_levels(c->amg);
+  }
   NW_HIP(hipEventRecord(k->ev0, st));
   Solve S{c, k, cfg, lanes_per_row((double)nnz / std::max<int64_t>(n, 1))};
   S.st = st;
@@ -853,6 +803,10 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
   }
   double* x = c->g.sln;
   if (cfg->x0 == NW_X0_ZERO) NW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
+  // the guess this solve starts from, kept for recording training pairs
+  if (!c->x_init) NW_HIP(hipMalloc(&c->x_init, sizeof(double) * n));
+  NW_HIP(hipMemcpyAsync(c->x_init, x, sizeof(double) * n,
+                        hipMemcpyDeviceToDevice, st));
   double bb = 0;
   rc = S.dot64(c->g.rhs, c->g.rhs, &bb);
   if (rc) return rc;
@@ -861,6 +815,28 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
   double rnorm = 0;
   rc = S.residual(x, &rnorm);
   if (rc) return rc;
+  out->guess = cfg->x0;
+  if (cfg->x0 == NW_X0_LEARNED) {
+    // the network's prediction from (A, b, x0); kept only if it lowers the
+    // residual, otherwise GMRES starts from x0 (the previous-step guess)
+    if (!k->xg) NW_HIP(hipMalloc(&k->xg, sizeof(double) * n));
+    rc = mpnn_predict_device(c->mpnn, c->x_init, k->xg);
+    out->guess_ms = mpnn_last_ms(c->mpnn);
+    double rl = 0;
+    if (rc == NW_OK) rc = S.residual(k->xg, &rl);
+    if (rc != NW_OK && rc != NW_E_NONFINITE) return rc;
+    if (rc == NW_OK && std::isfinite(rl) && rl < rnorm) {
+      NW_HIP(hipMemcpyAsync(x, k->xg, sizeof(double) * n,
+                            hipMemcpyDeviceToDevice, st));
+      rnorm = rl;
+    } else {
+      out->guess = NW_X0_SOLUTION;
+      rc = S.residual(x, &rnorm);  // the cycle starts from the residual in k->r
+      if (rc) return rc;
+    }
+    rc = NW_OK;
+  }
+  out->x0_rel = rnorm / bnorm;
   while (rnorm > cfg->tol * bnorm && S.iters < cfg->max_iters &&
          std::isfinite(rnorm)) {
     const double stop_abs = cfg->tol * bnorm;

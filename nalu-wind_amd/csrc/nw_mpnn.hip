@@ -1,0 +1,1050 @@
+// nw_mpnn.hip -- learned initial guess for the pressure-Poisson solve.
+//
+// A message-passing network on the graph of the assembled 1-dof matrix
+// predicts the change of the solution from the previous-step guess x0:
+//
+//   features f_n = (b/s_b, x0/s_x, r/s_r, log|a_nn| - mean, dirichlet),
+//                  r = b - A x0, s_* the RMS of each vector
+//   h0      = relu(W_in f + b_in)
+//   m_l     = sum_{m != n} w_nm h_{l-1,m},  w_nm = sqrt|a_nm a_mn| / sqrt|a_nn a_mm|
+//   h_l     = relu(W_self,l h_{l-1} + W_nbr,l m_l + b_l),   l = 1..L
+//   delta_n = w_out . h_L + b_out,     x = x0 + s_e delta,  s_e = rms(r / diag)
+//
+// The reference has no counterpart (its pressure solve starts from x0 = 0,
+// src/LinearSolver.C:209-254); the network only changes where GMRES starts,
+// and GMRES to the same tolerance still decides the answer.  When the
+// prediction has a larger residual than x0 (or is not finite) the solve falls
+// back to x0, the previous-step guess.
+//
+// Training pairs are recorded from solves on the device: the features of
+// (A, b, x0) and the target (x - x0)/s_e in FP32 SoA buffers.  Training is
+// mean-squared error with Adam; forward and backward run in the kernels
+// below (FP32 storage and arithmetic, FP64 reductions in a fixed order, so a
+// gradient is reproducible run to run), the Adam update on the host over the
+// FP64 master parameters.
+//
+// Layouts: node activations are node-major [n][H] float rows, so a
+// neighbour's whole row is one contiguous 256-byte read; features are SoA
+// [5][n].  Parameters are the flat vector W_in[H][5], b_in[H], per layer
+// W_self[H][H], W_nbr[H][H], b[H], then w_out[H], b_out; the forward reads a
+// transposed copy (k-major) so that the H outputs of one input are
+// consecutive and wave-uniform.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "nw_internal.h"
+
+namespace nw {
+
+namespace {
+
+constexpr int kF = 5;           // node features
+constexpr int kTileN = 32;      // nodes per staging tile (weight gradients)
+constexpr int kGradBlocks = 512;
+constexpr int kStats = 5;       // feature statistics
+
+__device__ inline double mp_wave_sum(double v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <int NA>
+__device__ inline void mp_block_store(double (&acc)[NA], double* __restrict__ out)
+{
+  __shared__ double sh[kBlock / kWave][NA];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NA; ++k) {
+    const double s = mp_wave_sum(acc[k]);
+    if (lane == 0) sh[wv][k] = s;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < NA; k += blockDim.x) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / kWave; ++w) s += sh[w][k];
+    out[k] = s;
+  }
+}
+
+// out[v] = sum_b partial[b * nv + v], one workgroup per value, fixed order.
+__global__ __launch_bounds__(kBlock) void k_mp_reduce(
+  const double* __restrict__ partial, int nb, int nv, double* __restrict__ out)
+{
+  const int v = blockIdx.x;
+  double a[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    a[0] += partial[(int64_t)b * nv + v];
+  __shared__ double tot[1];
+  mp_block_store<1>(a, tot);
+  if (threadIdx.x == 0) out[v] = tot[0];
+}
+
+__device__ inline float relu(float z) { return z > 0.f ? z : 0.f; }
+
+// w[s] for every slot s of row i; the transposed entry a_ci is found by a
+// search of row c (the pattern of an edge graph is symmetric; a missing
+// entry counts as zero).
+__global__ void k_slot_weights(const int32_t* __restrict__ rp,
+                               const int32_t* __restrict__ ci,
+                               const double* __restrict__ va,
+                               const int32_t* __restrict__ diag,
+                               float* __restrict__ w, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double di = fabs(va[diag[i]]);
+  for (int s = rp[i]; s < rp[i + 1]; ++s) {
+    const int c = ci[s];
+    if (c == i) {
+      w[s] = 0.f;
+      continue;
+    }
+    const double dc = fabs(va[diag[c]]);
+    double act = 0.0;
+    for (int t = rp[c]; t < rp[c + 1]; ++t)
+      if (ci[t] == i) {
+        act = va[t];
+        break;
+      }
+    const double amn = fabs(va[s]) * fabs(act);
+    w[s] = (di > 0 && dc > 0) ? (float)(sqrt(amn) / sqrt(di * dc)) : 0.f;
+  }
+}
+
+// r = b - A x0 and the sums of b^2, x0^2, r^2, (r/|d|)^2, log|d|.
+__global__ __launch_bounds__(kBlock) void k_feat_stats(
+  const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+  const double* __restrict__ va, const int32_t* __restrict__ diag,
+  const double* __restrict__ b, const double* __restrict__ x0,
+  double* __restrict__ r, int n, double* __restrict__ partial)
+{
+  double acc[kStats] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    double ax = 0.0;
+    for (int s = rp[i]; s < rp[i + 1]; ++s) ax += va[s] * x0[ci[s]];
+    const double ri = b[i] - ax;
+    r[i] = ri;
+    const double d = fabs(va[diag[i]]);
+    const double e = d > 0 ? ri / d : 0.0;
+    acc[0] += b[i] * b[i];
+    acc[1] += x0[i] * x0[i];
+    acc[2] += ri * ri;
+    acc[3] += e * e;
+    acc[4] += log(d > 0 ? d : 1.0);
+  }
+  mp_block_store<kStats>(acc, partial + (int64_t)blockIdx.x * kStats);
+}
+
+struct FeatScale {
+  double isb, isx, isr, mlog, ise;
+};
+
+// Features (SoA) and, when xs is given, the target (xs - x0)/s_e.
+__global__ void k_features(const int32_t* __restrict__ rp,
+                           const double* __restrict__ va,
+                           const int32_t* __restrict__ ci,
+                           const int32_t* __restrict__ diag,
+                           const double* __restrict__ b,
+                           const double* __restrict__ x0,
+                           const double* __restrict__ r,
+                           const double* __restrict__ xs, FeatScale sc,
+                           float* __restrict__ feat, float* __restrict__ target,
+                           int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = fabs(va[diag[i]]);
+  int dir = 1;
+  for (int s = rp[i]; s < rp[i + 1]; ++s)
+    if (ci[s] != i && va[s] != 0.0) dir = 0;
+  feat[i] = (float)(b[i] * sc.isb);
+  feat[(int64_t)n + i] = (float)(x0[i] * sc.isx);
+  feat[2 * (int64_t)n + i] = (float)(r[i] * sc.isr);
+  feat[3 * (int64_t)n + i] = (float)(log(d > 0 ? d : 1.0) - sc.mlog);
+  feat[4 * (int64_t)n + i] = (float)dir;
+  if (xs) target[i] = (float)((xs[i] - x0[i]) * sc.ise);
+}
+
+template <int H>
+__global__ __launch_bounds__(kBlock) void k_input_layer(
+  const float* __restrict__ feat, const float* __restrict__ WinT,
+  const float* __restrict__ bin, float* __restrict__ h0, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float f[kF];
+#pragma unroll
+  for (int k = 0; k < kF; ++k) f[k] = feat[(int64_t)k * n + i];
+  float4* out = reinterpret_cast<float4*>(h0 + (int64_t)i * H);
+#pragma unroll
+  for (int j4 = 0; j4 < H / 4; ++j4) {
+    float z[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = 4 * j4 + q;
+      float a = bin[j];
+#pragma unroll
+      for (int k = 0; k < kF; ++k) a = fmaf(WinT[k * H + j], f[k], a);
+      z[q] = relu(a);
+    }
+    out[j4] = make_float4(z[0], z[1], z[2], z[3]);
+  }
+}
+
+// h_next = relu(W_self h + W_nbr m + b) with m = sum_s w_s h[col_s]; one
+// thread per node, the H accumulators in registers, weights k-major.
+template <int H>
+__global__ __launch_bounds__(kBlock) void k_mp_layer(
+  const float* __restrict__ hp, const int32_t* __restrict__ rp,
+  const int32_t* __restrict__ ci, const float* __restrict__ w,
+  const float* __restrict__ WsT, const float* __restrict__ WnT,
+  const float* __restrict__ bias, float* __restrict__ hn, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc[H];
+#pragma unroll
+  for (int j = 0; j < H; ++j) acc[j] = bias[j];
+  const int s0 = rp[i], s1 = rp[i + 1];
+  const float4* own = reinterpret_cast<const float4*>(hp + (int64_t)i * H);
+  for (int k4 = 0; k4 < H / 4; ++k4) {
+    const float4 a = own[k4];
+    float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int s = s0; s < s1; ++s) {
+      const float ws = w[s];
+      if (ws != 0.f) {
+        const float4 hc =
+          reinterpret_cast<const float4*>(hp + (int64_t)ci[s] * H)[k4];
+        m.x = fmaf(ws, hc.x, m.x);
+        m.y = fmaf(ws, hc.y, m.y);
+        m.z = fmaf(ws, hc.z, m.z);
+        m.w = fmaf(ws, hc.w, m.w);
+      }
+    }
+    const float av[4] = {a.x, a.y, a.z, a.w};
+    const float mv[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float* ws = WsT + (4 * k4 + q) * H;
+      const float* wn = WnT + (4 * k4 + q) * H;
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+        acc[j] = fmaf(ws[j], av[q], fmaf(wn[j], mv[q], acc[j]));
+    }
+  }
+  float4* out = reinterpret_cast<float4*>(hn + (int64_t)i * H);
+#pragma unroll
+  for (int j4 = 0; j4 < H / 4; ++j4)
+    out[j4] = make_float4(relu(acc[4 * j4]), relu(acc[4 * j4 + 1]),
+                          relu(acc[4 * j4 + 2]), relu(acc[4 * j4 + 3]));
+}
+
+template <int H>
+__global__ void k_output(const float* __restrict__ hL,
+                         const float* __restrict__ wout, float bout,
+                         float* __restrict__ delta, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4* h = reinterpret_cast<const float4*>(hL + (int64_t)i * H);
+  float s = bout;
+#pragma unroll
+  for (int j4 = 0; j4 < H / 4; ++j4) {
+    const float4 v = h[j4];
+    s = fmaf(wout[4 * j4], v.x, s);
+    s = fmaf(wout[4 * j4 + 1], v.y, s);
+    s = fmaf(wout[4 * j4 + 2], v.z, s);
+    s = fmaf(wout[4 * j4 + 3], v.w, s);
+  }
+  delta[i] = s;
+}
+
+// x = x0 + s_e delta
+__global__ void k_apply_guess(const double* __restrict__ x0,
+                              const float* __restrict__ delta, double se,
+                              double* __restrict__ x, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x0[i] + se * (double)delta[i];
+}
+
+// dd = coef (delta - target); partial[blk * nv + off] = sum (delta - t)^2
+__global__ __launch_bounds__(kBlock) void k_loss(
+  const float* __restrict__ delta, const float* __restrict__ target,
+  float coef, float* __restrict__ dd, int n, double* __restrict__ partial,
+  int nv, int off)
+{
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const float e = delta[i] - target[i];
+    dd[i] = coef * e;
+    acc[0] += (double)e * (double)e;
+  }
+  __shared__ double tot[1];
+  mp_block_store<1>(acc, tot);
+  if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nv + off] = tot[0];
+}
+
+// gz[i][j] = (hL[i][j] > 0) ? dd[i] wout[j] : 0
+template <int H>
+__global__ void k_output_back(const float* __restrict__ hL,
+                              const float* __restrict__ dd,
+                              const float* __restrict__ wout,
+                              float* __restrict__ gz, int n)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * (H / 4)) return;
+  const int i = (int)(t / (H / 4)), j4 = (int)(t % (H / 4));
+  const float4 h = reinterpret_cast<const float4*>(hL + (int64_t)i * H)[j4];
+  const float d = dd[i];
+  float4 g;
+  g.x = h.x > 0.f ? d * wout[4 * j4] : 0.f;
+  g.y = h.y > 0.f ? d * wout[4 * j4 + 1] : 0.f;
+  g.z = h.z > 0.f ? d * wout[4 * j4 + 2] : 0.f;
+  g.w = h.w > 0.f ? d * wout[4 * j4 + 3] : 0.f;
+  reinterpret_cast<float4*>(gz + (int64_t)i * H)[j4] = g;
+}
+
+// gs = W_self^T gz, q = W_nbr^T gz (weights row-major [j][k]).
+template <int H>
+__global__ __launch_bounds__(kBlock) void k_back_dense(
+  const float* __restrict__ gz, const float* __restrict__ Ws,
+  const float* __restrict__ Wn, float* __restrict__ gs, float* __restrict__ q,
+  int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4* g4 = reinterpret_cast<const float4*>(gz + (int64_t)i * H);
+  for (int pass = 0; pass < 2; ++pass) {
+    const float* W = pass ? Wn : Ws;
+    float acc[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) acc[k] = 0.f;
+    for (int j4 = 0; j4 < H / 4; ++j4) {
+      const float4 g = g4[j4];
+      const float gv[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const float* wr = W + (4 * j4 + qq) * H;
+#pragma unroll
+        for (int k = 0; k < H; ++k) acc[k] = fmaf(wr[k], gv[qq], acc[k]);
+      }
+    }
+    float4* out = reinterpret_cast<float4*>((pass ? q : gs) + (int64_t)i * H);
+#pragma unroll
+    for (int k4 = 0; k4 < H / 4; ++k4)
+      out[k4] = make_float4(acc[4 * k4], acc[4 * k4 + 1], acc[4 * k4 + 2],
+                            acc[4 * k4 + 3]);
+  }
+}
+
+// gz_prev[i] = mask(h_prev[i] > 0) * (gs[i] + sum_s w_s q[col_s]); the
+// aggregation is the forward's (the edge weights are symmetric), so the
+// transpose needs no scatter.  gz_prev may alias gs.
+template <int H>
+__global__ void k_back_aggregate(const float* gs, const float* __restrict__ q,
+                                 const int32_t* __restrict__ rp,
+                                 const int32_t* __restrict__ ci,
+                                 const float* __restrict__ w,
+                                 const float* __restrict__ hp, float* gz_prev,
+                                 int n)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * (H / 4)) return;
+  const int i = (int)(t / (H / 4)), k4 = (int)(t % (H / 4));
+  float4 g = reinterpret_cast<const float4*>(gs + (int64_t)i * H)[k4];
+  for (int s = rp[i]; s < rp[i + 1]; ++s) {
+    const float ws = w[s];
+    if (ws != 0.f) {
+      const float4 qc = reinterpret_cast<const float4*>(q + (int64_t)ci[s] * H)[k4];
+      g.x = fmaf(ws, qc.x, g.x);
+      g.y = fmaf(ws, qc.y, g.y);
+      g.z = fmaf(ws, qc.z, g.z);
+      g.w = fmaf(ws, qc.w, g.w);
+    }
+  }
+  const float4 h = reinterpret_cast<const float4*>(hp + (int64_t)i * H)[k4];
+  g.x = h.x > 0.f ? g.x : 0.f;
+  g.y = h.y > 0.f ? g.y : 0.f;
+  g.z = h.z > 0.f ? g.z : 0.f;
+  g.w = h.w > 0.f ? g.w : 0.f;
+  reinterpret_cast<float4*>(gz_prev + (int64_t)i * H)[k4] = g;
+}
+
+enum { kModeLayer = 0, kModeInput = 1, kModeOutput = 2 };
+
+template <int H, int MODE>
+struct WgradShape {
+  static constexpr int JA = MODE == kModeOutput ? 1 : H;
+  static constexpr int KB =
+    MODE == kModeLayer ? 2 * H + 1 : (MODE == kModeInput ? kF + 1 : H + 1);
+  static constexpr int NOUT = JA * KB;
+};
+
+// Weight gradients as node sums of outer products A_i B_i^T:
+//   layer : A = gz [n][H], B = [h_prev | m | 1], m recomputed from h_prev
+//   input : A = gz0 [n][H], B = [features | 1]
+//   output: A = dd [n],     B = [h_L | 1]
+// Block b sums a contiguous node range in tiles of kTileN staged in LDS
+// (FP32 products within a tile, FP64 across tiles) and writes
+// partial[b * nv + off + o], o = j * KB + k.
+template <int H, int MODE>
+__global__ __launch_bounds__(kBlock) void k_wgrad(
+  const float* __restrict__ A, const float* __restrict__ B,
+  const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+  const float* __restrict__ w, int n, double* __restrict__ partial, int nv,
+  int off)
+{
+  using S = WgradShape<H, MODE>;
+  constexpr int JA = S::JA, KB = S::KB, NOUT = S::NOUT;
+  constexpr int PER = (NOUT + kBlock - 1) / kBlock;
+  __shared__ float sA[kTileN][JA];
+  __shared__ float sB[kTileN][KB];
+  double acc[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) acc[q] = 0.0;
+  const int per_block = ((n + gridDim.x - 1) / gridDim.x + kTileN - 1) / kTileN * kTileN;
+  const int i0 = min(n, (int)blockIdx.x * per_block);
+  const int i1 = min(n, i0 + per_block);
+  for (int t0 = i0; t0 < i1; t0 += kTileN) {
+    for (int idx = threadIdx.x; idx < kTileN * JA; idx += kBlock) {
+      const int t = idx / JA, j = idx % JA, node = t0 + t;
+      sA[t][j] = node < i1 ? A[(int64_t)node * JA + j] : 0.f;
+    }
+    for (int idx = threadIdx.x; idx < kTileN * KB; idx += kBlock) {
+      const int t = idx / KB, k = idx % KB, node = t0 + t;
+      float v = 0.f;
+      if (node < i1) {
+        if (MODE == kModeLayer) {
+          if (k < H) {
+            v = B[(int64_t)node * H + k];
+          } else if (k < 2 * H) {
+            for (int s = rp[node]; s < rp[node + 1]; ++s) {
+              const float ws = w[s];
+              if (ws != 0.f) v = fmaf(ws, B[(int64_t)ci[s] * H + (k - H)], v);
+            }
+          } else {
+            v = 1.f;
+          }
+        } else if (MODE == kModeInput) {
+          v = k < kF ? B[(int64_t)k * n + node] : 1.f;
+        } else {
+          v = k < H ? B[(int64_t)node * H + k] : 1.f;
+        }
+      }
+      sB[t][k] = v;
+    }
+    __syncthreads();
+    float tacc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) tacc[q] = 0.f;
+    for (int t = 0; t < kTileN; ++t) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int o = threadIdx.x + q * kBlock;
+        if (o < NOUT) tacc[q] = fmaf(sA[t][o / KB], sB[t][o % KB], tacc[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) acc[q] += (double)tacc[q];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int o = threadIdx.x + q * kBlock;
+    if (o < NOUT) partial[(int64_t)blockIdx.x * nv + off + o] = acc[q];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- model
+
+struct Sample {
+  float* feat = nullptr;    // [5][n]
+  float* target = nullptr;  // [n]
+};
+
+struct Mpnn {
+  Ctx* c = nullptr;
+  int H = 0, L = 0;
+  int64_t n = 0, nnz = 0;
+  int64_t np = 0;
+  std::vector<double> P;   // FP64 master parameters
+  std::vector<double> am, av;  // Adam moments
+  int64_t adam_t = 0;
+  float* dP = nullptr;     // device parameters, flat layout
+  float* dPT = nullptr;    // W_in^T, then per layer W_self^T, W_nbr^T
+  float* w = nullptr;      // [nnz] slot weights
+  std::vector<float*> h;   // L+1 activations [n][H]
+  float *gs = nullptr, *gq = nullptr, *gz = nullptr;
+  float *feat = nullptr, *target = nullptr, *delta = nullptr, *dd = nullptr;
+  double* r = nullptr;     // FP64 residual of x0
+  double* partial = nullptr;
+  double* red = nullptr;
+  int nv = 0;              // values in one gradient partial row
+  std::vector<int64_t> gmap;  // reduced value -> parameter index (-1: loss)
+  std::vector<Sample> samples;
+  double se = 1.0;         // scale of the last prediction
+  double last_ms = 0.0;
+};
+
+void mpnn_detach(Ctx* c, Mpnn* m);
+
+}  // namespace nw
+
+struct nw_mpnn : nw::Mpnn {};
+
+namespace nw {
+namespace {
+
+struct Off {
+  int64_t win, bin, l0, ls, wout, bout, n;
+  Off(int H, int L)
+  {
+    win = 0;
+    bin = (int64_t)H * kF;
+    l0 = bin + H;
+    ls = 2 * (int64_t)H * H + H;
+    wout = l0 + L * ls;
+    bout = wout + H;
+    n = bout + 1;
+  }
+  int64_t ws(int l) const { return l0 + l * ls; }
+  int64_t wn(int l, int H) const { return ws(l) + (int64_t)H * H; }
+  int64_t b(int l, int H) const { return ws(l) + 2 * (int64_t)H * H; }
+};
+
+int upload_params(Mpnn* m)
+{
+  const int H = m->H, L = m->L;
+  Off o(H, L);
+  std::vector<float> f(m->np), t((size_t)H * kF + 2 * (size_t)L * H * H);
+  for (int64_t i = 0; i < m->np; ++i) f[i] = (float)m->P[i];
+  for (int j = 0; j < H; ++j)
+    for (int k = 0; k < kF; ++k) t[(size_t)k * H + j] = f[o.win + j * kF + k];
+  for (int l = 0; l < L; ++l) {
+    float* ws = t.data() + (size_t)H * kF + 2 * (size_t)l * H * H;
+    float* wn = ws + (size_t)H * H;
+    for (int j = 0; j < H; ++j)
+      for (int k = 0; k < H; ++k) {
+        ws[(size_t)k * H + j] = f[o.ws(l) + (int64_t)j * H + k];
+        wn[(size_t)k * H + j] = f[o.wn(l, H) + (int64_t)j * H + k];
+      }
+  }
+  NW_HIP(hipMemcpyAsync(m->dP, f.data(), sizeof(float) * m->np,
+                        hipMemcpyHostToDevice, m->c->stream));
+  NW_HIP(hipMemcpyAsync(m->dPT, t.data(), sizeof(float) * t.size(),
+                        hipMemcpyHostToDevice, m->c->stream));
+  NW_HIP(hipStreamSynchronize(m->c->stream));
+  return NW_OK;
+}
+
+int check_graph(Mpnn* m)
+{
+  const Graph& g = m->c->g;
+  NW_REQUIRE(g.ndof == 1 && g.nnode > 0, NW_E_STATE,
+             "the learned guess needs a built 1-dof system");
+  NW_REQUIRE(g.nnode == m->n && g.nnzb == m->nnz, NW_E_STATE,
+             "the system changed size since the network was created");
+  return NW_OK;
+}
+
+int ensure_buffers(Mpnn* m, bool training)
+{
+  const int64_t n = m->n;
+  if (!m->w) {
+    NW_HIP(hipMalloc(&m->w, sizeof(float) * m->nnz));
+    NW_HIP(hipMalloc(&m->feat, sizeof(float) * kF * n));
+    NW_HIP(hipMalloc(&m->target, sizeof(float) * n));
+    NW_HIP(hipMalloc(&m->delta, sizeof(float) * n));
+    NW_HIP(hipMalloc(&m->dd, sizeof(float) * n));
+    NW_HIP(hipMalloc(&m->r, sizeof(double) * n));
+    NW_HIP(hipMalloc(&m->partial, sizeof(double) * (size_t)kGradBlocks * m->nv));
+    NW_HIP(hipMalloc(&m->red, sizeof(double) * m->nv));
+    m->h.assign(2, nullptr);
+    for (auto& p : m->h) NW_HIP(hipMalloc(&p, sizeof(float) * n * m->H));
+  }
+  if (training && (int)m->h.size() < m->L + 1) {
+    while ((int)m->h.size() < m->L + 1) {
+      float* p = nullptr;
+      NW_HIP(hipMalloc(&p, sizeof(float) * n * m->H));
+      m->h.push_back(p);
+    }
+  }
+  if (training && !m->gs) {
+    NW_HIP(hipMalloc(&m->gs, sizeof(float) * n * m->H));
+    NW_HIP(hipMalloc(&m->gq, sizeof(float) * n * m->H));
+    NW_HIP(hipMalloc(&m->gz, sizeof(float) * n * m->H));
+  }
+  return NW_OK;
+}
+
+int slot_weights(Mpnn* m)
+{
+  const Graph& g = m->c->g;
+  k_slot_weights<<<grid_for(m->n, kBlock), kBlock, 0, m->c->stream>>>(
+    g.row_ptr, g.cols, g.vals, g.diag, m->w, (int)m->n);
+  NW_LAUNCH_CHECK();
+  return NW_OK;
+}
+
+// features of (A, b, x0) into feat; target from xs when given
+int features(Mpnn* m, const double* x0, const double* xs, float* feat,
+             float* target)
+{
+  Ctx* c = m->c;
+  const Graph& g = c->g;
+  const int n = (int)m->n;
+  hipStream_t st = c->stream;
+  k_feat_stats<<<kGradBlocks, kBlock, 0, st>>>(g.row_ptr, g.cols, g.vals,
+                                              g.diag, g.rhs, x0, m->r, n,
+                                              m->partial);
+  NW_LAUNCH_CHECK();
+  k_mp_reduce<<<kStats, kBlock, 0, st>>>(m->partial, kGradBlocks, kStats, m->red);
+  NW_LAUNCH_CHECK();
+  double s[kStats];
+  NW_HIP(hipMemcpyAsync(s, m->red, sizeof(s), hipMemcpyDeviceToHost, st));
+  NW_HIP(hipStreamSynchronize(st));
+  auto rms = [n](double ss) {
+    const double v = std::sqrt(ss / (double)std::max(n, 1));
+    return v > 0 ? v : 1.0;
+  };
+  FeatScale sc;
+  sc.isb = 1.0 / rms(s[0]);
+  sc.isx = 1.0 / rms(s[1]);
+  sc.isr = 1.0 / rms(s[2]);
+  m->se = rms(s[3]);
+  sc.ise = 1.0 / m->se;
+  sc.mlog = s[4] / (double)std::max(n, 1);
+  NW_REQUIRE(std::isfinite(s[0] + s[1] + s[2] + s[3] + s[4]), NW_E_NONFINITE,
+             "non-finite input to the learned guess");
+  k_features<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.row_ptr, g.vals, g.cols,
+                                                    g.diag, g.rhs, x0, m->r, xs,
+                                                    sc, feat, target, n);
+  NW_LAUNCH_CHECK();
+  return NW_OK;
+}
+
+template <int H>
+int forward_h(Mpnn* m, const float* feat, bool keep)
+{
+  Ctx* c = m->c;
+  const Graph& g = c->g;
+  const int n = (int)m->n, L = m->L;
+  hipStream_t st = c->stream;
+  const Off o(H, L);
+  const float* WT = m->dPT;
+  k_input_layer<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(
+    feat, WT, m->dP + o.bin, m->h[0], n);
+  NW_LAUNCH_CHECK();
+  int cur = 0;
+  for (int l = 0; l < L; ++l) {
+    const float* WsT = WT + (size_t)H * kF + 2 * (size_t)l * H * H;
+    const int nxt = keep ? l + 1 : 1 - cur;
+    k_mp_layer<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(
+      m->h[cur], g.row_ptr, g.cols, m->w, WsT, WsT + (size_t)H * H,
+      m->dP + o.b(l, H), m->h[nxt], n);
+    NW_LAUNCH_CHECK();
+    cur = nxt;
+  }
+  float bout = (float)m->P[o.bout];
+  k_output<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(m->h[cur], m->dP + o.wout,
+                                                      bout, m->delta, n);
+  NW_LAUNCH_CHECK();
+  return NW_OK;
+}
+
+int forward(Mpnn* m, const float* feat, bool keep)
+{
+  return m->H == 64 ? forward_h<64>(m, feat, keep) : forward_h<32>(m, feat, keep);
+}
+
+// Loss and gradient of one sample accumulated into grad (scaled by 1/batch).
+template <int H>
+int loss_grad_h(Mpnn* m, const Sample& smp, double scale, double* loss,
+                std::vector<double>& grad)
+{
+  Ctx* c = m->c;
+  const Graph& g = c->g;
+  const int n = (int)m->n, L = m->L, nv = m->nv;
+  hipStream_t st = c->stream;
+  const Off o(H, L);
+  int rc = forward_h<H>(m, smp.feat, true);
+  if (rc) return rc;
+  const int off_loss = nv - 1;
+  k_loss<<<kGradBlocks, kBlock, 0, st>>>(m->delta, smp.target,
+                                        (float)(2.0 * scale / n), m->dd, n,
+                                        m->partial, nv, off_loss);
+  NW_LAUNCH_CHECK();
+  // value offsets: output block, then layers L-1..0, then input block
+  int off = 0;
+  k_wgrad<H, kModeOutput><<<kGradBlocks, kBlock, 0, st>>>(
+    m->dd, m->h[L], g.row_ptr, g.cols, m->w, n, m->partial, nv, off);
+  NW_LAUNCH_CHECK();
+  off += WgradShape<H, kModeOutput>::NOUT;
+  const unsigned gq = (unsigned)(((int64_t)n * (H / 4) + kBlock - 1) / kBlock);
+  k_output_back<H><<<gq, kBlock, 0, st>>>(m->h[L], m->dd, m->dP + o.wout, m->gz, n);
+  NW_LAUNCH_CHECK();
+  float* gz = m->gz;
+  float* gs = m->gs;
+  for (int l = L - 1; l >= 0; --l) {
+    k_wgrad<H, kModeLayer><<<kGradBlocks, kBlock, 0, st>>>(
+      gz, m->h[l], g.row_ptr, g.cols, m->w, n, m->partial, nv, off);
+    NW_LAUNCH_CHECK();
+    off += WgradShape<H, kModeLayer>::NOUT;
+    k_back_dense<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(
+      gz, m->dP + o.ws(l), m->dP + o.wn(l, H), gs, m->gq, n);
+    NW_LAUNCH_CHECK();
+    k_back_aggregate<H><<<gq, kBlock, 0, st>>>(gs, m->gq, g.row_ptr, g.cols,
+                                               m->w, m->h[l], gs, n);
+    NW_LAUNCH_CHECK();
+    std::swap(gz, gs);
+  }
+  k_wgrad<H, kModeInput><<<kGradBlocks, kBlock, 0, st>>>(
+    gz, smp.feat, g.row_ptr, g.cols, m->w, n, m->partial, nv, off);
+  NW_LAUNCH_CHECK();
+  k_mp_reduce<<<nv, kBlock, 0, st>>>(m->partial, kGradBlocks, nv, m->red);
+  NW_LAUNCH_CHECK();
+  std::vector<double> red(nv);
+  NW_HIP(hipMemcpyAsync(red.data(), m->red, sizeof(double) * nv,
+                        hipMemcpyDeviceToHost, st));
+  NW_HIP(hipStreamSynchronize(st));
+  for (int v = 0; v < nv - 1; ++v) grad[m->gmap[v]] += red[v];
+  *loss = red[off_loss] / n;
+  NW_REQUIRE(std::isfinite(*loss), NW_E_NONFINITE, "non-finite training loss");
+  return NW_OK;
+}
+
+int loss_grad(Mpnn* m, const Sample& s, double scale, double* loss,
+              std::vector<double>& grad)
+{
+  return m->H == 64 ? loss_grad_h<64>(m, s, scale, loss, grad)
+                    : loss_grad_h<32>(m, s, scale, loss, grad);
+}
+
+template <int H>
+void build_gmap(Mpnn* m)
+{
+  const int L = m->L;
+  const Off o(H, L);
+  m->gmap.clear();
+  using SO = WgradShape<H, kModeOutput>;
+  for (int k = 0; k < SO::KB; ++k) m->gmap.push_back(k < H ? o.wout + k : o.bout);
+  using SL = WgradShape<H, kModeLayer>;
+  for (int l = L - 1; l >= 0; --l)
+    for (int j = 0; j < SL::JA; ++j)
+      for (int k = 0; k < SL::KB; ++k)
+        m->gmap.push_back(k < H ? o.ws(l) + (int64_t)j * H + k
+                          : k < 2 * H ? o.wn(l, H) + (int64_t)j * H + (k - H)
+                                      : o.b(l, H) + j);
+  using SI = WgradShape<H, kModeInput>;
+  for (int j = 0; j < SI::JA; ++j)
+    for (int k = 0; k < SI::KB; ++k)
+      m->gmap.push_back(k < kF ? o.win + (int64_t)j * kF + k : o.bin + j);
+  m->gmap.push_back(-1);  // loss
+  m->nv = (int)m->gmap.size();
+}
+
+void free_sample(Sample& s)
+{
+  (void)hipFree(s.feat);
+  (void)hipFree(s.target);
+  s = Sample();
+}
+
+}  // namespace
+
+void mpnn_detach(Ctx* c, Mpnn* m)
+{
+  if (c && c->mpnn == m) c->mpnn = nullptr;
+}
+
+// Prediction for the solver: x = x0 + s_e delta from the current system
+// and x0 (device); returns the elapsed wall time in *ms.
+int mpnn_predict_device(Mpnn* m, const double* x0, double* x)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = check_graph(m);
+  if (rc) return rc;
+  rc = ensure_buffers(m, false);
+  if (rc) return rc;
+  rc = slot_weights(m);
+  if (rc) return rc;
+  rc = features(m, x0, nullptr, m->feat, nullptr);
+  if (rc) return rc;
+  rc = forward(m, m->feat, false);
+  if (rc) return rc;
+  k_apply_guess<<<grid_for(m->n, kBlock), kBlock, 0, m->c->stream>>>(
+    x0, m->delta, m->se, x, (int)m->n);
+  NW_LAUNCH_CHECK();
+  NW_HIP(hipStreamSynchronize(m->c->stream));
+  m->last_ms = 1e3 * std::chrono::duration<double>(
+                       std::chrono::steady_clock::now() - t0).count();
+  return NW_OK;
+}
+
+double mpnn_last_ms(const Mpnn* m) { return m->last_ms; }
+
+}  // namespace nw
+
+using namespace nw;
+
+extern "C" {
+
+int64_t nw_mpnn_nparams(int hidden, int layers)
+{
+  if (hidden <= 0 || layers < 0) return -1;
+  return Off(hidden, layers).n;
+}
+
+int nw_mpnn_create(nw_ctx* c, int hidden, int layers, uint64_t seed,
+                   nw_mpnn** out)
+{
+  NW_REQUIRE(c && out, NW_E_ARG, "null argument");
+  NW_REQUIRE(hidden == 32 || hidden == 64, NW_E_ARG,
+             "hidden width must be 32 or 64 (got %d)", hidden);
+  NW_REQUIRE(layers >= 1 && layers <= 8, NW_E_ARG,
+             "message-passing layers must be in [1, 8] (got %d)", layers);
+  NW_REQUIRE(c->g.ndof == 1 && c->g.nnode > 0, NW_E_STATE,
+             "the learned guess needs a built 1-dof system");
+  NW_HIP(hipSetDevice(c->device));
+  nw_mpnn* m = new nw_mpnn();
+  m->c = c;
+  m->H = hidden;
+  m->L = layers;
+  m->n = c->g.nnode;
+  m->nnz = c->g.nnzb;
+  Off o(hidden, layers);
+  m->np = o.n;
+  m->P.assign(m->np, 0.0);
+  m->am.assign(m->np, 0.0);
+  m->av.assign(m->np, 0.0);
+  // He-uniform initialisation from a seeded generator; biases zero
+  std::mt19937_64 gen(seed);
+  auto fill = [&](int64_t at, int64_t cnt, int fan_in) {
+    const double a = std::sqrt(6.0 / fan_in);
+    std::uniform_real_distribution<double> u(-a, a);
+    for (int64_t i = 0; i < cnt; ++i) m->P[at + i] = u(gen);
+  };
+  fill(o.win, (int64_t)hidden * kF, kF);
+  for (int l = 0; l < layers; ++l) {
+    fill(o.ws(l), (int64_t)hidden * hidden, 2 * hidden);
+    fill(o.wn(l, hidden), (int64_t)hidden * hidden, 2 * hidden);
+  }
+  fill(o.wout, hidden, hidden);
+  for (int j = 0; j < hidden; ++j) m->P[o.wout + j] *= 0.1;
+  if (hidden == 64) build_gmap<64>(m);
+  else build_gmap<32>(m);
+  int rc = NW_OK;
+  if (hipMalloc(&m->dP, sizeof(float) * m->np) != hipSuccess ||
+      hipMalloc(&m->dPT, sizeof(float) * ((size_t)hidden * kF +
+                                          2 * (size_t)layers * hidden * hidden)) !=
+        hipSuccess) {
+    set_error("device allocation of the network parameters failed");
+    rc = NW_E_ALLOC;
+  }
+  if (!rc) rc = upload_params(m);
+  if (rc) {
+    nw_mpnn_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return NW_OK;
+}
+
+void nw_mpnn_destroy(nw_mpnn* m)
+{
+  if (!m) return;
+  (void)hipSetDevice(m->c->device);
+  (void)hipStreamSynchronize(m->c->stream);
+  mpnn_detach(m->c, m);
+  for (auto& s : m->samples) free_sample(s);
+  for (auto p : m->h) (void)hipFree(p);
+  (void)hipFree(m->dP);
+  (void)hipFree(m->dPT);
+  (void)hipFree(m->w);
+  (void)hipFree(m->gs);
+  (void)hipFree(m->gq);
+  (void)hipFree(m->gz);
+  (void)hipFree(m->feat);
+  (void)hipFree(m->target);
+  (void)hipFree(m->delta);
+  (void)hipFree(m->dd);
+  (void)hipFree(m->r);
+  (void)hipFree(m->partial);
+  (void)hipFree(m->red);
+  delete m;
+}
+
+int nw_mpnn_set_params(nw_mpnn* m, const double* p, int64_t n)
+{
+  NW_REQUIRE(m && p, NW_E_ARG, "null argument");
+  NW_REQUIRE(n == m->np, NW_E_ARG, "expected %lld parameters, got %lld",
+             (long long)m->np, (long long)n);
+  for (int64_t i = 0; i < n; ++i)
+    NW_REQUIRE(std::isfinite(p[i]), NW_E_NONFINITE, "non-finite parameter %lld",
+               (long long)i);
+  NW_HIP(hipSetDevice(m->c->device));
+  m->P.assign(p, p + n);
+  std::fill(m->am.begin(), m->am.end(), 0.0);
+  std::fill(m->av.begin(), m->av.end(), 0.0);
+  m->adam_t = 0;
+  return upload_params(m);
+}
+
+int nw_mpnn_get_params(const nw_mpnn* m, double* p, int64_t n)
+{
+  NW_REQUIRE(m && p, NW_E_ARG, "null argument");
+  NW_REQUIRE(n == m->np, NW_E_ARG, "expected %lld parameters, got %lld",
+             (long long)m->np, (long long)n);
+  std::memcpy(p, m->P.data(), sizeof(double) * n);
+  return NW_OK;
+}
+
+int nw_mpnn_record(nw_mpnn* m)
+{
+  NW_REQUIRE(m, NW_E_ARG, "null argument");
+  int rc = check_graph(m);
+  if (rc) return rc;
+  NW_REQUIRE(m->c->x_init, NW_E_STATE,
+             "no solve has run on this system yet (nothing to record)");
+  NW_HIP(hipSetDevice(m->c->device));
+  rc = ensure_buffers(m, false);
+  if (rc) return rc;
+  Sample s;
+  if (hipMalloc(&s.feat, sizeof(float) * kF * m->n) != hipSuccess ||
+      hipMalloc(&s.target, sizeof(float) * m->n) != hipSuccess) {
+    free_sample(s);
+    set_error("device allocation of a training sample failed");
+    return NW_E_ALLOC;
+  }
+  rc = features(m, m->c->x_init, m->c->g.sln, s.feat, s.target);
+  if (rc) {
+    free_sample(s);
+    return rc;
+  }
+  NW_HIP(hipStreamSynchronize(m->c->stream));
+  m->samples.push_back(s);
+  return NW_OK;
+}
+
+int nw_mpnn_samples(const nw_mpnn* m) { return m ? (int)m->samples.size() : -1; }
+
+int nw_mpnn_clear_samples(nw_mpnn* m)
+{
+  NW_REQUIRE(m, NW_E_ARG, "null argument");
+  NW_HIP(hipSetDevice(m->c->device));
+  NW_HIP(hipStreamSynchronize(m->c->stream));
+  for (auto& s : m->samples) free_sample(s);
+  m->samples.clear();
+  return NW_OK;
+}
+
+int nw_mpnn_loss_grad(nw_mpnn* m, int sample, double* loss, double* grad)
+{
+  NW_REQUIRE(m && loss, NW_E_ARG, "null argument");
+  NW_REQUIRE(sample >= 0 && sample < (int)m->samples.size(), NW_E_ARG,
+             "sample %d out of range [0, %d)", sample, (int)m->samples.size());
+  int rc = check_graph(m);
+  if (rc) return rc;
+  NW_HIP(hipSetDevice(m->c->device));
+  rc = ensure_buffers(m, true);
+  if (rc) return rc;
+  rc = slot_weights(m);
+  if (rc) return rc;
+  std::vector<double> g(m->np, 0.0);
+  rc = loss_grad(m, m->samples[sample], 1.0, loss, g);
+  if (rc) return rc;
+  if (grad) std::memcpy(grad, g.data(), sizeof(double) * m->np);
+  return NW_OK;
+}
+
+int nw_mpnn_train(nw_mpnn* m, const nw_mpnn_train_cfg* cfg, double* last_loss)
+{
+  NW_REQUIRE(m && cfg, NW_E_ARG, "null argument");
+  NW_REQUIRE(!m->samples.empty(), NW_E_STATE, "no recorded training samples");
+  NW_REQUIRE(cfg->epochs >= 0 && cfg->batch >= 1 && cfg->lr > 0 &&
+               cfg->beta1 >= 0 && cfg->beta1 < 1 && cfg->beta2 >= 0 &&
+               cfg->beta2 < 1 && cfg->eps > 0,
+             NW_E_ARG, "bad training configuration");
+  int rc = check_graph(m);
+  if (rc) return rc;
+  NW_HIP(hipSetDevice(m->c->device));
+  rc = ensure_buffers(m, true);
+  if (rc) return rc;
+  rc = slot_weights(m);
+  if (rc) return rc;
+  const int ns = (int)m->samples.size();
+  std::vector<int> order(ns);
+  for (int i = 0; i < ns; ++i) order[i] = i;
+  std::mt19937_64 gen(cfg->seed);
+  std::vector<double> g(m->np);
+  double epoch_loss = 0.0;
+  for (int ep = 0; ep < cfg->epochs; ++ep) {
+    std::shuffle(order.begin(), order.end(), gen);
+    epoch_loss = 0.0;
+    for (int b0 = 0; b0 < ns; b0 += cfg->batch) {
+      const int nb = std::min(cfg->batch, ns - b0);
+      std::fill(g.begin(), g.end(), 0.0);
+      for (int q = 0; q < nb; ++q) {
+        double l = 0.0;
+        rc = loss_grad(m, m->samples[order[b0 + q]], 1.0 / nb, &l, g);
+        if (rc) return rc;
+        epoch_loss += l;
+      }
+      // Adam (Kingma & Ba 2015) on the FP64 master parameters
+      ++m->adam_t;
+      const double c1 = 1.0 - std::pow(cfg->beta1, (double)m->adam_t);
+      const double c2 = 1.0 - std::pow(cfg->beta2, (double)m->adam_t);
+      for (int64_t i = 0; i < m->np; ++i) {
+        m->am[i] = cfg->beta1 * m->am[i] + (1.0 - cfg->beta1) * g[i];
+        m->av[i] = cfg->beta2 * m->av[i] + (1.0 - cfg->beta2) * g[i] * g[i];
+        m->P[i] -= cfg->lr * (m->am[i] / c1) / (std::sqrt(m->av[i] / c2) + cfg->eps);
+      }
+      rc = upload_params(m);
+      if (rc) return rc;
+    }
+    epoch_loss /= ns;
+  }
+  if (last_loss) *last_loss = epoch_loss;
+  return NW_OK;
+}
+
+int nw_mpnn_predict(nw_mpnn* m, double* x_host, double* scale)
+{
+  NW_REQUIRE(m && x_host, NW_E_ARG, "null argument");
+  NW_HIP(hipSetDevice(m->c->device));
+  const int64_t n = m->n;
+  double* x = nullptr;
+  NW_HIP(hipMalloc(&x, sizeof(double) * n));
+  int rc = mpnn_predict_device(m, m->c->g.sln, x);
+  if (!rc) {
+    if (hipMemcpy(x_host, x, sizeof(double) * n, hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+      set_error("copy of the prediction failed");
+      rc = NW_E_HIP;
+    }
+  }
+  (void)hipFree(x);
+  if (scale) *scale = m->se;
+  return rc;
+}
+
+int nw_linsys_set_learned_guess(nw_ctx* c, nw_mpnn* m)
+{
+  NW_REQUIRE(c, NW_E_ARG, "null context");
+  NW_REQUIRE(!m || m->c == c, NW_E_ARG,
+             "the network was created on another context");
+  c->mpnn = m;
+  return NW_OK;
+}
+
+}  // extern "C"

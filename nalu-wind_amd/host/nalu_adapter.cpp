@@ -1,0 +1,258 @@
+// nalu_adapter.cpp -- the reference-shaped host classes over the C ABI.
+#include "nalu_adapter.h"
+
+#include <chrono>
+#include <string>
+
+namespace nalu_amd {
+
+void check(int rc, const char* what)
+{
+  if (rc != NW_OK)
+    throw std::runtime_error(std::string(what) + ": " + nw_last_error());
+}
+
+nw_peclet SolutionOptions::peclet(const std::string& dof) const
+{
+  // EquationSystem::ngp_create_peclet_function (include/EquationSystem.h:
+  // 401-417): classic form with the hybrid factor, or tanh(trans, width)
+  nw_peclet p{};
+  p.form = tanhFormDefault == "classic" ? 0 : 1;
+  p.hybrid = get(hybridMap, dof, hybridDefault);
+  p.c1 = tanhTransDefault;
+  p.c2 = tanhWidthDefault;
+  return p;
+}
+
+Realm::Realm(int device) { check(nw_ctx_create(device, &ctx_), "nw_ctx_create"); }
+
+Realm::~Realm() { nw_ctx_destroy(ctx_); }
+
+void Realm::set_mesh(int64_t nnode, const std::vector<int32_t>& edge_nodes)
+{
+  check(nw_mesh_set(ctx_, nnode, (int64_t)edge_nodes.size() / 2,
+                    edge_nodes.data()),
+        "nw_mesh_set");
+  nnode_ = nnode;
+  nedge_ = (int64_t)edge_nodes.size() / 2;
+}
+
+void Realm::set_field(int field, const std::vector<double>& data)
+{
+  check(nw_field_upload(ctx_, field, data.data(), (int64_t)data.size()),
+        "nw_field_upload");
+}
+
+std::vector<double> Realm::get_field(int field) const
+{
+  const bool edge = field >= NW_E_AREA_VECTOR;
+  std::vector<double> out((edge ? nedge_ : nnode_) * nw_field_components(field));
+  check(nw_field_download(ctx_, field, out.data(), (int64_t)out.size()),
+        "nw_field_download");
+  return out;
+}
+
+namespace {
+
+class GpuCoeffApplier : public CoeffApplier {
+ public:
+  explicit GpuCoeffApplier(nw_ctx* ctx) : ctx_(ctx) {}
+  void operator()(unsigned numEntities, const int32_t* entities,
+                  const double* rhs, const double* lhs) override
+  {
+    check(nw_linsys_sum_into(ctx_, 1, (int)numEntities, entities, lhs, rhs),
+          "sumInto");
+  }
+  void resetRows(unsigned numNodes, const int32_t* nodeList, unsigned beginPos,
+                 unsigned endPos, double diag_value,
+                 double rhs_residual) override
+  {
+    check(nw_linsys_reset_rows(ctx_, numNodes, nodeList, (int)beginPos,
+                               (int)endPos, diag_value, rhs_residual),
+          "resetRows");
+  }
+
+ private:
+  nw_ctx* ctx_;
+};
+
+double now()
+{
+  return std::chrono::duration<double>(
+           std::chrono::steady_clock::now().time_since_epoch())
+    .count();
+}
+
+}  // namespace
+
+GpuLinearSystem::GpuLinearSystem(Realm& realm, unsigned numDof)
+  : realm_(realm), applier_(new GpuCoeffApplier(realm.ctx()))
+{
+  numDof_ = numDof;
+}
+
+void GpuLinearSystem::buildEdgeToNodeGraph()
+{
+  check(nw_linsys_build_edge_graph(realm_.ctx(), (int)numDof_),
+        "buildEdgeToNodeGraph");
+}
+
+void GpuLinearSystem::zeroSystem()
+{
+  check(nw_linsys_zero(realm_.ctx()), "zeroSystem");
+}
+
+void GpuLinearSystem::sumInto(unsigned numEntities, const int32_t* entities,
+                              const double* rhs, const double* lhs)
+{
+  (*applier_)(numEntities, entities, rhs, lhs);
+}
+
+void GpuLinearSystem::applyDirichletBCs(int solutionField, int bcValuesField,
+                                        const std::vector<int32_t>& nodes,
+                                        unsigned beginPos, unsigned endPos)
+{
+  check(nw_linsys_apply_dirichlet(realm_.ctx(), (int64_t)nodes.size(),
+                                  nodes.data(), solutionField, bcValuesField,
+                                  (int)beginPos, (int)endPos),
+        "applyDirichletBCs");
+}
+
+void GpuLinearSystem::resetRows(const std::vector<int32_t>& nodeList,
+                                unsigned beginPos, unsigned endPos,
+                                double diag_value, double rhs_residual)
+{
+  applier_->resetRows((unsigned)nodeList.size(), nodeList.data(), beginPos,
+                      endPos, diag_value, rhs_residual);
+}
+
+void GpuLinearSystem::loadComplete()
+{
+  check(nw_linsys_load_complete(realm_.ctx()), "loadComplete");
+}
+
+int GpuLinearSystem::solve(const nw_solver_cfg& cfg)
+{
+  const int rc = nw_linsys_solve(realm_.ctx(), &cfg, &stats_);
+  if (rc != NW_OK && rc != NW_E_NOCONV) check(rc, "solve");
+  iters_ = stats_.iters;
+  linearResidual_ = stats_.final_abs;  // ||A x - b||_2 (LinearSolver.C:251)
+  nonLinearResidual_ = stats_.bnorm;   // ||rhs||_2 (TpetraLinearSystem.C:1853-1859)
+  return rc;
+}
+
+void GpuLinearSystem::download(std::vector<int64_t>& rowOffsets,
+                               std::vector<int32_t>& cols,
+                               std::vector<double>& vals,
+                               std::vector<double>& rhs) const
+{
+  int64_t nrows = 0, nnz = 0;
+  check(nw_linsys_sizes(realm_.ctx(), &nrows, &nnz), "sizes");
+  rowOffsets.resize(nrows + 1);
+  cols.resize(nnz);
+  vals.resize(nnz);
+  rhs.resize(nrows);
+  check(nw_linsys_download(realm_.ctx(), rowOffsets.data(), cols.data(),
+                           vals.data(), rhs.data()),
+        "download");
+}
+
+std::vector<double> GpuLinearSystem::solution() const
+{
+  int64_t nrows = 0;
+  check(nw_linsys_sizes(realm_.ctx(), &nrows, nullptr), "sizes");
+  std::vector<double> x(nrows);
+  check(nw_linsys_solution_download(realm_.ctx(), x.data()), "solution");
+  return x;
+}
+
+void AssembleEdgeSolverAlgorithm::initialize_connectivity()
+{
+  // the edge graph is built once by the linear system of the equation
+  // system (buildEdgeToNodeGraph); nothing per algorithm
+}
+
+void ScalarEdgeSolverAlg::execute()
+{
+  const SolutionOptions& so = realm_.solutionOptions;
+  nw_scalar_opts o{};
+  o.alpha = so.get(so.alphaMap, dofName_, so.alphaDefault);
+  o.alpha_upw = so.get(so.alphaUpwMap, dofName_, so.alphaUpwDefault);
+  o.upw = so.get(so.upwMap, dofName_, so.upwDefault);
+  o.relax = so.get(so.relaxMap, dofName_, so.relaxDefault);
+  auto it = so.limiterMap.find(dofName_);
+  o.use_limiter = (it != so.limiterMap.end() && it->second) ? 1 : 0;
+  o.peclet = so.peclet(dofName_);
+  o.accumulate = 1;  // sumInto semantics after zeroSystem
+  check(nw_assemble_scalar(realm_.ctx(), &o), "ScalarEdgeSolverAlg");
+}
+
+void MomentumEdgeSolverAlg::execute()
+{
+  const SolutionOptions& so = realm_.solutionOptions;
+  const std::string dof = "velocity";
+  nw_momentum_opts o{};
+  o.alpha = so.get(so.alphaMap, dof, so.alphaDefault);
+  o.alpha_upw = so.get(so.alphaUpwMap, dof, so.alphaUpwDefault);
+  o.upw = so.get(so.upwMap, dof, so.upwDefault);
+  o.relax = so.get(so.relaxMap, dof, so.relaxDefault);
+  o.include_divU = so.includeDivU;
+  auto it = so.limiterMap.find(dof);
+  o.use_limiter = (it != so.limiterMap.end() && it->second) ? 1 : 0;
+  o.accumulate = 1;
+  check(nw_assemble_momentum(realm_.ctx(), &o), "MomentumEdgeSolverAlg");
+}
+
+void ContinuityEdgeSolverAlg::execute()
+{
+  const SolutionOptions& so = realm_.solutionOptions;
+  nw_continuity_opts o{};
+  o.dt = realm_.timeIntegrator.timeStepN;
+  o.gamma1 = realm_.timeIntegrator.gamma1;
+  auto it = so.nocMap.find("pressure");
+  o.noc = (it == so.nocMap.end() ? so.nocDefault : it->second) ? 1.0 : 0.0;
+  o.interp_together = so.mdotInterpRhoUTogether ? 1.0 : 0.0;
+  o.incompressible = so.solveIncompressibleContinuity ? 1.0 : 0.0;
+  o.accumulate = 1;
+  check(nw_assemble_continuity(realm_.ctx(), &o), "ContinuityEdgeSolverAlg");
+}
+
+void MomentumEdgePecletAlg::execute()
+{
+  const nw_peclet p = realm_.solutionOptions.peclet("velocity");
+  check(nw_momentum_edge_peclet(realm_.ctx(), &p), "MomentumEdgePecletAlg");
+}
+
+void FixPressureAtNodeAlgorithm::execute()
+{
+  check(nw_fix_pressure_at_node(realm_.ctx(), node_, refPressure_),
+        "FixPressureAtNodeAlgorithm");
+}
+
+void EquationSystem::initialize()
+{
+  linsys_->buildEdgeToNodeGraph();
+  for (auto& a : algs_) a->initialize_connectivity();
+  linsys_->finalizeLinearSystem();
+}
+
+void EquationSystem::assemble_and_solve(const nw_solver_cfg* cfg)
+{
+  // src/EquationSystem.C:301-350: zero, all algorithms, loadComplete, solve
+  double t0 = now();
+  linsys_->zeroSystem();
+  for (auto& a : algs_) a->execute();
+  check(nw_ctx_synchronize(realm_.ctx()), "synchronize");
+  double t1 = now();
+  timerAssemble_ += t1 - t0;
+  linsys_->loadComplete();
+  check(nw_ctx_synchronize(realm_.ctx()), "synchronize");
+  double t2 = now();
+  timerLoadComplete_ += t2 - t1;
+  if (cfg) {
+    linsys_->solve(*cfg);
+    timerSolve_ += now() - t2;
+  }
+}
+
+}  // namespace nalu_amd

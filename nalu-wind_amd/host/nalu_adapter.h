@@ -1,0 +1,262 @@
+// nalu_adapter.h -- C++ host classes that mirror the reference's
+// solver-algorithm / linear-system interface for the edge path and forward
+// to the C ABI of libnaluwind_amd (include/nalu_amd.h).  They show how the
+// reference's call sites map onto the library:
+//
+//   reference (Exawind/nalu-wind)                      here
+//   SolverAlgorithm::execute / initialize_connectivity  same names
+//     (include/SolverAlgorithm.h:85-86)
+//   AssembleEdgeSolverAlgorithm                         same name; one launch
+//     (include/AssembleEdgeSolverAlgorithm.h:31-109)    per execute()
+//   ScalarEdgeSolverAlg / MomentumEdgeSolverAlg /       same names
+//   ContinuityEdgeSolverAlg, MomentumEdgePecletAlg
+//     (src/edge_kernels/*.C)
+//   FixPressureAtNodeAlgorithm                          same name
+//     (src/FixPressureAtNodeAlgorithm.C:57-124)
+//   LinearSystem / TpetraLinearSystem                   LinearSystem /
+//     (include/LinearSystem.h:44-256)                   GpuLinearSystem
+//   CoeffApplier (include/LinearSystem.h:44-74)         CoeffApplier
+//   EquationSystem::assemble_and_solve                  EquationSystem::
+//     (src/EquationSystem.C:301-350)                    assemble_and_solve
+//
+// Errors from the library become std::runtime_error, as the reference's
+// host code throws; nothing is thrown across the C ABI itself.
+#pragma once
+
+#include <chrono>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nalu_amd.h"
+
+namespace nalu_amd {
+
+void check(int rc, const char* what);
+
+// Scheme options the edge algorithms read (SolutionOptions defaults,
+// src/SolutionOptions.C:32-52), per degree-of-freedom name.
+struct SolutionOptions {
+  double hybridDefault = 0.0, alphaDefault = 0.0, alphaUpwDefault = 1.0,
+         upwDefault = 1.0, relaxDefault = 1.0;
+  std::string tanhFormDefault = "classic";
+  double tanhTransDefault = 2.0, tanhWidthDefault = 4.0;
+  bool nocDefault = true;
+  double includeDivU = 0.0;
+  bool mdotInterpRhoUTogether = true;
+  bool solveIncompressibleContinuity = false;
+  std::map<std::string, double> hybridMap, alphaMap, alphaUpwMap, upwMap,
+    relaxMap;
+  std::map<std::string, bool> limiterMap, nocMap;
+
+  double get(const std::map<std::string, double>& m, const std::string& k,
+             double d) const
+  {
+    auto it = m.find(k);
+    return it == m.end() ? d : it->second;
+  }
+  nw_peclet peclet(const std::string& dof) const;
+};
+
+struct TimeIntegrator {
+  double timeStepN = 1.0;
+  double gamma1 = 1.0;
+};
+
+// The part of Realm the edge path needs: one GPU context holding the mesh
+// (nodes, edges) and its fields, the options and the time integrator.
+class Realm {
+ public:
+  explicit Realm(int device = 0);
+  ~Realm();
+  Realm(const Realm&) = delete;
+  Realm& operator=(const Realm&) = delete;
+
+  void set_mesh(int64_t nnode, const std::vector<int32_t>& edge_nodes);
+  void set_field(int field, const std::vector<double>& data);
+  std::vector<double> get_field(int field) const;
+  int64_t nnode() const { return nnode_; }
+  int64_t nedge() const { return nedge_; }
+  nw_ctx* ctx() const { return ctx_; }
+
+  SolutionOptions solutionOptions;
+  TimeIntegrator timeIntegrator;
+
+ private:
+  nw_ctx* ctx_ = nullptr;
+  int64_t nnode_ = 0, nedge_ = 0;
+};
+
+// LinearSystem::CoeffApplier: additive block scatter + row reset.
+class CoeffApplier {
+ public:
+  virtual ~CoeffApplier() = default;
+  virtual void operator()(unsigned numEntities, const int32_t* entities,
+                          const double* rhs, const double* lhs) = 0;
+  virtual void resetRows(unsigned numNodes, const int32_t* nodeList,
+                         unsigned beginPos, unsigned endPos,
+                         double diag_value = 0.0, double rhs_residual = 0.0) = 0;
+};
+
+class LinearSystem {
+ public:
+  virtual ~LinearSystem() = default;
+  virtual void buildEdgeToNodeGraph() = 0;
+  virtual void finalizeLinearSystem() = 0;
+  virtual void zeroSystem() = 0;
+  virtual CoeffApplier* get_coeff_applier() = 0;
+  virtual void sumInto(unsigned numEntities, const int32_t* entities,
+                       const double* rhs, const double* lhs) = 0;
+  virtual void applyDirichletBCs(int solutionField, int bcValuesField,
+                                 const std::vector<int32_t>& nodes,
+                                 unsigned beginPos, unsigned endPos) = 0;
+  virtual void resetRows(const std::vector<int32_t>& nodeList,
+                         unsigned beginPos, unsigned endPos,
+                         double diag_value = 0.0,
+                         double rhs_residual = 0.0) = 0;
+  virtual void loadComplete() = 0;
+  virtual int solve(const nw_solver_cfg& cfg) = 0;
+  int linearSolveIterations() const { return iters_; }
+  double linearResidual() const { return linearResidual_; }
+  double nonLinearResidual() const { return nonLinearResidual_; }
+  unsigned numDof() const { return numDof_; }
+
+ protected:
+  unsigned numDof_ = 1;
+  int iters_ = 0;
+  double linearResidual_ = 0.0, nonLinearResidual_ = 0.0;
+};
+
+// The GPU-resident system of one Realm (TpetraLinearSystem analogue).
+class GpuLinearSystem : public LinearSystem {
+ public:
+  GpuLinearSystem(Realm& realm, unsigned numDof);
+  void buildEdgeToNodeGraph() override;
+  void finalizeLinearSystem() override {}
+  void zeroSystem() override;
+  CoeffApplier* get_coeff_applier() override { return applier_.get(); }
+  void sumInto(unsigned numEntities, const int32_t* entities,
+               const double* rhs, const double* lhs) override;
+  void applyDirichletBCs(int solutionField, int bcValuesField,
+                         const std::vector<int32_t>& nodes, unsigned beginPos,
+                         unsigned endPos) override;
+  void resetRows(const std::vector<int32_t>& nodeList, unsigned beginPos,
+                 unsigned endPos, double diag_value = 0.0,
+                 double rhs_residual = 0.0) override;
+  void loadComplete() override;
+  int solve(const nw_solver_cfg& cfg) override;
+
+  // Tpetra point-CSR view of the owned matrix and rhs (host copies).
+  void download(std::vector<int64_t>& rowOffsets, std::vector<int32_t>& cols,
+                std::vector<double>& vals, std::vector<double>& rhs) const;
+  std::vector<double> solution() const;
+  Realm& realm() { return realm_; }
+  nw_solve_stats lastStats() const { return stats_; }
+
+ private:
+  Realm& realm_;
+  std::unique_ptr<CoeffApplier> applier_;
+  nw_solve_stats stats_{};
+};
+
+class EquationSystem;
+
+// SolverAlgorithm (include/SolverAlgorithm.h:30-101)
+class SolverAlgorithm {
+ public:
+  SolverAlgorithm(Realm& realm, EquationSystem* eqSystem)
+    : realm_(realm), eqSystem_(eqSystem)
+  {
+  }
+  virtual ~SolverAlgorithm() = default;
+  virtual void execute() = 0;
+  virtual void initialize_connectivity() = 0;
+
+ protected:
+  Realm& realm_;
+  EquationSystem* eqSystem_;
+};
+
+// AssembleEdgeSolverAlgorithm: the whole edge set in one launch.
+class AssembleEdgeSolverAlgorithm : public SolverAlgorithm {
+ public:
+  using SolverAlgorithm::SolverAlgorithm;
+  void initialize_connectivity() override;
+};
+
+class ScalarEdgeSolverAlg : public AssembleEdgeSolverAlgorithm {
+ public:
+  ScalarEdgeSolverAlg(Realm& realm, EquationSystem* eqSystem,
+                      std::string dofName)
+    : AssembleEdgeSolverAlgorithm(realm, eqSystem), dofName_(std::move(dofName))
+  {
+  }
+  void execute() override;
+
+ private:
+  std::string dofName_;
+};
+
+class MomentumEdgeSolverAlg : public AssembleEdgeSolverAlgorithm {
+ public:
+  using AssembleEdgeSolverAlgorithm::AssembleEdgeSolverAlgorithm;
+  void execute() override;
+};
+
+class ContinuityEdgeSolverAlg : public AssembleEdgeSolverAlgorithm {
+ public:
+  using AssembleEdgeSolverAlgorithm::AssembleEdgeSolverAlgorithm;
+  void execute() override;
+};
+
+// MomentumEdgePecletAlg: writes the edge peclet_factor field
+class MomentumEdgePecletAlg {
+ public:
+  explicit MomentumEdgePecletAlg(Realm& realm) : realm_(realm) {}
+  void execute();
+
+ private:
+  Realm& realm_;
+};
+
+class FixPressureAtNodeAlgorithm : public SolverAlgorithm {
+ public:
+  FixPressureAtNodeAlgorithm(Realm& realm, EquationSystem* eqSystem,
+                             int32_t node, double refPressure)
+    : SolverAlgorithm(realm, eqSystem), node_(node), refPressure_(refPressure)
+  {
+  }
+  void execute() override;
+  void initialize_connectivity() override {}
+
+ private:
+  int32_t node_;
+  double refPressure_;
+};
+
+// EquationSystem::assemble_and_solve with the reference's phase timers.
+class EquationSystem {
+ public:
+  EquationSystem(Realm& realm, unsigned numDof)
+    : realm_(realm), linsys_(new GpuLinearSystem(realm, numDof))
+  {
+  }
+  void add_algorithm(std::unique_ptr<SolverAlgorithm> alg)
+  {
+    algs_.push_back(std::move(alg));
+  }
+  void initialize();
+  // zeroSystem -> algorithms -> loadComplete -> solve (when cfg is given)
+  void assemble_and_solve(const nw_solver_cfg* cfg);
+  GpuLinearSystem& linsys() { return *linsys_; }
+  double timerAssemble_ = 0, timerLoadComplete_ = 0, timerSolve_ = 0;
+
+ private:
+  Realm& realm_;
+  std::unique_ptr<GpuLinearSystem> linsys_;
+  std::vector<std::unique_ptr<SolverAlgorithm>> algs_;
+};
+
+}  // namespace nalu_amd

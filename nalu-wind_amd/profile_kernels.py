@@ -1,0 +1,83 @@
+"""Profiling driver: runs one assembly kernel a few times on a benchmark
+workload, plus a device-to-device copy of known size that calibrates the
+HBM byte counters.  Meant to run under rocprofv3, e.g.
+
+    rocprofv3 --pmc FETCH_SIZE -d OUT -o pmc -- python3 \
+        nalu-wind_amd/profile_kernels.py --kernel continuity --n 256
+
+    python3 nalu-wind_amd/profile_kernels.py --summarize OUT_FETCH OUT_WRITE
+"""
+import argparse
+import ctypes as C
+import glob
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+
+def run(kernel, n, reps, copy_mib):
+    import naluwind_amd as nw
+    import workloads as wl
+    hip = C.CDLL("libamdhip64.so.7")
+    if kernel == "momentum":
+        W = wl.momentum_abl(n)
+        ndof = 3
+        fn = lambda ctx: ctx.assemble_momentum(W.opts)
+    elif kernel == "continuity":
+        W = wl.continuity_rf(n)
+        ndof = 1
+        fn = lambda ctx: ctx.assemble_continuity(W.opts)
+    else:
+        W = wl.scalar_tg(n)
+        ndof = 1
+        fn = lambda ctx: ctx.assemble_scalar(W.opts)
+    with nw.Context(0) as ctx:
+        W.upload(ctx)
+        ctx.build_graph(ndof)
+        if kernel == "momentum":
+            ctx.momentum_edge_peclet(W.peclet)
+        for _ in range(reps):
+            fn(ctx)
+        ctx.synchronize()
+    # calibration copy: copy_mib MiB device to device
+    nbytes = copy_mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # D2D
+    assert hip.hipDeviceSynchronize() == 0
+    print("ran %s n=%d reps=%d copy=%d MiB nodes=%d edges=%d" %
+          (kernel, n, reps, copy_mib, W.n, W.ne), flush=True)
+
+
+def summarize(dirs):
+    """Average counter value (KiB) per dispatch for each kernel name."""
+    import csv
+    for d in dirs:
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"),
+                           recursive=True):
+            acc = {}
+            for r in csv.DictReader(open(f)):
+                key = (r["Kernel_Name"].split("(")[0][-60:], r["Counter_Name"])
+                acc.setdefault(key, []).append(float(r["Counter_Value"]))
+            for (k, cn), v in sorted(acc.items()):
+                print("%-62s %-12s dispatches=%4d avg=%.1f" % (k, cn, len(v),
+                                                              sum(v) / len(v)))
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--kernel", default="continuity")
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--summarize", nargs="*")
+    a = ap.parse_args()
+    if a.summarize:
+        summarize(a.summarize)
+    else:
+        run(a.kernel, a.n, a.reps, a.copy_mib)

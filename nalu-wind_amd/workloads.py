@@ -126,3 +126,26 @@ def continuity_rf(n=256, seed=1234, stretch=1.05, dt=1.0e-3, t=0.0):
 def apply_pressure_bc(ctx, W):
     """Dirichlet p = 0 on the top face (solution field = bc field = 0)."""
     ctx.apply_dirichlet(W.top, nw.F_PRESSURE, nw.F_DIFF_FLUX_COEFF, 0, 1)
+
+
+class PressureSeries:
+    """Time-evolving pressure systems of the continuity_rf workload: step k
+    assembles the continuity system for the Fourier velocity at
+    t = k * dt_flow (same mesh, same seed, phases advancing with time) and
+    applies the top-face Dirichlet condition.  The matrix is the same at
+    every step (a static mesh); only the right-hand side changes."""
+
+    def __init__(self, ctx, n, seed=1234, stretch=1.05, dt_flow=0.02):
+        self.ctx = ctx
+        self.W = continuity_rf(n, seed, stretch)
+        self.W.upload(ctx)
+        ctx.build_graph(1)
+        self.dt_flow = dt_flow
+        self.seed = seed
+
+    def step(self, k):
+        W = self.W
+        vel = fourier_velocity(W.coords, self.seed, t=k * self.dt_flow)
+        self.ctx.field_upload(nw.F_VELOCITY, vel)
+        self.ctx.assemble_continuity(W.opts)
+        apply_pressure_bc(self.ctx, W)

@@ -299,6 +299,23 @@ int orc_dense_solve(int n, double* A, double* b);
 /* Largest eigenvalue estimate of D^-1 A by power iteration. */
 double orc_jacobi_lambda_max(const orc_csr* A, int iters, uint64_t seed);
 
+/* ------------------------------------------------------------------ */
+/* Learned initial guess: message-passing network (orc_mpnn.c)         */
+/* ------------------------------------------------------------------ */
+
+int64_t orc_mpnn_nparams(int H, int L);
+/* w[nnz]: sqrt(|a_nm a_mn|)/sqrt(|a_nn a_mm|) off the diagonal, 0 on it */
+void orc_mpnn_slot_weights(const orc_csr* A, double* w);
+/* feat[N][5]; scales = {s_b, s_x, s_r, s_e} */
+void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
+                       double* feat, double* scales);
+void orc_mpnn_forward(const double* P, int H, int L, const orc_csr* A,
+                      const double* w, const double* feat, double* delta,
+                      double* acts);
+double orc_mpnn_loss_grad(const double* P, int H, int L, const orc_csr* A,
+                          const double* w, const double* feat,
+                          const double* target, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

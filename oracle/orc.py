@@ -223,6 +223,20 @@ class CsrSystem:
                                           C.byref(self.c))
         assert rc == 0
 
+    @classmethod
+    def from_dense(cls, D, rhs):
+        """1-dof system whose graph is the symmetric nonzero pattern of D."""
+        D = f64(D)
+        iu, ju = np.nonzero(np.triu((D != 0) | (D.T != 0), 1))
+        edges = np.stack([iu, ju], axis=1).astype(np.int32)
+        A = cls(D.shape[0], edges, 1)
+        rp, cols, vals = A.row_ptr, A.cols, A.vals
+        for r in range(D.shape[0]):
+            for k in range(rp[r], rp[r + 1]):
+                vals[k] = D[r, cols[k]]
+        A.rhs[:] = f64(rhs)
+        return A
+
     def __del__(self):
         try:
             lib.orc_csr_free(C.byref(self.c))
@@ -360,6 +374,47 @@ def assemble_continuity(opts, nnode, edges, coords, vel, dpdx, rho, p, udiag,
                                 csr.ref if csr is not None else None, _ptr(ld),
                                 _ptr(rd))
     return csr if csr is not None else (ld, rd)
+
+
+lib.orc_mpnn_nparams.argtypes = [C.c_int, C.c_int]
+lib.orc_mpnn_nparams.restype = C.c_int64
+lib.orc_mpnn_slot_weights.argtypes = [C.POINTER(Csr), _f64p]
+lib.orc_mpnn_features.argtypes = [C.POINTER(Csr), _f64p, _f64p, _f64p, _f64p]
+lib.orc_mpnn_forward.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr), _f64p,
+                                 _f64p, _f64p, C.c_void_p]
+lib.orc_mpnn_loss_grad.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr),
+                                   _f64p, _f64p, _f64p, _f64p]
+lib.orc_mpnn_loss_grad.restype = C.c_double
+
+
+def mpnn_nparams(H, L):
+    return lib.orc_mpnn_nparams(H, L)
+
+
+def mpnn_slot_weights(A):
+    w = np.zeros(A.c.nnz)
+    lib.orc_mpnn_slot_weights(A.ref, w)
+    return w
+
+
+def mpnn_features(A, b, x0):
+    feat = np.zeros((A.c.nrows, 5))
+    sc = np.zeros(4)
+    lib.orc_mpnn_features(A.ref, f64(b), f64(x0), feat, sc)
+    return feat, sc
+
+
+def mpnn_forward(P, H, L, A, w, feat):
+    d = np.zeros(A.c.nrows)
+    lib.orc_mpnn_forward(f64(P), H, L, A.ref, f64(w), f64(feat), d, None)
+    return d
+
+
+def mpnn_loss_grad(P, H, L, A, w, feat, target):
+    g = np.zeros(len(P))
+    loss = lib.orc_mpnn_loss_grad(f64(P), H, L, A.ref, f64(w), f64(feat),
+                                  f64(target), g)
+    return loss, g
 
 
 def peclet_classic(hf, pe):

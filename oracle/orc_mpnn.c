@@ -1,0 +1,256 @@
+/*
+ * orc_mpnn.c -- FP64 oracle of the learned initial guess for the pressure
+ * solve: a message-passing network on the matrix graph.  Test
+ * infrastructure only; see orc.h.  The reference has no such component
+ * (it is an addition of this project); this file is the definition the HIP
+ * kernels are checked against, and its analytic gradient is pinned by
+ * finite differences in tests/test_oracle.py.
+ *
+ * Network (F = 5 inputs, H hidden, L message-passing layers):
+ *   h0      = relu(W_in f + b_in)
+ *   m_l     = sum_{m in N(n)} w_nm h_{l-1,m}
+ *   h_l     = relu(W_self,l h_{l-1} + W_nbr,l m_l + b_l),   l = 1..L
+ *   delta_n = w_out . h_L + b_out
+ * Edge weights w_nm = sqrt(|a_nm a_mn|) / sqrt(|a_nn a_mm|) (symmetric).
+ * Node features: b/s_b, x0/s_x, r/s_r (r = b - A x0), log|a_nn| - mean,
+ * Dirichlet flag; prediction x = x0 + s_e delta with s_e = rms(r / diag).
+ * Parameters, flat: W_in[H][F], b_in[H], then per layer W_self[H][H],
+ * W_nbr[H][H], b[H], then w_out[H], b_out.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "orc.h"
+
+#define NF 5
+
+int64_t orc_mpnn_nparams(int H, int L)
+{
+  return (int64_t)H * NF + H + (int64_t)L * (2 * H * H + H) + H + 1;
+}
+
+static double rms(const double* v, int64_t n)
+{
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += v[i] * v[i];
+  s = sqrt(s / (double)(n > 0 ? n : 1));
+  return s > 0 ? s : 1.0;
+}
+
+static double entry(const orc_csr* A, int64_t r, int32_t c)
+{
+  for (int64_t k = A->row_ptr[r]; k < A->row_ptr[r + 1]; ++k)
+    if (A->cols[k] == c) return A->vals[k];
+  return 0.0;
+}
+
+void orc_mpnn_slot_weights(const orc_csr* A, double* w)
+{
+  for (int64_t r = 0; r < A->nrows; ++r) {
+    const double dr = fabs(entry(A, r, (int32_t)r));
+    for (int64_t k = A->row_ptr[r]; k < A->row_ptr[r + 1]; ++k) {
+      const int32_t c = A->cols[k];
+      if (c == r) {
+        w[k] = 0.0;
+        continue;
+      }
+      const double dc = fabs(entry(A, c, c));
+      const double amn = fabs(A->vals[k]) * fabs(entry(A, c, (int32_t)r));
+      w[k] = (dr > 0 && dc > 0) ? sqrt(amn) / sqrt(dr * dc) : 0.0;
+    }
+  }
+}
+
+void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
+                       double* feat, double* scales)
+{
+  const int64_t n = A->nrows;
+  double* r = (double*)malloc(sizeof(double) * (size_t)n);
+  double* e = (double*)malloc(sizeof(double) * (size_t)n);
+  orc_spmv(A, x0, r);
+  double mlog = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    r[i] = b[i] - r[i];
+    const double d = fabs(entry(A, i, (int32_t)i));
+    e[i] = d > 0 ? r[i] / d : 0.0;
+    mlog += log(d > 0 ? d : 1.0);
+  }
+  mlog /= (double)(n > 0 ? n : 1);
+  const double sb = rms(b, n), sx = rms(x0, n), sr = rms(r, n), se = rms(e, n);
+  for (int64_t i = 0; i < n; ++i) {
+    const double d = fabs(entry(A, i, (int32_t)i));
+    int dir = 1;
+    for (int64_t k = A->row_ptr[i]; k < A->row_ptr[i + 1]; ++k)
+      if (A->cols[k] != i && A->vals[k] != 0.0) dir = 0;
+    feat[NF * i + 0] = b[i] / sb;
+    feat[NF * i + 1] = x0[i] / sx;
+    feat[NF * i + 2] = r[i] / sr;
+    feat[NF * i + 3] = log(d > 0 ? d : 1.0) - mlog;
+    feat[NF * i + 4] = (double)dir;
+  }
+  scales[0] = sb;
+  scales[1] = sx;
+  scales[2] = sr;
+  scales[3] = se;
+  free(r);
+  free(e);
+}
+
+static void aggregate(const orc_csr* A, const double* w, int H,
+                      const double* h, double* m)
+{
+  for (int64_t i = 0; i < A->nrows; ++i) {
+    for (int f = 0; f < H; ++f) m[H * i + f] = 0.0;
+    for (int64_t k = A->row_ptr[i]; k < A->row_ptr[i + 1]; ++k) {
+      const double wk = w[k];
+      if (wk == 0.0) continue;
+      const int32_t c = A->cols[k];
+      for (int f = 0; f < H; ++f) m[H * i + f] += wk * h[(int64_t)H * c + f];
+    }
+  }
+}
+
+/* Forward; if acts != NULL it receives z_0..z_L and m_1..m_L for the
+ * backward: acts = [z0 (N*H)] [m_l, z_l for l = 1..L] */
+void orc_mpnn_forward(const double* P, int H, int L, const orc_csr* A,
+                      const double* w, const double* feat, double* delta,
+                      double* acts)
+{
+  const int64_t n = A->nrows;
+  const double* Win = P;
+  const double* bin = Win + (int64_t)H * NF;
+  double* h = (double*)malloc(sizeof(double) * (size_t)(n * H));
+  double* h2 = (double*)malloc(sizeof(double) * (size_t)(n * H));
+  double* m = (double*)malloc(sizeof(double) * (size_t)(n * H));
+  for (int64_t i = 0; i < n; ++i)
+    for (int f = 0; f < H; ++f) {
+      double z = bin[f];
+      for (int k = 0; k < NF; ++k) z += Win[f * NF + k] * feat[NF * i + k];
+      if (acts) acts[H * i + f] = z;
+      h[H * i + f] = z > 0 ? z : 0.0;
+    }
+  const double* Pl = bin + H;
+  for (int l = 0; l < L; ++l) {
+    const double* Ws = Pl;
+    const double* Wn = Ws + H * H;
+    const double* bl = Wn + H * H;
+    aggregate(A, w, H, h, m);
+    double* ml = acts ? acts + n * H * (1 + 2 * l) : NULL;
+    double* zl = acts ? acts + n * H * (2 + 2 * l) : NULL;
+    if (ml) memcpy(ml, m, sizeof(double) * (size_t)(n * H));
+    for (int64_t i = 0; i < n; ++i)
+      for (int f = 0; f < H; ++f) {
+        double z = bl[f];
+        for (int k = 0; k < H; ++k)
+          z += Ws[f * H + k] * h[H * i + k] + Wn[f * H + k] * m[H * i + k];
+        if (zl) zl[H * i + f] = z;
+        h2[H * i + f] = z > 0 ? z : 0.0;
+      }
+    double* t = h;
+    h = h2;
+    h2 = t;
+    Pl = bl + H;
+  }
+  const double* wout = Pl;
+  const double bout = wout[H];
+  for (int64_t i = 0; i < n; ++i) {
+    double s = bout;
+    for (int f = 0; f < H; ++f) s += wout[f] * h[H * i + f];
+    delta[i] = s;
+  }
+  free(h);
+  free(h2);
+  free(m);
+}
+
+/* Loss = mean((delta - t)^2); grad (nparams) receives dLoss/dP. */
+double orc_mpnn_loss_grad(const double* P, int H, int L, const orc_csr* A,
+                          const double* w, const double* feat,
+                          const double* target, double* grad)
+{
+  const int64_t n = A->nrows;
+  const int64_t np = orc_mpnn_nparams(H, L);
+  double* acts = (double*)malloc(sizeof(double) * (size_t)(n * H * (1 + 2 * L)));
+  double* delta = (double*)malloc(sizeof(double) * (size_t)n);
+  orc_mpnn_forward(P, H, L, A, w, feat, delta, acts);
+  memset(grad, 0, sizeof(double) * (size_t)np);
+  double loss = 0.0;
+  double* dd = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    const double e = delta[i] - target[i];
+    loss += e * e;
+    dd[i] = 2.0 * e / (double)n;
+  }
+  loss /= (double)n;
+  /* locate parameter blocks */
+  const double* Win = P;
+  const double* bin = Win + (int64_t)H * NF;
+  double* gWin = grad;
+  double* gbin = gWin + (int64_t)H * NF;
+  const double* wout = bin + H + (int64_t)L * (2 * H * H + H);
+  double* gwout = gbin + H + (int64_t)L * (2 * H * H + H);
+  /* h_L = relu(z_L) */
+  double* g = (double*)malloc(sizeof(double) * (size_t)(n * H));   /* dL/dh */
+  double* gz = (double*)malloc(sizeof(double) * (size_t)(n * H));
+  double* q = (double*)malloc(sizeof(double) * (size_t)(n * H));
+  double* gm = (double*)malloc(sizeof(double) * (size_t)(n * H));
+  const double* zL = acts + n * H * (2 * L);
+  for (int64_t i = 0; i < n; ++i) {
+    gwout[H] += dd[i];
+    for (int f = 0; f < H; ++f) {
+      const double hf = zL[H * i + f] > 0 ? zL[H * i + f] : 0.0;
+      gwout[f] += dd[i] * hf;
+      g[H * i + f] = dd[i] * wout[f];
+    }
+  }
+  for (int l = L - 1; l >= 0; --l) {
+    const double* Ws = bin + H + (int64_t)l * (2 * H * H + H);
+    const double* Wn = Ws + H * H;
+    double* gWs = gbin + H + (int64_t)l * (2 * H * H + H);
+    double* gWn = gWs + H * H;
+    double* gbl = gWn + H * H;
+    const double* zl = acts + n * H * (2 + 2 * l);
+    const double* ml = acts + n * H * (1 + 2 * l);
+    const double* zprev = acts + (l == 0 ? 0 : n * H * (2 * l));
+    for (int64_t i = 0; i < n * H; ++i) gz[i] = zl[i] > 0 ? g[i] : 0.0;
+    for (int64_t i = 0; i < n; ++i)
+      for (int f = 0; f < H; ++f) {
+        const double gzf = gz[H * i + f];
+        gbl[f] += gzf;
+        for (int k = 0; k < H; ++k) {
+          const double hp = zprev[H * i + k] > 0 ? zprev[H * i + k] : 0.0;
+          gWs[f * H + k] += gzf * hp;
+          gWn[f * H + k] += gzf * ml[H * i + k];
+        }
+      }
+    /* dL/dh_{l-1} = Ws^T gz + aggregate(Wn^T gz) (weights are symmetric) */
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < H; ++k) {
+        double a = 0.0, b = 0.0;
+        for (int f = 0; f < H; ++f) {
+          a += Ws[f * H + k] * gz[H * i + f];
+          b += Wn[f * H + k] * gz[H * i + f];
+        }
+        g[H * i + k] = a;
+        q[H * i + k] = b;
+      }
+    aggregate(A, w, H, q, gm);
+    for (int64_t i = 0; i < n * H; ++i) g[i] += gm[i];
+  }
+  /* input layer */
+  for (int64_t i = 0; i < n; ++i)
+    for (int f = 0; f < H; ++f) {
+      const double gzf = acts[H * i + f] > 0 ? g[H * i + f] : 0.0;
+      gbin[f] += gzf;
+      for (int k = 0; k < NF; ++k) gWin[f * NF + k] += gzf * feat[NF * i + k];
+    }
+  free(acts);
+  free(delta);
+  free(dd);
+  free(g);
+  free(gz);
+  free(q);
+  free(gm);
+  return loss;
+}

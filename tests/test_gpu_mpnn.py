@@ -1,0 +1,191 @@
+"""Learned initial guess of the pressure solve through the C ABI: the GPU
+forward and backward of the message-passing network against the FP64
+oracle (whose analytic gradient is pinned by finite differences in
+test_oracle.py), training determinism, the fallback to the previous-step
+guess, and the gain over the previous-step guess on a time-evolving flow."""
+import numpy as np
+import pytest
+
+from oracle import orc
+from tests import problems as pb
+
+pytestmark = pytest.mark.gpu
+
+H, L = 64, 3
+
+
+@pytest.fixture(scope="module")
+def nw():
+    return pb.nw()
+
+
+@pytest.fixture
+def ctx(nw):
+    c = nw.Context(0)
+    yield c
+    c.close()
+
+
+def _system(nw, ctx, nx=7, seed=4):
+    P = pb.Problem(nx, nx - 1, nx + 1, stretch=1.1, jitter=0.15, seed=seed)
+    P.upload(ctx)
+    ctx.build_graph(1)
+    ctx.assemble_continuity(nw.continuity_defaults())
+    top = np.where(np.abs(P.coords[:, 2] - 1.0) < 1e-12)[0].astype(np.int32)
+    ctx.field_upload(nw.F_PRESSURE, np.zeros(P.n))
+    ctx.field_upload(nw.F_DIFF_FLUX_COEFF, np.zeros(P.n))
+    ctx.apply_dirichlet(top, nw.F_PRESSURE, nw.F_DIFF_FLUX_COEFF, 0, 1)
+    rp, cols, vals, rhs = ctx.download()
+    D = np.zeros((P.n, P.n))
+    for r in range(P.n):
+        D[r, cols[rp[r]:rp[r + 1]]] = vals[rp[r]:rp[r + 1]]
+    return P, orc.CsrSystem.from_dense(D, rhs), top
+
+
+def _params(nw, seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(size=nw.lib.nw_mpnn_nparams(H, L)) * scale
+    return p.astype(np.float32).astype(np.float64)  # exactly representable
+
+
+def _solved_pair(nw, ctx, A, seed):
+    """x0 random, x the FP64 solution from it; the solve leaves x0 as the
+    recorded guess."""
+    x0 = np.random.default_rng(seed).normal(size=A.c.nrows)
+    ctx.set_solution(x0)
+    st = ctx.solve(nw.solver_cfg(kspace=30, max_iters=4000, tol=1e-12,
+                                 x0=nw.X0_SOLUTION))
+    assert st.converged
+    return x0, ctx.solution()
+
+
+def test_gradient_matches_oracle(nw, ctx):
+    P, A, top = _system(nw, ctx)
+    x0, xs = _solved_pair(nw, ctx, A, 11)
+    net = nw.Mpnn(ctx, H, L, seed=3)
+    p = _params(nw, 5)
+    net.params = p
+    net.record()
+    assert net.samples == 1
+    loss, g = net.loss_grad(0)
+    w = orc.mpnn_slot_weights(A)
+    feat, sc = orc.mpnn_features(A, A.rhs, x0)
+    target = (xs - x0) / sc[3]
+    loss_o, g_o = orc.mpnn_loss_grad(p, H, L, A, w, feat, target)
+    # FP32 device arithmetic against the FP64 oracle
+    assert loss == pytest.approx(loss_o, rel=2e-5)
+    assert np.linalg.norm(g - g_o) <= 2e-5 * np.linalg.norm(g_o)
+    np.testing.assert_allclose(g, g_o, rtol=0, atol=2e-5 * np.abs(g_o).max())
+    # every parameter block receives gradient
+    o = 0
+    for size in [H * 5, H] + [H * H, H * H, H] * L + [H, 1]:
+        assert np.abs(g_o[o:o + size]).max() > 0
+        o += size
+    # the same call again gives identical bits (fixed-order reductions)
+    loss2, g2 = net.loss_grad(0)
+    assert loss2 == loss and np.array_equal(g, g2)
+
+
+def test_prediction_matches_oracle(nw, ctx):
+    P, A, top = _system(nw, ctx, seed=6)
+    net = nw.Mpnn(ctx, H, L, seed=3)
+    p = _params(nw, 7)
+    net.params = p
+    x0 = np.random.default_rng(8).normal(size=P.n)
+    ctx.set_solution(x0)
+    x, se = net.predict()
+    w = orc.mpnn_slot_weights(A)
+    feat, sc = orc.mpnn_features(A, A.rhs, x0)
+    d = orc.mpnn_forward(p, H, L, A, w, feat)
+    assert se == pytest.approx(sc[3], rel=1e-12)
+    np.testing.assert_allclose((x - x0) / se, d, rtol=0,
+                               atol=2e-5 * np.abs(d).max())
+
+
+def test_training_is_deterministic_and_lowers_loss(nw, ctx):
+    P, A, top = _system(nw, ctx, seed=9)
+    nets = []
+    for rep in range(2):
+        net = nw.Mpnn(ctx, H, L, seed=21)
+        for s in range(3):
+            _solved_pair(nw, ctx, A, 100 + s)
+            net.record()
+        l0 = np.mean([net.loss_grad(s)[0] for s in range(3)])
+        net.train(nw.mpnn_train_cfg(epochs=20, batch=2, lr=2e-3, seed=5))
+        l1 = np.mean([net.loss_grad(s)[0] for s in range(3)])
+        assert l1 < 0.5 * l0, (l0, l1)
+        nets.append(net.params)
+        net.close()
+    assert np.array_equal(nets[0], nets[1])
+
+
+def test_bad_network_falls_back_to_previous_guess(nw, ctx):
+    P, A, top = _system(nw, ctx, seed=12)
+    net = nw.Mpnn(ctx, H, L, seed=1)
+    p = np.zeros(net.nparams)
+    p[-1] = 1e6  # b_out: a huge constant correction
+    net.params = p
+    ctx.set_learned_guess(net)
+    x0 = np.random.default_rng(3).normal(size=P.n)
+    cfg = dict(kspace=30, max_iters=4000, tol=1e-10)
+    ctx.set_solution(x0)
+    s_prev = ctx.solve(nw.solver_cfg(x0=nw.X0_SOLUTION, **cfg))
+    x_prev = ctx.solution()
+    ctx.set_solution(x0)
+    s_l = ctx.solve(nw.solver_cfg(x0=nw.X0_LEARNED, **cfg))
+    assert s_l.guess == nw.X0_SOLUTION and s_prev.guess == nw.X0_SOLUTION
+    assert s_l.iters == s_prev.iters
+    assert s_l.x0_rel == s_prev.x0_rel
+    np.testing.assert_array_equal(ctx.solution(), x_prev)
+
+
+def test_learned_guess_needs_a_network(nw, ctx):
+    P, A, top = _system(nw, ctx, seed=13)
+    with pytest.raises(nw.NwError) as e:
+        ctx.solve(nw.solver_cfg(x0=nw.X0_LEARNED))
+    assert e.value.code == nw.E_STATE
+    with pytest.raises(nw.NwError):
+        nw.Mpnn(ctx, 48, 3)
+    net = nw.Mpnn(ctx, H, L)
+    with pytest.raises(nw.NwError):
+        net.record()  # no solve has run on this system
+    with pytest.raises(nw.NwError):
+        net.params = np.zeros(net.nparams - 1)
+
+
+def test_learned_guess_beats_previous_step_guess(nw, ctx):
+    """Train on solves of steps 0..29 of a time-evolving flow, then compare
+    steps 30..37 started from the previous-step solution with and without
+    the network: same tolerance, fewer iterations and a smaller starting
+    residual with it."""
+    from workloads import PressureSeries
+    S = PressureSeries(ctx, 16, seed=77, stretch=1.05, dt_flow=0.02)
+    cfg = dict(kspace=50, max_iters=20000, tol=1e-8, precond=nw.PREC_JACOBI)
+    net = nw.Mpnn(ctx, H, L, seed=2)
+    ctx.set_solution(np.zeros(S.W.n))
+    for k in range(30):
+        S.step(k)
+        ctx.solve(nw.solver_cfg(x0=nw.X0_SOLUTION, **cfg))
+        if k > 0:
+            net.record()
+    loss = net.train(nw.mpnn_train_cfg(epochs=40, batch=1, lr=1e-3, seed=1))
+    ctx.set_learned_guess(net)
+    it_prev, it_l, r_prev, r_l = [], [], [], []
+    for k in range(30, 38):
+        x_before = ctx.solution()
+        S.step(k)
+        s_p = ctx.solve(nw.solver_cfg(x0=nw.X0_SOLUTION, **cfg))
+        x_p = ctx.solution()
+        ctx.set_solution(x_before)
+        s_l = ctx.solve(nw.solver_cfg(x0=nw.X0_LEARNED, **cfg))
+        assert s_p.converged and s_l.converged and s_l.final_rel <= 1e-8
+        np.testing.assert_allclose(ctx.solution(), x_p, rtol=0,
+                                   atol=1e-6 * np.abs(x_p).max())
+        it_prev.append(s_p.iters)
+        it_l.append(s_l.iters)
+        r_prev.append(s_p.x0_rel)
+        r_l.append(s_l.x0_rel)
+    print("loss", loss, "iters prev", it_prev, "learned", it_l,
+          "x0_rel prev", np.round(r_prev, 5), "learned", np.round(r_l, 5))
+    assert np.mean(r_l) < 0.7 * np.mean(r_prev)
+    assert sum(it_l) < sum(it_prev)

@@ -351,3 +351,97 @@ def test_gmres_matches_dense_solve():
         assert st.final_abs <= 1e-10 * st.bnorm * 1.0001
         np.testing.assert_allclose(x, xd, rtol=0,
                                    atol=1e-7 * np.abs(xd).max())
+
+
+def _mpnn_system(seed=3):
+    coords, conn, edges, areav, dv = _skewed_box(3)
+    n = coords.shape[0]
+    rng = np.random.default_rng(seed)
+    A = orc.CsrSystem(n, edges, 1)
+    orc.assemble_continuity(CONT_OPTS, n, edges, coords, rng.normal(size=(n, 3)),
+                            np.zeros((n, 3)), np.ones(n), np.zeros(n),
+                            np.ones(n), areav, csr=A)
+    top = np.where(np.abs(coords[:, 2] - 1.0) < 1e-12)[0].astype(np.int32)
+    A.apply_dirichlet(top, 0, 1, np.zeros(n), np.zeros(n))
+    return A, top, rng
+
+
+def test_mpnn_gradient_matches_finite_differences():
+    """The analytic backward of the learned-guess network agrees with
+    central differences on every parameter block."""
+    A, top, rng = _mpnn_system()
+    H, L = 6, 2
+    npar = orc.mpnn_nparams(H, L)
+    assert npar == H * 5 + H + L * (2 * H * H + H) + H + 1
+    P = rng.normal(size=npar) * 0.4
+    w = orc.mpnn_slot_weights(A)
+    feat, _ = orc.mpnn_features(A, A.rhs, rng.normal(size=A.c.nrows))
+    t = rng.normal(size=A.c.nrows)
+    loss, g = orc.mpnn_loss_grad(P, H, L, A, w, feat, t)
+    d = orc.mpnn_forward(P, H, L, A, w, feat)
+    assert loss == pytest.approx(np.mean((d - t) ** 2), rel=1e-14)
+    eps = 1e-6
+    # every parameter of a small network
+    fd = np.empty(npar)
+    for i in range(npar):
+        p1 = P.copy()
+        p1[i] += eps
+        p2 = P.copy()
+        p2[i] -= eps
+        fd[i] = (orc.mpnn_loss_grad(p1, H, L, A, w, feat, t)[0] -
+                 orc.mpnn_loss_grad(p2, H, L, A, w, feat, t)[0]) / (2 * eps)
+    np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-8)
+
+
+def test_mpnn_permutation_equivariance():
+    """Relabelling the nodes permutes the prediction (the aggregation uses
+    the right neighbour indices)."""
+    A, top, rng = _mpnn_system(4)
+    n = A.c.nrows
+    H, L = 5, 3
+    P = rng.normal(size=orc.mpnn_nparams(H, L)) * 0.5
+    D = A.dense()
+    x0 = rng.normal(size=n)
+    w = orc.mpnn_slot_weights(A)
+    feat, _ = orc.mpnn_features(A, A.rhs, x0)
+    d = orc.mpnn_forward(P, H, L, A, w, feat)
+    perm = rng.permutation(n)
+    Dp = D[np.ix_(perm, perm)]
+    B = orc.CsrSystem.from_dense(Dp, A.rhs[perm])
+    wp = orc.mpnn_slot_weights(B)
+    featp, _ = orc.mpnn_features(B, A.rhs[perm], x0[perm])
+    dp = orc.mpnn_forward(P, H, L, B, wp, featp)
+    np.testing.assert_allclose(dp, d[perm], rtol=1e-12, atol=1e-12)
+
+
+def test_mpnn_slot_weights_and_features():
+    """Edge weights are symmetric, zero on the diagonal and on Dirichlet
+    rows; features follow their closed forms."""
+    A, top, rng = _mpnn_system(5)
+    n = A.c.nrows
+    D = A.dense()
+    w = orc.mpnn_slot_weights(A)
+    rp, cols = A.row_ptr, A.cols
+    W = np.zeros((n, n))
+    for r in range(n):
+        for k in range(rp[r], rp[r + 1]):
+            W[r, cols[k]] = w[k]
+    np.testing.assert_allclose(W, W.T, rtol=1e-15, atol=0)
+    dg = np.abs(np.diag(D))
+    ref = np.sqrt(np.abs(D * D.T)) / np.sqrt(np.outer(dg, dg))
+    np.fill_diagonal(ref, 0.0)
+    np.testing.assert_allclose(W, ref, rtol=1e-14, atol=1e-300)
+    assert np.all(W[top] == 0) and np.all(W[:, top] == 0)
+    x0 = rng.normal(size=n)
+    b = A.rhs.copy()
+    feat, sc = orc.mpnn_features(A, b, x0)
+    r = b - D @ x0
+    rms = lambda v: np.sqrt(np.mean(v * v))
+    np.testing.assert_allclose(sc, [rms(b), rms(x0), rms(r), rms(r / dg)],
+                               rtol=1e-13)
+    np.testing.assert_allclose(feat[:, 2], r / rms(r), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(feat[:, 3], np.log(dg) - np.log(dg).mean(),
+                               rtol=1e-12, atol=1e-14)
+    flag = np.zeros(n)
+    flag[top] = 1
+    np.testing.assert_array_equal(feat[:, 4], flag)
