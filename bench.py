@@ -99,9 +99,10 @@ def time_launches(hip, ctx, fn, steps, warmup):
 
 
 def cpu_momentum_baseline(seconds=12.0, n=48):
-    """The oracle (single-threaded C restatement of MomentumEdgeSolverAlg +
-    sum_into) on a bounded sample: a 48^3 box of the same configuration,
-    repeated for about `seconds`."""
+    """The oracle (C restatement of MomentumEdgeSolverAlg + sum_into) on a
+    bounded sample: a 48^3 box of the same configuration, repeated for
+    about `seconds`, with OpenMP over edges and atomic scatter adds on the
+    host cores (the reference's OpenMP execution), plus a 1-thread run."""
     sys.path.insert(0, ROOT)
     from oracle import orc
     import workloads as wl
@@ -113,21 +114,125 @@ def cpu_momentum_baseline(seconds=12.0, n=48):
                                       orc.Peclet(0, 1.0, 2.0, 4.0))
     o = orc.MomentumOpts(0.5, 1.0, 1.0, 1.0, 0.0, 0)
     A = orc.CsrSystem(W.n, W.edges, 3)
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        orc.assemble_momentum(o, W.n, W.edges, W.coords, f[nw.F_VELOCITY],
-                              f[nw.F_DUDX], f[nw.F_VISCOSITY], f[nw.F_DENSITY],
-                              W.areav, f[nw.E_MASS_FLOW_RATE], pecfac,
-                              f[nw.F_WALL_MASK], csr=A)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > seconds:
-            break
-    return {"value": round(reps * W.ne / el / 1e6, 3), "unit": "Medges/s",
-            "cores": 1, "kind": "port",
-            "sample": "oracle momentum assembly (C, serial, CSR sum_into), "
-                      "%d^3-cell box of the same configuration (%d edges) x %d "
-                      "repetitions in %.1f s" % (n, W.ne, reps, el)}
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+
+    def rate(threads, secs):
+        orc.set_threads(threads)
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            orc.assemble_momentum(o, W.n, W.edges, W.coords, f[nw.F_VELOCITY],
+                                  f[nw.F_DUDX], f[nw.F_VISCOSITY], f[nw.F_DENSITY],
+                                  W.areav, f[nw.E_MASS_FLOW_RATE], pecfac,
+                                  f[nw.F_WALL_MASK], csr=A)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > secs:
+                return reps * W.ne / el / 1e6, reps, el
+
+    serial, _, _ = rate(1, 0.3 * seconds)
+    par, reps, el = rate(cores, 0.7 * seconds)
+    orc.set_threads(1)
+    return {"value": round(par, 3), "unit": "Medges/s", "cores": cores,
+            "kind": "port", "serial_value": round(serial, 3),
+            "sample": "oracle momentum assembly (C restatement, CSR sum_into with "
+                      "atomic adds, OpenMP over edges on %d threads), %d^3-cell box "
+                      "of the same configuration (%d edges) x %d repetitions in "
+                      "%.1f s; serial_value = the same on 1 thread" %
+                      (cores, n, W.ne, reps, el)}
+
+
+def pressure_solves(ns, ntrain, neval, epochs, log):
+    """Pressure-Poisson solves on a time-evolving flow (PressureSeries): the
+    plain FP64 method from x0 = 0 (the reference's choice), the
+    FP32-inner / FP64-refined method, and the refined method started from
+    the learned guess.  Steps 1..ntrain are plain FP64 solves from the
+    previous-step solution whose (features, solution) pairs train the
+    network; steps ntrain+1.. compare the methods from the same state."""
+    import numpy as np
+    import naluwind_amd as nw
+    import workloads as wl
+    res = {}
+    with nw.Context(0) as ctx:
+        t0 = time.perf_counter()
+        S = wl.PressureSeries(ctx, ns, seed=1234, dt_flow=0.02)
+        W = S.W
+        res["workload"] = (
+            "pressure Poisson of the continuity assembly, %d^3-cell box with the "
+            "z grading of the 128^3 box (per-cell ratio %.5f), random-Fourier "
+            "velocity advancing 0.02 time units per step, Dirichlet p=0 on z=1, "
+            "tol 1e-8 on ||b-Ax||/||b||, GMRES(50) right-preconditioned by "
+            "SA-AMG (Chebyshev(2) smoothing, drop tol 0.02)" % (ns, wl.grading_128(ns)))
+        res["nodes"] = W.n
+        log("pressure series %d^3: %d nodes (%.1fs)" % (ns, W.n, time.perf_counter() - t0))
+        base = dict(kspace=50, max_iters=2000, tol=1e-8, precond=nw.PREC_AMG,
+                    reuse_preconditioner=1)
+        fp64 = lambda x0: nw.solver_cfg(mode=nw.SOLVE_FP64, x0=x0, **base)
+        fp32 = lambda x0: nw.solver_cfg(mode=nw.SOLVE_FP32_REFINE, inner_tol=1e-5,
+                                        x0=x0, **base)
+        S.step(0)
+        st = ctx.solve(fp64(nw.X0_ZERO))
+        res["amg_setup_ms"] = round(st.setup_ms, 1)
+        res["amg_levels"] = st.precond_levels
+        net = nw.Mpnn(ctx, 64, 3, seed=7)
+        t0 = time.perf_counter()
+        for k in range(1, ntrain + 1):
+            S.step(k)
+            ctx.solve(fp64(nw.X0_SOLUTION))
+            net.record()
+        t_gen = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        loss = net.train(nw.mpnn_train_cfg(epochs=epochs, batch=1, lr=1e-3, seed=3))
+        t_train = time.perf_counter() - t0
+        res["learned_guess"] = {
+            "network": "3 message-passing layers, 64 hidden, FP32",
+            "training_pairs": ntrain, "epochs": epochs,
+            "final_training_loss": loss, "pair_generation_s": round(t_gen, 1),
+            "training_s": round(t_train, 1)}
+        log("training: %d pairs in %.1fs, %d epochs in %.1fs, loss %.4g" %
+            (ntrain, t_gen, epochs, t_train, loss))
+        ctx.set_learned_guess(net)
+        runs = {"fp64_x0_zero": [], "fp64_prev_step": [], "fp32_refined_x0_zero": [],
+                "fp32_refined_prev_step": [], "fp32_refined_learned": []}
+        for k in range(ntrain + 1, ntrain + 1 + neval):
+            x_prev = ctx.solution()
+            S.step(k)
+            # learned first: its inputs are x0 = p(k-1) and xp = p(k-2), the
+            # guess of the previous step's last solve
+            sol = {}
+            for name, cfg in [("fp32_refined_learned", fp32(nw.X0_LEARNED)),
+                              ("fp32_refined_prev_step", fp32(nw.X0_SOLUTION)),
+                              ("fp32_refined_x0_zero", fp32(nw.X0_ZERO)),
+                              ("fp64_x0_zero", fp64(nw.X0_ZERO)),
+                              ("fp64_prev_step", fp64(nw.X0_SOLUTION))]:
+                ctx.set_solution(x_prev)
+                st = ctx.solve(cfg)
+                sol[name] = ctx.solution()
+                runs[name].append(st)
+            ref = sol["fp64_x0_zero"]
+            for name in sol:
+                runs[name][-1] = (runs[name][-1], float(np.linalg.norm(sol[name] - ref) /
+                                                        np.linalg.norm(ref)))
+        for name, lst in runs.items():
+            sts = [s for s, _ in lst]
+            res[name] = {
+                "ms": round(sum(s.ms for s in sts) / len(sts), 2),
+                "iters": round(sum(s.iters for s in sts) / len(sts), 2),
+                "refinements": round(sum(s.refinements for s in sts) / len(sts), 2),
+                "final_rel_residual_max": max(s.final_rel for s in sts),
+                "x0_rel_residual": round(sum(s.x0_rel for s in sts) / len(sts), 6),
+                "rel_error_vs_fp64": max(e for _, e in lst),
+                "converged": all(bool(s.converged) for s in sts)}
+            if name == "fp32_refined_learned":
+                res[name]["learned_guess_used"] = sum(s.guess == nw.X0_LEARNED for s in sts)
+                res[name]["guess_ms"] = round(sum(s.guess_ms for s in sts) / len(sts), 2)
+            log("solve %s: %s" % (name, res[name]))
+        res["eval_steps"] = neval
+        z = res["fp64_x0_zero"]["ms"]
+        res["speedup_vs_fp64_x0_zero"] = {
+            n: round(z / res[n]["ms"], 3) for n in runs if n != "fp64_x0_zero"}
+        net.close()
+    return res
 
 
 def read_traffic(name):
@@ -148,7 +253,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="small sizes (testing)")
-    ap.add_argument("--solve-size", type=int, default=128)
+    ap.add_argument("--solve-size", type=int, default=256)
+    ap.add_argument("--train-steps", type=int, default=24)
+    ap.add_argument("--eval-steps", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     a = ap.parse_args()
@@ -270,31 +378,8 @@ def main():
     # ---------------------------------------------------------- solve
     if not a.no_solve:
         ns = 24 if a.quick else a.solve_size
-        W = wl.continuity_rf(ns)
-        res = {"workload": "pressure Poisson from the continuity assembly, %d^3 "
-                           "cells stretched 1.05, Dirichlet p=0 on z=1, tol 1e-8 "
-                           "on ||b-Ax||/||b||, GMRES(50), x0=0" % ns,
-               "nodes": W.n}
-        with nw.Context(0) as ctx:
-            W.upload(ctx)
-            ctx.build_graph(1)
-            ctx.assemble_continuity(W.opts)
-            wl.apply_pressure_bc(ctx, W)
-            for name, mode in [("fp64", nw.SOLVE_FP64),
-                               ("fp32_refined", nw.SOLVE_FP32_REFINE)]:
-                cfg = nw.solver_cfg(kspace=50, max_iters=4000, tol=1e-8,
-                                    precond=nw.PREC_CHEBYSHEV, cheb_degree=8,
-                                    cheb_ratio=30.0, mode=mode)
-                st = ctx.solve(cfg, raise_on_noconv=False)
-                res[name] = {"precond": "chebyshev(8) on D^-1 A", "ms": round(st.ms, 1),
-                             "iters": st.iters, "cycles": st.refinements,
-                             "final_rel_residual": st.final_rel,
-                             "converged": bool(st.converged)}
-                log("solve %s: %s" % (name, res[name]))
-        if res["fp64"]["converged"] and res["fp32_refined"]["converged"]:
-            res["speedup_fp32_refined_vs_fp64"] = round(res["fp64"]["ms"] / res["fp32_refined"]["ms"], 3)
-        out["pressure_solve"] = res
-        del W
+        out["pressure_solve"] = pressure_solves(ns, a.train_steps, a.eval_steps,
+                                                a.epochs, log)
 
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_momentum_baseline(3.0 if a.quick else 12.0,

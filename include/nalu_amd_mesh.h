@@ -4,7 +4,8 @@
  * edge extraction, CVFEM edge area vectors and dual volumes, the edge
  * nodal gradient and the edge mass flow rate.  These produce the inputs the
  * edge solver algorithms consume; they run multithreaded on the host.
- * All arrays are host arrays; return 0 or a negative NW_E_* code.
+ * All arrays are host arrays; return 0 or a negative NW_E_* code.  The
+ * last entry generates a synthetic velocity field on the device.
  */
 #ifndef NALU_AMD_MESH_H
 #define NALU_AMD_MESH_H
@@ -57,6 +58,18 @@ int nw_nodal_grad_edge(int64_t nnode, int64_t nedge, const int32_t* edges,
  * unit_tests/kernels/UnitTestKernelUtils.C:725-769). */
 int nw_mdot_edge(int64_t nedge, const int32_t* edges, const double* areav,
                  const double* rho, const double* vel, double* mdot);
+
+/* ---- synthetic workload input (no reference counterpart) ---------------- */
+typedef struct nw_ctx nw_ctx;
+
+/* Sets the context's NW_F_VELOCITY field, on the device, to the Fourier sum
+ * u(x) = sum_q b_q sin(k_q . x + phase_q + omega_q t) at the node
+ * coordinates (NW_F_COORDINATES must be set): k, b [nmodes][3], phase,
+ * omega [nmodes], nmodes <= 32.  Used to step the pressure workloads in
+ * time without host traffic. */
+int nw_fourier_velocity(nw_ctx* ctx, int nmodes, const double* k,
+                        const double* b, const double* phase,
+                        const double* omega, double t);
 
 #ifdef __cplusplus
 }

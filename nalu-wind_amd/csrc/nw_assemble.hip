@@ -365,10 +365,10 @@ __device__ inline int xcd_block(int nblocks_padded)
   return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
 }
 
-template <class P>
+template <class P, int PIPE>
 __global__ __launch_bounds__(kBlock) void k_assemble_rows(
   P phys, const int32_t* __restrict__ row_ptr,
-  const int32_t* __restrict__ code, const int2* __restrict__ enodes,
+  const int32_t* __restrict__ code, const int32_t* __restrict__ cols,
   const int2* __restrict__ sched, int nchunks, double* __restrict__ vals,
   double* __restrict__ rhs, int accumulate)
 {
@@ -386,27 +386,40 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
   int dslot = -1;
   const int s0 = row_ptr[row], s1 = row_ptr[row + 1];
+  // slot s: code c (diagonal < 0, else 2e + side) and column (the other
+  // node).  PIPE = 1 loads the next slot's neighbour record while the
+  // current edge is evaluated.
+  int cn = code[s0], on = cols[s0];
+  if (PIPE) {
+    if (cn >= 0) phys.load(on, ot);
+  }
   for (int s = s0; s < s1; ++s) {
-    const int c = code[s];
+    const int c = cn, o = on;
+    typename P::Node nx;
+    if (s + 1 < s1) {
+      cn = code[s + 1];
+      on = cols[s + 1];
+      if (PIPE && cn >= 0) phys.load(on, nx);
+    }
     if (c < 0) {
       dslot = s;
-      continue;
-    }
-    const int e = c >> 1, side = c & 1;
-    const int2 lr = enodes[e];
-    phys.load(side ? lr.x : lr.y, ot);
-    double off[NB], con[NV];
-    if (side)
-      phys.edge(e, 1, ot, me, off, con);
-    else
-      phys.edge(e, 0, me, ot, off, con);
-    double* vp = vals + (int64_t)s * NB;
-    if (accumulate) {
-      for (int k = 0; k < NB; ++k) vp[k] += off[k];
     } else {
-      for (int k = 0; k < NB; ++k) vp[k] = off[k];
+      if (!PIPE) phys.load(o, ot);
+      const int e = c >> 1, side = c & 1;
+      double off[NB], con[NV];
+      if (side)
+        phys.edge(e, 1, ot, me, off, con);
+      else
+        phys.edge(e, 0, me, ot, off, con);
+      double* vp = vals + (int64_t)s * NB;
+      if (accumulate) {
+        for (int k = 0; k < NB; ++k) vp[k] += off[k];
+      } else {
+        for (int k = 0; k < NB; ++k) vp[k] = off[k];
+      }
+      for (int v = 0; v < NV; ++v) acc[v] += con[v];
     }
-    for (int v = 0; v < NV; ++v) acc[v] += con[v];
+    if (PIPE) ot = nx;
   }
   if (dslot >= 0) {
     double* dp = vals + (int64_t)dslot * NB;
@@ -520,9 +533,15 @@ int launch_rows(Ctx* c, const P& phys, int accumulate)
   const int64_t wpb = kBlock / kWave;
   int64_t blocks = (g.nchunks + wpb - 1) / wpb;
   blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
-  k_assemble_rows<P><<<(unsigned)blocks, kBlock, 0, c->stream>>>(
-    phys, g.row_ptr, g.code, reinterpret_cast<const int2*>(c->edges), g.sched,
-    (int)g.nchunks, g.vals, g.rhs, accumulate);
+  const char* var = getenv("NW_ASM_PIPE");
+  if (var && var[0] == '1')
+    k_assemble_rows<P, 1><<<(unsigned)blocks, kBlock, 0, c->stream>>>(
+      phys, g.row_ptr, g.code, g.cols, g.sched, (int)g.nchunks, g.vals, g.rhs,
+      accumulate);
+  else
+    k_assemble_rows<P, 0><<<(unsigned)blocks, kBlock, 0, c->stream>>>(
+      phys, g.row_ptr, g.code, g.cols, g.sched, (int)g.nchunks, g.vals, g.rhs,
+      accumulate);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }

@@ -199,6 +199,7 @@ int nw_ctx_destroy(nw_ctx* c)
   free_krylov(c->kry);
   amg_free(c->amg);
   (void)hipFree(c->x_init);
+  (void)hipFree(c->x_hist);
   free_graph(c->g);
   for (auto& f : c->field) (void)hipFree(f);
   (void)hipFree(c->edges);
@@ -242,6 +243,8 @@ int nw_mesh_set(nw_ctx* c, int64_t nnode, int64_t nedge,
   c->amg = nullptr;
   (void)hipFree(c->x_init);
   c->x_init = nullptr;
+  (void)hipFree(c->x_hist);
+  c->x_hist = nullptr;
   free_graph(c->g);
   for (auto& f : c->field) {
     (void)hipFree(f);
@@ -349,6 +352,8 @@ int nw_linsys_build_edge_graph(nw_ctx* c, int ndof)
   c->amg = nullptr;
   (void)hipFree(c->x_init);
   c->x_init = nullptr;
+  (void)hipFree(c->x_hist);
+  c->x_hist = nullptr;
   free_graph(c->g);
   Graph& g = c->g;
   const int64_t N = c->nnode, E = c->nedge;

@@ -88,6 +88,7 @@ struct Ctx {
   Krylov* kry = nullptr;
   AmgDevice* amg = nullptr;
   double* x_init = nullptr;  // [nnode] guess the last solve started from
+  double* x_hist = nullptr;  // [nnode] guess the solve before it started from
   Mpnn* mpnn = nullptr;      // network for NW_X0_LEARNED (not owned)
 };
 
@@ -99,9 +100,10 @@ int ensure_field(Ctx* c, int f, double** out);
 int require_field(Ctx* c, int f, const double** out);
 void free_krylov(Krylov* k);
 void amg_free(AmgDevice* a);
-// x = x0 + s_e * delta predicted from the context's system and x0 (device
-// vectors of nnode doubles); wall time in mpnn_last_ms.
-int mpnn_predict_device(Mpnn* m, const double* x0, double* x);
+// x = x0 + s_o * delta predicted from the context's system, x0 and the
+// guess before it xp (device vectors of nnode doubles); wall time in
+// mpnn_last_ms.
+int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp, double* x);
 double mpnn_last_ms(const Mpnn* m);
 
 inline unsigned grid_for(int64_t n, int block) {

@@ -803,7 +803,11 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
   }
   double* x = c->g.sln;
   if (cfg->x0 == NW_X0_ZERO) NW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
-  // the guess this solve starts from, kept for recording training pairs
+  // the guess this solve starts from and the one the previous solve started
+  // from (the learned guess's inputs and recorded training pairs)
+  if (!c->x_hist) NW_HIP(hipMalloc(&c->x_hist, sizeof(double) * n));
+  NW_HIP(hipMemcpyAsync(c->x_hist, c->x_init ? c->x_init : x,
+                        sizeof(double) * n, hipMemcpyDeviceToDevice, st));
   if (!c->x_init) NW_HIP(hipMalloc(&c->x_init, sizeof(double) * n));
   NW_HIP(hipMemcpyAsync(c->x_init, x, sizeof(double) * n,
                         hipMemcpyDeviceToDevice, st));
@@ -820,7 +824,7 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
     // the network's prediction from (A, b, x0); kept only if it lowers the
     // residual, otherwise GMRES starts from x0 (the previous-step guess)
     if (!k->xg) NW_HIP(hipMalloc(&k->xg, sizeof(double) * n));
-    rc = mpnn_predict_device(c->mpnn, c->x_init, k->xg);
+    rc = mpnn_predict_device(c->mpnn, c->x_init, c->x_hist, k->xg);
     out->guess_ms = mpnn_last_ms(c->mpnn);
     double rl = 0;
     if (rc == NW_OK) rc = S.residual(k->xg, &rl);

@@ -3,12 +3,20 @@
 // A message-passing network on the graph of the assembled 1-dof matrix
 // predicts the change of the solution from the previous-step guess x0:
 //
-//   features f_n = (b/s_b, x0/s_x, r/s_r, log|a_nn| - mean, dirichlet),
-//                  r = b - A x0, s_* the RMS of each vector
+//   features f_n = (b/s_b, x0/s_x, r/s_r, log|a_nn| - mean, dirichlet,
+//                   (x0 - xp)/s_d),  r = b - A x0, xp the guess before x0
+//                   (two steps back), s_* the RMS of each vector
 //   h0      = relu(W_in f + b_in)
 //   m_l     = sum_{m != n} w_nm h_{l-1,m},  w_nm = sqrt|a_nm a_mn| / sqrt|a_nn a_mm|
 //   h_l     = relu(W_self,l h_{l-1} + W_nbr,l m_l + b_l),   l = 1..L
-//   delta_n = w_out . h_L + b_out,     x = x0 + s_e delta,  s_e = rms(r / diag)
+//   delta_n = w_out . h_L + b_out
+//   x       = xb + s_o delta,  xb = x0 + (x0 - xp) (linear extrapolation
+//             in time; xb = x0 without history), s_o = s_d, or
+//             s_e = rms(r / diag) without history
+// so the network learns the correction to the extrapolated guess (a skip
+// connection from the history feature to the output).  Rows without
+// off-diagonal entries (Dirichlet rows) are decoupled: their prediction is
+// b_n / a_nn exactly, so no boundary error enters the guess.
 //
 // The reference has no counterpart (its pressure solve starts from x0 = 0,
 // src/LinearSolver.C:209-254); the network only changes where GMRES starts,
@@ -17,7 +25,7 @@
 // back to x0, the previous-step guess.
 //
 // Training pairs are recorded from solves on the device: the features of
-// (A, b, x0) and the target (x - x0)/s_e in FP32 SoA buffers.  Training is
+// (A, b, x0, xp) and the target (x - xb)/s_o in FP32 SoA buffers.  Training is
 // mean-squared error with Adam; forward and backward run in the kernels
 // below (FP32 storage and arithmetic, FP64 reductions in a fixed order, so a
 // gradient is reproducible run to run), the Adam update on the host over the
@@ -25,7 +33,7 @@
 //
 // Layouts: node activations are node-major [n][H] float rows, so a
 // neighbour's whole row is one contiguous 256-byte read; features are SoA
-// [5][n].  Parameters are the flat vector W_in[H][5], b_in[H], per layer
+// [6][n].  Parameters are the flat vector W_in[H][6], b_in[H], per layer
 // W_self[H][H], W_nbr[H][H], b[H], then w_out[H], b_out; the forward reads a
 // transposed copy (k-major) so that the H outputs of one input are
 // consecutive and wave-uniform.
@@ -42,10 +50,10 @@ namespace nw {
 
 namespace {
 
-constexpr int kF = 5;           // node features
+constexpr int kF = 6;           // node features
 constexpr int kTileN = 32;      // nodes per staging tile (weight gradients)
 constexpr int kGradBlocks = 512;
-constexpr int kStats = 5;       // feature statistics
+constexpr int kStats = 6;       // feature statistics
 
 __device__ inline double mp_wave_sum(double v)
 {
@@ -116,14 +124,16 @@ __global__ void k_slot_weights(const int32_t* __restrict__ rp,
   }
 }
 
-// r = b - A x0 and the sums of b^2, x0^2, r^2, (r/|d|)^2, log|d|.
+// r = b - A x0 and the sums of b^2, x0^2, r^2, (r/|d|)^2, log|d|,
+// (x0 - xp)^2.
 __global__ __launch_bounds__(kBlock) void k_feat_stats(
   const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
   const double* __restrict__ va, const int32_t* __restrict__ diag,
   const double* __restrict__ b, const double* __restrict__ x0,
-  double* __restrict__ r, int n, double* __restrict__ partial)
+  const double* __restrict__ xp, double* __restrict__ r, int n,
+  double* __restrict__ partial)
 {
-  double acc[kStats] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double acc[kStats] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
     double ax = 0.0;
@@ -137,21 +147,25 @@ __global__ __launch_bounds__(kBlock) void k_feat_stats(
     acc[2] += ri * ri;
     acc[3] += e * e;
     acc[4] += log(d > 0 ? d : 1.0);
+    const double dx = x0[i] - xp[i];
+    acc[5] += dx * dx;
   }
   mp_block_store<kStats>(acc, partial + (int64_t)blockIdx.x * kStats);
 }
 
 struct FeatScale {
-  double isb, isx, isr, mlog, ise;
+  double isb, isx, isr, mlog, isd, iso;
+  double cb;  // 1 with history (xb = 2 x0 - xp), 0 without (xb = x0)
 };
 
-// Features (SoA) and, when xs is given, the target (xs - x0)/s_e.
+// Features (SoA) and, when xs is given, the target (xs - xb)/s_o.
 __global__ void k_features(const int32_t* __restrict__ rp,
                            const double* __restrict__ va,
                            const int32_t* __restrict__ ci,
                            const int32_t* __restrict__ diag,
                            const double* __restrict__ b,
                            const double* __restrict__ x0,
+                           const double* __restrict__ xp,
                            const double* __restrict__ r,
                            const double* __restrict__ xs, FeatScale sc,
                            float* __restrict__ feat, float* __restrict__ target,
@@ -168,7 +182,11 @@ __global__ void k_features(const int32_t* __restrict__ rp,
   feat[2 * (int64_t)n + i] = (float)(r[i] * sc.isr);
   feat[3 * (int64_t)n + i] = (float)(log(d > 0 ? d : 1.0) - sc.mlog);
   feat[4 * (int64_t)n + i] = (float)dir;
-  if (xs) target[i] = (float)((xs[i] - x0[i]) * sc.ise);
+  feat[5 * (int64_t)n + i] = (float)((x0[i] - xp[i]) * sc.isd);
+  if (xs) {
+    const double xb = x0[i] + sc.cb * (x0[i] - xp[i]);
+    target[i] = (float)((xs[i] - xb) * sc.iso);
+  }
 }
 
 template <int H>
@@ -265,13 +283,24 @@ __global__ void k_output(const float* __restrict__ hL,
   delta[i] = s;
 }
 
-// x = x0 + s_e delta
+// x = x0 + s_o delta
+// x = xb + s_o delta; Dirichlet rows (feature 4 set) get b / a_nn
 __global__ void k_apply_guess(const double* __restrict__ x0,
-                              const float* __restrict__ delta, double se,
+                              const double* __restrict__ xp,
+                              const float* __restrict__ delta,
+                              const float* __restrict__ dir, double so,
+                              double cb, const double* __restrict__ b,
+                              const double* __restrict__ va,
+                              const int32_t* __restrict__ diag,
                               double* __restrict__ x, int n)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] = x0[i] + se * (double)delta[i];
+  if (i >= n) return;
+  if (dir[i] != 0.f) {
+    x[i] = b[i] / va[diag[i]];
+  } else {
+    x[i] = x0[i] + cb * (x0[i] - xp[i]) + so * (double)delta[i];
+  }
 }
 
 // dd = coef (delta - target); partial[blk * nv + off] = sum (delta - t)^2
@@ -492,7 +521,8 @@ struct Mpnn {
   int nv = 0;              // values in one gradient partial row
   std::vector<int64_t> gmap;  // reduced value -> parameter index (-1: loss)
   std::vector<Sample> samples;
-  double se = 1.0;         // scale of the last prediction
+  double so = 1.0;         // output scale of the last features
+  double cb = 0.0;         // 1 when the last features had history
   double last_ms = 0.0;
 };
 
@@ -596,16 +626,16 @@ int slot_weights(Mpnn* m)
   return NW_OK;
 }
 
-// features of (A, b, x0) into feat; target from xs when given
-int features(Mpnn* m, const double* x0, const double* xs, float* feat,
-             float* target)
+// features of (A, b, x0, xp) into feat; target from xs when given
+int features(Mpnn* m, const double* x0, const double* xp, const double* xs,
+             float* feat, float* target)
 {
   Ctx* c = m->c;
   const Graph& g = c->g;
   const int n = (int)m->n;
   hipStream_t st = c->stream;
   k_feat_stats<<<kGradBlocks, kBlock, 0, st>>>(g.row_ptr, g.cols, g.vals,
-                                              g.diag, g.rhs, x0, m->r, n,
+                                              g.diag, g.rhs, x0, xp, m->r, n,
                                               m->partial);
   NW_LAUNCH_CHECK();
   k_mp_reduce<<<kStats, kBlock, 0, st>>>(m->partial, kGradBlocks, kStats, m->red);
@@ -621,14 +651,17 @@ int features(Mpnn* m, const double* x0, const double* xs, float* feat,
   sc.isb = 1.0 / rms(s[0]);
   sc.isx = 1.0 / rms(s[1]);
   sc.isr = 1.0 / rms(s[2]);
-  m->se = rms(s[3]);
-  sc.ise = 1.0 / m->se;
   sc.mlog = s[4] / (double)std::max(n, 1);
-  NW_REQUIRE(std::isfinite(s[0] + s[1] + s[2] + s[3] + s[4]), NW_E_NONFINITE,
-             "non-finite input to the learned guess");
+  sc.isd = 1.0 / rms(s[5]);
+  m->so = s[5] > 0 ? rms(s[5]) : rms(s[3]);
+  m->cb = s[5] > 0 ? 1.0 : 0.0;
+  sc.iso = 1.0 / m->so;
+  sc.cb = m->cb;
+  NW_REQUIRE(std::isfinite(s[0] + s[1] + s[2] + s[3] + s[4] + s[5]),
+             NW_E_NONFINITE, "non-finite input to the learned guess");
   k_features<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.row_ptr, g.vals, g.cols,
-                                                    g.diag, g.rhs, x0, m->r, xs,
-                                                    sc, feat, target, n);
+                                                    g.diag, g.rhs, x0, xp, m->r,
+                                                    xs, sc, feat, target, n);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }
@@ -767,9 +800,9 @@ void mpnn_detach(Ctx* c, Mpnn* m)
   if (c && c->mpnn == m) c->mpnn = nullptr;
 }
 
-// Prediction for the solver: x = x0 + s_e delta from the current system
-// and x0 (device); returns the elapsed wall time in *ms.
-int mpnn_predict_device(Mpnn* m, const double* x0, double* x)
+// Prediction for the solver: x = x0 + s_o delta from the current system,
+// x0 and the guess before it xp (device vectors).
+int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp, double* x)
 {
   const auto t0 = std::chrono::steady_clock::now();
   int rc = check_graph(m);
@@ -778,12 +811,14 @@ int mpnn_predict_device(Mpnn* m, const double* x0, double* x)
   if (rc) return rc;
   rc = slot_weights(m);
   if (rc) return rc;
-  rc = features(m, x0, nullptr, m->feat, nullptr);
+  rc = features(m, x0, xp, nullptr, m->feat, nullptr);
   if (rc) return rc;
   rc = forward(m, m->feat, false);
   if (rc) return rc;
+  const Graph& g = m->c->g;
   k_apply_guess<<<grid_for(m->n, kBlock), kBlock, 0, m->c->stream>>>(
-    x0, m->delta, m->se, x, (int)m->n);
+    x0, xp, m->delta, m->feat + 4 * m->n, m->so, m->cb, g.rhs, g.vals, g.diag,
+    x, (int)m->n);
   NW_LAUNCH_CHECK();
   NW_HIP(hipStreamSynchronize(m->c->stream));
   m->last_ms = 1e3 * std::chrono::duration<double>(
@@ -926,7 +961,7 @@ int nw_mpnn_record(nw_mpnn* m)
     set_error("device allocation of a training sample failed");
     return NW_E_ALLOC;
   }
-  rc = features(m, m->c->x_init, m->c->g.sln, s.feat, s.target);
+  rc = features(m, m->c->x_init, m->c->x_hist, m->c->g.sln, s.feat, s.target);
   if (rc) {
     free_sample(s);
     return rc;
@@ -1025,7 +1060,9 @@ int nw_mpnn_predict(nw_mpnn* m, double* x_host, double* scale)
   const int64_t n = m->n;
   double* x = nullptr;
   NW_HIP(hipMalloc(&x, sizeof(double) * n));
-  int rc = mpnn_predict_device(m, m->c->g.sln, x);
+  // x0 = the solution vector, xp = the guess the last solve started from
+  const double* xp = m->c->x_init ? m->c->x_init : m->c->g.sln;
+  int rc = mpnn_predict_device(m, m->c->g.sln, xp, x);
   if (!rc) {
     if (hipMemcpy(x_host, x, sizeof(double) * n, hipMemcpyDeviceToHost) !=
         hipSuccess) {
@@ -1034,7 +1071,7 @@ int nw_mpnn_predict(nw_mpnn* m, double* x_host, double* scale)
     }
   }
   (void)hipFree(x);
-  if (scale) *scale = m->se;
+  if (scale) *scale = m->so;
   return rc;
 }
 

@@ -177,6 +177,8 @@ _sig = {
     "nw_mpnn_train": ([_P, C.POINTER(MpnnTrainCfg), C.POINTER(C.c_double)], C.c_int),
     "nw_mpnn_predict": ([_P, _f64p, C.POINTER(C.c_double)], C.c_int),
     "nw_linsys_set_learned_guess": ([_P, _P], C.c_int),
+    "nw_fourier_velocity": ([_P, C.c_int, _f64p, _f64p, _f64p, _f64p,
+                             C.c_double], C.c_int),
     "nw_hex_box_sizes": ([C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64),
                           C.POINTER(C.c_int64)], C.c_int),
     "nw_hex_box": ([C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -420,6 +422,12 @@ class Context:
 
     def spmv_device(self, d_x, d_y):
         check(lib.nw_linsys_spmv(self.h, d_x, d_y))
+
+    def fourier_velocity(self, modes, t):
+        """Device-generated velocity field sum_q b_q sin(k_q.x + ph_q + w_q t)
+        from modes = (k [m][3], b [m][3], phase [m], omega [m])."""
+        K, B, PH, OM = (_f64(a).ravel() for a in modes)
+        check(lib.nw_fourier_velocity(self.h, PH.size, K, B, PH, OM, t))
 
     def set_learned_guess(self, net):
         check(lib.nw_linsys_set_learned_guess(self.h, net.h if net else None))

@@ -11,9 +11,11 @@ edges and geometry, and the seeded node/edge fields of one configuration:
                  jitter, log-law u(z) plus 5 % Gaussian noise, nu_t ~
                  U(0, 1e-2), mu_eff = mu + rho nu_t, du/dx from the edge
                  nodal gradient, scheme alpha = 0.5
-  continuity_rf  stretched box, divergence-free random Fourier velocity
-                 (8 modes), rho = 1, p = 0, dp/dx = 0, Udiag = gamma1/dt;
-                 Dirichlet p = 0 on the top face z = 1
+  continuity_rf  box with the z grading of the momentum box (per-cell ratio
+                 1.05^(128/n): the 128-cell geometry refined), divergence-free
+                 random Fourier velocity (8 modes), rho = 1, p = 0,
+                 dp/dx = 0, Udiag = gamma1/dt; Dirichlet p = 0 on the top
+                 face z = 1
 """
 import math
 import os
@@ -92,22 +94,42 @@ def momentum_abl(n=128, seed=1234, mu=1.8e-5):
     return W
 
 
-def fourier_velocity(coords, seed, modes=8, t=0.0):
-    """Divergence-free sum of 8 seeded Fourier modes b_m sin(k_m.x + phi_m +
-    w_m t) with b_m orthogonal to k_m."""
+def fourier_modes(seed, modes=8):
+    """Seeded modes (k [m][3], b [m][3], phase [m], omega [m]) with b_m
+    orthogonal to k_m, so that each mode is divergence-free."""
     rng = np.random.default_rng(seed)
-    u = np.zeros_like(coords)
+    K, B, PH, OM = [], [], [], []
     for _ in range(modes):
         k = 2.0 * math.pi * rng.integers(1, 4, size=3) * rng.choice([-1, 1], 3)
         b = rng.standard_normal(3)
         b -= (b @ k) / (k @ k) * k
-        ph = rng.uniform(0, 2 * math.pi)
-        om = rng.uniform(0.5, 2.0)
+        K.append(k)
+        B.append(b)
+        PH.append(rng.uniform(0, 2 * math.pi))
+        OM.append(rng.uniform(0.5, 2.0))
+    return (np.array(K, float), np.array(B, float), np.array(PH), np.array(OM))
+
+
+def fourier_velocity(coords, seed, modes=8, t=0.0):
+    """Divergence-free sum of 8 seeded Fourier modes b_m sin(k_m.x + phi_m +
+    w_m t) with b_m orthogonal to k_m."""
+    K, B, PH, OM = fourier_modes(seed, modes)
+    u = np.zeros_like(coords)
+    for k, b, ph, om in zip(K, B, PH, OM):
         u += np.sin(coords @ k + ph + om * t)[:, None] * b[None, :]
     return u
 
 
-def continuity_rf(n=256, seed=1234, stretch=1.05, dt=1.0e-3, t=0.0):
+def grading_128(n):
+    """Per-cell z ratio giving an n-cell box the grading of the 128-cell
+    momentum box (1.05 per cell, first/last spacing ratio 1.05^127): the
+    same stretched geometry at another resolution."""
+    return 1.05 ** (128.0 / n)
+
+
+def continuity_rf(n=256, seed=1234, stretch=None, dt=1.0e-3, t=0.0):
+    if stretch is None:
+        stretch = grading_128(n)
     W = Workload("continuity_rf", n, n, n, stretch, 0.0, seed)
     vel = fourier_velocity(W.coords, seed, t=t)
     W.fields = {
@@ -135,17 +157,18 @@ class PressureSeries:
     applies the top-face Dirichlet condition.  The matrix is the same at
     every step (a static mesh); only the right-hand side changes."""
 
-    def __init__(self, ctx, n, seed=1234, stretch=1.05, dt_flow=0.02):
+    def __init__(self, ctx, n, seed=1234, stretch=None, dt_flow=0.02):
         self.ctx = ctx
         self.W = continuity_rf(n, seed, stretch)
         self.W.upload(ctx)
         ctx.build_graph(1)
         self.dt_flow = dt_flow
         self.seed = seed
+        self.modes = fourier_modes(seed)
 
     def step(self, k):
-        W = self.W
-        vel = fourier_velocity(W.coords, self.seed, t=k * self.dt_flow)
-        self.ctx.field_upload(nw.F_VELOCITY, vel)
-        self.ctx.assemble_continuity(W.opts)
-        apply_pressure_bc(self.ctx, W)
+        """Velocity at t = k dt_flow (generated on the device), continuity
+        assembly, Dirichlet rows."""
+        self.ctx.fourier_velocity(self.modes, k * self.dt_flow)
+        self.ctx.assemble_continuity(self.W.opts)
+        apply_pressure_bc(self.ctx, self.W)

@@ -251,6 +251,11 @@ void orc_assemble_momentum(const orc_momentum_opts* o, int64_t nnode,
                            const double* wallmask, orc_csr* A,
                            double* lhs_dense, double* rhs_dense);
 
+/* Threads used by the orc_assemble_* loops over a CSR (default 1).  With
+ * more than one, edges are processed in parallel and the scatter adds are
+ * atomic (the reference's OpenMP execution); the sum order then varies. */
+void orc_set_threads(int n);
+
 void orc_assemble_continuity(const orc_continuity_opts* o, int64_t nnode,
                              int64_t nedge, const int32_t* edges,
                              const double* coords, const double* vel,
@@ -306,9 +311,9 @@ double orc_jacobi_lambda_max(const orc_csr* A, int iters, uint64_t seed);
 int64_t orc_mpnn_nparams(int H, int L);
 /* w[nnz]: sqrt(|a_nm a_mn|)/sqrt(|a_nn a_mm|) off the diagonal, 0 on it */
 void orc_mpnn_slot_weights(const orc_csr* A, double* w);
-/* feat[N][5]; scales = {s_b, s_x, s_r, s_e} */
+/* feat[N][6]; scales = {s_b, s_x, s_r, s_e, s_o} (s_o: output scale) */
 void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
-                       double* feat, double* scales);
+                       const double* xp, double* feat, double* scales);
 void orc_mpnn_forward(const double* P, int H, int L, const orc_csr* A,
                       const double* w, const double* feat, double* delta,
                       double* acts);

@@ -122,6 +122,7 @@ lib.orc_assemble_momentum.argtypes = [C.POINTER(MomentumOpts), C.c_int64,
 lib.orc_assemble_continuity.argtypes = [C.POINTER(ContinuityOpts), C.c_int64,
                                         C.c_int64, _i32p] + [_f64p] * 7 + [
                                             C.c_void_p, _dense, _dense]
+lib.orc_set_threads.argtypes = [C.c_int]
 lib.orc_spmv.argtypes = [C.POINTER(Csr), _f64p, _f64p]
 lib.orc_gmres.argtypes = [C.POINTER(Csr), _f64p, _f64p, C.POINTER(GmresOpts),
                           C.POINTER(GmresStats)]
@@ -130,6 +131,11 @@ lib.orc_dense_solve.argtypes = [C.c_int, _f64p, _f64p]
 lib.orc_dense_solve.restype = C.c_int
 lib.orc_jacobi_lambda_max.argtypes = [C.POINTER(Csr), C.c_int, C.c_uint64]
 lib.orc_jacobi_lambda_max.restype = C.c_double
+
+
+def set_threads(n):
+    """Threads of the CSR assembly loops (atomic scatter above 1)."""
+    lib.orc_set_threads(int(n))
 
 
 def f64(a):
@@ -379,7 +385,8 @@ def assemble_continuity(opts, nnode, edges, coords, vel, dpdx, rho, p, udiag,
 lib.orc_mpnn_nparams.argtypes = [C.c_int, C.c_int]
 lib.orc_mpnn_nparams.restype = C.c_int64
 lib.orc_mpnn_slot_weights.argtypes = [C.POINTER(Csr), _f64p]
-lib.orc_mpnn_features.argtypes = [C.POINTER(Csr), _f64p, _f64p, _f64p, _f64p]
+lib.orc_mpnn_features.argtypes = [C.POINTER(Csr), _f64p, _f64p, _f64p, _f64p,
+                                  _f64p]
 lib.orc_mpnn_forward.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr), _f64p,
                                  _f64p, _f64p, C.c_void_p]
 lib.orc_mpnn_loss_grad.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr),
@@ -397,10 +404,16 @@ def mpnn_slot_weights(A):
     return w
 
 
-def mpnn_features(A, b, x0):
-    feat = np.zeros((A.c.nrows, 5))
-    sc = np.zeros(4)
-    lib.orc_mpnn_features(A.ref, f64(b), f64(x0), feat, sc)
+MPNN_NF = 6
+
+
+def mpnn_features(A, b, x0, xp=None):
+    """Node features and scales {s_b, s_x, s_r, s_e, s_o}; xp (the guess
+    before x0) defaults to x0 (no history)."""
+    feat = np.zeros((A.c.nrows, MPNN_NF))
+    sc = np.zeros(5)
+    lib.orc_mpnn_features(A.ref, f64(b), f64(x0), f64(x0 if xp is None else xp),
+                          feat, sc)
     return feat, sc
 
 

@@ -6,6 +6,23 @@
  */
 #include "orc.h"
 
+/* Threads of the CSR assembly loops (orc_set_threads); above 1 the loops
+ * run in parallel over edges and the scatter adds are atomic, as the
+ * reference's Kokkos-OpenMP build does (src/TpetraLinearSystem.C:1304-1305). */
+static int g_threads = 1;
+
+void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
+#define ORC_ADD(p, v)                                   \
+  do {                                                  \
+    if (g_threads > 1) {                                \
+      _Pragma("omp atomic")(p) += (v);                  \
+    } else {                                            \
+      (p) += (v);                                       \
+    }                                                   \
+  } while (0)
+
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -451,9 +468,9 @@ static void sum_into_row(orc_csr* A, int64_t row, int nent, int ndof,
         if (offset >= length) return;
       }
       const int entry_offset = perm[id_index];
-      value[offset + 0] += input[entry_offset + 0];
-      value[offset + 1] += input[entry_offset + 1];
-      value[offset + 2] += input[entry_offset + 2];
+      ORC_ADD(value[offset + 0], input[entry_offset + 0]);
+      ORC_ADD(value[offset + 1], input[entry_offset + 1]);
+      ORC_ADD(value[offset + 2], input[entry_offset + 2]);
       offset += 3;
     }
     return;
@@ -463,7 +480,7 @@ static void sum_into_row(orc_csr* A, int64_t row, int nent, int ndof,
     const int perm_index = perm[j];
     const int32_t cur = localIds[j];
     while (offset < length && colidx[offset] != cur) ++offset;
-    if (offset < length) value[offset] += input[perm_index];
+    if (offset < length) ORC_ADD(value[offset], input[perm_index]);
   }
 }
 
@@ -495,7 +512,7 @@ void orc_csr_sum_into(orc_csr* A, int nent, const int32_t* nodes,
     const int64_t rowLid = (int64_t)nodes[i] * ndof + perm[r] % ndof;
     const double* cur_lhs = lhs + (int64_t)perm[r] * numRows;
     sum_into_row(A, rowLid, nent, ndof, localIds, perm, cur_lhs);
-    A->rhs[rowLid] += rhs[perm[r]];
+    ORC_ADD(A->rhs[rowLid], rhs[perm[r]]);
   }
 }
 
@@ -588,6 +605,7 @@ void orc_assemble_scalar(const orc_scalar_opts* o, int64_t nnode,
                          const double* areav, const double* mdot, orc_csr* A,
                          double* lhs_dense, double* rhs_dense)
 {
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (A && g_threads > 1)
   for (int64_t e = 0; e < nedge; ++e) {
     double lhs[4], rhs[2];
     orc_scalar_edge(o, e, edges[2 * e], edges[2 * e + 1], coords, vrtm, q,
@@ -606,6 +624,7 @@ void orc_assemble_momentum(const orc_momentum_opts* o, int64_t nnode,
                            const double* wallmask, orc_csr* A,
                            double* lhs_dense, double* rhs_dense)
 {
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (A && g_threads > 1)
   for (int64_t e = 0; e < nedge; ++e) {
     double lhs[36], rhs[6];
     orc_momentum_edge(o, e, edges[2 * e], edges[2 * e + 1], coords, vel, dudx,
@@ -623,6 +642,7 @@ void orc_assemble_continuity(const orc_continuity_opts* o, int64_t nnode,
                              const double* areav, orc_csr* A,
                              double* lhs_dense, double* rhs_dense)
 {
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (A && g_threads > 1)
   for (int64_t e = 0; e < nedge; ++e) {
     double lhs[4], rhs[2];
     orc_continuity_edge(o, e, edges[2 * e], edges[2 * e + 1], coords, vel,

@@ -6,14 +6,18 @@
  * kernels are checked against, and its analytic gradient is pinned by
  * finite differences in tests/test_oracle.py.
  *
- * Network (F = 5 inputs, H hidden, L message-passing layers):
+ * Network (F = 6 inputs, H hidden, L message-passing layers):
  *   h0      = relu(W_in f + b_in)
  *   m_l     = sum_{m in N(n)} w_nm h_{l-1,m}
  *   h_l     = relu(W_self,l h_{l-1} + W_nbr,l m_l + b_l),   l = 1..L
  *   delta_n = w_out . h_L + b_out
  * Edge weights w_nm = sqrt(|a_nm a_mn|) / sqrt(|a_nn a_mm|) (symmetric).
  * Node features: b/s_b, x0/s_x, r/s_r (r = b - A x0), log|a_nn| - mean,
- * Dirichlet flag; prediction x = x0 + s_e delta with s_e = rms(r / diag).
+ * Dirichlet flag, (x0 - xp)/s_d with xp the guess before x0 (the solution
+ * two steps back) and s_d = rms(x0 - xp).  Prediction x = xb + s_o delta
+ * with xb = 2 x0 - xp and s_o = s_d, or xb = x0 and s_o = s_e =
+ * rms(r / diag) when x0 == xp (no history); rows without off-diagonal
+ * entries take b / a_nn.
  * Parameters, flat: W_in[H][F], b_in[H], then per layer W_self[H][H],
  * W_nbr[H][H], b[H], then w_out[H], b_out.
  */
@@ -23,7 +27,7 @@
 
 #include "orc.h"
 
-#define NF 5
+#define NF 6
 
 int64_t orc_mpnn_nparams(int H, int L)
 {
@@ -63,38 +67,45 @@ void orc_mpnn_slot_weights(const orc_csr* A, double* w)
 }
 
 void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
-                       double* feat, double* scales)
+                       const double* xp, double* feat, double* scales)
 {
   const int64_t n = A->nrows;
   double* r = (double*)malloc(sizeof(double) * (size_t)n);
   double* e = (double*)malloc(sizeof(double) * (size_t)n);
+  double* d = (double*)malloc(sizeof(double) * (size_t)n);
   orc_spmv(A, x0, r);
-  double mlog = 0.0;
+  double mlog = 0.0, dd = 0.0;
   for (int64_t i = 0; i < n; ++i) {
     r[i] = b[i] - r[i];
-    const double d = fabs(entry(A, i, (int32_t)i));
-    e[i] = d > 0 ? r[i] / d : 0.0;
-    mlog += log(d > 0 ? d : 1.0);
+    const double a = fabs(entry(A, i, (int32_t)i));
+    e[i] = a > 0 ? r[i] / a : 0.0;
+    d[i] = x0[i] - xp[i];
+    dd += d[i] * d[i];
+    mlog += log(a > 0 ? a : 1.0);
   }
   mlog /= (double)(n > 0 ? n : 1);
   const double sb = rms(b, n), sx = rms(x0, n), sr = rms(r, n), se = rms(e, n);
+  const double sd = rms(d, n);
   for (int64_t i = 0; i < n; ++i) {
-    const double d = fabs(entry(A, i, (int32_t)i));
+    const double a = fabs(entry(A, i, (int32_t)i));
     int dir = 1;
     for (int64_t k = A->row_ptr[i]; k < A->row_ptr[i + 1]; ++k)
       if (A->cols[k] != i && A->vals[k] != 0.0) dir = 0;
     feat[NF * i + 0] = b[i] / sb;
     feat[NF * i + 1] = x0[i] / sx;
     feat[NF * i + 2] = r[i] / sr;
-    feat[NF * i + 3] = log(d > 0 ? d : 1.0) - mlog;
+    feat[NF * i + 3] = log(a > 0 ? a : 1.0) - mlog;
     feat[NF * i + 4] = (double)dir;
+    feat[NF * i + 5] = d[i] / sd;
   }
   scales[0] = sb;
   scales[1] = sx;
   scales[2] = sr;
   scales[3] = se;
+  scales[4] = dd > 0 ? sd : se; /* output scale */
   free(r);
   free(e);
+  free(d);
 }
 
 static void aggregate(const orc_csr* A, const double* w, int H,

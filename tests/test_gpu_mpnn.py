@@ -49,19 +49,22 @@ def _params(nw, seed, scale=0.3):
 
 
 def _solved_pair(nw, ctx, A, seed):
-    """x0 random, x the FP64 solution from it; the solve leaves x0 as the
-    recorded guess."""
-    x0 = np.random.default_rng(seed).normal(size=A.c.nrows)
-    ctx.set_solution(x0)
-    st = ctx.solve(nw.solver_cfg(kspace=30, max_iters=4000, tol=1e-12,
-                                 x0=nw.X0_SOLUTION))
-    assert st.converged
-    return x0, ctx.solution()
+    """Two solves, from xp and then from x0 (both random): afterwards x0 is
+    the guess of the last solve, xp the guess of the one before, x the FP64
+    solution."""
+    rng = np.random.default_rng(seed)
+    cfg = nw.solver_cfg(kspace=30, max_iters=4000, tol=1e-12, x0=nw.X0_SOLUTION)
+    xp = rng.normal(size=A.c.nrows)
+    x0 = rng.normal(size=A.c.nrows)
+    for g in (xp, x0):
+        ctx.set_solution(g)
+        assert ctx.solve(cfg).converged
+    return xp, x0, ctx.solution()
 
 
 def test_gradient_matches_oracle(nw, ctx):
     P, A, top = _system(nw, ctx)
-    x0, xs = _solved_pair(nw, ctx, A, 11)
+    xp, x0, xs = _solved_pair(nw, ctx, A, 11)
     net = nw.Mpnn(ctx, H, L, seed=3)
     p = _params(nw, 5)
     net.params = p
@@ -69,8 +72,8 @@ def test_gradient_matches_oracle(nw, ctx):
     assert net.samples == 1
     loss, g = net.loss_grad(0)
     w = orc.mpnn_slot_weights(A)
-    feat, sc = orc.mpnn_features(A, A.rhs, x0)
-    target = (xs - x0) / sc[3]
+    feat, sc = orc.mpnn_features(A, A.rhs, x0, xp)
+    target = (xs - (2 * x0 - xp)) / sc[4]  # from the extrapolated guess
     loss_o, g_o = orc.mpnn_loss_grad(p, H, L, A, w, feat, target)
     # FP32 device arithmetic against the FP64 oracle
     assert loss == pytest.approx(loss_o, rel=2e-5)
@@ -78,7 +81,7 @@ def test_gradient_matches_oracle(nw, ctx):
     np.testing.assert_allclose(g, g_o, rtol=0, atol=2e-5 * np.abs(g_o).max())
     # every parameter block receives gradient
     o = 0
-    for size in [H * 5, H] + [H * H, H * H, H] * L + [H, 1]:
+    for size in [H * 6, H] + [H * H, H * H, H] * L + [H, 1]:
         assert np.abs(g_o[o:o + size]).max() > 0
         o += size
     # the same call again gives identical bits (fixed-order reductions)
@@ -91,15 +94,27 @@ def test_prediction_matches_oracle(nw, ctx):
     net = nw.Mpnn(ctx, H, L, seed=3)
     p = _params(nw, 7)
     net.params = p
-    x0 = np.random.default_rng(8).normal(size=P.n)
-    ctx.set_solution(x0)
-    x, se = net.predict()
+    rng = np.random.default_rng(8)
+    x0, xp = rng.normal(size=P.n), rng.normal(size=P.n)
     w = orc.mpnn_slot_weights(A)
-    feat, sc = orc.mpnn_features(A, A.rhs, x0)
-    d = orc.mpnn_forward(p, H, L, A, w, feat)
-    assert se == pytest.approx(sc[3], rel=1e-12)
-    np.testing.assert_allclose((x - x0) / se, d, rtol=0,
-                               atol=2e-5 * np.abs(d).max())
+    # without history (no solve yet), then with the last solve's guess as xp
+    for hist in (False, True):
+        if hist:
+            ctx.set_solution(xp)
+            ctx.solve(nw.solver_cfg(kspace=30, max_iters=10, tol=1e-3,
+                                    x0=nw.X0_SOLUTION), raise_on_noconv=False)
+        ctx.set_solution(x0)
+        x, so = net.predict()
+        feat, sc = orc.mpnn_features(A, A.rhs, x0, xp if hist else None)
+        d = orc.mpnn_forward(p, H, L, A, w, feat)
+        assert so == pytest.approx(sc[4], rel=1e-12)
+        assert (so == pytest.approx(sc[3], rel=1e-12)) != hist
+        xb = 2 * x0 - xp if hist else x0
+        inner = feat[:, 4] == 0
+        np.testing.assert_allclose((x[inner] - xb[inner]) / so, d[inner],
+                                   rtol=0, atol=2e-5 * np.abs(d).max())
+        # Dirichlet rows take b / a_nn (here 0)
+        np.testing.assert_array_equal(x[~inner], A.rhs[~inner])
 
 
 def test_training_is_deterministic_and_lowers_loss(nw, ctx):
@@ -172,20 +187,36 @@ def test_learned_guess_beats_previous_step_guess(nw, ctx):
     ctx.set_learned_guess(net)
     it_prev, it_l, r_prev, r_l = [], [], [], []
     for k in range(30, 38):
+        # learned solve first (its inputs: x0 = the previous solution, xp =
+        # the guess of the previous step's last solve), then the plain
+        # previous-step solve from the same x0, which leaves the history as
+        # the next step expects it
         x_before = ctx.solution()
         S.step(k)
+        s_l = ctx.solve(nw.solver_cfg(x0=nw.X0_LEARNED, **cfg))
+        x_l = ctx.solution()
+        ctx.set_solution(x_before)
         s_p = ctx.solve(nw.solver_cfg(x0=nw.X0_SOLUTION, **cfg))
         x_p = ctx.solution()
-        ctx.set_solution(x_before)
-        s_l = ctx.solve(nw.solver_cfg(x0=nw.X0_LEARNED, **cfg))
         assert s_p.converged and s_l.converged and s_l.final_rel <= 1e-8
-        np.testing.assert_allclose(ctx.solution(), x_p, rtol=0,
+        assert s_l.guess == nw.X0_LEARNED
+        np.testing.assert_allclose(x_l, x_p, rtol=0,
                                    atol=1e-6 * np.abs(x_p).max())
         it_prev.append(s_p.iters)
         it_l.append(s_l.iters)
         r_prev.append(s_p.x0_rel)
         r_l.append(s_l.x0_rel)
     print("loss", loss, "iters prev", it_prev, "learned", it_l,
-          "x0_rel prev", np.round(r_prev, 5), "learned", np.round(r_l, 5))
+          "x0_rel prev", np.round(r_prev, 6), "learned", np.round(r_l, 6))
     assert np.mean(r_l) < 0.7 * np.mean(r_prev)
     assert sum(it_l) < sum(it_prev)
+
+
+def test_device_fourier_velocity_matches_host(nw, ctx):
+    import workloads as wl
+    W = wl.continuity_rf(6, seed=5)
+    W.upload(ctx)
+    ctx.fourier_velocity(wl.fourier_modes(5), 0.37)
+    u = ctx.field_download(nw.F_VELOCITY)
+    np.testing.assert_allclose(u, wl.fourier_velocity(W.coords, 5, t=0.37),
+                               rtol=0, atol=1e-12)
