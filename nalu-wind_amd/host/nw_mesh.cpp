@@ -14,6 +14,7 @@
 
 #include "nalu_amd.h"
 #include "nalu_amd_mesh.h"
+#include "nw_geom.h"
 
 namespace {
 
@@ -157,6 +158,11 @@ double unit(uint64_t seed, uint64_t i)
 constexpr int64_t kChunk = 1 << 18;
 
 }  // namespace
+
+void nw::geom::hex_scs(const double c[8][3], double a[12][3])
+{
+  ::hex_scs(c, a);
+}
 
 extern "C" {
 

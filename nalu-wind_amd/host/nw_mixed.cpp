@@ -1,0 +1,431 @@
+// nw_mixed.cpp -- unstructured mixed Hex8 / Tet4 / Wed6 meshes for the edge
+// path: a box generator (cells split into Kuhn tetrahedra or prisms by a
+// seeded smooth random field, nodes randomly permuted and then renumbered by
+// reverse Cuthill-McKee), edge extraction for all three topologies and the
+// CVFEM edge area vectors of Tet4 and Wed6 sub-control surfaces.
+//
+// Element geometry runs in parallel in element chunks and is accumulated
+// onto edges serially in element order (thread-count independent results).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "nalu_amd.h"
+#include "nalu_amd_mesh.h"
+#include "nw_geom.h"
+
+namespace {
+
+// Sub-control surfaces: a point of the subdivided element is the average of
+// a set of its vertices; each SCS is a quad of four such points, and its
+// area vector points from the L to the R node of its edge.  The point lists
+// and quads follow TetSCS::determinant_scs
+// (src/master_element/Tet4CVFEM.C:522-618) and WedSCS::determinant_scs
+// (src/master_element/Wed6CVFEM.C:541-646); L/R pairs are the lrscv tables
+// (include/master_element/Tet4CVFEM.h:266, Wed6CVFEM.h:268).
+struct Topo {
+  int npe, nscs, npts;
+  const int (*pts)[7];  // {count, vertices...}
+  const int (*quad)[4];
+  const int (*lr)[2];
+};
+
+const int kTetPts[15][7] = {
+  {1, 0}, {1, 1}, {1, 2}, {1, 3}, {2, 0, 1}, {2, 1, 2}, {2, 2, 0},
+  {3, 0, 1, 2}, {2, 2, 3}, {2, 3, 1}, {3, 1, 2, 3}, {2, 0, 3}, {3, 0, 2, 3},
+  {3, 0, 1, 3}, {4, 0, 1, 2, 3}};
+const int kTetQuad[6][4] = {{4, 7, 14, 13}, {7, 14, 10, 5}, {6, 12, 14, 7},
+                            {11, 13, 14, 12}, {13, 9, 10, 14}, {10, 8, 12, 14}};
+const int kTetLr[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+
+const int kWedPts[21][7] = {
+  {1, 0}, {1, 1}, {1, 2}, {1, 3}, {1, 4}, {1, 5}, {2, 0, 1}, {2, 1, 2},
+  {2, 2, 0}, {3, 0, 1, 2}, {2, 3, 4}, {2, 4, 5}, {2, 5, 3}, {3, 3, 4, 5},
+  {2, 1, 4}, {2, 0, 3}, {4, 0, 1, 4, 3}, {2, 2, 5}, {4, 1, 4, 5, 2},
+  {4, 5, 3, 0, 2}, {6, 0, 1, 2, 3, 4, 5}};
+const int kWedQuad[9][4] = {{6, 9, 20, 16},   {7, 9, 20, 18},  {9, 8, 19, 20},
+                            {10, 16, 20, 13}, {13, 11, 18, 20}, {12, 13, 20, 19},
+                            {15, 16, 20, 19}, {16, 14, 18, 20}, {19, 20, 18, 17}};
+const int kWedLr[9][2] = {{0, 1}, {1, 2}, {0, 2}, {3, 4}, {4, 5},
+                          {3, 5}, {0, 3}, {1, 4}, {2, 5}};
+
+// Hex8 edges in the order of its SCS (include/master_element/Hex8CVFEM.h:
+// 292-293); its area vectors come from the structured path's Hex8 SCS.
+const int kHexLr[12][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {4, 5}, {5, 6},
+                           {6, 7}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+const Topo kTet = {4, 6, 15, kTetPts, kTetQuad, kTetLr};
+const Topo kWed = {6, 9, 21, kWedPts, kWedQuad, kWedLr};
+
+int npe_of(int t) { return t == 0 ? 8 : (t == 1 ? 4 : 6); }
+int nedge_of(int t) { return t == 0 ? 12 : (t == 1 ? 6 : 9); }
+const int (*lr_of(int t))[2] { return t == 0 ? kHexLr : (t == 1 ? kTetLr : kWedLr); }
+
+void quad_area4(const double q[4][3], double a[3])
+{
+  // four triangles about the quad centroid (Grandy)
+  double m[3];
+  for (int d = 0; d < 3; ++d) m[d] = 0.25 * (q[0][d] + q[1][d] + q[2][d] + q[3][d]);
+  a[0] = a[1] = a[2] = 0.0;
+  for (int t = 0; t < 4; ++t) {
+    const double* p = q[t];
+    const double* r = q[(t + 1) & 3];
+    const double u[3] = {p[0] - m[0], p[1] - m[1], p[2] - m[2]};
+    const double v[3] = {r[0] - m[0], r[1] - m[1], r[2] - m[2]};
+    a[0] += u[1] * v[2] - v[1] * u[2];
+    a[1] += u[2] * v[0] - v[2] * u[0];
+    a[2] += u[0] * v[1] - v[0] * u[1];
+  }
+  for (int d = 0; d < 3; ++d) a[d] *= 0.5;
+}
+
+void scs_areas(const Topo& T, const double c[][3], double a[][3])
+{
+  double p[21][3];
+  for (int k = 0; k < T.npts; ++k) {
+    const int cnt = T.pts[k][0];
+    for (int d = 0; d < 3; ++d) {
+      double s = 0.0;
+      for (int i = 1; i <= cnt; ++i) s += c[T.pts[k][i]][d];
+      p[k][d] = s / cnt;
+    }
+  }
+  for (int s = 0; s < T.nscs; ++s) {
+    double q[4][3];
+    for (int i = 0; i < 4; ++i)
+      for (int d = 0; d < 3; ++d) q[i][d] = p[T.quad[s][i]][d];
+    quad_area4(q, a[s]);
+  }
+}
+
+uint64_t mix64(uint64_t z)
+{
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+double uni(uint64_t seed, uint64_t i)
+{
+  return (double)(mix64(seed + 0x9e3779b97f4a7c15ULL * (i + 1)) >> 11) *
+         (1.0 / 9007199254740992.0);
+}
+
+// Kuhn subdivision of the unit cube (hex vertex numbering) into 6
+// tetrahedra around the 0-6 diagonal; each path 0 -> 6 through one
+// coordinate step at a time is one tetrahedron.
+const int kKuhn[6][4] = {{0, 1, 2, 6}, {0, 2, 3, 6}, {0, 3, 7, 6},
+                         {0, 7, 4, 6}, {0, 4, 5, 6}, {0, 5, 1, 6}};
+// two prisms: bottom/top triangles split along the 0-2 / 4-6 diagonals
+const int kPrism[2][6] = {{0, 1, 2, 4, 5, 6}, {0, 2, 3, 4, 6, 7}};
+
+double tet_volume(const double c[][3])
+{
+  double u[3], v[3], w[3];
+  for (int d = 0; d < 3; ++d) {
+    u[d] = c[1][d] - c[0][d];
+    v[d] = c[2][d] - c[0][d];
+    w[d] = c[3][d] - c[0][d];
+  }
+  return (u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) +
+          u[2] * (v[0] * w[1] - v[1] * w[0])) / 6.0;
+}
+
+// Unique sorted (lo, hi) node pairs of the element edges, bucketed by lo.
+struct EdgeSet {
+  std::vector<int64_t> ptr;  // [nnode+1]
+  std::vector<int32_t> hi;
+};
+
+int build_edges(int64_t nelem, const int32_t* type, const int32_t* conn,
+                int64_t nnode, EdgeSet& es)
+{
+  std::vector<int64_t> cnt(nnode + 1, 0);
+  for (int64_t el = 0; el < nelem; ++el) {
+    const int t = type[el];
+    if (t < 0 || t > 2) return NW_E_ARG;
+    const auto lr = lr_of(t);
+    for (int k = 0; k < nedge_of(t); ++k) {
+      const int32_t a = conn[8 * el + lr[k][0]], b = conn[8 * el + lr[k][1]];
+      if (a < 0 || b < 0 || a >= nnode || b >= nnode || a == b) return NW_E_ARG;
+      cnt[std::min(a, b) + 1]++;
+    }
+  }
+  for (int64_t n = 0; n < nnode; ++n) cnt[n + 1] += cnt[n];
+  std::vector<int32_t> nb(cnt[nnode]);
+  {
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t el = 0; el < nelem; ++el) {
+      const auto lr = lr_of(type[el]);
+      for (int k = 0; k < nedge_of(type[el]); ++k) {
+        const int32_t a = conn[8 * el + lr[k][0]], b = conn[8 * el + lr[k][1]];
+        nb[pos[std::min(a, b)]++] = std::max(a, b);
+      }
+    }
+  }
+  es.ptr.assign(nnode + 1, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t n = 0; n < nnode; ++n) {
+    auto b = nb.begin() + cnt[n], e = nb.begin() + cnt[n + 1];
+    std::sort(b, e);
+    es.ptr[n + 1] = std::unique(b, e) - b;
+  }
+  for (int64_t n = 0; n < nnode; ++n) es.ptr[n + 1] += es.ptr[n];
+  es.hi.resize(es.ptr[nnode]);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t n = 0; n < nnode; ++n)
+    std::copy(nb.begin() + cnt[n], nb.begin() + cnt[n] + (es.ptr[n + 1] - es.ptr[n]),
+              es.hi.begin() + es.ptr[n]);
+  return NW_OK;
+}
+
+// Reverse Cuthill-McKee order of the node graph (each component from a
+// minimum-degree node, neighbours by increasing degree, ties by id).
+std::vector<int32_t> rcm(int64_t nnode, const EdgeSet& es)
+{
+  std::vector<int64_t> deg(nnode, 0);
+  for (int64_t a = 0; a < nnode; ++a)
+    for (int64_t k = es.ptr[a]; k < es.ptr[a + 1]; ++k) {
+      deg[a]++;
+      deg[es.hi[k]]++;
+    }
+  std::vector<int64_t> aptr(nnode + 1, 0);
+  for (int64_t a = 0; a < nnode; ++a) aptr[a + 1] = aptr[a] + deg[a];
+  std::vector<int32_t> adj(aptr[nnode]);
+  {
+    std::vector<int64_t> pos(aptr.begin(), aptr.end() - 1);
+    for (int64_t a = 0; a < nnode; ++a)
+      for (int64_t k = es.ptr[a]; k < es.ptr[a + 1]; ++k) {
+        const int32_t b = es.hi[k];
+        adj[pos[a]++] = b;
+        adj[pos[b]++] = (int32_t)a;
+      }
+  }
+  std::vector<int32_t> order;
+  order.reserve(nnode);
+  std::vector<char> seen(nnode, 0);
+  std::vector<int32_t> by_deg(nnode);
+  std::iota(by_deg.begin(), by_deg.end(), 0);
+  std::stable_sort(by_deg.begin(), by_deg.end(),
+                   [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
+  std::vector<int32_t> nbr;
+  for (int32_t start : by_deg) {
+    if (seen[start]) continue;
+    size_t head = order.size();
+    order.push_back(start);
+    seen[start] = 1;
+    while (head < order.size()) {
+      const int32_t v = order[head++];
+      nbr.clear();
+      for (int64_t k = aptr[v]; k < aptr[v + 1]; ++k)
+        if (!seen[adj[k]]) {
+          seen[adj[k]] = 1;
+          nbr.push_back(adj[k]);
+        }
+      std::stable_sort(nbr.begin(), nbr.end(), [&](int32_t a, int32_t b) {
+        return deg[a] != deg[b] ? deg[a] < deg[b] : a < b;
+      });
+      order.insert(order.end(), nbr.begin(), nbr.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+constexpr int64_t kChunk = 1 << 18;
+
+}  // namespace
+
+extern "C" {
+
+int nw_mixed_box(int n, double f_hex, double f_tet, double jitter,
+                 uint64_t seed, int64_t* nnode_out, int64_t* nelem_out,
+                 double* coords, int32_t* type, int32_t* conn)
+{
+  if (n < 1 || f_hex < 0 || f_tet < 0 || f_hex + f_tet > 1.0 || jitter < 0 ||
+      jitter >= 0.5 || !nnode_out || !nelem_out)
+    return NW_E_ARG;
+  const int64_t p = n + 1, nn = p * p * p, nc = (int64_t)n * n * n;
+  if (nn >= INT32_MAX) return NW_E_ARG;
+  // cell types: quantiles of a smooth seeded field (4 random Fourier modes)
+  double k[4][3], ph[4];
+  for (int m = 0; m < 4; ++m) {
+    for (int d = 0; d < 3; ++d) k[m][d] = 2.0 * M_PI * (1.0 + 2.0 * uni(seed, 10 * m + d));
+    ph[m] = 2.0 * M_PI * uni(seed, 10 * m + 5);
+  }
+  std::vector<double> f(nc);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < nc; ++c) {
+    const int64_t i = c % n, j = (c / n) % n, l = c / ((int64_t)n * n);
+    const double x[3] = {(i + 0.5) / n, (j + 0.5) / n, (l + 0.5) / n};
+    double s = 0.0;
+    for (int m = 0; m < 4; ++m)
+      s += std::sin(k[m][0] * x[0] + k[m][1] * x[1] + k[m][2] * x[2] + ph[m]);
+    f[c] = s;
+  }
+  std::vector<int64_t> ord(nc);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return f[a] < f[b]; });
+  std::vector<int8_t> ct(nc);
+  const int64_t nh = (int64_t)std::llround(f_hex * nc);
+  const int64_t nt = (int64_t)std::llround(f_tet * nc);
+  for (int64_t r = 0; r < nc; ++r) ct[ord[r]] = r < nh ? 0 : (r < nh + nt ? 1 : 2);
+  int64_t ne = 0;
+  for (int64_t c = 0; c < nc; ++c) ne += ct[c] == 0 ? 1 : (ct[c] == 1 ? 6 : 2);
+  *nnode_out = nn;
+  *nelem_out = ne;
+  if (!coords) return NW_OK;  // sizes only
+  if (!type || !conn) return NW_E_ARG;
+  // base numbering x fastest; interior nodes jittered
+  std::vector<double> xb(3 * nn);
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < nn; ++v) {
+    const int64_t i = v % p, j = (v / p) % p, l = v / (p * p);
+    const int64_t idx[3] = {i, j, l};
+    for (int d = 0; d < 3; ++d) {
+      double x = (double)idx[d] / n;
+      if (jitter > 0 && i > 0 && i < n && j > 0 && j < n && l > 0 && l < n)
+        x += (2.0 * uni(seed ^ 0x5bd1e995ULL, 3 * v + d) - 1.0) * jitter / n;
+      xb[3 * v + d] = x;
+    }
+  }
+  std::vector<int32_t> tb(ne), cb(8 * ne, -1);
+  {
+    int64_t e = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+      const int64_t i = c % n, j = (c / n) % n, l = c / ((int64_t)n * n);
+      const int64_t v0 = i + p * (j + p * l);
+      const int64_t hv[8] = {v0, v0 + 1, v0 + 1 + p, v0 + p, v0 + p * p,
+                             v0 + 1 + p * p, v0 + 1 + p + p * p, v0 + p + p * p};
+      if (ct[c] == 0) {
+        tb[e] = 0;
+        for (int a = 0; a < 8; ++a) cb[8 * e + a] = (int32_t)hv[a];
+        ++e;
+      } else if (ct[c] == 1) {
+        for (int t = 0; t < 6; ++t, ++e) {
+          tb[e] = 1;
+          int32_t q[4];
+          double cc[4][3];
+          for (int a = 0; a < 4; ++a) {
+            q[a] = (int32_t)hv[kKuhn[t][a]];
+            for (int d = 0; d < 3; ++d) cc[a][d] = xb[3 * (int64_t)q[a] + d];
+          }
+          if (tet_volume(cc) < 0) std::swap(q[1], q[2]);
+          for (int a = 0; a < 4; ++a) cb[8 * e + a] = q[a];
+        }
+      } else {
+        for (int t = 0; t < 2; ++t, ++e) {
+          tb[e] = 2;
+          for (int a = 0; a < 6; ++a) cb[8 * e + a] = (int32_t)hv[kPrism[t][a]];
+        }
+      }
+    }
+  }
+  // random relabelling, then reverse Cuthill-McKee of the relabelled graph
+  std::vector<int32_t> perm(nn);
+  std::iota(perm.begin(), perm.end(), 0);
+  for (int64_t v = nn - 1; v > 0; --v) {
+    const int64_t w = (int64_t)(mix64(seed * 0x2545f4914f6cdd1dULL + v) % (uint64_t)(v + 1));
+    std::swap(perm[v], perm[w]);
+  }
+  for (auto& v : cb)
+    if (v >= 0) v = perm[v];
+  EdgeSet es;
+  int rc = build_edges(ne, tb.data(), cb.data(), nn, es);
+  if (rc) return rc;
+  const std::vector<int32_t> order = rcm(nn, es);
+  std::vector<int32_t> newid(nn);
+  for (int64_t r = 0; r < nn; ++r) newid[order[r]] = (int32_t)r;
+  // perm maps base -> relabelled; newid maps relabelled -> final
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < nn; ++v)
+    for (int d = 0; d < 3; ++d) coords[3 * (int64_t)newid[perm[v]] + d] = xb[3 * v + d];
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < ne; ++e) {
+    type[e] = tb[e];
+    for (int a = 0; a < 8; ++a) conn[8 * e + a] = cb[8 * e + a] >= 0 ? newid[cb[8 * e + a]] : -1;
+  }
+  return NW_OK;
+}
+
+int nw_mixed_edges(int64_t nelem, const int32_t* type, const int32_t* conn,
+                   int64_t nnode, int32_t* edges, int64_t cap, int64_t* nedge)
+{
+  if (nelem < 0 || !type || !conn || nnode <= 0 || !nedge) return NW_E_ARG;
+  EdgeSet es;
+  int rc = build_edges(nelem, type, conn, nnode, es);
+  if (rc) return rc;
+  *nedge = es.ptr[nnode];
+  if (!edges) return NW_OK;
+  if (*nedge > cap) return NW_E_ARG;
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t a = 0; a < nnode; ++a)
+    for (int64_t k = es.ptr[a]; k < es.ptr[a + 1]; ++k) {
+      edges[2 * k] = (int32_t)a;
+      edges[2 * k + 1] = es.hi[k];
+    }
+  return NW_OK;
+}
+
+int nw_mixed_edge_area_vector(int64_t nelem, const int32_t* type,
+                              const int32_t* conn, const double* coords,
+                              const int32_t* edges, int64_t nedge,
+                              double* areav)
+{
+  if (nelem < 0 || !type || !conn || !coords || !edges || !areav) return NW_E_ARG;
+  std::memset(areav, 0, sizeof(double) * 3 * nedge);
+  // edge lookup: edges are sorted by (node[0], node[1])
+  auto find = [&](int32_t a, int32_t b) -> int64_t {
+    int64_t lo = 0, hi = nedge;
+    while (lo < hi) {
+      const int64_t m = (lo + hi) / 2;
+      if (edges[2 * m] < a || (edges[2 * m] == a && edges[2 * m + 1] < b))
+        lo = m + 1;
+      else
+        hi = m;
+    }
+    return (lo < nedge && edges[2 * lo] == a && edges[2 * lo + 1] == b) ? lo : -1;
+  };
+  std::vector<double> buf((size_t)std::min<int64_t>(nelem, kChunk) * 36);
+  std::vector<int64_t> eid((size_t)std::min<int64_t>(nelem, kChunk) * 12);
+  int bad = 0;
+  for (int64_t e0 = 0; e0 < nelem; e0 += kChunk) {
+    const int64_t e1 = std::min(nelem, e0 + kChunk);
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t el = e0; el < e1; ++el) {
+      const int t = type[el];
+      double c[8][3];
+      for (int a = 0; a < npe_of(t); ++a)
+        for (int d = 0; d < 3; ++d) c[a][d] = coords[3 * (int64_t)conn[8 * el + a] + d];
+      double(*a)[3] = reinterpret_cast<double(*)[3]>(&buf[(el - e0) * 36]);
+      if (t == 0) {
+        nw::geom::hex_scs(c, a);
+      } else {
+        scs_areas(t == 1 ? kTet : kWed, c, a);
+      }
+      const auto lr = lr_of(t);
+      for (int s = 0; s < nedge_of(t); ++s) {
+        const int32_t nl = conn[8 * el + lr[s][0]], nr = conn[8 * el + lr[s][1]];
+        const int64_t e = find(std::min(nl, nr), std::max(nl, nr));
+        if (e < 0) bad = 1;
+        eid[(el - e0) * 12 + s] = nl < nr ? e : -(e + 1);  // sign in the id
+      }
+    }
+    if (bad) return NW_E_ARG;
+    // element-ordered accumulation, +1 when L is the edge's node[0]
+    for (int64_t el = e0; el < e1; ++el)
+      for (int s = 0; s < nedge_of(type[el]); ++s) {
+        const int64_t code = eid[(el - e0) * 12 + s];
+        const int64_t e = code >= 0 ? code : -code - 1;
+        const double sgn = code >= 0 ? 1.0 : -1.0;
+        const double* a = &buf[(el - e0) * 36 + 3 * s];
+        for (int d = 0; d < 3; ++d) areav[3 * e + d] += a[d] * sgn;
+      }
+  }
+  return NW_OK;
+}
+
+}  // extern "C"

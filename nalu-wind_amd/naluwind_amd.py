@@ -177,6 +177,13 @@ _sig = {
     "nw_mpnn_train": ([_P, C.POINTER(MpnnTrainCfg), C.POINTER(C.c_double)], C.c_int),
     "nw_mpnn_predict": ([_P, _f64p, C.POINTER(C.c_double)], C.c_int),
     "nw_linsys_set_learned_guess": ([_P, _P], C.c_int),
+    "nw_mixed_box": ([C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                      C.POINTER(C.c_int64), C.POINTER(C.c_int64), _P, _P, _P],
+                     C.c_int),
+    "nw_mixed_edges": ([C.c_int64, _i32p, _i32p, C.c_int64, _P, C.c_int64,
+                        C.POINTER(C.c_int64)], C.c_int),
+    "nw_mixed_edge_area_vector": ([C.c_int64, _i32p, _i32p, _f64p, _i32p,
+                                   C.c_int64, _f64p], C.c_int),
     "nw_fourier_velocity": ([_P, C.c_int, _f64p, _f64p, _f64p, _f64p,
                              C.c_double], C.c_int),
     "nw_hex_box_sizes": ([C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64),
@@ -227,6 +234,38 @@ def hex_box(nx, ny, nz, stretch_z=1.0, jitter=0.0, seed=0):
     conn = np.zeros((ne.value, 8), np.int32)
     check(lib.nw_hex_box(nx, ny, nz, stretch_z, jitter, seed, coords, conn))
     return coords, conn
+
+
+def mixed_box(n, f_hex=0.6, f_tet=0.3, jitter=0.0, seed=0):
+    """Mixed Hex8/Tet4/Wed6 box: (coords [N][3], type [E], conn [E][8])."""
+    nn, ne = C.c_int64(), C.c_int64()
+    check(lib.nw_mixed_box(n, f_hex, f_tet, jitter, seed, C.byref(nn),
+                           C.byref(ne), None, None, None))
+    coords = np.zeros((nn.value, 3))
+    types = np.zeros(ne.value, np.int32)
+    conn = np.zeros((ne.value, 8), np.int32)
+    check(lib.nw_mixed_box(n, f_hex, f_tet, jitter, seed, C.byref(nn),
+                           C.byref(ne), coords.ctypes.data_as(_P),
+                           types.ctypes.data_as(_P), conn.ctypes.data_as(_P)))
+    return coords, types, conn
+
+
+def mixed_edges(types, conn, nnode):
+    types, conn = _i32(types), _i32(conn)
+    ne = C.c_int64()
+    check(lib.nw_mixed_edges(types.size, types, conn, nnode, None, 0, C.byref(ne)))
+    edges = np.zeros((ne.value, 2), np.int32)
+    check(lib.nw_mixed_edges(types.size, types, conn, nnode,
+                             edges.ctypes.data_as(_P), ne.value, C.byref(ne)))
+    return edges
+
+
+def mixed_edge_area_vector(types, conn, coords, edges):
+    out = np.zeros((edges.shape[0], 3))
+    check(lib.nw_mixed_edge_area_vector(len(types), _i32(types), _i32(conn),
+                                        _f64(coords), _i32(edges),
+                                        edges.shape[0], out))
+    return out
 
 
 def hex_edges(conn, nnode):

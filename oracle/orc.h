@@ -54,6 +54,23 @@ void orc_hex_scv_volume(const double coords[24], double vol[8]);
  * include/master_element/Hex8CVFEM.h:292-293). */
 extern const int orc_hex_lrscv[24];
 
+/* Tet4 / Wed6 SCS area vectors (6 / 9 sub-control surfaces, one per edge),
+ * restating TetSCS::determinant_scs (src/master_element/Tet4CVFEM.C:
+ * 522-618) and WedSCS::determinant_scs (src/master_element/Wed6CVFEM.C:
+ * 541-646), and their L/R tables (include/master_element/Tet4CVFEM.h:266,
+ * Wed6CVFEM.h:268). */
+void orc_tet_scs_areav(const double coords[12], double areav[18]);
+void orc_wed_scs_areav(const double coords[18], double areav[27]);
+extern const int orc_tet_lrscv[12];
+extern const int orc_wed_lrscv[18];
+
+/* Edge area vectors of a mixed mesh (type 0 Hex8, 1 Tet4, 2 Wed6; conn
+ * [nelem][8]); edges sorted (node[0] < node[1]).  Returns -1 if an element
+ * edge is missing from the list. */
+int orc_edge_area_vec_mixed(int64_t nelem, const int32_t* type,
+                            const int32_t* conn, const double* coords,
+                            const int32_t* edges, int64_t nedge, double* areav);
+
 /* ------------------------------------------------------------------ */
 /* Mesh: edges and edge/node geometry                                  */
 /* ------------------------------------------------------------------ */

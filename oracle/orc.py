@@ -123,6 +123,11 @@ lib.orc_assemble_continuity.argtypes = [C.POINTER(ContinuityOpts), C.c_int64,
                                         C.c_int64, _i32p] + [_f64p] * 7 + [
                                             C.c_void_p, _dense, _dense]
 lib.orc_set_threads.argtypes = [C.c_int]
+lib.orc_tet_scs_areav.argtypes = [_f64p, _f64p]
+lib.orc_wed_scs_areav.argtypes = [_f64p, _f64p]
+lib.orc_edge_area_vec_mixed.argtypes = [C.c_int64, _i32p, _i32p, _f64p, _i32p,
+                                        C.c_int64, _f64p]
+lib.orc_edge_area_vec_mixed.restype = C.c_int
 lib.orc_spmv.argtypes = [C.POINTER(Csr), _f64p, _f64p]
 lib.orc_gmres.argtypes = [C.POINTER(Csr), _f64p, _f64p, C.POINTER(GmresOpts),
                           C.POINTER(GmresStats)]
@@ -157,6 +162,34 @@ def hex_scs_areav(coords8):
 def hex_scv_volume(coords8):
     out = np.zeros(8)
     lib.orc_hex_scv_volume(f64(coords8).ravel(), out)
+    return out
+
+
+def tet_scs_areav(coords4):
+    out = np.zeros(18)
+    lib.orc_tet_scs_areav(f64(coords4).ravel(), out)
+    return out.reshape(6, 3)
+
+
+def wed_scs_areav(coords6):
+    out = np.zeros(27)
+    lib.orc_wed_scs_areav(f64(coords6).ravel(), out)
+    return out.reshape(9, 3)
+
+
+TET_LRSCV = np.array([0, 1, 1, 2, 0, 2, 0, 3, 1, 3, 2, 3]).reshape(6, 2)
+WED_LRSCV = np.array([0, 1, 1, 2, 0, 2, 3, 4, 4, 5, 3, 5, 0, 3, 1, 4, 2,
+                      5]).reshape(9, 2)
+HEX_LRSCV = np.array([0, 1, 1, 2, 2, 3, 0, 3, 4, 5, 5, 6, 6, 7, 4, 7, 0, 4,
+                      1, 5, 2, 6, 3, 7]).reshape(12, 2)
+
+
+def edge_area_vec_mixed(types, conn, coords, edges):
+    out = np.zeros((edges.shape[0], 3))
+    rc = lib.orc_edge_area_vec_mixed(len(types), i32(types), i32(conn).ravel(),
+                                     f64(coords).ravel(), i32(edges).ravel(),
+                                     edges.shape[0], out)
+    assert rc == 0
     return out
 
 

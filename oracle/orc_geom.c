@@ -93,6 +93,134 @@ void orc_hex_scs_areav(const double coords[24], double areav[36])
   }
 }
 
+/* TetSCS::determinant_scs, src/master_element/Tet4CVFEM.C:522-618: edge
+ * midpoints, face centroids and the centroid, then one quad per edge. */
+void orc_tet_scs_areav(const double c[12], double areav[18])
+{
+  static const int facet[6][4] = {{4, 7, 14, 13},  {7, 14, 10, 5},
+                                  {6, 12, 14, 7},  {11, 13, 14, 12},
+                                  {13, 9, 10, 14}, {10, 8, 12, 14}};
+  double v[15][3];
+#define C(n, d) c[3 * (n) + (d)]
+  for (int d = 0; d < 3; ++d) {
+    for (int n = 0; n < 4; ++n) v[n][d] = C(n, d);
+    v[4][d] = 0.5 * (C(0, d) + C(1, d));
+    v[5][d] = 0.5 * (C(1, d) + C(2, d));
+    v[6][d] = 0.5 * (C(2, d) + C(0, d));
+    v[7][d] = (1.0 / 3.0) * (C(0, d) + C(1, d) + C(2, d));
+    v[8][d] = 0.5 * (C(2, d) + C(3, d));
+    v[9][d] = 0.5 * (C(3, d) + C(1, d));
+    v[10][d] = (1.0 / 3.0) * (C(1, d) + C(2, d) + C(3, d));
+    v[11][d] = 0.5 * (C(0, d) + C(3, d));
+    v[12][d] = (1.0 / 3.0) * (C(0, d) + C(2, d) + C(3, d));
+    v[13][d] = (1.0 / 3.0) * (C(0, d) + C(1, d) + C(3, d));
+    v[14][d] = 0.0;
+    for (int n = 0; n < 4; ++n) v[14][d] += 0.25 * C(n, d);
+  }
+#undef C
+  for (int ics = 0; ics < 6; ++ics) {
+    double q[4][3];
+    for (int i = 0; i < 4; ++i)
+      for (int d = 0; d < 3; ++d) q[i][d] = v[facet[ics][i]][d];
+    quad_area(q, &areav[3 * ics]);
+  }
+}
+
+/* WedSCS::determinant_scs, src/master_element/Wed6CVFEM.C:541-646 */
+void orc_wed_scs_areav(const double c[18], double areav[27])
+{
+  static const int facet[9][4] = {
+    {6, 9, 20, 16},  {7, 9, 20, 18},  {9, 8, 19, 20},
+    {10, 16, 20, 13}, {13, 11, 18, 20}, {12, 13, 20, 19},
+    {15, 16, 20, 19}, {16, 14, 18, 20}, {19, 20, 18, 17}};
+  double v[21][3];
+#define C(n, d) c[3 * (n) + (d)]
+  for (int d = 0; d < 3; ++d) {
+    for (int n = 0; n < 6; ++n) v[n][d] = C(n, d);
+    v[6][d] = 0.5 * (C(0, d) + C(1, d));
+    v[7][d] = 0.5 * (C(1, d) + C(2, d));
+    v[8][d] = 0.5 * (C(2, d) + C(0, d));
+    v[9][d] = (1.0 / 3.0) * (C(0, d) + C(1, d) + C(2, d));
+    v[10][d] = 0.5 * (C(3, d) + C(4, d));
+    v[11][d] = 0.5 * (C(4, d) + C(5, d));
+    v[12][d] = 0.5 * (C(5, d) + C(3, d));
+    v[13][d] = (1.0 / 3.0) * (C(3, d) + C(4, d) + C(5, d));
+    v[14][d] = 0.5 * (C(1, d) + C(4, d));
+    v[15][d] = 0.5 * (C(0, d) + C(3, d));
+    v[16][d] = 0.25 * (C(0, d) + C(1, d) + C(4, d) + C(3, d));
+    v[17][d] = 0.5 * (C(2, d) + C(5, d));
+    v[18][d] = 0.25 * (C(1, d) + C(4, d) + C(5, d) + C(2, d));
+    v[19][d] = 0.25 * (C(5, d) + C(3, d) + C(0, d) + C(2, d));
+    v[20][d] = 0.0;
+    for (int n = 0; n < 6; ++n) v[20][d] += (1.0 / 6.0) * C(n, d);
+  }
+#undef C
+  for (int ics = 0; ics < 9; ++ics) {
+    double q[4][3];
+    for (int i = 0; i < 4; ++i)
+      for (int d = 0; d < 3; ++d) q[i][d] = v[facet[ics][i]][d];
+    quad_area(q, &areav[3 * ics]);
+  }
+}
+
+/* include/master_element/Tet4CVFEM.h:266, Wed6CVFEM.h:268 */
+const int orc_tet_lrscv[12] = {0, 1, 1, 2, 0, 2, 0, 3, 1, 3, 2, 3};
+const int orc_wed_lrscv[18] = {0, 1, 1, 2, 0, 2, 3, 4, 4, 5,
+                               3, 5, 0, 3, 1, 4, 2, 5};
+
+/* edge of sorted (L, R) pairs by binary search, -1 if absent */
+static int64_t find_edge(const int32_t* edges, int64_t nedge, int32_t a,
+                         int32_t b)
+{
+  int64_t lo = 0, hi = nedge - 1;
+  while (lo <= hi) {
+    const int64_t m = (lo + hi) / 2;
+    const int32_t ea = edges[2 * m], eb = edges[2 * m + 1];
+    if (ea == a && eb == b) return m;
+    if (ea < a || (ea == a && eb < b))
+      lo = m + 1;
+    else
+      hi = m - 1;
+  }
+  return -1;
+}
+
+/* Edge area vectors of a mixed Hex8/Tet4/Wed6 mesh: the element loop of
+ * GeometryInteriorAlg::impl_compute_edge_area_vector
+ * (src/ngp_algorithms/GeometryInteriorAlg.C:163-225) with the SCS of each
+ * topology; edges sorted by (node[0], node[1]) with node[0] < node[1].
+ * type: 0 Hex8, 1 Tet4, 2 Wed6; conn [nelem][8] (unused slots ignored).
+ * Returns -1 if an element edge is not in the list. */
+int orc_edge_area_vec_mixed(int64_t nelem, const int32_t* type,
+                            const int32_t* conn, const double* coords,
+                            const int32_t* edges, int64_t nedge, double* areav)
+{
+  memset(areav, 0, sizeof(double) * 3 * (size_t)nedge);
+  for (int64_t el = 0; el < nelem; ++el) {
+    const int t = type[el];
+    const int npe = t == 0 ? 8 : (t == 1 ? 4 : 6);
+    const int nscs = t == 0 ? 12 : (t == 1 ? 6 : 9);
+    const int* lr = t == 0 ? orc_hex_lrscv : (t == 1 ? orc_tet_lrscv : orc_wed_lrscv);
+    double c[24], a[36];
+    for (int n = 0; n < npe; ++n)
+      for (int d = 0; d < 3; ++d) c[3 * n + d] = coords[3 * conn[8 * el + n] + d];
+    if (t == 0)
+      orc_hex_scs_areav(c, a);
+    else if (t == 1)
+      orc_tet_scs_areav(c, a);
+    else
+      orc_wed_scs_areav(c, a);
+    for (int ip = 0; ip < nscs; ++ip) {
+      const int32_t nl = conn[8 * el + lr[2 * ip]], nr = conn[8 * el + lr[2 * ip + 1]];
+      const int64_t e = find_edge(edges, nedge, nl < nr ? nl : nr, nl < nr ? nr : nl);
+      if (e < 0) return -1;
+      const double sgn = (nl == edges[2 * e]) ? 1.0 : -1.0;
+      for (int d = 0; d < 3; ++d) areav[3 * e + d] += a[3 * ip + d] * sgn;
+    }
+  }
+  return 0;
+}
+
 /* hex_volume_grandy, Hex8GeometryFunctions.h:85-160 */
 static double hex_volume_grandy(const double h[8][3])
 {

@@ -18,15 +18,26 @@ def nw():
 
 
 class Problem:
-    def __init__(self, nx, ny, nz, stretch=1.0, jitter=0.0, seed=0):
+    def __init__(self, nx, ny, nz, stretch=1.0, jitter=0.0, seed=0, mixed=None):
+        """Hex8 box (nx, ny, nz, stretch, jitter), or with mixed =
+        (f_hex, f_tet) an nx^3 mixed Hex8/Tet4/Wed6 box (RCM-numbered)."""
         m = nw()
-        self.coords, self.conn = m.hex_box(nx, ny, nz, stretch, jitter, seed)
-        self.n = self.coords.shape[0]
-        self.edges, self.elem_edges = m.hex_edges(self.conn, self.n)
+        if mixed is None:
+            self.coords, self.conn = m.hex_box(nx, ny, nz, stretch, jitter, seed)
+            self.n = self.coords.shape[0]
+            self.edges, self.elem_edges = m.hex_edges(self.conn, self.n)
+            self.areav = m.hex_edge_area_vector(self.conn, self.coords,
+                                                self.elem_edges, self.edges)
+            self.dualvol = m.hex_dual_volume(self.conn, self.coords)
+        else:
+            self.coords, self.types, self.conn = m.mixed_box(
+                nx, mixed[0], mixed[1], jitter, seed)
+            self.n = self.coords.shape[0]
+            self.edges = m.mixed_edges(self.types, self.conn, self.n)
+            self.areav = m.mixed_edge_area_vector(self.types, self.conn,
+                                                  self.coords, self.edges)
+            self.dualvol = np.ones(self.n)  # not read by the edge kernels
         self.ne = self.edges.shape[0]
-        self.areav = m.hex_edge_area_vector(self.conn, self.coords,
-                                            self.elem_edges, self.edges)
-        self.dualvol = m.hex_dual_volume(self.conn, self.coords)
         rng = np.random.default_rng(seed + 1000)
         n, ne = self.n, self.ne
         self.vel = rng.normal(size=(n, 3))

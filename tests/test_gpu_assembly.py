@@ -186,6 +186,34 @@ def test_momentum_matches_oracle(nw, ctx, skewed, kw):
     pb.assert_csr_close(ctx.download(), skewed.oracle_momentum(pb.to_orc_momentum(o)))
 
 
+@pytest.fixture(scope="module")
+def mixed():
+    """Mixed Hex8/Tet4/Wed6 box (60/30/10 % of the cells), jittered, nodes
+    permuted then RCM-numbered: irregular rows."""
+    return pb.Problem(9, 0, 0, jitter=0.15, seed=6, mixed=(0.6, 0.3))
+
+
+@pytest.mark.parametrize("kw", CONT_CASES[:2])
+def test_mixed_mesh_continuity_matches_oracle(nw, ctx, mixed, kw):
+    o = nw.continuity_defaults(**kw)
+    mixed.upload(ctx)
+    ctx.build_graph(1)
+    ctx.assemble_continuity(o)
+    pb.assert_csr_close(ctx.download(), mixed.oracle_continuity(pb.to_orc_continuity(o)))
+
+
+def test_mixed_mesh_scalar_and_momentum_match_oracle(nw, ctx, mixed):
+    mixed.upload(ctx)
+    ctx.build_graph(1)
+    o = _scal_opts(nw, dict(alpha=0.5, use_limiter=1))
+    ctx.assemble_scalar(o)
+    pb.assert_csr_close(ctx.download(), mixed.oracle_scalar(pb.to_orc_scalar(o)))
+    ctx.build_graph(3)
+    m = nw.momentum_defaults(alpha=0.5)
+    ctx.assemble_momentum(m)
+    pb.assert_csr_close(ctx.download(), mixed.oracle_momentum(pb.to_orc_momentum(m)))
+
+
 def test_uniform_32_cubed_scalar_config(nw, ctx):
     """32^3 uniform box, Taylor-Green velocity, mu = 1e-3, seeded phi with
     its edge nodal gradient: the scalar workload, against the oracle."""

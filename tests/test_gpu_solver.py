@@ -148,3 +148,30 @@ def test_amg_beats_jacobi_and_is_reused(nw, ctx):
     assert s2.converged and s2.setup_ms == 0 and s2.iters == s1.iters
     r = A.rhs - A.spmv(ctx.solution())
     assert np.linalg.norm(r) <= 1.0001e-8 * np.linalg.norm(A.rhs)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mixed_mesh_pressure_solve(nw, ctx, mode):
+    """Pressure system of a mixed Hex8/Tet4/Wed6 mesh (RCM-numbered): AMG-
+    preconditioned GMRES, FP64 and FP32-inner/FP64-refined, against the
+    dense solve."""
+    P = pb.Problem(10, 0, 0, jitter=0.1, seed=8, mixed=(0.6, 0.3))
+    P.upload(ctx)
+    ctx.build_graph(1)
+    o = nw.continuity_defaults()
+    ctx.assemble_continuity(o)
+    top = np.where(P.coords[:, 2] >= 1.0 - 1e-12)[0].astype(np.int32)
+    ctx.field_upload(nw.F_PRESSURE, np.zeros(P.n))
+    ctx.field_upload(nw.F_DIFF_FLUX_COEFF, np.zeros(P.n))
+    ctx.apply_dirichlet(top, nw.F_PRESSURE, nw.F_DIFF_FLUX_COEFF, 0, 1)
+    A = P.oracle_continuity(pb.to_orc_continuity(o))
+    A.apply_dirichlet(top, 0, 1, np.zeros(P.n), np.zeros(P.n))
+    xd = orc.dense_solve(A.dense(), A.rhs)
+    st = ctx.solve(nw.solver_cfg(kspace=50, max_iters=500, tol=1e-9,
+                                 precond=nw.PREC_AMG, mode=mode, inner_tol=1e-5,
+                                 reuse_preconditioner=0))
+    x = ctx.solution()
+    assert st.converged and st.final_rel <= 1e-9
+    r = A.rhs - A.spmv(x)
+    assert np.linalg.norm(r) <= 1.0001e-9 * np.linalg.norm(A.rhs)
+    np.testing.assert_allclose(x, xd, rtol=0, atol=1e-6 * np.abs(xd).max())
