@@ -179,14 +179,15 @@ def pressure_solves(ns, ntrain, neval, epochs, log):
         for k in range(1, ntrain + 1):
             S.step(k)
             ctx.solve(fp64(nw.X0_SOLUTION))
-            net.record()
+            if k > 2:  # the guess history is complete from step 3 on
+                net.record()
         t_gen = time.perf_counter() - t0
         t0 = time.perf_counter()
         loss = net.train(nw.mpnn_train_cfg(epochs=epochs, batch=1, lr=1e-3, seed=3))
         t_train = time.perf_counter() - t0
         res["learned_guess"] = {
             "network": "3 message-passing layers, 64 hidden, FP32",
-            "training_pairs": ntrain, "epochs": epochs,
+            "training_pairs": net.samples, "epochs": epochs,
             "final_training_loss": loss, "pair_generation_s": round(t_gen, 1),
             "training_s": round(t_train, 1)}
         log("training: %d pairs in %.1fs, %d epochs in %.1fs, loss %.4g" %

@@ -256,9 +256,9 @@ int nw_linsys_spmv(nw_ctx* ctx, const double* d_x, double* d_y);
 /* No counterpart in the reference, whose pressure solve always starts
  * from x0 = 0 (src/LinearSolver.C:209-254, Belos setProblem with a zero
  * LHS).  A message-passing network on the graph of the context's 1-dof
- * system predicts x = xb + s * delta from the features of (A, b, x0, xp),
- * where x0 is the previous-step solution, xp the one before it and
- * xb = 2 x0 - xp their linear extrapolation in time (xb = x0 without
+ * system predicts x = xb + s * delta from the features of (A, b, x0, xp,
+ * xpp), where x0 is the previous-step solution, xp and xpp the two before
+ * it and xb = 2 x0 - xp the linear extrapolation in time (xb = x0 without
  * history); rows without off-diagonal entries get b / a_nn.  GMRES to the requested tolerance
  * still decides the answer; a prediction whose residual is not below that
  * of x0 is discarded (nw_solve_stats.guess reports which guess was used).
@@ -275,18 +275,24 @@ typedef struct {
 
 /* Parameter count of a network with the given hidden width and layers. */
 int64_t nw_mpnn_nparams(int hidden, int layers);
-/* hidden in {32, 64}, layers in [1, 8]; seeded He-uniform initialisation. */
+/* hidden in {32, 64}, layers in [1, 8]; seeded He-uniform initialisation
+ * with a zero output layer (the untrained prediction is xb). */
 int nw_mpnn_create(nw_ctx* ctx, int hidden, int layers, uint64_t seed,
                    nw_mpnn** out);
 void nw_mpnn_destroy(nw_mpnn* net);
-/* Flat FP64 parameters: W_in[H][6], b_in[H], per layer W_self[H][H],
- * W_nbr[H][H], b[H], then w_out[H], b_out. */
+/* Flat FP64 parameters: W_in[H][5], b_in[H], per layer W_self[H][H],
+ * W_nbr[H][H], b[H], then w_out[2][H], b_out[2]: the network's two
+ * outputs are per-node coefficients of the history differences
+ * (x0 - xp)/s and (xp - xpp)/s added to xb. */
 int nw_mpnn_set_params(nw_mpnn* net, const double* params, int64_t n);
 int nw_mpnn_get_params(const nw_mpnn* net, double* params, int64_t n);
 /* Records one training pair from the context's last solve: the features
- * of (A, b, x0, xp) with x0 the guess that solve started from and xp the
- * guess of the solve before it, and the target (x - xb) / s with x the
- * current solution vector. */
+ * of (A, b, x0, xp, xpp) with x0 the guess that solve started from and
+ * xp, xpp the guesses of the two solves before it, and the target
+ * (x - xb) / s with x the current solution vector.  The guess history
+ * advances only on solves that start from the solution vector with a guess
+ * different from the last one (a solve from x0 = 0, or a second method on
+ * the same step from the same guess, leaves it unchanged). */
 int nw_mpnn_record(nw_mpnn* net);
 int nw_mpnn_samples(const nw_mpnn* net);
 int nw_mpnn_clear_samples(nw_mpnn* net);
@@ -298,8 +304,9 @@ int nw_mpnn_loss_grad(nw_mpnn* net, int sample, double* loss, double* grad);
 int nw_mpnn_train(nw_mpnn* net, const nw_mpnn_train_cfg* cfg,
                   double* last_loss);
 /* Prediction from the current system with x0 = the solution vector and
- * xp = the guess the last solve started from (x0 if none): writes the
- * prediction to x_host (nrows doubles) and the scale s to *scale. */
+ * xp, xpp = the guesses the last two solves started from (x0 if none):
+ * writes the prediction to x_host (nrows doubles) and the scale s to
+ * *scale. */
 int nw_mpnn_predict(nw_mpnn* net, double* x_host, double* scale);
 /* Network used by solves with x0 = NW_X0_LEARNED (NULL detaches). */
 int nw_linsys_set_learned_guess(nw_ctx* ctx, nw_mpnn* net);

@@ -7,18 +7,20 @@
 // side = 0 when n == L and 1 when n == R.  Node and edge fields are stored
 // component-major (SoA): component d of node n is f[d * nnode + n].
 //
-// Kernel shape ("row owner").  Each lane owns one row.  It loads its own
-// node's fields once, then walks the row's slots in column order: for each
-// off-diagonal slot it gathers the neighbour's fields and the edge's area
-// vector, evaluates the edge flux in registers exactly as the reference
-// lambda does for that edge, stores the (row, neighbour) block of the edge
-// LHS into the slot, and adds the edge's (row, row) block and rhs(row)
-// contributions to register accumulators in column order.  The diagonal
-// block and the rhs are written once at the end.  Every edge is therefore
-// evaluated twice (once from each end), each matrix entry is written once,
-// no atomics are used, and the result is bitwise reproducible.
+// Kernel shape ("row owner").  G lanes own one row (G = 2 for the 1-dof
+// systems, 1 for momentum).  Each lane loads the row node's fields, then
+// walks every G-th slot of the row: for each off-diagonal slot it gathers
+// the neighbour's fields (the slot's column) and the edge's area vector,
+// evaluates the edge flux in registers exactly as the reference lambda does
+// for that edge, stores the (row, neighbour) block of the edge LHS into the
+// slot, and adds the edge's (row, row) block and rhs(row) contributions to
+// register accumulators.  The G partial sums are combined by a fixed
+// butterfly and the diagonal block and rhs are written once.  Every edge is
+// therefore evaluated twice (once from each end), each matrix entry is
+// written once, no atomics are used, and the result is bitwise
+// reproducible.
 //
-// Wavefronts take chunks of 64 consecutive rows from a schedule sorted by
+// Rows are taken in chunks of 64 consecutive rows from a schedule sorted by
 // the Morton code of the chunk centroids, and workgroups are renumbered so
 // that each XCD works on one contiguous stretch of that schedule: rows
 // processed together are neighbours in space and their shared node data
@@ -365,46 +367,41 @@ __device__ inline int xcd_block(int nblocks_padded)
   return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
 }
 
-template <class P, int PIPE>
+// G lanes per row: lane g of a row evaluates the row's slots g, g+G, ...
+// (consecutive lanes touch consecutive slots, so the code/column loads and
+// the off-diagonal stores are coalesced), then the diagonal-block and rhs
+// contributions are summed over the G lanes by a fixed butterfly; the
+// result does not depend on scheduling.  A wave covers 64/G consecutive
+// rows of a scheduled chunk; G waves cover the chunk.
+template <class P, int G>
 __global__ __launch_bounds__(kBlock) void k_assemble_rows(
   P phys, const int32_t* __restrict__ row_ptr,
   const int32_t* __restrict__ code, const int32_t* __restrict__ cols,
-  const int2* __restrict__ sched, int nchunks, double* __restrict__ vals,
-  double* __restrict__ rhs, int accumulate)
+  const int32_t* __restrict__ diag, const int2* __restrict__ sched,
+  int nchunks, double* __restrict__ vals, double* __restrict__ rhs,
+  int accumulate)
 {
   constexpr int ND = P::NDOF, NB = ND * ND, NV = NB + ND;
+  constexpr int RPW = kWave / G;  // rows per wave
   const int lane = threadIdx.x & (kWave - 1);
   const int gw = xcd_block((int)gridDim.x) * (kBlock / kWave) +
                  (int)(threadIdx.x / kWave);
-  if (gw >= nchunks) return;
-  const int2 chunk = sched[gw];
-  const int row = chunk.x + lane;
-  if (row >= chunk.y) return;
-  typename P::Node me, ot;
-  phys.load(row, me);
+  const int chunk_id = gw / G;
+  if (chunk_id >= nchunks) return;
+  const int2 chunk = sched[chunk_id];
+  const int row = chunk.x + (gw % G) * RPW + lane / G;
+  const int g = lane & (G - 1);
+  const bool active = row < chunk.y;
   double acc[NV];
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  int dslot = -1;
-  const int s0 = row_ptr[row], s1 = row_ptr[row + 1];
-  // slot s: code c (diagonal < 0, else 2e + side) and column (the other
-  // node).  PIPE = 1 loads the next slot's neighbour record while the
-  // current edge is evaluated.
-  int cn = code[s0], on = cols[s0];
-  if (PIPE) {
-    if (cn >= 0) phys.load(on, ot);
-  }
-  for (int s = s0; s < s1; ++s) {
-    const int c = cn, o = on;
-    typename P::Node nx;
-    if (s + 1 < s1) {
-      cn = code[s + 1];
-      on = cols[s + 1];
-      if (PIPE && cn >= 0) phys.load(on, nx);
-    }
-    if (c < 0) {
-      dslot = s;
-    } else {
-      if (!PIPE) phys.load(o, ot);
+  if (active) {
+    typename P::Node me, ot;
+    phys.load(row, me);
+    const int s1 = row_ptr[row + 1];
+    for (int s = row_ptr[row] + g; s < s1; s += G) {
+      const int c = code[s];
+      if (c < 0) continue;  // the diagonal slot, written below
+      phys.load(cols[s], ot);
       const int e = c >> 1, side = c & 1;
       double off[NB], con[NV];
       if (side)
@@ -419,10 +416,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(
       }
       for (int v = 0; v < NV; ++v) acc[v] += con[v];
     }
-    if (PIPE) ot = nx;
   }
-  if (dslot >= 0) {
-    double* dp = vals + (int64_t)dslot * NB;
+  // butterfly over the G lanes of the row (all lanes take part)
+  for (int o = G / 2; o > 0; o >>= 1)
+    for (int v = 0; v < NV; ++v) acc[v] += __shfl_xor(acc[v], o, G);
+  if (active && g == 0) {
+    double* dp = vals + (int64_t)diag[row] * NB;
     double* rp = rhs + (int64_t)row * ND;
     if (accumulate) {
       for (int k = 0; k < NB; ++k) dp[k] += acc[k];
@@ -531,17 +530,15 @@ int launch_rows(Ctx* c, const P& phys, int accumulate)
   }
   if (g.nchunks == 0) return NW_OK;
   const int64_t wpb = kBlock / kWave;
-  int64_t blocks = (g.nchunks + wpb - 1) / wpb;
+  // lanes per row: 2 for the 1-dof systems (shorter dependent-load chains
+  // per lane, coalesced slot access); 1 for the 3-dof system, whose register
+  // footprint already limits occupancy
+  constexpr int G = P::NDOF == 1 ? 2 : 1;
+  int64_t blocks = (g.nchunks * G + wpb - 1) / wpb;
   blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
-  const char* var = getenv("NW_ASM_PIPE");
-  if (var && var[0] == '1')
-    k_assemble_rows<P, 1><<<(unsigned)blocks, kBlock, 0, c->stream>>>(
-      phys, g.row_ptr, g.code, g.cols, g.sched, (int)g.nchunks, g.vals, g.rhs,
-      accumulate);
-  else
-    k_assemble_rows<P, 0><<<(unsigned)blocks, kBlock, 0, c->stream>>>(
-      phys, g.row_ptr, g.code, g.cols, g.sched, (int)g.nchunks, g.vals, g.rhs,
-      accumulate);
+  k_assemble_rows<P, G><<<(unsigned)blocks, kBlock, 0, c->stream>>>(
+    phys, g.row_ptr, g.code, g.cols, g.diag, g.sched, (int)g.nchunks, g.vals,
+    g.rhs, accumulate);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }

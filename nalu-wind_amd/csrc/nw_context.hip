@@ -200,6 +200,7 @@ int nw_ctx_destroy(nw_ctx* c)
   amg_free(c->amg);
   (void)hipFree(c->x_init);
   (void)hipFree(c->x_hist);
+  (void)hipFree(c->x_hist2);
   free_graph(c->g);
   for (auto& f : c->field) (void)hipFree(f);
   (void)hipFree(c->edges);
@@ -245,6 +246,8 @@ int nw_mesh_set(nw_ctx* c, int64_t nnode, int64_t nedge,
   c->x_init = nullptr;
   (void)hipFree(c->x_hist);
   c->x_hist = nullptr;
+  (void)hipFree(c->x_hist2);
+  c->x_hist2 = nullptr;
   free_graph(c->g);
   for (auto& f : c->field) {
     (void)hipFree(f);
@@ -354,6 +357,8 @@ int nw_linsys_build_edge_graph(nw_ctx* c, int ndof)
   c->x_init = nullptr;
   (void)hipFree(c->x_hist);
   c->x_hist = nullptr;
+  (void)hipFree(c->x_hist2);
+  c->x_hist2 = nullptr;
   free_graph(c->g);
   Graph& g = c->g;
   const int64_t N = c->nnode, E = c->nedge;

@@ -89,6 +89,7 @@ struct Ctx {
   AmgDevice* amg = nullptr;
   double* x_init = nullptr;  // [nnode] guess the last solve started from
   double* x_hist = nullptr;  // [nnode] guess the solve before it started from
+  double* x_hist2 = nullptr; // [nnode] and the one before that
   Mpnn* mpnn = nullptr;      // network for NW_X0_LEARNED (not owned)
 };
 
@@ -100,10 +101,11 @@ int ensure_field(Ctx* c, int f, double** out);
 int require_field(Ctx* c, int f, const double** out);
 void free_krylov(Krylov* k);
 void amg_free(AmgDevice* a);
-// x = x0 + s_o * delta predicted from the context's system, x0 and the
-// guess before it xp (device vectors of nnode doubles); wall time in
+// Learned guess predicted from the context's system, x0 and the guesses
+// before it xp, xpp (device vectors of nnode doubles); wall time in
 // mpnn_last_ms.
-int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp, double* x);
+int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp,
+                        const double* xpp, double* x);
 double mpnn_last_ms(const Mpnn* m);
 
 inline unsigned grid_for(int64_t n, int block) {

@@ -276,6 +276,15 @@ __global__ void k_vmul(double* __restrict__ y, const double* __restrict__ d,
   if (i < n) y[i] *= d[i];
 }
 
+// *flag = 1 if a and b differ anywhere (flag cleared by the caller)
+__global__ void k_differs(const double* __restrict__ a,
+                          const double* __restrict__ b, int64_t n,
+                          int* __restrict__ flag)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && a[i] != b[i]) *flag = 1;
+}
+
 
 unsigned stream_grid(int64_t n)
 {
@@ -803,14 +812,36 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
   }
   double* x = c->g.sln;
   if (cfg->x0 == NW_X0_ZERO) NW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
-  // the guess this solve starts from and the one the previous solve started
-  // from (the learned guess's inputs and recorded training pairs)
-  if (!c->x_hist) NW_HIP(hipMalloc(&c->x_hist, sizeof(double) * n));
-  NW_HIP(hipMemcpyAsync(c->x_hist, c->x_init ? c->x_init : x,
-                        sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-  if (!c->x_init) NW_HIP(hipMalloc(&c->x_init, sizeof(double) * n));
-  NW_HIP(hipMemcpyAsync(c->x_init, x, sizeof(double) * n,
-                        hipMemcpyDeviceToDevice, st));
+  // History of starting guesses (the learned guess's inputs and recorded
+  // training pairs): x_init, x_hist, x_hist2 are the guesses of the last
+  // three solves that started from the solution vector with distinct
+  // guesses.  A solve from x0 = 0, or from the same guess as the last one
+  // (another method on the same step), leaves the history as it is.
+  if (cfg->x0 != NW_X0_ZERO) {
+    bool shift = true;
+    if (c->x_init) {
+      NW_HIP(hipMemsetAsync(k->bad, 0, sizeof(int), st));
+      k_differs<<<grid_for(n, kBlock), kBlock, 0, st>>>(x, c->x_init, n, k->bad);
+      NW_LAUNCH_CHECK();
+      int differs = 0;
+      NW_HIP(hipMemcpyAsync(&differs, k->bad, sizeof(int),
+                            hipMemcpyDeviceToHost, st));
+      NW_HIP(hipStreamSynchronize(st));
+      shift = differs != 0;
+    }
+    if (shift) {
+      if (!c->x_hist2) NW_HIP(hipMalloc(&c->x_hist2, sizeof(double) * n));
+      const double* older = c->x_hist ? c->x_hist : (c->x_init ? c->x_init : x);
+      NW_HIP(hipMemcpyAsync(c->x_hist2, older, sizeof(double) * n,
+                            hipMemcpyDeviceToDevice, st));
+      if (!c->x_hist) NW_HIP(hipMalloc(&c->x_hist, sizeof(double) * n));
+      NW_HIP(hipMemcpyAsync(c->x_hist, c->x_init ? c->x_init : x,
+                            sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      if (!c->x_init) NW_HIP(hipMalloc(&c->x_init, sizeof(double) * n));
+      NW_HIP(hipMemcpyAsync(c->x_init, x, sizeof(double) * n,
+                            hipMemcpyDeviceToDevice, st));
+    }
+  }
   double bb = 0;
   rc = S.dot64(c->g.rhs, c->g.rhs, &bb);
   if (rc) return rc;
@@ -824,7 +855,7 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
     // the network's prediction from (A, b, x0); kept only if it lowers the
     // residual, otherwise GMRES starts from x0 (the previous-step guess)
     if (!k->xg) NW_HIP(hipMalloc(&k->xg, sizeof(double) * n));
-    rc = mpnn_predict_device(c->mpnn, c->x_init, c->x_hist, k->xg);
+    rc = mpnn_predict_device(c->mpnn, c->x_init, c->x_hist, c->x_hist2, k->xg);
     out->guess_ms = mpnn_last_ms(c->mpnn);
     double rl = 0;
     if (rc == NW_OK) rc = S.residual(k->xg, &rl);

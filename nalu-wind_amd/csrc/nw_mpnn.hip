@@ -3,18 +3,24 @@
 // A message-passing network on the graph of the assembled 1-dof matrix
 // predicts the change of the solution from the previous-step guess x0:
 //
-//   features f_n = (b/s_b, x0/s_x, r/s_r, log|a_nn| - mean, dirichlet,
-//                   (x0 - xp)/s_d),  r = b - A x0, xp the guess before x0
-//                   (two steps back), s_* the RMS of each vector
+//   features f_n = (r/s_r, log|a_nn| - mean, dirichlet, (x0 - xp)/s_d,
+//                   (xp - xpp)/s_d),  r = b - A x0, xp and xpp the guesses
+//                   of the two solves before (two and three steps back),
+//                   s_* the RMS of each vector, s_d of x0 - xp; differences
+//                   only, so what is learned on earlier steps carries over
 //   h0      = relu(W_in f + b_in)
 //   m_l     = sum_{m != n} w_nm h_{l-1,m},  w_nm = sqrt|a_nm a_mn| / sqrt|a_nn a_mm|
 //   h_l     = relu(W_self,l h_{l-1} + W_nbr,l m_l + b_l),   l = 1..L
-//   delta_n = w_out . h_L + b_out
+//   a_k,n   = w_out,k . h_L + b_out,k                   k = 1, 2
+//   delta_n = a_1,n f_3,n + a_2,n f_4,n   (the two history features)
 //   x       = xb + s_o delta,  xb = x0 + (x0 - xp) (linear extrapolation
 //             in time; xb = x0 without history), s_o = s_d, or
 //             s_e = rms(r / diag) without history
-// so the network learns the correction to the extrapolated guess (a skip
-// connection from the history feature to the output).  Rows without
+// so the network learns per-node coefficients of the history differences
+// on top of the linear extrapolation: a multi-step predictor whose output
+// vanishes with the history and has no offset of its own (an offset would be
+// the smoothest, slowest error for GMRES).  The output layer starts at zero,
+// so an untrained network gives the extrapolation.  Rows without
 // off-diagonal entries (Dirichlet rows) are decoupled: their prediction is
 // b_n / a_nn exactly, so no boundary error enters the guess.
 //
@@ -33,8 +39,8 @@
 //
 // Layouts: node activations are node-major [n][H] float rows, so a
 // neighbour's whole row is one contiguous 256-byte read; features are SoA
-// [6][n].  Parameters are the flat vector W_in[H][6], b_in[H], per layer
-// W_self[H][H], W_nbr[H][H], b[H], then w_out[H], b_out; the forward reads a
+// [5][n].  Parameters are the flat vector W_in[H][5], b_in[H], per layer
+// W_self[H][H], W_nbr[H][H], b[H], then w_out[2][H], b_out[2]; the forward reads a
 // transposed copy (k-major) so that the H outputs of one input are
 // consecutive and wave-uniform.
 #include <algorithm>
@@ -50,7 +56,7 @@ namespace nw {
 
 namespace {
 
-constexpr int kF = 6;           // node features
+constexpr int kF = 5;           // node features
 constexpr int kTileN = 32;      // nodes per staging tile (weight gradients)
 constexpr int kGradBlocks = 512;
 constexpr int kStats = 6;       // feature statistics
@@ -154,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_feat_stats(
 }
 
 struct FeatScale {
-  double isb, isx, isr, mlog, isd, iso;
+  double isr, mlog, isd, iso;
   double cb;  // 1 with history (xb = 2 x0 - xp), 0 without (xb = x0)
 };
 
@@ -166,6 +172,7 @@ __global__ void k_features(const int32_t* __restrict__ rp,
                            const double* __restrict__ b,
                            const double* __restrict__ x0,
                            const double* __restrict__ xp,
+                           const double* __restrict__ xpp,
                            const double* __restrict__ r,
                            const double* __restrict__ xs, FeatScale sc,
                            float* __restrict__ feat, float* __restrict__ target,
@@ -177,12 +184,11 @@ __global__ void k_features(const int32_t* __restrict__ rp,
   int dir = 1;
   for (int s = rp[i]; s < rp[i + 1]; ++s)
     if (ci[s] != i && va[s] != 0.0) dir = 0;
-  feat[i] = (float)(b[i] * sc.isb);
-  feat[(int64_t)n + i] = (float)(x0[i] * sc.isx);
-  feat[2 * (int64_t)n + i] = (float)(r[i] * sc.isr);
-  feat[3 * (int64_t)n + i] = (float)(log(d > 0 ? d : 1.0) - sc.mlog);
-  feat[4 * (int64_t)n + i] = (float)dir;
-  feat[5 * (int64_t)n + i] = (float)((x0[i] - xp[i]) * sc.isd);
+  feat[i] = (float)(r[i] * sc.isr);
+  feat[(int64_t)n + i] = (float)(log(d > 0 ? d : 1.0) - sc.mlog);
+  feat[2 * (int64_t)n + i] = (float)dir;
+  feat[3 * (int64_t)n + i] = (float)((x0[i] - xp[i]) * sc.isd);
+  feat[4 * (int64_t)n + i] = (float)((xp[i] - xpp[i]) * sc.isd);
   if (xs) {
     const double xb = x0[i] + sc.cb * (x0[i] - xp[i]);
     target[i] = (float)((xs[i] - xb) * sc.iso);
@@ -263,28 +269,32 @@ __global__ __launch_bounds__(kBlock) void k_mp_layer(
                           relu(acc[4 * j4 + 2]), relu(acc[4 * j4 + 3]));
 }
 
+// delta = a1 f3 + a2 f4 with a_k = w_out,k . h + b_out,k
 template <int H>
 __global__ void k_output(const float* __restrict__ hL,
-                         const float* __restrict__ wout, float bout,
+                         const float* __restrict__ wout, float b1, float b2,
+                         const float* __restrict__ f3,
+                         const float* __restrict__ f4,
                          float* __restrict__ delta, int n)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4* h = reinterpret_cast<const float4*>(hL + (int64_t)i * H);
-  float s = bout;
+  float a1 = b1, a2 = b2;
 #pragma unroll
   for (int j4 = 0; j4 < H / 4; ++j4) {
     const float4 v = h[j4];
-    s = fmaf(wout[4 * j4], v.x, s);
-    s = fmaf(wout[4 * j4 + 1], v.y, s);
-    s = fmaf(wout[4 * j4 + 2], v.z, s);
-    s = fmaf(wout[4 * j4 + 3], v.w, s);
+    const float hv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a1 = fmaf(wout[4 * j4 + q], hv[q], a1);
+      a2 = fmaf(wout[H + 4 * j4 + q], hv[q], a2);
+    }
   }
-  delta[i] = s;
+  delta[i] = a1 * f3[i] + a2 * f4[i];
 }
 
-// x = x0 + s_o delta
-// x = xb + s_o delta; Dirichlet rows (feature 4 set) get b / a_nn
+// x = xb + s_o delta; Dirichlet rows (feature 2 set) get b / a_nn
 __global__ void k_apply_guess(const double* __restrict__ x0,
                               const double* __restrict__ xp,
                               const float* __restrict__ delta,
@@ -303,17 +313,21 @@ __global__ void k_apply_guess(const double* __restrict__ x0,
   }
 }
 
-// dd = coef (delta - target); partial[blk * nv + off] = sum (delta - t)^2
+// dd2[i] = coef (delta - target) (f3, f4), the loss gradient at the two
+// output coefficients; partial[blk * nv + off] = sum (delta - t)^2
 __global__ __launch_bounds__(kBlock) void k_loss(
   const float* __restrict__ delta, const float* __restrict__ target,
-  float coef, float* __restrict__ dd, int n, double* __restrict__ partial,
-  int nv, int off)
+  const float* __restrict__ f3, const float* __restrict__ f4, float coef,
+  float* __restrict__ dd2, int n, double* __restrict__ partial, int nv,
+  int off)
 {
   double acc[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
     const float e = delta[i] - target[i];
-    dd[i] = coef * e;
+    const float d = coef * e;
+    dd2[2 * (int64_t)i] = d * f3[i];
+    dd2[2 * (int64_t)i + 1] = d * f4[i];
     acc[0] += (double)e * (double)e;
   }
   __shared__ double tot[1];
@@ -321,10 +335,10 @@ __global__ __launch_bounds__(kBlock) void k_loss(
   if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nv + off] = tot[0];
 }
 
-// gz[i][j] = (hL[i][j] > 0) ? dd[i] wout[j] : 0
+// gz[i][j] = (hL[i][j] > 0) ? dd2[i] . (wout[0][j], wout[1][j]) : 0
 template <int H>
 __global__ void k_output_back(const float* __restrict__ hL,
-                              const float* __restrict__ dd,
+                              const float* __restrict__ dd2,
                               const float* __restrict__ wout,
                               float* __restrict__ gz, int n)
 {
@@ -332,12 +346,14 @@ __global__ void k_output_back(const float* __restrict__ hL,
   if (t >= (int64_t)n * (H / 4)) return;
   const int i = (int)(t / (H / 4)), j4 = (int)(t % (H / 4));
   const float4 h = reinterpret_cast<const float4*>(hL + (int64_t)i * H)[j4];
-  const float d = dd[i];
+  const float d1 = dd2[2 * (int64_t)i], d2 = dd2[2 * (int64_t)i + 1];
+  const float* w1 = wout + 4 * j4;
+  const float* w2 = wout + H + 4 * j4;
   float4 g;
-  g.x = h.x > 0.f ? d * wout[4 * j4] : 0.f;
-  g.y = h.y > 0.f ? d * wout[4 * j4 + 1] : 0.f;
-  g.z = h.z > 0.f ? d * wout[4 * j4 + 2] : 0.f;
-  g.w = h.w > 0.f ? d * wout[4 * j4 + 3] : 0.f;
+  g.x = h.x > 0.f ? d1 * w1[0] + d2 * w2[0] : 0.f;
+  g.y = h.y > 0.f ? d1 * w1[1] + d2 * w2[1] : 0.f;
+  g.z = h.z > 0.f ? d1 * w1[2] + d2 * w2[2] : 0.f;
+  g.w = h.w > 0.f ? d1 * w1[3] + d2 * w2[3] : 0.f;
   reinterpret_cast<float4*>(gz + (int64_t)i * H)[j4] = g;
 }
 
@@ -411,7 +427,7 @@ enum { kModeLayer = 0, kModeInput = 1, kModeOutput = 2 };
 
 template <int H, int MODE>
 struct WgradShape {
-  static constexpr int JA = MODE == kModeOutput ? 1 : H;
+  static constexpr int JA = MODE == kModeOutput ? 2 : H;
   static constexpr int KB =
     MODE == kModeLayer ? 2 * H + 1 : (MODE == kModeInput ? kF + 1 : H + 1);
   static constexpr int NOUT = JA * KB;
@@ -420,7 +436,7 @@ struct WgradShape {
 // Weight gradients as node sums of outer products A_i B_i^T:
 //   layer : A = gz [n][H], B = [h_prev | m | 1], m recomputed from h_prev
 //   input : A = gz0 [n][H], B = [features | 1]
-//   output: A = dd [n],     B = [h_L | 1]
+//   output: A = dd2 [n][2], B = [h_L | 1]
 // Block b sums a contiguous node range in tiles of kTileN staged in LDS
 // (FP32 products within a tile, FP64 across tiles) and writes
 // partial[b * nv + off + o], o = j * KB + k.
@@ -544,8 +560,8 @@ struct Off {
     l0 = bin + H;
     ls = 2 * (int64_t)H * H + H;
     wout = l0 + L * ls;
-    bout = wout + H;
-    n = bout + 1;
+    bout = wout + 2 * H;
+    n = bout + 2;
   }
   int64_t ws(int l) const { return l0 + l * ls; }
   int64_t wn(int l, int H) const { return ws(l) + (int64_t)H * H; }
@@ -595,7 +611,7 @@ int ensure_buffers(Mpnn* m, bool training)
     NW_HIP(hipMalloc(&m->feat, sizeof(float) * kF * n));
     NW_HIP(hipMalloc(&m->target, sizeof(float) * n));
     NW_HIP(hipMalloc(&m->delta, sizeof(float) * n));
-    NW_HIP(hipMalloc(&m->dd, sizeof(float) * n));
+    NW_HIP(hipMalloc(&m->dd, sizeof(float) * 2 * n));
     NW_HIP(hipMalloc(&m->r, sizeof(double) * n));
     NW_HIP(hipMalloc(&m->partial, sizeof(double) * (size_t)kGradBlocks * m->nv));
     NW_HIP(hipMalloc(&m->red, sizeof(double) * m->nv));
@@ -626,9 +642,9 @@ int slot_weights(Mpnn* m)
   return NW_OK;
 }
 
-// features of (A, b, x0, xp) into feat; target from xs when given
-int features(Mpnn* m, const double* x0, const double* xp, const double* xs,
-             float* feat, float* target)
+// features of (A, b, x0, xp, xpp) into feat; target from xs when given
+int features(Mpnn* m, const double* x0, const double* xp, const double* xpp,
+             const double* xs, float* feat, float* target)
 {
   Ctx* c = m->c;
   const Graph& g = c->g;
@@ -648,8 +664,6 @@ int features(Mpnn* m, const double* x0, const double* xp, const double* xs,
     return v > 0 ? v : 1.0;
   };
   FeatScale sc;
-  sc.isb = 1.0 / rms(s[0]);
-  sc.isx = 1.0 / rms(s[1]);
   sc.isr = 1.0 / rms(s[2]);
   sc.mlog = s[4] / (double)std::max(n, 1);
   sc.isd = 1.0 / rms(s[5]);
@@ -660,8 +674,8 @@ int features(Mpnn* m, const double* x0, const double* xp, const double* xs,
   NW_REQUIRE(std::isfinite(s[0] + s[1] + s[2] + s[3] + s[4] + s[5]),
              NW_E_NONFINITE, "non-finite input to the learned guess");
   k_features<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.row_ptr, g.vals, g.cols,
-                                                    g.diag, g.rhs, x0, xp, m->r,
-                                                    xs, sc, feat, target, n);
+                                                    g.diag, g.rhs, x0, xp, xpp,
+                                                    m->r, xs, sc, feat, target, n);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }
@@ -688,9 +702,9 @@ int forward_h(Mpnn* m, const float* feat, bool keep)
     NW_LAUNCH_CHECK();
     cur = nxt;
   }
-  float bout = (float)m->P[o.bout];
-  k_output<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(m->h[cur], m->dP + o.wout,
-                                                      bout, m->delta, n);
+  k_output<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(
+    m->h[cur], m->dP + o.wout, (float)m->P[o.bout], (float)m->P[o.bout + 1],
+    feat + 3 * (int64_t)n, feat + 4 * (int64_t)n, m->delta, n);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }
@@ -713,9 +727,9 @@ int loss_grad_h(Mpnn* m, const Sample& smp, double scale, double* loss,
   int rc = forward_h<H>(m, smp.feat, true);
   if (rc) return rc;
   const int off_loss = nv - 1;
-  k_loss<<<kGradBlocks, kBlock, 0, st>>>(m->delta, smp.target,
-                                        (float)(2.0 * scale / n), m->dd, n,
-                                        m->partial, nv, off_loss);
+  k_loss<<<kGradBlocks, kBlock, 0, st>>>(
+    m->delta, smp.target, smp.feat + 3 * (int64_t)n, smp.feat + 4 * (int64_t)n,
+    (float)(2.0 * scale / n), m->dd, n, m->partial, nv, off_loss);
   NW_LAUNCH_CHECK();
   // value offsets: output block, then layers L-1..0, then input block
   int off = 0;
@@ -770,7 +784,9 @@ void build_gmap(Mpnn* m)
   const Off o(H, L);
   m->gmap.clear();
   using SO = WgradShape<H, kModeOutput>;
-  for (int k = 0; k < SO::KB; ++k) m->gmap.push_back(k < H ? o.wout + k : o.bout);
+  for (int j = 0; j < SO::JA; ++j)
+    for (int k = 0; k < SO::KB; ++k)
+      m->gmap.push_back(k < H ? o.wout + (int64_t)j * H + k : o.bout + j);
   using SL = WgradShape<H, kModeLayer>;
   for (int l = L - 1; l >= 0; --l)
     for (int j = 0; j < SL::JA; ++j)
@@ -800,9 +816,10 @@ void mpnn_detach(Ctx* c, Mpnn* m)
   if (c && c->mpnn == m) c->mpnn = nullptr;
 }
 
-// Prediction for the solver: x = x0 + s_o delta from the current system,
-// x0 and the guess before it xp (device vectors).
-int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp, double* x)
+// Prediction for the solver: x = xb + s_o delta from the current system,
+// x0 and the guesses before it xp, xpp (device vectors).
+int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp,
+                        const double* xpp, double* x)
 {
   const auto t0 = std::chrono::steady_clock::now();
   int rc = check_graph(m);
@@ -811,13 +828,13 @@ int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp, double* x)
   if (rc) return rc;
   rc = slot_weights(m);
   if (rc) return rc;
-  rc = features(m, x0, xp, nullptr, m->feat, nullptr);
+  rc = features(m, x0, xp, xpp, nullptr, m->feat, nullptr);
   if (rc) return rc;
   rc = forward(m, m->feat, false);
   if (rc) return rc;
   const Graph& g = m->c->g;
   k_apply_guess<<<grid_for(m->n, kBlock), kBlock, 0, m->c->stream>>>(
-    x0, xp, m->delta, m->feat + 4 * m->n, m->so, m->cb, g.rhs, g.vals, g.diag,
+    x0, xp, m->delta, m->feat + 2 * m->n, m->so, m->cb, g.rhs, g.vals, g.diag,
     x, (int)m->n);
   NW_LAUNCH_CHECK();
   NW_HIP(hipStreamSynchronize(m->c->stream));
@@ -874,8 +891,7 @@ int nw_mpnn_create(nw_ctx* c, int hidden, int layers, uint64_t seed,
     fill(o.ws(l), (int64_t)hidden * hidden, 2 * hidden);
     fill(o.wn(l, hidden), (int64_t)hidden * hidden, 2 * hidden);
   }
-  fill(o.wout, hidden, hidden);
-  for (int j = 0; j < hidden; ++j) m->P[o.wout + j] *= 0.1;
+  // output layer zero: the untrained prediction is the extrapolated guess
   if (hidden == 64) build_gmap<64>(m);
   else build_gmap<32>(m);
   int rc = NW_OK;
@@ -961,7 +977,8 @@ int nw_mpnn_record(nw_mpnn* m)
     set_error("device allocation of a training sample failed");
     return NW_E_ALLOC;
   }
-  rc = features(m, m->c->x_init, m->c->x_hist, m->c->g.sln, s.feat, s.target);
+  rc = features(m, m->c->x_init, m->c->x_hist, m->c->x_hist2, m->c->g.sln,
+                s.feat, s.target);
   if (rc) {
     free_sample(s);
     return rc;
@@ -1060,9 +1077,11 @@ int nw_mpnn_predict(nw_mpnn* m, double* x_host, double* scale)
   const int64_t n = m->n;
   double* x = nullptr;
   NW_HIP(hipMalloc(&x, sizeof(double) * n));
-  // x0 = the solution vector, xp = the guess the last solve started from
+  // x0 = the solution vector, xp / xpp = the guesses the last two solves
+  // started from
   const double* xp = m->c->x_init ? m->c->x_init : m->c->g.sln;
-  int rc = mpnn_predict_device(m, m->c->g.sln, xp, x);
+  const double* xpp = m->c->x_hist ? m->c->x_hist : xp;
+  int rc = mpnn_predict_device(m, m->c->g.sln, xp, xpp, x);
   if (!rc) {
     if (hipMemcpy(x_host, x, sizeof(double) * n, hipMemcpyDeviceToHost) !=
         hipSuccess) {

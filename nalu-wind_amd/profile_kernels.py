@@ -6,6 +6,7 @@ HBM byte counters.  Meant to run under rocprofv3, e.g.
         nalu-wind_amd/profile_kernels.py --kernel continuity --n 256
 
     python3 nalu-wind_amd/profile_kernels.py --summarize OUT_FETCH OUT_WRITE
+    python3 nalu-wind_amd/profile_kernels.py --stats OUT/x_kernel_stats.csv
 """
 import argparse
 import ctypes as C
@@ -54,6 +55,28 @@ def run(kernel, n, reps, copy_mib):
           (kernel, n, reps, copy_mib, W.n, W.ne), flush=True)
 
 
+def short_name(name):
+    """Kernel name without argument list and namespaces, templates kept."""
+    n = name.replace("void ", "")
+    for ns in ("(anonymous namespace)::", "nw::"):
+        n = n.replace(ns, "")
+    return n.split("(")[0]
+
+
+def stats(files, top):
+    """Per-kernel call count and average duration from rocprofv3
+    kernel_stats.csv files."""
+    import csv
+    for f in files:
+        rows = list(csv.DictReader(open(f)))
+        rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+        for r in rows[:top]:
+            print("  %-70s calls=%5d avg_us=%10.2f min_us=%10.2f max_us=%10.2f" % (
+                short_name(r["Name"])[:70], int(r["Calls"]),
+                float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
+                float(r["MaxNs"]) / 1e3))
+
+
 def summarize(dirs):
     """Average counter value (KiB) per dispatch for each kernel name."""
     import csv
@@ -62,7 +85,7 @@ def summarize(dirs):
                            recursive=True):
             acc = {}
             for r in csv.DictReader(open(f)):
-                key = (r["Kernel_Name"].split("(")[0][-60:], r["Counter_Name"])
+                key = (short_name(r["Kernel_Name"])[:60], r["Counter_Name"])
                 acc.setdefault(key, []).append(float(r["Counter_Value"]))
             for (k, cn), v in sorted(acc.items()):
                 print("%-62s %-12s dispatches=%4d avg=%.1f" % (k, cn, len(v),
@@ -76,8 +99,12 @@ if __name__ == "__main__":
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--summarize", nargs="*")
+    ap.add_argument("--stats", nargs="*")
+    ap.add_argument("--top", type=int, default=12)
     a = ap.parse_args()
-    if a.summarize:
+    if a.stats:
+        stats(a.stats, a.top)
+    elif a.summarize:
         summarize(a.summarize)
     else:
         run(a.kernel, a.n, a.reps, a.copy_mib)

@@ -325,12 +325,14 @@ double orc_jacobi_lambda_max(const orc_csr* A, int iters, uint64_t seed);
 /* Learned initial guess: message-passing network (orc_mpnn.c)         */
 /* ------------------------------------------------------------------ */
 
+/* H*5 + H + L*(2H^2 + H) + 2H + 2 */
 int64_t orc_mpnn_nparams(int H, int L);
 /* w[nnz]: sqrt(|a_nm a_mn|)/sqrt(|a_nn a_mm|) off the diagonal, 0 on it */
 void orc_mpnn_slot_weights(const orc_csr* A, double* w);
-/* feat[N][6]; scales = {s_b, s_x, s_r, s_e, s_o} (s_o: output scale) */
+/* feat[N][5]; scales = {s_b, s_x, s_r, s_e, s_o} (s_o: output scale) */
 void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
-                       const double* xp, double* feat, double* scales);
+                       const double* xp, const double* xpp, double* feat,
+                       double* scales);
 void orc_mpnn_forward(const double* P, int H, int L, const orc_csr* A,
                       const double* w, const double* feat, double* delta,
                       double* acts);

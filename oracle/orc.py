@@ -419,7 +419,7 @@ lib.orc_mpnn_nparams.argtypes = [C.c_int, C.c_int]
 lib.orc_mpnn_nparams.restype = C.c_int64
 lib.orc_mpnn_slot_weights.argtypes = [C.POINTER(Csr), _f64p]
 lib.orc_mpnn_features.argtypes = [C.POINTER(Csr), _f64p, _f64p, _f64p, _f64p,
-                                  _f64p]
+                                  _f64p, _f64p]
 lib.orc_mpnn_forward.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr), _f64p,
                                  _f64p, _f64p, C.c_void_p]
 lib.orc_mpnn_loss_grad.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr),
@@ -437,16 +437,17 @@ def mpnn_slot_weights(A):
     return w
 
 
-MPNN_NF = 6
+MPNN_NF = 5
 
 
-def mpnn_features(A, b, x0, xp=None):
-    """Node features and scales {s_b, s_x, s_r, s_e, s_o}; xp (the guess
-    before x0) defaults to x0 (no history)."""
+def mpnn_features(A, b, x0, xp=None, xpp=None):
+    """Node features and scales {s_b, s_x, s_r, s_e, s_o}; xp and xpp (the
+    guesses before x0) default to x0 and xp (no history)."""
     feat = np.zeros((A.c.nrows, MPNN_NF))
     sc = np.zeros(5)
-    lib.orc_mpnn_features(A.ref, f64(b), f64(x0), f64(x0 if xp is None else xp),
-                          feat, sc)
+    xp = x0 if xp is None else xp
+    xpp = xp if xpp is None else xpp
+    lib.orc_mpnn_features(A.ref, f64(b), f64(x0), f64(xp), f64(xpp), feat, sc)
     return feat, sc
 
 

@@ -6,20 +6,26 @@
  * kernels are checked against, and its analytic gradient is pinned by
  * finite differences in tests/test_oracle.py.
  *
- * Network (F = 6 inputs, H hidden, L message-passing layers):
+ * Network (F = 5 inputs, H hidden, L message-passing layers):
  *   h0      = relu(W_in f + b_in)
  *   m_l     = sum_{m in N(n)} w_nm h_{l-1,m}
  *   h_l     = relu(W_self,l h_{l-1} + W_nbr,l m_l + b_l),   l = 1..L
- *   delta_n = w_out . h_L + b_out
+ *   a_k,n   = w_out,k . h_L + b_out,k,   k = 1, 2
+ *   delta_n = a_1,n f_3,n + a_2,n f_4,n   (f_3, f_4 the history features)
  * Edge weights w_nm = sqrt(|a_nm a_mn|) / sqrt(|a_nn a_mm|) (symmetric).
- * Node features: b/s_b, x0/s_x, r/s_r (r = b - A x0), log|a_nn| - mean,
- * Dirichlet flag, (x0 - xp)/s_d with xp the guess before x0 (the solution
- * two steps back) and s_d = rms(x0 - xp).  Prediction x = xb + s_o delta
+ * Node features: r/s_r (r = b - A x0), log|a_nn| - mean, Dirichlet flag,
+ * (x0 - xp)/s_d and (xp - xpp)/s_d with xp, xpp the guesses of the two
+ * solves before (the solutions two and three steps back) and
+ * s_d = rms(x0 - xp): differences only, no absolute levels, so that what
+ * is learned on earlier steps carries over to later ones.  Prediction x = xb + s_o delta
  * with xb = 2 x0 - xp and s_o = s_d, or xb = x0 and s_o = s_e =
  * rms(r / diag) when x0 == xp (no history); rows without off-diagonal
  * entries take b / a_nn.
+ * The network predicts per-node coefficients of the two history
+ * differences (a learned multi-step extrapolation in time), so its output
+ * vanishes with the history and carries no offset of its own.
  * Parameters, flat: W_in[H][F], b_in[H], then per layer W_self[H][H],
- * W_nbr[H][H], b[H], then w_out[H], b_out.
+ * W_nbr[H][H], b[H], then w_out[2][H], b_out[2].
  */
 #include <math.h>
 #include <stdlib.h>
@@ -27,11 +33,11 @@
 
 #include "orc.h"
 
-#define NF 6
+#define NF 5
 
 int64_t orc_mpnn_nparams(int H, int L)
 {
-  return (int64_t)H * NF + H + (int64_t)L * (2 * H * H + H) + H + 1;
+  return (int64_t)H * NF + H + (int64_t)L * (2 * H * H + H) + 2 * H + 2;
 }
 
 static double rms(const double* v, int64_t n)
@@ -67,7 +73,8 @@ void orc_mpnn_slot_weights(const orc_csr* A, double* w)
 }
 
 void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
-                       const double* xp, double* feat, double* scales)
+                       const double* xp, const double* xpp, double* feat,
+                       double* scales)
 {
   const int64_t n = A->nrows;
   double* r = (double*)malloc(sizeof(double) * (size_t)n);
@@ -91,12 +98,11 @@ void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
     int dir = 1;
     for (int64_t k = A->row_ptr[i]; k < A->row_ptr[i + 1]; ++k)
       if (A->cols[k] != i && A->vals[k] != 0.0) dir = 0;
-    feat[NF * i + 0] = b[i] / sb;
-    feat[NF * i + 1] = x0[i] / sx;
-    feat[NF * i + 2] = r[i] / sr;
-    feat[NF * i + 3] = log(a > 0 ? a : 1.0) - mlog;
-    feat[NF * i + 4] = (double)dir;
-    feat[NF * i + 5] = d[i] / sd;
+    feat[NF * i + 0] = r[i] / sr;
+    feat[NF * i + 1] = log(a > 0 ? a : 1.0) - mlog;
+    feat[NF * i + 2] = (double)dir;
+    feat[NF * i + 3] = d[i] / sd;
+    feat[NF * i + 4] = (xp[i] - xpp[i]) / sd;
   }
   scales[0] = sb;
   scales[1] = sx;
@@ -163,12 +169,14 @@ void orc_mpnn_forward(const double* P, int H, int L, const orc_csr* A,
     h2 = t;
     Pl = bl + H;
   }
-  const double* wout = Pl;
-  const double bout = wout[H];
+  const double* wout = Pl; /* [2][H], then b_out[2] */
   for (int64_t i = 0; i < n; ++i) {
-    double s = bout;
-    for (int f = 0; f < H; ++f) s += wout[f] * h[H * i + f];
-    delta[i] = s;
+    double a1 = wout[2 * H], a2 = wout[2 * H + 1];
+    for (int f = 0; f < H; ++f) {
+      a1 += wout[f] * h[H * i + f];
+      a2 += wout[H + f] * h[H * i + f];
+    }
+    delta[i] = a1 * feat[NF * i + 3] + a2 * feat[NF * i + 4];
   }
   free(h);
   free(h2);
@@ -208,11 +216,14 @@ double orc_mpnn_loss_grad(const double* P, int H, int L, const orc_csr* A,
   double* gm = (double*)malloc(sizeof(double) * (size_t)(n * H));
   const double* zL = acts + n * H * (2 * L);
   for (int64_t i = 0; i < n; ++i) {
-    gwout[H] += dd[i];
+    const double g1 = dd[i] * feat[NF * i + 3], g2 = dd[i] * feat[NF * i + 4];
+    gwout[2 * H] += g1;
+    gwout[2 * H + 1] += g2;
     for (int f = 0; f < H; ++f) {
       const double hf = zL[H * i + f] > 0 ? zL[H * i + f] : 0.0;
-      gwout[f] += dd[i] * hf;
-      g[H * i + f] = dd[i] * wout[f];
+      gwout[f] += g1 * hf;
+      gwout[H + f] += g2 * hf;
+      g[H * i + f] = g1 * wout[f] + g2 * wout[H + f];
     }
   }
   for (int l = L - 1; l >= 0; --l) {

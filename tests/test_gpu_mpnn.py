@@ -49,22 +49,23 @@ def _params(nw, seed, scale=0.3):
 
 
 def _solved_pair(nw, ctx, A, seed):
-    """Two solves, from xp and then from x0 (both random): afterwards x0 is
-    the guess of the last solve, xp the guess of the one before, x the FP64
-    solution."""
+    """Three solves, from xpp, xp and then x0 (all random): afterwards x0 is
+    the guess of the last solve, xp and xpp those of the two before, x the
+    FP64 solution."""
     rng = np.random.default_rng(seed)
     cfg = nw.solver_cfg(kspace=30, max_iters=4000, tol=1e-12, x0=nw.X0_SOLUTION)
+    xpp = rng.normal(size=A.c.nrows)
     xp = rng.normal(size=A.c.nrows)
     x0 = rng.normal(size=A.c.nrows)
-    for g in (xp, x0):
+    for g in (xpp, xp, x0):
         ctx.set_solution(g)
         assert ctx.solve(cfg).converged
-    return xp, x0, ctx.solution()
+    return xpp, xp, x0, ctx.solution()
 
 
 def test_gradient_matches_oracle(nw, ctx):
     P, A, top = _system(nw, ctx)
-    xp, x0, xs = _solved_pair(nw, ctx, A, 11)
+    xpp, xp, x0, xs = _solved_pair(nw, ctx, A, 11)
     net = nw.Mpnn(ctx, H, L, seed=3)
     p = _params(nw, 5)
     net.params = p
@@ -72,7 +73,7 @@ def test_gradient_matches_oracle(nw, ctx):
     assert net.samples == 1
     loss, g = net.loss_grad(0)
     w = orc.mpnn_slot_weights(A)
-    feat, sc = orc.mpnn_features(A, A.rhs, x0, xp)
+    feat, sc = orc.mpnn_features(A, A.rhs, x0, xp, xpp)
     target = (xs - (2 * x0 - xp)) / sc[4]  # from the extrapolated guess
     loss_o, g_o = orc.mpnn_loss_grad(p, H, L, A, w, feat, target)
     # FP32 device arithmetic against the FP64 oracle
@@ -81,7 +82,7 @@ def test_gradient_matches_oracle(nw, ctx):
     np.testing.assert_allclose(g, g_o, rtol=0, atol=2e-5 * np.abs(g_o).max())
     # every parameter block receives gradient
     o = 0
-    for size in [H * 6, H] + [H * H, H * H, H] * L + [H, 1]:
+    for size in [H * 5, H] + [H * H, H * H, H] * L + [2 * H, 2]:
         assert np.abs(g_o[o:o + size]).max() > 0
         o += size
     # the same call again gives identical bits (fixed-order reductions)
@@ -95,22 +96,25 @@ def test_prediction_matches_oracle(nw, ctx):
     p = _params(nw, 7)
     net.params = p
     rng = np.random.default_rng(8)
-    x0, xp = rng.normal(size=P.n), rng.normal(size=P.n)
+    x0, xp, xpp = (rng.normal(size=P.n) for _ in range(3))
     w = orc.mpnn_slot_weights(A)
-    # without history (no solve yet), then with the last solve's guess as xp
+    # without history (no solve yet), then with the guesses of the last two
+    # solves as xp, xpp
     for hist in (False, True):
         if hist:
-            ctx.set_solution(xp)
-            ctx.solve(nw.solver_cfg(kspace=30, max_iters=10, tol=1e-3,
-                                    x0=nw.X0_SOLUTION), raise_on_noconv=False)
+            for g in (xpp, xp):
+                ctx.set_solution(g)
+                ctx.solve(nw.solver_cfg(kspace=30, max_iters=10, tol=1e-3,
+                                        x0=nw.X0_SOLUTION), raise_on_noconv=False)
         ctx.set_solution(x0)
         x, so = net.predict()
-        feat, sc = orc.mpnn_features(A, A.rhs, x0, xp if hist else None)
+        feat, sc = orc.mpnn_features(A, A.rhs, x0, xp if hist else None,
+                                     xpp if hist else None)
         d = orc.mpnn_forward(p, H, L, A, w, feat)
         assert so == pytest.approx(sc[4], rel=1e-12)
         assert (so == pytest.approx(sc[3], rel=1e-12)) != hist
         xb = 2 * x0 - xp if hist else x0
-        inner = feat[:, 4] == 0
+        inner = feat[:, 2] == 0
         np.testing.assert_allclose((x[inner] - xb[inner]) / so, d[inner],
                                    rtol=0, atol=2e-5 * np.abs(d).max())
         # Dirichlet rows take b / a_nn (here 0)
@@ -138,20 +142,24 @@ def test_bad_network_falls_back_to_previous_guess(nw, ctx):
     P, A, top = _system(nw, ctx, seed=12)
     net = nw.Mpnn(ctx, H, L, seed=1)
     p = np.zeros(net.nparams)
-    p[-1] = 1e6  # b_out: a huge constant correction
+    p[-1] = 1e6  # b_out of the second coefficient: a huge correction
     net.params = p
     ctx.set_learned_guess(net)
-    x0 = np.random.default_rng(3).normal(size=P.n)
+    rng = np.random.default_rng(3)
     cfg = dict(kspace=30, max_iters=4000, tol=1e-10)
-    ctx.set_solution(x0)
-    s_prev = ctx.solve(nw.solver_cfg(x0=nw.X0_SOLUTION, **cfg))
-    x_prev = ctx.solution()
+    for _ in range(2):  # two earlier solves make the guess history
+        ctx.set_solution(rng.normal(size=P.n))
+        ctx.solve(nw.solver_cfg(x0=nw.X0_SOLUTION, **cfg))
+    x0 = rng.normal(size=P.n)
     ctx.set_solution(x0)
     s_l = ctx.solve(nw.solver_cfg(x0=nw.X0_LEARNED, **cfg))
+    x_l = ctx.solution()
+    ctx.set_solution(x0)  # the same guess: the history stays as it is
+    s_prev = ctx.solve(nw.solver_cfg(x0=nw.X0_SOLUTION, **cfg))
     assert s_l.guess == nw.X0_SOLUTION and s_prev.guess == nw.X0_SOLUTION
     assert s_l.iters == s_prev.iters
     assert s_l.x0_rel == s_prev.x0_rel
-    np.testing.assert_array_equal(ctx.solution(), x_prev)
+    np.testing.assert_array_equal(x_l, ctx.solution())
 
 
 def test_learned_guess_needs_a_network(nw, ctx):
@@ -181,7 +189,7 @@ def test_learned_guess_beats_previous_step_guess(nw, ctx):
     for k in range(30):
         S.step(k)
         ctx.solve(nw.solver_cfg(x0=nw.X0_SOLUTION, **cfg))
-        if k > 0:
+        if k > 2:  # full guess history from step 3 on
             net.record()
     loss = net.train(nw.mpnn_train_cfg(epochs=40, batch=1, lr=1e-3, seed=1))
     ctx.set_learned_guess(net)

@@ -372,10 +372,11 @@ def test_mpnn_gradient_matches_finite_differences():
     A, top, rng = _mpnn_system()
     H, L = 6, 2
     npar = orc.mpnn_nparams(H, L)
-    assert npar == H * 6 + H + L * (2 * H * H + H) + H + 1
+    assert npar == H * 5 + H + L * (2 * H * H + H) + 2 * H + 2
     P = rng.normal(size=npar) * 0.4
     w = orc.mpnn_slot_weights(A)
     feat, _ = orc.mpnn_features(A, A.rhs, rng.normal(size=A.c.nrows),
+                                rng.normal(size=A.c.nrows),
                                 rng.normal(size=A.c.nrows))
     t = rng.normal(size=A.c.nrows)
     loss, g = orc.mpnn_loss_grad(P, H, L, A, w, feat, t)
@@ -404,14 +405,14 @@ def test_mpnn_permutation_equivariance():
     D = A.dense()
     x0 = rng.normal(size=n)
     w = orc.mpnn_slot_weights(A)
-    xp = rng.normal(size=n)
-    feat, _ = orc.mpnn_features(A, A.rhs, x0, xp)
+    xp, xpp = rng.normal(size=n), rng.normal(size=n)
+    feat, _ = orc.mpnn_features(A, A.rhs, x0, xp, xpp)
     d = orc.mpnn_forward(P, H, L, A, w, feat)
     perm = rng.permutation(n)
     Dp = D[np.ix_(perm, perm)]
     B = orc.CsrSystem.from_dense(Dp, A.rhs[perm])
     wp = orc.mpnn_slot_weights(B)
-    featp, _ = orc.mpnn_features(B, A.rhs[perm], x0[perm], xp[perm])
+    featp, _ = orc.mpnn_features(B, A.rhs[perm], x0[perm], xp[perm], xpp[perm])
     dp = orc.mpnn_forward(P, H, L, B, wp, featp)
     np.testing.assert_allclose(dp, d[perm], rtol=1e-12, atol=1e-12)
 
@@ -437,19 +438,22 @@ def test_mpnn_slot_weights_and_features():
     x0 = rng.normal(size=n)
     b = A.rhs.copy()
     xp = x0 + 0.1 * rng.normal(size=n)
-    feat, sc = orc.mpnn_features(A, b, x0, xp)
+    xpp = xp + 0.1 * rng.normal(size=n)
+    feat, sc = orc.mpnn_features(A, b, x0, xp, xpp)
     r = b - D @ x0
     rms = lambda v: np.sqrt(np.mean(v * v))
     np.testing.assert_allclose(sc, [rms(b), rms(x0), rms(r), rms(r / dg),
                                     rms(x0 - xp)], rtol=1e-13)
-    np.testing.assert_allclose(feat[:, 5], (x0 - xp) / rms(x0 - xp),
+    np.testing.assert_allclose(feat[:, 3], (x0 - xp) / rms(x0 - xp),
+                               rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(feat[:, 4], (xp - xpp) / rms(x0 - xp),
                                rtol=1e-12, atol=1e-14)
     # without history the last feature is zero and the output scale is s_e
     f0, s0 = orc.mpnn_features(A, b, x0)
-    assert np.all(f0[:, 5] == 0) and s0[4] == s0[3]
-    np.testing.assert_allclose(feat[:, 2], r / rms(r), rtol=1e-12, atol=1e-14)
-    np.testing.assert_allclose(feat[:, 3], np.log(dg) - np.log(dg).mean(),
+    assert np.all(f0[:, 3:] == 0) and s0[4] == s0[3]
+    np.testing.assert_allclose(feat[:, 0], r / rms(r), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(feat[:, 1], np.log(dg) - np.log(dg).mean(),
                                rtol=1e-12, atol=1e-14)
     flag = np.zeros(n)
     flag[top] = 1
-    np.testing.assert_array_equal(feat[:, 4], flag)
+    np.testing.assert_array_equal(feat[:, 2], flag)
