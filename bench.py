@@ -192,9 +192,10 @@ def pressure_solves(ns, ntrain, neval, epochs, log):
             "training_s": round(t_train, 1)}
         log("training: %d pairs in %.1fs, %d epochs in %.1fs, loss %.4g" %
             (ntrain, t_gen, epochs, t_train, loss))
-        ctx.set_learned_guess(net)
+        prior = nw.Mpnn(ctx, 64, 3, seed=7)  # untrained: quadratic extrapolation
         runs = {"fp64_x0_zero": [], "fp64_prev_step": [], "fp32_refined_x0_zero": [],
-                "fp32_refined_prev_step": [], "fp32_refined_learned": []}
+                "fp32_refined_prev_step": [], "fp32_refined_untrained_net": [],
+                "fp32_refined_learned": []}
         for k in range(ntrain + 1, ntrain + 1 + neval):
             x_prev = ctx.solution()
             S.step(k)
@@ -202,11 +203,13 @@ def pressure_solves(ns, ntrain, neval, epochs, log):
             # guess of the previous step's last solve
             sol = {}
             for name, cfg in [("fp32_refined_learned", fp32(nw.X0_LEARNED)),
+                              ("fp32_refined_untrained_net", fp32(nw.X0_LEARNED)),
                               ("fp32_refined_prev_step", fp32(nw.X0_SOLUTION)),
                               ("fp32_refined_x0_zero", fp32(nw.X0_ZERO)),
                               ("fp64_x0_zero", fp64(nw.X0_ZERO)),
                               ("fp64_prev_step", fp64(nw.X0_SOLUTION))]:
                 ctx.set_solution(x_prev)
+                ctx.set_learned_guess(prior if name == "fp32_refined_untrained_net" else net)
                 st = ctx.solve(cfg)
                 sol[name] = ctx.solution()
                 runs[name].append(st)
@@ -224,7 +227,7 @@ def pressure_solves(ns, ntrain, neval, epochs, log):
                 "x0_rel_residual": round(sum(s.x0_rel for s in sts) / len(sts), 6),
                 "rel_error_vs_fp64": max(e for _, e in lst),
                 "converged": all(bool(s.converged) for s in sts)}
-            if name == "fp32_refined_learned":
+            if name in ("fp32_refined_learned", "fp32_refined_untrained_net"):
                 res[name]["learned_guess_used"] = sum(s.guess == nw.X0_LEARNED for s in sts)
                 res[name]["guess_ms"] = round(sum(s.guess_ms for s in sts) / len(sts), 2)
             log("solve %s: %s" % (name, res[name]))
@@ -232,7 +235,12 @@ def pressure_solves(ns, ntrain, neval, epochs, log):
         z = res["fp64_x0_zero"]["ms"]
         res["speedup_vs_fp64_x0_zero"] = {
             n: round(z / res[n]["ms"], 3) for n in runs if n != "fp64_x0_zero"}
+        res["note"] = ("fp32_refined_untrained_net starts from the network before "
+                       "training, whose output head is the quadratic extrapolation "
+                       "3 x(n-1) - 3 x(n-2) + x(n-3); fp32_refined_learned from the "
+                       "trained network")
         net.close()
+        prior.close()
     return res
 
 

@@ -275,8 +275,9 @@ typedef struct {
 
 /* Parameter count of a network with the given hidden width and layers. */
 int64_t nw_mpnn_nparams(int hidden, int layers);
-/* hidden in {32, 64}, layers in [1, 8]; seeded He-uniform initialisation
- * with a zero output layer (the untrained prediction is xb). */
+/* hidden in {32, 64}, layers in [1, 8]; seeded He-uniform initialisation;
+ * the output head starts at w_out = 0, b_out = (1, -1), so the untrained
+ * prediction is the quadratic extrapolation 3 x0 - 3 xp + xpp. */
 int nw_mpnn_create(nw_ctx* ctx, int hidden, int layers, uint64_t seed,
                    nw_mpnn** out);
 void nw_mpnn_destroy(nw_mpnn* net);

@@ -19,8 +19,10 @@
 // so the network learns per-node coefficients of the history differences
 // on top of the linear extrapolation: a multi-step predictor whose output
 // vanishes with the history and has no offset of its own (an offset would be
-// the smoothest, slowest error for GMRES).  The output layer starts at zero,
-// so an untrained network gives the extrapolation.  Rows without
+// the smoothest, slowest error for GMRES).  The output head starts at
+// coefficients (1, -1): an untrained network gives the quadratic
+// extrapolation 3 x0 - 3 xp + xpp, and training learns departures from it
+// (per node, from the local features).  Rows without
 // off-diagonal entries (Dirichlet rows) are decoupled: their prediction is
 // b_n / a_nn exactly, so no boundary error enters the guess.
 //
@@ -221,45 +223,70 @@ __global__ __launch_bounds__(kBlock) void k_input_layer(
   }
 }
 
-// h_next = relu(W_self h + W_nbr m + b) with m = sum_s w_s h[col_s]; one
-// thread per node, the H accumulators in registers, weights k-major.
+// m[i] = sum_s w_s h[col_s] over the row's slots: one thread per (node,
+// 4-feature chunk), so the 16 threads of a node read a neighbour's 256-byte
+// row together.
 template <int H>
-__global__ __launch_bounds__(kBlock) void k_mp_layer(
-  const float* __restrict__ hp, const int32_t* __restrict__ rp,
-  const int32_t* __restrict__ ci, const float* __restrict__ w,
-  const float* __restrict__ WsT, const float* __restrict__ WnT,
-  const float* __restrict__ bias, float* __restrict__ hn, int n)
+__global__ void k_aggregate(const float* __restrict__ h,
+                            const int32_t* __restrict__ rp,
+                            const int32_t* __restrict__ ci,
+                            const float* __restrict__ w, float* __restrict__ m,
+                            int n)
 {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * (H / 4)) return;
+  const int i = (int)(t / (H / 4)), k4 = (int)(t % (H / 4));
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int s = rp[i]; s < rp[i + 1]; ++s) {
+    const float ws = w[s];
+    if (ws != 0.f) {
+      const float4 hc = reinterpret_cast<const float4*>(h + (int64_t)ci[s] * H)[k4];
+      a.x = fmaf(ws, hc.x, a.x);
+      a.y = fmaf(ws, hc.y, a.y);
+      a.z = fmaf(ws, hc.z, a.z);
+      a.w = fmaf(ws, hc.w, a.w);
+    }
+  }
+  reinterpret_cast<float4*>(m + (int64_t)i * H)[k4] = a;
+}
+
+// h_next = relu(W_self h + W_nbr m + b): one thread per node with the H
+// accumulators in registers; the block stages the layer's weights (k-major,
+// 2 H x H floats) in LDS once and every thread reads them as broadcasts.
+template <int H>
+__global__ __launch_bounds__(kBlock) void k_dense_layer(
+  const float* __restrict__ hp, const float* __restrict__ mp,
+  const float* __restrict__ WT, const float* __restrict__ bias,
+  float* __restrict__ hn, int n)
+{
+  __shared__ float4 sW[2 * H * H / 4];  // [k][j], k < H: W_self^T, else W_nbr^T
+  for (int q = threadIdx.x; q < 2 * H * H / 4; q += blockDim.x)
+    sW[q] = reinterpret_cast<const float4*>(WT)[q];
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float acc[H];
 #pragma unroll
   for (int j = 0; j < H; ++j) acc[j] = bias[j];
-  const int s0 = rp[i], s1 = rp[i + 1];
-  const float4* own = reinterpret_cast<const float4*>(hp + (int64_t)i * H);
-  for (int k4 = 0; k4 < H / 4; ++k4) {
-    const float4 a = own[k4];
-    float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int s = s0; s < s1; ++s) {
-      const float ws = w[s];
-      if (ws != 0.f) {
-        const float4 hc =
-          reinterpret_cast<const float4*>(hp + (int64_t)ci[s] * H)[k4];
-        m.x = fmaf(ws, hc.x, m.x);
-        m.y = fmaf(ws, hc.y, m.y);
-        m.z = fmaf(ws, hc.z, m.z);
-        m.w = fmaf(ws, hc.w, m.w);
+  for (int src = 0; src < 2; ++src) {
+    const float4* row =
+      reinterpret_cast<const float4*>((src ? mp : hp) + (int64_t)i * H);
+    const float4* Wk = sW + src * H * H / 4;
+    for (int k4 = 0; k4 < H / 4; ++k4) {
+      const float4 x = row[k4];
+      const float xv[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4* wr = Wk + (4 * k4 + q) * (H / 4);
+#pragma unroll
+        for (int j4 = 0; j4 < H / 4; ++j4) {
+          const float4 wv = wr[j4];
+          acc[4 * j4] = fmaf(wv.x, xv[q], acc[4 * j4]);
+          acc[4 * j4 + 1] = fmaf(wv.y, xv[q], acc[4 * j4 + 1]);
+          acc[4 * j4 + 2] = fmaf(wv.z, xv[q], acc[4 * j4 + 2]);
+          acc[4 * j4 + 3] = fmaf(wv.w, xv[q], acc[4 * j4 + 3]);
+        }
       }
-    }
-    const float av[4] = {a.x, a.y, a.z, a.w};
-    const float mv[4] = {m.x, m.y, m.z, m.w};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float* ws = WsT + (4 * k4 + q) * H;
-      const float* wn = WnT + (4 * k4 + q) * H;
-#pragma unroll
-      for (int j = 0; j < H; ++j)
-        acc[j] = fmaf(ws[j], av[q], fmaf(wn[j], mv[q], acc[j]));
     }
   }
   float4* out = reinterpret_cast<float4*>(hn + (int64_t)i * H);
@@ -433,8 +460,8 @@ struct WgradShape {
   static constexpr int NOUT = JA * KB;
 };
 
-// Weight gradients as node sums of outer products A_i B_i^T:
-//   layer : A = gz [n][H], B = [h_prev | m | 1], m recomputed from h_prev
+// Weight gradients as node sums of outer products A_i B_i^T (the layer
+// gradients, the large ones, have their own kernel k_wgrad_layer):
 //   input : A = gz0 [n][H], B = [features | 1]
 //   output: A = dd2 [n][2], B = [h_L | 1]
 // Block b sums a contiguous node range in tiles of kTileN staged in LDS
@@ -468,16 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(
       float v = 0.f;
       if (node < i1) {
         if (MODE == kModeLayer) {
-          if (k < H) {
-            v = B[(int64_t)node * H + k];
-          } else if (k < 2 * H) {
-            for (int s = rp[node]; s < rp[node + 1]; ++s) {
-              const float ws = w[s];
-              if (ws != 0.f) v = fmaf(ws, B[(int64_t)ci[s] * H + (k - H)], v);
-            }
-          } else {
-            v = 1.f;
-          }
+          v = 0.f;  // layer gradients: k_wgrad_layer
         } else if (MODE == kModeInput) {
           v = k < kF ? B[(int64_t)k * n + node] : 1.f;
         } else {
@@ -508,6 +526,83 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(
   }
 }
 
+// Layer weight gradients G[j][k] = sum_i gz[i][j] [h_prev | m][i][k]
+// (k < 2H) and gb[j] = sum_i gz[i][j], written as partial[b][off + j (2H+1)
+// + k] with the bias at k = 2H.  Thread t owns the 4 x 8 tile
+// j in [4 (t / 16), +4), k in [8 (t % 16), +8) of the 64 x 128 product
+// (H = 64; H = 32 uses the first 4 x 4 tiles of each row group); tiles of
+// kTileN nodes are staged in LDS, FP32 within a tile, FP64 across tiles.
+template <int H>
+__global__ __launch_bounds__(kBlock) void k_wgrad_layer(
+  const float* __restrict__ gz, const float* __restrict__ hp,
+  const float* __restrict__ mp, int n, double* __restrict__ partial, int nv,
+  int off)
+{
+  constexpr int RJ = 4, RK = 2 * H / 16;  // 16 column groups of 2H / 16
+  __shared__ float sA[kTileN][H];
+  __shared__ float sB[kTileN][2 * H];
+  const int jb = threadIdx.x / 16, kb = threadIdx.x % 16;
+  const bool own = jb * RJ < H;  // H = 32: threads with jb >= 8 idle
+  double acc[RJ][RK], accb[RJ];
+#pragma unroll
+  for (int a = 0; a < RJ; ++a) {
+    accb[a] = 0.0;
+#pragma unroll
+    for (int b = 0; b < RK; ++b) acc[a][b] = 0.0;
+  }
+  const int per_block = ((n + gridDim.x - 1) / gridDim.x + kTileN - 1) / kTileN * kTileN;
+  const int i0 = min(n, (int)blockIdx.x * per_block);
+  const int i1 = min(n, i0 + per_block);
+  for (int t0 = i0; t0 < i1; t0 += kTileN) {
+    for (int idx = threadIdx.x; idx < kTileN * H; idx += kBlock) {
+      const int t = idx / H, k = idx % H, node = t0 + t;
+      const bool in = node < i1;
+      sA[t][k] = in ? gz[(int64_t)node * H + k] : 0.f;
+      sB[t][k] = in ? hp[(int64_t)node * H + k] : 0.f;
+      sB[t][H + k] = in ? mp[(int64_t)node * H + k] : 0.f;
+    }
+    __syncthreads();
+    if (own) {
+      float ta[RJ][RK], tb[RJ];
+#pragma unroll
+      for (int a = 0; a < RJ; ++a) {
+        tb[a] = 0.f;
+#pragma unroll
+        for (int b = 0; b < RK; ++b) ta[a][b] = 0.f;
+      }
+      for (int t = 0; t < kTileN; ++t) {
+        float av[RJ], bv[RK];
+#pragma unroll
+        for (int a = 0; a < RJ; ++a) av[a] = sA[t][jb * RJ + a];
+#pragma unroll
+        for (int b = 0; b < RK; ++b) bv[b] = sB[t][kb * RK + b];
+#pragma unroll
+        for (int a = 0; a < RJ; ++a) {
+          tb[a] += av[a];
+#pragma unroll
+          for (int b = 0; b < RK; ++b) ta[a][b] = fmaf(av[a], bv[b], ta[a][b]);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < RJ; ++a) {
+        accb[a] += (double)tb[a];
+#pragma unroll
+        for (int b = 0; b < RK; ++b) acc[a][b] += (double)ta[a][b];
+      }
+    }
+    __syncthreads();
+  }
+  if (!own) return;
+  double* out = partial + (int64_t)blockIdx.x * nv + off;
+#pragma unroll
+  for (int a = 0; a < RJ; ++a) {
+    const int j = jb * RJ + a;
+#pragma unroll
+    for (int b = 0; b < RK; ++b) out[j * (2 * H + 1) + kb * RK + b] = acc[a][b];
+    if (kb == 0) out[j * (2 * H + 1) + 2 * H] = accb[a];
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- model
@@ -529,6 +624,7 @@ struct Mpnn {
   float* dPT = nullptr;    // W_in^T, then per layer W_self^T, W_nbr^T
   float* w = nullptr;      // [nnz] slot weights
   std::vector<float*> h;   // L+1 activations [n][H]
+  std::vector<float*> msg; // messages m_l [n][H] (L when training, else 1)
   float *gs = nullptr, *gq = nullptr, *gz = nullptr;
   float *feat = nullptr, *target = nullptr, *delta = nullptr, *dd = nullptr;
   double* r = nullptr;     // FP64 residual of x0
@@ -617,6 +713,15 @@ int ensure_buffers(Mpnn* m, bool training)
     NW_HIP(hipMalloc(&m->red, sizeof(double) * m->nv));
     m->h.assign(2, nullptr);
     for (auto& p : m->h) NW_HIP(hipMalloc(&p, sizeof(float) * n * m->H));
+    m->msg.assign(1, nullptr);
+    NW_HIP(hipMalloc(&m->msg[0], sizeof(float) * n * m->H));
+  }
+  if (training && (int)m->msg.size() < m->L) {
+    while ((int)m->msg.size() < m->L) {
+      float* p = nullptr;
+      NW_HIP(hipMalloc(&p, sizeof(float) * n * m->H));
+      m->msg.push_back(p);
+    }
   }
   if (training && (int)m->h.size() < m->L + 1) {
     while ((int)m->h.size() < m->L + 1) {
@@ -693,12 +798,16 @@ int forward_h(Mpnn* m, const float* feat, bool keep)
     feat, WT, m->dP + o.bin, m->h[0], n);
   NW_LAUNCH_CHECK();
   int cur = 0;
+  const unsigned gq = (unsigned)(((int64_t)n * (H / 4) + kBlock - 1) / kBlock);
   for (int l = 0; l < L; ++l) {
     const float* WsT = WT + (size_t)H * kF + 2 * (size_t)l * H * H;
     const int nxt = keep ? l + 1 : 1 - cur;
-    k_mp_layer<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(
-      m->h[cur], g.row_ptr, g.cols, m->w, WsT, WsT + (size_t)H * H,
-      m->dP + o.b(l, H), m->h[nxt], n);
+    float* msg = m->msg[keep ? l : 0];
+    k_aggregate<H><<<gq, kBlock, 0, st>>>(m->h[cur], g.row_ptr, g.cols, m->w,
+                                          msg, n);
+    NW_LAUNCH_CHECK();
+    k_dense_layer<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(
+      m->h[cur], msg, WsT, m->dP + o.b(l, H), m->h[nxt], n);
     NW_LAUNCH_CHECK();
     cur = nxt;
   }
@@ -743,8 +852,8 @@ int loss_grad_h(Mpnn* m, const Sample& smp, double scale, double* loss,
   float* gz = m->gz;
   float* gs = m->gs;
   for (int l = L - 1; l >= 0; --l) {
-    k_wgrad<H, kModeLayer><<<kGradBlocks, kBlock, 0, st>>>(
-      gz, m->h[l], g.row_ptr, g.cols, m->w, n, m->partial, nv, off);
+    k_wgrad_layer<H><<<kGradBlocks, kBlock, 0, st>>>(gz, m->h[l], m->msg[l], n,
+                                                     m->partial, nv, off);
     NW_LAUNCH_CHECK();
     off += WgradShape<H, kModeLayer>::NOUT;
     k_back_dense<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(
@@ -891,7 +1000,11 @@ int nw_mpnn_create(nw_ctx* c, int hidden, int layers, uint64_t seed,
     fill(o.ws(l), (int64_t)hidden * hidden, 2 * hidden);
     fill(o.wn(l, hidden), (int64_t)hidden * hidden, 2 * hidden);
   }
-  // output layer zero: the untrained prediction is the extrapolated guess
+  // output head: w_out = 0 and b_out = (1, -1), so the untrained network
+  // predicts the quadratic extrapolation 3 x0 - 3 xp + xpp (xb plus the
+  // second difference); training learns per-node departures from it
+  m->P[o.bout] = 1.0;
+  m->P[o.bout + 1] = -1.0;
   if (hidden == 64) build_gmap<64>(m);
   else build_gmap<32>(m);
   int rc = NW_OK;
@@ -919,6 +1032,7 @@ void nw_mpnn_destroy(nw_mpnn* m)
   mpnn_detach(m->c, m);
   for (auto& s : m->samples) free_sample(s);
   for (auto p : m->h) (void)hipFree(p);
+  for (auto p : m->msg) (void)hipFree(p);
   (void)hipFree(m->dP);
   (void)hipFree(m->dPT);
   (void)hipFree(m->w);

@@ -138,6 +138,28 @@ def test_training_is_deterministic_and_lowers_loss(nw, ctx):
     assert np.array_equal(nets[0], nets[1])
 
 
+def test_untrained_network_is_quadratic_extrapolation(nw, ctx):
+    """The output head starts at coefficients (1, -1): before training the
+    prediction is 3 x0 - 3 xp + xpp on rows with couplings, b / a_nn on
+    Dirichlet rows."""
+    P, A, top = _system(nw, ctx, seed=14)
+    net = nw.Mpnn(ctx, H, L, seed=5)
+    rng = np.random.default_rng(4)
+    xpp, xp, x0 = (rng.normal(size=P.n) for _ in range(3))
+    for g in (xpp, xp):
+        ctx.set_solution(g)
+        ctx.solve(nw.solver_cfg(kspace=30, max_iters=10, tol=1e-3,
+                                x0=nw.X0_SOLUTION), raise_on_noconv=False)
+    ctx.set_solution(x0)
+    x, so = net.predict()
+    quad = 3 * x0 - 3 * xp + xpp
+    inner = np.ones(P.n, bool)
+    inner[top] = False
+    np.testing.assert_allclose(x[inner], quad[inner], rtol=0,
+                               atol=1e-6 * np.abs(quad).max())
+    np.testing.assert_array_equal(x[top], 0.0)
+
+
 def test_bad_network_falls_back_to_previous_guess(nw, ctx):
     P, A, top = _system(nw, ctx, seed=12)
     net = nw.Mpnn(ctx, H, L, seed=1)
