@@ -183,11 +183,13 @@ def pressure_solves(ns, ntrain, neval, epochs, log):
                 net.record()
         t_gen = time.perf_counter() - t0
         t0 = time.perf_counter()
-        loss = net.train(nw.mpnn_train_cfg(epochs=epochs, batch=1, lr=1e-3, seed=3))
+        # small steps: the head starts at the quadratic extrapolation and
+        # training adjusts it (large Adam steps on the head destroy the prior)
+        loss = net.train(nw.mpnn_train_cfg(epochs=epochs, batch=1, lr=1e-4, seed=3))
         t_train = time.perf_counter() - t0
         res["learned_guess"] = {
             "network": "3 message-passing layers, 64 hidden, FP32",
-            "training_pairs": net.samples, "epochs": epochs,
+            "training_pairs": net.samples, "epochs": epochs, "adam_lr": 1e-4,
             "final_training_loss": loss, "pair_generation_s": round(t_gen, 1),
             "training_s": round(t_train, 1)}
         log("training: %d pairs in %.1fs, %d epochs in %.1fs, loss %.4g" %
@@ -266,6 +268,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=24)
     ap.add_argument("--eval-steps", type=int, default=3)
     ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--only", choices=["assembly", "solve"],
+                    help="run one part (profiling); the JSON line is then partial")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     a = ap.parse_args()
@@ -287,110 +291,111 @@ def main():
            "warmup": a.warmup, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
 
-    # ---------------------------------------------------------- momentum
-    t0 = time.perf_counter()
-    W = wl.momentum_abl(n2)
-    log("momentum workload %d^3: %d nodes %d edges (%.1fs)" % (n2, W.n, W.ne,
-                                                               time.perf_counter() - t0))
-    with nw.Context(0) as ctx:
-        W.upload(ctx)
-        ctx.build_graph(3)
-        _, nnz = ctx.sizes()
-        nnzb = nnz // 9
+    if a.only != "solve":
+        # ---------------------------------------------------------- momentum
+        t0 = time.perf_counter()
+        W = wl.momentum_abl(n2)
+        log("momentum workload %d^3: %d nodes %d edges (%.1fs)" % (n2, W.n, W.ne,
+                                                                   time.perf_counter() - t0))
+        with nw.Context(0) as ctx:
+            W.upload(ctx)
+            ctx.build_graph(3)
+            _, nnz = ctx.sizes()
+            nnzb = nnz // 9
 
-        def step():
-            ctx.momentum_edge_peclet(W.peclet)
-            ctx.assemble_momentum(W.opts)
+            def step():
+                ctx.momentum_edge_peclet(W.peclet)
+                ctx.assemble_momentum(W.opts)
 
-        for _ in range(a.warmup):
-            step()
-        hip.sync()
-        s = ctx.stream()
-        ev = [hip.event() for _ in range(2 * a.steps + 2)]
-        wall0 = time.perf_counter()
-        hip.record(ev[0], s)
-        for i in range(a.steps):
-            ctx.momentum_edge_peclet(W.peclet)
-            hip.record(ev[1 + 2 * i], s)
-            ctx.assemble_momentum(W.opts)
-            hip.record(ev[2 + 2 * i], s)
-        hip.record(ev[-1], s)
-        hip.sync()
-        wall = time.perf_counter() - wall0
-        total_ms = hip.elapsed(ev[0], ev[-1])
-        kern_ms = sum(hip.elapsed(ev[1 + 2 * i], ev[2 + 2 * i])
-                      for i in range(a.steps)) / a.steps
-        ms_step = total_ms / a.steps
-        mbytes = momentum_bytes(W, nnzb)
-        achieved = mbytes / (kern_ms * 1e-3) / 1e9
-        out["value"] = round(W.ne / (ms_step * 1e-3) / 1e6, 2)
-        out["ms_per_step"] = round(ms_step, 4)
-        out["config"] = {
-            "workload": "momentum edge assembly, %d^3-cell Hex8 box stretched "
-                        "1.05 toward z=0, +-10%% jitter, log-law u + 5%% noise, "
-                        "nu_t~U(0,1e-2), alpha=0.5, hybrid 1" % n2,
-            "nodes": W.n, "edges": W.ne, "block_nnz": nnzb, "ndof": 3,
-            "step": "MomentumEdgePecletAlg + MomentumEdgeSolverAlg (rows "
-                    "overwritten: fused zeroSystem)",
-            "wall_s_timed_region": round(wall, 4)}
-        traffic = read_traffic("momentum_abl_128_assembly.txt")
-        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1),
-                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4),
-                           "traffic": traffic,
-                           "kernel": "k_assemble_rows<MomentumPhys>",
-                           "kernel_avg_ms": round(kern_ms, 4),
-                           "algorithmic_bytes_per_launch": mbytes}
-        log("momentum: %.3f ms/step, assembly kernel %.3f ms, %.0f GB/s" %
-            (ms_step, kern_ms, achieved))
-    del W
+            for _ in range(a.warmup):
+                step()
+            hip.sync()
+            s = ctx.stream()
+            ev = [hip.event() for _ in range(2 * a.steps + 2)]
+            wall0 = time.perf_counter()
+            hip.record(ev[0], s)
+            for i in range(a.steps):
+                ctx.momentum_edge_peclet(W.peclet)
+                hip.record(ev[1 + 2 * i], s)
+                ctx.assemble_momentum(W.opts)
+                hip.record(ev[2 + 2 * i], s)
+            hip.record(ev[-1], s)
+            hip.sync()
+            wall = time.perf_counter() - wall0
+            total_ms = hip.elapsed(ev[0], ev[-1])
+            kern_ms = sum(hip.elapsed(ev[1 + 2 * i], ev[2 + 2 * i])
+                          for i in range(a.steps)) / a.steps
+            ms_step = total_ms / a.steps
+            mbytes = momentum_bytes(W, nnzb)
+            achieved = mbytes / (kern_ms * 1e-3) / 1e9
+            out["value"] = round(W.ne / (ms_step * 1e-3) / 1e6, 2)
+            out["ms_per_step"] = round(ms_step, 4)
+            out["config"] = {
+                "workload": "momentum edge assembly, %d^3-cell Hex8 box stretched "
+                            "1.05 toward z=0, +-10%% jitter, log-law u + 5%% noise, "
+                            "nu_t~U(0,1e-2), alpha=0.5, hybrid 1" % n2,
+                "nodes": W.n, "edges": W.ne, "block_nnz": nnzb, "ndof": 3,
+                "step": "MomentumEdgePecletAlg + MomentumEdgeSolverAlg (rows "
+                        "overwritten: fused zeroSystem)",
+                "wall_s_timed_region": round(wall, 4)}
+            traffic = read_traffic("momentum_abl_128_assembly.txt")
+            out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1),
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(achieved / HBM_PEAK_GBS, 4),
+                               "traffic": traffic,
+                               "kernel": "k_assemble_rows<MomentumPhys>",
+                               "kernel_avg_ms": round(kern_ms, 4),
+                               "algorithmic_bytes_per_launch": mbytes}
+            log("momentum: %.3f ms/step, assembly kernel %.3f ms, %.0f GB/s" %
+                (ms_step, kern_ms, achieved))
+        del W
 
-    # ---------------------------------------------------------- scalar
-    W = wl.scalar_tg(n1)
-    with nw.Context(0) as ctx:
-        W.upload(ctx)
-        ctx.build_graph(1)
-        _, nnz = ctx.sizes()
-        kms, tot = time_launches(hip, ctx, lambda: ctx.assemble_scalar(W.opts),
-                                 a.steps, a.warmup)
-        out["scalar_assembly"] = {
-            "workload": "scalar adv-diff %d^3 uniform, Taylor-Green u, mu=1e-3" % n1,
-            "nodes": W.n, "edges": W.ne, "Medges_per_s": round(W.ne / (kms * 1e-3) / 1e6, 1),
-            "kernel_avg_ms": round(kms, 5),
-            "GBps_algorithmic": round(scalar_bytes(W, nnz) / (kms * 1e-3) / 1e9, 1),
-            "note": "~10 MB working set: cache-resident, no HBM claim"}
-    del W
+        # ---------------------------------------------------------- scalar
+        W = wl.scalar_tg(n1)
+        with nw.Context(0) as ctx:
+            W.upload(ctx)
+            ctx.build_graph(1)
+            _, nnz = ctx.sizes()
+            kms, tot = time_launches(hip, ctx, lambda: ctx.assemble_scalar(W.opts),
+                                     a.steps, a.warmup)
+            out["scalar_assembly"] = {
+                "workload": "scalar adv-diff %d^3 uniform, Taylor-Green u, mu=1e-3" % n1,
+                "nodes": W.n, "edges": W.ne, "Medges_per_s": round(W.ne / (kms * 1e-3) / 1e6, 1),
+                "kernel_avg_ms": round(kms, 5),
+                "GBps_algorithmic": round(scalar_bytes(W, nnz) / (kms * 1e-3) / 1e9, 1),
+                "note": "~10 MB working set: cache-resident, no HBM claim"}
+        del W
 
-    # ---------------------------------------------------------- continuity
-    t0 = time.perf_counter()
-    W = wl.continuity_rf(n3)
-    log("continuity workload %d^3: %d nodes %d edges (%.1fs)" % (n3, W.n, W.ne,
-                                                                 time.perf_counter() - t0))
-    with nw.Context(0) as ctx:
-        W.upload(ctx)
-        ctx.build_graph(1)
-        _, nnz = ctx.sizes()
-        kms, tot = time_launches(hip, ctx, lambda: ctx.assemble_continuity(W.opts),
-                                 a.steps, a.warmup)
-        cb = continuity_bytes(W, nnz)
-        out["continuity_assembly"] = {
-            "workload": "continuity %d^3-cell box stretched 1.05, random-Fourier u (8 modes)" % n3,
-            "nodes": W.n, "edges": W.ne, "nnz": nnz,
-            "Medges_per_s": round(W.ne / (kms * 1e-3) / 1e6, 1),
-            "kernel_avg_ms": round(kms, 4),
-            "GBps_algorithmic": round(cb / (kms * 1e-3) / 1e9, 1),
-            "frac_hbm_peak": round(cb / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "algorithmic_bytes_per_launch": cb}
-        log("continuity: %.3f ms, %.0f GB/s" % (kms, cb / (kms * 1e-3) / 1e9))
-    del W
+        # ---------------------------------------------------------- continuity
+        t0 = time.perf_counter()
+        W = wl.continuity_rf(n3)
+        log("continuity workload %d^3: %d nodes %d edges (%.1fs)" % (n3, W.n, W.ne,
+                                                                     time.perf_counter() - t0))
+        with nw.Context(0) as ctx:
+            W.upload(ctx)
+            ctx.build_graph(1)
+            _, nnz = ctx.sizes()
+            kms, tot = time_launches(hip, ctx, lambda: ctx.assemble_continuity(W.opts),
+                                     a.steps, a.warmup)
+            cb = continuity_bytes(W, nnz)
+            out["continuity_assembly"] = {
+                "workload": "continuity %d^3-cell box stretched 1.05, random-Fourier u (8 modes)" % n3,
+                "nodes": W.n, "edges": W.ne, "nnz": nnz,
+                "Medges_per_s": round(W.ne / (kms * 1e-3) / 1e6, 1),
+                "kernel_avg_ms": round(kms, 4),
+                "GBps_algorithmic": round(cb / (kms * 1e-3) / 1e9, 1),
+                "frac_hbm_peak": round(cb / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes_per_launch": cb}
+            log("continuity: %.3f ms, %.0f GB/s" % (kms, cb / (kms * 1e-3) / 1e9))
+        del W
 
     # ---------------------------------------------------------- solve
-    if not a.no_solve:
+    if not a.no_solve and a.only != "assembly":
         ns = 24 if a.quick else a.solve_size
         out["pressure_solve"] = pressure_solves(ns, a.train_steps, a.eval_steps,
                                                 a.epochs, log)
 
-    if not a.no_cpu:
+    if not a.no_cpu and a.only is None:
         out["cpu_baseline"] = cpu_momentum_baseline(3.0 if a.quick else 12.0,
                                                     16 if a.quick else 48)
         log("cpu baseline: %s" % out["cpu_baseline"])

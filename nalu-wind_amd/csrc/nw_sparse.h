@@ -8,9 +8,12 @@
 
 namespace nw {
 
+// Lanes per row for rows of `avg` entries: about two entries per lane
+// (measured on the 7-entry rows of the hex pressure system: 4 lanes beat 8
+// by 4-12 %, 2 lanes are 1.5x slower, 1 lane 2.7x).
 inline int lanes_per_row(double avg)
 {
-  return avg <= 4.5 ? 4 : (avg <= 9.0 ? 8 : (avg <= 24.0 ? 16 : 32));
+  return avg <= 9.0 ? 4 : (avg <= 18.0 ? 8 : (avg <= 36.0 ? 16 : 32));
 }
 
 // y = alpha * (A x); G lanes per row.
@@ -108,6 +111,12 @@ __global__ __launch_bounds__(kBlock) void k_resid(
   do {                                                                       \
     const unsigned nw_blocks_ = grid_for((int64_t)(n) * (G), kBlock);        \
     switch (G) {                                                             \
+    case 1:                                                                  \
+      kernel<T, 1><<<nw_blocks_, kBlock, 0, stream>>>(__VA_ARGS__);          \
+      break;                                                                 \
+    case 2:                                                                  \
+      kernel<T, 2><<<nw_blocks_, kBlock, 0, stream>>>(__VA_ARGS__);          \
+      break;                                                                 \
     case 4:                                                                  \
       kernel<T, 4><<<nw_blocks_, kBlock, 0, stream>>>(__VA_ARGS__);          \
       break;                                                                 \
