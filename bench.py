@@ -185,12 +185,16 @@ def pressure_solves(ns, ntrain, neval, epochs, log):
         t0 = time.perf_counter()
         # small steps: the head starts at the quadratic extrapolation and
         # training adjusts it (large Adam steps on the head destroy the prior)
-        loss = net.train(nw.mpnn_train_cfg(epochs=epochs, batch=1, lr=1e-4, seed=3))
+        # the most recent pairs validate: the parameters that predict them
+        # best (the untrained ones included) are kept
+        loss = net.train(nw.mpnn_train_cfg(epochs=epochs, batch=1, lr=1e-4, seed=3,
+                                           holdout=3))
         t_train = time.perf_counter() - t0
         res["learned_guess"] = {
             "network": "3 message-passing layers, 64 hidden, FP32",
-            "training_pairs": net.samples, "epochs": epochs, "adam_lr": 1e-4,
-            "final_training_loss": loss, "pair_generation_s": round(t_gen, 1),
+            "pairs": net.samples, "validation_pairs": 3, "epochs": epochs,
+            "adam_lr": 1e-4,
+            "best_validation_loss": loss, "pair_generation_s": round(t_gen, 1),
             "training_s": round(t_train, 1)}
         log("training: %d pairs in %.1fs, %d epochs in %.1fs, loss %.4g" %
             (ntrain, t_gen, epochs, t_train, loss))

@@ -271,6 +271,10 @@ typedef struct {
   double lr;        /* Adam step size */
   double beta1, beta2, eps; /* Adam moment decay and denominator guard */
   uint64_t seed;    /* sample order shuffling */
+  int holdout;      /* most recent samples kept for validation: after each
+                       epoch the validation loss is measured and the
+                       parameters with the lowest one (the initial ones
+                       included) are kept (0: no validation) */
 } nw_mpnn_train_cfg;
 
 /* Parameter count of a network with the given hidden width and layers. */
@@ -300,8 +304,8 @@ int nw_mpnn_clear_samples(nw_mpnn* net);
 /* Mean-squared loss of recorded sample s and its gradient (nparams
  * doubles, may be NULL). */
 int nw_mpnn_loss_grad(nw_mpnn* net, int sample, double* loss, double* grad);
-/* Adam over the recorded samples; *last_loss = mean loss of the last
- * epoch. */
+/* Adam over the recorded samples; *last_loss = the best validation loss
+ * (holdout > 0) or the mean training loss of the last epoch. */
 int nw_mpnn_train(nw_mpnn* net, const nw_mpnn_train_cfg* cfg,
                   double* last_loss);
 /* Prediction from the current system with x0 = the solution vector and

@@ -1139,8 +1139,11 @@ int nw_mpnn_train(nw_mpnn* m, const nw_mpnn_train_cfg* cfg, double* last_loss)
   NW_REQUIRE(!m->samples.empty(), NW_E_STATE, "no recorded training samples");
   NW_REQUIRE(cfg->epochs >= 0 && cfg->batch >= 1 && cfg->lr > 0 &&
                cfg->beta1 >= 0 && cfg->beta1 < 1 && cfg->beta2 >= 0 &&
-               cfg->beta2 < 1 && cfg->eps > 0,
+               cfg->beta2 < 1 && cfg->eps > 0 && cfg->holdout >= 0,
              NW_E_ARG, "bad training configuration");
+  NW_REQUIRE(cfg->holdout < (int)m->samples.size(), NW_E_ARG,
+             "holdout %d leaves no training samples (%d recorded)",
+             cfg->holdout, (int)m->samples.size());
   int rc = check_graph(m);
   if (rc) return rc;
   NW_HIP(hipSetDevice(m->c->device));
@@ -1148,11 +1151,33 @@ int nw_mpnn_train(nw_mpnn* m, const nw_mpnn_train_cfg* cfg, double* last_loss)
   if (rc) return rc;
   rc = slot_weights(m);
   if (rc) return rc;
-  const int ns = (int)m->samples.size();
+  // the last `holdout` samples (the most recent steps) validate; the
+  // parameters with the lowest validation loss, the initial ones included,
+  // are kept
+  const int nv = cfg->holdout;
+  const int ns = (int)m->samples.size() - nv;
   std::vector<int> order(ns);
   for (int i = 0; i < ns; ++i) order[i] = i;
   std::mt19937_64 gen(cfg->seed);
-  std::vector<double> g(m->np);
+  std::vector<double> g(m->np), best;
+  double best_val = 0.0;
+  auto validate = [&](double* out) -> int {
+    double s = 0.0;
+    std::vector<double> gg(m->np);
+    for (int q = 0; q < nv; ++q) {
+      double l = 0.0;
+      int r = loss_grad(m, m->samples[ns + q], 1.0, &l, gg);
+      if (r) return r;
+      s += l;
+    }
+    *out = s / std::max(nv, 1);
+    return NW_OK;
+  };
+  if (nv > 0) {
+    rc = validate(&best_val);
+    if (rc) return rc;
+    best = m->P;
+  }
   double epoch_loss = 0.0;
   for (int ep = 0; ep < cfg->epochs; ++ep) {
     std::shuffle(order.begin(), order.end(), gen);
@@ -1179,8 +1204,22 @@ int nw_mpnn_train(nw_mpnn* m, const nw_mpnn_train_cfg* cfg, double* last_loss)
       if (rc) return rc;
     }
     epoch_loss /= ns;
+    if (nv > 0) {
+      double v = 0.0;
+      rc = validate(&v);
+      if (rc) return rc;
+      if (v < best_val) {
+        best_val = v;
+        best = m->P;
+      }
+    }
   }
-  if (last_loss) *last_loss = epoch_loss;
+  if (nv > 0) {
+    m->P = best;
+    rc = upload_params(m);
+    if (rc) return rc;
+  }
+  if (last_loss) *last_loss = nv > 0 ? best_val : epoch_loss;
   return NW_OK;
 }
 

@@ -91,11 +91,11 @@ class SolveStats(C.Structure):
 class MpnnTrainCfg(C.Structure):
     _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double),
                 ("beta1", C.c_double), ("beta2", C.c_double),
-                ("eps", C.c_double), ("seed", C.c_uint64)]
+                ("eps", C.c_double), ("seed", C.c_uint64), ("holdout", C.c_int)]
 
 
 def mpnn_train_cfg(**kw):
-    o = MpnnTrainCfg(10, 4, 1e-3, 0.9, 0.999, 1e-8, 0)
+    o = MpnnTrainCfg(10, 4, 1e-3, 0.9, 0.999, 1e-8, 0, 0)
     for k, v in kw.items():
         setattr(o, k, v)
     return o

@@ -138,6 +138,31 @@ def test_training_is_deterministic_and_lowers_loss(nw, ctx):
     assert np.array_equal(nets[0], nets[1])
 
 
+def test_training_keeps_best_validated_parameters(nw, ctx):
+    """With held-out recent samples, training keeps the parameters with the
+    lowest validation loss: an oversized step cannot leave the network worse
+    on them than it started."""
+    from workloads import PressureSeries
+    S = PressureSeries(ctx, 10, seed=31, stretch=1.05, dt_flow=0.02)
+    net = nw.Mpnn(ctx, H, L, seed=6)
+    ctx.set_solution(np.zeros(S.W.n))
+    for k in range(9):
+        S.step(k)
+        ctx.solve(nw.solver_cfg(x0=nw.X0_SOLUTION, kspace=30, max_iters=4000,
+                                tol=1e-10))
+        if k > 2:
+            net.record()
+    val = lambda: np.mean([net.loss_grad(s)[0] for s in (4, 5)])
+    v0 = val()
+    p0 = net.params
+    best = net.train(nw.mpnn_train_cfg(epochs=3, batch=1, lr=0.3, seed=2,
+                                       holdout=2))
+    assert best <= v0 * (1 + 1e-12)
+    assert val() == pytest.approx(best, rel=1e-12)
+    if best == pytest.approx(v0, rel=1e-12):
+        assert np.array_equal(net.params, p0)
+
+
 def test_untrained_network_is_quadratic_extrapolation(nw, ctx):
     """The output head starts at coefficients (1, -1): before training the
     prediction is 3 x0 - 3 xp + xpp on rows with couplings, b / a_nn on
