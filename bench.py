@@ -347,7 +347,7 @@ def main():
                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": round(achieved / HBM_PEAK_GBS, 4),
                                "traffic": traffic,
-                               "kernel": "k_assemble_rows<MomentumPhys>",
+                               "kernel": "k_assemble<MomentumPhys>",
                                "kernel_avg_ms": round(kern_ms, 4),
                                "algorithmic_bytes_per_launch": mbytes}
             log("momentum: %.3f ms/step, assembly kernel %.3f ms, %.0f GB/s" %

@@ -7,18 +7,21 @@
 // side = 0 when n == L and 1 when n == R.  Node and edge fields are stored
 // component-major (SoA): component d of node n is f[d * nnode + n].
 //
-// Kernel shape ("row owner").  G lanes own one row (G = 2 for the 1-dof
-// systems, 1 for momentum).  Each lane loads the row node's fields, then
-// walks every G-th slot of the row: for each off-diagonal slot it gathers
-// the neighbour's fields (the slot's column) and the edge's area vector,
-// evaluates the edge flux in registers exactly as the reference lambda does
-// for that edge, stores the (row, neighbour) block of the edge LHS into the
-// slot, and adds the edge's (row, row) block and rhs(row) contributions to
-// register accumulators.  The G partial sums are combined by a fixed
-// butterfly and the diagonal block and rhs are written once.  Every edge is
-// therefore evaluated twice (once from each end), each matrix entry is
-// written once, no atomics are used, and the result is bitwise
-// reproducible.
+// Kernel shape ("row owner").  G lanes own one row (G = 1 for the 1-dof
+// systems, 2 for momentum), and the lanes of an instruction sit on
+// consecutive rows.  Each lane loads the row node's fields, then walks
+// every G-th slot of the row: for each off-diagonal slot it gathers the
+// neighbour's fields (the slot's column) and the edge's fields, evaluates
+// the edge flux in registers exactly as the reference lambda does for that
+// edge, puts the (row, neighbour) block of the edge LHS into the slot, and
+// adds the edge's (row, row) block and rhs(row) contributions to register
+// accumulators.  The G partial sums are combined by a fixed butterfly and
+// the diagonal block and rhs are written once.  A wave's rows are
+// consecutive, so their slots are one contiguous range: the wave stages
+// that range's codes, columns and values in LDS and moves them to and from
+// HBM as whole cache lines.  Every edge is evaluated twice (once from each
+// end), each matrix entry is written once, no atomics are used, and the
+// result is bitwise reproducible.
 //
 // Rows are taken in chunks of 64 consecutive rows from a schedule sorted by
 // the Morton code of the chunk centroids, and workgroups are renumbered so
@@ -73,8 +76,16 @@ struct ContNode {
   double x[3], u[3], g[3], rho, p, ud;
 };
 
+// Launch shape of each physics (measured on the 256^3 continuity and 128^3
+// momentum workloads): kLanes lanes per row, kUnroll slots in flight per
+// lane, non-temporal streaming, LDS slot capacity per wave (the large one
+// is used when some wave's rows hold more than kSlotsSmall slots) and the
+// occupancy the register allocation must allow.
 struct ContinuityPhys {
   static constexpr int NDOF = 1;
+  static constexpr int kLanes = 1, kUnroll = 1, kMinWaves = 2;
+  static constexpr bool kNonTemporal = false;
+  static constexpr int kSlotsSmall = 512, kSlotsLarge = 1024;
   using Node = ContNode;
   const double *coords, *vel, *dpdx, *rho, *p, *udiag, *areav;
   int64_t N, E;
@@ -92,12 +103,21 @@ struct ContinuityPhys {
     r.ud = udiag[n];
   }
 
-  __device__ void edge(int e, int side, const Node& L, const Node& R,
+  struct Edge {
+    double av[3];
+  };
+
+  __device__ void load_edge(int e, Edge& r) const
+  {
+    for (int d = 0; d < 3; ++d) r.av[d] = areav[d * E + e];
+  }
+
+  __device__ void edge(const Edge& er, int side, const Node& L, const Node& R,
                        double* off, double* acc) const
   {
     const double om_interpTogether = 1.0 - interpTogether;
     const double om_incomp = 1.0 - incompressible;
-    const double av[3] = {areav[e], areav[E + e], areav[2 * E + e]};
+    const double* av = er.av;
     const double projTimeScale = 0.5 * (1.0 / L.ud + 1.0 / R.ud);
     const double rhoIp = 0.5 * (L.rho + R.rho);
     const double denScale = (1.0 / rhoIp) * incompressible + om_incomp;
@@ -134,6 +154,9 @@ struct ScalNode {
 
 struct ScalarPhys {
   static constexpr int NDOF = 1;
+  static constexpr int kLanes = 1, kUnroll = 1, kMinWaves = 2;
+  static constexpr bool kNonTemporal = false;
+  static constexpr int kSlotsSmall = 512, kSlotsLarge = 1024;
   using Node = ScalNode;
   const double *coords, *vrtm, *q, *dqdx, *rho, *dflux, *areav, *mdotf;
   int64_t N, E;
@@ -153,13 +176,23 @@ struct ScalarPhys {
     r.dflux = dflux[n];
   }
 
-  __device__ void edge(int e, int side, const Node& L, const Node& R,
+  struct Edge {
+    double av[3], mdot;
+  };
+
+  __device__ void load_edge(int e, Edge& r) const
+  {
+    for (int d = 0; d < 3; ++d) r.av[d] = areav[d * E + e];
+    r.mdot = mdotf[e];
+  }
+
+  __device__ void edge(const Edge& er, int side, const Node& L, const Node& R,
                        double* off, double* acc) const
   {
     const double eps = 1.0e-16;
     const double om_alpha = 1.0 - alpha, om_alphaUpw = 1.0 - alphaUpw;
-    const double av[3] = {areav[e], areav[E + e], areav[2 * E + e]};
-    const double mdot = mdotf[e];
+    const double* av = er.av;
+    const double mdot = er.mdot;
     const double qNp1L = L.q, qNp1R = R.q;
     const double viscIp = 0.5 * (L.dflux + R.dflux);
     const double diffIp = 0.5 * (L.dflux / L.rho + R.dflux / R.rho);
@@ -229,6 +262,10 @@ struct MomNode {
 
 struct MomentumPhys {
   static constexpr int NDOF = 3;
+  static constexpr int kLanes = 2, kUnroll = 2, kMinWaves = 2;
+  static constexpr bool kNonTemporal = true;
+  // 32 rows of 7 slots (Hex8 edge graph); 72 bytes per slot of LDS
+  static constexpr int kSlotsSmall = 224, kSlotsLarge = 224;
   using Node = MomNode;
   const double *coords, *vel, *dudx, *visc, *areav, *mdotf, *pecfacf, *wall;
   int64_t N, E;
@@ -246,13 +283,24 @@ struct MomentumPhys {
     r.wall = wall[n];
   }
 
-  __device__ void edge(int e, int side, const Node& L, const Node& R,
+  struct Edge {
+    double av[3], mdot, pecfac;
+  };
+
+  __device__ void load_edge(int e, Edge& r) const
+  {
+    for (int d = 0; d < 3; ++d) r.av[d] = areav[d * E + e];
+    r.mdot = mdotf[e];
+    r.pecfac = pecfacf[e];
+  }
+
+  __device__ void edge(const Edge& er, int side, const Node& L, const Node& R,
                        double* off, double* acc) const
   {
     const double eps = 1.0e-16;
     const double om_alpha = 1.0 - alpha, om_alphaUpw = 1.0 - alphaUpw;
-    const double av[3] = {areav[e], areav[E + e], areav[2 * E + e]};
-    const double mdot = mdotf[e];
+    const double* av = er.av;
+    const double mdot = er.mdot;
     const double viscIp = 0.5 * (L.visc + R.visc);
     double dx[3];
     double axdx = 0.0, asq = 0.0;
@@ -280,7 +328,7 @@ struct MomentumPhys {
         limitR[d] = van_leer(4.0 * duR[d] - du, du, eps);
       }
     }
-    const double pecfac = pecfacf[e];
+    const double pecfac = er.pecfac;
     const double om_pecfac = 1.0 - pecfac;
     double uIpL[3], uIpR[3];
     for (int d = 0; d < 3; ++d) {
@@ -367,14 +415,98 @@ __device__ inline int xcd_block(int nblocks_padded)
   return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
 }
 
-// G lanes per row: lane g of a row evaluates the row's slots g, g+G, ...
-// (consecutive lanes touch consecutive slots, so the code/column loads and
-// the off-diagonal stores are coalesced), then the diagonal-block and rhs
-// contributions are summed over the G lanes by a fixed butterfly; the
-// result does not depend on scheduling.  A wave covers 64/G consecutive
-// rows of a scheduled chunk; G waves cover the chunk.
-template <class P, int G>
-__global__ __launch_bounds__(kBlock) void k_assemble_rows(
+// Walks the slots g, g+G, ... of `row`: for each off-diagonal slot s
+// evaluates the edge, hands the (row, neighbour) block to put(s, off) and
+// adds the edge's (row, row) block and rhs(row) contributions to acc.
+// code_at(s) and col_at(s) read the slot's code and column.  U slots are
+// taken per step: their neighbour and edge gathers are issued together
+// before any of them is evaluated, and they are evaluated and summed in
+// slot order, so U changes only the overlap of the loads.
+template <class P, int G, int U, class Code, class Col, class Put>
+__device__ inline void walk_row(const P& phys,
+                                const int32_t* __restrict__ row_ptr, int row,
+                                int g, Code code_at, Col col_at, Put put,
+                                double* acc)
+{
+  constexpr int ND = P::NDOF, NB = ND * ND, NV = NB + ND;
+  typename P::Node me, ot[U];
+  typename P::Edge er[U];
+  phys.load(row, me);
+  const int s1 = row_ptr[row + 1];
+#pragma clang loop unroll(disable)
+  for (int s = row_ptr[row] + g; s < s1; s += G * U) {
+    int c[U];
+    for (int u = 0; u < U; ++u) c[u] = s + u * G < s1 ? code_at(s + u * G) : -1;
+    for (int u = 0; u < U; ++u) {
+      if (c[u] < 0) continue;  // a diagonal slot (written by the caller)
+      phys.load(col_at(s + u * G), ot[u]);
+      phys.load_edge(c[u] >> 1, er[u]);
+    }
+    for (int u = 0; u < U; ++u) {
+      if (c[u] < 0) continue;
+      double off[NB], con[NV];
+      if (c[u] & 1)
+        phys.edge(er[u], 1, ot[u], me, off, con);
+      else
+        phys.edge(er[u], 0, me, ot[u], off, con);
+      put(s + u * G, off);
+      for (int v = 0; v < NV; ++v) acc[v] += con[v];
+    }
+  }
+}
+
+// Sums acc over the G lanes of a row by a fixed butterfly (all lanes of the
+// wave take part), so the result does not depend on scheduling.
+template <int G, int NV>
+__device__ inline void row_butterfly(double* acc)
+{
+  for (int o = G / 2; o > 0; o >>= 1)
+    for (int v = 0; v < NV; ++v) acc[v] += __shfl_xor(acc[v], o, G);
+}
+
+// Orders this wave's LDS accesses: LDS operations of one wave complete in
+// issue order, so only the compiler has to be kept from moving them.
+__device__ inline void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Writes the (row, row) block and rhs(row) of a row, straight to HBM.
+template <int ND>
+__device__ inline void put_diag_rhs(double* __restrict__ dp,
+                                    double* __restrict__ rp, const double* acc,
+                                    int accumulate)
+{
+  constexpr int NB = ND * ND;
+  if (accumulate) {
+    for (int k = 0; k < NB; ++k) dp[k] += acc[k];
+    for (int d = 0; d < ND; ++d) rp[d] += acc[NB + d];
+  } else {
+    for (int k = 0; k < NB; ++k) dp[k] = acc[k];
+    for (int d = 0; d < ND; ++d) rp[d] = acc[NB + d];
+  }
+}
+
+// The assembly kernel.  G lanes own one row and a wave takes 64/G
+// consecutive rows of a scheduled chunk (G waves cover the chunk); lane g of
+// a row walks the row's slots g, g+G, ... (walk_row) and the G partial sums
+// of the diagonal block and rhs are combined by a fixed butterfly.  The
+// lanes of one instruction sit on consecutive rows, so their neighbour and
+// edge gathers go in one stencil direction and stay compact.
+//
+// The slots of a wave's rows form one contiguous range [s_lo, s_hi).  The
+// wave loads the range's codes and columns into LDS with coalesced loads,
+// puts every evaluated block, the diagonal included, into an LDS copy of
+// the range, and writes the range to HBM at the end as whole cache lines:
+// no partially written lines linger in L2, where they would evict the node
+// and edge data that neighbouring rows reuse.  A wave whose range holds
+// more than SCAP slots writes straight to HBM instead.  NT marks the
+// streamed code/column reads and value writes non-temporal.
+template <class P, int G, int U, int SCAP, bool NT>
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
   P phys, const int32_t* __restrict__ row_ptr,
   const int32_t* __restrict__ code, const int32_t* __restrict__ cols,
   const int32_t* __restrict__ diag, const int2* __restrict__ sched,
@@ -382,54 +514,86 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(
   int accumulate)
 {
   constexpr int ND = P::NDOF, NB = ND * ND, NV = NB + ND;
-  constexpr int RPW = kWave / G;  // rows per wave
+  constexpr int RPW = kWave / G;
+  constexpr int WPB = kBlock / kWave;
+  __shared__ int code_s[WPB][SCAP];
+  __shared__ int col_s[WPB][SCAP];
+  __shared__ double val_s[WPB][SCAP * NB];
   const int lane = threadIdx.x & (kWave - 1);
-  const int gw = xcd_block((int)gridDim.x) * (kBlock / kWave) +
-                 (int)(threadIdx.x / kWave);
+  const int wv = (int)(threadIdx.x / kWave);
+  const int gw = xcd_block((int)gridDim.x) * WPB + wv;
   const int chunk_id = gw / G;
   if (chunk_id >= nchunks) return;
   const int2 chunk = sched[chunk_id];
-  const int row = chunk.x + (gw % G) * RPW + lane / G;
+  const int rw0 = chunk.x + (gw % G) * RPW;
+  const int nr = min(RPW, chunk.y - rw0);
+  if (nr <= 0) return;
+  const int row = rw0 + lane / G;
   const int g = lane & (G - 1);
-  const bool active = row < chunk.y;
+  const bool active = lane / G < nr;
+  const int s_lo = row_ptr[rw0], s_hi = row_ptr[rw0 + nr];
+  const int ns = s_hi - s_lo;
   double acc[NV];
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  if (active) {
-    typename P::Node me, ot;
-    phys.load(row, me);
-    const int s1 = row_ptr[row + 1];
-    for (int s = row_ptr[row] + g; s < s1; s += G) {
-      const int c = code[s];
-      if (c < 0) continue;  // the diagonal slot, written below
-      phys.load(cols[s], ot);
-      const int e = c >> 1, side = c & 1;
-      double off[NB], con[NV];
-      if (side)
-        phys.edge(e, 1, ot, me, off, con);
-      else
-        phys.edge(e, 0, me, ot, off, con);
-      double* vp = vals + (int64_t)s * NB;
-      if (accumulate) {
-        for (int k = 0; k < NB; ++k) vp[k] += off[k];
-      } else {
-        for (int k = 0; k < NB; ++k) vp[k] = off[k];
-      }
-      for (int v = 0; v < NV; ++v) acc[v] += con[v];
+  if (ns > SCAP) {
+    if (active)
+      walk_row<P, G, U>(
+        phys, row_ptr, row, g, [&](int s) { return code[s]; },
+        [&](int s) { return cols[s]; },
+        [&](int s, const double* off) {
+          double* vp = vals + (int64_t)s * NB;
+          if (accumulate) {
+            for (int k = 0; k < NB; ++k) vp[k] += off[k];
+          } else {
+            for (int k = 0; k < NB; ++k) vp[k] = off[k];
+          }
+        },
+        acc);
+    row_butterfly<G, NV>(acc);
+    if (active && g == 0)
+      put_diag_rhs<ND>(vals + (int64_t)diag[row] * NB, rhs + (int64_t)row * ND,
+                       acc, accumulate);
+    return;
+  }
+  for (int i = lane; i < ns; i += kWave) {
+    if (NT) {
+      code_s[wv][i] = __builtin_nontemporal_load(code + s_lo + i);
+      col_s[wv][i] = __builtin_nontemporal_load(cols + s_lo + i);
+    } else {
+      code_s[wv][i] = code[s_lo + i];
+      col_s[wv][i] = cols[s_lo + i];
     }
   }
-  // butterfly over the G lanes of the row (all lanes take part)
-  for (int o = G / 2; o > 0; o >>= 1)
-    for (int v = 0; v < NV; ++v) acc[v] += __shfl_xor(acc[v], o, G);
+  wave_lds_sync();
+  if (active)
+    walk_row<P, G, U>(
+      phys, row_ptr, row, g, [&](int s) { return code_s[wv][s - s_lo]; },
+      [&](int s) { return col_s[wv][s - s_lo]; },
+      [&](int s, const double* off) {
+        double* vp = &val_s[wv][(s - s_lo) * NB];
+        for (int k = 0; k < NB; ++k) vp[k] = off[k];
+      },
+      acc);
+  row_butterfly<G, NV>(acc);
   if (active && g == 0) {
-    double* dp = vals + (int64_t)diag[row] * NB;
+    double* dp = &val_s[wv][(diag[row] - s_lo) * NB];
+    for (int k = 0; k < NB; ++k) dp[k] = acc[k];
     double* rp = rhs + (int64_t)row * ND;
     if (accumulate) {
-      for (int k = 0; k < NB; ++k) dp[k] += acc[k];
       for (int d = 0; d < ND; ++d) rp[d] += acc[NB + d];
     } else {
-      for (int k = 0; k < NB; ++k) dp[k] = acc[k];
       for (int d = 0; d < ND; ++d) rp[d] = acc[NB + d];
     }
+  }
+  wave_lds_sync();
+  double* dst = vals + (int64_t)s_lo * NB;
+  for (int i = lane; i < ns * NB; i += kWave) {
+    double v = val_s[wv][i];
+    if (accumulate) v += dst[i];
+    if (NT)
+      __builtin_nontemporal_store(v, dst + i);
+    else
+      dst[i] = v;
   }
 }
 
@@ -516,8 +680,22 @@ int build_schedule(Ctx* c)
                         hipMemcpyHostToDevice, c->stream));
   NW_HIP(hipStreamSynchronize(c->stream));
   g.nchunks = nch;
+  g.max_chunk_slots = 0;
+  for (int64_t i = 0; i < nch; ++i)
+    g.max_chunk_slots = std::max<int64_t>(
+      g.max_chunk_slots, g.h_row_ptr[ch[i].y] - g.h_row_ptr[ch[i].x]);
   g.sched_valid = true;
   return NW_OK;
+}
+
+template <class P, int SCAP>
+void launch_assemble(Ctx* c, const P& phys, int accumulate, unsigned blocks)
+{
+  const Graph& g = c->g;
+  k_assemble<P, P::kLanes, P::kUnroll, SCAP, P::kNonTemporal>
+    <<<blocks, kBlock, 0, c->stream>>>(phys, g.row_ptr, g.code, g.cols, g.diag,
+                                       g.sched, (int)g.nchunks, g.vals, g.rhs,
+                                       accumulate);
 }
 
 template <class P>
@@ -530,15 +708,13 @@ int launch_rows(Ctx* c, const P& phys, int accumulate)
   }
   if (g.nchunks == 0) return NW_OK;
   const int64_t wpb = kBlock / kWave;
-  // lanes per row: 2 for the 1-dof systems (shorter dependent-load chains
-  // per lane, coalesced slot access); 1 for the 3-dof system, whose register
-  // footprint already limits occupancy
-  constexpr int G = P::NDOF == 1 ? 2 : 1;
-  int64_t blocks = (g.nchunks * G + wpb - 1) / wpb;
+  int64_t blocks = (g.nchunks * P::kLanes + wpb - 1) / wpb;
   blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
-  k_assemble_rows<P, G><<<(unsigned)blocks, kBlock, 0, c->stream>>>(
-    phys, g.row_ptr, g.code, g.cols, g.diag, g.sched, (int)g.nchunks, g.vals,
-    g.rhs, accumulate);
+  // LDS capacity per wave: the small one when every wave's slot range fits
+  if (g.max_chunk_slots <= P::kSlotsSmall)
+    launch_assemble<P, P::kSlotsSmall>(c, phys, accumulate, (unsigned)blocks);
+  else
+    launch_assemble<P, P::kSlotsLarge>(c, phys, accumulate, (unsigned)blocks);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }

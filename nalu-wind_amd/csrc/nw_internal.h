@@ -69,6 +69,7 @@ struct Graph {
   // chunks processed together are neighbours in space.
   int2* sched = nullptr;
   int64_t nchunks = 0;
+  int64_t max_chunk_slots = 0;  // most slots held by one chunk's rows
   bool sched_valid = false;
 };
 
