@@ -52,6 +52,19 @@ constexpr int kBlock = 256;
 // off-diagonal slot reached through edge e from its node[side] -> 2e+side.
 __host__ __device__ inline int32_t diag_code(int32_t n) { return -(n + 1); }
 
+// Sliced ELL copy of a CSR structure: slices of 64 consecutive rows (one
+// wave each); entry j of row 64k + l is stored at sp[k] + 64 j + l, the
+// slice's width being its longest row; shorter rows (and the rows past n in
+// the last slice) are padded with column = the row itself (0 past n) and
+// value 0.  slot[p] is the CSR entry stored at position p, -1 for padding.
+struct Sell {
+  int n = 0, nslices = 0;
+  int64_t len = 0;          // positions, padding included
+  int64_t* sp = nullptr;    // [nslices + 1]
+  int32_t* ci = nullptr;    // [len]
+  int32_t* slot = nullptr;  // [len]
+};
+
 struct Graph {
   int ndof = 0;
   int64_t nnode = 0;
@@ -70,6 +83,7 @@ struct Graph {
   int2* sched = nullptr;
   int64_t nchunks = 0;
   int64_t max_chunk_slots = 0;  // most slots held by one chunk's rows
+  Sell sell;  // sliced ELL form of the 1-dof graph (built by the first solve)
   bool sched_valid = false;
 };
 

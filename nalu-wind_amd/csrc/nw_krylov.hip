@@ -291,7 +291,72 @@ unsigned stream_grid(int64_t n)
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(n, kBlock), 4096));
 }
 
+// width[k] = longest row of slice k (a wave per slice)
+__global__ __launch_bounds__(kBlock) void k_sell_width(
+  const int32_t* __restrict__ rp, int n, int nslices, int32_t* __restrict__ width)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int len = t < n ? rp[t + 1] - rp[t] : 0;
+  for (int o = kWave / 2; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0 && (t >> 6) < nslices) width[t >> 6] = len;
+}
+
+// columns and CSR slots of every sliced-ELL position (a lane per row)
+__global__ __launch_bounds__(kBlock) void k_sell_struct(
+  const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, int n,
+  const int64_t* __restrict__ sp, int nslices, int32_t* __restrict__ sci,
+  int32_t* __restrict__ sslot)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t >> 6;
+  if (k >= nslices) return;
+  const int64_t b = sp[k] + (t & 63);
+  const int w = (int)((sp[k + 1] - sp[k]) >> 6);
+  const int r0 = t < n ? rp[t] : 0;
+  const int len = t < n ? rp[t + 1] - r0 : 0;
+  for (int j = 0; j < w; ++j) {
+    const int64_t p = b + 64 * (int64_t)j;
+    if (j < len) {
+      sci[p] = ci[r0 + j];
+      sslot[p] = r0 + j;
+    } else {
+      sci[p] = t < n ? t : 0;
+      sslot[p] = -1;
+    }
+  }
+}
+
 }  // namespace
+
+int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
+               hipStream_t st)
+{
+  s.n = n;
+  s.nslices = (n + kWave - 1) / kWave;
+  const int ns = s.nslices;
+  int32_t* dw = nullptr;
+  NW_HIP(hipMalloc(&dw, sizeof(int32_t) * std::max(ns, 1)));
+  k_sell_width<<<grid_for((int64_t)ns * kWave, kBlock), kBlock, 0, st>>>(rp, n, ns, dw);
+  std::vector<int32_t> w(ns);
+  hipError_t e = hipMemcpyAsync(w.data(), dw, sizeof(int32_t) * ns,
+                                hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(dw);
+  NW_HIP(e);
+  std::vector<int64_t> sp(ns + 1, 0);
+  for (int k = 0; k < ns; ++k) sp[k + 1] = sp[k] + (int64_t)kWave * w[k];
+  s.len = sp[ns];
+  NW_HIP(hipMalloc(&s.sp, sizeof(int64_t) * (ns + 1)));
+  NW_HIP(hipMalloc(&s.ci, sizeof(int32_t) * std::max<int64_t>(s.len, 1)));
+  NW_HIP(hipMalloc(&s.slot, sizeof(int32_t) * std::max<int64_t>(s.len, 1)));
+  NW_HIP(hipMemcpyAsync(s.sp, sp.data(), sizeof(int64_t) * (ns + 1),
+                        hipMemcpyHostToDevice, st));
+  k_sell_struct<<<grid_for((int64_t)ns * kWave, kBlock), kBlock, 0, st>>>(
+    rp, ci, n, s.sp, ns, s.ci, s.slot);
+  NW_LAUNCH_CHECK();
+  NW_HIP(hipStreamSynchronize(st));
+  return NW_OK;
+}
 
 // ---------------------------------------------------------------- workspace
 
@@ -359,7 +424,7 @@ template <typename T>
 static int alloc_space(Space<T>& s, int64_t n, int64_t nnz, int m, bool cheb)
 {
   if (!s.V) NW_HIP(hipMalloc(&s.V, sizeof(T) * n * (m + 1)));
-  if (!s.mat) NW_HIP(hipMalloc(&s.mat, sizeof(T) * nnz));
+  if (!s.mat) NW_HIP(hipMalloc(&s.mat, sizeof(T) * nnz));  // sliced ELL order
   if (!s.u) NW_HIP(hipMalloc(&s.u, sizeof(T) * n));
   if (cheb) {
     if (!s.dinv) NW_HIP(hipMalloc(&s.dinv, sizeof(T) * n));
@@ -392,11 +457,23 @@ struct Solve {
     return NW_OK;
   }
 
+  // y = alpha A x with A in sliced ELL order (the space's matrix copy)
   template <typename T>
   int spmv(const T* va, const T* x, T* y, T alpha)
   {
     const int n = (int)k->n;
-    NW_ROW_LAUNCH(k_spmv, T, G, n, st, c->g.row_ptr, c->g.cols, va, x, y, n, alpha);
+    k_sell_spmv<T><<<grid_for(n, kBlock), kBlock, 0, st>>>(
+      c->g.sell.sp, c->g.sell.ci, va, x, y, n, alpha);
+    NW_LAUNCH_CHECK();
+    return NW_OK;
+  }
+
+  // y = alpha A x with A the assembled CSR values
+  int spmv_csr(const double* va, const double* x, double* y, double alpha)
+  {
+    const int n = (int)k->n;
+    NW_ROW_LAUNCH(k_spmv, double, G, n, st, c->g.row_ptr, c->g.cols, va, x, y,
+                  n, alpha);
     NW_LAUNCH_CHECK();
     return NW_OK;
   }
@@ -406,8 +483,8 @@ struct Solve {
                 T c1, T c2)
   {
     const int n = (int)k->n;
-    NW_ROW_LAUNCH(k_cheb_step, T, G, n, st, c->g.row_ptr, c->g.cols, va, z, v,
-                  dinv, d, zo, n, c1, c2);
+    k_sell_cheb_step<T><<<grid_for(n, kBlock), kBlock, 0, st>>>(
+      c->g.sell.sp, c->g.sell.ci, va, z, v, dinv, d, zo, n, c1, c2);
     NW_LAUNCH_CHECK();
     return NW_OK;
   }
@@ -680,7 +757,7 @@ struct Solve {
       int rc = dot64(xa, xa, &xx);
       if (rc) return rc;
       const double inv = 1.0 / std::sqrt(xx);
-      rc = spmv<double>(c->g.vals, xa, ya, inv);
+      rc = spmv_csr(c->g.vals, xa, ya, inv);
       if (rc) return rc;
       k_vmul<<<grid_for(n, kBlock), kBlock, 0, st>>>(ya, k->dinv, n);
       NW_LAUNCH_CHECK();
@@ -760,8 +837,13 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
     NW_HIP(hipEventCreate(&k->ev0));
     NW_HIP(hipEventCreate(&k->ev1));
   }
-  int rc = f32 ? alloc_space(k->f, n, nnz, k->m, cheb || amgp)
-               : alloc_space(k->d, n, nnz, k->m, cheb || amgp);
+  if (!c->g.sell.ci) {
+    int rc0 = sell_build(c->g.row_ptr, c->g.cols, (int)n, c->g.sell, c->stream);
+    if (rc0) return rc0;
+  }
+  const int64_t slen = c->g.sell.len;
+  int rc = f32 ? alloc_space(k->f, n, slen, k->m, cheb || amgp)
+               : alloc_space(k->d, n, slen, k->m, cheb || amgp);
   if (rc) return rc;
   if (cheb && !k->pe) NW_HIP(hipMalloc(&k->pe, sizeof(double) * 2 * n));
 
@@ -791,14 +873,15 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
   NW_HIP(hipStreamSynchronize(st));
   NW_REQUIRE(!bad, NW_E_NONFINITE, "zero or non-finite diagonal in the matrix");
   const double* colscale = (cfg->precond == NW_PREC_JACOBI) ? k->dinv : nullptr;
+  const Sell& sl = c->g.sell;
   if (f32) {
-    k_scale_copy<float><<<grid_for(nnz, kBlock), kBlock, 0, st>>>(
-      c->g.cols, c->g.vals, colscale, k->f.mat, nnz);
+    k_sell_values<float><<<grid_for(sl.len, kBlock), kBlock, 0, st>>>(
+      sl.slot, sl.ci, c->g.vals, colscale, k->f.mat, sl.len);
     if (cheb)
       k_convert<float><<<grid_for(n, kBlock), kBlock, 0, st>>>(k->dinv, k->f.dinv, 1.0, n);
   } else {
-    k_scale_copy<double><<<grid_for(nnz, kBlock), kBlock, 0, st>>>(
-      c->g.cols, c->g.vals, colscale, k->d.mat, nnz);
+    k_sell_values<double><<<grid_for(sl.len, kBlock), kBlock, 0, st>>>(
+      sl.slot, sl.ci, c->g.vals, colscale, k->d.mat, sl.len);
     if (cheb)
       k_convert<double><<<grid_for(n, kBlock), kBlock, 0, st>>>(k->dinv, k->d.dinv, 1.0, n);
   }

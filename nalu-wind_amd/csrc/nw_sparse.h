@@ -1,7 +1,9 @@
-// nw_sparse.h -- CSR kernels shared by the Krylov solver and the AMG
-// preconditioner: SpMV with G lanes per row (rows are contiguous, so the
-// lanes of a row read consecutive entries), the fused Chebyshev step and
-// its zero-start initialisation.
+// nw_sparse.h -- sparse kernels shared by the Krylov solver and the AMG
+// preconditioner: SpMV, residual and the fused Chebyshev step in two
+// layouts.  Sliced ELL (one lane per row, independent loads) serves short
+// rows: the fine matrix and the prolongators.  CSR with G lanes per row
+// serves long rows (coarse AMG operators, restrictions) and the assembled
+// matrix as it is (power iteration, nw_linsys_spmv).
 #pragma once
 
 #include "nw_internal.h"
@@ -57,20 +59,6 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(
   }
 }
 
-// d = dinv * v * inv_theta; z = d
-template <typename T>
-__global__ void k_cheb_init(const T* __restrict__ v, const T* __restrict__ dinv,
-                            T* __restrict__ d, T* __restrict__ z, int n,
-                            T inv_theta)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const T di = dinv[i] * v[i] * inv_theta;
-  d[i] = di;
-  z[i] = di;
-}
-
-
 // y += A x (prolongation-correction); G lanes per row.
 template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_spmv_add(
@@ -105,6 +93,120 @@ __global__ __launch_bounds__(kBlock) void k_resid(
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   if (row < n && g == 0) r[row] = b[row] - s;
 }
+
+// d = dinv * v * inv_theta; z = d
+template <typename T>
+__global__ void k_cheb_init(const T* __restrict__ v, const T* __restrict__ dinv,
+                            T* __restrict__ d, T* __restrict__ z, int n,
+                            T inv_theta)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T di = dinv[i] * v[i] * inv_theta;
+  d[i] = di;
+  z[i] = di;
+}
+
+
+// ------------------------------------------------------------ sliced ELL
+// One lane per row, a wave per slice: for every j the 64 lanes read 64
+// consecutive values and columns, and all loads of a row are independent
+// (issued eight at a time), so a wave keeps a whole row's gathers in
+// flight instead of walking a dependent chain.  Padding reads x at a valid
+// column with value 0.  The row sum runs over j in order.
+template <typename T>
+__device__ inline T sell_dot(const int64_t* __restrict__ sp,
+                             const int32_t* __restrict__ ci,
+                             const T* __restrict__ va, const T* __restrict__ x,
+                             int row)
+{
+  const int k = row >> 6;
+  const int64_t b = sp[k] + (row & 63);
+  const int w = (int)((sp[k + 1] - sp[k]) >> 6);
+  T s = 0;
+  for (int j0 = 0; j0 < w; j0 += 8) {
+    int c[8];
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = min(j0 + u, w - 1);
+      c[u] = ci[b + 64 * (int64_t)j];
+      v[u] = j0 + u < w ? va[b + 64 * (int64_t)j] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u] * x[c[u]];
+  }
+  return s;
+}
+
+// y = alpha * (A x)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sell_spmv(
+  const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
+  const T* __restrict__ va, const T* __restrict__ x, T* __restrict__ y, int n,
+  T alpha)
+{
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  y[row] = alpha * sell_dot(sp, ci, va, x, row);
+}
+
+// y += A x
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sell_spmv_add(
+  const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
+  const T* __restrict__ va, const T* __restrict__ x, T* __restrict__ y, int n)
+{
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  y[row] += sell_dot(sp, ci, va, x, row);
+}
+
+// Chebyshev step: r = v - A z; d = c1 d + c2 dinv r; zo = z + d.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sell_cheb_step(
+  const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
+  const T* __restrict__ va, const T* __restrict__ z, const T* __restrict__ v,
+  const T* __restrict__ dinv, T* __restrict__ d, T* __restrict__ zo, int n,
+  T c1, T c2)
+{
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const T s = sell_dot(sp, ci, va, z, row);
+  const T dn = c1 * d[row] + c2 * dinv[row] * (v[row] - s);
+  d[row] = dn;
+  zo[row] = z[row] + dn;
+}
+
+// r = b - A x
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sell_resid(
+  const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
+  const T* __restrict__ va, const T* __restrict__ x, const T* __restrict__ b,
+  T* __restrict__ r, int n)
+{
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  r[row] = b[row] - sell_dot(sp, ci, va, x, row);
+}
+
+// out[p] = T(a[slot[p]] * (colscale ? colscale[ci[p]] : 1)), 0 for padding
+template <typename T>
+__global__ void k_sell_values(const int32_t* __restrict__ slot,
+                              const int32_t* __restrict__ ci,
+                              const double* __restrict__ a,
+                              const double* __restrict__ colscale,
+                              T* __restrict__ out, int64_t len)
+{
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= len) return;
+  const int32_t k = slot[p];
+  out[p] = k < 0 ? (T)0 : (T)(colscale ? a[k] * colscale[ci[p]] : a[k]);
+}
+
+// Builds the sliced ELL structure of the CSR graph (rp, ci) with n rows.
+int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
+               hipStream_t st);
 
 // launch helper: dispatch a G-templated row kernel by lanes-per-row
 #define NW_ROW_LAUNCH(kernel, T, G, n, stream, ...)                          \
