@@ -313,6 +313,18 @@ int nw_mpnn_train(nw_mpnn* net, const nw_mpnn_train_cfg* cfg,
  * writes the prediction to x_host (nrows doubles) and the scale s to
  * *scale. */
 int nw_mpnn_predict(nw_mpnn* net, double* x_host, double* scale);
+/* Arithmetic of the prediction (training always runs in FP32 against the
+ * FP64 master parameters).  NW_MPNN_BF16, the default: each hidden layer is
+ * one fused kernel that gathers the messages, rounds its inputs [h, m] and
+ * the layer weights to bfloat16 and multiplies them on the matrix cores
+ * with FP32 accumulation; activations are stored in bfloat16 and the output
+ * head runs in FP32 on the last layer's FP32 result.  NW_MPNN_FP32: the
+ * FP32 training forward. */
+enum {
+  NW_MPNN_FP32 = 0,
+  NW_MPNN_BF16 = 1
+};
+int nw_mpnn_set_precision(nw_mpnn* net, int precision);
 /* Network used by solves with x0 = NW_X0_LEARNED (NULL detaches). */
 int nw_linsys_set_learned_guess(nw_ctx* ctx, nw_mpnn* net);
 

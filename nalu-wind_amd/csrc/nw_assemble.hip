@@ -464,15 +464,6 @@ __device__ inline void row_butterfly(double* acc)
     for (int v = 0; v < NV; ++v) acc[v] += __shfl_xor(acc[v], o, G);
 }
 
-// Orders this wave's LDS accesses: LDS operations of one wave complete in
-// issue order, so only the compiler has to be kept from moving them.
-__device__ inline void wave_lds_sync()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Writes the (row, row) block and rhs(row) of a row, straight to HBM.
 template <int ND>
 __device__ inline void put_diag_rhs(double* __restrict__ dp,

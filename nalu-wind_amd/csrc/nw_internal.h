@@ -48,6 +48,15 @@ constexpr int kRowsPerWave = 64;
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 
+// Orders this wave's LDS accesses: LDS operations of one wave complete in
+// issue order, so only the compiler has to be kept from moving them.
+__device__ inline void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Slot code of a CSR (block) slot: diagonal slot of row n -> -(n+1);
 // off-diagonal slot reached through edge e from its node[side] -> 2e+side.
 __host__ __device__ inline int32_t diag_code(int32_t n) { return -(n + 1); }

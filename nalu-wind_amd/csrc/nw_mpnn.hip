@@ -48,6 +48,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -102,34 +103,35 @@ __global__ __launch_bounds__(kBlock) void k_mp_reduce(
 
 __device__ inline float relu(float z) { return z > 0.f ? z : 0.f; }
 
-// w[s] for every slot s of row i; the transposed entry a_ci is found by a
-// search of row c (the pattern of an edge graph is symmetric; a missing
-// entry counts as zero).
-__global__ void k_slot_weights(const int32_t* __restrict__ rp,
-                               const int32_t* __restrict__ ci,
+// eslot[code] = s for every off-diagonal slot (code = 2e + side): the
+// two slots (L, R) and (R, L) of edge e
+__global__ void k_edge_slots(const int32_t* __restrict__ code,
+                             int32_t* __restrict__ eslot, int64_t nnz)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const int32_t c = code[s];
+  if (c >= 0) eslot[c] = (int32_t)s;
+}
+
+// w = sqrt|a_LR a_RL| / sqrt|a_LL a_RR| for edge e = (L, R), written to
+// both of its slots (one thread per edge; the diagonal slots keep 0).
+__global__ void k_edge_weights(const int2* __restrict__ edges,
+                               const int2* __restrict__ eslot,
                                const double* __restrict__ va,
                                const int32_t* __restrict__ diag,
-                               float* __restrict__ w, int n)
+                               float* __restrict__ w, int64_t nedge)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double di = fabs(va[diag[i]]);
-  for (int s = rp[i]; s < rp[i + 1]; ++s) {
-    const int c = ci[s];
-    if (c == i) {
-      w[s] = 0.f;
-      continue;
-    }
-    const double dc = fabs(va[diag[c]]);
-    double act = 0.0;
-    for (int t = rp[c]; t < rp[c + 1]; ++t)
-      if (ci[t] == i) {
-        act = va[t];
-        break;
-      }
-    const double amn = fabs(va[s]) * fabs(act);
-    w[s] = (di > 0 && dc > 0) ? (float)(sqrt(amn) / sqrt(di * dc)) : 0.f;
-  }
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nedge) return;
+  const int2 lr = edges[e];
+  const int2 sl = eslot[e];
+  if (sl.x < 0 || sl.y < 0) return;  // not in the graph
+  const double dl = fabs(va[diag[lr.x]]), dr = fabs(va[diag[lr.y]]);
+  const double a = fabs(va[sl.x]) * fabs(va[sl.y]);
+  const float we = (dl > 0 && dr > 0) ? (float)(sqrt(a) / sqrt(dl * dr)) : 0.f;
+  w[sl.x] = we;
+  w[sl.y] = we;
 }
 
 // r = b - A x0 and the sums of b^2, x0^2, r^2, (r/|d|)^2, log|d|,
@@ -319,6 +321,224 @@ __global__ void k_output(const float* __restrict__ hL,
     }
   }
   delta[i] = a1 * f3[i] + a2 * f4[i];
+}
+
+// ------------------------------------------------- bfloat16 inference path
+// bfloat16 = the upper 16 bits of an FP32 value, rounded to nearest even.
+__host__ __device__ inline uint16_t f2bf(float f)
+{
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ inline float bf2f(uint32_t b16)
+{
+  return __uint_as_float(b16 << 16);
+}
+
+__device__ inline uint32_t pack_bf2(float lo, float hi)
+{
+  return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTileNodes = 32;  // nodes per wave tile of the fused layer
+
+// h0 = relu(W_in f + b_in) stored in bfloat16 (one thread per node)
+template <int H>
+__global__ __launch_bounds__(kBlock) void k_input_bf16(
+  const float* __restrict__ feat, const float* __restrict__ WinT,
+  const float* __restrict__ bin, uint16_t* __restrict__ h0, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float f[kF];
+#pragma unroll
+  for (int k = 0; k < kF; ++k) f[k] = feat[(int64_t)k * n + i];
+  uint4* out = reinterpret_cast<uint4*>(h0 + (int64_t)i * H);
+#pragma unroll
+  for (int j8 = 0; j8 < H / 8; ++j8) {
+    float z[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int j = 8 * j8 + q;
+      float a = bin[j];
+#pragma unroll
+      for (int k = 0; k < kF; ++k) a = fmaf(WinT[k * H + j], f[k], a);
+      z[q] = relu(a);
+    }
+    out[j8] = make_uint4(pack_bf2(z[0], z[1]), pack_bf2(z[2], z[3]),
+                         pack_bf2(z[4], z[5]), pack_bf2(z[6], z[7]));
+  }
+}
+
+// One hidden layer, fused: for a tile of 32 nodes a wave
+//   1. gathers the messages m_n = sum_s w_s h[col_s] in FP32 (H/8 lanes per
+//      node, each on 8 consecutive features, so a neighbour's 2H-byte row
+//      is read by consecutive lanes) and puts [h_n, bf16(m_n)] in LDS;
+//   2. multiplies the [32 x 2H] tile by [W_self W_nbr]^T (bfloat16, held in
+//      registers in the B-operand layout of v_mfma_f32_16x16x16_bf16) on
+//      the matrix cores, FP32 accumulation starting from the bias;
+//   3. LAST = false: relu, bfloat16, written back through LDS as whole
+//      rows; LAST = true: relu in FP32 and the output head
+//      delta = (w_out,1 . h + b_out,1) f3 + (w_out,2 . h + b_out,2) f4,
+//      the 16 columns of a tile summed by a fixed butterfly.
+// MFMA 16x16x16 operand layout: lane l holds A[l%16][4(l/16)..+3] and
+// B[4(l/16)..+3][l%16]; the result D[4(l/16)+t][l%16], t = 0..3.
+// The grid is persistent: wave w takes tiles w, w + waves, ...
+template <int H, bool LAST, int UNR>
+__global__ __launch_bounds__(kBlock) void k_layer_bf16(
+  const uint16_t* __restrict__ hin, const int32_t* __restrict__ rp,
+  const int32_t* __restrict__ ci, const float* __restrict__ w,
+  const bf16x4* __restrict__ wfrag, const float* __restrict__ bias,
+  uint16_t* __restrict__ hout, const float* __restrict__ wout, float b1,
+  float b2, const float* __restrict__ f3, const float* __restrict__ f4,
+  float* __restrict__ delta, int n)
+{
+  constexpr int NT = H / 16, KS = 2 * H / 16;
+  constexpr int LPN = H / 8;         // lanes per node row
+  constexpr int NPR = kWave / LPN;   // nodes per gather round
+  constexpr int TN = kTileNodes;
+  constexpr int AS = 2 * H + 8;      // LDS row stride (bf16), padded
+  constexpr int WPB = kBlock / kWave;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[WPB][TN * AS];
+  static_assert((TN * AS * 2) % 16 == 0, "16-byte aligned rows per wave");
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = (int)(threadIdx.x / kWave);
+  const int col = lane & 15, grp = lane >> 4;
+  bf16x4 wf[NT][KS];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) wf[nt][ks] = wfrag[(nt * KS + ks) * kWave + lane];
+  float bs[NT], wo1[NT], wo2[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    bs[nt] = bias[nt * 16 + col];
+    wo1[nt] = LAST ? wout[nt * 16 + col] : 0.f;
+    wo2[nt] = LAST ? wout[H + nt * 16 + col] : 0.f;
+  }
+  uint16_t* A = sA[wv];
+  const int sub = lane % LPN;
+  const int ntiles = (n + TN - 1) / TN;
+  const int nwaves = (int)gridDim.x * WPB;
+  for (int t = (int)blockIdx.x * WPB + wv; t < ntiles; t += nwaves) {
+    const int i0 = t * TN;
+    // 1. gather
+#pragma unroll 1
+    for (int r = 0; r < TN / NPR; ++r) {
+      const int q = r * NPR + lane / LPN;
+      const int i = i0 + q;
+      uint4 own = make_uint4(0u, 0u, 0u, 0u);
+      float a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = 0.f;
+      if (i < n) {
+        const uint4* h4 = reinterpret_cast<const uint4*>(hin);
+        own = h4[(int64_t)i * LPN + sub];
+        const int s1 = rp[i + 1];
+        // UNR slots at a time: their columns, weights and neighbour rows
+        // are independent loads; the sum runs in slot order
+        for (int s0 = rp[i]; s0 < s1; s0 += UNR) {
+          int cc[UNR];
+          float ww[UNR];
+#pragma unroll
+          for (int u = 0; u < UNR; ++u) {
+            const bool ok = s0 + u < s1;
+            cc[u] = ok ? ci[s0 + u] : i;
+            ww[u] = ok ? w[s0 + u] : 0.f;
+          }
+          uint4 hv[UNR];
+#pragma unroll
+          for (int u = 0; u < UNR; ++u) hv[u] = h4[(int64_t)cc[u] * LPN + sub];
+#pragma unroll
+          for (int u = 0; u < UNR; ++u) {
+            const uint32_t hw[4] = {hv[u].x, hv[u].y, hv[u].z, hv[u].w};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              a[2 * v] = fmaf(ww[u], bf2f(hw[v] & 0xFFFFu), a[2 * v]);
+              a[2 * v + 1] = fmaf(ww[u], bf2f(hw[v] >> 16), a[2 * v + 1]);
+            }
+          }
+        }
+      }
+      uint4* row = reinterpret_cast<uint4*>(A + q * AS);
+      row[sub] = own;
+      row[LPN + sub] = make_uint4(pack_bf2(a[0], a[1]), pack_bf2(a[2], a[3]),
+                                  pack_bf2(a[4], a[5]), pack_bf2(a[6], a[7]));
+    }
+    wave_lds_sync();
+    // 2./3. matrix cores, then the epilogue
+#pragma unroll
+    for (int mt = 0; mt < TN / 16; ++mt) {
+      f32x4 acc[NT];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[nt] = (f32x4){bs[nt], bs[nt], bs[nt], bs[nt]};
+      const uint16_t* ar = A + (mt * 16 + col) * AS + 4 * grp;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const bf16x4 av = *reinterpret_cast<const bf16x4*>(ar + ks * 16);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wf[nt][ks],
+                                                             acc[nt], 0, 0, 0);
+      }
+      if (!LAST) {
+        // this M tile's rows of A are no longer read: reuse them for the
+        // output rows (first H columns)
+        wave_lds_sync();
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+          for (int tt = 0; tt < 4; ++tt)
+            A[(mt * 16 + 4 * grp + tt) * AS + nt * 16 + col] =
+              f2bf(fmaxf(acc[nt][tt], 0.f));
+      } else {
+        float p1[4], p2[4];
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) {
+          p1[tt] = 0.f;
+          p2[tt] = 0.f;
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) {
+            const float hv = fmaxf(acc[nt][tt], 0.f);
+            p1[tt] = fmaf(wo1[nt], hv, p1[tt]);
+            p2[tt] = fmaf(wo2[nt], hv, p2[tt]);
+          }
+        }
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+          for (int tt = 0; tt < 4; ++tt) {
+            p1[tt] += __shfl_xor(p1[tt], o, 16);
+            p2[tt] += __shfl_xor(p2[tt], o, 16);
+          }
+        if (col == 0) {
+#pragma unroll
+          for (int tt = 0; tt < 4; ++tt) {
+            const int i = i0 + mt * 16 + 4 * grp + tt;
+            if (i < n) delta[i] = (p1[tt] + b1) * f3[i] + (p2[tt] + b2) * f4[i];
+          }
+        }
+      }
+    }
+    wave_lds_sync();
+    if (!LAST) {
+#pragma unroll 1
+      for (int r = 0; r < TN / NPR; ++r) {
+        const int q = r * NPR + lane / LPN;
+        const int i = i0 + q;
+        if (i < n)
+          reinterpret_cast<uint4*>(hout + (int64_t)i * H)[sub] =
+            reinterpret_cast<const uint4*>(A + q * AS)[sub];
+      }
+      wave_lds_sync();
+    }
+  }
 }
 
 // x = xb + s_o delta; Dirichlet rows (feature 2 set) get b / a_nn
@@ -636,6 +856,12 @@ struct Mpnn {
   double so = 1.0;         // output scale of the last features
   double cb = 0.0;         // 1 when the last features had history
   double last_ms = 0.0;
+  int precision = NW_MPNN_BF16;  // arithmetic of the prediction
+  uint16_t* hb[2] = {nullptr, nullptr};  // bfloat16 activations [n][H]
+  uint16_t* wfrag = nullptr;  // per layer [H/16][2H/16][64 lanes][4] bf16
+  int32_t* eslot = nullptr;   // [nedge][2] the slots (L, R), (R, L) of an edge
+  const int32_t* eslot_code = nullptr;  // the graph eslot was built for
+  int64_t eslot_nedge = -1;
 };
 
 void mpnn_detach(Ctx* c, Mpnn* m);
@@ -681,9 +907,28 @@ int upload_params(Mpnn* m)
         wn[(size_t)k * H + j] = f[o.wn(l, H) + (int64_t)j * H + k];
       }
   }
+  // bfloat16 weights of the fused layers in the MFMA B-operand layout:
+  // fragment (nt, ks), lane l, item t holds W^T[k][j] with
+  // k = 16 ks + 4 (l / 16) + t over [h, m] (W_self then W_nbr) and
+  // j = 16 nt + l % 16 the output feature
+  const int NT = H / 16, KS = 2 * H / 16;
+  std::vector<uint16_t> wb((size_t)L * NT * KS * kWave * 4);
+  for (int l = 0; l < L; ++l)
+    for (int nt = 0; nt < NT; ++nt)
+      for (int ks = 0; ks < KS; ++ks)
+        for (int ln = 0; ln < kWave; ++ln)
+          for (int tt = 0; tt < 4; ++tt) {
+            const int k = 16 * ks + 4 * (ln / 16) + tt, j = 16 * nt + ln % 16;
+            const float v = k < H ? f[o.ws(l) + (int64_t)j * H + k]
+                                  : f[o.wn(l, H) + (int64_t)j * H + (k - H)];
+            wb[((((size_t)l * NT + nt) * KS + ks) * kWave + ln) * 4 + tt] = f2bf(v);
+          }
+  if (!m->wfrag) NW_HIP(hipMalloc(&m->wfrag, sizeof(uint16_t) * wb.size()));
   NW_HIP(hipMemcpyAsync(m->dP, f.data(), sizeof(float) * m->np,
                         hipMemcpyHostToDevice, m->c->stream));
   NW_HIP(hipMemcpyAsync(m->dPT, t.data(), sizeof(float) * t.size(),
+                        hipMemcpyHostToDevice, m->c->stream));
+  NW_HIP(hipMemcpyAsync(m->wfrag, wb.data(), sizeof(uint16_t) * wb.size(),
                         hipMemcpyHostToDevice, m->c->stream));
   NW_HIP(hipStreamSynchronize(m->c->stream));
   return NW_OK;
@@ -740,9 +985,29 @@ int ensure_buffers(Mpnn* m, bool training)
 
 int slot_weights(Mpnn* m)
 {
-  const Graph& g = m->c->g;
-  k_slot_weights<<<grid_for(m->n, kBlock), kBlock, 0, m->c->stream>>>(
-    g.row_ptr, g.cols, g.vals, g.diag, m->w, (int)m->n);
+  Ctx* c = m->c;
+  const Graph& g = c->g;
+  hipStream_t st = c->stream;
+  const int64_t ne = std::max<int64_t>(c->nedge, 0);
+  if (m->eslot && (m->eslot_code != g.code || m->eslot_nedge != ne)) {
+    (void)hipFree(m->eslot);  // the graph was rebuilt
+    m->eslot = nullptr;
+  }
+  if (!m->eslot) {
+    // the two slots of every edge, built once per network; the diagonal
+    // (and any slot not reached through an edge) keeps weight 0
+    NW_HIP(hipMalloc(&m->eslot, sizeof(int32_t) * std::max<int64_t>(2 * ne, 1)));
+    NW_HIP(hipMemsetAsync(m->eslot, 0xFF, sizeof(int32_t) * std::max<int64_t>(2 * ne, 1), st));
+    NW_HIP(hipMemsetAsync(m->w, 0, sizeof(float) * m->nnz, st));
+    k_edge_slots<<<grid_for(m->nnz, kBlock), kBlock, 0, st>>>(g.code, m->eslot, m->nnz);
+    NW_LAUNCH_CHECK();
+    m->eslot_code = g.code;
+    m->eslot_nedge = ne;
+  }
+  if (ne == 0) return NW_OK;
+  k_edge_weights<<<grid_for(ne, kBlock), kBlock, 0, st>>>(
+    reinterpret_cast<const int2*>(c->edges), reinterpret_cast<const int2*>(m->eslot),
+    g.vals, g.diag, m->w, ne);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }
@@ -821,6 +1086,54 @@ int forward_h(Mpnn* m, const float* feat, bool keep)
 int forward(Mpnn* m, const float* feat, bool keep)
 {
   return m->H == 64 ? forward_h<64>(m, feat, keep) : forward_h<32>(m, feat, keep);
+}
+
+// The bfloat16 prediction forward: delta from the features.
+template <int H>
+int forward_bf16_h(Mpnn* m, const float* feat)
+{
+  Ctx* c = m->c;
+  const Graph& g = c->g;
+  const int n = (int)m->n, L = m->L;
+  hipStream_t st = c->stream;
+  const Off o(H, L);
+  for (auto& p : m->hb)
+    if (!p) NW_HIP(hipMalloc(&p, sizeof(uint16_t) * (size_t)n * H));
+  k_input_bf16<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(
+    feat, m->dPT, m->dP + o.bin, m->hb[0], n);
+  NW_LAUNCH_CHECK();
+  const int64_t ntiles = ((int64_t)n + kTileNodes - 1) / kTileNodes;
+  const int wpb = kBlock / kWave;
+  const unsigned blocks =
+    (unsigned)std::max<int64_t>(1, std::min<int64_t>((ntiles + wpb - 1) / wpb, 2048));
+  const size_t fl = (size_t)(H / 16) * (2 * H / 16) * kWave;  // fragments per layer
+  const bf16x4* wf = reinterpret_cast<const bf16x4*>(m->wfrag);
+  const float b1 = (float)m->P[o.bout], b2 = (float)m->P[o.bout + 1];
+  const float* f3 = feat + 3 * (int64_t)n;
+  const float* f4 = feat + 4 * (int64_t)n;
+  int cur = 0;
+  for (int l = 0; l < L; ++l) {
+#define NW_LAYER(LASTV)                                                        \
+  k_layer_bf16<H, LASTV, 8><<<blocks, kBlock, 0, st>>>(                        \
+    m->hb[cur], g.row_ptr, g.cols, m->w, wf + l * fl, m->dP + o.b(l, H),       \
+    LASTV ? nullptr : m->hb[1 - cur], m->dP + o.wout, b1, b2, f3, f4,          \
+    m->delta, n)
+    if (l + 1 < L)
+      NW_LAYER(false);
+    else
+      NW_LAYER(true);
+#undef NW_LAYER
+    NW_LAUNCH_CHECK();
+    cur = 1 - cur;
+  }
+  return NW_OK;
+}
+
+int forward_predict(Mpnn* m, const float* feat)
+{
+  if (m->precision == NW_MPNN_BF16)
+    return m->H == 64 ? forward_bf16_h<64>(m, feat) : forward_bf16_h<32>(m, feat);
+  return forward(m, feat, false);
 }
 
 // Loss and gradient of one sample accumulated into grad (scaled by 1/batch).
@@ -939,7 +1252,7 @@ int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp,
   if (rc) return rc;
   rc = features(m, x0, xp, xpp, nullptr, m->feat, nullptr);
   if (rc) return rc;
-  rc = forward(m, m->feat, false);
+  rc = forward_predict(m, m->feat);
   if (rc) return rc;
   const Graph& g = m->c->g;
   k_apply_guess<<<grid_for(m->n, kBlock), kBlock, 0, m->c->stream>>>(
@@ -1046,6 +1359,9 @@ void nw_mpnn_destroy(nw_mpnn* m)
   (void)hipFree(m->r);
   (void)hipFree(m->partial);
   (void)hipFree(m->red);
+  for (auto p : m->hb) (void)hipFree(p);
+  (void)hipFree(m->wfrag);
+  (void)hipFree(m->eslot);
   delete m;
 }
 
@@ -1245,6 +1561,15 @@ int nw_mpnn_predict(nw_mpnn* m, double* x_host, double* scale)
   (void)hipFree(x);
   if (scale) *scale = m->so;
   return rc;
+}
+
+int nw_mpnn_set_precision(nw_mpnn* m, int precision)
+{
+  NW_REQUIRE(m, NW_E_ARG, "null argument");
+  NW_REQUIRE(precision == NW_MPNN_FP32 || precision == NW_MPNN_BF16, NW_E_ARG,
+             "bad precision %d", precision);
+  m->precision = precision;
+  return NW_OK;
 }
 
 int nw_linsys_set_learned_guess(nw_ctx* c, nw_mpnn* m)

@@ -35,6 +35,7 @@ OK, E_ARG, E_STATE, E_HIP, E_NOGRAPHCOL, E_NONFINITE, E_NOCONV, E_ALLOC = (
 PREC_NONE, PREC_JACOBI, PREC_CHEBYSHEV, PREC_AMG = 0, 1, 2, 3
 SOLVE_FP64, SOLVE_FP32_REFINE = 0, 1
 X0_ZERO, X0_SOLUTION, X0_LEARNED = 0, 1, 2
+MPNN_FP32, MPNN_BF16 = 0, 1
 
 
 class NwError(RuntimeError):
@@ -176,6 +177,7 @@ _sig = {
     "nw_mpnn_loss_grad": ([_P, C.c_int, C.POINTER(C.c_double), _P], C.c_int),
     "nw_mpnn_train": ([_P, C.POINTER(MpnnTrainCfg), C.POINTER(C.c_double)], C.c_int),
     "nw_mpnn_predict": ([_P, _f64p, C.POINTER(C.c_double)], C.c_int),
+    "nw_mpnn_set_precision": ([_P, C.c_int], C.c_int),
     "nw_linsys_set_learned_guess": ([_P, _P], C.c_int),
     "nw_mixed_box": ([C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                       C.POINTER(C.c_int64), C.POINTER(C.c_int64), _P, _P, _P],
@@ -482,6 +484,17 @@ class Mpnn:
         ctx._nets.add(self)
         self.hidden, self.layers = hidden, layers
         self.nparams = lib.nw_mpnn_nparams(hidden, layers)
+        self._precision = MPNN_BF16
+
+    @property
+    def precision(self):
+        """Arithmetic of the prediction: MPNN_BF16 (default) or MPNN_FP32."""
+        return self._precision
+
+    @precision.setter
+    def precision(self, p):
+        check(lib.nw_mpnn_set_precision(self.h, p))
+        self._precision = p
 
     def close(self):
         if self.h:

@@ -336,6 +336,10 @@ void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
 void orc_mpnn_forward(const double* P, int H, int L, const orc_csr* A,
                       const double* w, const double* feat, double* delta,
                       double* acts);
+/* The same with bfloat16 activations, messages and layer weights, as the
+ * NW_MPNN_BF16 prediction computes them. */
+void orc_mpnn_forward_bf16(const double* P, int H, int L, const orc_csr* A,
+                           const double* w, const double* feat, double* delta);
 double orc_mpnn_loss_grad(const double* P, int H, int L, const orc_csr* A,
                           const double* w, const double* feat,
                           const double* target, double* grad);

@@ -422,6 +422,9 @@ lib.orc_mpnn_features.argtypes = [C.POINTER(Csr), _f64p, _f64p, _f64p, _f64p,
                                   _f64p, _f64p]
 lib.orc_mpnn_forward.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr), _f64p,
                                  _f64p, _f64p, C.c_void_p]
+lib.orc_mpnn_forward_bf16.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr),
+                                      _f64p, _f64p, _f64p]
+lib.orc_mpnn_forward_bf16.restype = None
 lib.orc_mpnn_loss_grad.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr),
                                    _f64p, _f64p, _f64p, _f64p]
 lib.orc_mpnn_loss_grad.restype = C.c_double
@@ -454,6 +457,12 @@ def mpnn_features(A, b, x0, xp=None, xpp=None):
 def mpnn_forward(P, H, L, A, w, feat):
     d = np.zeros(A.c.nrows)
     lib.orc_mpnn_forward(f64(P), H, L, A.ref, f64(w), f64(feat), d, None)
+    return d
+
+
+def mpnn_forward_bf16(P, H, L, A, w, feat):
+    d = np.zeros(A.c.nrows)
+    lib.orc_mpnn_forward_bf16(f64(P), H, L, A.ref, f64(w), f64(feat), d)
     return d
 
 

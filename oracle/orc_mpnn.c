@@ -28,6 +28,7 @@
  * W_nbr[H][H], b[H], then w_out[2][H], b_out[2].
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -181,6 +182,79 @@ void orc_mpnn_forward(const double* P, int H, int L, const orc_csr* A,
   free(h);
   free(h2);
   free(m);
+}
+
+/* Nearest-even bfloat16 rounding of x taken to FP32 first (the device
+ * rounds FP32 values). */
+static double bf16_round(double x)
+{
+  float f = (float)x;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  u &= 0xFFFF0000u;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+static double fl(double x) { return (double)(float)x; }
+
+/* The reduced-precision forward of the prediction path (NW_MPNN_BF16):
+ * the hidden activations h_0..h_{L-1}, the messages m_l and the layer
+ * weights W_self, W_nbr are rounded to bfloat16; the features, slot
+ * weights, W_in, the biases and the output head are rounded to FP32; the
+ * last layer's activations stay unrounded; sums in FP64. */
+void orc_mpnn_forward_bf16(const double* P, int H, int L, const orc_csr* A,
+                           const double* w, const double* feat, double* delta)
+{
+  const int64_t n = A->nrows;
+  const double* Win = P;
+  const double* bin = Win + (int64_t)H * NF;
+  double* h = (double*)malloc(sizeof(double) * (size_t)(n * H));
+  double* h2 = (double*)malloc(sizeof(double) * (size_t)(n * H));
+  double* m = (double*)malloc(sizeof(double) * (size_t)(n * H));
+  double* wf = (double*)malloc(sizeof(double) * (size_t)A->row_ptr[n]);
+  for (int64_t k = 0; k < A->row_ptr[n]; ++k) wf[k] = fl(w[k]);
+  for (int64_t i = 0; i < n; ++i)
+    for (int f = 0; f < H; ++f) {
+      double z = fl(bin[f]);
+      for (int k = 0; k < NF; ++k) z += fl(Win[f * NF + k]) * fl(feat[NF * i + k]);
+      h[H * i + f] = bf16_round(z > 0 ? z : 0.0);
+    }
+  const double* Pl = bin + H;
+  for (int l = 0; l < L; ++l) {
+    const double* Ws = Pl;
+    const double* Wn = Ws + H * H;
+    const double* bl = Wn + H * H;
+    aggregate(A, wf, H, h, m);
+    for (int64_t i = 0; i < n * H; ++i) m[i] = bf16_round(m[i]);
+    for (int64_t i = 0; i < n; ++i)
+      for (int f = 0; f < H; ++f) {
+        double z = fl(bl[f]);
+        for (int k = 0; k < H; ++k)
+          z += bf16_round(Ws[f * H + k]) * h[H * i + k] +
+               bf16_round(Wn[f * H + k]) * m[H * i + k];
+        const double r = z > 0 ? z : 0.0;
+        h2[H * i + f] = l + 1 < L ? bf16_round(r) : r;
+      }
+    double* t = h;
+    h = h2;
+    h2 = t;
+    Pl = bl + H;
+  }
+  const double* wout = Pl;
+  for (int64_t i = 0; i < n; ++i) {
+    double a1 = fl(wout[2 * H]), a2 = fl(wout[2 * H + 1]);
+    for (int f = 0; f < H; ++f) {
+      a1 += fl(wout[f]) * h[H * i + f];
+      a2 += fl(wout[H + f]) * h[H * i + f];
+    }
+    delta[i] = a1 * fl(feat[NF * i + 3]) + a2 * fl(feat[NF * i + 4]);
+  }
+  free(h);
+  free(h2);
+  free(m);
+  free(wf);
 }
 
 /* Loss = mean((delta - t)^2); grad (nparams) receives dLoss/dP. */

@@ -91,8 +91,11 @@ def test_gradient_matches_oracle(nw, ctx):
 
 
 def test_prediction_matches_oracle(nw, ctx):
+    """The FP32 prediction (the training forward) against the FP64 oracle."""
     P, A, top = _system(nw, ctx, seed=6)
     net = nw.Mpnn(ctx, H, L, seed=3)
+    assert net.precision == nw.MPNN_BF16  # the default
+    net.precision = nw.MPNN_FP32
     p = _params(nw, 7)
     net.params = p
     rng = np.random.default_rng(8)
@@ -119,6 +122,42 @@ def test_prediction_matches_oracle(nw, ctx):
                                    rtol=0, atol=2e-5 * np.abs(d).max())
         # Dirichlet rows take b / a_nn (here 0)
         np.testing.assert_array_equal(x[~inner], A.rhs[~inner])
+
+
+@pytest.mark.parametrize("seed", [6, 12])
+def test_bf16_prediction_matches_oracle(nw, ctx, seed):
+    """The default prediction (fused bfloat16 layers on the matrix cores)
+    against the oracle's bfloat16 forward, which rounds at the same points:
+    agreement to FP32 summation-order level, apart from the odd activation
+    whose FP32 value lands on the other side of a bfloat16 rounding boundary.
+    The graph (504 nodes) ends in a partial 32-node tile."""
+    P, A, top = _system(nw, ctx, seed=seed)
+    net = nw.Mpnn(ctx, H, L, seed=3)
+    p = _params(nw, seed + 1)
+    net.params = p
+    rng = np.random.default_rng(seed + 2)
+    x0, xp, xpp = (rng.normal(size=P.n) for _ in range(3))
+    for g in (xpp, xp):
+        ctx.set_solution(g)
+        ctx.solve(nw.solver_cfg(kspace=30, max_iters=10, tol=1e-3,
+                                x0=nw.X0_SOLUTION), raise_on_noconv=False)
+    ctx.set_solution(x0)
+    x, so = net.predict()
+    w = orc.mpnn_slot_weights(A)
+    feat, sc = orc.mpnn_features(A, A.rhs, x0, xp, xpp)
+    inner = feat[:, 2] == 0
+    got = ((x - (2 * x0 - xp)) / so)[inner]
+    d_bf = orc.mpnn_forward_bf16(p, H, L, A, w, feat)[inner]
+    d_64 = orc.mpnn_forward(p, H, L, A, w, feat)[inner]
+    err_bf = np.linalg.norm(got - d_bf) / np.linalg.norm(d_bf)
+    err_64 = np.linalg.norm(got - d_64) / np.linalg.norm(d_64)
+    print("bf16 prediction: rel. error vs bf16 oracle %.2e, vs FP64 oracle %.2e"
+          % (err_bf, err_64))
+    assert err_bf < 3e-3
+    assert np.abs(got - d_bf).max() < 3e-2 * np.abs(d_bf).max()
+    # the bfloat16 effect itself is visible against the FP64 forward
+    assert err_bf < err_64 < 5e-2
+    np.testing.assert_array_equal(x[~inner], A.rhs[~inner])
 
 
 def test_training_is_deterministic_and_lowers_loss(nw, ctx):

@@ -457,3 +457,65 @@ def test_mpnn_slot_weights_and_features():
     flag = np.zeros(n)
     flag[top] = 1
     np.testing.assert_array_equal(feat[:, 2], flag)
+
+
+def _bf16(x):
+    """Nearest-even bfloat16 of x via FP32 (numpy bit arithmetic)."""
+    u = np.array([x], dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return float(u.astype(np.uint32).view(np.float32)[0])
+
+
+def test_bf16_rounding_helper_closed_forms():
+    """The rounding used below: ties go to the even mantissa."""
+    assert _bf16(1.0) == 1.0
+    assert _bf16(1 + 2 ** -8) == 1.0                      # tie, even below
+    assert _bf16(1 + 3 * 2 ** -8) == 1 + 2 ** -6          # tie, even above
+    assert _bf16(1 + 2 ** -7 + 2 ** -9) == 1 + 2 ** -7    # quarter ulp down
+    assert _bf16(2.5078125) == 2.5                        # tie, even below
+
+
+def test_mpnn_bf16_forward_rounds_where_the_device_rounds():
+    """Closed form on a two-node graph whose values sit on bfloat16 ties:
+    h0 = bf16(1 + 2^-8) = 1, m = bf16(0.5 h0) = 0.5, W_self = 1 + 2^-7 + 2^-9
+    rounds to 1 + 2^-7, W_nbr = 3, and the last layer is not rounded, so
+    delta = (1 + 2^-7) + 3 * 0.5 = 2.5078125 exactly.  Rounding the last
+    layer would give 2.5, keeping W_self 2.5078125 + 2^-9, keeping h0 or m
+    other values again; the FP64 forward gives its own exact sum."""
+    D = np.array([[2.0, -1.0], [-1.0, 2.0]])
+    A = orc.CsrSystem.from_dense(D, np.zeros(2))
+    w = orc.mpnn_slot_weights(A)
+    H, L = 16, 1
+    P = np.zeros(orc.mpnn_nparams(H, L))
+    win, bin_, ws = 0, H * 5, H * 5 + H
+    wn, wout = ws + H * H, ws + 2 * H * H + H
+    P[win + 0 * 5 + 0] = 1 + 2 ** -8        # W_in[0][0]
+    P[ws + 0 * H + 0] = 1 + 2 ** -7 + 2 ** -9  # W_self[0][0]
+    P[wn + 0 * H + 0] = 3.0                 # W_nbr[0][0]
+    P[wout + 0] = 1.0                       # w_out,1[0]
+    feat = np.zeros((2, 5))
+    feat[:, 0] = 1.0
+    feat[:, 3] = 1.0
+    np.testing.assert_array_equal(w[w != 0], 0.5)
+    d = orc.mpnn_forward_bf16(P, H, L, A, w, feat)
+    np.testing.assert_array_equal(d, [2.5078125, 2.5078125])
+    h0 = 1 + 2 ** -8
+    exact = (1 + 2 ** -7 + 2 ** -9) * h0 + 3.0 * 0.5 * h0
+    np.testing.assert_allclose(orc.mpnn_forward(P, H, L, A, w, feat),
+                               [exact, exact], rtol=1e-15)
+
+
+def test_mpnn_bf16_forward_stays_near_fp64():
+    """On a random network the bfloat16 forward differs from the FP64 one
+    by the size bfloat16 rounding explains (not more, not zero)."""
+    A, top, rng = _mpnn_system(6)
+    n = A.c.nrows
+    H, L = 32, 3
+    P = rng.normal(size=orc.mpnn_nparams(H, L)) * 0.3
+    w = orc.mpnn_slot_weights(A)
+    feat, _ = orc.mpnn_features(A, A.rhs, rng.normal(size=n),
+                                rng.normal(size=n), rng.normal(size=n))
+    d = orc.mpnn_forward(P, H, L, A, w, feat)
+    db = orc.mpnn_forward_bf16(P, H, L, A, w, feat)
+    rel = np.linalg.norm(db - d) / np.linalg.norm(d)
+    assert 1e-4 < rel < 3e-2
