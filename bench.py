@@ -191,7 +191,7 @@ def pressure_solves(ns, ntrain, neval, epochs, log):
                                            holdout=3))
         t_train = time.perf_counter() - t0
         res["learned_guess"] = {
-            "network": "3 message-passing layers, 64 hidden, FP32",
+            "network": "3 message-passing layers, 64 hidden; prediction in bfloat16 weights and activations on the matrix cores with FP32 accumulation, training in FP32",
             "pairs": net.samples, "validation_pairs": 3, "epochs": epochs,
             "adam_lr": 1e-4,
             "best_validation_loss": loss, "pair_generation_s": round(t_gen, 1),

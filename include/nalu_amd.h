@@ -250,6 +250,13 @@ int nw_linsys_solution_device_ptr(nw_ctx* ctx, double** dptr);
 /* y = A x on device arrays of nrows doubles (FP64 residual checks). */
 int nw_linsys_spmv(nw_ctx* ctx, const double* d_x, double* d_y);
 
+/* Rows and stored entries of level `level` of the context's AMG hierarchy
+ * (0 = the fine matrix; built by the first AMG-preconditioned solve), the
+ * hierarchy summary MueLu prints at setup (its operator complexity is
+ * sum(nnz) / nnz(level 0)).  NW_E_ARG past the last level or without a
+ * hierarchy. */
+int nw_linsys_amg_level(nw_ctx* ctx, int level, int64_t* rows, int64_t* nnz);
+
 /* ------------------------------------------------------------------ */
 /* Learned initial guess for the pressure solve                        */
 /* ------------------------------------------------------------------ */

@@ -13,7 +13,8 @@ int amg_setup(Ctx* c, AmgDevice** out);
 void amg_free(AmgDevice* a);
 int amg_levels(const AmgDevice* a);
 double amg_setup_seconds(const AmgDevice* a);
-int64_t amg_level_rows(const AmgDevice* a, int level);
+// rows and stored entries of level `level` (0 = the fine matrix)
+int amg_level_info(const AmgDevice* a, int level, int64_t* rows, int64_t* nnz);
 // Uploads the hierarchy in type T on first use and refreshes the fine-level
 // values from the current matrix.
 template <typename T>

@@ -328,9 +328,13 @@ int amg_setup(Ctx* c, AmgDevice** out)
 
 int amg_levels(const AmgDevice* a) { return a ? a->nlev : 0; }
 double amg_setup_seconds(const AmgDevice* a) { return a ? a->setup_seconds : 0; }
-int64_t amg_level_rows(const AmgDevice* a, int l)
+int amg_level_info(const AmgDevice* a, int l, int64_t* rows, int64_t* nnz)
 {
-  return (a && l < a->nlev) ? (l == 0 ? -1 : a->hostlv[l].A.nrows) : 0;
+  if (!a || l < 0 || l >= a->nlev) return NW_E_ARG;
+  const amg::Level& h = a->hostlv[l];
+  *rows = l == 0 ? (int64_t)h.dinv.size() : h.A.nrows;
+  *nnz = l == 0 ? a->fine_nnz : h.A.nnz();
+  return NW_OK;
 }
 
 template <typename T>

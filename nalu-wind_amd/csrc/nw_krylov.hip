@@ -213,9 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(
     const int64_t t = t0 + threadIdx.x;
     const int row = (int)(t / G);
     const int g = threadIdx.x & (G - 1);
-    double s = 0.0;
-    if (row < n)
-      for (int k = rp[row] + g; k < rp[row + 1]; k += G) s += va[k] * x[ci[k]];
+    double s = row < n ? csr_dot<double, G>(rp, ci, va, x, row, g) : 0.0;
     for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
     if (row < n && g == 0) {
       const double ri = b[row] - s;
@@ -590,6 +588,12 @@ struct Solve {
     const int nb = (int)std::min<int64_t>(grid_for((int64_t)n * G, kBlock),
                                           kPartialCap);
     switch (G) {
+    case 1:
+      k_residual<1><<<nb, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, c->g.vals, x, c->g.rhs, k->r, n, k->partial);
+      break;
+    case 2:
+      k_residual<2><<<nb, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, c->g.vals, x, c->g.rhs, k->r, n, k->partial);
+      break;
     case 4:
       k_residual<4><<<nb, kBlock, 0, st>>>(c->g.row_ptr, c->g.cols, c->g.vals, x, c->g.rhs, k->r, n, k->partial);
       break;
@@ -777,6 +781,15 @@ struct Solve {
 using namespace nw;
 
 extern "C" {
+
+int nw_linsys_amg_level(nw_ctx* c, int level, int64_t* rows, int64_t* nnz)
+{
+  NW_REQUIRE(c && rows && nnz, NW_E_ARG, "null argument");
+  NW_REQUIRE(c->amg, NW_E_ARG, "no AMG hierarchy (run an AMG solve first)");
+  NW_REQUIRE(amg_level_info(c->amg, level, rows, nnz) == NW_OK, NW_E_ARG,
+             "level %d out of range", level);
+  return NW_OK;
+}
 
 int nw_linsys_spmv(nw_ctx* c, const double* d_x, double* d_y)
 {

@@ -10,12 +10,44 @@
 
 namespace nw {
 
-// Lanes per row for rows of `avg` entries: about two entries per lane
-// (measured on the 7-entry rows of the hex pressure system: 4 lanes beat 8
-// by 4-12 %, 2 lanes are 1.5x slower, 1 lane 2.7x).
+// Lanes per row for CSR rows of `avg` entries: each lane takes kCsrUnroll
+// entries per step as independent loads, so a row of average length is
+// covered by about one step (measured on the 20-entry AMG coarse operator
+// at 256^3: 8 lanes x 4 loads beat 16 lanes x 1 load by ~1.6x).
+constexpr int kCsrUnroll = 4;
 inline int lanes_per_row(double avg)
 {
-  return avg <= 9.0 ? 4 : (avg <= 18.0 ? 8 : (avg <= 36.0 ? 16 : 32));
+  int g = 1;
+  while (g < 32 && g * kCsrUnroll < avg) g *= 2;
+  return g;
+}
+
+// Partial sum of row `row` over the entries g, g+G, ... (lane g of G),
+// kCsrUnroll entries per step loaded before any is used; padding entries
+// repeat the row's last column with value 0.  Rows are contiguous, so the
+// G lanes of a row read consecutive entries.
+template <typename T, int G>
+__device__ inline T csr_dot(const int32_t* __restrict__ rp,
+                            const int32_t* __restrict__ ci,
+                            const T* __restrict__ va, const T* __restrict__ x,
+                            int row, int g)
+{
+  const int k0 = rp[row], k1 = rp[row + 1];
+  T s = 0;
+  for (int k = k0 + g; k < k1; k += G * kCsrUnroll) {
+    int c[kCsrUnroll];
+    T v[kCsrUnroll];
+#pragma unroll
+    for (int u = 0; u < kCsrUnroll; ++u) {
+      const int kk = k + u * G;
+      const bool ok = kk < k1;
+      c[u] = ci[ok ? kk : k1 - 1];
+      v[u] = ok ? va[kk] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kCsrUnroll; ++u) s += v[u] * x[c[u]];
+  }
+  return s;
 }
 
 // y = alpha * (A x); G lanes per row.
@@ -28,12 +60,22 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int row = (int)(t / G);
   const int g = threadIdx.x & (G - 1);
-  T s = 0;
-  if (row < n) {
-    for (int k = rp[row] + g; k < rp[row + 1]; k += G) s += va[k] * x[ci[k]];
-  }
+  T s = row < n ? csr_dot<T, G>(rp, ci, va, x, row, g) : (T)0;
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   if (row < n && g == 0) y[row] = alpha * s;
+}
+
+// d = dinv * v * inv_theta; z = d
+template <typename T>
+__global__ void k_cheb_init(const T* __restrict__ v, const T* __restrict__ dinv,
+                            T* __restrict__ d, T* __restrict__ z, int n,
+                            T inv_theta)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T di = dinv[i] * v[i] * inv_theta;
+  d[i] = di;
+  z[i] = di;
 }
 
 // Chebyshev step: r = v - A z; d = c1 d + c2 dinv r; zo = z + d.
@@ -47,10 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int row = (int)(t / G);
   const int g = threadIdx.x & (G - 1);
-  T s = 0;
-  if (row < n) {
-    for (int k = rp[row] + g; k < rp[row + 1]; k += G) s += va[k] * z[ci[k]];
-  }
+  T s = row < n ? csr_dot<T, G>(rp, ci, va, z, row, g) : (T)0;
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   if (row < n && g == 0) {
     const T dn = c1 * d[row] + c2 * dinv[row] * (v[row] - s);
@@ -68,10 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_add(
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int row = (int)(t / G);
   const int g = threadIdx.x & (G - 1);
-  T s = 0;
-  if (row < n) {
-    for (int k = rp[row] + g; k < rp[row + 1]; k += G) s += va[k] * x[ci[k]];
-  }
+  T s = row < n ? csr_dot<T, G>(rp, ci, va, x, row, g) : (T)0;
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   if (row < n && g == 0) y[row] += s;
 }
@@ -86,27 +122,10 @@ __global__ __launch_bounds__(kBlock) void k_resid(
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int row = (int)(t / G);
   const int g = threadIdx.x & (G - 1);
-  T s = 0;
-  if (row < n) {
-    for (int k = rp[row] + g; k < rp[row + 1]; k += G) s += va[k] * x[ci[k]];
-  }
+  T s = row < n ? csr_dot<T, G>(rp, ci, va, x, row, g) : (T)0;
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   if (row < n && g == 0) r[row] = b[row] - s;
 }
-
-// d = dinv * v * inv_theta; z = d
-template <typename T>
-__global__ void k_cheb_init(const T* __restrict__ v, const T* __restrict__ dinv,
-                            T* __restrict__ d, T* __restrict__ z, int n,
-                            T inv_theta)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const T di = dinv[i] * v[i] * inv_theta;
-  d[i] = di;
-  z[i] = di;
-}
-
 
 // ------------------------------------------------------------ sliced ELL
 // One lane per row, a wave per slice: for every j the 64 lanes read 64

@@ -178,6 +178,8 @@ _sig = {
     "nw_mpnn_train": ([_P, C.POINTER(MpnnTrainCfg), C.POINTER(C.c_double)], C.c_int),
     "nw_mpnn_predict": ([_P, _f64p, C.POINTER(C.c_double)], C.c_int),
     "nw_mpnn_set_precision": ([_P, C.c_int], C.c_int),
+    "nw_linsys_amg_level": ([_P, C.c_int, C.POINTER(C.c_int64),
+                             C.POINTER(C.c_int64)], C.c_int),
     "nw_linsys_set_learned_guess": ([_P, _P], C.c_int),
     "nw_mixed_box": ([C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                       C.POINTER(C.c_int64), C.POINTER(C.c_int64), _P, _P, _P],
@@ -439,6 +441,14 @@ class Context:
 
     def momentum_edge_peclet(self, peclet):
         check(lib.nw_momentum_edge_peclet(self.h, C.byref(peclet)))
+
+    def amg_levels(self):
+        """[(rows, nnz)] of the AMG hierarchy built by the last AMG solve."""
+        out = []
+        r, z = C.c_int64(), C.c_int64()
+        while lib.nw_linsys_amg_level(self.h, len(out), C.byref(r), C.byref(z)) == OK:
+            out.append((r.value, z.value))
+        return out
 
     def solve(self, cfg, raise_on_noconv=True):
         st = SolveStats()
