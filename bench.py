@@ -262,6 +262,88 @@ def read_traffic(name):
     return None
 
 
+def polynomial_solves(n, log):
+    """The pressure solve with the polynomial preconditioner the config
+    names (Chebyshev on D^-1 A, degree 8), FP64 against FP32-inner /
+    FP64-refined, x0 = 0.  At 256^3 it needs several thousand iterations, so
+    it runs on the 128^3 box of the same geometry."""
+    import naluwind_amd as nw
+    import workloads as wl
+    res = {}
+    W = wl.continuity_rf(n)
+    with nw.Context(0) as ctx:
+        W.upload(ctx)
+        ctx.build_graph(1)
+        ctx.assemble_continuity(W.opts)
+        wl.apply_pressure_bc(ctx, W)
+        res["workload"] = ("pressure Poisson of the continuity assembly, %d^3-cell "
+                           "box (z grading of the 128^3 box), x0 = 0, tol 1e-8, "
+                           "GMRES(50) right-preconditioned by Chebyshev(8) on "
+                           "D^-1 A (eigenvalue ratio 30)" % n)
+        res["nodes"] = W.n
+        for name, mode in (("fp64", nw.SOLVE_FP64), ("fp32_refined", nw.SOLVE_FP32_REFINE)):
+            cfg = nw.solver_cfg(kspace=50, max_iters=5000, tol=1e-8,
+                                precond=nw.PREC_CHEBYSHEV, cheb_degree=8,
+                                mode=mode, inner_tol=1e-5)
+            ctx.solve(cfg, raise_on_noconv=False)  # warm-up
+            st = ctx.solve(cfg, raise_on_noconv=False)
+            res[name] = {"ms": round(st.ms, 2), "iters": st.iters,
+                         "refinements": st.refinements,
+                         "final_rel_residual": st.final_rel,
+                         "converged": bool(st.converged)}
+            log("polynomial %s: %s" % (name, res[name]))
+        res["speedup_fp32_refined"] = round(res["fp64"]["ms"] / res["fp32_refined"]["ms"], 3)
+    return res
+
+
+def mixed_mesh(hip, n, steps, warmup, log):
+    """Config of the unstructured mixed mesh: continuity assembly and the
+    AMG-preconditioned pressure solve (FP64 and FP32-inner/FP64-refined,
+    x0 = 0) on an n^3 box cut into Hex8/Tet4/Wed6 cells (60/30/10 %),
+    jittered, randomly permuted and RCM-renumbered."""
+    import naluwind_amd as nw
+    import workloads as wl
+    t0 = time.perf_counter()
+    W = wl.continuity_mixed(n)
+    log("mixed mesh %d^3: %d nodes %d edges (%.1fs)" % (n, W.n, W.ne,
+                                                        time.perf_counter() - t0))
+    res = {"workload": "continuity on a %d^3 box cut into Hex8/Tet4/Wed6 "
+                       "(60/30/10 %% of the cells), +-10 %% jitter, nodes randomly "
+                       "permuted then RCM-renumbered" % n,
+           "nodes": W.n, "edges": W.ne,
+           "elements_hex_tet_wedge": [int(v) for v in W.elements]}
+    with nw.Context(0) as ctx:
+        W.upload(ctx)
+        ctx.build_graph(1)
+        _, nnz = ctx.sizes()
+        kms, _ = time_launches(hip, ctx, lambda: ctx.assemble_continuity(W.opts),
+                               steps, warmup)
+        cb = continuity_bytes(W, nnz)
+        res["nnz"] = nnz
+        res["assembly"] = {"Medges_per_s": round(W.ne / (kms * 1e-3) / 1e6, 1),
+                           "kernel_avg_ms": round(kms, 4),
+                           "GBps_algorithmic": round(cb / (kms * 1e-3) / 1e9, 1),
+                           "frac_hbm_peak": round(cb / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log("mixed assembly: %s" % res["assembly"])
+        ctx.assemble_continuity(W.opts)
+        wl.apply_pressure_bc(ctx, W)
+        for name, mode in (("fp64", nw.SOLVE_FP64), ("fp32_refined", nw.SOLVE_FP32_REFINE)):
+            cfg = nw.solver_cfg(kspace=50, max_iters=2000, tol=1e-8,
+                                precond=nw.PREC_AMG, mode=mode, inner_tol=1e-5,
+                                reuse_preconditioner=1)
+            st = ctx.solve(cfg, raise_on_noconv=False)
+            if name == "fp64":
+                res["amg_setup_ms"] = round(st.setup_ms, 1)
+                st = ctx.solve(cfg, raise_on_noconv=False)
+            res["solve_" + name] = {"ms": round(st.ms, 2), "iters": st.iters,
+                                    "refinements": st.refinements,
+                                    "final_rel_residual": st.final_rel,
+                                    "converged": bool(st.converged)}
+            log("mixed solve %s: %s" % (name, res["solve_" + name]))
+        res["amg_levels_rows_nnz"] = ctx.amg_levels()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,7 +354,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=24)
     ap.add_argument("--eval-steps", type=int, default=3)
     ap.add_argument("--epochs", type=int, default=4)
-    ap.add_argument("--only", choices=["assembly", "solve"],
+    ap.add_argument("--only", choices=["assembly", "solve", "mixed"],
                     help="run one part (profiling); the JSON line is then partial")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
@@ -295,7 +377,7 @@ def main():
            "warmup": a.warmup, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
 
-    if a.only != "solve":
+    if a.only in (None, "assembly"):
         # ---------------------------------------------------------- momentum
         t0 = time.perf_counter()
         W = wl.momentum_abl(n2)
@@ -394,10 +476,17 @@ def main():
         del W
 
     # ---------------------------------------------------------- solve
-    if not a.no_solve and a.only != "assembly":
+    if not a.no_solve and a.only in (None, "solve"):
         ns = 24 if a.quick else a.solve_size
         out["pressure_solve"] = pressure_solves(ns, a.train_steps, a.eval_steps,
                                                 a.epochs, log)
+        out["pressure_solve_polynomial"] = polynomial_solves(
+            24 if a.quick else 128, log)
+
+    # ---------------------------------------------------------- mixed mesh
+    if not a.no_solve and a.only in (None, "mixed"):
+        out["mixed_mesh"] = mixed_mesh(hip, 20 if a.quick else 160, a.steps,
+                                       a.warmup, log)
 
     if not a.no_cpu and a.only is None:
         out["cpu_baseline"] = cpu_momentum_baseline(3.0 if a.quick else 12.0,

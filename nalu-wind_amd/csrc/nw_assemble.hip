@@ -78,14 +78,14 @@ struct ContNode {
 
 // Launch shape of each physics (measured on the 256^3 continuity and 128^3
 // momentum workloads): kLanes lanes per row, kUnroll slots in flight per
-// lane, non-temporal streaming, LDS slot capacity per wave (the large one
-// is used when some wave's rows hold more than kSlotsSmall slots) and the
-// occupancy the register allocation must allow.
+// lane, non-temporal streaming, the LDS slot capacities per wave and the
+// occupancy the register allocation must allow.  A launch uses the
+// smallest of kSlotsSmall/Mid/Large that 99 % of the graph's chunks fit.
 struct ContinuityPhys {
   static constexpr int NDOF = 1;
   static constexpr int kLanes = 1, kUnroll = 1, kMinWaves = 2;
   static constexpr bool kNonTemporal = false;
-  static constexpr int kSlotsSmall = 512, kSlotsLarge = 1024;
+  static constexpr int kSlotsSmall = 512, kSlotsMid = 768, kSlotsLarge = 1024;
   using Node = ContNode;
   const double *coords, *vel, *dpdx, *rho, *p, *udiag, *areav;
   int64_t N, E;
@@ -156,7 +156,7 @@ struct ScalarPhys {
   static constexpr int NDOF = 1;
   static constexpr int kLanes = 1, kUnroll = 1, kMinWaves = 2;
   static constexpr bool kNonTemporal = false;
-  static constexpr int kSlotsSmall = 512, kSlotsLarge = 1024;
+  static constexpr int kSlotsSmall = 512, kSlotsMid = 768, kSlotsLarge = 1024;
   using Node = ScalNode;
   const double *coords, *vrtm, *q, *dqdx, *rho, *dflux, *areav, *mdotf;
   int64_t N, E;
@@ -265,7 +265,7 @@ struct MomentumPhys {
   static constexpr int kLanes = 2, kUnroll = 2, kMinWaves = 2;
   static constexpr bool kNonTemporal = true;
   // 32 rows of 7 slots (Hex8 edge graph); 72 bytes per slot of LDS
-  static constexpr int kSlotsSmall = 224, kSlotsLarge = 224;
+  static constexpr int kSlotsSmall = 224, kSlotsMid = 224, kSlotsLarge = 224;
   using Node = MomNode;
   const double *coords, *vel, *dudx, *visc, *areav, *mdotf, *pecfacf, *wall;
   int64_t N, E;
@@ -671,10 +671,18 @@ int build_schedule(Ctx* c)
                         hipMemcpyHostToDevice, c->stream));
   NW_HIP(hipStreamSynchronize(c->stream));
   g.nchunks = nch;
-  g.max_chunk_slots = 0;
+  // the slot count that 99 % of the chunks stay within: the kernels size
+  // their LDS staging for it, the rest take the direct path
+  std::vector<int64_t> slots(nch);
   for (int64_t i = 0; i < nch; ++i)
-    g.max_chunk_slots = std::max<int64_t>(
-      g.max_chunk_slots, g.h_row_ptr[ch[i].y] - g.h_row_ptr[ch[i].x]);
+    slots[i] = g.h_row_ptr[ch[i].y] - g.h_row_ptr[ch[i].x];
+  if (nch > 0) {
+    const int64_t k = std::min<int64_t>(nch - 1, (nch * 99) / 100);
+    std::nth_element(slots.begin(), slots.begin() + k, slots.end());
+    g.chunk_slots_p99 = slots[k];
+  } else {
+    g.chunk_slots_p99 = 0;
+  }
   g.sched_valid = true;
   return NW_OK;
 }
@@ -701,9 +709,11 @@ int launch_rows(Ctx* c, const P& phys, int accumulate)
   const int64_t wpb = kBlock / kWave;
   int64_t blocks = (g.nchunks * P::kLanes + wpb - 1) / wpb;
   blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
-  // LDS capacity per wave: the small one when every wave's slot range fits
-  if (g.max_chunk_slots <= P::kSlotsSmall)
+  // LDS capacity per wave: the smallest that 99 % of the chunks fit
+  if (g.chunk_slots_p99 <= P::kSlotsSmall)
     launch_assemble<P, P::kSlotsSmall>(c, phys, accumulate, (unsigned)blocks);
+  else if (g.chunk_slots_p99 <= P::kSlotsMid)
+    launch_assemble<P, P::kSlotsMid>(c, phys, accumulate, (unsigned)blocks);
   else
     launch_assemble<P, P::kSlotsLarge>(c, phys, accumulate, (unsigned)blocks);
   NW_LAUNCH_CHECK();

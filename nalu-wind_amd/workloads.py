@@ -16,6 +16,10 @@ edges and geometry, and the seeded node/edge fields of one configuration:
                  random Fourier velocity (8 modes), rho = 1, p = 0,
                  dp/dx = 0, Udiag = gamma1/dt; Dirichlet p = 0 on the top
                  face z = 1
+  continuity_mixed  the continuity fields on the unstructured mixed mesh: an
+                 n^3 box cut into Hex8/Tet4/Wed6 cells (60/30/10 % of the
+                 cells, seeded), +-10 % node jitter, nodes randomly permuted
+                 and then RCM-renumbered by the library's generator
 """
 import math
 import os
@@ -46,7 +50,8 @@ class Workload:
         ctx.mesh_set(self.n, self.edges)
         ctx.field_upload(nw.F_COORDINATES, self.coords)
         ctx.field_upload(nw.E_AREA_VECTOR, self.areav)
-        ctx.field_upload(nw.F_DUAL_VOLUME, self.dualvol)
+        if self.dualvol is not None:
+            ctx.field_upload(nw.F_DUAL_VOLUME, self.dualvol)
         for fid, a in self.fields.items():
             ctx.field_upload(fid, a)
 
@@ -142,6 +147,42 @@ def continuity_rf(n=256, seed=1234, stretch=None, dt=1.0e-3, t=0.0):
     W.top = np.where(W.coords[:, 2] >= 1.0 - 1e-12)[0].astype(np.int32)
     W.seed = seed
     W.drop_elements()
+    return W
+
+
+class MixedWorkload(Workload):
+    """Mixed Hex8/Tet4/Wed6 box from the library's generator (RCM node
+    order), its edges and edge area vectors."""
+
+    def __init__(self, name, n, f_hex, f_tet, jitter, seed):
+        self.name = name
+        self.cells = (n, n, n)
+        self.coords, types, conn = nw.mixed_box(n, f_hex, f_tet, jitter, seed)
+        self.n = self.coords.shape[0]
+        self.edges = nw.mixed_edges(types, conn, self.n)
+        self.ne = self.edges.shape[0]
+        self.areav = nw.mixed_edge_area_vector(types, conn, self.coords,
+                                               self.edges)
+        self.elements = np.bincount(types, minlength=3)  # hex, tet, wedge
+        self.dualvol = None
+        self.fields = {}
+        self.top = None
+        self.conn = None
+
+
+def continuity_mixed(n=160, seed=1234, dt=1.0e-3, f_hex=0.6, f_tet=0.3,
+                     jitter=0.10):
+    W = MixedWorkload("continuity_mixed", n, f_hex, f_tet, jitter, seed)
+    vel = fourier_velocity(W.coords, seed)
+    W.fields = {
+        nw.F_VELOCITY: vel, nw.F_DENSITY: np.ones(W.n),
+        nw.F_PRESSURE: np.zeros(W.n), nw.F_DPDX: np.zeros((W.n, 3)),
+        nw.F_UDIAG: np.full(W.n, 1.0 / dt),
+        nw.F_DIFF_FLUX_COEFF: np.zeros(W.n),
+    }
+    W.opts = nw.continuity_defaults(dt=dt, gamma1=1.0)
+    W.top = np.where(W.coords[:, 2] >= 1.0 - 1e-12)[0].astype(np.int32)
+    W.seed = seed
     return W
 
 
