@@ -76,16 +76,16 @@ struct ContNode {
   double x[3], u[3], g[3], rho, p, ud;
 };
 
-// Launch shape of each physics (measured on the 256^3 continuity and 128^3
-// momentum workloads): kLanes lanes per row, kUnroll slots in flight per
-// lane, non-temporal streaming, the LDS slot capacities per wave and the
-// occupancy the register allocation must allow.  A launch uses the
-// smallest of kSlotsSmall/Mid/Large that 99 % of the graph's chunks fit.
+// Launch shape of each physics (measured on the 256^3 continuity, 128^3
+// momentum and 160^3 mixed-mesh workloads): kLanes lanes per row (the 1-dof
+// systems choose 1 or 2 from the graph, see launch_rows), kUnroll slots in
+// flight per lane, non-temporal streaming, the LDS slot capacities per wave
+// and the occupancy the register allocation must allow.
 struct ContinuityPhys {
   static constexpr int NDOF = 1;
   static constexpr int kLanes = 1, kUnroll = 1, kMinWaves = 2;
   static constexpr bool kNonTemporal = false;
-  static constexpr int kSlotsSmall = 512, kSlotsMid = 768, kSlotsLarge = 1024;
+  static constexpr int kSlotsSmall = 512, kSlotsLarge = 1024;
   using Node = ContNode;
   const double *coords, *vel, *dpdx, *rho, *p, *udiag, *areav;
   int64_t N, E;
@@ -156,7 +156,7 @@ struct ScalarPhys {
   static constexpr int NDOF = 1;
   static constexpr int kLanes = 1, kUnroll = 1, kMinWaves = 2;
   static constexpr bool kNonTemporal = false;
-  static constexpr int kSlotsSmall = 512, kSlotsMid = 768, kSlotsLarge = 1024;
+  static constexpr int kSlotsSmall = 512, kSlotsLarge = 1024;
   using Node = ScalNode;
   const double *coords, *vrtm, *q, *dqdx, *rho, *dflux, *areav, *mdotf;
   int64_t N, E;
@@ -265,7 +265,7 @@ struct MomentumPhys {
   static constexpr int kLanes = 2, kUnroll = 2, kMinWaves = 2;
   static constexpr bool kNonTemporal = true;
   // 32 rows of 7 slots (Hex8 edge graph); 72 bytes per slot of LDS
-  static constexpr int kSlotsSmall = 224, kSlotsMid = 224, kSlotsLarge = 224;
+  static constexpr int kSlotsSmall = 224, kSlotsLarge = 224;
   using Node = MomNode;
   const double *coords, *vel, *dudx, *visc, *areav, *mdotf, *pecfacf, *wall;
   int64_t N, E;
@@ -623,8 +623,48 @@ uint64_t spread3(uint64_t v)
   return v;
 }
 
-// Chunks of <= kRowsPerWave consecutive rows, ordered by the Morton code of
-// their coordinate centroid (NW_ROW_ORDER=natural keeps the row order).
+// Fraction of (row, neighbour) pairs whose chunks lie more than kWindow
+// positions apart in the schedule `ch`, over every 16th row: the pairs whose
+// node data is unlikely to be still in L2 when the second row is assembled
+// (kWindow ~ the chunks one XCD keeps in flight).
+double far_pair_fraction(const Graph& g, const std::vector<int2>& ch)
+{
+  constexpr int64_t kWindow = 512;
+  const int64_t nch = (int64_t)ch.size();
+  std::vector<int64_t> pos(nch);
+  for (int64_t p = 0; p < nch; ++p) pos[ch[p].x / kRowsPerWave] = p;
+  int64_t far = 0, tot = 0;
+  for (int64_t r = 0; r < g.nnode; r += 16) {
+    const int64_t pr = pos[r / kRowsPerWave];
+    for (int32_t s = g.h_row_ptr[r]; s < g.h_row_ptr[r + 1]; ++s) {
+      const int64_t pc = pos[g.h_cols[s] / kRowsPerWave];
+      far += std::llabs(pr - pc) > kWindow;
+      ++tot;
+    }
+  }
+  return tot ? (double)far / (double)tot : 0.0;
+}
+
+// p-quantile of the slot counts of consecutive row groups of `rows` rows
+int64_t slot_quantile(const Graph& g, int64_t rows, double q)
+{
+  const int64_t ng = (g.nnode + rows - 1) / rows;
+  if (ng == 0) return 0;
+  std::vector<int64_t> v(ng);
+  for (int64_t i = 0; i < ng; ++i)
+    v[i] = g.h_row_ptr[std::min<int64_t>(g.nnode, (i + 1) * rows)] -
+           g.h_row_ptr[i * rows];
+  const int64_t k = std::min<int64_t>(ng - 1, (int64_t)(q * (double)ng));
+  std::nth_element(v.begin(), v.begin() + k, v.end());
+  return v[k];
+}
+
+// Chunks of <= kRowsPerWave consecutive rows, in the row order or ordered by
+// the Morton code of their coordinate centroid, whichever leaves fewer
+// neighbour pairs far apart in the schedule: Morton for lexicographically
+// numbered boxes (whose z neighbours are a whole plane apart), the row order
+// for bandwidth-reducing (RCM) numberings, whose level sets are compact.
+// NW_ROW_ORDER=natural|morton forces one.
 int build_schedule(Ctx* c)
 {
   Graph& g = c->g;
@@ -635,8 +675,8 @@ int build_schedule(Ctx* c)
     ch[i] = make_int2((int)(i * kRowsPerWave),
                       (int)std::min<int64_t>(N, (i + 1) * kRowsPerWave));
   const char* env = getenv("NW_ROW_ORDER");
-  const bool natural = env && std::string(env) == "natural";
-  if (!natural && c->field[NW_F_COORDINATES]) {
+  const std::string force = env ? std::string(env) : std::string();
+  if (force != "natural" && c->field[NW_F_COORDINATES]) {
     std::vector<double> x(3 * N);  // SoA: x[d * N + n]
     NW_HIP(hipMemcpyAsync(x.data(), c->field[NW_F_COORDINATES],
                           sizeof(double) * 3 * N, hipMemcpyDeviceToHost,
@@ -664,37 +704,34 @@ int build_schedule(Ctx* c)
     std::sort(key.begin(), key.end());
     std::vector<int2> o(nch);
     for (int64_t i = 0; i < nch; ++i) o[i] = ch[key[i].second];
-    ch.swap(o);
+    if (force == "morton" || far_pair_fraction(g, o) < far_pair_fraction(g, ch))
+      ch.swap(o);
   }
   if (!g.sched) NW_HIP(hipMalloc(&g.sched, sizeof(int2) * std::max<int64_t>(nch, 1)));
   NW_HIP(hipMemcpyAsync(g.sched, ch.data(), sizeof(int2) * nch,
                         hipMemcpyHostToDevice, c->stream));
   NW_HIP(hipStreamSynchronize(c->stream));
   g.nchunks = nch;
-  // the slot count that 99 % of the chunks stay within: the kernels size
-  // their LDS staging for it, the rest take the direct path
-  std::vector<int64_t> slots(nch);
-  for (int64_t i = 0; i < nch; ++i)
-    slots[i] = g.h_row_ptr[ch[i].y] - g.h_row_ptr[ch[i].x];
-  if (nch > 0) {
-    const int64_t k = std::min<int64_t>(nch - 1, (nch * 99) / 100);
-    std::nth_element(slots.begin(), slots.begin() + k, slots.end());
-    g.chunk_slots_p99 = slots[k];
-  } else {
-    g.chunk_slots_p99 = 0;
-  }
+  // the slot counts that 99 % of the chunks (and of the half chunks) stay
+  // within: the kernels size their LDS staging for them, the rest take the
+  // direct path
+  g.chunk_slots_p99 = slot_quantile(g, kRowsPerWave, 0.99);
+  g.half_chunk_slots_p99 = slot_quantile(g, kRowsPerWave / 2, 0.99);
   g.sched_valid = true;
   return NW_OK;
 }
 
-template <class P, int SCAP>
-void launch_assemble(Ctx* c, const P& phys, int accumulate, unsigned blocks)
+template <class P, int G, int SCAP>
+void launch_assemble(Ctx* c, const P& phys, int accumulate)
 {
   const Graph& g = c->g;
-  k_assemble<P, P::kLanes, P::kUnroll, SCAP, P::kNonTemporal>
-    <<<blocks, kBlock, 0, c->stream>>>(phys, g.row_ptr, g.code, g.cols, g.diag,
-                                       g.sched, (int)g.nchunks, g.vals, g.rhs,
-                                       accumulate);
+  const int64_t wpb = kBlock / kWave;
+  int64_t blocks = (g.nchunks * G + wpb - 1) / wpb;
+  blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
+  k_assemble<P, G, P::kUnroll, SCAP, P::kNonTemporal>
+    <<<(unsigned)blocks, kBlock, 0, c->stream>>>(phys, g.row_ptr, g.code, g.cols,
+                                                 g.diag, g.sched, (int)g.nchunks,
+                                                 g.vals, g.rhs, accumulate);
 }
 
 template <class P>
@@ -706,16 +743,19 @@ int launch_rows(Ctx* c, const P& phys, int accumulate)
     if (rc) return rc;
   }
   if (g.nchunks == 0) return NW_OK;
-  const int64_t wpb = kBlock / kWave;
-  int64_t blocks = (g.nchunks * P::kLanes + wpb - 1) / wpb;
-  blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
-  // LDS capacity per wave: the smallest that 99 % of the chunks fit
-  if (g.chunk_slots_p99 <= P::kSlotsSmall)
-    launch_assemble<P, P::kSlotsSmall>(c, phys, accumulate, (unsigned)blocks);
-  else if (g.chunk_slots_p99 <= P::kSlotsMid)
-    launch_assemble<P, P::kSlotsMid>(c, phys, accumulate, (unsigned)blocks);
-  else
-    launch_assemble<P, P::kSlotsLarge>(c, phys, accumulate, (unsigned)blocks);
+  if constexpr (P::NDOF == 1) {
+    // one lane per row when 99 % of the 64-row chunks fit kSlotsSmall slots
+    // (Hex8 graphs); else two lanes per row, which also balances rows of
+    // uneven length (mixed meshes), with the LDS the half chunks need
+    if (g.chunk_slots_p99 <= P::kSlotsSmall)
+      launch_assemble<P, 1, P::kSlotsSmall>(c, phys, accumulate);
+    else if (g.half_chunk_slots_p99 <= P::kSlotsSmall)
+      launch_assemble<P, 2, P::kSlotsSmall>(c, phys, accumulate);
+    else
+      launch_assemble<P, 2, P::kSlotsLarge>(c, phys, accumulate);
+  } else {
+    launch_assemble<P, P::kLanes, P::kSlotsSmall>(c, phys, accumulate);
+  }
   NW_LAUNCH_CHECK();
   return NW_OK;
 }

@@ -91,7 +91,8 @@ struct Graph {
   // chunks processed together are neighbours in space.
   int2* sched = nullptr;
   int64_t nchunks = 0;
-  int64_t chunk_slots_p99 = 0;  // slots 99 % of the chunks stay within
+  int64_t chunk_slots_p99 = 0;       // slots 99 % of the chunks stay within
+  int64_t half_chunk_slots_p99 = 0;  // the same for 32-row half chunks
   Sell sell;  // sliced ELL form of the 1-dof graph (built by the first solve)
   bool sched_valid = false;
 };
