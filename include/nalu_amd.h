@@ -190,6 +190,33 @@ int nw_assemble_continuity(nw_ctx* ctx, const nw_continuity_opts* opts);
  * 49-102): writes NW_E_PECLET_NUMBER and NW_E_PECLET_FACTOR. */
 int nw_momentum_edge_peclet(nw_ctx* ctx, const nw_peclet* peclet);
 
+/* ---- edge algorithms around the pressure solve ----------------------------
+ * MdotEdgeAlg::execute (src/ngp_algorithms/MdotEdgeAlg.C:43-194), interior
+ * edges: the mass flow rate from the new pressure,
+ *   mdot = -tau (p_R - p_L) |A|^2/(A.dx) + sum_d [(rho u)_ip + G_ip] A_d
+ *          - noc * sum_d k_d G_ip,   tau = (1/Udiag_L + 1/Udiag_R)/2,
+ *   G_ip = (dpdx_R/Udiag_R + dpdx_L/Udiag_L)/2, k = A - |A|^2/(A.dx) dx,
+ * with (rho u)_ip interpolated as rho*U together (interp_together = 1) or
+ * as rho_ip u_ip.  Reads COORDINATES, VELOCITY, DPDX, DENSITY, PRESSURE,
+ * UDIAG and AREA_VECTOR; writes NW_E_MASS_FLOW_RATE (one value per edge,
+ * one launch over the edges, no graph needed).  The balanced-buoyancy and
+ * mesh-motion (GCL) terms of the reference are not built. */
+typedef struct {
+  double noc;             /* non-orthogonal correction on (1) / off (0) */
+  double interp_together; /* 1: (rho u)_ip = avg(rho u); 0: avg(rho) avg(u) */
+} nw_mdot_opts;
+int nw_compute_mdot(nw_ctx* ctx, const nw_mdot_opts* opts);
+/* NodalGradEdgeAlg::execute (src/ngp_algorithms/NodalGradEdgeAlg.C:58-111),
+ * interior edges: grad_n = sum over the edges of n of +-q_ip A / V_n
+ * (+ for n = L, - for n = R), q_ip = (q_L + q_R)/2, V the dual volume
+ * (NW_F_DUAL_VOLUME).  field_in with c components (PRESSURE, VELOCITY,
+ * SCALAR_Q), field_out with 3c (DPDX, DUDX, DQDX), component 3 j + i =
+ * d q_j / d x_i; overwritten.  One pass over the rows of the context's
+ * edge graph (built by nw_linsys_build_edge_graph): each node's sum is
+ * formed by one lane in slot order, so the result is bitwise reproducible.
+ * Boundary-face contributions (NodalGradBndryElemAlg) are not built. */
+int nw_nodal_grad(nw_ctx* ctx, int field_in, int field_out);
+
 /* ---- linear solve (TpetraLinearSystem::solve -> Belos GMRES) ------------ */
 enum {
   NW_PREC_NONE = 0,

@@ -1,0 +1,165 @@
+// nw_edge_algs.hip -- the edge algorithms on either side of the pressure
+// solve, on gfx950:
+//
+//   k_mdot       MdotEdgeAlg (src/ngp_algorithms/MdotEdgeAlg.C:121-186):
+//                one lane per edge, edge-ordered (coalesced) area-vector
+//                reads and mass-flow-rate writes, the two nodes' fields
+//                gathered from the component-major node arrays;
+//   k_nodal_grad NodalGradEdgeAlg (src/ngp_algorithms/NodalGradEdgeAlg.C:
+//                58-111): one lane per row of the edge graph; the lane walks
+//                the row's slots (slot code 2e + side names the edge and
+//                which end the row is), sums +-q_ip A_e / V and writes the
+//                row's 3c components once.  Each node's sum is owned by one
+//                lane, so no atomics are needed and the result does not
+//                depend on scheduling (the reference scatters both ends of
+//                every edge with atomic adds).
+// Both are gather-bound: their roofline is HBM bandwidth.
+#include <algorithm>
+#include <cmath>
+
+#include "nw_internal.h"
+
+namespace nw {
+
+namespace {
+
+struct MdotArgs {
+  const double *coords, *vel, *dpdx, *rho, *p, *udiag, *areav;
+  int64_t N, E;
+  double noc, interp;
+};
+
+__global__ __launch_bounds__(kBlock) void k_mdot(MdotArgs a,
+                                                 const int2* __restrict__ edges,
+                                                 double* __restrict__ mdot)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.E) return;
+  const int2 lr = edges[e];
+  const int64_t L = lr.x, R = lr.y, N = a.N, E = a.E;
+  const double om_interp = 1.0 - a.interp;
+  double av[3];
+  for (int d = 0; d < 3; ++d) av[d] = a.areav[d * E + e];
+  const double udL = a.udiag[L], udR = a.udiag[R];
+  const double rhoL = a.rho[L], rhoR = a.rho[R];
+  const double projTimeScale = 0.5 * (1.0 / udL + 1.0 / udR);
+  const double rhoIp = 0.5 * (rhoL + rhoR);
+  double dx[3], axdx = 0.0, asq = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    dx[d] = a.coords[d * N + R] - a.coords[d * N + L];
+    asq += av[d] * av[d];
+    axdx += av[d] * dx[d];
+  }
+  const double inv_axdx = 1.0 / axdx;
+  double tmdot = -projTimeScale * (a.p[R] - a.p[L]) * asq * inv_axdx;
+  for (int d = 0; d < 3; ++d) {
+    const double kxj = av[d] - asq * inv_axdx * dx[d];
+    const double uR = a.vel[d * N + R], uL = a.vel[d * N + L];
+    const double rhoUjIp = 0.5 * (rhoR * uR + rhoL * uL);
+    const double ujIp = 0.5 * (uR + uL);
+    const double GjIp = 0.5 * (a.dpdx[d * N + R] / udR + a.dpdx[d * N + L] / udL);
+    tmdot += (a.interp * rhoUjIp + om_interp * rhoIp * ujIp + GjIp) * av[d] -
+             kxj * GjIp * a.noc;
+  }
+  mdot[e] = tmdot;
+}
+
+// grad[(3 j + i) * N + n] = sum_slots sign * (q_ip,j A_e,i) / V_n
+template <int C>
+__global__ __launch_bounds__(kBlock) void k_nodal_grad(
+  const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ code,
+  const int32_t* __restrict__ cols, const double* __restrict__ q,
+  const double* __restrict__ areav, const double* __restrict__ dualvol,
+  double* __restrict__ grad, int64_t N, int64_t E)
+{
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double qn[C], acc[3 * C];
+  const double inv = 1.0 / dualvol[n];
+  for (int j = 0; j < C; ++j) qn[j] = q[j * N + n];
+  for (int k = 0; k < 3 * C; ++k) acc[k] = 0.0;
+  const int s1 = row_ptr[n + 1];
+  for (int s = row_ptr[n]; s < s1; ++s) {
+    const int c = code[s];
+    if (c < 0) continue;  // the diagonal slot
+    const int64_t e = c >> 1;
+    const double sgn = (c & 1) ? -1.0 : 1.0;  // n = R: minus, n = L: plus
+    const int64_t m = cols[s];
+    double av[3];
+    for (int i = 0; i < 3; ++i) av[i] = areav[i * E + e];
+    for (int j = 0; j < C; ++j) {
+      const double qip = 0.5 * (qn[j] + q[j * N + m]);
+      for (int i = 0; i < 3; ++i) acc[3 * j + i] += sgn * ((av[i] * qip) * inv);
+    }
+  }
+  for (int k = 0; k < 3 * C; ++k) grad[k * N + n] = acc[k];
+}
+
+}  // namespace
+
+}  // namespace nw
+
+using namespace nw;
+
+extern "C" {
+
+int nw_compute_mdot(nw_ctx* c, const nw_mdot_opts* o)
+{
+  NW_REQUIRE(c && o, NW_E_ARG, "null argument");
+  NW_REQUIRE(c->nnode > 0 && c->nedge >= 0, NW_E_STATE, "mesh not set");
+  const double *coords, *vel, *dpdx, *rho, *p, *udiag, *areav;
+  int rc = 0;
+  if ((rc = require_field(c, NW_F_COORDINATES, &coords)) ||
+      (rc = require_field(c, NW_F_VELOCITY, &vel)) ||
+      (rc = require_field(c, NW_F_DPDX, &dpdx)) ||
+      (rc = require_field(c, NW_F_DENSITY, &rho)) ||
+      (rc = require_field(c, NW_F_PRESSURE, &p)) ||
+      (rc = require_field(c, NW_F_UDIAG, &udiag)) ||
+      (rc = require_field(c, NW_E_AREA_VECTOR, &areav)))
+    return rc;
+  double* mdot = nullptr;
+  if ((rc = ensure_field(c, NW_E_MASS_FLOW_RATE, &mdot))) return rc;
+  if (c->nedge == 0) return NW_OK;
+  const MdotArgs a{coords, vel, dpdx, rho, p, udiag, areav, c->nnode, c->nedge,
+                   o->noc, o->interp_together};
+  k_mdot<<<grid_for(c->nedge, kBlock), kBlock, 0, c->stream>>>(
+    a, reinterpret_cast<const int2*>(c->edges), mdot);
+  NW_LAUNCH_CHECK();
+  return NW_OK;
+}
+
+int nw_nodal_grad(nw_ctx* c, int fin, int fout)
+{
+  NW_REQUIRE(c, NW_E_ARG, "null context");
+  NW_REQUIRE(fin >= 0 && fin < NW_NUM_FIELDS && fout >= 0 && fout < NW_NUM_FIELDS &&
+               !field_is_edge(fin) && !field_is_edge(fout) && fin != fout,
+             NW_E_ARG, "nodal gradient needs two distinct node fields");
+  const int C = field_components(fin);
+  NW_REQUIRE(C == 1 || C == 3, NW_E_ARG, "input field must have 1 or 3 components");
+  NW_REQUIRE(field_components(fout) == 3 * C, NW_E_ARG,
+             "output field must have %d components", 3 * C);
+  const Graph& g = c->g;
+  NW_REQUIRE(g.row_ptr && g.nnode == c->nnode, NW_E_STATE,
+             "nodal gradient needs the edge graph (nw_linsys_build_edge_graph)");
+  const double *q, *areav, *dualvol;
+  int rc = 0;
+  if ((rc = require_field(c, fin, &q)) ||
+      (rc = require_field(c, NW_E_AREA_VECTOR, &areav)) ||
+      (rc = require_field(c, NW_F_DUAL_VOLUME, &dualvol)))
+    return rc;
+  double* grad = nullptr;
+  if ((rc = ensure_field(c, fout, &grad))) return rc;
+  const unsigned nb = grid_for(c->nnode, kBlock);
+  if (C == 1)
+    k_nodal_grad<1><<<nb, kBlock, 0, c->stream>>>(g.row_ptr, g.code, g.cols, q,
+                                                  areav, dualvol, grad, c->nnode,
+                                                  c->nedge);
+  else
+    k_nodal_grad<3><<<nb, kBlock, 0, c->stream>>>(g.row_ptr, g.code, g.cols, q,
+                                                  areav, dualvol, grad, c->nnode,
+                                                  c->nedge);
+  NW_LAUNCH_CHECK();
+  return NW_OK;
+}
+
+}  // extern "C"

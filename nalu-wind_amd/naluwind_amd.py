@@ -63,6 +63,10 @@ class MomentumOpts(C.Structure):
                 ("accumulate", C.c_int)]
 
 
+class MdotOpts(C.Structure):
+    _fields_ = [("noc", C.c_double), ("interp_together", C.c_double)]
+
+
 class ContinuityOpts(C.Structure):
     _fields_ = [("dt", C.c_double), ("gamma1", C.c_double), ("noc", C.c_double),
                 ("interp_together", C.c_double),
@@ -161,6 +165,8 @@ _sig = {
     "nw_assemble_momentum": ([_P, C.POINTER(MomentumOpts)], C.c_int),
     "nw_assemble_continuity": ([_P, C.POINTER(ContinuityOpts)], C.c_int),
     "nw_momentum_edge_peclet": ([_P, C.POINTER(Peclet)], C.c_int),
+    "nw_compute_mdot": ([_P, C.POINTER(MdotOpts)], C.c_int),
+    "nw_nodal_grad": ([_P, C.c_int, C.c_int], C.c_int),
     "nw_linsys_solve": ([_P, C.POINTER(SolverCfg), C.POINTER(SolveStats)], C.c_int),
     "nw_linsys_solution_upload": ([_P, _f64p], C.c_int),
     "nw_linsys_solution_download": ([_P, _f64p], C.c_int),
@@ -441,6 +447,14 @@ class Context:
 
     def momentum_edge_peclet(self, peclet):
         check(lib.nw_momentum_edge_peclet(self.h, C.byref(peclet)))
+
+    def compute_mdot(self, noc=1.0, interp_together=1.0):
+        """MdotEdgeAlg: NW_E_MASS_FLOW_RATE from the node fields."""
+        check(lib.nw_compute_mdot(self.h, C.byref(MdotOpts(noc, interp_together))))
+
+    def nodal_grad(self, field_in, field_out):
+        """NodalGradEdgeAlg: field_out = edge-based gradient of field_in."""
+        check(lib.nw_nodal_grad(self.h, field_in, field_out))
 
     def amg_levels(self):
         """[(rows, nnz)] of the AMG hierarchy built by the last AMG solve."""

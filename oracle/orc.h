@@ -107,6 +107,13 @@ void orc_nodal_grad_edge(int64_t nnode, int64_t nedge, const int32_t* edges,
  * (unit_tests/kernels/UnitTestKernelUtils.C:725-769). */
 void orc_mdot_edge(int64_t nedge, const int32_t* edges, const double* areav,
                    const double* rho, const double* vel, double* mdot);
+/* MdotEdgeAlg (src/ngp_algorithms/MdotEdgeAlg.C:121-186), interior edges,
+ * without the balanced-buoyancy and mesh-motion terms. */
+void orc_mdot_edge_alg(int64_t nedge, const int32_t* edges,
+                       const double* coords, const double* vel,
+                       const double* dpdx, const double* rho, const double* p,
+                       const double* udiag, const double* areav, double noc,
+                       double interp, double* mdot);
 
 /* ------------------------------------------------------------------ */
 /* Peclet blending and limiter                                         */

@@ -85,6 +85,9 @@ lib.orc_dual_volume_hex.argtypes = [C.c_int64, _i32p, _f64p, C.c_int64, _f64p]
 lib.orc_nodal_grad_edge.argtypes = [C.c_int64, C.c_int64, _i32p, _f64p, _f64p,
                                     C.c_int, _f64p, _f64p]
 lib.orc_mdot_edge.argtypes = [C.c_int64, _i32p, _f64p, _f64p, _f64p, _f64p]
+lib.orc_mdot_edge_alg.argtypes = [C.c_int64, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                  _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                  _f64p]
 lib.orc_peclet_classic.argtypes = [C.c_double, C.c_double]
 lib.orc_peclet_classic.restype = C.c_double
 lib.orc_peclet_tanh.argtypes = [C.c_double, C.c_double, C.c_double]
@@ -238,6 +241,16 @@ def mdot_edge(edges, areav, rho, vel):
     out = np.zeros(edges.shape[0])
     lib.orc_mdot_edge(edges.shape[0], i32(edges).ravel(), f64(areav).ravel(),
                       f64(rho), f64(vel).ravel(), out)
+    return out
+
+
+def mdot_edge_alg(edges, coords, vel, dpdx, rho, p, udiag, areav, noc=1.0,
+                  interp=1.0):
+    """MdotEdgeAlg interior edges: node arrays [n][3] / [n], areav [e][3]."""
+    out = np.zeros(edges.shape[0])
+    lib.orc_mdot_edge_alg(edges.shape[0], i32(edges).ravel(), f64(coords).ravel(),
+                          f64(vel).ravel(), f64(dpdx).ravel(), f64(rho), f64(p),
+                          f64(udiag), f64(areav).ravel(), noc, interp, out)
     return out
 
 

@@ -387,3 +387,37 @@ void orc_mdot_edge(int64_t nedge, const int32_t* edges, const double* areav,
     mdot[e] = t;
   }
 }
+
+/* MdotEdgeAlg::execute interior edges (src/ngp_algorithms/MdotEdgeAlg.C:
+ * 121-186) without the balanced-buoyancy and mesh-motion terms: node
+ * arrays [n][3] / [n], areav [e][3]. */
+void orc_mdot_edge_alg(int64_t nedge, const int32_t* edges,
+                       const double* coords, const double* vel,
+                       const double* dpdx, const double* rho, const double* p,
+                       const double* udiag, const double* areav, double noc,
+                       double interp, double* mdot)
+{
+  for (int64_t e = 0; e < nedge; ++e) {
+    const int32_t L = edges[2 * e], R = edges[2 * e + 1];
+    const double* av = areav + 3 * e;
+    const double projTimeScale = 0.5 * (1.0 / udiag[L] + 1.0 / udiag[R]);
+    const double rhoIp = 0.5 * (rho[L] + rho[R]);
+    double axdx = 0.0, asq = 0.0, dx[3];
+    for (int d = 0; d < 3; ++d) {
+      dx[d] = coords[3 * R + d] - coords[3 * L + d];
+      asq += av[d] * av[d];
+      axdx += av[d] * dx[d];
+    }
+    const double inv_axdx = 1.0 / axdx;
+    double t = -projTimeScale * (p[R] - p[L]) * asq * inv_axdx;
+    for (int d = 0; d < 3; ++d) {
+      const double kxj = av[d] - asq * inv_axdx * dx[d];
+      const double rhoUjIp = 0.5 * (rho[R] * vel[3 * R + d] + rho[L] * vel[3 * L + d]);
+      const double ujIp = 0.5 * (vel[3 * R + d] + vel[3 * L + d]);
+      const double GjIp = 0.5 * (dpdx[3 * R + d] / udiag[R] + dpdx[3 * L + d] / udiag[L]);
+      t += (interp * rhoUjIp + (1.0 - interp) * rhoIp * ujIp + GjIp) * av[d] -
+           kxj * GjIp * noc;
+    }
+    mdot[e] = t;
+  }
+}
