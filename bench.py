@@ -354,7 +354,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=24)
     ap.add_argument("--eval-steps", type=int, default=3)
     ap.add_argument("--epochs", type=int, default=4)
-    ap.add_argument("--only", choices=["assembly", "solve", "mixed"],
+    ap.add_argument("--only", choices=["assembly", "solve", "polynomial", "mixed"],
                     help="run one part (profiling); the JSON line is then partial")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
@@ -473,6 +473,22 @@ def main():
                 "frac_hbm_peak": round(cb / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "algorithmic_bytes_per_launch": cb}
             log("continuity: %.3f ms, %.0f GB/s" % (kms, cb / (kms * 1e-3) / 1e9))
+            # the edge algorithms after the solve on the same mesh
+            edge_algs = {}
+            for name, fn, nb in (
+                    ("mdot", lambda: ctx.compute_mdot(1.0, 1.0),
+                     12 * 8 * W.n + (24 + 8 + 8) * W.ne),
+                    ("nodal_grad_pressure", lambda: ctx.nodal_grad(nw.F_PRESSURE, nw.F_DPDX),
+                     (8 + 8 + 24) * W.n + 24 * W.ne),
+                    ("nodal_grad_velocity", lambda: ctx.nodal_grad(nw.F_VELOCITY, nw.F_DUDX),
+                     (24 + 8 + 72) * W.n + 24 * W.ne)):
+                kms, _ = time_launches(hip, ctx, fn, a.steps, a.warmup)
+                edge_algs[name] = {"kernel_avg_ms": round(kms, 4),
+                                   "Medges_per_s": round(W.ne / (kms * 1e-3) / 1e6, 1),
+                                   "GBps_algorithmic": round(nb / (kms * 1e-3) / 1e9, 1),
+                                   "frac_hbm_peak": round(nb / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                log("%s: %s" % (name, edge_algs[name]))
+            out["edge_algorithms_256"] = edge_algs
         del W
 
     # ---------------------------------------------------------- solve
@@ -480,6 +496,7 @@ def main():
         ns = 24 if a.quick else a.solve_size
         out["pressure_solve"] = pressure_solves(ns, a.train_steps, a.eval_steps,
                                                 a.epochs, log)
+    if not a.no_solve and a.only in (None, "polynomial"):
         out["pressure_solve_polynomial"] = polynomial_solves(
             24 if a.quick else 128, log)
 
