@@ -335,8 +335,21 @@ int nw_mpnn_get_params(const nw_mpnn* net, double* params, int64_t n);
 int nw_mpnn_record(nw_mpnn* net);
 int nw_mpnn_samples(const nw_mpnn* net);
 int nw_mpnn_clear_samples(nw_mpnn* net);
-/* Mean-squared loss of recorded sample s and its gradient (nparams
- * doubles, may be NULL). */
+/* Training loss (nw_mpnn_loss_grad, nw_mpnn_train).  With u the error of
+ * the prediction in units of the output scale, u = delta - t on the rows
+ * with couplings (t = (x - xb)/s the recorded target):
+ *   NW_MPNN_LOSS_RESIDUAL (default): mean((D^-1 A u)^2), the Jacobi-scaled
+ *     residual of the prediction error -- what GMRES starts from; the
+ *     context's current matrix is used (the pressure matrix of a static
+ *     mesh does not change between steps);
+ *   NW_MPNN_LOSS_SOLUTION: mean((delta - t)^2) over all rows. */
+enum {
+  NW_MPNN_LOSS_SOLUTION = 0,
+  NW_MPNN_LOSS_RESIDUAL = 1
+};
+int nw_mpnn_set_loss(nw_mpnn* net, int kind);
+/* Loss of recorded sample s and its gradient (nparams doubles, may be
+ * NULL). */
 int nw_mpnn_loss_grad(nw_mpnn* net, int sample, double* loss, double* grad);
 /* Adam over the recorded samples; *last_loss = the best validation loss
  * (holdout > 0) or the mean training loss of the last epoch. */

@@ -582,6 +582,60 @@ __global__ __launch_bounds__(kBlock) void k_loss(
   if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nv + off] = tot[0];
 }
 
+// Residual loss, forward: u = delta - t off the Dirichlet rows (dir set),
+// v = D^-1 A u; vd = D^-1 v; partial[blk * nv + off] = sum v^2.  One lane
+// per row, FP64 sums.
+__global__ __launch_bounds__(kBlock) void k_res_fwd(
+  const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+  const double* __restrict__ va, const int32_t* __restrict__ diag,
+  const float* __restrict__ delta, const float* __restrict__ target,
+  const float* __restrict__ dir, float* __restrict__ vd, int n,
+  double* __restrict__ partial, int nv, int off)
+{
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    double au = 0.0;
+    for (int s = rp[i]; s < rp[i + 1]; ++s) {
+      const int j = ci[s];
+      const double u = dir[j] != 0.f ? 0.0 : (double)delta[j] - (double)target[j];
+      au += va[s] * u;
+    }
+    const double di = va[diag[i]];
+    const double v = au / di;
+    acc[0] += v * v;
+    vd[i] = (float)(v / di);
+  }
+  __shared__ double tot[1];
+  mp_block_store<1>(acc, tot);
+  if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nv + off] = tot[0];
+}
+
+// Residual loss, backward: g = coef A^T vd (0 on Dirichlet rows) and
+// dd2[i] = g_i (f3_i, f4_i).  (A^T vd)_i sums a_ji vd_j over the row's
+// slots (i, j); a_ji is the edge's other slot (eslot), the diagonal its own.
+__global__ __launch_bounds__(kBlock) void k_res_back(
+  const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+  const int32_t* __restrict__ code, const double* __restrict__ va,
+  const int32_t* __restrict__ eslot, const float* __restrict__ vd,
+  const float* __restrict__ dir, const float* __restrict__ f3,
+  const float* __restrict__ f4, float coef, float* __restrict__ dd2, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double g = 0.0;
+  if (dir[i] == 0.f) {
+    for (int s = rp[i]; s < rp[i + 1]; ++s) {
+      const int c = code[s];
+      const int t = c < 0 ? s : eslot[c ^ 1];
+      if (t >= 0) g += va[t] * (double)vd[ci[s]];
+    }
+  }
+  const float d = coef * (float)g;
+  dd2[2 * (int64_t)i] = d * f3[i];
+  dd2[2 * (int64_t)i + 1] = d * f4[i];
+}
+
 // gz[i][j] = (hL[i][j] > 0) ? dd2[i] . (wout[0][j], wout[1][j]) : 0
 template <int H>
 __global__ void k_output_back(const float* __restrict__ hL,
@@ -857,6 +911,8 @@ struct Mpnn {
   double cb = 0.0;         // 1 when the last features had history
   double last_ms = 0.0;
   int precision = NW_MPNN_BF16;  // arithmetic of the prediction
+  int loss_kind = NW_MPNN_LOSS_RESIDUAL;  // training loss
+  float* vd = nullptr;        // [n] residual loss: D^-2 A u
   uint16_t* hb[2] = {nullptr, nullptr};  // bfloat16 activations [n][H]
   uint16_t* wfrag = nullptr;  // per layer [H/16][2H/16][64 lanes][4] bf16
   int32_t* eslot = nullptr;   // [nedge][2] the slots (L, R), (R, L) of an edge
@@ -975,6 +1031,7 @@ int ensure_buffers(Mpnn* m, bool training)
       m->h.push_back(p);
     }
   }
+  if (!m->vd) NW_HIP(hipMalloc(&m->vd, sizeof(float) * n));
   if (training && !m->gs) {
     NW_HIP(hipMalloc(&m->gs, sizeof(float) * n * m->H));
     NW_HIP(hipMalloc(&m->gq, sizeof(float) * n * m->H));
@@ -990,7 +1047,8 @@ int slot_weights(Mpnn* m)
   hipStream_t st = c->stream;
   const int64_t ne = std::max<int64_t>(c->nedge, 0);
   if (m->eslot && (m->eslot_code != g.code || m->eslot_nedge != ne)) {
-    (void)hipFree(m->eslot);  // the graph was rebuilt
+    (void)hipFree(m->eslot);
+  (void)hipFree(m->vd);  // the graph was rebuilt
     m->eslot = nullptr;
   }
   if (!m->eslot) {
@@ -1149,10 +1207,22 @@ int loss_grad_h(Mpnn* m, const Sample& smp, double scale, double* loss,
   int rc = forward_h<H>(m, smp.feat, true);
   if (rc) return rc;
   const int off_loss = nv - 1;
-  k_loss<<<kGradBlocks, kBlock, 0, st>>>(
-    m->delta, smp.target, smp.feat + 3 * (int64_t)n, smp.feat + 4 * (int64_t)n,
-    (float)(2.0 * scale / n), m->dd, n, m->partial, nv, off_loss);
-  NW_LAUNCH_CHECK();
+  if (m->loss_kind == NW_MPNN_LOSS_RESIDUAL) {
+    k_res_fwd<<<kGradBlocks, kBlock, 0, st>>>(
+      g.row_ptr, g.cols, g.vals, g.diag, m->delta, smp.target,
+      smp.feat + 2 * (int64_t)n, m->vd, n, m->partial, nv, off_loss);
+    NW_LAUNCH_CHECK();
+    k_res_back<<<grid_for(n, kBlock), kBlock, 0, st>>>(
+      g.row_ptr, g.cols, g.code, g.vals, m->eslot, m->vd,
+      smp.feat + 2 * (int64_t)n, smp.feat + 3 * (int64_t)n,
+      smp.feat + 4 * (int64_t)n, (float)(2.0 * scale / n), m->dd, n);
+    NW_LAUNCH_CHECK();
+  } else {
+    k_loss<<<kGradBlocks, kBlock, 0, st>>>(
+      m->delta, smp.target, smp.feat + 3 * (int64_t)n, smp.feat + 4 * (int64_t)n,
+      (float)(2.0 * scale / n), m->dd, n, m->partial, nv, off_loss);
+    NW_LAUNCH_CHECK();
+  }
   // value offsets: output block, then layers L-1..0, then input block
   int off = 0;
   k_wgrad<H, kModeOutput><<<kGradBlocks, kBlock, 0, st>>>(
@@ -1362,6 +1432,7 @@ void nw_mpnn_destroy(nw_mpnn* m)
   for (auto p : m->hb) (void)hipFree(p);
   (void)hipFree(m->wfrag);
   (void)hipFree(m->eslot);
+  (void)hipFree(m->vd);
   delete m;
 }
 
@@ -1569,6 +1640,15 @@ int nw_mpnn_set_precision(nw_mpnn* m, int precision)
   NW_REQUIRE(precision == NW_MPNN_FP32 || precision == NW_MPNN_BF16, NW_E_ARG,
              "bad precision %d", precision);
   m->precision = precision;
+  return NW_OK;
+}
+
+int nw_mpnn_set_loss(nw_mpnn* m, int kind)
+{
+  NW_REQUIRE(m, NW_E_ARG, "null argument");
+  NW_REQUIRE(kind == NW_MPNN_LOSS_SOLUTION || kind == NW_MPNN_LOSS_RESIDUAL,
+             NW_E_ARG, "bad loss kind %d", kind);
+  m->loss_kind = kind;
   return NW_OK;
 }
 

@@ -36,6 +36,7 @@ PREC_NONE, PREC_JACOBI, PREC_CHEBYSHEV, PREC_AMG = 0, 1, 2, 3
 SOLVE_FP64, SOLVE_FP32_REFINE = 0, 1
 X0_ZERO, X0_SOLUTION, X0_LEARNED = 0, 1, 2
 MPNN_FP32, MPNN_BF16 = 0, 1
+MPNN_LOSS_SOLUTION, MPNN_LOSS_RESIDUAL = 0, 1
 
 
 class NwError(RuntimeError):
@@ -184,6 +185,7 @@ _sig = {
     "nw_mpnn_train": ([_P, C.POINTER(MpnnTrainCfg), C.POINTER(C.c_double)], C.c_int),
     "nw_mpnn_predict": ([_P, _f64p, C.POINTER(C.c_double)], C.c_int),
     "nw_mpnn_set_precision": ([_P, C.c_int], C.c_int),
+    "nw_mpnn_set_loss": ([_P, C.c_int], C.c_int),
     "nw_linsys_amg_level": ([_P, C.c_int, C.POINTER(C.c_int64),
                              C.POINTER(C.c_int64)], C.c_int),
     "nw_linsys_set_learned_guess": ([_P, _P], C.c_int),
@@ -509,6 +511,17 @@ class Mpnn:
         self.hidden, self.layers = hidden, layers
         self.nparams = lib.nw_mpnn_nparams(hidden, layers)
         self._precision = MPNN_BF16
+        self._loss = MPNN_LOSS_RESIDUAL
+
+    @property
+    def loss(self):
+        """Training loss: MPNN_LOSS_RESIDUAL (default) or MPNN_LOSS_SOLUTION."""
+        return self._loss
+
+    @loss.setter
+    def loss(self, kind):
+        check(lib.nw_mpnn_set_loss(self.h, kind))
+        self._loss = kind
 
     @property
     def precision(self):

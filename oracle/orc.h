@@ -350,6 +350,12 @@ void orc_mpnn_forward_bf16(const double* P, int H, int L, const orc_csr* A,
 double orc_mpnn_loss_grad(const double* P, int H, int L, const orc_csr* A,
                           const double* w, const double* feat,
                           const double* target, double* grad);
+/* Residual loss: mean over rows of (D^-1 A u)^2 with u = delta - t on the
+ * rows with couplings (0 on Dirichlet rows); grad receives dLoss/dP. */
+double orc_mpnn_loss_grad_residual(const double* P, int H, int L,
+                                   const orc_csr* A, const double* w,
+                                   const double* feat, const double* target,
+                                   double* grad);
 
 #ifdef __cplusplus
 }

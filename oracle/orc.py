@@ -441,6 +441,10 @@ lib.orc_mpnn_forward_bf16.restype = None
 lib.orc_mpnn_loss_grad.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr),
                                    _f64p, _f64p, _f64p, _f64p]
 lib.orc_mpnn_loss_grad.restype = C.c_double
+lib.orc_mpnn_loss_grad_residual.argtypes = [_f64p, C.c_int, C.c_int,
+                                            C.POINTER(Csr), _f64p, _f64p,
+                                            _f64p, _f64p]
+lib.orc_mpnn_loss_grad_residual.restype = C.c_double
 
 
 def mpnn_nparams(H, L):
@@ -483,6 +487,13 @@ def mpnn_loss_grad(P, H, L, A, w, feat, target):
     g = np.zeros(len(P))
     loss = lib.orc_mpnn_loss_grad(f64(P), H, L, A.ref, f64(w), f64(feat),
                                   f64(target), g)
+    return loss, g
+
+
+def mpnn_loss_grad_residual(P, H, L, A, w, feat, target):
+    g = np.zeros(len(P))
+    loss = lib.orc_mpnn_loss_grad_residual(f64(P), H, L, A.ref, f64(w),
+                                           f64(feat), f64(target), g)
     return loss, g
 
 

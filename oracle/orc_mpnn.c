@@ -257,25 +257,14 @@ void orc_mpnn_forward_bf16(const double* P, int H, int L, const orc_csr* A,
   free(wf);
 }
 
-/* Loss = mean((delta - t)^2); grad (nparams) receives dLoss/dP. */
-double orc_mpnn_loss_grad(const double* P, int H, int L, const orc_csr* A,
+/* Backward of the network from dd = dLoss/ddelta (n values); acts from
+ * orc_mpnn_forward; grad (nparams) is overwritten with dLoss/dP. */
+static void mpnn_backward(const double* P, int H, int L, const orc_csr* A,
                           const double* w, const double* feat,
-                          const double* target, double* grad)
+                          const double* acts, const double* dd, double* grad)
 {
   const int64_t n = A->nrows;
-  const int64_t np = orc_mpnn_nparams(H, L);
-  double* acts = (double*)malloc(sizeof(double) * (size_t)(n * H * (1 + 2 * L)));
-  double* delta = (double*)malloc(sizeof(double) * (size_t)n);
-  orc_mpnn_forward(P, H, L, A, w, feat, delta, acts);
-  memset(grad, 0, sizeof(double) * (size_t)np);
-  double loss = 0.0;
-  double* dd = (double*)malloc(sizeof(double) * (size_t)n);
-  for (int64_t i = 0; i < n; ++i) {
-    const double e = delta[i] - target[i];
-    loss += e * e;
-    dd[i] = 2.0 * e / (double)n;
-  }
-  loss /= (double)n;
+  memset(grad, 0, sizeof(double) * (size_t)orc_mpnn_nparams(H, L));
   /* locate parameter blocks */
   const double* Win = P;
   const double* bin = Win + (int64_t)H * NF;
@@ -341,12 +330,77 @@ double orc_mpnn_loss_grad(const double* P, int H, int L, const orc_csr* A,
       gbin[f] += gzf;
       for (int k = 0; k < NF; ++k) gWin[f * NF + k] += gzf * feat[NF * i + k];
     }
-  free(acts);
-  free(delta);
-  free(dd);
   free(g);
   free(gz);
   free(q);
   free(gm);
+}
+
+/* Loss = mean((delta - t)^2); grad (nparams) receives dLoss/dP. */
+double orc_mpnn_loss_grad(const double* P, int H, int L, const orc_csr* A,
+                          const double* w, const double* feat,
+                          const double* target, double* grad)
+{
+  const int64_t n = A->nrows;
+  double* acts = (double*)malloc(sizeof(double) * (size_t)(n * H * (1 + 2 * L)));
+  double* delta = (double*)malloc(sizeof(double) * (size_t)n);
+  double* dd = (double*)malloc(sizeof(double) * (size_t)n);
+  orc_mpnn_forward(P, H, L, A, w, feat, delta, acts);
+  double loss = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double e = delta[i] - target[i];
+    loss += e * e;
+    dd[i] = 2.0 * e / (double)n;
+  }
+  loss /= (double)n;
+  mpnn_backward(P, H, L, A, w, feat, acts, dd, grad);
+  free(acts);
+  free(delta);
+  free(dd);
+  return loss;
+}
+
+/* Residual loss: u = delta - t on the rows with couplings (0 on Dirichlet
+ * rows, feature 2 set), v = D^-1 A u, Loss = mean(v^2): the Jacobi-scaled
+ * residual of the prediction error (in units of the output scale), the
+ * quantity GMRES starts from.  dLoss/du = (2/n) A^T D^-1 v, masked like u. */
+double orc_mpnn_loss_grad_residual(const double* P, int H, int L,
+                                   const orc_csr* A, const double* w,
+                                   const double* feat, const double* target,
+                                   double* grad)
+{
+  const int64_t n = A->nrows;
+  double* acts = (double*)malloc(sizeof(double) * (size_t)(n * H * (1 + 2 * L)));
+  double* delta = (double*)malloc(sizeof(double) * (size_t)n);
+  double* u = (double*)malloc(sizeof(double) * (size_t)n);
+  double* vd = (double*)malloc(sizeof(double) * (size_t)n);
+  double* dd = (double*)malloc(sizeof(double) * (size_t)n);
+  orc_mpnn_forward(P, H, L, A, w, feat, delta, acts);
+  for (int64_t i = 0; i < n; ++i)
+    u[i] = feat[NF * i + 2] != 0.0 ? 0.0 : delta[i] - target[i];
+  double loss = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double au = 0.0, di = 0.0;
+    for (int64_t k = A->row_ptr[i]; k < A->row_ptr[i + 1]; ++k) {
+      au += A->vals[k] * u[A->cols[k]];
+      if (A->cols[k] == i) di = A->vals[k];
+    }
+    const double v = au / di;
+    loss += v * v;
+    vd[i] = v / di;  /* D^-1 v */
+  }
+  loss /= (double)n;
+  for (int64_t i = 0; i < n; ++i) dd[i] = 0.0;
+  for (int64_t i = 0; i < n; ++i)  /* dd = (2/n) A^T vd */
+    for (int64_t k = A->row_ptr[i]; k < A->row_ptr[i + 1]; ++k)
+      dd[A->cols[k]] += 2.0 / (double)n * A->vals[k] * vd[i];
+  for (int64_t i = 0; i < n; ++i)
+    if (feat[NF * i + 2] != 0.0) dd[i] = 0.0;
+  mpnn_backward(P, H, L, A, w, feat, acts, dd, grad);
+  free(acts);
+  free(delta);
+  free(u);
+  free(vd);
+  free(dd);
   return loss;
 }

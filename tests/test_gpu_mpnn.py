@@ -64,9 +64,12 @@ def _solved_pair(nw, ctx, A, seed):
 
 
 def test_gradient_matches_oracle(nw, ctx):
+    """The solution loss mean((delta - t)^2) and its gradient."""
     P, A, top = _system(nw, ctx)
     xpp, xp, x0, xs = _solved_pair(nw, ctx, A, 11)
     net = nw.Mpnn(ctx, H, L, seed=3)
+    assert net.loss == nw.MPNN_LOSS_RESIDUAL  # the default
+    net.loss = nw.MPNN_LOSS_SOLUTION
     p = _params(nw, 5)
     net.params = p
     net.record()
@@ -86,6 +89,27 @@ def test_gradient_matches_oracle(nw, ctx):
         assert np.abs(g_o[o:o + size]).max() > 0
         o += size
     # the same call again gives identical bits (fixed-order reductions)
+    loss2, g2 = net.loss_grad(0)
+    assert loss2 == loss and np.array_equal(g, g2)
+
+
+def test_residual_loss_gradient_matches_oracle(nw, ctx):
+    """The default residual loss mean((D^-1 A (delta - t))^2) (Dirichlet rows
+    excluded) and its gradient, through A^T on the device, against the FP64
+    oracle (pinned there by finite differences)."""
+    P, A, top = _system(nw, ctx, seed=13)
+    xpp, xp, x0, xs = _solved_pair(nw, ctx, A, 14)
+    net = nw.Mpnn(ctx, H, L, seed=3)
+    p = _params(nw, 15)
+    net.params = p
+    net.record()
+    loss, g = net.loss_grad(0)
+    w = orc.mpnn_slot_weights(A)
+    feat, sc = orc.mpnn_features(A, A.rhs, x0, xp, xpp)
+    target = (xs - (2 * x0 - xp)) / sc[4]
+    loss_o, g_o = orc.mpnn_loss_grad_residual(p, H, L, A, w, feat, target)
+    assert loss == pytest.approx(loss_o, rel=2e-5)
+    assert np.linalg.norm(g - g_o) <= 2e-5 * np.linalg.norm(g_o)
     loss2, g2 = net.loss_grad(0)
     assert loss2 == loss and np.array_equal(g, g2)
 
