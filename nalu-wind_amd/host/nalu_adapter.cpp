@@ -223,6 +223,20 @@ void MomentumEdgePecletAlg::execute()
   check(nw_momentum_edge_peclet(realm_.ctx(), &p), "MomentumEdgePecletAlg");
 }
 
+void MdotEdgeAlg::execute()
+{
+  const SolutionOptions& so = realm_.solutionOptions;
+  auto it = so.nocMap.find("pressure");
+  const bool noc = it == so.nocMap.end() ? so.nocDefault : it->second;
+  const nw_mdot_opts o{noc ? 1.0 : 0.0, so.mdotInterpRhoUTogether ? 1.0 : 0.0};
+  check(nw_compute_mdot(realm_.ctx(), &o), "MdotEdgeAlg");
+}
+
+void NodalGradEdgeAlg::execute()
+{
+  check(nw_nodal_grad(realm_.ctx(), phi_, gradPhi_), "NodalGradEdgeAlg");
+}
+
 void FixPressureAtNodeAlgorithm::execute()
 {
   check(nw_fix_pressure_at_node(realm_.ctx(), node_, refPressure_),

@@ -13,6 +13,8 @@
 //     (src/edge_kernels/*.C)
 //   FixPressureAtNodeAlgorithm                          same name
 //     (src/FixPressureAtNodeAlgorithm.C:57-124)
+//   MdotEdgeAlg, NodalGradEdgeAlg                       same names
+//     (src/ngp_algorithms/{MdotEdgeAlg,NodalGradEdgeAlg}.C)
 //   LinearSystem / TpetraLinearSystem                   LinearSystem /
 //     (include/LinearSystem.h:44-256)                   GpuLinearSystem
 //   CoeffApplier (include/LinearSystem.h:44-74)         CoeffApplier
@@ -219,6 +221,34 @@ class MomentumEdgePecletAlg {
 
  private:
   Realm& realm_;
+};
+
+// MdotEdgeAlg (src/ngp_algorithms/MdotEdgeAlg.C:43-194): writes the edge
+// mass_flow_rate field from the new pressure (interior edges)
+class MdotEdgeAlg {
+ public:
+  explicit MdotEdgeAlg(Realm& realm) : realm_(realm) {}
+  void execute();
+
+ private:
+  Realm& realm_;
+};
+
+// NodalGradEdgeAlg (src/ngp_algorithms/NodalGradEdgeAlg.C:58-111): the
+// interior-edge nodal gradient of node field `phi` into `gradPhi`
+// (NW_F_PRESSURE -> NW_F_DPDX, NW_F_VELOCITY -> NW_F_DUDX, ...); needs the
+// context's edge graph (built by the linear system).
+class NodalGradEdgeAlg {
+ public:
+  NodalGradEdgeAlg(Realm& realm, int phi, int gradPhi)
+    : realm_(realm), phi_(phi), gradPhi_(gradPhi)
+  {
+  }
+  void execute();
+
+ private:
+  Realm& realm_;
+  int phi_, gradPhi_;
 };
 
 class FixPressureAtNodeAlgorithm : public SolverAlgorithm {

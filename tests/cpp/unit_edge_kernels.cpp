@@ -253,8 +253,30 @@ void test_scalar_adv_diff()
     realm.set_field(NW_F_DIFF_FLUX_COEFF, bc);
     std::vector<int32_t> all = {0, 1, 2, 3, 4, 5, 6, 7};
     eq.linsys().applyDirichletBCs(NW_F_SCALAR_Q, NW_F_DIFF_FLUX_COEFF, all, 0, 1);
-    print_system("scalar_adv_diff_dirichlet", eq.linsys(), true);
+    print_system("scalar_adv_diff_dirichlet", eq.linsys(), false);
   }
+}
+
+// UnitTestMdotAlg.C:21-78 (NGP_mdot_calc_edge): density 1, velocity 10,
+// pressure and its gradient 0, rho*U interpolated together
+void test_mdot_calc_edge()
+{
+  Hex8Mesh m = fill_hex8_mesh();
+  Realm realm(0);
+  realm.set_mesh(8, m.edges);
+  realm.set_field(NW_F_COORDINATES, m.coords);
+  realm.set_field(NW_E_AREA_VECTOR, m.areav);
+  realm.set_field(NW_F_DENSITY, std::vector<double>(8, 1.0));
+  realm.set_field(NW_F_VELOCITY, std::vector<double>(24, 10.0));
+  realm.set_field(NW_F_PRESSURE, std::vector<double>(8, 0.0));
+  realm.set_field(NW_F_DPDX, std::vector<double>(24, 0.0));
+  realm.set_field(NW_F_UDIAG, std::vector<double>(8, 1.0));
+  realm.solutionOptions.mdotInterpRhoUTogether = true;
+  MdotEdgeAlg(realm).execute();
+  const std::vector<double> mdot = realm.get_field(NW_E_MASS_FLOW_RATE);
+  std::printf("\"mdot_calc_edge\": [");
+  for (size_t i = 0; i < mdot.size(); ++i) std::printf("%s%.17g", i ? "," : "", mdot[i]);
+  std::printf("]\n");
 }
 
 }  // namespace
@@ -266,6 +288,7 @@ int main()
     test_continuity_advection();
     test_momentum_advection_diffusion();
     test_scalar_adv_diff();
+    test_mdot_calc_edge();
     std::printf("}\n");
   } catch (const std::exception& e) {
     std::fprintf(stderr, "FAILED: %s\n", e.what());

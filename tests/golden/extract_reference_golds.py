@@ -12,6 +12,8 @@ Sources (reference paths):
   unit_tests/edge_kernels/UnitTestMomentumAdvDiffEdge.C:16-230
   unit_tests/UnitTestPecletFunction.C:36-110
   unit_tests/ngp_algorithms/UnitTestGeometryAlg.C:20-100
+  unit_tests/ngp_algorithms/UnitTestMdotAlg.C:21-78 (MdotEdgeAlg on one
+    unit-cube Hex8: the field fills and the expected edge value)
 
 Only numbers are kept.  The inputs that generate them (one unit-cube Hex8,
 the trigonometric test fields, scheme options) are restated in
@@ -83,6 +85,18 @@ def main(ref):
                               "area_mag_sq": 0.0625, "num_edges": 12,
                               "tol": 1e-16},
     }
+    mdot = rd("unit_tests/ngp_algorithms/UnitTestMdotAlg.C")
+    body = mdot[mdot.index("NGP_mdot_calc_edge"):]
+    body = body[:body.index("TEST_F", 10)]
+    fill = lambda field: float(re.search(
+        r"field_fill\(\s*([-+0-9.eE]+)\s*,\s*\*%s\s*\)" % field, body).group(1))
+    near = re.search(r"EXPECT_NEAR\(\s*mdot\[0\]\s*,\s*([-+0-9.eE]+)", body)
+    out["mdot_edge_unit_hex"] = {
+        "density": fill("density_"), "velocity": fill("velocity_"),
+        "pressure": fill("pressure_"), "dpdx": fill("dpdx_"),
+        "interp_together": 1.0, "mdot": float(near.group(1)),
+        "tol": float(re.search(r"tol\s*=\s*([-+0-9.eE]+)", body).group(1)),
+        "num_edges": 12}
     out["scalar_adv_diff_edge_tpetra"]["tol"] = 1.0e-14
     out["scalar_adv_diff_edge_tpetra"]["fix_node_id"] = 1
     out["scalar_adv_diff_edge_tpetra"]["fix_ref_pressure"] = 1.0

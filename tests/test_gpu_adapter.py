@@ -66,3 +66,11 @@ def test_scalar_adv_diff_tpetra(out, name, vals, rhs):
     assert s["cols"] == g["cols_serial"]
     np.testing.assert_allclose(s["vals"], g[vals], rtol=0, atol=g["tol"])
     np.testing.assert_allclose(s["rhs"], g[rhs], rtol=0, atol=g["tol"])
+
+
+def test_mdot_calc_edge(out):
+    """UnitTestMdotAlg.C:21-78 through MdotEdgeAlg of the adapter."""
+    g = GOLD["mdot_edge_unit_hex"]
+    md = np.array(out["mdot_calc_edge"])
+    assert md.size == g["num_edges"]
+    np.testing.assert_allclose(md, g["mdot"], rtol=0, atol=g["tol"])

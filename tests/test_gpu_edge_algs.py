@@ -124,3 +124,28 @@ def test_edge_alg_errors(nw, ctx):
         with pytest.raises(nw.NwError) as e:
             ctx.nodal_grad(fin, fout)
         assert e.value.code == nw.E_ARG
+
+
+def test_mdot_reference_gold(nw, ctx):
+    """UnitTestMdotAlg.C:21-78 through the C ABI: unit-cube Hex8, density 1,
+    velocity 10, zero pressure and gradient -> mdot = 2.5 on all 12 edges."""
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "reference_golds.json")))["mdot_edge_unit_hex"]
+    coords, conn = nw.hex_box(1, 1, 1, 1.0, 0.0, 0)
+    n = coords.shape[0]
+    edges, elem_edges = nw.hex_edges(conn, n)
+    areav = nw.hex_edge_area_vector(conn, coords, elem_edges, edges)
+    ctx.mesh_set(n, edges)
+    for fid, v in ((nw.F_COORDINATES, coords), (nw.E_AREA_VECTOR, areav),
+                   (nw.F_VELOCITY, np.full((n, 3), g["velocity"])),
+                   (nw.F_DPDX, np.full((n, 3), g["dpdx"])),
+                   (nw.F_DENSITY, np.full(n, g["density"])),
+                   (nw.F_PRESSURE, np.full(n, g["pressure"])),
+                   (nw.F_UDIAG, np.ones(n))):
+        ctx.field_upload(fid, v)
+    ctx.compute_mdot(1.0, g["interp_together"])
+    md = ctx.field_download(nw.E_MASS_FLOW_RATE)
+    assert md.size == g["num_edges"]
+    np.testing.assert_allclose(md, g["mdot"], rtol=0, atol=g["tol"])

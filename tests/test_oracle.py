@@ -591,3 +591,19 @@ def test_mpnn_residual_loss_gradient_matches_finite_differences():
     # the solution loss is unchanged by the refactoring of the backward
     l0, g0 = orc.mpnn_loss_grad(P, H, L, A, w, feat, t)
     assert l0 == pytest.approx(np.mean((d - t) ** 2), rel=1e-14)
+
+
+def test_mdot_alg_reference_gold():
+    """UnitTestMdotAlg.C:21-78: one unit-cube Hex8, density 1, velocity 10,
+    zero pressure and pressure gradient: every edge carries mdot = 2.5."""
+    g = GOLD["mdot_edge_unit_hex"]
+    coords, conn = fx.hex_box(1, 1, 1)
+    n = coords.shape[0]
+    edges, elem_edges = orc.create_edges_hex(conn, n)
+    areav = orc.edge_area_vec_hex(conn, coords, elem_edges, edges)
+    assert edges.shape[0] == g["num_edges"]
+    md = orc.mdot_edge_alg(edges, coords, np.full((n, 3), g["velocity"]),
+                           np.full((n, 3), g["dpdx"]), np.full(n, g["density"]),
+                           np.full(n, g["pressure"]), np.ones(n), areav, 1.0,
+                           g["interp_together"])
+    np.testing.assert_allclose(md, g["mdot"], rtol=0, atol=g["tol"])
