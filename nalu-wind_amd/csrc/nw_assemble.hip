@@ -23,11 +23,12 @@
 // end), each matrix entry is written once, no atomics are used, and the
 // result is bitwise reproducible.
 //
-// Rows are taken in chunks of 64 consecutive rows from a schedule sorted by
-// the Morton code of the chunk centroids, and workgroups are renumbered so
-// that each XCD works on one contiguous stretch of that schedule: rows
-// processed together are neighbours in space and their shared node data
-// stays in the XCD's L2.
+// Rows are taken in chunks of 64 consecutive rows from a schedule ordered
+// for locality (the row order, a plane-by-plane sweep of strips, or the
+// Morton code of the chunk centroids, see build_schedule), and workgroups
+// are renumbered so that each XCD works on one contiguous stretch of that
+// schedule: rows processed together are neighbours in space and their
+// shared node and edge data stays in the XCD's L2.
 //
 // The physics restates the reference edge kernels:
 //   ScalarEdgeSolverAlg.C:87-205, MomentumEdgeSolverAlg.C:107-312,
@@ -623,6 +624,121 @@ uint64_t spread3(uint64_t v)
   return v;
 }
 
+// Centroid and extent of every chunk, and the bounding box of the nodes.
+struct ChunkGeom {
+  std::vector<double> cen[3], ext[3];
+  double lo[3], hi[3];
+};
+
+ChunkGeom chunk_geometry(const std::vector<double>& x, int64_t N,
+                         const std::vector<int2>& ch)
+{
+  ChunkGeom g;
+  const int64_t nch = (int64_t)ch.size();
+  for (int d = 0; d < 3; ++d) {
+    const auto mm = std::minmax_element(x.begin() + d * N, x.begin() + (d + 1) * N);
+    g.lo[d] = *mm.first;
+    g.hi[d] = *mm.second;
+    g.cen[d].resize(nch);
+    g.ext[d].resize(nch);
+    for (int64_t i = 0; i < nch; ++i) {
+      double m = 0.0, a = x[d * N + ch[i].x], b = a;
+      for (int r = ch[i].x; r < ch[i].y; ++r) {
+        const double v = x[d * N + r];
+        m += v;
+        a = std::min(a, v);
+        b = std::max(b, v);
+      }
+      g.cen[d][i] = m / (ch[i].y - ch[i].x);
+      g.ext[d][i] = b - a;
+    }
+  }
+  return g;
+}
+
+std::vector<int2> morton_order(const ChunkGeom& geo, const std::vector<int2>& ch)
+{
+  const int64_t nch = (int64_t)ch.size();
+  std::vector<std::pair<uint64_t, int64_t>> key(nch);
+  for (int64_t i = 0; i < nch; ++i) {
+    uint64_t k = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double span = geo.hi[d] > geo.lo[d] ? geo.hi[d] - geo.lo[d] : 1.0;
+      const double t = (geo.cen[d][i] - geo.lo[d]) / span;
+      const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
+      k |= spread3(q) << d;
+    }
+    key[i] = {k, i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int2> o(nch);
+  for (int64_t i = 0; i < nch; ++i) o[i] = ch[key[i].second];
+  return o;
+}
+
+// Sweep order.  The chunks' own axis (the one along which they extend
+// furthest) is the inner axis; of the other two the longer is swept in
+// planes, and the remaining one is cut into strips of equal chunk count,
+// about kPlaneChunks chunks of a strip per plane.  A plane of a strip is a
+// run of chunks (sorted along the sweep axis) whose centroids lie within
+// half a mesh spacing h = (box volume / nodes)^(1/3) of the run's first:
+// on a grid this is one grid plane however the spacing is graded, and it
+// never cuts a grid plane in two.  The schedule runs strip by
+// strip, plane by plane, ordered inside a plane along the strip axis then
+// the inner axis.  Neighbours in the sweep direction are then one
+// plane-tile (~kPlaneChunks chunks) apart, neighbours along the strip axis
+// a few chunks, so the node and edge data a plane shares with the next is
+// still in L2 when it is read again; each XCD's contiguous share of the
+// schedule is a strip or a part of one.
+std::vector<int2> sweep_order(const ChunkGeom& geo, int64_t N, const std::vector<int2>& ch)
+{
+  constexpr double kPlaneChunks = 128.0;
+  const int64_t nch = (int64_t)ch.size();
+  double span[3], mext[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < 3; ++d) {
+    span[d] = geo.hi[d] - geo.lo[d];
+    for (int64_t i = 0; i < nch; ++i) mext[d] += geo.ext[d][i];
+  }
+  const int ai = (int)(std::max_element(mext, mext + 3) - mext);
+  const int o1 = (ai + 1) % 3, o2 = (ai + 2) % 3;
+  const int as = span[o1] > span[o2] ? o1 : o2, at = as == o1 ? o2 : o1;
+  const double vol = std::max(span[0], 1e-300) * std::max(span[1], 1e-300) *
+                     std::max(span[2], 1e-300);
+  const double h = std::cbrt(vol / (double)N);
+  const int64_t nplanes = std::max<int64_t>(1, std::llround(span[as] / h));
+  const int64_t nstrips = std::max<int64_t>(
+    1, std::llround((double)nch / (double)nplanes / kPlaneChunks));
+  // strips: equal chunk counts along the strip axis
+  std::vector<int64_t> idx(nch), strip(nch);
+  for (int64_t i = 0; i < nch; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+    return geo.cen[at][a] < geo.cen[at][b] || (geo.cen[at][a] == geo.cen[at][b] && a < b);
+  });
+  for (int64_t r = 0; r < nch; ++r) strip[idx[r]] = r * nstrips / nch;
+  std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+    if (strip[a] != strip[b]) return strip[a] < strip[b];
+    return geo.cen[as][a] < geo.cen[as][b] || (geo.cen[as][a] == geo.cen[as][b] && a < b);
+  });
+  std::vector<int64_t> plane(nch);
+  for (int64_t r = 0, p = 0; r < nch; ++r) {
+    const int64_t i = idx[r];
+    if (r > 0 && (strip[i] != strip[idx[r - 1]] ||
+                  geo.cen[as][i] - geo.cen[as][idx[p]] > 0.5 * h))
+      p = r;
+    plane[i] = p;  // the rank of the plane's first chunk
+  }
+  std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+    if (strip[a] != strip[b]) return strip[a] < strip[b];
+    if (plane[a] != plane[b]) return plane[a] < plane[b];
+    if (geo.cen[at][a] != geo.cen[at][b]) return geo.cen[at][a] < geo.cen[at][b];
+    if (geo.cen[ai][a] != geo.cen[ai][b]) return geo.cen[ai][a] < geo.cen[ai][b];
+    return a < b;
+  });
+  std::vector<int2> o(nch);
+  for (int64_t i = 0; i < nch; ++i) o[i] = ch[idx[i]];
+  return o;
+}
+
 // Fraction of (row, neighbour) pairs whose chunks lie more than kWindow
 // positions apart in the schedule `ch`, over every 16th row: the pairs whose
 // node data is unlikely to be still in L2 when the second row is assembled
@@ -659,12 +775,15 @@ int64_t slot_quantile(const Graph& g, int64_t rows, double q)
   return v[k];
 }
 
-// Chunks of <= kRowsPerWave consecutive rows, in the row order or ordered by
-// the Morton code of their coordinate centroid, whichever leaves fewer
-// neighbour pairs far apart in the schedule: Morton for lexicographically
-// numbered boxes (whose z neighbours are a whole plane apart), the row order
-// for bandwidth-reducing (RCM) numberings, whose level sets are compact.
-// NW_ROW_ORDER=natural|morton forces one.
+// Chunks of <= kRowsPerWave consecutive rows in one of three orders,
+// whichever leaves the fewest neighbour pairs far apart in the schedule:
+//   natural  the row order: compact for bandwidth-reducing (RCM)
+//            numberings, whose level sets are local;
+//   sweep    strips along one axis, each swept plane by plane along another
+//            (see sweep_order): for lexicographically numbered grids, whose
+//            chunks are segments of grid lines;
+//   morton   the Morton code of the chunk centroids.
+// NW_ROW_ORDER=natural|sweep|morton forces one.
 int build_schedule(Ctx* c)
 {
   Graph& g = c->g;
@@ -676,36 +795,26 @@ int build_schedule(Ctx* c)
                       (int)std::min<int64_t>(N, (i + 1) * kRowsPerWave));
   const char* env = getenv("NW_ROW_ORDER");
   const std::string force = env ? std::string(env) : std::string();
-  if (force != "natural" && c->field[NW_F_COORDINATES]) {
+  if (force != "natural" && c->field[NW_F_COORDINATES] && nch > 1) {
     std::vector<double> x(3 * N);  // SoA: x[d * N + n]
     NW_HIP(hipMemcpyAsync(x.data(), c->field[NW_F_COORDINATES],
                           sizeof(double) * 3 * N, hipMemcpyDeviceToHost,
                           c->stream));
     NW_HIP(hipStreamSynchronize(c->stream));
-    double lo[3], hi[3];
-    for (int d = 0; d < 3; ++d) {
-      const auto mm = std::minmax_element(x.begin() + d * N, x.begin() + (d + 1) * N);
-      lo[d] = *mm.first;
-      hi[d] = *mm.second;
+    const ChunkGeom geo = chunk_geometry(x, N, ch);
+    std::vector<int2> mo = morton_order(geo, ch), sw = sweep_order(geo, N, ch);
+    if (force == "morton") {
+      ch.swap(mo);
+    } else if (force == "sweep") {
+      ch.swap(sw);
+    } else {
+      const double fn = far_pair_fraction(g, ch), fs = far_pair_fraction(g, sw),
+                   fm = far_pair_fraction(g, mo);
+      if (fs < fn && fs <= fm)
+        ch.swap(sw);
+      else if (fm < fn && fm < fs)
+        ch.swap(mo);
     }
-    std::vector<std::pair<uint64_t, int64_t>> key(nch);
-    for (int64_t i = 0; i < nch; ++i) {
-      uint64_t k = 0;
-      for (int d = 0; d < 3; ++d) {
-        double m = 0.0;
-        for (int r = ch[i].x; r < ch[i].y; ++r) m += x[d * N + r];
-        const double span = hi[d] > lo[d] ? hi[d] - lo[d] : 1.0;
-        const double t = (m / (ch[i].y - ch[i].x) - lo[d]) / span;
-        const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
-        k |= spread3(q) << d;
-      }
-      key[i] = {k, i};
-    }
-    std::sort(key.begin(), key.end());
-    std::vector<int2> o(nch);
-    for (int64_t i = 0; i < nch; ++i) o[i] = ch[key[i].second];
-    if (force == "morton" || far_pair_fraction(g, o) < far_pair_fraction(g, ch))
-      ch.swap(o);
   }
   if (!g.sched) NW_HIP(hipMalloc(&g.sched, sizeof(int2) * std::max<int64_t>(nch, 1)));
   NW_HIP(hipMemcpyAsync(g.sched, ch.data(), sizeof(int2) * nch,
