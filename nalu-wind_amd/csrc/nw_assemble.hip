@@ -407,15 +407,6 @@ struct MomentumPhys {
 
 // ------------------------------------------------------------------ kernel
 
-// XCD-aware block order: the hardware deals workgroups round-robin to the
-// 8 XCDs; renumbering them as below hands each XCD one contiguous stretch
-// of the (locality-ordered) chunk list, so neighbouring rows share an L2.
-__device__ inline int xcd_block(int nblocks_padded)
-{
-  const int per = nblocks_padded / 8;
-  return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-}
-
 // Walks the slots g, g+G, ... of `row`: for each off-diagonal slot s
 // evaluates the edge, hands the (row, neighbour) block to put(s, off) and
 // adds the edge's (row, row) block and rhs(row) contributions to acc.
@@ -877,6 +868,8 @@ int launch_rows(Ctx* c, const P& phys, int accumulate)
   } while (0)
 
 }  // namespace
+
+int ensure_schedule(Ctx* c) { return c->g.sched_valid ? NW_OK : build_schedule(c); }
 
 }  // namespace nw
 

@@ -6,10 +6,11 @@
 //                reads and mass-flow-rate writes, the two nodes' fields
 //                gathered from the component-major node arrays;
 //   k_nodal_grad NodalGradEdgeAlg (src/ngp_algorithms/NodalGradEdgeAlg.C:
-//                58-111): one lane per row of the edge graph; the lane walks
-//                the row's slots (slot code 2e + side names the edge and
-//                which end the row is), sums +-q_ip A_e / V and writes the
-//                row's 3c components once.  Each node's sum is owned by one
+//                58-111): one lane per row of the edge graph, rows taken in
+//                the assembly's locality-ordered 64-row chunks; the lane
+//                walks the row's slots (slot code 2e + side names the edge
+//                and which end the row is), sums +-q_ip A_e / V and writes
+//                the row's 3c components once.  Each node's sum is owned by one
 //                lane, so no atomics are needed and the result does not
 //                depend on scheduling (the reference scatters both ends of
 //                every edge with atomic adds).
@@ -64,32 +65,67 @@ __global__ __launch_bounds__(kBlock) void k_mdot(MdotArgs a,
   mdot[e] = tmdot;
 }
 
-// grad[(3 j + i) * N + n] = sum_slots sign * (q_ip,j A_e,i) / V_n
-template <int C>
+// Slots of a chunk staged in LDS (64 Hex8 rows hold 448); larger chunks
+// read codes and columns from HBM directly.
+constexpr int kGradSlots = 512;
+
+// grad[(3 j + i) * N + n] = sum_slots sign * (q_ip,j A_e,i) / V_n, summed
+// in slot order.  A wave takes one 64-row chunk of the assembly's locality
+// schedule (XCD-aware block order): the chunk's slot codes and columns are
+// staged in LDS by coalesced loads, and each lane issues the neighbour and
+// edge gathers of U slots before it uses any of them.
+template <int C, int U>
 __global__ __launch_bounds__(kBlock) void k_nodal_grad(
   const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ code,
-  const int32_t* __restrict__ cols, const double* __restrict__ q,
-  const double* __restrict__ areav, const double* __restrict__ dualvol,
-  double* __restrict__ grad, int64_t N, int64_t E)
+  const int32_t* __restrict__ cols, const int2* __restrict__ sched,
+  int nchunks, const double* __restrict__ q, const double* __restrict__ areav,
+  const double* __restrict__ dualvol, double* __restrict__ grad, int64_t N,
+  int64_t E)
 {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+  constexpr int WPB = kBlock / kWave;
+  __shared__ int code_s[WPB][kGradSlots];
+  __shared__ int col_s[WPB][kGradSlots];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = (int)(threadIdx.x / kWave);
+  const int chunk_id = xcd_block((int)gridDim.x) * WPB + wv;
+  if (chunk_id >= nchunks) return;
+  const int2 ch = sched[chunk_id];
+  const int s_lo = row_ptr[ch.x], ns = row_ptr[ch.y] - s_lo;
+  const bool staged = ns <= kGradSlots;
+  if (staged)
+    for (int i = lane; i < ns; i += kWave) {
+      code_s[wv][i] = code[s_lo + i];
+      col_s[wv][i] = cols[s_lo + i];
+    }
+  wave_lds_sync();
+  const int n = ch.x + lane;
+  if (n >= ch.y) return;
   double qn[C], acc[3 * C];
   const double inv = 1.0 / dualvol[n];
   for (int j = 0; j < C; ++j) qn[j] = q[j * N + n];
   for (int k = 0; k < 3 * C; ++k) acc[k] = 0.0;
   const int s1 = row_ptr[n + 1];
-  for (int s = row_ptr[n]; s < s1; ++s) {
-    const int c = code[s];
-    if (c < 0) continue;  // the diagonal slot
-    const int64_t e = c >> 1;
-    const double sgn = (c & 1) ? -1.0 : 1.0;  // n = R: minus, n = L: plus
-    const int64_t m = cols[s];
-    double av[3];
-    for (int i = 0; i < 3; ++i) av[i] = areav[i * E + e];
-    for (int j = 0; j < C; ++j) {
-      const double qip = 0.5 * (qn[j] + q[j * N + m]);
-      for (int i = 0; i < 3; ++i) acc[3 * j + i] += sgn * ((av[i] * qip) * inv);
+  for (int s = row_ptr[n]; s < s1; s += U) {
+    int cu[U], mu[U];
+    for (int u = 0; u < U; ++u) {
+      const int t = s + u;
+      cu[u] = t < s1 ? (staged ? code_s[wv][t - s_lo] : code[t]) : -1;
+      mu[u] = t < s1 ? (staged ? col_s[wv][t - s_lo] : cols[t]) : n;
+    }
+    double av[U][3], qm[U][C];
+    for (int u = 0; u < U; ++u) {
+      if (cu[u] < 0) continue;  // the diagonal slot, or past the row
+      const int64_t e = cu[u] >> 1;
+      for (int i = 0; i < 3; ++i) av[u][i] = areav[i * E + e];
+      for (int j = 0; j < C; ++j) qm[u][j] = q[j * N + mu[u]];
+    }
+    for (int u = 0; u < U; ++u) {
+      if (cu[u] < 0) continue;
+      const double sgn = (cu[u] & 1) ? -1.0 : 1.0;  // n = R: minus, n = L: plus
+      for (int j = 0; j < C; ++j) {
+        const double qip = 0.5 * (qn[j] + qm[u][j]);
+        for (int i = 0; i < 3; ++i) acc[3 * j + i] += sgn * ((av[u][i] * qip) * inv);
+      }
     }
   }
   for (int k = 0; k < 3 * C; ++k) grad[k * N + n] = acc[k];
@@ -149,15 +185,18 @@ int nw_nodal_grad(nw_ctx* c, int fin, int fout)
     return rc;
   double* grad = nullptr;
   if ((rc = ensure_field(c, fout, &grad))) return rc;
-  const unsigned nb = grid_for(c->nnode, kBlock);
+  if ((rc = ensure_schedule(c))) return rc;
+  if (g.nchunks == 0) return NW_OK;
+  constexpr int64_t wpb = kBlock / kWave;
+  const unsigned nb = (unsigned)(((g.nchunks + wpb - 1) / wpb + 7) / 8 * 8);
   if (C == 1)
-    k_nodal_grad<1><<<nb, kBlock, 0, c->stream>>>(g.row_ptr, g.code, g.cols, q,
-                                                  areav, dualvol, grad, c->nnode,
-                                                  c->nedge);
+    k_nodal_grad<1, 4><<<nb, kBlock, 0, c->stream>>>(
+      g.row_ptr, g.code, g.cols, g.sched, (int)g.nchunks, q, areav, dualvol,
+      grad, c->nnode, c->nedge);
   else
-    k_nodal_grad<3><<<nb, kBlock, 0, c->stream>>>(g.row_ptr, g.code, g.cols, q,
-                                                  areav, dualvol, grad, c->nnode,
-                                                  c->nedge);
+    k_nodal_grad<3, 4><<<nb, kBlock, 0, c->stream>>>(
+      g.row_ptr, g.code, g.cols, g.sched, (int)g.nchunks, q, areav, dualvol,
+      grad, c->nnode, c->nedge);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }

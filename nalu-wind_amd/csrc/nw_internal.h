@@ -57,6 +57,16 @@ __device__ inline void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// XCD-aware block order: the hardware deals workgroups round-robin to the
+// 8 XCDs; renumbering them as below hands each XCD one contiguous stretch
+// of a locality-ordered chunk list, so neighbouring rows share an L2.
+// nblocks_padded must be a multiple of 8.
+__device__ inline int xcd_block(int nblocks_padded)
+{
+  const int per = nblocks_padded / 8;
+  return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+}
+
 // Slot code of a CSR (block) slot: diagonal slot of row n -> -(n+1);
 // off-diagonal slot reached through edge e from its node[side] -> 2e+side.
 __host__ __device__ inline int32_t diag_code(int32_t n) { return -(n + 1); }
@@ -126,6 +136,9 @@ int ensure_field(Ctx* c, int f, double** out);
 int require_field(Ctx* c, int f, const double** out);
 void free_krylov(Krylov* k);
 void amg_free(AmgDevice* a);
+// Builds the graph's locality-ordered chunk schedule (g.sched, g.nchunks)
+// if it is not current.
+int ensure_schedule(Ctx* c);
 // Learned guess predicted from the context's system, x0 and the guesses
 // before it xp, xpp (device vectors of nnode doubles); wall time in
 // mpnn_last_ms.
