@@ -296,6 +296,32 @@ def polynomial_solves(n, log):
     return res
 
 
+def device_geometry(hip, ctx, W, types, steps, warmup):
+    """GeometryInteriorAlg on the device for a workload that kept its
+    elements: the host setup of the incidence lists (once per mesh), the
+    device time of one nw_compute_geometry call (scratch allocation
+    included), and the largest deviation from the host-computed area vectors
+    and dual volumes.  Overwrites the context's area vectors and dual
+    volumes with the device values."""
+    import numpy as np
+    import naluwind_amd as nw
+    t0 = time.perf_counter()
+    ctx.mesh_elements(types, W.conn)
+    setup_s = time.perf_counter() - t0
+    kms, _ = time_launches(hip, ctx, ctx.compute_geometry, steps, warmup)
+    a = ctx.field_download(nw.E_AREA_VECTOR)
+    nel = int(W.conn.shape[0])
+    res = {"elements": nel, "incidence_setup_s": round(setup_s, 2),
+           "ms": round(kms, 3), "Melems_per_s": round(nel / (kms * 1e-3) / 1e6, 1),
+           "max_rel_diff_area_vs_host": float(np.abs(a - W.areav).max() /
+                                              np.abs(W.areav).max())}
+    if W.dualvol is not None:
+        v = ctx.field_download(nw.F_DUAL_VOLUME)
+        res["max_rel_diff_dual_volume_vs_host"] = float(
+            np.abs(v - W.dualvol).max() / np.abs(W.dualvol).max())
+    return res
+
+
 def mixed_mesh(hip, n, steps, warmup, log):
     """Config of the unstructured mixed mesh: continuity assembly and the
     AMG-preconditioned pressure solve (FP64 and FP32-inner/FP64-refined,
@@ -304,7 +330,7 @@ def mixed_mesh(hip, n, steps, warmup, log):
     import naluwind_amd as nw
     import workloads as wl
     t0 = time.perf_counter()
-    W = wl.continuity_mixed(n)
+    W = wl.continuity_mixed(n, keep_elements=True)
     log("mixed mesh %d^3: %d nodes %d edges (%.1fs)" % (n, W.n, W.ne,
                                                         time.perf_counter() - t0))
     res = {"workload": "continuity on a %d^3 box cut into Hex8/Tet4/Wed6 "
@@ -341,6 +367,8 @@ def mixed_mesh(hip, n, steps, warmup, log):
                                     "converged": bool(st.converged)}
             log("mixed solve %s: %s" % (name, res["solve_" + name]))
         res["amg_levels_rows_nnz"] = ctx.amg_levels()
+        res["geometry"] = device_geometry(hip, ctx, W, W.types, min(steps, 5), 1)
+        log("mixed geometry: %s" % res["geometry"])
     return res
 
 
@@ -454,7 +482,7 @@ def main():
 
         # ---------------------------------------------------------- continuity
         t0 = time.perf_counter()
-        W = wl.continuity_rf(n3)
+        W = wl.continuity_rf(n3, keep_elements=True)
         log("continuity workload %d^3: %d nodes %d edges (%.1fs)" % (n3, W.n, W.ne,
                                                                      time.perf_counter() - t0))
         with nw.Context(0) as ctx:
@@ -489,6 +517,9 @@ def main():
                                    "frac_hbm_peak": round(nb / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
                 log("%s: %s" % (name, edge_algs[name]))
             out["edge_algorithms_256"] = edge_algs
+            # GeometryInteriorAlg on the device, same mesh
+            out["geometry_256"] = device_geometry(hip, ctx, W, None, min(a.steps, 5), 1)
+            log("geometry: %s" % out["geometry_256"])
         del W
 
     # ---------------------------------------------------------- solve

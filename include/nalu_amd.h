@@ -217,6 +217,27 @@ int nw_compute_mdot(nw_ctx* ctx, const nw_mdot_opts* opts);
  * Boundary-face contributions (NodalGradBndryElemAlg) are not built. */
 int nw_nodal_grad(nw_ctx* ctx, int field_in, int field_out);
 
+/* ---- geometry (GeometryInteriorAlg, src/ngp_algorithms/
+ * GeometryInteriorAlg.C:70-225) ---------------------------------------------
+ * nw_mesh_elements: the mesh's elements over the nodes and edges of
+ *   nw_mesh_set: type [nelem] (0 Hex8, 1 Tet4, 2 Wed6; NULL = all Hex8),
+ *   conn [nelem][8] in exodus node order (unused slots -1).  Builds, on the
+ *   host, the lists of element sub-control surfaces per edge and
+ *   sub-control volumes per node (element order) that the device sums
+ *   through.  NW_E_ARG when a node is out of range, an element edge is not
+ *   in the edge list, or nelem > 2^26.  Once per mesh.
+ * nw_compute_geometry: impl_compute_edge_area_vector and
+ *   impl_compute_dual_nodal_volume on the device from NW_F_COORDINATES:
+ *   writes NW_E_AREA_VECTOR (sum of the element sub-control-surface area
+ *   vectors of each edge, HexSCS/TetSCS/WedSCS::determinant_scs, oriented
+ *   from node[0] to node[1]) and NW_F_DUAL_VOLUME (sum of the sub-control
+ *   volumes of each node, HexSCV/TetSCV/WedSCV::determinant_scv).  Sums run
+ *   in element order (the reference adds with atomics), so the result is
+ *   reproducible.  Call again after the coordinates move. */
+int nw_mesh_elements(nw_ctx* ctx, int64_t nelem, const int32_t* type,
+                     const int32_t* conn);
+int nw_compute_geometry(nw_ctx* ctx);
+
 /* ---- linear solve (TpetraLinearSystem::solve -> Belos GMRES) ------------ */
 enum {
   NW_PREC_NONE = 0,

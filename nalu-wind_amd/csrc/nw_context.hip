@@ -205,6 +205,7 @@ int nw_ctx_destroy(nw_ctx* c)
   (void)hipFree(c->x_hist);
   (void)hipFree(c->x_hist2);
   free_graph(c->g);
+  geom_free(c->geom);
   for (auto& f : c->field) (void)hipFree(f);
   (void)hipFree(c->edges);
   (void)hipStreamDestroy(c->stream);
@@ -252,6 +253,8 @@ int nw_mesh_set(nw_ctx* c, int64_t nnode, int64_t nedge,
   (void)hipFree(c->x_hist2);
   c->x_hist2 = nullptr;
   free_graph(c->g);
+  geom_free(c->geom);
+  c->geom = nullptr;
   for (auto& f : c->field) {
     (void)hipFree(f);
     f = nullptr;

@@ -1,0 +1,386 @@
+// nw_geometry.hip -- GeometryInteriorAlg (src/ngp_algorithms/
+// GeometryInteriorAlg.C:70-225) on gfx950: the edge area vectors and dual
+// nodal volumes of a Hex8 / Tet4 / Wed6 mesh from its node coordinates.
+//
+//   k_elem_scs,    one lane per element: the element's sub-control-surface
+//   k_elem_scv     (SCS) area vectors and sub-control volumes (SCV) into
+//                  element-minor buffers (entry k of element el at
+//                  k * nelem + el, so a wave's stores are coalesced); each
+//                  sub-point is formed from the corners where it is used;
+//   k_edge_area    one lane per edge: sums +-SCS over the edge's surfaces in
+//                  element order, + when the surface's L node is the edge's
+//                  node[0] (GeometryInteriorAlg.C:209-219);
+//   k_dual_volume  one lane per node: sums its SCVs in element order.
+// The reference adds element contributions into the edge and node fields
+// with atomics; here the incidence lists built once per mesh fix the order
+// of every sum, so the result is reproducible and is the same sum, in the
+// same order, as the host producers in host/nw_mesh.cpp and nw_mixed.cpp.
+//
+// Each SCS is a quad of four points of the subdivided element and its area
+// vector is taken over four triangles about the quad's centroid; each SCV is
+// a hexahedron of eight such points whose volume is Grandy's
+// (hex_volume_grandy, include/master_element/Hex8GeometryFunctions.h:85).
+// A point is the average of a set of element vertices.
+#include <algorithm>
+#include <vector>
+
+#include "nw_geom.h"
+#include "nw_internal.h"
+
+namespace nw {
+
+namespace {
+
+// Points of the subdivided element as {count, vertices...}: corners first.
+// Hex8 (include/master_element/Hex8GeometryFunctions.h:258-330): edge
+// midpoints, face centroids face by face, centroid.
+struct HexTopo {
+  static constexpr int kNodes = 8, kScs = 12, kScv = 8, kPts = 27;
+  static constexpr int8_t pts[27][9] = {
+    {1, 0},          {1, 1},          {1, 2},          {1, 3},
+    {1, 4},          {1, 5},          {1, 6},          {1, 7},
+    {2, 0, 1},       {2, 1, 2},       {2, 2, 3},       {2, 3, 0},
+    {4, 0, 1, 2, 3}, {2, 4, 5},       {2, 5, 6},       {2, 6, 7},
+    {2, 7, 4},       {4, 4, 5, 6, 7}, {2, 1, 5},       {2, 0, 4},
+    {4, 0, 1, 4, 5}, {2, 3, 7},       {2, 2, 6},       {4, 2, 3, 6, 7},
+    {4, 1, 2, 5, 6}, {4, 0, 3, 4, 7}, {8, 0, 1, 2, 3, 4, 5, 6, 7}};
+  // HexSCS::determinant_scs (src/master_element/Hex8CVFEM.C:567-591)
+  static constexpr int8_t quad[12][4] = {
+    {20, 8, 12, 26},  {24, 9, 12, 26},  {10, 12, 26, 23}, {11, 25, 26, 12},
+    {13, 20, 26, 17}, {17, 14, 24, 26}, {17, 15, 23, 26}, {16, 17, 26, 25},
+    {19, 20, 26, 25}, {20, 18, 24, 26}, {22, 23, 26, 24}, {21, 25, 26, 23}};
+  // HexSCV::determinant_scv (src/master_element/Hex8CVFEM.C:365-389)
+  static constexpr int8_t scv[8][8] = {
+    {0, 8, 12, 11, 19, 20, 26, 25},  {8, 1, 9, 12, 20, 18, 24, 26},
+    {12, 9, 2, 10, 26, 24, 22, 23},  {11, 12, 10, 3, 25, 26, 23, 21},
+    {19, 20, 26, 25, 4, 13, 17, 16}, {20, 18, 24, 26, 13, 5, 14, 17},
+    {26, 24, 22, 23, 17, 14, 6, 15}, {25, 26, 23, 21, 16, 17, 15, 7}};
+};
+
+// Tet4 (src/master_element/Tet4CVFEM.C:243-343, 522-618)
+struct TetTopo {
+  static constexpr int kNodes = 4, kScs = 6, kScv = 4, kPts = 15;
+  static constexpr int8_t pts[15][9] = {
+    {1, 0},       {1, 1},       {1, 2},          {1, 3},       {2, 0, 1},
+    {2, 1, 2},    {2, 2, 0},    {3, 0, 1, 2},    {2, 2, 3},    {2, 3, 1},
+    {3, 1, 2, 3}, {2, 0, 3},    {3, 0, 2, 3},    {3, 0, 1, 3}, {4, 0, 1, 2, 3}};
+  static constexpr int8_t quad[6][4] = {{4, 7, 14, 13}, {7, 14, 10, 5},
+                                        {6, 12, 14, 7}, {11, 13, 14, 12},
+                                        {13, 9, 10, 14}, {10, 8, 12, 14}};
+  static constexpr int8_t scv[4][8] = {{0, 4, 7, 6, 11, 13, 14, 12},
+                                       {1, 5, 7, 4, 9, 10, 14, 13},
+                                       {2, 6, 7, 5, 8, 12, 14, 10},
+                                       {3, 9, 13, 11, 8, 10, 14, 12}};
+};
+
+// Wed6 (src/master_element/Wed6CVFEM.C:267-369, 541-646)
+struct WedTopo {
+  static constexpr int kNodes = 6, kScs = 9, kScv = 6, kPts = 21;
+  static constexpr int8_t pts[21][9] = {
+    {1, 0},          {1, 1},          {1, 2},       {1, 3},
+    {1, 4},          {1, 5},          {2, 0, 1},    {2, 1, 2},
+    {2, 2, 0},       {3, 0, 1, 2},    {2, 3, 4},    {2, 4, 5},
+    {2, 5, 3},       {3, 3, 4, 5},    {2, 1, 4},    {2, 0, 3},
+    {4, 0, 1, 4, 3}, {2, 2, 5},       {4, 1, 4, 5, 2},
+    {4, 5, 3, 0, 2}, {6, 0, 1, 2, 3, 4, 5}};
+  static constexpr int8_t quad[9][4] = {
+    {6, 9, 20, 16},   {7, 9, 20, 18},   {9, 8, 19, 20},
+    {10, 16, 20, 13}, {13, 11, 18, 20}, {12, 13, 20, 19},
+    {15, 16, 20, 19}, {16, 14, 18, 20}, {19, 20, 18, 17}};
+  static constexpr int8_t scv[6][8] = {
+    {0, 15, 16, 6, 8, 19, 20, 9},    {9, 6, 1, 7, 20, 16, 14, 18},
+    {8, 9, 7, 2, 19, 20, 18, 17},    {19, 15, 16, 20, 12, 3, 10, 13},
+    {20, 16, 14, 18, 13, 10, 4, 11}, {19, 20, 18, 17, 12, 13, 11, 5}};
+};
+
+constexpr int kMaxScs = 12, kMaxScv = 8;
+
+// Area vector of the quad (q0, q1, q2, q3): half the sum of the cross
+// products of consecutive corners about the quad's centroid.
+__device__ inline void quad_area(const double (&q)[4][3], double (&a)[3])
+{
+  double m[3];
+  for (int d = 0; d < 3; ++d) m[d] = 0.25 * (q[0][d] + q[1][d] + q[2][d] + q[3][d]);
+  a[0] = a[1] = a[2] = 0.0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int k = (t + 1) & 3;
+    const double u[3] = {q[t][0] - m[0], q[t][1] - m[1], q[t][2] - m[2]};
+    const double v[3] = {q[k][0] - m[0], q[k][1] - m[1], q[k][2] - m[2]};
+    a[0] += u[1] * v[2] - v[1] * u[2];
+    a[1] += u[2] * v[0] - v[2] * u[0];
+    a[2] += u[0] * v[1] - v[0] * u[1];
+  }
+  for (int d = 0; d < 3; ++d) a[d] *= 0.5;
+}
+
+// Volume of the hexahedron h (exodus order) by 24 triangles about its face
+// centroids and the divergence theorem (Grandy 1997).
+__device__ inline double grandy_volume(const double (&h)[8][3])
+{
+  constexpr int8_t face[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                 {2, 3, 7, 6}, {1, 2, 6, 5}, {0, 4, 3, 7}};
+  constexpr int8_t tri[24][3] = {
+    {0, 8, 1},  {8, 2, 1},  {3, 2, 8},  {3, 8, 0},  {6, 9, 5},  {7, 9, 6},
+    {4, 9, 7},  {4, 5, 9},  {10, 0, 1}, {5, 10, 1}, {4, 10, 5}, {4, 0, 10},
+    {7, 6, 11}, {6, 2, 11}, {2, 3, 11}, {3, 7, 11}, {6, 12, 2}, {5, 12, 6},
+    {5, 1, 12}, {1, 2, 12}, {0, 4, 13}, {4, 7, 13}, {7, 3, 13}, {3, 0, 13}};
+  double p[14][3];
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+    for (int d = 0; d < 3; ++d) p[n][d] = h[n][d];
+#pragma unroll
+  for (int f = 0; f < 6; ++f)
+    for (int d = 0; d < 3; ++d)
+      p[8 + f][d] = 0.25 * (p[face[f][0]][d] + p[face[f][1]][d] +
+                            p[face[f][2]][d] + p[face[f][3]][d]);
+  double vol = 0.0;
+#pragma unroll
+  for (int t = 0; t < 24; ++t) {
+    const double* a = p[tri[t][0]];
+    const double* b = p[tri[t][1]];
+    const double* c = p[tri[t][2]];
+    const double m[3] = {a[0] + b[0] + c[0], a[1] + b[1] + c[1], a[2] + b[2] + c[2]};
+    const double n0 = (b[1] - a[1]) * (c[2] - a[2]) - (c[1] - a[1]) * (b[2] - a[2]);
+    const double n1 = (c[0] - a[0]) * (b[2] - a[2]) - (b[0] - a[0]) * (c[2] - a[2]);
+    const double n2 = (b[0] - a[0]) * (c[1] - a[1]) - (c[0] - a[0]) * (b[1] - a[1]);
+    vol += m[0] * n0 + m[1] * n1 + m[2] * n2;
+  }
+  return vol / 18.0;
+}
+
+// Point k of the subdivided element of topology T from its corners c.
+template <class T>
+__device__ inline void sub_point(const double (&c)[T::kNodes][3], int k,
+                                 double (&p)[3])
+{
+  const int cnt = T::pts[k][0];
+  for (int d = 0; d < 3; ++d) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 1; i <= 8; ++i)
+      if (i <= cnt) s += c[T::pts[k][i]][d];
+    p[d] = s / cnt;
+  }
+}
+
+template <class T>
+__device__ inline void corners(int64_t el, const int32_t* __restrict__ conn,
+                               const double* __restrict__ coords, int64_t N,
+                               double (&c)[T::kNodes][3])
+{
+#pragma unroll
+  for (int a = 0; a < T::kNodes; ++a) {
+    const int64_t n = conn[8 * el + a];
+    for (int d = 0; d < 3; ++d) c[a][d] = coords[d * N + n];
+  }
+}
+
+// SCS area vectors of element el of topology T: scs[(3 s + d) nelem + el]
+template <class T>
+__device__ inline void elem_scs(int64_t el, int64_t nelem,
+                                const int32_t* __restrict__ conn,
+                                const double* __restrict__ coords, int64_t N,
+                                double* __restrict__ scs)
+{
+  double c[T::kNodes][3];
+  corners<T>(el, conn, coords, N, c);
+#pragma unroll
+  for (int s = 0; s < T::kScs; ++s) {
+    double q[4][3], a[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sub_point<T>(c, T::quad[s][i], q[i]);
+    quad_area(q, a);
+    for (int d = 0; d < 3; ++d) scs[(3 * s + d) * nelem + el] = a[d];
+  }
+}
+
+// SCV volumes of element el of topology T: scv[v nelem + el]
+template <class T>
+__device__ inline void elem_scv(int64_t el, int64_t nelem,
+                                const int32_t* __restrict__ conn,
+                                const double* __restrict__ coords, int64_t N,
+                                double* __restrict__ scv)
+{
+  double c[T::kNodes][3];
+  corners<T>(el, conn, coords, N, c);
+#pragma unroll
+  for (int v = 0; v < T::kScv; ++v) {
+    double h[8][3];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) sub_point<T>(c, T::scv[v][i], h[i]);
+    scv[v * nelem + el] = grandy_volume(h);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_elem_scs(
+  int64_t nelem, const int32_t* __restrict__ type,
+  const int32_t* __restrict__ conn, const double* __restrict__ coords,
+  int64_t N, double* __restrict__ scs)
+{
+  const int64_t el = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= nelem) return;
+  const int t = type[el];
+  if (t == 0)
+    elem_scs<HexTopo>(el, nelem, conn, coords, N, scs);
+  else if (t == 1)
+    elem_scs<TetTopo>(el, nelem, conn, coords, N, scs);
+  else
+    elem_scs<WedTopo>(el, nelem, conn, coords, N, scs);
+}
+
+__global__ __launch_bounds__(kBlock) void k_elem_scv(
+  int64_t nelem, const int32_t* __restrict__ type,
+  const int32_t* __restrict__ conn, const double* __restrict__ coords,
+  int64_t N, double* __restrict__ scv)
+{
+  const int64_t el = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= nelem) return;
+  const int t = type[el];
+  if (t == 0)
+    elem_scv<HexTopo>(el, nelem, conn, coords, N, scv);
+  else if (t == 1)
+    elem_scv<TetTopo>(el, nelem, conn, coords, N, scv);
+  else
+    elem_scv<WedTopo>(el, nelem, conn, coords, N, scv);
+}
+
+// areav[d * E + e] = sum over the edge's surfaces (element order) of
+// +-scs[(3 s + d) * nelem + el]
+__global__ __launch_bounds__(kBlock) void k_edge_area(
+  int64_t E, int64_t nelem, const int32_t* __restrict__ e_ptr,
+  const int32_t* __restrict__ e_inc, const double* __restrict__ scs,
+  double* __restrict__ areav)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double a[3] = {0.0, 0.0, 0.0};
+  for (int k = e_ptr[e]; k < e_ptr[e + 1]; ++k) {
+    const int32_t c = e_inc[k];
+    const int32_t i = c >> 1;
+    const int64_t el = i / kMaxScs;
+    const int s = i - (int)el * kMaxScs;
+    const double sgn = (c & 1) ? -1.0 : 1.0;
+    for (int d = 0; d < 3; ++d) a[d] += scs[(3 * s + d) * nelem + el] * sgn;
+  }
+  for (int d = 0; d < 3; ++d) areav[d * E + e] = a[d];
+}
+
+// dualvol[n] = sum over the node's elements (element order) of its SCV
+__global__ __launch_bounds__(kBlock) void k_dual_volume(
+  int64_t N, int64_t nelem, const int32_t* __restrict__ n_ptr,
+  const int32_t* __restrict__ n_inc, const double* __restrict__ scv,
+  double* __restrict__ dualvol)
+{
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double v = 0.0;
+  for (int k = n_ptr[n]; k < n_ptr[n + 1]; ++k) {
+    const int32_t c = n_inc[k];
+    const int64_t el = c >> 3;
+    v += scv[(int64_t)(c & 7) * nelem + el];
+  }
+  dualvol[n] = v;
+}
+
+}  // namespace
+
+void geom_free(Geom* g)
+{
+  if (!g) return;
+  (void)hipFree(g->type);
+  (void)hipFree(g->conn);
+  (void)hipFree(g->e_ptr);
+  (void)hipFree(g->e_inc);
+  (void)hipFree(g->n_ptr);
+  (void)hipFree(g->n_inc);
+  delete g;
+}
+
+}  // namespace nw
+
+using namespace nw;
+
+namespace {
+
+template <class T>
+int upload(T** dst, const T* src, int64_t n, hipStream_t s)
+{
+  NW_HIP(hipMalloc(dst, sizeof(T) * std::max<int64_t>(n, 1)));
+  NW_HIP(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, s));
+  return NW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nw_mesh_elements(nw_ctx* c, int64_t nelem, const int32_t* type,
+                     const int32_t* conn)
+{
+  NW_REQUIRE(c && nelem >= 0 && (conn || nelem == 0), NW_E_ARG, "bad element arguments");
+  NW_REQUIRE(c->nnode > 0 && c->nedge >= 0, NW_E_STATE, "mesh not set");
+  std::vector<int32_t> e_ptr, e_inc, n_ptr, n_inc;
+  const int rc = geom::build_incidence(nelem, type, conn, c->nnode, c->nedge,
+                                       c->h_edges.data(), e_ptr, e_inc, n_ptr,
+                                       n_inc);
+  NW_REQUIRE(rc == NW_OK, NW_E_ARG,
+             "bad elements (type or node out of range, an element edge not in "
+             "the edge list, or more than 2^26 elements)");
+  NW_HIP(hipSetDevice(c->device));
+  NW_HIP(hipStreamSynchronize(c->stream));
+  geom_free(c->geom);
+  c->geom = new Geom;
+  Geom* g = c->geom;
+  g->nelem = nelem;
+  std::vector<int32_t> t(nelem, 0);
+  if (type) t.assign(type, type + nelem);
+  int r = 0;
+  if ((r = upload(&g->type, t.data(), nelem, c->stream)) ||
+      (r = upload(&g->conn, conn, 8 * nelem, c->stream)) ||
+      (r = upload(&g->e_ptr, e_ptr.data(), (int64_t)e_ptr.size(), c->stream)) ||
+      (r = upload(&g->e_inc, e_inc.data(), (int64_t)e_inc.size(), c->stream)) ||
+      (r = upload(&g->n_ptr, n_ptr.data(), (int64_t)n_ptr.size(), c->stream)) ||
+      (r = upload(&g->n_inc, n_inc.data(), (int64_t)n_inc.size(), c->stream)))
+    return r;
+  NW_HIP(hipStreamSynchronize(c->stream));
+  return NW_OK;
+}
+
+int nw_compute_geometry(nw_ctx* c)
+{
+  NW_REQUIRE(c, NW_E_ARG, "null context");
+  NW_REQUIRE(c->geom, NW_E_STATE, "no elements (nw_mesh_elements)");
+  const Geom* g = c->geom;
+  const double* coords = nullptr;
+  double *areav = nullptr, *dualvol = nullptr;
+  int rc = 0;
+  if ((rc = require_field(c, NW_F_COORDINATES, &coords)) ||
+      (rc = ensure_field(c, NW_E_AREA_VECTOR, &areav)) ||
+      (rc = ensure_field(c, NW_F_DUAL_VOLUME, &dualvol)))
+    return rc;
+  const int64_t ne = g->nelem;
+  double *scs = nullptr, *scv = nullptr;
+  NW_HIP(hipMalloc(&scs, sizeof(double) * kMaxScs * 3 * std::max<int64_t>(ne, 1)));
+  NW_HIP(hipMalloc(&scv, sizeof(double) * kMaxScv * std::max<int64_t>(ne, 1)));
+  if (ne > 0) {
+    k_elem_scs<<<grid_for(ne, kBlock), kBlock, 0, c->stream>>>(
+      ne, g->type, g->conn, coords, c->nnode, scs);
+    k_elem_scv<<<grid_for(ne, kBlock), kBlock, 0, c->stream>>>(
+      ne, g->type, g->conn, coords, c->nnode, scv);
+  }
+  if (c->nedge > 0)
+    k_edge_area<<<grid_for(c->nedge, kBlock), kBlock, 0, c->stream>>>(
+      c->nedge, ne, g->e_ptr, g->e_inc, scs, areav);
+  k_dual_volume<<<grid_for(c->nnode, kBlock), kBlock, 0, c->stream>>>(
+    c->nnode, ne, g->n_ptr, g->n_inc, scv, dualvol);
+  const hipError_t le = hipGetLastError();
+  const hipError_t se = hipStreamSynchronize(c->stream);
+  (void)hipFree(scs);
+  (void)hipFree(scv);
+  NW_HIP(le);
+  NW_HIP(se);
+  return NW_OK;
+}
+
+}  // extern "C"

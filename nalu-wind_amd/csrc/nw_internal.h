@@ -111,6 +111,19 @@ struct Krylov;     // solver workspace (nw_krylov.hip)
 struct AmgDevice;  // AMG hierarchy (nw_amg.hip)
 struct Mpnn;       // learned initial guess (nw_mpnn.hip)
 
+// Elements of the mesh and the incidence lists of the device geometry
+// (nw_mesh_elements, nw_compute_geometry); all device arrays.
+struct Geom {
+  int64_t nelem = 0;
+  int32_t* type = nullptr;   // [nelem] 0 Hex8, 1 Tet4, 2 Wed6
+  int32_t* conn = nullptr;   // [nelem][8]
+  int32_t* e_ptr = nullptr;  // [nedge + 1]
+  int32_t* e_inc = nullptr;  // 2 (12 el + s) + flip, element order per edge
+  int32_t* n_ptr = nullptr;  // [nnode + 1]
+  int32_t* n_inc = nullptr;  // 8 el + a, element order per node
+};
+void geom_free(Geom* g);
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -126,6 +139,7 @@ struct Ctx {
   double* x_hist = nullptr;  // [nnode] guess the solve before it started from
   double* x_hist2 = nullptr; // [nnode] and the one before that
   Mpnn* mpnn = nullptr;      // network for NW_X0_LEARNED (not owned)
+  Geom* geom = nullptr;      // elements for the device geometry
 };
 
 int field_components(int f);

@@ -223,6 +223,18 @@ void MomentumEdgePecletAlg::execute()
   check(nw_momentum_edge_peclet(realm_.ctx(), &p), "MomentumEdgePecletAlg");
 }
 
+GeometryInteriorAlg::GeometryInteriorAlg(Realm& realm, int64_t nelem,
+                                         const int32_t* type, const int32_t* conn)
+  : realm_(realm)
+{
+  check(nw_mesh_elements(realm_.ctx(), nelem, type, conn), "GeometryInteriorAlg");
+}
+
+void GeometryInteriorAlg::execute()
+{
+  check(nw_compute_geometry(realm_.ctx()), "GeometryInteriorAlg");
+}
+
 void MdotEdgeAlg::execute()
 {
   const SolutionOptions& so = realm_.solutionOptions;

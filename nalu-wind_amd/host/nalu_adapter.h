@@ -13,8 +13,8 @@
 //     (src/edge_kernels/*.C)
 //   FixPressureAtNodeAlgorithm                          same name
 //     (src/FixPressureAtNodeAlgorithm.C:57-124)
-//   MdotEdgeAlg, NodalGradEdgeAlg                       same names
-//     (src/ngp_algorithms/{MdotEdgeAlg,NodalGradEdgeAlg}.C)
+//   MdotEdgeAlg, NodalGradEdgeAlg, GeometryInteriorAlg  same names
+//     (src/ngp_algorithms/{MdotEdgeAlg,NodalGradEdgeAlg,GeometryInteriorAlg}.C)
 //   LinearSystem / TpetraLinearSystem                   LinearSystem /
 //     (include/LinearSystem.h:44-256)                   GpuLinearSystem
 //   CoeffApplier (include/LinearSystem.h:44-74)         CoeffApplier
@@ -217,6 +217,21 @@ class ContinuityEdgeSolverAlg : public AssembleEdgeSolverAlgorithm {
 class MomentumEdgePecletAlg {
  public:
   explicit MomentumEdgePecletAlg(Realm& realm) : realm_(realm) {}
+  void execute();
+
+ private:
+  Realm& realm_;
+};
+
+// GeometryInteriorAlg (src/ngp_algorithms/GeometryInteriorAlg.C:70-225):
+// the edge area vectors and dual nodal volumes of the realm's elements
+// (type [nelem]: 0 Hex8, 1 Tet4, 2 Wed6, or null for all Hex8; conn
+// [nelem][8]) from its coordinates, on the device.  The constructor hands
+// the elements over once; execute() recomputes after the nodes move.
+class GeometryInteriorAlg {
+ public:
+  GeometryInteriorAlg(Realm& realm, int64_t nelem, const int32_t* type,
+                      const int32_t* conn);
   void execute();
 
  private:

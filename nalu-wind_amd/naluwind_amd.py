@@ -168,6 +168,8 @@ _sig = {
     "nw_momentum_edge_peclet": ([_P, C.POINTER(Peclet)], C.c_int),
     "nw_compute_mdot": ([_P, C.POINTER(MdotOpts)], C.c_int),
     "nw_nodal_grad": ([_P, C.c_int, C.c_int], C.c_int),
+    "nw_mesh_elements": ([_P, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
+    "nw_compute_geometry": ([_P], C.c_int),
     "nw_linsys_solve": ([_P, C.POINTER(SolverCfg), C.POINTER(SolveStats)], C.c_int),
     "nw_linsys_solution_upload": ([_P, _f64p], C.c_int),
     "nw_linsys_solution_download": ([_P, _f64p], C.c_int),
@@ -457,6 +459,21 @@ class Context:
     def nodal_grad(self, field_in, field_out):
         """NodalGradEdgeAlg: field_out = edge-based gradient of field_in."""
         check(lib.nw_nodal_grad(self.h, field_in, field_out))
+
+    def mesh_elements(self, types, conn):
+        """The mesh's elements for the device geometry: types [nelem]
+        (0 Hex8, 1 Tet4, 2 Wed6; None = all Hex8), conn [nelem][8]."""
+        conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 8)
+        t = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
+        check(lib.nw_mesh_elements(
+            self.h, conn.shape[0],
+            None if t is None else t.ctypes.data_as(C.c_void_p),
+            conn.ctypes.data_as(C.c_void_p)))
+
+    def compute_geometry(self):
+        """GeometryInteriorAlg on the device: NW_E_AREA_VECTOR and
+        NW_F_DUAL_VOLUME from the coordinates."""
+        check(lib.nw_compute_geometry(self.h))
 
     def amg_levels(self):
         """[(rows, nnz)] of the AMG hierarchy built by the last AMG solve."""

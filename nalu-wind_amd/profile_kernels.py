@@ -30,6 +30,15 @@ def run(kernel, n, reps, copy_mib):
         W = wl.continuity_rf(n)
         ndof = 1
         fn = lambda ctx: ctx.assemble_continuity(W.opts)
+    elif kernel == "edge_algs":
+        # MdotEdgeAlg and NodalGradEdgeAlg (dp/dx, du/dx) on the continuity box
+        W = wl.continuity_rf(n)
+        ndof = 1
+
+        def fn(ctx):
+            ctx.compute_mdot(1.0, 1.0)
+            ctx.nodal_grad(nw.F_PRESSURE, nw.F_DPDX)
+            ctx.nodal_grad(nw.F_VELOCITY, nw.F_DUDX)
     else:
         W = wl.scalar_tg(n)
         ndof = 1

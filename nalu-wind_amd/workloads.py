@@ -132,7 +132,8 @@ def grading_128(n):
     return 1.05 ** (128.0 / n)
 
 
-def continuity_rf(n=256, seed=1234, stretch=None, dt=1.0e-3, t=0.0):
+def continuity_rf(n=256, seed=1234, stretch=None, dt=1.0e-3, t=0.0,
+                  keep_elements=False):
     if stretch is None:
         stretch = grading_128(n)
     W = Workload("continuity_rf", n, n, n, stretch, 0.0, seed)
@@ -146,7 +147,8 @@ def continuity_rf(n=256, seed=1234, stretch=None, dt=1.0e-3, t=0.0):
     W.opts = nw.continuity_defaults(dt=dt, gamma1=1.0)
     W.top = np.where(W.coords[:, 2] >= 1.0 - 1e-12)[0].astype(np.int32)
     W.seed = seed
-    W.drop_elements()
+    if not keep_elements:
+        W.drop_elements()
     return W
 
 
@@ -154,7 +156,7 @@ class MixedWorkload(Workload):
     """Mixed Hex8/Tet4/Wed6 box from the library's generator (RCM node
     order), its edges and edge area vectors."""
 
-    def __init__(self, name, n, f_hex, f_tet, jitter, seed):
+    def __init__(self, name, n, f_hex, f_tet, jitter, seed, keep_elements=False):
         self.name = name
         self.cells = (n, n, n)
         self.coords, types, conn = nw.mixed_box(n, f_hex, f_tet, jitter, seed)
@@ -167,12 +169,13 @@ class MixedWorkload(Workload):
         self.dualvol = None
         self.fields = {}
         self.top = None
-        self.conn = None
+        self.types, self.conn = (types, conn) if keep_elements else (None, None)
 
 
 def continuity_mixed(n=160, seed=1234, dt=1.0e-3, f_hex=0.6, f_tet=0.3,
-                     jitter=0.10):
-    W = MixedWorkload("continuity_mixed", n, f_hex, f_tet, jitter, seed)
+                     jitter=0.10, keep_elements=False):
+    W = MixedWorkload("continuity_mixed", n, f_hex, f_tet, jitter, seed,
+                      keep_elements)
     vel = fourier_velocity(W.coords, seed)
     W.fields = {
         nw.F_VELOCITY: vel, nw.F_DENSITY: np.ones(W.n),

@@ -64,6 +64,19 @@ void orc_wed_scs_areav(const double coords[18], double areav[27]);
 extern const int orc_tet_lrscv[12];
 extern const int orc_wed_lrscv[18];
 
+/* Tet4 / Wed6 SCV volumes (4 / 6 sub-control volumes, sub-volume i at
+ * node i), restating TetSCV::determinant_scv (src/master_element/
+ * Tet4CVFEM.C:243-343) and WedSCV::determinant_scv (src/master_element/
+ * Wed6CVFEM.C:267-369): hex_volume_grandy of eight sub-points each. */
+void orc_tet_scv_volume(const double coords[12], double vol[4]);
+void orc_wed_scv_volume(const double coords[18], double vol[6]);
+
+/* Dual nodal volumes of a mixed mesh (type 0 Hex8, 1 Tet4, 2 Wed6; conn
+ * [nelem][8]), src/ngp_algorithms/GeometryInteriorAlg.C:70-112. */
+void orc_dual_volume_mixed(int64_t nelem, const int32_t* type,
+                           const int32_t* conn, const double* coords,
+                           int64_t nnode, double* dualvol);
+
 /* Edge area vectors of a mixed mesh (type 0 Hex8, 1 Tet4, 2 Wed6; conn
  * [nelem][8]); edges sorted (node[0] < node[1]).  Returns -1 if an element
  * edge is missing from the list. */

@@ -131,6 +131,10 @@ lib.orc_wed_scs_areav.argtypes = [_f64p, _f64p]
 lib.orc_edge_area_vec_mixed.argtypes = [C.c_int64, _i32p, _i32p, _f64p, _i32p,
                                         C.c_int64, _f64p]
 lib.orc_edge_area_vec_mixed.restype = C.c_int
+lib.orc_tet_scv_volume.argtypes = [_f64p, _f64p]
+lib.orc_wed_scv_volume.argtypes = [_f64p, _f64p]
+lib.orc_dual_volume_mixed.argtypes = [C.c_int64, _i32p, _i32p, _f64p, C.c_int64,
+                                      _f64p]
 lib.orc_spmv.argtypes = [C.POINTER(Csr), _f64p, _f64p]
 lib.orc_gmres.argtypes = [C.POINTER(Csr), _f64p, _f64p, C.POINTER(GmresOpts),
                           C.POINTER(GmresStats)]
@@ -185,6 +189,26 @@ WED_LRSCV = np.array([0, 1, 1, 2, 0, 2, 3, 4, 4, 5, 3, 5, 0, 3, 1, 4, 2,
                       5]).reshape(9, 2)
 HEX_LRSCV = np.array([0, 1, 1, 2, 2, 3, 0, 3, 4, 5, 5, 6, 6, 7, 4, 7, 0, 4,
                       1, 5, 2, 6, 3, 7]).reshape(12, 2)
+
+
+def tet_scv_volume(coords4):
+    out = np.zeros(4)
+    lib.orc_tet_scv_volume(f64(coords4).ravel(), out)
+    return out
+
+
+def wed_scv_volume(coords6):
+    out = np.zeros(6)
+    lib.orc_wed_scv_volume(f64(coords6).ravel(), out)
+    return out
+
+
+def dual_volume_mixed(types, conn, coords):
+    coords = f64(coords)
+    out = np.zeros(coords.shape[0])
+    lib.orc_dual_volume_mixed(len(types), i32(types), i32(conn).ravel(),
+                              coords.ravel(), coords.shape[0], out)
+    return out
 
 
 def edge_area_vec_mixed(types, conn, coords, edges):

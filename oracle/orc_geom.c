@@ -95,12 +95,8 @@ void orc_hex_scs_areav(const double coords[24], double areav[36])
 
 /* TetSCS::determinant_scs, src/master_element/Tet4CVFEM.C:522-618: edge
  * midpoints, face centroids and the centroid, then one quad per edge. */
-void orc_tet_scs_areav(const double c[12], double areav[18])
+static void tet_points(const double c[12], double v[15][3])
 {
-  static const int facet[6][4] = {{4, 7, 14, 13},  {7, 14, 10, 5},
-                                  {6, 12, 14, 7},  {11, 13, 14, 12},
-                                  {13, 9, 10, 14}, {10, 8, 12, 14}};
-  double v[15][3];
 #define C(n, d) c[3 * (n) + (d)]
   for (int d = 0; d < 3; ++d) {
     for (int n = 0; n < 4; ++n) v[n][d] = C(n, d);
@@ -118,6 +114,15 @@ void orc_tet_scs_areav(const double c[12], double areav[18])
     for (int n = 0; n < 4; ++n) v[14][d] += 0.25 * C(n, d);
   }
 #undef C
+}
+
+void orc_tet_scs_areav(const double c[12], double areav[18])
+{
+  static const int facet[6][4] = {{4, 7, 14, 13},  {7, 14, 10, 5},
+                                  {6, 12, 14, 7},  {11, 13, 14, 12},
+                                  {13, 9, 10, 14}, {10, 8, 12, 14}};
+  double v[15][3];
+  tet_points(c, v);
   for (int ics = 0; ics < 6; ++ics) {
     double q[4][3];
     for (int i = 0; i < 4; ++i)
@@ -127,13 +132,8 @@ void orc_tet_scs_areav(const double c[12], double areav[18])
 }
 
 /* WedSCS::determinant_scs, src/master_element/Wed6CVFEM.C:541-646 */
-void orc_wed_scs_areav(const double c[18], double areav[27])
+static void wed_points(const double c[18], double v[21][3])
 {
-  static const int facet[9][4] = {
-    {6, 9, 20, 16},  {7, 9, 20, 18},  {9, 8, 19, 20},
-    {10, 16, 20, 13}, {13, 11, 18, 20}, {12, 13, 20, 19},
-    {15, 16, 20, 19}, {16, 14, 18, 20}, {19, 20, 18, 17}};
-  double v[21][3];
 #define C(n, d) c[3 * (n) + (d)]
   for (int d = 0; d < 3; ++d) {
     for (int n = 0; n < 6; ++n) v[n][d] = C(n, d);
@@ -155,6 +155,16 @@ void orc_wed_scs_areav(const double c[18], double areav[27])
     for (int n = 0; n < 6; ++n) v[20][d] += (1.0 / 6.0) * C(n, d);
   }
 #undef C
+}
+
+void orc_wed_scs_areav(const double c[18], double areav[27])
+{
+  static const int facet[9][4] = {
+    {6, 9, 20, 16},  {7, 9, 20, 18},  {9, 8, 19, 20},
+    {10, 16, 20, 13}, {13, 11, 18, 20}, {12, 13, 20, 19},
+    {15, 16, 20, 19}, {16, 14, 18, 20}, {19, 20, 18, 17}};
+  double v[21][3];
+  wed_points(c, v);
   for (int ics = 0; ics < 9; ++ics) {
     double q[4][3];
     for (int i = 0; i < 4; ++i)
@@ -272,6 +282,68 @@ void orc_hex_scv_volume(const double coords[24], double vol[8])
     for (int n = 0; n < 8; ++n)
       for (int d = 0; d < 3; ++d) h[n][d] = v[sub[ip][n]][d];
     vol[ip] = hex_volume_grandy(h);
+  }
+}
+
+/* TetSCV::determinant_scv, src/master_element/Tet4CVFEM.C:243-343: each
+ * sub-control volume is the hexahedron of eight points of the subdivided
+ * tetrahedron, its volume hex_volume_grandy's. */
+void orc_tet_scv_volume(const double c[12], double vol[4])
+{
+  static const int sub[4][8] = {{0, 4, 7, 6, 11, 13, 14, 12},
+                                {1, 5, 7, 4, 9, 10, 14, 13},
+                                {2, 6, 7, 5, 8, 12, 14, 10},
+                                {3, 9, 13, 11, 8, 10, 14, 12}};
+  double v[15][3];
+  tet_points(c, v);
+  for (int s = 0; s < 4; ++s) {
+    double h[8][3];
+    for (int n = 0; n < 8; ++n)
+      for (int d = 0; d < 3; ++d) h[n][d] = v[sub[s][n]][d];
+    vol[s] = hex_volume_grandy(h);
+  }
+}
+
+/* WedSCV::determinant_scv, src/master_element/Wed6CVFEM.C:267-369 */
+void orc_wed_scv_volume(const double c[18], double vol[6])
+{
+  static const int sub[6][8] = {
+    {0, 15, 16, 6, 8, 19, 20, 9},    {9, 6, 1, 7, 20, 16, 14, 18},
+    {8, 9, 7, 2, 19, 20, 18, 17},    {19, 15, 16, 20, 12, 3, 10, 13},
+    {20, 16, 14, 18, 13, 10, 4, 11}, {19, 20, 18, 17, 12, 13, 11, 5}};
+  double v[21][3];
+  wed_points(c, v);
+  for (int s = 0; s < 6; ++s) {
+    double h[8][3];
+    for (int n = 0; n < 8; ++n)
+      for (int d = 0; d < 3; ++d) h[n][d] = v[sub[s][n]][d];
+    vol[s] = hex_volume_grandy(h);
+  }
+}
+
+/* Dual nodal volumes of a mixed Hex8/Tet4/Wed6 mesh: the element loop of
+ * GeometryInteriorAlg::impl_compute_dual_nodal_volume
+ * (src/ngp_algorithms/GeometryInteriorAlg.C:70-112); sub-volume ip of an
+ * element belongs to its local node ip (ipNodeMap is the identity for all
+ * three, include/master_element/{Hex8,Tet4,Wed6}CVFEM.h). */
+void orc_dual_volume_mixed(int64_t nelem, const int32_t* type,
+                           const int32_t* conn, const double* coords,
+                           int64_t nnode, double* dualvol)
+{
+  memset(dualvol, 0, sizeof(double) * (size_t)nnode);
+  for (int64_t el = 0; el < nelem; ++el) {
+    const int t = type[el];
+    const int npe = t == 0 ? 8 : (t == 1 ? 4 : 6);
+    double c[24], v[8];
+    for (int n = 0; n < npe; ++n)
+      for (int d = 0; d < 3; ++d) c[3 * n + d] = coords[3 * conn[8 * el + n] + d];
+    if (t == 0)
+      orc_hex_scv_volume(c, v);
+    else if (t == 1)
+      orc_tet_scv_volume(c, v);
+    else
+      orc_wed_scv_volume(c, v);
+    for (int ip = 0; ip < npe; ++ip) dualvol[conn[8 * el + ip]] += v[ip];
   }
 }
 

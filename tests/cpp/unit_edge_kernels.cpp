@@ -276,7 +276,25 @@ void test_mdot_calc_edge()
   const std::vector<double> mdot = realm.get_field(NW_E_MASS_FLOW_RATE);
   std::printf("\"mdot_calc_edge\": [");
   for (size_t i = 0; i < mdot.size(); ++i) std::printf("%s%.17g", i ? "," : "", mdot[i]);
-  std::printf("]\n");
+  std::printf("],\n");
+}
+
+// UnitTestGeometryAlg.C:20-100 through GeometryInteriorAlg on the device
+void test_geometry_unit_hex()
+{
+  Hex8Mesh m = fill_hex8_mesh();
+  Realm realm(0);
+  realm.set_mesh(8, m.edges);
+  realm.set_field(NW_F_COORDINATES, m.coords);
+  GeometryInteriorAlg geom(realm, 1, nullptr, m.conn.data());
+  geom.execute();
+  const std::vector<double> dv = realm.get_field(NW_F_DUAL_VOLUME);
+  const std::vector<double> av = realm.get_field(NW_E_AREA_VECTOR);
+  std::printf("\"geometry_unit_hex\": {\"dual_volume\": [");
+  for (size_t i = 0; i < dv.size(); ++i) std::printf("%s%.17g", i ? "," : "", dv[i]);
+  std::printf("], \"area_vector\": [");
+  for (size_t i = 0; i < av.size(); ++i) std::printf("%s%.17g", i ? "," : "", av[i]);
+  std::printf("]}\n");
 }
 
 }  // namespace
@@ -289,6 +307,7 @@ int main()
     test_momentum_advection_diffusion();
     test_scalar_adv_diff();
     test_mdot_calc_edge();
+    test_geometry_unit_hex();
     std::printf("}\n");
   } catch (const std::exception& e) {
     std::fprintf(stderr, "FAILED: %s\n", e.what());

@@ -74,3 +74,15 @@ def test_mdot_calc_edge(out):
     md = np.array(out["mdot_calc_edge"])
     assert md.size == g["num_edges"]
     np.testing.assert_allclose(md, g["mdot"], rtol=0, atol=g["tol"])
+
+
+def test_geometry_unit_hex(out):
+    """UnitTestGeometryAlg.C:20-100 through GeometryInteriorAlg of the
+    adapter: dual volume 0.125 per node, |A|^2 = 0.0625 per edge, and the
+    same area vectors as the host producer."""
+    g = GOLD["geometry_unit_hex"]
+    r = out["geometry_unit_hex"]
+    np.testing.assert_allclose(r["dual_volume"], g["dual_volume"], rtol=0, atol=g["tol"])
+    a = np.array(r["area_vector"]).reshape(-1, 3)
+    assert a.shape[0] == g["num_edges"]
+    np.testing.assert_allclose((a ** 2).sum(1), g["area_mag_sq"], rtol=0, atol=g["tol"])

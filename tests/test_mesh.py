@@ -64,6 +64,63 @@ def test_rigid_motion_and_scaling():
                                6.25 * a @ Q.T, atol=1e-12)
 
 
+_HEX_REF = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                     [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+_WED_REF = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0],
+                     [0, 0, 1], [1, 0, 1], [0, 1, 1]], float)
+
+
+def _affine(rng, ref):
+    A = rng.normal(size=(3, 3))
+    if np.linalg.det(A) < 0:
+        A[:, 0] = -A[:, 0]
+    return ref @ A.T + rng.normal(size=3), np.linalg.det(A)
+
+
+def test_scv_volumes_split_affine_elements_evenly():
+    """Affine maps keep volume ratios, and on the reference elements the
+    median-dual cut gives every node the same share (a third of a triangle
+    per corner, half a prism's height), so every sub-control volume of an
+    affine tetrahedron is V/4, of an affine prism V/6 and of a
+    parallelepiped V/8 (their sub-volumes are planar-faced, where Grandy's
+    formula is exact).  A wrong or swapped point in a sub-volume table, or a
+    misassigned node, breaks the equality."""
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        c = _rand_tet(rng)
+        v = np.dot(np.cross(c[1] - c[0], c[2] - c[0]), c[3] - c[0]) / 6
+        np.testing.assert_allclose(orc.tet_scv_volume(c), v / 4, rtol=1e-12)
+        p, det = _affine(rng, _WED_REF)
+        np.testing.assert_allclose(orc.wed_scv_volume(p), det / 12, rtol=1e-12)
+        h, det = _affine(rng, _HEX_REF)
+        np.testing.assert_allclose(orc.hex_scv_volume(h), det / 8, rtol=1e-12)
+
+
+def test_scv_volumes_follow_their_nodes():
+    """Relabelling the element's vertices by a symmetry of its topology
+    (a rotation of the tetrahedron's vertices, the prism's end swap) permutes
+    the sub-volumes the same way, on a distorted element."""
+    rng = np.random.default_rng(4)
+    c = _rand_tet(rng)
+    perm = [1, 2, 0, 3]  # rotation about vertex 3 keeps the orientation
+    np.testing.assert_allclose(orc.tet_scv_volume(c[perm]),
+                               orc.tet_scv_volume(c)[perm], rtol=1e-12)
+    p = _affine(rng, _WED_REF)[0] + 0.05 * rng.normal(size=(6, 3))
+    rot = [1, 2, 0, 4, 5, 3]
+    np.testing.assert_allclose(orc.wed_scv_volume(p[rot]),
+                               orc.wed_scv_volume(p)[rot], rtol=1e-12)
+
+
+@pytest.mark.parametrize("mix", [(0.6, 0.3), (0.0, 1.0), (0.0, 0.0)])
+def test_mixed_dual_volume_fills_the_box(nw, mix):
+    """Without jitter every face is planar and conforming, so the dual
+    volumes tile the unit box; each is positive."""
+    c, t, conn = nw.mixed_box(6, mix[0], mix[1], 0.0, seed=5)
+    v = orc.dual_volume_mixed(t, conn, c)
+    assert (v > 0).all()
+    assert abs(v.sum() - 1.0) < 1e-13
+
+
 @pytest.mark.parametrize("mix", [(0.6, 0.3), (0.0, 1.0), (0.0, 0.0), (1.0, 0.0)])
 def test_mixed_mesh_geometry(nw, mix):
     c, t, conn = nw.mixed_box(7, mix[0], mix[1], 0.0, seed=5)
