@@ -407,26 +407,24 @@ struct MomentumPhys {
 
 // ------------------------------------------------------------------ kernel
 
-// Walks the slots g, g+G, ... of `row`: for each off-diagonal slot s
-// evaluates the edge, hands the (row, neighbour) block to put(s, off) and
-// adds the edge's (row, row) block and rhs(row) contributions to acc.
+// Walks the slots s0 + g, s0 + g + G, ... < s1 of `row`: for each
+// off-diagonal slot s evaluates the edge, hands the (row, neighbour) block
+// to put(s, off) and adds the edge's (row, row) block and rhs(row)
+// contributions to acc.
 // code_at(s) and col_at(s) read the slot's code and column.  U slots are
 // taken per step: their neighbour and edge gathers are issued together
 // before any of them is evaluated, and they are evaluated and summed in
 // slot order, so U changes only the overlap of the loads.
 template <class P, int G, int U, class Code, class Col, class Put>
-__device__ inline void walk_row(const P& phys,
-                                const int32_t* __restrict__ row_ptr, int row,
-                                int g, Code code_at, Col col_at, Put put,
-                                double* acc)
+__device__ inline void walk_row(const P& phys, int row, int s0, int s1, int g,
+                                Code code_at, Col col_at, Put put, double* acc)
 {
   constexpr int ND = P::NDOF, NB = ND * ND, NV = NB + ND;
   typename P::Node me, ot[U];
   typename P::Edge er[U];
   phys.load(row, me);
-  const int s1 = row_ptr[row + 1];
 #pragma clang loop unroll(disable)
-  for (int s = row_ptr[row] + g; s < s1; s += G * U) {
+  for (int s = s0 + g; s < s1; s += G * U) {
     int c[U];
     for (int u = 0; u < U; ++u) c[u] = s + u * G < s1 ? code_at(s + u * G) : -1;
     for (int u = 0; u < U; ++u) {
@@ -516,12 +514,20 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
   const bool active = lane / G < nr;
   const int s_lo = row_ptr[rw0], s_hi = row_ptr[rw0 + nr];
   const int ns = s_hi - s_lo;
+  // the row's slot range and diagonal slot do not depend on the staged
+  // codes: load them now, so that they arrive with the staging loads
+  int r_s0 = 0, r_s1 = 0, r_diag = 0;
+  if (active) {
+    r_s0 = row_ptr[row];
+    r_s1 = row_ptr[row + 1];
+    r_diag = diag[row];
+  }
   double acc[NV];
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
   if (ns > SCAP) {
     if (active)
       walk_row<P, G, U>(
-        phys, row_ptr, row, g, [&](int s) { return code[s]; },
+        phys, row, r_s0, r_s1, g, [&](int s) { return code[s]; },
         [&](int s) { return cols[s]; },
         [&](int s, const double* off) {
           double* vp = vals + (int64_t)s * NB;
@@ -534,7 +540,7 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
         acc);
     row_butterfly<G, NV>(acc);
     if (active && g == 0)
-      put_diag_rhs<ND>(vals + (int64_t)diag[row] * NB, rhs + (int64_t)row * ND,
+      put_diag_rhs<ND>(vals + (int64_t)r_diag * NB, rhs + (int64_t)row * ND,
                        acc, accumulate);
     return;
   }
@@ -550,7 +556,7 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
   wave_lds_sync();
   if (active)
     walk_row<P, G, U>(
-      phys, row_ptr, row, g, [&](int s) { return code_s[wv][s - s_lo]; },
+      phys, row, r_s0, r_s1, g, [&](int s) { return code_s[wv][s - s_lo]; },
       [&](int s) { return col_s[wv][s - s_lo]; },
       [&](int s, const double* off) {
         double* vp = &val_s[wv][(s - s_lo) * NB];
@@ -559,7 +565,7 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
       acc);
   row_butterfly<G, NV>(acc);
   if (active && g == 0) {
-    double* dp = &val_s[wv][(diag[row] - s_lo) * NB];
+    double* dp = &val_s[wv][(r_diag - s_lo) * NB];
     for (int k = 0; k < NB; ++k) dp[k] = acc[k];
     double* rp = rhs + (int64_t)row * ND;
     if (accumulate) {
