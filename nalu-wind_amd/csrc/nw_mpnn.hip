@@ -346,7 +346,7 @@ __device__ inline uint32_t pack_bf2(float lo, float hi)
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTileNodes = 32;  // nodes per wave tile of the fused layer
+constexpr int kTileNodes = 16;  // nodes per wave tile of the fused layer
 
 // h0 = relu(W_in f + b_in) stored in bfloat16 (one thread per node)
 template <int H>
@@ -376,12 +376,13 @@ __global__ __launch_bounds__(kBlock) void k_input_bf16(
   }
 }
 
-// One hidden layer, fused: for a tile of 32 nodes a wave
+// One hidden layer, fused: for a tile of 16 nodes a wave
 //   1. gathers the messages m_n = sum_s w_s h[col_s] in FP32 (H/8 lanes per
 //      node, each on 8 consecutive features, so a neighbour's 2H-byte row
 //      is read by consecutive lanes) and puts [h_n, bf16(m_n)] in LDS;
-//   2. multiplies the [32 x 2H] tile by [W_self W_nbr]^T (bfloat16, held in
-//      registers in the B-operand layout of v_mfma_f32_16x16x16_bf16) on
+//   2. multiplies the [16 x 2H] tile by [W_self W_nbr]^T (bfloat16, in the
+//      B-operand layout of v_mfma_f32_16x16x16_bf16, staged once per block
+//      in LDS, so that registers leave room for four waves per SIMD) on
 //      the matrix cores, FP32 accumulation starting from the bias;
 //   3. LAST = false: relu, bfloat16, written back through LDS as whole
 //      rows; LAST = true: relu in FP32 and the output head
@@ -406,15 +407,14 @@ __global__ __launch_bounds__(kBlock) void k_layer_bf16(
   constexpr int AS = 2 * H + 8;      // LDS row stride (bf16), padded
   constexpr int WPB = kBlock / kWave;
   __shared__ __attribute__((aligned(16))) uint16_t sA[WPB][TN * AS];
+  __shared__ bf16x4 sW[NT * KS * kWave];
   static_assert((TN * AS * 2) % 16 == 0, "16-byte aligned rows per wave");
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = (int)(threadIdx.x / kWave);
   const int col = lane & 15, grp = lane >> 4;
-  bf16x4 wf[NT][KS];
-#pragma unroll
-  for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) wf[nt][ks] = wfrag[(nt * KS + ks) * kWave + lane];
+  for (int k = (int)threadIdx.x; k < NT * KS * kWave; k += (int)blockDim.x)
+    sW[k] = wfrag[k];
+  __syncthreads();
   float bs[NT], wo1[NT], wo2[NT];
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
@@ -484,8 +484,8 @@ __global__ __launch_bounds__(kBlock) void k_layer_bf16(
         const bf16x4 av = *reinterpret_cast<const bf16x4*>(ar + ks * 16);
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
-          acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wf[nt][ks],
-                                                             acc[nt], 0, 0, 0);
+          acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+            av, sW[(nt * KS + ks) * kWave + lane], acc[nt], 0, 0, 0);
       }
       if (!LAST) {
         // this M tile's rows of A are no longer read: reuse them for the
