@@ -43,6 +43,12 @@ class Hip:
         self.lib.hipFree.argtypes = [C.c_void_p]
         self.lib.hipMemsetAsync.argtypes = [C.c_void_p, C.c_int, C.c_size_t,
                                             C.c_void_p]
+        self.lib.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t,
+                                            C.c_int, C.c_void_p]
+
+    def copy_async(self, dst, src, nbytes, stream):
+        assert self.lib.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), 3,  # D2D
+                                       C.c_void_p(stream)) == 0
 
     def malloc(self, nbytes):
         p = C.c_void_p()
@@ -112,6 +118,27 @@ def time_launches(hip, ctx, fn, steps, warmup):
     total = hip.elapsed(ev[0], ev[-1])
     per = [hip.elapsed(ev[1 + 2 * i], ev[2 + 2 * i]) for i in range(steps)]
     return sum(per) / steps, total
+
+
+def copy_peak(hip, stream, nbytes=1 << 30, reps=5):
+    """Device-to-device copy bandwidth (bytes read + written per second,
+    GB/s) of a 1 GiB buffer on this GPU: the achievable HBM rate that the
+    nominal 8 TB/s overstates."""
+    a, b = hip.malloc(nbytes), hip.malloc(nbytes)
+    try:
+        hip.memset_async(a, 1, nbytes, stream)
+        hip.copy_async(b, a, nbytes, stream)
+        ev = [hip.event() for _ in range(2)]
+        hip.record(ev[0], stream)
+        for _ in range(reps):
+            hip.copy_async(b, a, nbytes, stream)
+        hip.record(ev[1], stream)
+        ms = hip.elapsed(ev[0], ev[1]) / reps
+    finally:
+        hip.sync()
+        hip.free(a)
+        hip.free(b)
+    return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
 def time_cold(hip, ctx, fn, steps, flush_bytes=1 << 29):
@@ -490,13 +517,16 @@ def main():
                         "overwritten: fused zeroSystem)",
                 "wall_s_timed_region": round(wall, 4)}
             traffic = read_traffic("momentum_abl_128_assembly.txt")
+            cpk = copy_peak(hip, s)
             out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1),
                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": round(achieved / HBM_PEAK_GBS, 4),
                                "traffic": traffic,
                                "kernel": "k_assemble<MomentumPhys>",
                                "kernel_avg_ms": round(kern_ms, 4),
-                               "algorithmic_bytes_per_launch": mbytes}
+                               "algorithmic_bytes_per_launch": mbytes,
+                               "copy_peak_measured": round(cpk, 1),
+                               "frac_of_copy_peak": round(achieved / cpk, 4)}
             log("momentum: %.3f ms/step, assembly kernel %.3f ms, %.0f GB/s" %
                 (ms_step, kern_ms, achieved))
         del W

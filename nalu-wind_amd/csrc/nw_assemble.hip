@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
   P phys, const int32_t* __restrict__ row_ptr,
   const int32_t* __restrict__ code, const int32_t* __restrict__ cols,
-  const int32_t* __restrict__ diag, const int2* __restrict__ sched,
+  const int32_t* __restrict__ diag, const SchedChunk* __restrict__ sched,
   int nchunks, double* __restrict__ vals, double* __restrict__ rhs,
   int accumulate)
 {
@@ -505,14 +505,17 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
   const int gw = xcd_block((int)gridDim.x) * WPB + wv;
   const int chunk_id = gw / G;
   if (chunk_id >= nchunks) return;
-  const int2 chunk = sched[chunk_id];
-  const int rw0 = chunk.x + (gw % G) * RPW;
-  const int nr = min(RPW, chunk.y - rw0);
+  const SchedChunk chunk = sched[chunk_id];
+  static_assert(G == 1 || G == 2, "one or two waves per chunk");
+  const int half = gw % G;
+  const int rw0 = chunk.r0 + half * RPW;
+  const int nr = min(RPW, chunk.r1 - rw0);
   if (nr <= 0) return;
   const int row = rw0 + lane / G;
   const int g = lane & (G - 1);
   const bool active = lane / G < nr;
-  const int s_lo = row_ptr[rw0], s_hi = row_ptr[rw0 + nr];
+  const int s_lo = G == 1 ? chunk.s0 : (half ? chunk.smid : chunk.s0);
+  const int s_hi = G == 1 ? chunk.s1 : (half ? chunk.s1 : chunk.smid);
   const int ns = s_hi - s_lo;
   // the row's slot range and diagonal slot do not depend on the staged
   // codes: load them now, so that they arrive with the staging loads
@@ -813,8 +816,16 @@ int build_schedule(Ctx* c)
         ch.swap(mo);
     }
   }
-  if (!g.sched) NW_HIP(hipMalloc(&g.sched, sizeof(int2) * std::max<int64_t>(nch, 1)));
-  NW_HIP(hipMemcpyAsync(g.sched, ch.data(), sizeof(int2) * nch,
+  std::vector<SchedChunk> rec(nch);
+  for (int64_t i = 0; i < nch; ++i) {
+    const int r0 = ch[i].x, r1 = ch[i].y;
+    rec[i] = SchedChunk{r0, r1, g.h_row_ptr[r0],
+                        g.h_row_ptr[std::min(r1, r0 + kRowsPerWave / 2)],
+                        g.h_row_ptr[r1], {0, 0, 0}};
+  }
+  if (!g.sched)
+    NW_HIP(hipMalloc(&g.sched, sizeof(SchedChunk) * std::max<int64_t>(nch, 1)));
+  NW_HIP(hipMemcpyAsync(g.sched, rec.data(), sizeof(SchedChunk) * nch,
                         hipMemcpyHostToDevice, c->stream));
   NW_HIP(hipStreamSynchronize(c->stream));
   g.nchunks = nch;

@@ -77,7 +77,7 @@ constexpr int kGradSlots = 512;
 template <int C, int U>
 __global__ __launch_bounds__(kBlock) void k_nodal_grad(
   const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ code,
-  const int32_t* __restrict__ cols, const int2* __restrict__ sched,
+  const int32_t* __restrict__ cols, const SchedChunk* __restrict__ sched,
   int nchunks, const double* __restrict__ q, const double* __restrict__ areav,
   const double* __restrict__ dualvol, double* __restrict__ grad, int64_t N,
   int64_t E)
@@ -89,8 +89,8 @@ __global__ __launch_bounds__(kBlock) void k_nodal_grad(
   const int wv = (int)(threadIdx.x / kWave);
   const int chunk_id = xcd_block((int)gridDim.x) * WPB + wv;
   if (chunk_id >= nchunks) return;
-  const int2 ch = sched[chunk_id];
-  const int s_lo = row_ptr[ch.x], ns = row_ptr[ch.y] - s_lo;
+  const SchedChunk ch = sched[chunk_id];
+  const int s_lo = ch.s0, ns = ch.s1 - ch.s0;
   const bool staged = ns <= kGradSlots;
   if (staged)
     for (int i = lane; i < ns; i += kWave) {
@@ -98,8 +98,8 @@ __global__ __launch_bounds__(kBlock) void k_nodal_grad(
       col_s[wv][i] = cols[s_lo + i];
     }
   wave_lds_sync();
-  const int n = ch.x + lane;
-  if (n >= ch.y) return;
+  const int n = ch.r0 + lane;
+  if (n >= ch.r1) return;
   double qn[C], acc[3 * C];
   const double inv = 1.0 / dualvol[n];
   for (int j = 0; j < C; ++j) qn[j] = q[j * N + n];

@@ -71,6 +71,14 @@ __device__ inline int xcd_block(int nblocks_padded)
 // off-diagonal slot reached through edge e from its node[side] -> 2e+side.
 __host__ __device__ inline int32_t diag_code(int32_t n) { return -(n + 1); }
 
+// One chunk of a row schedule: rows [r0, r1) and the row pointer at the
+// chunk's start, at its middle row min(r1, r0 + kRowsPerWave / 2) and at its
+// end, so that a wave finds its slot range in the one 32-byte record it
+// reads first instead of in a second, dependent load of the row pointer.
+struct alignas(32) SchedChunk {
+  int32_t r0, r1, s0, smid, s1, pad[3];
+};
+
 // Sliced ELL copy of a CSR structure: slices of 64 consecutive rows (one
 // wave each); entry j of row 64k + l is stored at sp[k] + 64 j + l, the
 // slice's width being its longest row; shorter rows (and the rows past n in
@@ -97,9 +105,10 @@ struct Graph {
   double* sln = nullptr;      // [nnode][ndof]
   std::vector<int32_t> h_row_ptr, h_cols;
   // Row chunks (<= kRowsPerWave consecutive rows each) in the order the
-  // assembly visits them: a Morton order of the chunk centroids, so that
-  // chunks processed together are neighbours in space.
-  int2* sched = nullptr;
+  // row kernels visit them: a locality order (the row order, a sweep or a
+  // Morton order of the chunk centroids), so that chunks processed
+  // together are neighbours in space.
+  SchedChunk* sched = nullptr;
   int64_t nchunks = 0;
   int64_t chunk_slots_p99 = 0;       // slots 99 % of the chunks stay within
   int64_t half_chunk_slots_p99 = 0;  // the same for 32-row half chunks
