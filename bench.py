@@ -433,6 +433,19 @@ def mixed_mesh(hip, n, steps, warmup, log):
         res["amg_levels_rows_nnz"] = ctx.amg_levels()
         res["geometry"] = device_geometry(hip, ctx, W, W.types, min(steps, 5), 1)
         log("mixed geometry: %s" % res["geometry"])
+        # the edge algorithms on the mixed mesh (dual volumes: the device
+        # geometry's)
+        ea = {}
+        for name, fn in (("mdot", lambda: ctx.compute_mdot(1.0, 1.0)),
+                         ("nodal_grad_pressure",
+                          lambda: ctx.nodal_grad(nw.F_PRESSURE, nw.F_DPDX)),
+                         ("nodal_grad_velocity",
+                          lambda: ctx.nodal_grad(nw.F_VELOCITY, nw.F_DUDX))):
+            kms, _ = time_launches(hip, ctx, fn, steps, warmup)
+            ea[name] = {"kernel_avg_ms": round(kms, 4),
+                        "Medges_per_s": round(W.ne / (kms * 1e-3) / 1e6, 1)}
+        res["edge_algorithms"] = ea
+        log("mixed edge algorithms: %s" % ea)
     return res
 
 

@@ -65,16 +65,17 @@ __global__ __launch_bounds__(kBlock) void k_mdot(MdotArgs a,
   mdot[e] = tmdot;
 }
 
-// Slots of a chunk staged in LDS (64 Hex8 rows hold 448); larger chunks
-// read codes and columns from HBM directly.
-constexpr int kGradSlots = 512;
+// Slots of a chunk staged in LDS: 512 (64 Hex8 rows hold 448) or 1024
+// (mixed meshes), chosen from the graph's 99th-percentile chunk; larger
+// chunks read codes and columns from HBM directly.
+constexpr int kGradSlotsSmall = 512, kGradSlotsLarge = 1024;
 
 // grad[(3 j + i) * N + n] = sum_slots sign * (q_ip,j A_e,i) / V_n, summed
 // in slot order.  A wave takes one 64-row chunk of the assembly's locality
 // schedule (XCD-aware block order): the chunk's slot codes and columns are
 // staged in LDS by coalesced loads, and each lane issues the neighbour and
 // edge gathers of U slots before it uses any of them.
-template <int C, int U>
+template <int C, int U, int SCAP>
 __global__ __launch_bounds__(kBlock) void k_nodal_grad(
   const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ code,
   const int32_t* __restrict__ cols, const SchedChunk* __restrict__ sched,
@@ -83,15 +84,15 @@ __global__ __launch_bounds__(kBlock) void k_nodal_grad(
   int64_t E)
 {
   constexpr int WPB = kBlock / kWave;
-  __shared__ int code_s[WPB][kGradSlots];
-  __shared__ int col_s[WPB][kGradSlots];
+  __shared__ int code_s[WPB][SCAP];
+  __shared__ int col_s[WPB][SCAP];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = (int)(threadIdx.x / kWave);
   const int chunk_id = xcd_block((int)gridDim.x) * WPB + wv;
   if (chunk_id >= nchunks) return;
   const SchedChunk ch = sched[chunk_id];
   const int s_lo = ch.s0, ns = ch.s1 - ch.s0;
-  const bool staged = ns <= kGradSlots;
+  const bool staged = ns <= SCAP;
   if (staged)
     for (int i = lane; i < ns; i += kWave) {
       code_s[wv][i] = code[s_lo + i];
@@ -189,14 +190,20 @@ int nw_nodal_grad(nw_ctx* c, int fin, int fout)
   if (g.nchunks == 0) return NW_OK;
   constexpr int64_t wpb = kBlock / kWave;
   const unsigned nb = (unsigned)(((g.nchunks + wpb - 1) / wpb + 7) / 8 * 8);
-  if (C == 1)
-    k_nodal_grad<1, 4><<<nb, kBlock, 0, c->stream>>>(
-      g.row_ptr, g.code, g.cols, g.sched, (int)g.nchunks, q, areav, dualvol,
-      grad, c->nnode, c->nedge);
+  const bool small = g.chunk_slots_p99 <= kGradSlotsSmall;
+#define NW_GRAD(CC, SC)                                                       \
+  k_nodal_grad<CC, 4, SC><<<nb, kBlock, 0, c->stream>>>(                      \
+    g.row_ptr, g.code, g.cols, g.sched, (int)g.nchunks, q, areav, dualvol,    \
+    grad, c->nnode, c->nedge)
+  if (C == 1 && small)
+    NW_GRAD(1, kGradSlotsSmall);
+  else if (C == 1)
+    NW_GRAD(1, kGradSlotsLarge);
+  else if (small)
+    NW_GRAD(3, kGradSlotsSmall);
   else
-    k_nodal_grad<3, 4><<<nb, kBlock, 0, c->stream>>>(
-      g.row_ptr, g.code, g.cols, g.sched, (int)g.nchunks, q, areav, dualvol,
-      grad, c->nnode, c->nedge);
+    NW_GRAD(3, kGradSlotsLarge);
+#undef NW_GRAD
   NW_LAUNCH_CHECK();
   return NW_OK;
 }
