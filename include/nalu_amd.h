@@ -189,6 +189,15 @@ int nw_assemble_continuity(nw_ctx* ctx, const nw_continuity_opts* opts);
 /* MomentumEdgePecletAlg::execute (src/edge_kernels/MomentumEdgePecletAlg.C:
  * 49-102): writes NW_E_PECLET_NUMBER and NW_E_PECLET_FACTOR. */
 int nw_momentum_edge_peclet(nw_ctx* ctx, const nw_peclet* peclet);
+/* MomentumEdgePecletAlg::execute (src/edge_kernels/MomentumEdgePecletAlg.C:
+ * 49-102) and MomentumEdgeSolverAlg::execute (src/edge_kernels/
+ * MomentumEdgeSolverAlg.C:71-313) run as one pass: the Peclet factor of
+ * each edge is computed where the momentum assembly evaluates the edge
+ * (same operations as nw_momentum_edge_peclet, which the solver algorithm
+ * would otherwise read back), and NW_E_PECLET_NUMBER / NW_E_PECLET_FACTOR
+ * are still written.  Also reads NW_F_DENSITY. */
+int nw_assemble_momentum_with_peclet(nw_ctx* ctx, const nw_momentum_opts* opts,
+                                     const nw_peclet* peclet);
 
 /* ---- edge algorithms around the pressure solve ----------------------------
  * MdotEdgeAlg::execute (src/ngp_algorithms/MdotEdgeAlg.C:43-194), interior

@@ -257,21 +257,45 @@ struct ScalarPhys {
   }
 };
 
-struct MomNode {
+// Node and edge records of the momentum edge; with PEC the Peclet factor
+// of the edge is computed in the kernel (MomentumEdgePecletAlg fused in),
+// which needs the density of both ends and the edge's id.
+template <bool PEC>
+struct MomNodeT {
   double x[3], u[3], g[9], visc, wall;
 };
+template <>
+struct MomNodeT<true> {
+  double x[3], u[3], g[9], visc, wall, rho;
+};
+template <bool PEC>
+struct MomEdgeT {
+  double av[3], mdot, pecfac;
+};
+template <>
+struct MomEdgeT<true> {
+  double av[3], mdot;
+  int e;
+};
 
-struct MomentumPhys {
+template <bool PEC>
+struct MomentumPhysT {
   static constexpr int NDOF = 3;
   static constexpr int kLanes = 2, kUnroll = 2, kMinWaves = 2;
   static constexpr bool kNonTemporal = true;
   // 32 rows of 7 slots (Hex8 edge graph); 72 bytes per slot of LDS
   static constexpr int kSlotsSmall = 224, kSlotsLarge = 224;
-  using Node = MomNode;
+  using Node = MomNodeT<PEC>;
+  using Edge = MomEdgeT<PEC>;
   const double *coords, *vel, *dudx, *visc, *areav, *mdotf, *pecfacf, *wall;
   int64_t N, E;
   double alpha, alphaUpw, hoUpwind, relaxFacU, includeDivU;
   int useLimiter;
+  // PEC only: density in, the edge Peclet number and factor out (written
+  // once, by the L end's row)
+  const double* rho;
+  double *pecnum_out, *pecfac_out;
+  PecletDev pec;
 
   __device__ void load(int n, Node& r) const
   {
@@ -282,22 +306,49 @@ struct MomentumPhys {
     for (int k = 0; k < 9; ++k) r.g[k] = dudx[k * N + n];
     r.visc = visc[n];
     r.wall = wall[n];
+    if constexpr (PEC) r.rho = rho[n];
   }
-
-  struct Edge {
-    double av[3], mdot, pecfac;
-  };
 
   __device__ void load_edge(int e, Edge& r) const
   {
     for (int d = 0; d < 3; ++d) r.av[d] = areav[d * E + e];
     r.mdot = mdotf[e];
-    r.pecfac = pecfacf[e];
+    if constexpr (PEC)
+      r.e = e;
+    else
+      r.pecfac = pecfacf[e];
+  }
+
+  // MomentumEdgePecletAlg::execute (src/edge_kernels/MomentumEdgePecletAlg.C:
+  // 74-101), the same operations in the same order as k_momentum_peclet
+  __device__ double peclet_factor(const Edge& er, int side, const Node& L,
+                                  const Node& R) const
+  {
+    if constexpr (PEC) {
+      const double eps = 1.0e-16;
+      const double diffIp = 0.5 * (L.visc / L.rho + R.visc / R.rho);
+      double udotx = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double dxj = R.x[d] - L.x[d];
+        udotx += 0.5 * dxj * (R.u[d] + L.u[d]);
+      }
+      const double pn = fabs(udotx) / (diffIp + eps);
+      const double pf = pec(pn);
+      if (side == 0) {
+        pecnum_out[er.e] = pn;
+        pecfac_out[er.e] = pf;
+      }
+      return pf;
+    } else {
+      return er.pecfac;
+    }
   }
 
   __device__ void edge(const Edge& er, int side, const Node& L, const Node& R,
                        double* off, double* acc) const
   {
+    // first, so that the densities it reads are dead before the blocks
+    const double pecfac = peclet_factor(er, side, L, R);
     const double eps = 1.0e-16;
     const double om_alpha = 1.0 - alpha, om_alphaUpw = 1.0 - alphaUpw;
     const double* av = er.av;
@@ -329,7 +380,6 @@ struct MomentumPhys {
         limitR[d] = van_leer(4.0 * duR[d] - du, du, eps);
       }
     }
-    const double pecfac = er.pecfac;
     const double om_pecfac = 1.0 - pecfac;
     double uIpL[3], uIpR[3];
     for (int d = 0; d < 3; ++d) {
@@ -425,13 +475,17 @@ __device__ inline void walk_row(const P& phys, int row, int s0, int s1, int g,
   phys.load(row, me);
 #pragma clang loop unroll(disable)
   for (int s = s0 + g; s < s1; s += G * U) {
+    // the u loops are unrolled so that ot[] and er[] stay in registers
     int c[U];
+#pragma unroll
     for (int u = 0; u < U; ++u) c[u] = s + u * G < s1 ? code_at(s + u * G) : -1;
+#pragma unroll
     for (int u = 0; u < U; ++u) {
       if (c[u] < 0) continue;  // a diagonal slot (written by the caller)
       phys.load(col_at(s + u * G), ot[u]);
       phys.load_edge(c[u] >> 1, er[u]);
     }
+#pragma unroll
     for (int u = 0; u < U; ++u) {
       if (c[u] < 0) continue;
       double off[NB], con[NV];
@@ -943,9 +997,36 @@ int nw_assemble_momentum(nw_ctx* c, const nw_momentum_opts* o)
   NW_GET(NW_E_AREA_VECTOR, areav);
   NW_GET(NW_E_MASS_FLOW_RATE, mdot);
   NW_GET(NW_E_PECLET_FACTOR, pecfac);
-  MomentumPhys P{coords, vel, dudx, visc, areav, mdot, pecfac, wall,
-                 c->nnode, c->nedge, o->alpha, o->alpha_upw, o->upw, o->relax,
-                 o->include_divU, o->use_limiter};
+  MomentumPhysT<false> P{coords, vel, dudx, visc, areav, mdot, pecfac, wall,
+                         c->nnode, c->nedge, o->alpha, o->alpha_upw, o->upw,
+                         o->relax, o->include_divU, o->use_limiter,
+                         nullptr, nullptr, nullptr, PecletDev{}};
+  return launch_rows(c, P, o->accumulate);
+}
+
+int nw_assemble_momentum_with_peclet(nw_ctx* c, const nw_momentum_opts* o,
+                                     const nw_peclet* pf)
+{
+  NW_REQUIRE(c && o && pf, NW_E_ARG, "null argument");
+  NW_REQUIRE(c->g.ndof == 3, NW_E_STATE, "momentum needs a 3-dof graph");
+  NW_GET(NW_F_COORDINATES, coords);
+  NW_GET(NW_F_VELOCITY, vel);
+  NW_GET(NW_F_DUDX, dudx);
+  NW_GET(NW_F_VISCOSITY, visc);
+  NW_GET(NW_F_WALL_MASK, wall);
+  NW_GET(NW_F_DENSITY, rho);
+  NW_GET(NW_E_AREA_VECTOR, areav);
+  NW_GET(NW_E_MASS_FLOW_RATE, mdot);
+  double *pn = nullptr, *pfac = nullptr;
+  int rc = ensure_field(c, NW_E_PECLET_NUMBER, &pn);
+  if (rc) return rc;
+  rc = ensure_field(c, NW_E_PECLET_FACTOR, &pfac);
+  if (rc) return rc;
+  MomentumPhysT<true> P{coords, vel, dudx, visc, areav, mdot, nullptr, wall,
+                        c->nnode, c->nedge, o->alpha, o->alpha_upw, o->upw,
+                        o->relax, o->include_divU, o->use_limiter,
+                        rho, pn, pfac,
+                        PecletDev{pf->form, pf->hybrid, pf->c1, pf->c2}};
   return launch_rows(c, P, o->accumulate);
 }
 

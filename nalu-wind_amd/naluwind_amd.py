@@ -166,6 +166,8 @@ _sig = {
     "nw_assemble_momentum": ([_P, C.POINTER(MomentumOpts)], C.c_int),
     "nw_assemble_continuity": ([_P, C.POINTER(ContinuityOpts)], C.c_int),
     "nw_momentum_edge_peclet": ([_P, C.POINTER(Peclet)], C.c_int),
+    "nw_assemble_momentum_with_peclet": ([_P, C.POINTER(MomentumOpts),
+                                          C.POINTER(Peclet)], C.c_int),
     "nw_compute_mdot": ([_P, C.POINTER(MdotOpts)], C.c_int),
     "nw_nodal_grad": ([_P, C.c_int, C.c_int], C.c_int),
     "nw_mesh_elements": ([_P, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
@@ -451,6 +453,11 @@ class Context:
 
     def momentum_edge_peclet(self, peclet):
         check(lib.nw_momentum_edge_peclet(self.h, C.byref(peclet)))
+
+    def assemble_momentum_with_peclet(self, opts, peclet):
+        """MomentumEdgePecletAlg + MomentumEdgeSolverAlg in one pass."""
+        check(lib.nw_assemble_momentum_with_peclet(self.h, C.byref(opts),
+                                                   C.byref(peclet)))
 
     def compute_mdot(self, noc=1.0, interp_together=1.0):
         """MdotEdgeAlg: NW_E_MASS_FLOW_RATE from the node fields."""

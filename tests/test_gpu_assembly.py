@@ -287,6 +287,41 @@ def test_momentum_peclet_matches_oracle(nw, ctx, skewed):
         np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-15)
 
 
+@pytest.mark.parametrize("pf_args", [(0, 1.0, 2.0, 4.0), (1, 0.0, 5.0, 3.0)])
+@pytest.mark.parametrize("problem", ["skewed", "mixed"])
+def test_momentum_with_peclet_equals_the_two_algorithms(nw, ctx, skewed, mixed,
+                                                         pf_args, problem):
+    """The fused pass (nw_assemble_momentum_with_peclet) gives the system and
+    the edge Peclet number and factor of MomentumEdgePecletAlg followed by
+    MomentumEdgeSolverAlg, and both match the oracle."""
+    P = skewed if problem == "skewed" else mixed
+    pf = nw.Peclet(*pf_args)
+    o = nw.momentum_defaults(use_limiter=1)
+    P.upload(ctx)
+    ctx.build_graph(3)
+    ctx.momentum_edge_peclet(pf)
+    ctx.assemble_momentum(o)
+    two = ctx.download()
+    pn2 = ctx.field_download(nw.E_PECLET_NUMBER)
+    pf2 = ctx.field_download(nw.E_PECLET_FACTOR)
+    ctx.field_fill(nw.E_PECLET_NUMBER, -1.0)
+    ctx.field_fill(nw.E_PECLET_FACTOR, -1.0)
+    ctx.assemble_momentum_with_peclet(o, pf)
+    one = ctx.download()
+    pn1 = ctx.field_download(nw.E_PECLET_NUMBER)
+    pf1 = ctx.field_download(nw.E_PECLET_FACTOR)
+    np.testing.assert_allclose(pn1, pn2, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(pf1, pf2, rtol=1e-15, atol=1e-300)
+    assert np.array_equal(one[0], two[0]) and np.array_equal(one[1], two[1])
+    scale = np.abs(two[2]).max()
+    np.testing.assert_allclose(one[2], two[2], rtol=0, atol=1e-14 * scale)
+    np.testing.assert_allclose(one[3], two[3], rtol=0,
+                               atol=1e-14 * np.abs(two[3]).max())
+    ref = orc.momentum_edge_peclet(P.edges, P.coords, P.vel, P.rho, P.visc,
+                                   orc.Peclet(pf.form, pf.hybrid, pf.c1, pf.c2))
+    np.testing.assert_allclose(pf1, ref, rtol=1e-13, atol=1e-15)
+
+
 # ------------------------------------------------------------ row operations
 
 def test_sum_into_reset_and_dirichlet_match_oracle(nw, ctx, skewed):
