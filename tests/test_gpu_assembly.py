@@ -412,3 +412,49 @@ def test_schedule_order_does_not_change_results(nw, monkeypatch):
                 assert np.array_equal(a, b), order
     pb.assert_csr_close(ref[0], P.oracle_continuity(pb.to_orc_continuity(oc)))
     pb.assert_csr_close(ref[2], P.oracle_momentum(pb.to_orc_momentum(om)))
+
+
+def test_edge_order_and_orientation_do_not_change_the_system(nw, skewed):
+    """STK hands edges in its own order, node[0] not always the smaller id;
+    the area vector and the mass flow rate follow the edge's orientation.
+    A permuted edge list with half the edges flipped (their area vectors
+    and mass flow rates negated) gives the same three systems."""
+    import types
+    P = skewed
+    rng = np.random.default_rng(2)
+    perm = rng.permutation(P.ne)
+    flip = rng.random(P.ne) < 0.5
+    sgn = np.where(flip, -1.0, 1.0)
+    e2 = P.edges[perm].copy()
+    e2[flip] = e2[flip][:, ::-1]
+    node_fields = [(nw.F_COORDINATES, P.coords), (nw.F_VELOCITY, P.vel),
+                   (nw.F_DENSITY, P.rho), (nw.F_PRESSURE, P.p), (nw.F_DPDX, P.dpdx),
+                   (nw.F_UDIAG, P.udiag), (nw.F_VISCOSITY, P.visc),
+                   (nw.F_DUDX, P.dudx), (nw.F_WALL_MASK, P.wall),
+                   (nw.F_SCALAR_Q, P.q), (nw.F_DQDX, P.dqdx),
+                   (nw.F_DIFF_FLUX_COEFF, P.dflux)]
+
+    def run(edges, areav, mdot, pecfac, ndof, fn):
+        c = nw.Context(0)
+        try:
+            c.mesh_set(P.n, edges)
+            for fid, a in node_fields:
+                c.field_upload(fid, a)
+            c.field_upload(nw.E_AREA_VECTOR, areav)
+            c.field_upload(nw.E_MASS_FLOW_RATE, mdot)
+            c.field_upload(nw.E_PECLET_FACTOR, pecfac)
+            c.build_graph(ndof)
+            fn(c)
+            return c.download()
+        finally:
+            c.close()
+
+    cases = [(1, lambda c: c.assemble_continuity(nw.continuity_defaults())),
+             (1, lambda c: c.assemble_scalar(nw.scalar_defaults(use_limiter=1))),
+             (3, lambda c: c.assemble_momentum(nw.momentum_defaults(use_limiter=1)))]
+    for ndof, fn in cases:
+        a = run(P.edges, P.areav, P.mdot, P.pecfac, ndof, fn)
+        b = run(e2, P.areav[perm] * sgn[:, None], P.mdot[perm] * sgn,
+                P.pecfac[perm], ndof, fn)
+        ref = types.SimpleNamespace(row_ptr=a[0], cols=a[1], vals=a[2], rhs=a[3])
+        pb.assert_csr_close(b, ref, rtol=1e-13)
