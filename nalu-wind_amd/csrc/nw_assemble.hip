@@ -644,15 +644,18 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
 }
 
 // MomentumEdgePecletAlg::execute (src/edge_kernels/MomentumEdgePecletAlg.C:
-// 74-101): one edge per lane, edge-ordered (coalesced) writes.
+// 74-101): one edge per lane, in the row schedule's edge order when the
+// graph is built (else in edge order), blocks in XCD-aware order.
 __global__ __launch_bounds__(kBlock) void k_momentum_peclet(
   int64_t nedge, int64_t N, const int2* __restrict__ enodes,
-  const double* __restrict__ coords, const double* __restrict__ vel,
-  const double* __restrict__ rho, const double* __restrict__ visc,
-  PecletDev pec, double* __restrict__ pecnum, double* __restrict__ pecfac)
+  const int32_t* __restrict__ order, const double* __restrict__ coords,
+  const double* __restrict__ vel, const double* __restrict__ rho,
+  const double* __restrict__ visc, PecletDev pec, double* __restrict__ pecnum,
+  double* __restrict__ pecfac)
 {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nedge) return;
+  const int64_t i = (int64_t)xcd_block((int)gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= nedge) return;
+  const int64_t e = order ? (int64_t)order[i] : i;
   const double eps = 1.0e-16;
   const int2 lr = enodes[e];
   const int nL = lr.x, nR = lr.y;
@@ -870,6 +873,29 @@ int build_schedule(Ctx* c)
         ch.swap(mo);
     }
   }
+  // the edges in the same order: by chunk, row, then the row's edges as
+  // node[0] (each edge once), for the edge-parallel kernels
+  {
+    const int64_t E = c->nedge;
+    const int32_t* ed = c->h_edges.data();
+    std::vector<int32_t> start(N + 1, 0), idx(std::max<int64_t>(E, 1));
+    for (int64_t e = 0; e < E; ++e) start[ed[2 * e] + 1]++;
+    for (int64_t n = 0; n < N; ++n) start[n + 1] += start[n];
+    {
+      std::vector<int32_t> pos(start.begin(), start.end() - 1);
+      for (int64_t e = 0; e < E; ++e) idx[pos[ed[2 * e]]++] = (int32_t)e;
+    }
+    std::vector<int32_t> order;
+    order.reserve(E);
+    for (int64_t i = 0; i < nch; ++i)
+      for (int r = ch[i].x; r < ch[i].y; ++r)
+        for (int32_t k = start[r]; k < start[r + 1]; ++k) order.push_back(idx[k]);
+    if (!g.edge_order)
+      NW_HIP(hipMalloc(&g.edge_order, sizeof(int32_t) * std::max<int64_t>(E, 1)));
+    NW_HIP(hipMemcpyAsync(g.edge_order, order.data(), sizeof(int32_t) * E,
+                          hipMemcpyHostToDevice, c->stream));
+    NW_HIP(hipStreamSynchronize(c->stream));
+  }
   std::vector<SchedChunk> rec(nch);
   for (int64_t i = 0; i < nch; ++i) {
     const int r0 = ch[i].x, r1 = ch[i].y;
@@ -1044,9 +1070,18 @@ int nw_momentum_edge_peclet(nw_ctx* c, const nw_peclet* pf)
   rc = ensure_field(c, NW_E_PECLET_FACTOR, &pfac);
   if (rc) return rc;
   if (c->nedge == 0) return NW_OK;
-  k_momentum_peclet<<<grid_for(c->nedge, kBlock), kBlock, 0, c->stream>>>(
-    c->nedge, c->nnode, reinterpret_cast<const int2*>(c->edges), coords, vel,
-    rho, visc, PecletDev{pf->form, pf->hybrid, pf->c1, pf->c2}, pn, pfac);
+  // with the edge graph built, the edges go in its schedule's order and
+  // each XCD takes one contiguous stretch of it (as the row kernels), so
+  // the node data of neighbouring edges is shared in one L2
+  const int32_t* order = nullptr;
+  if (c->g.row_ptr && c->g.nnode == c->nnode) {
+    if ((rc = ensure_schedule(c))) return rc;
+    order = c->g.edge_order;
+  }
+  const unsigned nb = (grid_for(c->nedge, kBlock) + 7) / 8 * 8;
+  k_momentum_peclet<<<nb, kBlock, 0, c->stream>>>(
+    c->nedge, c->nnode, reinterpret_cast<const int2*>(c->edges), order, coords,
+    vel, rho, visc, PecletDev{pf->form, pf->hybrid, pf->c1, pf->c2}, pn, pfac);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }

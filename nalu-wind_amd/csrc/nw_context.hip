@@ -82,6 +82,7 @@ static void free_graph(Graph& g)
   (void)hipFree(g.rhs);
   (void)hipFree(g.sln);
   (void)hipFree(g.sched);
+  (void)hipFree(g.edge_order);
   (void)hipFree(g.sell.sp);
   (void)hipFree(g.sell.ci);
   (void)hipFree(g.sell.slot);

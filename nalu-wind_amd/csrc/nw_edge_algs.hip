@@ -32,10 +32,12 @@ struct MdotArgs {
 
 __global__ __launch_bounds__(kBlock) void k_mdot(MdotArgs a,
                                                  const int2* __restrict__ edges,
+                                                 const int32_t* __restrict__ order,
                                                  double* __restrict__ mdot)
 {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.E) return;
+  const int64_t i = (int64_t)xcd_block((int)gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= a.E) return;
+  const int64_t e = order ? (int64_t)order[i] : i;
   const int2 lr = edges[e];
   const int64_t L = lr.x, R = lr.y, N = a.N, E = a.E;
   const double om_interp = 1.0 - a.interp;
@@ -159,8 +161,18 @@ int nw_compute_mdot(nw_ctx* c, const nw_mdot_opts* o)
   if (c->nedge == 0) return NW_OK;
   const MdotArgs a{coords, vel, dpdx, rho, p, udiag, areav, c->nnode, c->nedge,
                    o->noc, o->interp_together};
-  k_mdot<<<grid_for(c->nedge, kBlock), kBlock, 0, c->stream>>>(
-    a, reinterpret_cast<const int2*>(c->edges), mdot);
+  // with the edge graph built, the edges go in its schedule's order (rows
+  // of a chunk are neighbours in space, so the two ends' node data is
+  // shared by nearby lanes); otherwise in their own order
+  const int32_t* order = nullptr;
+  if (c->g.row_ptr && c->g.nnode == c->nnode) {
+    if ((rc = ensure_schedule(c))) return rc;
+    order = c->g.edge_order;
+  }
+  // XCD-aware block order: each XCD takes one contiguous stretch
+  const unsigned nb = (grid_for(c->nedge, kBlock) + 7) / 8 * 8;
+  k_mdot<<<nb, kBlock, 0, c->stream>>>(
+    a, reinterpret_cast<const int2*>(c->edges), order, mdot);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }

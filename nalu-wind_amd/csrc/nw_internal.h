@@ -109,6 +109,9 @@ struct Graph {
   // Morton order of the chunk centroids), so that chunks processed
   // together are neighbours in space.
   SchedChunk* sched = nullptr;
+  // [nedge] the edges in the schedule's order: for each chunk, each row,
+  // the edges whose node[0] the row is (edge-parallel kernels)
+  int32_t* edge_order = nullptr;
   int64_t nchunks = 0;
   int64_t chunk_slots_p99 = 0;       // slots 99 % of the chunks stay within
   int64_t half_chunk_slots_p99 = 0;  // the same for 32-row half chunks

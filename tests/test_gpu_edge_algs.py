@@ -149,3 +149,37 @@ def test_mdot_reference_gold(nw, ctx):
     md = ctx.field_download(nw.E_MASS_FLOW_RATE)
     assert md.size == g["num_edges"]
     np.testing.assert_allclose(md, g["mdot"], rtol=0, atol=g["tol"])
+
+
+def test_mdot_in_schedule_order_equals_edge_order(nw):
+    """With the edge graph built, MdotEdgeAlg visits the edges in the row
+    schedule's order; every edge is still computed once, with the same
+    operations, so the result equals the edge-order pass bit for bit (on a
+    permuted, half-flipped edge list as STK hands them)."""
+    P = pb.Problem(12, 10, 11, stretch=1.08, jitter=0.2, seed=11)
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(P.ne)
+    flip = rng.random(P.ne) < 0.5
+    e2 = P.edges[perm].copy()
+    e2[flip] = e2[flip][:, ::-1]
+    a2 = P.areav[perm] * np.where(flip, -1.0, 1.0)[:, None]
+    out = []
+    for graph in (False, True):
+        c = nw.Context(0)
+        try:
+            c.mesh_set(P.n, e2)
+            for fid, a in [(nw.F_COORDINATES, P.coords), (nw.F_VELOCITY, P.vel),
+                           (nw.F_DPDX, P.dpdx), (nw.F_DENSITY, P.rho),
+                           (nw.F_PRESSURE, P.p), (nw.F_UDIAG, P.udiag),
+                           (nw.E_AREA_VECTOR, a2)]:
+                c.field_upload(fid, a)
+            if graph:
+                c.build_graph(1)
+            c.compute_mdot(1.0, 1.0)
+            out.append(c.field_download(nw.E_MASS_FLOW_RATE))
+        finally:
+            c.close()
+    assert np.array_equal(out[0], out[1])
+    ref = orc.mdot_edge_alg(e2, P.coords, P.vel, P.dpdx, P.rho, P.p, P.udiag, a2,
+                            1.0, 1.0)
+    np.testing.assert_allclose(out[1], ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
