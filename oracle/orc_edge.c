@@ -65,6 +65,8 @@ void orc_momentum_edge_peclet(int64_t nedge, const int32_t* edges,
 {
   /* src/edge_kernels/MomentumEdgePecletAlg.C:74-101, eps_ = 1e-16 (.h:40) */
   const double eps = 1.0e-16;
+  /* independent edges: over orc_set_threads threads, like the assembly */
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
   for (int64_t e = 0; e < nedge; ++e) {
     const int32_t L = edges[2 * e], R = edges[2 * e + 1];
     const double diffIp = 0.5 * (visc[L] / rho[L] + visc[R] / rho[R]);
