@@ -37,10 +37,11 @@ enum {
 /* ---- node and edge fields ------------------------------------------------
  * Upload/download arrays are [n][ncomp] doubles (components of a node
  * adjacent, as in STK field data), n = nnode for node fields and nedge for
- * edge fields.  On the device the library keeps every field component-major
- * ([ncomp][n], SoA) so that lanes reading consecutive nodes or edges read
- * consecutive addresses; nw_field_device_ptr exposes that layout.  Field
- * names follow the reference's registered field names. */
+ * edge fields.  The device copy has the same layout, so a kernel gathering
+ * a neighbour's vector or gradient reads one contiguous record, and lanes on
+ * consecutive nodes or edges still read consecutive addresses;
+ * nw_field_device_ptr exposes that layout.  Field names follow the
+ * reference's registered field names. */
 enum {
   NW_F_COORDINATES = 0, /* 3  "coordinates" */
   NW_F_VELOCITY,        /* 3  "velocity" (== velocity_rtm on static mesh) */
@@ -119,8 +120,8 @@ int nw_mesh_set(nw_ctx* ctx, int64_t nnode, int64_t nedge,
 int nw_field_upload(nw_ctx* ctx, int field, const double* host, int64_t count);
 int nw_field_download(nw_ctx* ctx, int field, double* host, int64_t count);
 int nw_field_fill(nw_ctx* ctx, int field, double value);
-/* Device pointer of the field (allocated on first use), component-major
- * [ncomp][n]. */
+/* Device pointer of the field (allocated on first use), [n][ncomp] like
+ * the host arrays. */
 int nw_field_device_ptr(nw_ctx* ctx, int field, double** dptr);
 int nw_field_components(int field);
 

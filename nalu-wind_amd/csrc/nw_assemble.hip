@@ -5,7 +5,9 @@
 // graph).  Every slot carries a code: the diagonal slot of row n is
 // -(n+1); the slot (n, m) reached through edge e = (L, R) is 2e + side with
 // side = 0 when n == L and 1 when n == R.  Node and edge fields are stored
-// component-major (SoA): component d of node n is f[d * nnode + n].
+// node-major per field, the caller's own layout: component d of node n is
+// f[ncomp * n + d], so a neighbour's vector or gradient is one contiguous
+// record (a few vector loads) and consecutive nodes are still consecutive.
 //
 // Kernel shape ("row owner").  G lanes own one row (G = 1 for the 1-dof
 // systems, 2 for momentum), and the lanes of an instruction sit on
@@ -95,9 +97,9 @@ struct ContinuityPhys {
   __device__ void load(int n, Node& r) const
   {
     for (int d = 0; d < 3; ++d) {
-      r.x[d] = coords[d * N + n];
-      r.u[d] = vel[d * N + n];
-      r.g[d] = dpdx[d * N + n];
+      r.x[d] = coords[3 * (int64_t)n + d];
+      r.u[d] = vel[3 * (int64_t)n + d];
+      r.g[d] = dpdx[3 * (int64_t)n + d];
     }
     r.rho = rho[n];
     r.p = p[n];
@@ -110,7 +112,7 @@ struct ContinuityPhys {
 
   __device__ void load_edge(int e, Edge& r) const
   {
-    for (int d = 0; d < 3; ++d) r.av[d] = areav[d * E + e];
+    for (int d = 0; d < 3; ++d) r.av[d] = areav[3 * (int64_t)e + d];
   }
 
   __device__ void edge(const Edge& er, int side, const Node& L, const Node& R,
@@ -168,9 +170,9 @@ struct ScalarPhys {
   __device__ void load(int n, Node& r) const
   {
     for (int d = 0; d < 3; ++d) {
-      r.x[d] = coords[d * N + n];
-      r.u[d] = vrtm[d * N + n];
-      r.g[d] = dqdx[d * N + n];
+      r.x[d] = coords[3 * (int64_t)n + d];
+      r.u[d] = vrtm[3 * (int64_t)n + d];
+      r.g[d] = dqdx[3 * (int64_t)n + d];
     }
     r.q = q[n];
     r.rho = rho[n];
@@ -183,7 +185,7 @@ struct ScalarPhys {
 
   __device__ void load_edge(int e, Edge& r) const
   {
-    for (int d = 0; d < 3; ++d) r.av[d] = areav[d * E + e];
+    for (int d = 0; d < 3; ++d) r.av[d] = areav[3 * (int64_t)e + d];
     r.mdot = mdotf[e];
   }
 
@@ -300,10 +302,10 @@ struct MomentumPhysT {
   __device__ void load(int n, Node& r) const
   {
     for (int d = 0; d < 3; ++d) {
-      r.x[d] = coords[d * N + n];
-      r.u[d] = vel[d * N + n];
+      r.x[d] = coords[3 * (int64_t)n + d];
+      r.u[d] = vel[3 * (int64_t)n + d];
     }
-    for (int k = 0; k < 9; ++k) r.g[k] = dudx[k * N + n];
+    for (int k = 0; k < 9; ++k) r.g[k] = dudx[9 * (int64_t)n + k];
     r.visc = visc[n];
     r.wall = wall[n];
     if constexpr (PEC) r.rho = rho[n];
@@ -311,7 +313,7 @@ struct MomentumPhysT {
 
   __device__ void load_edge(int e, Edge& r) const
   {
-    for (int d = 0; d < 3; ++d) r.av[d] = areav[d * E + e];
+    for (int d = 0; d < 3; ++d) r.av[d] = areav[3 * (int64_t)e + d];
     r.mdot = mdotf[e];
     if constexpr (PEC)
       r.e = e;
@@ -662,8 +664,8 @@ __global__ __launch_bounds__(kBlock) void k_momentum_peclet(
   const double diffIp = 0.5 * (visc[nL] / rho[nL] + visc[nR] / rho[nR]);
   double udotx = 0.0;
   for (int d = 0; d < 3; ++d) {
-    const double dxj = coords[d * N + nR] - coords[d * N + nL];
-    udotx += 0.5 * dxj * (vel[d * N + nR] + vel[d * N + nL]);
+    const double dxj = coords[3 * (int64_t)nR + d] - coords[3 * (int64_t)nL + d];
+    udotx += 0.5 * dxj * (vel[3 * (int64_t)nR + d] + vel[3 * (int64_t)nL + d]);
   }
   const double pn = fabs(udotx) / (diffIp + eps);
   pecnum[e] = pn;
@@ -693,15 +695,18 @@ ChunkGeom chunk_geometry(const std::vector<double>& x, int64_t N,
   ChunkGeom g;
   const int64_t nch = (int64_t)ch.size();
   for (int d = 0; d < 3; ++d) {
-    const auto mm = std::minmax_element(x.begin() + d * N, x.begin() + (d + 1) * N);
-    g.lo[d] = *mm.first;
-    g.hi[d] = *mm.second;
+    g.lo[d] = x[d];
+    g.hi[d] = x[d];
+    for (int64_t n = 1; n < N; ++n) {
+      g.lo[d] = std::min(g.lo[d], x[3 * n + d]);
+      g.hi[d] = std::max(g.hi[d], x[3 * n + d]);
+    }
     g.cen[d].resize(nch);
     g.ext[d].resize(nch);
     for (int64_t i = 0; i < nch; ++i) {
-      double m = 0.0, a = x[d * N + ch[i].x], b = a;
+      double m = 0.0, a = x[3 * (int64_t)ch[i].x + d], b = a;
       for (int r = ch[i].x; r < ch[i].y; ++r) {
-        const double v = x[d * N + r];
+        const double v = x[3 * (int64_t)r + d];
         m += v;
         a = std::min(a, v);
         b = std::max(b, v);
@@ -853,7 +858,7 @@ int build_schedule(Ctx* c)
   const char* env = getenv("NW_ROW_ORDER");
   const std::string force = env ? std::string(env) : std::string();
   if (force != "natural" && c->field[NW_F_COORDINATES] && nch > 1) {
-    std::vector<double> x(3 * N);  // SoA: x[d * N + n]
+    std::vector<double> x(3 * N);  // x[3 n + d]
     NW_HIP(hipMemcpyAsync(x.data(), c->field[NW_F_COORDINATES],
                           sizeof(double) * 3 * N, hipMemcpyDeviceToHost,
                           c->stream));

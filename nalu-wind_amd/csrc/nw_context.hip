@@ -91,27 +91,6 @@ static void free_graph(Graph& g)
 
 // ------------------------------------------------------------- kernels
 
-// [n][nc] (components of a node adjacent) <-> [nc][n] (component-major)
-__global__ void k_aos_to_soa(const double* __restrict__ a, double* __restrict__ s,
-                             int64_t n, int nc)
-{
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * nc) return;
-  const int64_t i = t / nc;
-  const int d = (int)(t % nc);
-  s[d * n + i] = a[t];
-}
-
-__global__ void k_soa_to_aos(const double* __restrict__ s, double* __restrict__ a,
-                             int64_t n, int nc)
-{
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * nc) return;
-  const int64_t i = t / nc;
-  const int d = (int)(t % nc);
-  a[t] = s[d * n + i];
-}
-
 __global__ void k_fill(double* __restrict__ p, double v, int64_t n)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -128,7 +107,7 @@ __global__ void k_reset_rows(const int32_t* __restrict__ nodes, int64_t nn,
                              double* __restrict__ vals,
                              double* __restrict__ rhs, double diag_value,
                              double rhs_value, const double* __restrict__ bc,
-                             const double* __restrict__ sol, int64_t nnode)
+                             const double* __restrict__ sol, int fnc)
 {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nd = end - begin;
@@ -142,7 +121,7 @@ __global__ void k_reset_rows(const int32_t* __restrict__ nodes, int64_t nn,
       vals[(int64_t)s * nb + d * ndof + c] = (self && c == d) ? diag_value : 0.0;
   }
   rhs[(int64_t)n * ndof + d] =
-    bc ? (bc[d * nnode + n] - sol[d * nnode + n]) : rhs_value;
+    bc ? (bc[(int64_t)n * fnc + d] - sol[(int64_t)n * fnc + d]) : rhs_value;
 }
 
 // Segments of sorted (target, value) contributions, each summed in input
@@ -288,21 +267,9 @@ int nw_field_upload(nw_ctx* c, int field, const double* host, int64_t count)
   NW_REQUIRE(count == field_count(c, field), NW_E_ARG,
              "field %d expects %lld doubles, got %lld", field,
              (long long)field_count(c, field), (long long)count);
-  const int nc = field_components(field);
-  if (nc == 1) {
-    NW_HIP(hipMemcpyAsync(d, host, sizeof(double) * count,
-                          hipMemcpyHostToDevice, c->stream));
-  } else {
-    double* tmp = nullptr;
-    NW_HIP(hipMalloc(&tmp, sizeof(double) * count));
-    NW_HIP(hipMemcpyAsync(tmp, host, sizeof(double) * count,
-                          hipMemcpyHostToDevice, c->stream));
-    k_aos_to_soa<<<grid_for(count, 256), 256, 0, c->stream>>>(tmp, d, count / nc, nc);
-    hipError_t le = hipGetLastError();
-    NW_HIP(hipStreamSynchronize(c->stream));
-    (void)hipFree(tmp);
-    NW_REQUIRE(le == hipSuccess, NW_E_HIP, "transpose: %s", hipGetErrorString(le));
-  }
+  // the device keeps the host's node-major layout ([n][ncomp])
+  NW_HIP(hipMemcpyAsync(d, host, sizeof(double) * count, hipMemcpyHostToDevice,
+                        c->stream));
   NW_HIP(hipStreamSynchronize(c->stream));
   if (field == NW_F_COORDINATES) c->g.sched_valid = false;
   return NW_OK;
@@ -317,22 +284,9 @@ int nw_field_download(nw_ctx* c, int field, double* host, int64_t count)
   NW_REQUIRE(count == field_count(c, field), NW_E_ARG,
              "field %d holds %lld doubles, asked %lld", field,
              (long long)field_count(c, field), (long long)count);
-  const int nc = field_components(field);
-  if (nc == 1) {
-    NW_HIP(hipMemcpyAsync(host, d, sizeof(double) * count,
-                          hipMemcpyDeviceToHost, c->stream));
-    NW_HIP(hipStreamSynchronize(c->stream));
-    return NW_OK;
-  }
-  double* tmp = nullptr;
-  NW_HIP(hipMalloc(&tmp, sizeof(double) * count));
-  k_soa_to_aos<<<grid_for(count, 256), 256, 0, c->stream>>>(d, tmp, count / nc, nc);
-  hipError_t le = hipGetLastError();
-  NW_HIP(hipMemcpyAsync(host, tmp, sizeof(double) * count, hipMemcpyDeviceToHost,
+  NW_HIP(hipMemcpyAsync(host, d, sizeof(double) * count, hipMemcpyDeviceToHost,
                         c->stream));
   NW_HIP(hipStreamSynchronize(c->stream));
-  (void)hipFree(tmp);
-  NW_REQUIRE(le == hipSuccess, NW_E_HIP, "transpose: %s", hipGetErrorString(le));
   return NW_OK;
 }
 
@@ -585,7 +539,7 @@ int nw_linsys_sum_into(nw_ctx* c, int nblocks, int nent, const int32_t* nodes,
 static int reset_rows_impl(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
                            int begin, int end, double diag_value,
                            double rhs_value, const double* bc,
-                           const double* sol)
+                           const double* sol, int fnc)
 {
   NW_REQUIRE(c && c->g.ndof, NW_E_STATE, "linear system not built");
   NW_REQUIRE(nnodes >= 0 && (nodes || nnodes == 0), NW_E_ARG, "bad node list");
@@ -602,7 +556,7 @@ static int reset_rows_impl(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
   const int64_t nt = nnodes * (end - begin);
   k_reset_rows<<<grid_for(nt, 256), 256, 0, c->stream>>>(
     d_nodes, nnodes, begin, end, c->g.ndof, c->g.row_ptr, c->g.cols, c->g.vals,
-    c->g.rhs, diag_value, rhs_value, bc, sol, c->g.nnode);
+    c->g.rhs, diag_value, rhs_value, bc, sol, fnc);
   hipError_t le = hipGetLastError();
   NW_HIP(hipStreamSynchronize(c->stream));
   (void)hipFree(d_nodes);
@@ -616,7 +570,7 @@ int nw_linsys_reset_rows(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
                          double rhs_residual)
 {
   return reset_rows_impl(c, nnodes, nodes, begin, end, diag_value,
-                         rhs_residual, nullptr, nullptr);
+                         rhs_residual, nullptr, nullptr, 1);
 }
 
 int nw_linsys_apply_dirichlet(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
@@ -632,7 +586,8 @@ int nw_linsys_apply_dirichlet(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
   if (rc) return rc;
   rc = require_field(c, bc_field, &bc);
   if (rc) return rc;
-  return reset_rows_impl(c, nnodes, nodes, begin, end, 1.0, 0.0, bc, sol);
+  return reset_rows_impl(c, nnodes, nodes, begin, end, 1.0, 0.0, bc, sol,
+                         field_components(sol_field));
 }
 
 int nw_fix_pressure_at_node(nw_ctx* c, int32_t node, double pref)

@@ -4,7 +4,7 @@
 //   k_mdot       MdotEdgeAlg (src/ngp_algorithms/MdotEdgeAlg.C:121-186):
 //                one lane per edge, edge-ordered (coalesced) area-vector
 //                reads and mass-flow-rate writes, the two nodes' fields
-//                gathered from the component-major node arrays;
+//                gathered as node-major records;
 //   k_nodal_grad NodalGradEdgeAlg (src/ngp_algorithms/NodalGradEdgeAlg.C:
 //                58-111): one lane per row of the edge graph, rows taken in
 //                the assembly's locality-ordered 64-row chunks; the lane
@@ -39,17 +39,17 @@ __global__ __launch_bounds__(kBlock) void k_mdot(MdotArgs a,
   if (i >= a.E) return;
   const int64_t e = order ? (int64_t)order[i] : i;
   const int2 lr = edges[e];
-  const int64_t L = lr.x, R = lr.y, N = a.N, E = a.E;
+  const int64_t L = lr.x, R = lr.y;
   const double om_interp = 1.0 - a.interp;
   double av[3];
-  for (int d = 0; d < 3; ++d) av[d] = a.areav[d * E + e];
+  for (int d = 0; d < 3; ++d) av[d] = a.areav[3 * e + d];
   const double udL = a.udiag[L], udR = a.udiag[R];
   const double rhoL = a.rho[L], rhoR = a.rho[R];
   const double projTimeScale = 0.5 * (1.0 / udL + 1.0 / udR);
   const double rhoIp = 0.5 * (rhoL + rhoR);
   double dx[3], axdx = 0.0, asq = 0.0;
   for (int d = 0; d < 3; ++d) {
-    dx[d] = a.coords[d * N + R] - a.coords[d * N + L];
+    dx[d] = a.coords[3 * R + d] - a.coords[3 * L + d];
     asq += av[d] * av[d];
     axdx += av[d] * dx[d];
   }
@@ -57,10 +57,10 @@ __global__ __launch_bounds__(kBlock) void k_mdot(MdotArgs a,
   double tmdot = -projTimeScale * (a.p[R] - a.p[L]) * asq * inv_axdx;
   for (int d = 0; d < 3; ++d) {
     const double kxj = av[d] - asq * inv_axdx * dx[d];
-    const double uR = a.vel[d * N + R], uL = a.vel[d * N + L];
+    const double uR = a.vel[3 * R + d], uL = a.vel[3 * L + d];
     const double rhoUjIp = 0.5 * (rhoR * uR + rhoL * uL);
     const double ujIp = 0.5 * (uR + uL);
-    const double GjIp = 0.5 * (a.dpdx[d * N + R] / udR + a.dpdx[d * N + L] / udL);
+    const double GjIp = 0.5 * (a.dpdx[3 * R + d] / udR + a.dpdx[3 * L + d] / udL);
     tmdot += (a.interp * rhoUjIp + om_interp * rhoIp * ujIp + GjIp) * av[d] -
              kxj * GjIp * a.noc;
   }
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_mdot(MdotArgs a,
 // chunks read codes and columns from HBM directly.
 constexpr int kGradSlotsSmall = 512, kGradSlotsLarge = 1024;
 
-// grad[(3 j + i) * N + n] = sum_slots sign * (q_ip,j A_e,i) / V_n, summed
+// grad[3C n + 3 j + i] = sum_slots sign * (q_ip,j A_e,i) / V_n, summed
 // in slot order.  A wave takes one 64-row chunk of the assembly's locality
 // schedule (XCD-aware block order): the chunk's slot codes and columns are
 // staged in LDS by coalesced loads, and each lane issues the neighbour and
@@ -86,8 +86,10 @@ __global__ __launch_bounds__(kBlock) void k_nodal_grad(
   int64_t E)
 {
   constexpr int WPB = kBlock / kWave;
+  constexpr int NG = 3 * C;  // gradient components per node
   __shared__ int code_s[WPB][SCAP];
   __shared__ int col_s[WPB][SCAP];
+  __shared__ double out_s[WPB][kWave * NG];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = (int)(threadIdx.x / kWave);
   const int chunk_id = xcd_block((int)gridDim.x) * WPB + wv;
@@ -102,13 +104,13 @@ __global__ __launch_bounds__(kBlock) void k_nodal_grad(
     }
   wave_lds_sync();
   const int n = ch.r0 + lane;
-  if (n >= ch.r1) return;
-  double qn[C], acc[3 * C];
-  const double inv = 1.0 / dualvol[n];
-  for (int j = 0; j < C; ++j) qn[j] = q[j * N + n];
-  for (int k = 0; k < 3 * C; ++k) acc[k] = 0.0;
-  const int s1 = row_ptr[n + 1];
-  for (int s = row_ptr[n]; s < s1; s += U) {
+  const bool active = n < ch.r1;
+  double qn[C], acc[NG];
+  for (int k = 0; k < NG; ++k) acc[k] = 0.0;
+  const double inv = active ? 1.0 / dualvol[n] : 0.0;
+  for (int j = 0; j < C; ++j) qn[j] = active ? q[C * (int64_t)n + j] : 0.0;
+  const int s1 = active ? row_ptr[n + 1] : 0;
+  for (int s = active ? row_ptr[n] : 0; s < s1; s += U) {
     int cu[U], mu[U];
     for (int u = 0; u < U; ++u) {
       const int t = s + u;
@@ -119,8 +121,8 @@ __global__ __launch_bounds__(kBlock) void k_nodal_grad(
     for (int u = 0; u < U; ++u) {
       if (cu[u] < 0) continue;  // the diagonal slot, or past the row
       const int64_t e = cu[u] >> 1;
-      for (int i = 0; i < 3; ++i) av[u][i] = areav[i * E + e];
-      for (int j = 0; j < C; ++j) qm[u][j] = q[j * N + mu[u]];
+      for (int i = 0; i < 3; ++i) av[u][i] = areav[3 * e + i];
+      for (int j = 0; j < C; ++j) qm[u][j] = q[C * (int64_t)mu[u] + j];
     }
     for (int u = 0; u < U; ++u) {
       if (cu[u] < 0) continue;
@@ -131,7 +133,15 @@ __global__ __launch_bounds__(kBlock) void k_nodal_grad(
       }
     }
   }
-  for (int k = 0; k < 3 * C; ++k) grad[k * N + n] = acc[k];
+  // the wave's rows are consecutive nodes: stage their gradients in LDS and
+  // write them as one contiguous range (a lane's own NG values would be
+  // strided stores)
+  if (active)
+    for (int k = 0; k < NG; ++k) out_s[wv][NG * lane + k] = acc[k];
+  wave_lds_sync();
+  const int nv = (ch.r1 - ch.r0) * NG;
+  double* dst = grad + (int64_t)ch.r0 * NG;
+  for (int i = lane; i < nv; i += kWave) dst[i] = out_s[wv][i];
 }
 
 }  // namespace

@@ -172,7 +172,7 @@ __device__ inline void corners(int64_t el, const int32_t* __restrict__ conn,
 #pragma unroll
   for (int a = 0; a < T::kNodes; ++a) {
     const int64_t n = conn[8 * el + a];
-    for (int d = 0; d < 3; ++d) c[a][d] = coords[d * N + n];
+    for (int d = 0; d < 3; ++d) c[a][d] = coords[3 * n + d];
   }
 }
 
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_elem_scv(
     elem_scv<WedTopo>(el, nelem, conn, coords, N, scv);
 }
 
-// areav[d * E + e] = sum over the edge's surfaces (element order) of
+// areav[3 e + d] = sum over the edge's surfaces (element order) of
 // +-scs[(3 s + d) * nelem + el]
 __global__ __launch_bounds__(kBlock) void k_edge_area(
   int64_t E, int64_t nelem, const int32_t* __restrict__ e_ptr,
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_edge_area(
     const double sgn = (c & 1) ? -1.0 : 1.0;
     for (int d = 0; d < 3; ++d) a[d] += scs[(3 * s + d) * nelem + el] * sgn;
   }
-  for (int d = 0; d < 3; ++d) areav[d * E + e] = a[d];
+  for (int d = 0; d < 3; ++d) areav[3 * e + d] = a[d];
 }
 
 // dualvol[n] = sum over the node's elements (element order) of its SCV

@@ -23,7 +23,7 @@ __global__ void k_fourier_velocity(const double* __restrict__ x,
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double x0 = x[i], x1 = x[n + i], x2 = x[2 * n + i];
+  const double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
   double u0 = 0.0, u1 = 0.0, u2 = 0.0;
   for (int q = 0; q < m.n; ++q) {
     const double s = sin(m.k[q][0] * x0 + m.k[q][1] * x1 + m.k[q][2] * x2 + m.ph[q]);
@@ -31,9 +31,9 @@ __global__ void k_fourier_velocity(const double* __restrict__ x,
     u1 += s * m.b[q][1];
     u2 += s * m.b[q][2];
   }
-  u[i] = u0;
-  u[n + i] = u1;
-  u[2 * n + i] = u2;
+  u[3 * i] = u0;
+  u[3 * i + 1] = u1;
+  u[3 * i + 2] = u2;
 }
 
 }  // namespace
