@@ -116,9 +116,12 @@ def assert_csr_close(got, A, rtol=1e-12):
     np.testing.assert_array_equal(rp, A.row_ptr)
     np.testing.assert_array_equal(cols, A.cols)
     ov = A.vals
-    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    cnt = np.diff(rp)
+    rows = np.repeat(np.arange(len(rp) - 1), cnt)
     rowmax = np.zeros(len(rp) - 1)
-    np.maximum.at(rowmax, rows, np.abs(ov))
+    full = cnt > 0
+    if full.any():
+        rowmax[full] = np.maximum.reduceat(np.abs(ov), rp[:-1][full])
     scale = np.maximum(np.abs(ov), rowmax[rows])
     err = np.abs(vals - ov)
     bad = err > rtol * scale + 1e-300

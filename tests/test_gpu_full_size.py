@@ -1,0 +1,140 @@
+"""The benchmark configurations at their full sizes, through the C ABI:
+
+  momentum  128^3-cell box stretched 1.05 toward z = 0, +-10 % jitter
+            (2,146,689 nodes, 6,390,144 edges), hybrid factor 1, alpha 0.5
+  continuity 256^3-cell box with the same z grading (16,974,593 nodes,
+            50,725,632 edges), Dirichlet p = 0 on the top face
+
+Each system is compared with the oracle over every entry (row-scaled 1e-12,
+the bar of the small-mesh tests) and checked against properties that hold
+at any size and do not depend on the oracle:
+
+  continuity  the LHS is a symmetric graph Laplacian: A = A^T, every row and
+              column sums to zero; the fluxes are antisymmetric: sum(rhs) = 0
+  momentum    the scheme is conservative (relaxation 1): every column of the
+              LHS sums to zero (an edge adds -c to one row and +c to the
+              other in the same column), and sum(rhs) = 0 per component
+
+The oracle runs on up to 16 host threads (its atomic scatter only reorders
+the sums); each test takes 5-15 s on a 16-core host, most of it host mesh
+generation and the oracle."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import orc
+from tests import problems as pb
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def nw():
+    return pb.nw()
+
+
+@pytest.fixture(scope="module")
+def wl(nw):
+    import workloads
+    return workloads
+
+
+@pytest.fixture
+def host_threads():
+    orc.set_threads(min(16, os.cpu_count() or 1))
+    yield
+    orc.set_threads(1)
+
+
+def _sparse(got, n):
+    rp, cols, vals, _ = got
+    return sp.csr_matrix((vals, cols, rp), shape=(n, n))
+
+
+def _conservation(S, rhs, ncomp, rows=True, cols=True):
+    """Row and column sums of S within 1e-13 of the row / column sums of
+    |S|; sum(rhs) per component within 1e-12 of sum(|rhs|)."""
+    A = abs(S)
+    if rows:
+        rs = np.asarray(S.sum(1)).ravel()
+        ra = np.asarray(A.sum(1)).ravel()
+        assert (np.abs(rs) <= 1e-13 * ra).all(), np.abs(rs / ra).max()
+    if cols:
+        cs = np.asarray(S.sum(0)).ravel()
+        ca = np.asarray(A.sum(0)).ravel()
+        assert (np.abs(cs) <= 1e-13 * ca).all(), np.abs(cs / ca).max()
+    r = rhs.reshape(-1, ncomp)
+    assert (np.abs(r.sum(0)) <= 1e-12 * np.abs(r).sum(0)).all(), r.sum(0)
+
+
+@pytest.mark.timeout(600)
+def test_continuity_256_cubed_full_size(nw, wl, host_threads):
+    W = wl.continuity_rf(256)
+    assert (W.n, W.ne) == (16974593, 50725632)
+    f = W.fields
+    A = orc.CsrSystem(W.n, W.edges, 1)
+    orc.assemble_continuity(pb.to_orc_continuity(W.opts), W.n, W.edges,
+                            W.coords, f[nw.F_VELOCITY], f[nw.F_DPDX],
+                            f[nw.F_DENSITY], f[nw.F_PRESSURE], f[nw.F_UDIAG],
+                            W.areav, csr=A)
+    with nw.Context(0) as ctx:
+        W.upload(ctx)
+        ctx.build_graph(1)
+        ctx.assemble_continuity(W.opts)
+        got = ctx.download()
+        print("continuity 256^3: %d rows, %d entries" % (W.n, got[1].size), flush=True)
+        pb.assert_csr_close(got, A)
+        S = _sparse(got, W.n)
+        D = S - S.T
+        assert D.nnz == 0 or abs(D).max() <= 1e-14 * abs(S).max()
+        _conservation(S, got[3], 1)
+        del S, D, got
+        # the Dirichlet rows of the pressure solve
+        wl.apply_pressure_bc(ctx, W)
+        A.apply_dirichlet(W.top, 0, 1, np.zeros(W.n), np.zeros(W.n))
+        pb.assert_csr_close(ctx.download(), A)
+
+
+@pytest.mark.timeout(600)
+def test_momentum_128_cubed_full_size(nw, wl, host_threads):
+    W = wl.momentum_abl(128)
+    assert (W.n, W.ne) == (2146689, 6390144)
+    f = W.fields
+    pf = W.peclet
+    ref_pf = orc.momentum_edge_peclet(W.edges, W.coords, f[nw.F_VELOCITY],
+                                      f[nw.F_DENSITY], f[nw.F_VISCOSITY],
+                                      orc.Peclet(pf.form, pf.hybrid, pf.c1, pf.c2))
+    with nw.Context(0) as ctx:
+        W.upload(ctx)
+        ctx.build_graph(3)
+        ctx.momentum_edge_peclet(pf)
+        pec = ctx.field_download(nw.E_PECLET_FACTOR)
+        np.testing.assert_allclose(pec, ref_pf, rtol=1e-13, atol=1e-15)
+        # the blend is exercised: the factor spans (0, 1)
+        assert pec.min() < 0.1 and pec.max() > 0.9
+        A = orc.CsrSystem(W.n, W.edges, 3)
+        orc.assemble_momentum(pb.to_orc_momentum(W.opts), W.n, W.edges,
+                              W.coords, f[nw.F_VELOCITY], f[nw.F_DUDX],
+                              f[nw.F_VISCOSITY], f[nw.F_DENSITY], W.areav,
+                              f[nw.E_MASS_FLOW_RATE], ref_pf, f[nw.F_WALL_MASK],
+                              csr=A)
+        ctx.assemble_momentum(W.opts)
+        two = ctx.download()
+        print("momentum 128^3: %d rows, %d entries" % (3 * W.n, two[1].size), flush=True)
+        pb.assert_csr_close(two, A)
+        del A
+        assert W.opts.relax == 1.0
+        _conservation(_sparse(two, 3 * W.n), two[3], 3, rows=False)
+        # the one-pass form (Peclet factor in the assembly kernel) gives the
+        # same system
+        ctx.assemble_momentum_with_peclet(W.opts, pf)
+        one = ctx.download()
+        assert np.array_equal(one[0], two[0]) and np.array_equal(one[1], two[1])
+        np.testing.assert_allclose(one[2], two[2], rtol=0,
+                                   atol=1e-14 * np.abs(two[2]).max())
+        np.testing.assert_allclose(one[3], two[3], rtol=0,
+                                   atol=1e-14 * np.abs(two[3]).max())
+        np.testing.assert_allclose(ctx.field_download(nw.E_PECLET_FACTOR), pec,
+                                   rtol=1e-15, atol=1e-300)
