@@ -15,6 +15,9 @@ at any size and do not depend on the oracle:
               LHS sums to zero (an edge adds -c to one row and +c to the
               other in the same column), and sum(rhs) = 0 per component
 
+The pressure solves of the 256^3 system are checked by their residual,
+recomputed on the host from the downloaded system.
+
 The oracle runs on up to 16 host threads (its atomic scatter only reorders
 the sums); each test takes 5-15 s on a 16-core host, most of it host mesh
 generation and the oracle."""
@@ -138,3 +141,52 @@ def test_momentum_128_cubed_full_size(nw, wl, host_threads):
                                    atol=1e-14 * np.abs(two[3]).max())
         np.testing.assert_allclose(ctx.field_download(nw.E_PECLET_FACTOR), pec,
                                    rtol=1e-15, atol=1e-300)
+
+
+@pytest.mark.timeout(600)
+def test_pressure_solve_256_cubed_full_size(nw, wl):
+    """The pressure solves of the benchmark at 256^3 (GMRES(50) + SA-AMG, tol
+    1e-8): FP64 from x0 = 0 (the reference's method), the FP32-inner /
+    FP64-refined form from x0 = 0 and from the network's guess (untrained:
+    the quadratic extrapolation of the three previous solutions).  Every
+    solution's residual is recomputed here in FP64 from the downloaded
+    system, and the solutions agree to the accuracy the tolerance implies."""
+    with nw.Context(0) as ctx:
+        S = wl.PressureSeries(ctx, 256, seed=1234, dt_flow=0.02)
+        base = dict(kspace=50, max_iters=2000, tol=1e-8, precond=nw.PREC_AMG,
+                    reuse_preconditioner=1)
+        fp64 = lambda x0: nw.solver_cfg(mode=nw.SOLVE_FP64, x0=x0, **base)
+        fp32 = lambda x0: nw.solver_cfg(mode=nw.SOLVE_FP32_REFINE, inner_tol=1e-4,
+                                        x0=x0, **base)
+        # steps 0-3 give the guess its history of solutions
+        for k in range(4):
+            S.step(k)
+            assert ctx.solve(fp64(nw.X0_ZERO if k == 0 else nw.X0_SOLUTION)).converged
+        net = nw.Mpnn(ctx, 64, 3, seed=7)
+        try:
+            x_prev = ctx.solution()
+            S.step(4)
+            rp, cols, vals, b = ctx.download()
+            A = sp.csr_matrix((vals, cols, rp), shape=(S.W.n, S.W.n))
+            bn = np.linalg.norm(b)
+            out = {}
+            for name, cfg in [("learned", fp32(nw.X0_LEARNED)),
+                              ("fp32_refined", fp32(nw.X0_ZERO)),
+                              ("fp64", fp64(nw.X0_ZERO))]:
+                ctx.set_solution(x_prev)
+                ctx.set_learned_guess(net)
+                st = ctx.solve(cfg)
+                x = ctx.solution()
+                rel = np.linalg.norm(b - A @ x) / bn
+                print("256^3 %s: %d iterations, %.1f ms, residual %.3e (host %.3e)" %
+                      (name, st.iters, st.ms, st.final_rel, rel), flush=True)
+                assert st.converged and st.final_rel <= 1e-8 and rel <= 1.0001e-8
+                out[name] = (st, x)
+        finally:
+            net.close()
+        x64 = out["fp64"][1]
+        for name in ("fp32_refined", "learned"):
+            err = np.linalg.norm(out[name][1] - x64) / np.linalg.norm(x64)
+            assert err < 1e-5, (name, err)
+        assert out["learned"][0].guess == nw.X0_LEARNED
+        assert out["learned"][0].iters < out["fp32_refined"][0].iters
