@@ -552,7 +552,7 @@ def main():
                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": round(achieved / HBM_PEAK_GBS, 4),
                                "traffic": traffic,
-                               "kernel": "k_assemble<MomentumPhys>",
+                               "kernel": "k_assemble_persistent<MomentumPhysT<false>>",
                                "kernel_avg_ms": round(kern_ms, 4),
                                "algorithmic_bytes_per_launch": mbytes,
                                "copy_peak_measured": round(cpk, 1),

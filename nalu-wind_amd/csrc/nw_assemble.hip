@@ -82,12 +82,13 @@ struct ContNode {
 // Launch shape of each physics (measured on the 256^3 continuity, 128^3
 // momentum and 160^3 mixed-mesh workloads): kLanes lanes per row (the 1-dof
 // systems choose 1 or 2 from the graph, see launch_rows), kUnroll slots in
-// flight per lane, non-temporal streaming, the LDS slot capacities per wave
-// and the occupancy the register allocation must allow.
+// flight per lane, non-temporal streaming, the LDS slot capacities per wave,
+// the occupancy the register allocation must allow, and whether the
+// persistent kernel (k_assemble_persistent) runs it.
 struct ContinuityPhys {
   static constexpr int NDOF = 1;
   static constexpr int kLanes = 1, kUnroll = 1, kMinWaves = 2;
-  static constexpr bool kNonTemporal = false;
+  static constexpr bool kNonTemporal = false, kPersistent = false;
   static constexpr int kSlotsSmall = 512, kSlotsLarge = 1024;
   using Node = ContNode;
   const double *coords, *vel, *dpdx, *rho, *p, *udiag, *areav;
@@ -158,7 +159,7 @@ struct ScalNode {
 struct ScalarPhys {
   static constexpr int NDOF = 1;
   static constexpr int kLanes = 1, kUnroll = 1, kMinWaves = 2;
-  static constexpr bool kNonTemporal = false;
+  static constexpr bool kNonTemporal = false, kPersistent = false;
   static constexpr int kSlotsSmall = 512, kSlotsLarge = 1024;
   using Node = ScalNode;
   const double *coords, *vrtm, *q, *dqdx, *rho, *dflux, *areav, *mdotf;
@@ -284,7 +285,9 @@ template <bool PEC>
 struct MomentumPhysT {
   static constexpr int NDOF = 3;
   static constexpr int kLanes = 2, kUnroll = 2, kMinWaves = 2;
-  static constexpr bool kNonTemporal = true;
+  // persistent: 0.587 vs 0.643 ms on the 128^3 box; with the Peclet factor
+  // in the kernel (13 dwords of spill) it is slower (0.95 vs 0.76 ms)
+  static constexpr bool kNonTemporal = true, kPersistent = !PEC;
   // 32 rows of 7 slots (Hex8 edge graph); 72 bytes per slot of LDS
   static constexpr int kSlotsSmall = 224, kSlotsLarge = 224;
   using Node = MomNodeT<PEC>;
@@ -526,12 +529,13 @@ __device__ inline void put_diag_rhs(double* __restrict__ dp,
   }
 }
 
-// The assembly kernel.  G lanes own one row and a wave takes 64/G
-// consecutive rows of a scheduled chunk (G waves cover the chunk); lane g of
-// a row walks the row's slots g, g+G, ... (walk_row) and the G partial sums
-// of the diagonal block and rhs are combined by a fixed butterfly.  The
-// lanes of one instruction sit on consecutive rows, so their neighbour and
-// edge gathers go in one stencil direction and stay compact.
+// One task of the assembly: the rows of one wave.  G lanes own one row and
+// a wave takes 64/G consecutive rows of a scheduled chunk (G waves cover the
+// chunk, `half` says which); lane g of a row walks the row's slots g, g+G,
+// ... (walk_row) and the G partial sums of the diagonal block and rhs are
+// combined by a fixed butterfly.  The lanes of one instruction sit on
+// consecutive rows, so their neighbour and edge gathers go in one stencil
+// direction and stay compact.
 //
 // The slots of a wave's rows form one contiguous range [s_lo, s_hi).  The
 // wave loads the range's codes and columns into LDS with coalesced loads,
@@ -540,30 +544,21 @@ __device__ inline void put_diag_rhs(double* __restrict__ dp,
 // no partially written lines linger in L2, where they would evict the node
 // and edge data that neighbouring rows reuse.  A wave whose range holds
 // more than SCAP slots writes straight to HBM instead.  NT marks the
-// streamed code/column reads and value writes non-temporal.
+// streamed code/column reads and value writes non-temporal.  The LDS
+// arrays are the wave's own; a following task of the same wave writes its
+// values only after its staging barrier, behind this task's reads.
 template <class P, int G, int U, int SCAP, bool NT>
-__global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
-  P phys, const int32_t* __restrict__ row_ptr,
+__device__ inline void assemble_task(
+  const P& phys, const int32_t* __restrict__ row_ptr,
   const int32_t* __restrict__ code, const int32_t* __restrict__ cols,
-  const int32_t* __restrict__ diag, const SchedChunk* __restrict__ sched,
-  int nchunks, double* __restrict__ vals, double* __restrict__ rhs,
-  int accumulate)
+  const int32_t* __restrict__ diag, const SchedChunk& chunk, int half,
+  double* __restrict__ vals, double* __restrict__ rhs, int accumulate,
+  int* code_s, int* col_s, double* val_s)
 {
   constexpr int ND = P::NDOF, NB = ND * ND, NV = NB + ND;
   constexpr int RPW = kWave / G;
-  constexpr int WPB = kBlock / kWave;
-  __shared__ int code_s[WPB][SCAP];
-  __shared__ int col_s[WPB][SCAP];
-  __shared__ double val_s[WPB][SCAP * NB];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wv = (int)(threadIdx.x / kWave);
-  const int gw = xcd_block((int)gridDim.x) * WPB + wv;
-  const int chunk_id = gw / G;
-  if (chunk_id >= nchunks) return;
-  const SchedChunk chunk = sched[chunk_id];
   static_assert(G == 1 || G == 2, "one or two waves per chunk");
-  const int half = gw % G;
+  const int lane = threadIdx.x & (kWave - 1);
   const int rw0 = chunk.r0 + half * RPW;
   const int nr = min(RPW, chunk.r1 - rw0);
   if (nr <= 0) return;
@@ -605,26 +600,26 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
   }
   for (int i = lane; i < ns; i += kWave) {
     if (NT) {
-      code_s[wv][i] = __builtin_nontemporal_load(code + s_lo + i);
-      col_s[wv][i] = __builtin_nontemporal_load(cols + s_lo + i);
+      code_s[i] = __builtin_nontemporal_load(code + s_lo + i);
+      col_s[i] = __builtin_nontemporal_load(cols + s_lo + i);
     } else {
-      code_s[wv][i] = code[s_lo + i];
-      col_s[wv][i] = cols[s_lo + i];
+      code_s[i] = code[s_lo + i];
+      col_s[i] = cols[s_lo + i];
     }
   }
   wave_lds_sync();
   if (active)
     walk_row<P, G, U>(
-      phys, row, r_s0, r_s1, g, [&](int s) { return code_s[wv][s - s_lo]; },
-      [&](int s) { return col_s[wv][s - s_lo]; },
+      phys, row, r_s0, r_s1, g, [&](int s) { return code_s[s - s_lo]; },
+      [&](int s) { return col_s[s - s_lo]; },
       [&](int s, const double* off) {
-        double* vp = &val_s[wv][(s - s_lo) * NB];
+        double* vp = &val_s[(s - s_lo) * NB];
         for (int k = 0; k < NB; ++k) vp[k] = off[k];
       },
       acc);
   row_butterfly<G, NV>(acc);
   if (active && g == 0) {
-    double* dp = &val_s[wv][(r_diag - s_lo) * NB];
+    double* dp = &val_s[(r_diag - s_lo) * NB];
     for (int k = 0; k < NB; ++k) dp[k] = acc[k];
     double* rp = rhs + (int64_t)row * ND;
     if (accumulate) {
@@ -636,12 +631,84 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
   wave_lds_sync();
   double* dst = vals + (int64_t)s_lo * NB;
   for (int i = lane; i < ns * NB; i += kWave) {
-    double v = val_s[wv][i];
+    double v = val_s[i];
     if (accumulate) v += dst[i];
     if (NT)
       __builtin_nontemporal_store(v, dst + i);
     else
       dst[i] = v;
+  }
+}
+
+// The assembly kernel: one task per wave, G waves per chunk.
+template <class P, int G, int U, int SCAP, bool NT>
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
+  P phys, const int32_t* __restrict__ row_ptr,
+  const int32_t* __restrict__ code, const int32_t* __restrict__ cols,
+  const int32_t* __restrict__ diag, const SchedChunk* __restrict__ sched,
+  int nchunks, double* __restrict__ vals, double* __restrict__ rhs,
+  int accumulate)
+{
+  constexpr int NB = P::NDOF * P::NDOF;
+  constexpr int WPB = kBlock / kWave;
+  __shared__ int code_s[WPB][SCAP];
+  __shared__ int col_s[WPB][SCAP];
+  __shared__ double val_s[WPB][SCAP * NB];
+  const int wv = (int)(threadIdx.x / kWave);
+  const int gw = xcd_block((int)gridDim.x) * WPB + wv;
+  const int chunk_id = gw / G;
+  if (chunk_id >= nchunks) return;
+  const SchedChunk chunk = sched[chunk_id];
+  assemble_task<P, G, U, SCAP, NT>(phys, row_ptr, code, cols, diag, chunk,
+                                   gw % G, vals, rhs, accumulate, code_s[wv],
+                                   col_s[wv], val_s[wv]);
+}
+
+// Persistent form: a grid of resident waves walks the tasks t = w, w + W,
+// ... (W the grid's wave count, w the XCD-ordered wave id, task t = half
+// t % G of chunk t / G).  A wave reads the next task's 32-byte schedule
+// record with a vector load (lanes 0-7, one dword each) issued before the
+// current task's gathers, so it arrives with them: after a wave's first
+// task no task waits for its schedule record.  Every wave leaves the loop
+// when its next task is past the end.
+template <class P, int G, int U, int SCAP, bool NT>
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble_persistent(
+  P phys, const int32_t* __restrict__ row_ptr,
+  const int32_t* __restrict__ code, const int32_t* __restrict__ cols,
+  const int32_t* __restrict__ diag, const SchedChunk* __restrict__ sched,
+  int nchunks, double* __restrict__ vals, double* __restrict__ rhs,
+  int accumulate)
+{
+  constexpr int NB = P::NDOF * P::NDOF;
+  constexpr int WPB = kBlock / kWave;
+  __shared__ int code_s[WPB][SCAP];
+  __shared__ int col_s[WPB][SCAP];
+  __shared__ double val_s[WPB][SCAP * NB];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = (int)(threadIdx.x / kWave);
+  const int nw = (int)gridDim.x * WPB;
+  const int ntasks = nchunks * G;
+  int t = xcd_block((int)gridDim.x) * WPB + wv;
+  if (t >= ntasks) return;
+  const int32_t* rec = reinterpret_cast<const int32_t*>(sched);
+  int cur = lane < 8 ? rec[(int64_t)(t / G) * 8 + lane] : 0;
+  for (;;) {
+    const int tn = t + nw;
+    const int nxt = (tn < ntasks && lane < 8) ? rec[(int64_t)(tn / G) * 8 + lane] : 0;
+    SchedChunk chunk;
+    chunk.r0 = __builtin_amdgcn_readlane(cur, 0);
+    chunk.r1 = __builtin_amdgcn_readlane(cur, 1);
+    chunk.s0 = __builtin_amdgcn_readlane(cur, 2);
+    chunk.smid = __builtin_amdgcn_readlane(cur, 3);
+    chunk.s1 = __builtin_amdgcn_readlane(cur, 4);
+    assemble_task<P, G, U, SCAP, NT>(phys, row_ptr, code, cols, diag, chunk,
+                                     t % G, vals, rhs, accumulate, code_s[wv],
+                                     col_s[wv], val_s[wv]);
+    if (tn >= ntasks) break;
+    t = tn;
+    cur = nxt;
   }
 }
 
@@ -929,6 +996,29 @@ void launch_assemble(Ctx* c, const P& phys, int accumulate)
   const Graph& g = c->g;
   const int64_t wpb = kBlock / kWave;
   int64_t blocks = (g.nchunks * G + wpb - 1) / wpb;
+  if constexpr (P::kPersistent) {
+    // as many blocks as the device holds at once
+    static int resident = 0;
+    if (resident == 0) {
+      int ncu = 0, per_cu = 0;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                c->device) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_assemble_persistent<P, G, P::kUnroll, SCAP, P::kNonTemporal>,
+            kBlock, 0) != hipSuccess)
+        ncu = per_cu = 0;
+      resident = ncu * per_cu > 0 ? ncu * per_cu : -1;
+    }
+    if (resident > 0) {
+      blocks = std::min<int64_t>(blocks, resident);
+      blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
+      k_assemble_persistent<P, G, P::kUnroll, SCAP, P::kNonTemporal>
+        <<<(unsigned)blocks, kBlock, 0, c->stream>>>(
+          phys, g.row_ptr, g.code, g.cols, g.diag, g.sched, (int)g.nchunks,
+          g.vals, g.rhs, accumulate);
+      return;
+    }
+  }
   blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
   k_assemble<P, G, P::kUnroll, SCAP, P::kNonTemporal>
     <<<(unsigned)blocks, kBlock, 0, c->stream>>>(phys, g.row_ptr, g.code, g.cols,
