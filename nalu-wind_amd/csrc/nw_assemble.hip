@@ -285,8 +285,9 @@ template <bool PEC>
 struct MomentumPhysT {
   static constexpr int NDOF = 3;
   static constexpr int kLanes = 2, kUnroll = 2, kMinWaves = 2;
-  // persistent: 0.587 vs 0.643 ms on the 128^3 box; with the Peclet factor
-  // in the kernel (13 dwords of spill) it is slower (0.95 vs 0.76 ms)
+  // persistent: 0.577 vs 0.643 ms on the 128^3 box; with the Peclet factor
+  // in the kernel (18 dwords of spill, 33 persistent) it is slower (0.95 vs
+  // 0.76 ms per step)
   static constexpr bool kNonTemporal = true, kPersistent = !PEC;
   // 32 rows of 7 slots (Hex8 edge graph); 72 bytes per slot of LDS
   static constexpr int kSlotsSmall = 224, kSlotsLarge = 224;
@@ -462,32 +463,39 @@ struct MomentumPhysT {
 
 // ------------------------------------------------------------------ kernel
 
-// Walks the slots s0 + g, s0 + g + G, ... < s1 of `row`: for each
-// off-diagonal slot s evaluates the edge, hands the (row, neighbour) block
-// to put(s, off) and adds the edge's (row, row) block and rhs(row)
-// contributions to acc.
+// Walks the off-diagonal slots of `row` (slots [s0, s1) less the diagonal
+// slot sd): the row's k-th off-diagonal slot goes to lane k % G, so the G
+// lanes of a row share its edges evenly.  For each slot s evaluates the
+// edge, hands the (row, neighbour) block to put(s, off) and adds the edge's
+// (row, row) block and rhs(row) contributions to acc.
 // code_at(s) and col_at(s) read the slot's code and column.  U slots are
 // taken per step: their neighbour and edge gathers are issued together
 // before any of them is evaluated, and they are evaluated and summed in
 // slot order, so U changes only the overlap of the loads.
 template <class P, int G, int U, class Code, class Col, class Put>
-__device__ inline void walk_row(const P& phys, int row, int s0, int s1, int g,
-                                Code code_at, Col col_at, Put put, double* acc)
+__device__ inline void walk_row(const P& phys, int row, int s0, int s1, int sd,
+                                int g, Code code_at, Col col_at, Put put,
+                                double* acc)
 {
   constexpr int ND = P::NDOF, NB = ND * ND, NV = NB + ND;
   typename P::Node me, ot[U];
   typename P::Edge er[U];
   phys.load(row, me);
+  const int noff = s1 - s0 - 1;
 #pragma clang loop unroll(disable)
-  for (int s = s0 + g; s < s1; s += G * U) {
+  for (int k = g; k < noff; k += G * U) {
     // the u loops are unrolled so that ot[] and er[] stay in registers
-    int c[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) c[u] = s + u * G < s1 ? code_at(s + u * G) : -1;
+    int sl[U], c[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (c[u] < 0) continue;  // a diagonal slot (written by the caller)
-      phys.load(col_at(s + u * G), ot[u]);
+      const int kk = k + u * G;
+      sl[u] = s0 + kk + (s0 + kk >= sd ? 1 : 0);
+      c[u] = kk < noff ? code_at(sl[u]) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (c[u] < 0) continue;  // past the row's end
+      phys.load(col_at(sl[u]), ot[u]);
       phys.load_edge(c[u] >> 1, er[u]);
     }
 #pragma unroll
@@ -498,7 +506,7 @@ __device__ inline void walk_row(const P& phys, int row, int s0, int s1, int g,
         phys.edge(er[u], 1, ot[u], me, off, con);
       else
         phys.edge(er[u], 0, me, ot[u], off, con);
-      put(s + u * G, off);
+      put(sl[u], off);
       for (int v = 0; v < NV; ++v) acc[v] += con[v];
     }
   }
@@ -531,9 +539,9 @@ __device__ inline void put_diag_rhs(double* __restrict__ dp,
 
 // One task of the assembly: the rows of one wave.  G lanes own one row and
 // a wave takes 64/G consecutive rows of a scheduled chunk (G waves cover the
-// chunk, `half` says which); lane g of a row walks the row's slots g, g+G,
-// ... (walk_row) and the G partial sums of the diagonal block and rhs are
-// combined by a fixed butterfly.  The lanes of one instruction sit on
+// chunk, `half` says which); lane g of a row walks the row's off-diagonal
+// slots g, g+G, ... (walk_row) and the G partial sums of the diagonal block
+// and rhs are combined by a fixed butterfly.  The lanes of one instruction sit on
 // consecutive rows, so their neighbour and edge gathers go in one stencil
 // direction and stay compact.
 //
@@ -581,7 +589,7 @@ __device__ inline void assemble_task(
   if (ns > SCAP) {
     if (active)
       walk_row<P, G, U>(
-        phys, row, r_s0, r_s1, g, [&](int s) { return code[s]; },
+        phys, row, r_s0, r_s1, r_diag, g, [&](int s) { return code[s]; },
         [&](int s) { return cols[s]; },
         [&](int s, const double* off) {
           double* vp = vals + (int64_t)s * NB;
@@ -610,7 +618,7 @@ __device__ inline void assemble_task(
   wave_lds_sync();
   if (active)
     walk_row<P, G, U>(
-      phys, row, r_s0, r_s1, g, [&](int s) { return code_s[s - s_lo]; },
+      phys, row, r_s0, r_s1, r_diag, g, [&](int s) { return code_s[s - s_lo]; },
       [&](int s) { return col_s[s - s_lo]; },
       [&](int s, const double* off) {
         double* vp = &val_s[(s - s_lo) * NB];
