@@ -675,11 +675,13 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
 
 // Persistent form: a grid of resident waves walks the tasks t = w, w + W,
 // ... (W the grid's wave count, w the XCD-ordered wave id, task t = half
-// t % G of chunk t / G).  A wave reads the next task's 32-byte schedule
-// record with a vector load (lanes 0-7, one dword each) issued before the
-// current task's gathers, so it arrives with them: after a wave's first
-// task no task waits for its schedule record.  Every wave leaves the loop
-// when its next task is past the end.
+// t % G of chunk t / G), so the 8 XCDs work on neighbouring tasks at any
+// time (giving each XCD its own contiguous eighth of the tasks, as
+// k_assemble does, measured 2 % slower).  A wave reads the next task's
+// 32-byte schedule record with a vector load (lanes 0-7, one dword each)
+// issued before the current task's gathers, so it arrives with them: after
+// a wave's first task no task waits for its schedule record.  Every wave
+// leaves the loop when its next task is past the end.
 template <class P, int G, int U, int SCAP, bool NT>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble_persistent(
@@ -1018,8 +1020,10 @@ void launch_assemble(Ctx* c, const P& phys, int accumulate)
       resident = ncu * per_cu > 0 ? ncu * per_cu : -1;
     }
     if (resident > 0) {
-      blocks = std::min<int64_t>(blocks, resident);
-      blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
+      // a multiple of the XCD count, all resident (blocks never wait for
+      // each other, so the grid's size is a matter of speed only)
+      blocks = std::min<int64_t>(blocks, std::max(8, resident / 8 * 8));
+      blocks = (blocks + 7) / 8 * 8;
       k_assemble_persistent<P, G, P::kUnroll, SCAP, P::kNonTemporal>
         <<<(unsigned)blocks, kBlock, 0, c->stream>>>(
           phys, g.row_ptr, g.code, g.cols, g.diag, g.sched, (int)g.nchunks,
