@@ -36,6 +36,7 @@
 //   ScalarEdgeSolverAlg.C:87-205, MomentumEdgeSolverAlg.C:107-312,
 //   ContinuityEdgeSolverAlg.C:111-194 (src/edge_kernels/).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <string>
@@ -1007,8 +1008,10 @@ void launch_assemble(Ctx* c, const P& phys, int accumulate)
   const int64_t wpb = kBlock / kWave;
   int64_t blocks = (g.nchunks * G + wpb - 1) / wpb;
   if constexpr (P::kPersistent) {
-    // as many blocks as the device holds at once
-    static int resident = 0;
+    // as many blocks as the device holds at once (queried once; contexts
+    // on other host threads may ask at the same time)
+    static std::atomic<int> cached{0};
+    int resident = cached.load(std::memory_order_relaxed);
     if (resident == 0) {
       int ncu = 0, per_cu = 0;
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
@@ -1018,6 +1021,7 @@ void launch_assemble(Ctx* c, const P& phys, int accumulate)
             kBlock, 0) != hipSuccess)
         ncu = per_cu = 0;
       resident = ncu * per_cu > 0 ? ncu * per_cu : -1;
+      cached.store(resident, std::memory_order_relaxed);
     }
     if (resident > 0) {
       // a multiple of the XCD count, all resident (blocks never wait for
