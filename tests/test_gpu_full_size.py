@@ -16,10 +16,12 @@ at any size and do not depend on the oracle:
               other in the same column), and sum(rhs) = 0 per component
 
 The pressure solves of the 256^3 system are checked by their residual,
-recomputed on the host from the downloaded system.
+recomputed on the host from the downloaded system; the device geometry,
+NodalGradEdgeAlg and MdotEdgeAlg of the 256^3 box against the oracle, and
+the geometry by the tiling of the box and the closure of the area vectors.
 
 The oracle runs on up to 16 host threads (its atomic scatter only reorders
-the sums); each test takes 5-15 s on a 16-core host, most of it host mesh
+the sums); each test takes 5-30 s on a 16-core host, most of it host mesh
 generation and the oracle."""
 import os
 
@@ -190,3 +192,54 @@ def test_pressure_solve_256_cubed_full_size(nw, wl):
             assert err < 1e-5, (name, err)
         assert out["learned"][0].guess == nw.X0_LEARNED
         assert out["learned"][0].iters < out["fp32_refined"][0].iters
+
+
+@pytest.mark.timeout(900)
+def test_geometry_and_edge_algorithms_256_cubed_full_size(nw, wl, host_threads):
+    """GeometryInteriorAlg, NodalGradEdgeAlg and MdotEdgeAlg on the device
+    for the 256^3 box (16,777,216 Hex8 elements): against the oracle, and
+    the geometry against what holds at any size: the dual volumes tile the
+    unit box and the area vectors around every interior node close."""
+    W = wl.continuity_rf(256, keep_elements=True)
+    edges, elem_edges = nw.hex_edges(W.conn, W.n)
+    assert np.array_equal(edges, W.edges)
+    ra = orc.edge_area_vec_hex(W.conn, W.coords, elem_edges, edges)
+    del elem_edges
+    rv = orc.dual_volume_hex(W.conn, W.coords)
+    f = W.fields
+    x = W.coords
+    p = np.sin(3.0 * x[:, 0]) * np.cos(2.0 * x[:, 1]) + x[:, 2] ** 2
+    rdp = orc.nodal_grad_edge(edges, ra, rv, p)
+    rmd = orc.mdot_edge_alg(edges, x, f[nw.F_VELOCITY], rdp, f[nw.F_DENSITY], p,
+                            f[nw.F_UDIAG], ra)
+    with nw.Context(0) as ctx:
+        ctx.mesh_set(W.n, edges)
+        ctx.field_upload(nw.F_COORDINATES, x)
+        ctx.mesh_elements(None, W.conn)
+        ctx.compute_geometry()
+        a = ctx.field_download(nw.E_AREA_VECTOR)
+        v = ctx.field_download(nw.F_DUAL_VOLUME)
+        print("geometry 256^3: %d elements, %d edges" % (W.conn.shape[0], edges.shape[0]),
+              flush=True)
+        amax = np.abs(ra).max()
+        np.testing.assert_allclose(a, ra, rtol=0, atol=1e-14 * amax)
+        np.testing.assert_allclose(v, rv, rtol=1e-13, atol=0)
+        assert abs(v.sum() - 1.0) <= 1e-12
+        close = np.stack([np.bincount(edges[:, 0], weights=a[:, d], minlength=W.n) -
+                          np.bincount(edges[:, 1], weights=a[:, d], minlength=W.n)
+                          for d in range(3)], 1)
+        lo, hi = x.min(0), x.max(0)
+        inner = np.all((x > lo + 1e-12) & (x < hi - 1e-12), axis=1)
+        assert np.abs(close[inner]).max() <= 1e-12 * amax
+        del close
+        # the edge algorithms on the device geometry
+        for fid in (nw.F_VELOCITY, nw.F_DENSITY, nw.F_UDIAG):
+            ctx.field_upload(fid, f[fid])
+        ctx.field_upload(nw.F_PRESSURE, p)
+        ctx.build_graph(1)
+        ctx.nodal_grad(nw.F_PRESSURE, nw.F_DPDX)
+        dp = ctx.field_download(nw.F_DPDX).reshape(rdp.shape)
+        np.testing.assert_allclose(dp, rdp, rtol=0, atol=1e-12 * np.abs(rdp).max())
+        ctx.compute_mdot(1.0, 1.0)
+        md = ctx.field_download(nw.E_MASS_FLOW_RATE)
+        np.testing.assert_allclose(md, rmd, rtol=1e-12, atol=1e-12 * np.abs(rmd).max())
