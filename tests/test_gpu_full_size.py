@@ -18,7 +18,8 @@ at any size and do not depend on the oracle:
 The pressure solves of the 256^3 system are checked by their residual,
 recomputed on the host from the downloaded system; the device geometry,
 NodalGradEdgeAlg and MdotEdgeAlg of the 256^3 box against the oracle, and
-the geometry by the tiling of the box and the closure of the area vectors.
+the geometry by the tiling of the box and the closure of the area vectors;
+the mixed Hex8/Tet4/Wed6 box's continuity system and device geometry.
 
 The oracle runs on up to 16 host threads (its atomic scatter only reorders
 the sums); each test takes 5-30 s on a 16-core host, most of it host mesh
@@ -243,3 +244,45 @@ def test_geometry_and_edge_algorithms_256_cubed_full_size(nw, wl, host_threads):
         ctx.compute_mdot(1.0, 1.0)
         md = ctx.field_download(nw.E_MASS_FLOW_RATE)
         np.testing.assert_allclose(md, rmd, rtol=1e-12, atol=1e-12 * np.abs(rmd).max())
+
+
+@pytest.mark.timeout(900)
+def test_mixed_160_cubed_full_size(nw, wl, host_threads):
+    """The mixed Hex8/Tet4/Wed6 box of the benchmark (160^3 cells cut 60/30/10
+    %, +-10 % jitter, RCM-numbered: 4,173,281 nodes, 17,969,515 edges): the
+    continuity system against the oracle over every entry, symmetric with
+    zero row sums and sum(rhs) = 0; the device geometry against the oracle
+    (to 1e-13 of the largest area vector: the two sum an edge's
+    sub-control-surface terms in different element orders).  The box's
+    hex-tet interfaces are not conforming (a hex's quad face meets two tet
+    triangles), so its dual volumes do not tile the box exactly and are not
+    checked for it."""
+    W = wl.continuity_mixed(160, keep_elements=True)
+    assert (W.n, W.ne) == (4173281, 17969515)
+    f = W.fields
+    x = W.coords
+    A = orc.CsrSystem(W.n, W.edges, 1)
+    orc.assemble_continuity(pb.to_orc_continuity(W.opts), W.n, W.edges, x,
+                            f[nw.F_VELOCITY], f[nw.F_DPDX], f[nw.F_DENSITY],
+                            f[nw.F_PRESSURE], f[nw.F_UDIAG], W.areav, csr=A)
+    ra = orc.edge_area_vec_mixed(W.types, W.conn, x, W.edges)
+    rv = orc.dual_volume_mixed(W.types, W.conn, x)
+    with nw.Context(0) as ctx:
+        W.upload(ctx)
+        ctx.build_graph(1)
+        ctx.assemble_continuity(W.opts)
+        got = ctx.download()
+        print("mixed 160^3: %d rows, %d entries" % (W.n, got[1].size), flush=True)
+        pb.assert_csr_close(got, A)
+        S = _sparse(got, W.n)
+        D = S - S.T
+        assert D.nnz == 0 or abs(D).max() <= 1e-14 * abs(S).max()
+        _conservation(S, got[3], 1)
+        del S, D, got, A
+        ctx.mesh_elements(W.types, W.conn)
+        ctx.compute_geometry()
+        a = ctx.field_download(nw.E_AREA_VECTOR)
+        v = ctx.field_download(nw.F_DUAL_VOLUME)
+        np.testing.assert_allclose(a, ra, rtol=0, atol=1e-13 * np.abs(ra).max())
+        np.testing.assert_allclose(v, rv, rtol=1e-12, atol=0)
+        assert (v > 0).all()
