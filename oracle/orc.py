@@ -278,13 +278,17 @@ def mdot_edge_alg(edges, coords, vel, dpdx, rho, p, udiag, areav, noc=1.0,
     return out
 
 
-def momentum_edge_peclet(edges, coords, vel, rho, visc, peclet):
+def momentum_edge_peclet(edges, coords, vel, rho, visc, peclet,
+                         with_number=False):
+    """MomentumEdgePecletAlg: the edge Peclet factor, and with with_number
+    the pair (Peclet number, factor)."""
     out = np.zeros(edges.shape[0])
+    num = np.zeros(edges.shape[0]) if with_number else None
     lib.orc_momentum_edge_peclet(edges.shape[0], i32(edges).ravel(),
                                  f64(coords).ravel(), f64(vel).ravel(),
-                                 f64(rho), f64(visc), C.byref(peclet), None,
-                                 out)
-    return out
+                                 f64(rho), f64(visc), C.byref(peclet),
+                                 _ptr(num), out)
+    return (num, out) if with_number else out
 
 
 # ---------------------------------------------------------------- CSR

@@ -317,9 +317,12 @@ def test_momentum_with_peclet_equals_the_two_algorithms(nw, ctx, skewed, mixed,
     np.testing.assert_allclose(one[2], two[2], rtol=0, atol=1e-14 * scale)
     np.testing.assert_allclose(one[3], two[3], rtol=0,
                                atol=1e-14 * np.abs(two[3]).max())
-    ref = orc.momentum_edge_peclet(P.edges, P.coords, P.vel, P.rho, P.visc,
-                                   orc.Peclet(pf.form, pf.hybrid, pf.c1, pf.c2))
+    ref_n, ref = orc.momentum_edge_peclet(P.edges, P.coords, P.vel, P.rho, P.visc,
+                                          orc.Peclet(pf.form, pf.hybrid, pf.c1, pf.c2),
+                                          with_number=True)
     np.testing.assert_allclose(pf1, ref, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(pn1, ref_n, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(pn2, ref_n, rtol=1e-13, atol=0)
 
 
 # ------------------------------------------------------------ row operations

@@ -109,15 +109,18 @@ def test_momentum_128_cubed_full_size(nw, wl, host_threads):
     assert (W.n, W.ne) == (2146689, 6390144)
     f = W.fields
     pf = W.peclet
-    ref_pf = orc.momentum_edge_peclet(W.edges, W.coords, f[nw.F_VELOCITY],
-                                      f[nw.F_DENSITY], f[nw.F_VISCOSITY],
-                                      orc.Peclet(pf.form, pf.hybrid, pf.c1, pf.c2))
+    ref_pn, ref_pf = orc.momentum_edge_peclet(
+        W.edges, W.coords, f[nw.F_VELOCITY], f[nw.F_DENSITY], f[nw.F_VISCOSITY],
+        orc.Peclet(pf.form, pf.hybrid, pf.c1, pf.c2), with_number=True)
     with nw.Context(0) as ctx:
         W.upload(ctx)
         ctx.build_graph(3)
         ctx.momentum_edge_peclet(pf)
         pec = ctx.field_download(nw.E_PECLET_FACTOR)
         np.testing.assert_allclose(pec, ref_pf, rtol=1e-13, atol=1e-15)
+        # the Peclet number itself, over all 6.4 M edges
+        np.testing.assert_allclose(ctx.field_download(nw.E_PECLET_NUMBER), ref_pn,
+                                   rtol=1e-13, atol=0)
         # the blend is exercised: the factor spans (0, 1)
         assert pec.min() < 0.1 and pec.max() > 0.9
         A = orc.CsrSystem(W.n, W.edges, 3)

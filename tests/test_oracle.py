@@ -101,6 +101,96 @@ def test_peclet_golds():
         assert abs(orc.peclet_tanh(g["c1"], g["c2"], x) - fac) < g["tol"]
 
 
+def _box_edges(n, h):
+    coords, conn = fx.hex_box(n, n, n)
+    coords = coords * (n * h)
+    edges, _ = orc.create_edges_hex(conn, coords.shape[0])
+    return coords, edges
+
+
+def test_peclet_number_uniform_box_closed_form():
+    """MomentumEdgePecletAlg.C:74-101 on a uniform box of spacing h with
+    uniform u, rho, mu: every edge is axis-aligned, dx = h e_d, so
+    Pe = |u_d| h / (mu/rho + 1e-16), computed here from the mesh's own
+    edge directions, and the factor is the classic / tanh function of it."""
+    h, n = 0.25, 3
+    coords, edges = _box_edges(n, h)
+    nn = coords.shape[0]
+    u = np.array([2.0, -0.5, 0.125])
+    rho, mu = 1.6, 4.0e-3
+    vel = np.tile(u, (nn, 1))
+    for pf, fn in ((orc.Peclet(0, 0.7, 2.0, 4.0), lambda pe: (0.7 * pe) ** 2 / (5 + (0.7 * pe) ** 2)),
+                   (orc.Peclet(1, 0.0, 2.0, 4.0), lambda pe: 0.5 * (1 + math.tanh((pe - 2.0) / 4.0)))):
+        num, fac = orc.momentum_edge_peclet(edges, coords, vel, np.full(nn, rho),
+                                            np.full(nn, mu), pf, with_number=True)
+        dx = coords[edges[:, 1]] - coords[edges[:, 0]]
+        axis = np.argmax(np.abs(dx), axis=1)
+        assert np.allclose(np.abs(dx[np.arange(len(axis)), axis]), h, rtol=1e-14)
+        sign = np.sign(dx[np.arange(len(axis)), axis])
+        pe = np.abs(u[axis] * sign * h) / (mu / rho + 1e-16)
+        np.testing.assert_allclose(num, pe, rtol=1e-14, atol=0)
+        np.testing.assert_allclose(fac, [fn(p) for p in pe], rtol=1e-14, atol=1e-300)
+    # the three axis values are distinct, so a transposed component shows
+    assert len({round(abs(v) * h / (mu / rho), 9) for v in u}) == 3
+
+
+def test_peclet_number_two_nodes_unequal_properties():
+    """Two nodes with unequal rho and mu: diffIp = (muL/rhoL + muR/rhoR)/2
+    (MomentumEdgePecletAlg.C:85-86), u.dx with the mean of the two
+    velocities (:88-92).  A mean of mu over a mean of rho, or one side's
+    ratio, gives a different number here."""
+    coords = np.array([[0.1, 0.2, 0.3], [0.6, -0.1, 0.45]])
+    edges = np.array([[0, 1]], dtype=np.int32)
+    vel = np.array([[1.0, 2.0, -1.0], [3.0, -2.0, 0.5]])
+    rho = np.array([0.8, 2.5])
+    mu = np.array([3e-3, 1e-2])
+    dx = coords[1] - coords[0]
+    udotx = 0.5 * dx @ (vel[0] + vel[1])
+    diff = 0.5 * (mu[0] / rho[0] + mu[1] / rho[1])
+    pe = abs(udotx) / (diff + 1e-16)
+    assert abs(abs(udotx) / (0.5 * (mu.sum()) / (0.5 * rho.sum())) - pe) > 1e-3 * pe
+    for pf, f in ((orc.Peclet(0, 0.008, 2.0, 4.0), (0.008 * pe) ** 2 / (5 + (0.008 * pe) ** 2)),
+                  (orc.Peclet(1, 0.0, 250.0, 100.0), 0.5 * (1 + math.tanh((pe - 250.0) / 100.0)))):
+        num, fac = orc.momentum_edge_peclet(edges, coords, vel, rho, mu, pf,
+                                            with_number=True)
+        assert num[0] == pytest.approx(pe, rel=1e-14)
+        assert fac[0] == pytest.approx(f, rel=1e-13)
+        assert 0.05 < f < 0.95   # the blend is away from its limits here
+
+
+def test_scalar_peclet_number_through_the_advective_flux():
+    """ScalarEdgeSolverAlg.C:109-110,131-133: the scalar kernel's Peclet
+    number uses diffIp = (GammaL/rhoL + GammaR/rhoR)/2 and the mean velocity
+    along dx.  With dq/dx = 0, alpha = 0, alpha_upw = 1 and mdot > 0 its
+    flux is mdot (f qL + (1-f)(qL+qR)/2) plus the diffusion
+    -viscIp |A|^2/(A.dx) (qR - qL), so rhs(L) pins f = f(Pe) in closed
+    form, and so does lhs(L,L)."""
+    coords = np.array([[0.0, 0.0, 0.0], [0.4, 0.1, -0.05]])
+    edges = np.array([[0, 1]], dtype=np.int32)
+    vel = np.array([[1.5, -0.4, 0.2], [0.7, 0.3, -0.6]])
+    rho = np.array([1.3, 0.6])
+    gam = np.array([2e-2, 5e-3])
+    q = np.array([0.9, 0.2])
+    av = np.array([[0.3, 0.05, 0.02]])
+    mdot = np.array([0.8])
+    dx = coords[1] - coords[0]
+    pe = abs(0.5 * dx @ (vel[0] + vel[1])) / (0.5 * (gam[0] / rho[0] + gam[1] / rho[1]) + 1e-16)
+    for hf in (0.03, 0.08):
+        f = (hf * pe) ** 2 / (5 + (hf * pe) ** 2)
+        assert 0.05 < f < 0.95
+        o = orc.ScalarOpts(0.0, 1.0, 1.0, 1.0, 0, orc.Peclet(0, hf, 2.0, 4.0))
+        L, R = orc.assemble_scalar(o, 2, edges, coords, vel, q, np.zeros((2, 3)),
+                                   rho, gam, av, mdot)
+        a = av[0]
+        lhsfac = -0.5 * gam.sum() * (a @ a) / (a @ dx)
+        diffflux = lhsfac * (q[1] - q[0])
+        adv = mdot[0] * (f * q[0] + (1 - f) * 0.5 * (q[0] + q[1]))
+        assert R[0] == pytest.approx(-diffflux - adv, rel=1e-13)
+        assert R[1] == pytest.approx(diffflux + adv, rel=1e-13)
+        assert L[0, 0] == pytest.approx(-lhsfac + mdot[0] * f + 0.5 * mdot[0] * (1 - f),
+                                        rel=1e-13)
+
+
 def test_van_leer_limits():
     # smooth data (equal slopes) -> 1; opposite slopes -> 0; symmetric
     assert orc.van_leer(1.0, 1.0) == pytest.approx(1.0)
