@@ -61,6 +61,7 @@ enum {
   NW_E_MASS_FLOW_RATE,  /* 1  "mass_flow_rate" (edge field) */
   NW_E_PECLET_FACTOR,   /* 1  "peclet_factor" (edge field) */
   NW_E_PECLET_NUMBER,   /* 1  "peclet_number" (edge field) */
+  NW_F_PRESSURE_BC,     /* 1  "pressure_bc" (Dirichlet values of pressure) */
   NW_NUM_FIELDS
 };
 
@@ -87,6 +88,11 @@ typedef struct {
   double include_divU; /* default 0 */
   int use_limiter;
   int accumulate;
+  /* NGPApplyCoeff::extract_diagonal (src/SolverAlgorithm.C:89-105), on
+   * when tscaleType_ == TSCALE_UDIAGINV (src/LowMachEquationSystem.C:2736):
+   * the (0, 0) entry of each node's summed (node, node) edge block, x-velocity
+   * row, is added to NW_F_UDIAG ("momentum_diag"), one add per node */
+  int extract_diagonal;
 } nw_momentum_opts;
 
 typedef struct {
@@ -162,6 +168,19 @@ int nw_linsys_sum_into(nw_ctx* ctx, int nblocks, int nent,
 int nw_linsys_reset_rows(nw_ctx* ctx, int64_t nnodes, const int32_t* nodes,
                          int begin, int end, double diag_value,
                          double rhs_residual);
+/* HypreLinearSystem::buildDirichletNodeGraph (src/HypreLinearSystem.C:
+ * 720-805) with the skip semantics of its coefficient applier
+ * (HypreLinSysCoeffApplier::sum_into_1DoF, src/HypreLinearSystem.C:
+ * 2165-2239): the listed nodes become skipped rows.  While the skip check
+ * is on -- after nw_linsys_zero, until a nw_linsys_reset_rows call lifts it
+ * (the applier's checkSkippedRows_ flag, :2326, :1392) -- the edge
+ * assemblies and nw_linsys_sum_into add nothing to a skipped row: it keeps
+ * zero entries until nw_linsys_apply_dirichlet writes its diagonal 1 and
+ * rhs = bc - solution (HypreLinearSystem::applyDirichletBCs, :2408-2456).
+ * Columns of skipped nodes in other rows are assembled as usual.  0 nodes
+ * clears the list; a graph rebuild clears it too. Host node list. */
+int nw_linsys_set_skipped_rows(nw_ctx* ctx, int64_t nnodes,
+                               const int32_t* nodes);
 /* applyDirichletBCs (src/TpetraLinearSystem.C:1686-1757): rows -> identity,
  * rhs = bc_field - sol_field for dofs [begin, end). Host node list; the two
  * node fields have the same component count. */

@@ -556,12 +556,19 @@ __device__ inline void put_diag_rhs(double* __restrict__ dp,
 // streamed code/column reads and value writes non-temporal.  The LDS
 // arrays are the wave's own; a following task of the same wave writes its
 // values only after its staging barrier, behind this task's reads.
+//
+// Rows flagged in `skip` (the skipped rows of the hypre backend, may be
+// null) are evaluated but contribute nothing: their entries are written as
+// 0 (or left as they are when accumulating).  When `xdiag` is set, the
+// (0, 0) entry of every row's summed (row, row) block is added to
+// xdiag[row] (NGPApplyCoeff::extract_diagonal), skipped rows included.
 template <class P, int G, int U, int SCAP, bool NT>
 __device__ inline void assemble_task(
   const P& phys, const int32_t* __restrict__ row_ptr,
   const int32_t* __restrict__ code, const int32_t* __restrict__ cols,
   const int32_t* __restrict__ diag, const SchedChunk& chunk, int half,
   double* __restrict__ vals, double* __restrict__ rhs, int accumulate,
+  const uint8_t* __restrict__ skip, double* __restrict__ xdiag,
   int* code_s, int* col_s, double* val_s)
 {
   constexpr int ND = P::NDOF, NB = ND * ND, NV = NB + ND;
@@ -580,10 +587,12 @@ __device__ inline void assemble_task(
   // the row's slot range and diagonal slot do not depend on the staged
   // codes: load them now, so that they arrive with the staging loads
   int r_s0 = 0, r_s1 = 0, r_diag = 0;
+  bool skipped = false;
   if (active) {
     r_s0 = row_ptr[row];
     r_s1 = row_ptr[row + 1];
     r_diag = diag[row];
+    if (skip) skipped = skip[row] != 0;
   }
   double acc[NV];
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
@@ -595,16 +604,21 @@ __device__ inline void assemble_task(
         [&](int s, const double* off) {
           double* vp = vals + (int64_t)s * NB;
           if (accumulate) {
-            for (int k = 0; k < NB; ++k) vp[k] += off[k];
+            if (!skipped)
+              for (int k = 0; k < NB; ++k) vp[k] += off[k];
           } else {
-            for (int k = 0; k < NB; ++k) vp[k] = off[k];
+            for (int k = 0; k < NB; ++k) vp[k] = skipped ? 0.0 : off[k];
           }
         },
         acc);
     row_butterfly<G, NV>(acc);
-    if (active && g == 0)
+    if (active && g == 0) {
+      if (xdiag) xdiag[row] += acc[0];
+      if (skipped)
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
       put_diag_rhs<ND>(vals + (int64_t)r_diag * NB, rhs + (int64_t)row * ND,
                        acc, accumulate);
+    }
     return;
   }
   for (int i = lane; i < ns; i += kWave) {
@@ -623,11 +637,14 @@ __device__ inline void assemble_task(
       [&](int s) { return col_s[s - s_lo]; },
       [&](int s, const double* off) {
         double* vp = &val_s[(s - s_lo) * NB];
-        for (int k = 0; k < NB; ++k) vp[k] = off[k];
+        for (int k = 0; k < NB; ++k) vp[k] = skipped ? 0.0 : off[k];
       },
       acc);
   row_butterfly<G, NV>(acc);
   if (active && g == 0) {
+    if (xdiag) xdiag[row] += acc[0];
+    if (skipped)
+      for (int v = 0; v < NV; ++v) acc[v] = 0.0;
     double* dp = &val_s[(r_diag - s_lo) * NB];
     for (int k = 0; k < NB; ++k) dp[k] = acc[k];
     double* rp = rhs + (int64_t)row * ND;
@@ -657,7 +674,7 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
   const int32_t* __restrict__ code, const int32_t* __restrict__ cols,
   const int32_t* __restrict__ diag, const SchedChunk* __restrict__ sched,
   int nchunks, double* __restrict__ vals, double* __restrict__ rhs,
-  int accumulate)
+  int accumulate, const uint8_t* __restrict__ skip, double* __restrict__ xdiag)
 {
   constexpr int NB = P::NDOF * P::NDOF;
   constexpr int WPB = kBlock / kWave;
@@ -670,8 +687,8 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble(
   if (chunk_id >= nchunks) return;
   const SchedChunk chunk = sched[chunk_id];
   assemble_task<P, G, U, SCAP, NT>(phys, row_ptr, code, cols, diag, chunk,
-                                   gw % G, vals, rhs, accumulate, code_s[wv],
-                                   col_s[wv], val_s[wv]);
+                                   gw % G, vals, rhs, accumulate, skip, xdiag,
+                                   code_s[wv], col_s[wv], val_s[wv]);
 }
 
 // Persistent form: a grid of resident waves walks the tasks t = w, w + W,
@@ -690,7 +707,7 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble_persistent(
   const int32_t* __restrict__ code, const int32_t* __restrict__ cols,
   const int32_t* __restrict__ diag, const SchedChunk* __restrict__ sched,
   int nchunks, double* __restrict__ vals, double* __restrict__ rhs,
-  int accumulate)
+  int accumulate, const uint8_t* __restrict__ skip, double* __restrict__ xdiag)
 {
   constexpr int NB = P::NDOF * P::NDOF;
   constexpr int WPB = kBlock / kWave;
@@ -715,8 +732,8 @@ __attribute__((amdgpu_waves_per_eu(P::kMinWaves))) void k_assemble_persistent(
     chunk.smid = __builtin_amdgcn_readlane(cur, 3);
     chunk.s1 = __builtin_amdgcn_readlane(cur, 4);
     assemble_task<P, G, U, SCAP, NT>(phys, row_ptr, code, cols, diag, chunk,
-                                     t % G, vals, rhs, accumulate, code_s[wv],
-                                     col_s[wv], val_s[wv]);
+                                     t % G, vals, rhs, accumulate, skip, xdiag,
+                                     code_s[wv], col_s[wv], val_s[wv]);
     if (tn >= ntasks) break;
     t = tn;
     cur = nxt;
@@ -1002,9 +1019,10 @@ int build_schedule(Ctx* c)
 }
 
 template <class P, int G, int SCAP>
-void launch_assemble(Ctx* c, const P& phys, int accumulate)
+void launch_assemble(Ctx* c, const P& phys, int accumulate, double* xdiag)
 {
   const Graph& g = c->g;
+  const uint8_t* skip = active_skip_mask(g);
   const int64_t wpb = kBlock / kWave;
   int64_t blocks = (g.nchunks * G + wpb - 1) / wpb;
   if constexpr (P::kPersistent) {
@@ -1031,7 +1049,7 @@ void launch_assemble(Ctx* c, const P& phys, int accumulate)
       k_assemble_persistent<P, G, P::kUnroll, SCAP, P::kNonTemporal>
         <<<(unsigned)blocks, kBlock, 0, c->stream>>>(
           phys, g.row_ptr, g.code, g.cols, g.diag, g.sched, (int)g.nchunks,
-          g.vals, g.rhs, accumulate);
+          g.vals, g.rhs, accumulate, skip, xdiag);
       return;
     }
   }
@@ -1039,11 +1057,12 @@ void launch_assemble(Ctx* c, const P& phys, int accumulate)
   k_assemble<P, G, P::kUnroll, SCAP, P::kNonTemporal>
     <<<(unsigned)blocks, kBlock, 0, c->stream>>>(phys, g.row_ptr, g.code, g.cols,
                                                  g.diag, g.sched, (int)g.nchunks,
-                                                 g.vals, g.rhs, accumulate);
+                                                 g.vals, g.rhs, accumulate, skip,
+                                                 xdiag);
 }
 
 template <class P>
-int launch_rows(Ctx* c, const P& phys, int accumulate)
+int launch_rows(Ctx* c, const P& phys, int accumulate, double* xdiag = nullptr)
 {
   Graph& g = c->g;
   if (!g.sched_valid) {
@@ -1056,13 +1075,13 @@ int launch_rows(Ctx* c, const P& phys, int accumulate)
     // (Hex8 graphs); else two lanes per row, which also balances rows of
     // uneven length (mixed meshes), with the LDS the half chunks need
     if (g.chunk_slots_p99 <= P::kSlotsSmall)
-      launch_assemble<P, 1, P::kSlotsSmall>(c, phys, accumulate);
+      launch_assemble<P, 1, P::kSlotsSmall>(c, phys, accumulate, xdiag);
     else if (g.half_chunk_slots_p99 <= P::kSlotsSmall)
-      launch_assemble<P, 2, P::kSlotsSmall>(c, phys, accumulate);
+      launch_assemble<P, 2, P::kSlotsSmall>(c, phys, accumulate, xdiag);
     else
-      launch_assemble<P, 2, P::kSlotsLarge>(c, phys, accumulate);
+      launch_assemble<P, 2, P::kSlotsLarge>(c, phys, accumulate, xdiag);
   } else {
-    launch_assemble<P, P::kLanes, P::kSlotsSmall>(c, phys, accumulate);
+    launch_assemble<P, P::kLanes, P::kSlotsSmall>(c, phys, accumulate, xdiag);
   }
   NW_LAUNCH_CHECK();
   return NW_OK;
@@ -1138,7 +1157,12 @@ int nw_assemble_momentum(nw_ctx* c, const nw_momentum_opts* o)
                          c->nnode, c->nedge, o->alpha, o->alpha_upw, o->upw,
                          o->relax, o->include_divU, o->use_limiter,
                          nullptr, nullptr, nullptr, PecletDev{}};
-  return launch_rows(c, P, o->accumulate);
+  double* xdiag = nullptr;
+  if (o->extract_diagonal) {
+    int rc = ensure_field(c, NW_F_UDIAG, &xdiag);
+    if (rc) return rc;
+  }
+  return launch_rows(c, P, o->accumulate, xdiag);
 }
 
 int nw_assemble_momentum_with_peclet(nw_ctx* c, const nw_momentum_opts* o,
@@ -1164,7 +1188,12 @@ int nw_assemble_momentum_with_peclet(nw_ctx* c, const nw_momentum_opts* o,
                         o->relax, o->include_divU, o->use_limiter,
                         rho, pn, pfac,
                         PecletDev{pf->form, pf->hybrid, pf->c1, pf->c2}};
-  return launch_rows(c, P, o->accumulate);
+  double* xdiag = nullptr;
+  if (o->extract_diagonal) {
+    rc = ensure_field(c, NW_F_UDIAG, &xdiag);
+    if (rc) return rc;
+  }
+  return launch_rows(c, P, o->accumulate, xdiag);
 }
 
 int nw_momentum_edge_peclet(nw_ctx* c, const nw_peclet* pf)

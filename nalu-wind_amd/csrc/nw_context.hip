@@ -3,6 +3,7 @@
 // analogue), and the small row operations of the linear system:
 // zeroSystem, sumInto, resetRows, applyDirichletBCs, FixPressureAtNode.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -86,6 +87,7 @@ static void free_graph(Graph& g)
   (void)hipFree(g.sell.sp);
   (void)hipFree(g.sell.ci);
   (void)hipFree(g.sell.slot);
+  (void)hipFree(g.skip);
   g = Graph();
 }
 
@@ -363,6 +365,8 @@ int nw_linsys_build_edge_graph(nw_ctx* c, int ndof)
     }
   key.clear();
   key.shrink_to_fit();
+  static std::atomic<uint64_t> generation{0};
+  g.gen = ++generation;
   g.ndof = ndof;
   g.nnode = N;
   g.nnzb = nnzb;
@@ -398,6 +402,29 @@ int nw_linsys_zero(nw_ctx* c)
   k_fill<<<grid_for(nr, 256), 256, 0, c->stream>>>(g.rhs, 0.0, nr);
   k_fill<<<grid_for(nr, 256), 256, 0, c->stream>>>(g.sln, 0.0, nr);
   NW_LAUNCH_CHECK();
+  g.skip_check = true;  // HypreLinearSystem::resetCoeffApplierData
+  return NW_OK;
+}
+
+int nw_linsys_set_skipped_rows(nw_ctx* c, int64_t nnodes, const int32_t* nodes)
+{
+  NW_REQUIRE(c && c->g.ndof, NW_E_STATE, "linear system not built");
+  NW_REQUIRE(nnodes >= 0 && (nodes || nnodes == 0), NW_E_ARG, "bad node list");
+  Graph& g = c->g;
+  std::vector<uint8_t> m(g.nnode, 0);
+  for (int64_t i = 0; i < nnodes; ++i) {
+    NW_REQUIRE(nodes[i] >= 0 && nodes[i] < g.nnode, NW_E_ARG, "bad node id %d",
+               nodes[i]);
+    m[nodes[i]] = 1;
+  }
+  if (!g.skip) NW_HIP(hipMalloc(&g.skip, std::max<int64_t>(g.nnode, 1)));
+  NW_HIP(hipMemcpyAsync(g.skip, m.data(), g.nnode, hipMemcpyHostToDevice,
+                        c->stream));
+  NW_HIP(hipStreamSynchronize(c->stream));
+  g.nskip = 0;
+  for (uint8_t v : m) g.nskip += v;
+  g.h_skip.swap(m);
+  g.skip_check = true;
   return NW_OK;
 }
 
@@ -479,13 +506,18 @@ int nw_linsys_sum_into(nw_ctx* c, int nblocks, int nent, const int32_t* nodes,
   // target < 0 is rhs index -(t+1)
   std::vector<std::pair<int64_t, double>> contrib;
   contrib.reserve((size_t)nblocks * (nr * nr + nr));
+  // rows of skipped nodes take nothing while the skip check is on
+  // (HypreLinSysCoeffApplier::sum_into_1DoF, src/HypreLinearSystem.C:2190-2195)
+  const bool skipping = active_skip_mask(g) != nullptr;
   for (int b = 0; b < nblocks; ++b) {
     const int32_t* nn = nodes + (int64_t)b * nent;
     const double* L = lhs + (int64_t)b * nr * nr;
     const double* R = rhs + (int64_t)b * nr;
-    for (int i = 0; i < nent; ++i) {
+    for (int i = 0; i < nent; ++i)
       NW_REQUIRE(nn[i] >= 0 && nn[i] < g.nnode, NW_E_ARG, "bad node id %d",
                  nn[i]);
+    for (int i = 0; i < nent; ++i) {
+      if (skipping && g.h_skip[nn[i]]) continue;
       for (int di = 0; di < nd; ++di) {
         const int row = i * nd + di;
         contrib.push_back({-((int64_t)nn[i] * nd + di) - 1, R[row]});
@@ -569,8 +601,12 @@ int nw_linsys_reset_rows(nw_ctx* c, int64_t nnodes, const int32_t* nodes,
                          int begin, int end, double diag_value,
                          double rhs_residual)
 {
-  return reset_rows_impl(c, nnodes, nodes, begin, end, diag_value,
-                         rhs_residual, nullptr, nullptr, 1);
+  const int rc = reset_rows_impl(c, nnodes, nodes, begin, end, diag_value,
+                                 rhs_residual, nullptr, nullptr, 1);
+  // a row reset lifts the skip check until the next zeroSystem
+  // (HypreLinSysCoeffApplier::resetRows, src/HypreLinearSystem.C:2326)
+  if (rc == NW_OK) c->g.skip_check = false;
+  return rc;
 }
 
 int nw_linsys_apply_dirichlet(nw_ctx* c, int64_t nnodes, const int32_t* nodes,

@@ -93,6 +93,10 @@ struct Sell {
 };
 
 struct Graph {
+  // process-wide unique number of this build of the graph (0: none), so
+  // that objects derived from a graph notice a rebuild even when the new
+  // arrays reuse the old addresses
+  uint64_t gen = 0;
   int ndof = 0;
   int64_t nnode = 0;
   int64_t nnzb = 0;           // block slots (one per node pair)
@@ -117,7 +121,21 @@ struct Graph {
   int64_t half_chunk_slots_p99 = 0;  // the same for 32-row half chunks
   Sell sell;  // sliced ELL form of the 1-dof graph (built by the first solve)
   bool sched_valid = false;
+  // Skipped rows of the hypre backend (nw_linsys_set_skipped_rows): [nnode]
+  // flags, the number flagged, and whether contributions to them are
+  // currently dropped (on after zeroSystem, off after a row reset, as the
+  // reference's checkSkippedRows_ flag)
+  uint8_t* skip = nullptr;
+  std::vector<uint8_t> h_skip;
+  int64_t nskip = 0;
+  bool skip_check = true;
 };
+
+// The skip flags the scatter must honour now, or null.
+inline const uint8_t* active_skip_mask(const Graph& g)
+{
+  return (g.skip && g.nskip > 0 && g.skip_check) ? g.skip : nullptr;
+}
 
 struct Krylov;     // solver workspace (nw_krylov.hip)
 struct AmgDevice;  // AMG hierarchy (nw_amg.hip)

@@ -916,7 +916,7 @@ struct Mpnn {
   uint16_t* hb[2] = {nullptr, nullptr};  // bfloat16 activations [n][H]
   uint16_t* wfrag = nullptr;  // per layer [H/16][2H/16][64 lanes][4] bf16
   int32_t* eslot = nullptr;   // [nedge][2] the slots (L, R), (R, L) of an edge
-  const int32_t* eslot_code = nullptr;  // the graph eslot was built for
+  uint64_t eslot_gen = 0;     // generation of the graph eslot was built for
   int64_t eslot_nedge = -1;
 };
 
@@ -1046,9 +1046,10 @@ int slot_weights(Mpnn* m)
   const Graph& g = c->g;
   hipStream_t st = c->stream;
   const int64_t ne = std::max<int64_t>(c->nedge, 0);
-  if (m->eslot && (m->eslot_code != g.code || m->eslot_nedge != ne)) {
+  if (m->eslot && (m->eslot_gen != g.gen || m->eslot_nedge != ne)) {
+    // the graph was rebuilt (same sizes, check_graph passed): the slots of
+    // the edges are rebuilt; the node-sized buffers keep their size
     (void)hipFree(m->eslot);
-  (void)hipFree(m->vd);  // the graph was rebuilt
     m->eslot = nullptr;
   }
   if (!m->eslot) {
@@ -1059,7 +1060,7 @@ int slot_weights(Mpnn* m)
     NW_HIP(hipMemsetAsync(m->w, 0, sizeof(float) * m->nnz, st));
     k_edge_slots<<<grid_for(m->nnz, kBlock), kBlock, 0, st>>>(g.code, m->eslot, m->nnz);
     NW_LAUNCH_CHECK();
-    m->eslot_code = g.code;
+    m->eslot_gen = g.gen;
     m->eslot_nedge = ne;
   }
   if (ne == 0) return NW_OK;

@@ -35,6 +35,7 @@ void Realm::set_mesh(int64_t nnode, const std::vector<int32_t>& edge_nodes)
         "nw_mesh_set");
   nnode_ = nnode;
   nedge_ = (int64_t)edge_nodes.size() / 2;
+  active_ = nullptr;  // the new mesh has no graph yet
 }
 
 void Realm::set_field(int field, const std::vector<double>& data)
@@ -45,7 +46,8 @@ void Realm::set_field(int field, const std::vector<double>& data)
 
 std::vector<double> Realm::get_field(int field) const
 {
-  const bool edge = field >= NW_E_AREA_VECTOR;
+  const bool edge = field == NW_E_AREA_VECTOR || field == NW_E_MASS_FLOW_RATE ||
+                    field == NW_E_PECLET_FACTOR || field == NW_E_PECLET_NUMBER;
   std::vector<double> out((edge ? nedge_ : nnode_) * nw_field_components(field));
   check(nw_field_download(ctx_, field, out.data(), (int64_t)out.size()),
         "nw_field_download");
@@ -91,20 +93,39 @@ GpuLinearSystem::GpuLinearSystem(Realm& realm, unsigned numDof)
   numDof_ = numDof;
 }
 
-void GpuLinearSystem::buildEdgeToNodeGraph()
+GpuLinearSystem::~GpuLinearSystem()
+{
+  if (isActive()) realm_.setActiveSystem(nullptr);
+}
+
+void GpuLinearSystem::requireActive(const char* what) const
+{
+  if (!isActive())
+    throw std::runtime_error(
+      std::string(what) +
+      ": another linear system of this Realm holds the context (call "
+      "zeroSystem() of this system first)");
+}
+
+void GpuLinearSystem::activate()
 {
   check(nw_linsys_build_edge_graph(realm_.ctx(), (int)numDof_),
         "buildEdgeToNodeGraph");
+  realm_.setActiveSystem(this);
 }
+
+void GpuLinearSystem::buildEdgeToNodeGraph() { activate(); }
 
 void GpuLinearSystem::zeroSystem()
 {
+  if (!isActive()) activate();
   check(nw_linsys_zero(realm_.ctx()), "zeroSystem");
 }
 
 void GpuLinearSystem::sumInto(unsigned numEntities, const int32_t* entities,
                               const double* rhs, const double* lhs)
 {
+  requireActive("sumInto");
   (*applier_)(numEntities, entities, rhs, lhs);
 }
 
@@ -112,6 +133,7 @@ void GpuLinearSystem::applyDirichletBCs(int solutionField, int bcValuesField,
                                         const std::vector<int32_t>& nodes,
                                         unsigned beginPos, unsigned endPos)
 {
+  requireActive("applyDirichletBCs");
   check(nw_linsys_apply_dirichlet(realm_.ctx(), (int64_t)nodes.size(),
                                   nodes.data(), solutionField, bcValuesField,
                                   (int)beginPos, (int)endPos),
@@ -122,17 +144,20 @@ void GpuLinearSystem::resetRows(const std::vector<int32_t>& nodeList,
                                 unsigned beginPos, unsigned endPos,
                                 double diag_value, double rhs_residual)
 {
+  requireActive("resetRows");
   applier_->resetRows((unsigned)nodeList.size(), nodeList.data(), beginPos,
                       endPos, diag_value, rhs_residual);
 }
 
 void GpuLinearSystem::loadComplete()
 {
+  requireActive("loadComplete");
   check(nw_linsys_load_complete(realm_.ctx()), "loadComplete");
 }
 
 int GpuLinearSystem::solve(const nw_solver_cfg& cfg)
 {
+  requireActive("solve");
   const int rc = nw_linsys_solve(realm_.ctx(), &cfg, &stats_);
   if (rc != NW_OK && rc != NW_E_NOCONV) check(rc, "solve");
   iters_ = stats_.iters;
@@ -146,6 +171,7 @@ void GpuLinearSystem::download(std::vector<int64_t>& rowOffsets,
                                std::vector<double>& vals,
                                std::vector<double>& rhs) const
 {
+  requireActive("download");
   int64_t nrows = 0, nnz = 0;
   check(nw_linsys_sizes(realm_.ctx(), &nrows, &nnz), "sizes");
   rowOffsets.resize(nrows + 1);
@@ -159,12 +185,52 @@ void GpuLinearSystem::download(std::vector<int64_t>& rowOffsets,
 
 std::vector<double> GpuLinearSystem::solution() const
 {
+  requireActive("solution");
   int64_t nrows = 0;
   check(nw_linsys_sizes(realm_.ctx(), &nrows, nullptr), "sizes");
   std::vector<double> x(nrows);
   check(nw_linsys_solution_download(realm_.ctx(), x.data()), "solution");
   return x;
 }
+
+void HypreLinearSystem::buildDirichletNodeGraph(const std::vector<int32_t>& nodes)
+{
+  skipped_.insert(skipped_.end(), nodes.begin(), nodes.end());
+}
+
+void HypreLinearSystem::activate()
+{
+  GpuLinearSystem::activate();
+  check(nw_linsys_set_skipped_rows(realm_.ctx(), (int64_t)skipped_.size(),
+                                   skipped_.data()),
+        "buildDirichletNodeGraph");
+}
+
+void HypreLinearSystem::finalizeLinearSystem()
+{
+  // the skipped rows gathered by initialize_connectivity
+  requireActive("finalizeLinearSystem");
+  check(nw_linsys_set_skipped_rows(realm_.ctx(), (int64_t)skipped_.size(),
+                                   skipped_.data()),
+        "buildDirichletNodeGraph");
+}
+
+int HypreLinearSystem::solve(const nw_solver_cfg& cfg)
+{
+  const int rc = GpuLinearSystem::solve(cfg);
+  // hypre reports the relative residual; the linear system scales it by
+  // ||b|| to compare with the Tpetra report (HypreLinearSystem.C:2525-2533)
+  linearResidual_ = stats_.final_rel * stats_.bnorm;
+  return rc;
+}
+
+namespace {
+// throws unless the equation system's linear system holds the context
+void require_system(EquationSystem* eq, const char* what)
+{
+  if (eq) eq->linsys().requireActive(what);
+}
+}  // namespace
 
 void AssembleEdgeSolverAlgorithm::initialize_connectivity()
 {
@@ -184,6 +250,7 @@ void ScalarEdgeSolverAlg::execute()
   o.use_limiter = (it != so.limiterMap.end() && it->second) ? 1 : 0;
   o.peclet = so.peclet(dofName_);
   o.accumulate = 1;  // sumInto semantics after zeroSystem
+  require_system(eqSystem_, "ScalarEdgeSolverAlg");
   check(nw_assemble_scalar(realm_.ctx(), &o), "ScalarEdgeSolverAlg");
 }
 
@@ -200,6 +267,7 @@ void MomentumEdgeSolverAlg::execute()
   auto it = so.limiterMap.find(dof);
   o.use_limiter = (it != so.limiterMap.end() && it->second) ? 1 : 0;
   o.accumulate = 1;
+  require_system(eqSystem_, "MomentumEdgeSolverAlg");
   check(nw_assemble_momentum(realm_.ctx(), &o), "MomentumEdgeSolverAlg");
 }
 
@@ -214,6 +282,7 @@ void ContinuityEdgeSolverAlg::execute()
   o.interp_together = so.mdotInterpRhoUTogether ? 1.0 : 0.0;
   o.incompressible = so.solveIncompressibleContinuity ? 1.0 : 0.0;
   o.accumulate = 1;
+  require_system(eqSystem_, "ContinuityEdgeSolverAlg");
   check(nw_assemble_continuity(realm_.ctx(), &o), "ContinuityEdgeSolverAlg");
 }
 
@@ -249,8 +318,21 @@ void NodalGradEdgeAlg::execute()
   check(nw_nodal_grad(realm_.ctx(), phi_, gradPhi_), "NodalGradEdgeAlg");
 }
 
+void DirichletBC::initialize_connectivity()
+{
+  if (eqSystem_) eqSystem_->linsys().buildDirichletNodeGraph(nodes_);
+}
+
+void DirichletBC::execute()
+{
+  if (!eqSystem_) throw std::runtime_error("DirichletBC without an equation system");
+  eqSystem_->linsys().applyDirichletBCs(field_, bcValues_, nodes_, beginPos_,
+                                        endPos_);
+}
+
 void FixPressureAtNodeAlgorithm::execute()
 {
+  require_system(eqSystem_, "FixPressureAtNodeAlgorithm");
   check(nw_fix_pressure_at_node(realm_.ctx(), node_, refPressure_),
         "FixPressureAtNodeAlgorithm");
 }

@@ -67,8 +67,20 @@ struct TimeIntegrator {
   double gamma1 = 1.0;
 };
 
+class LinearSystem;
+
 // The part of Realm the edge path needs: one GPU context holding the mesh
 // (nodes, edges) and its fields, the options and the time integrator.
+//
+// The context holds one linear system at a time.  The reference gives each
+// equation system of a Realm its own linear system; here the systems of
+// one Realm take turns: the one whose graph the context holds is the
+// active one, zeroSystem() of another system makes that one active
+// (rebuilding its graph, which also drops the previous system's matrix,
+// solver workspace and preconditioner), and every other call on an
+// inactive system throws.  Assembling several systems therefore costs one
+// graph build per switch; a caller that keeps several systems resident
+// gives each its own Realm.
 class Realm {
  public:
   explicit Realm(int device = 0);
@@ -82,6 +94,9 @@ class Realm {
   int64_t nnode() const { return nnode_; }
   int64_t nedge() const { return nedge_; }
   nw_ctx* ctx() const { return ctx_; }
+  // the linear system whose graph the context holds (null: none)
+  const LinearSystem* activeSystem() const { return active_; }
+  void setActiveSystem(const LinearSystem* ls) { active_ = ls; }
 
   SolutionOptions solutionOptions;
   TimeIntegrator timeIntegrator;
@@ -89,6 +104,7 @@ class Realm {
  private:
   nw_ctx* ctx_ = nullptr;
   int64_t nnode_ = 0, nedge_ = 0;
+  const LinearSystem* active_ = nullptr;
 };
 
 // LinearSystem::CoeffApplier: additive block scatter + row reset.
@@ -106,6 +122,13 @@ class LinearSystem {
  public:
   virtual ~LinearSystem() = default;
   virtual void buildEdgeToNodeGraph() = 0;
+  // nodes whose rows a DirichletBC algorithm owns
+  // (LinearSystem::buildDirichletNodeGraph, include/LinearSystem.h:122-127):
+  // nothing for Tpetra, skipped rows for hypre
+  virtual void buildDirichletNodeGraph(const std::vector<int32_t>& nodes)
+  {
+    (void)nodes;
+  }
   virtual void finalizeLinearSystem() = 0;
   virtual void zeroSystem() = 0;
   virtual CoeffApplier* get_coeff_applier() = 0;
@@ -135,6 +158,7 @@ class LinearSystem {
 class GpuLinearSystem : public LinearSystem {
  public:
   GpuLinearSystem(Realm& realm, unsigned numDof);
+  ~GpuLinearSystem() override;
   void buildEdgeToNodeGraph() override;
   void finalizeLinearSystem() override {}
   void zeroSystem() override;
@@ -156,12 +180,40 @@ class GpuLinearSystem : public LinearSystem {
   std::vector<double> solution() const;
   Realm& realm() { return realm_; }
   nw_solve_stats lastStats() const { return stats_; }
+  bool isActive() const { return realm_.activeSystem() == this; }
+  // throws unless the context holds this system's graph
+  void requireActive(const char* what) const;
 
- private:
+ protected:
+  // the graph of this system into the context (both builds and switches)
+  virtual void activate();
   Realm& realm_;
   std::unique_ptr<CoeffApplier> applier_;
   nw_solve_stats stats_{};
 };
+
+// HypreLinearSystem semantics (src/HypreLinearSystem.C) over the same
+// device system: Dirichlet nodes are skipped rows that the assembly and
+// sumInto leave alone (sum_into_1DoF, :2165-2239) until applyDirichletBCs
+// writes their diagonal and rhs (:2408-2456), and the reported linear
+// residual is hypre's relative residual times ||b|| (:2533; hypre returns
+// the relative norm, src/HypreDirectSolver.C:124-130).
+class HypreLinearSystem : public GpuLinearSystem {
+ public:
+  using GpuLinearSystem::GpuLinearSystem;
+  void buildDirichletNodeGraph(const std::vector<int32_t>& nodes) override;
+  void finalizeLinearSystem() override;
+  int solve(const nw_solver_cfg& cfg) override;
+
+ protected:
+  void activate() override;
+
+ private:
+  std::vector<int32_t> skipped_;
+};
+
+// LinearSystem::create (src/LinearSystem.C:117-159): the backend by type
+enum class LinearSystemType { Tpetra, Hypre };
 
 class EquationSystem;
 
@@ -266,6 +318,27 @@ class NodalGradEdgeAlg {
   int phi_, gradPhi_;
 };
 
+// DirichletBC (src/DirichletBC.C): its nodes' rows become
+// solution = bc value through applyDirichletBCs of the equation system's
+// linear system; initialize_connectivity registers the nodes
+// (buildDirichletNodeGraph, the hypre backend's skipped rows).
+class DirichletBC : public SolverAlgorithm {
+ public:
+  DirichletBC(Realm& realm, EquationSystem* eqSystem, int field, int bcValues,
+              std::vector<int32_t> nodes, unsigned beginPos, unsigned endPos)
+    : SolverAlgorithm(realm, eqSystem), field_(field), bcValues_(bcValues),
+      nodes_(std::move(nodes)), beginPos_(beginPos), endPos_(endPos)
+  {
+  }
+  void execute() override;
+  void initialize_connectivity() override;
+
+ private:
+  int field_, bcValues_;
+  std::vector<int32_t> nodes_;
+  unsigned beginPos_, endPos_;
+};
+
 class FixPressureAtNodeAlgorithm : public SolverAlgorithm {
  public:
   FixPressureAtNodeAlgorithm(Realm& realm, EquationSystem* eqSystem,
@@ -284,8 +357,12 @@ class FixPressureAtNodeAlgorithm : public SolverAlgorithm {
 // EquationSystem::assemble_and_solve with the reference's phase timers.
 class EquationSystem {
  public:
-  EquationSystem(Realm& realm, unsigned numDof)
-    : realm_(realm), linsys_(new GpuLinearSystem(realm, numDof))
+  EquationSystem(Realm& realm, unsigned numDof,
+                 LinearSystemType type = LinearSystemType::Tpetra)
+    : realm_(realm),
+      linsys_(type == LinearSystemType::Hypre
+                ? new HypreLinearSystem(realm, numDof)
+                : new GpuLinearSystem(realm, numDof))
   {
   }
   void add_algorithm(std::unique_ptr<SolverAlgorithm> alg)

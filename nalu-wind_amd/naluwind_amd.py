@@ -30,7 +30,7 @@ OK, E_ARG, E_STATE, E_HIP, E_NOGRAPHCOL, E_NONFINITE, E_NOCONV, E_ALLOC = (
 (F_COORDINATES, F_VELOCITY, F_DENSITY, F_PRESSURE, F_DPDX, F_UDIAG,
  F_VISCOSITY, F_DUDX, F_WALL_MASK, F_SCALAR_Q, F_DQDX, F_DIFF_FLUX_COEFF,
  F_DUAL_VOLUME, F_BC_VALUE, E_AREA_VECTOR, E_MASS_FLOW_RATE, E_PECLET_FACTOR,
- E_PECLET_NUMBER) = range(18)
+ E_PECLET_NUMBER, F_PRESSURE_BC) = range(19)
 
 PREC_NONE, PREC_JACOBI, PREC_CHEBYSHEV, PREC_AMG = 0, 1, 2, 3
 SOLVE_FP64, SOLVE_FP32_REFINE = 0, 1
@@ -61,7 +61,7 @@ class MomentumOpts(C.Structure):
     _fields_ = [("alpha", C.c_double), ("alpha_upw", C.c_double),
                 ("upw", C.c_double), ("relax", C.c_double),
                 ("include_divU", C.c_double), ("use_limiter", C.c_int),
-                ("accumulate", C.c_int)]
+                ("accumulate", C.c_int), ("extract_diagonal", C.c_int)]
 
 
 class MdotOpts(C.Structure):
@@ -116,7 +116,7 @@ def scalar_defaults(**kw):
 
 
 def momentum_defaults(**kw):
-    o = MomentumOpts(0.0, 1.0, 1.0, 1.0, 0.0, 0, 0)
+    o = MomentumOpts(0.0, 1.0, 1.0, 1.0, 0.0, 0, 0, 0)
     for k, v in kw.items():
         setattr(o, k, v)
     return o
@@ -162,6 +162,7 @@ _sig = {
     "nw_linsys_apply_dirichlet": ([_P, C.c_int64, _i32p, C.c_int, C.c_int,
                                    C.c_int, C.c_int], C.c_int),
     "nw_fix_pressure_at_node": ([_P, C.c_int32, C.c_double], C.c_int),
+    "nw_linsys_set_skipped_rows": ([_P, C.c_int64, _i32p], C.c_int),
     "nw_assemble_scalar": ([_P, C.POINTER(ScalarOpts)], C.c_int),
     "nw_assemble_momentum": ([_P, C.POINTER(MomentumOpts)], C.c_int),
     "nw_assemble_continuity": ([_P, C.POINTER(ContinuityOpts)], C.c_int),
@@ -377,7 +378,8 @@ class Context:
 
     def field_download(self, fid):
         nc = lib.nw_field_components(fid)
-        edge = fid >= E_AREA_VECTOR
+        edge = fid in (E_AREA_VECTOR, E_MASS_FLOW_RATE, E_PECLET_FACTOR,
+                       E_PECLET_NUMBER)
         n = (self.nedge if edge else self.nnode) * nc
         out = np.zeros(n)
         check(lib.nw_field_download(self.h, fid, out, n))
@@ -438,6 +440,11 @@ class Context:
         nodes = _i32(nodes)
         check(lib.nw_linsys_apply_dirichlet(self.h, nodes.size, nodes, sol_field,
                                             bc_field, begin, end))
+
+    def set_skipped_rows(self, nodes):
+        """Skipped rows of the hypre backend (buildDirichletNodeGraph)."""
+        nodes = _i32(nodes)
+        check(lib.nw_linsys_set_skipped_rows(self.h, nodes.size, nodes))
 
     def fix_pressure_at_node(self, node, pref):
         check(lib.nw_fix_pressure_at_node(self.h, int(node), pref))

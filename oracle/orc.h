@@ -226,7 +226,25 @@ typedef struct {
   int32_t* cols;    /* nnz */
   double* vals;     /* nnz */
   double* rhs;      /* nrows */
+  /* hypre backend only (else NULL): [nnode] skipped-row flags of the
+   * Dirichlet nodes (HypreLinearSystem::buildDirichletNodeGraph) and the
+   * applier's checkSkippedRows_ flag */
+  const unsigned char* skip;
+  int skip_check;
 } orc_csr;
+
+/* HypreLinearSystem::zeroSystem + resetCoeffApplierData
+ * (src/HypreLinearSystem.C:1892-1925, 1386-1392): values and rhs to zero,
+ * the skipped-row check back on. */
+void orc_csr_zero(orc_csr* A);
+
+/* HypreLinearSystem::applyDirichletBCs (src/HypreLinearSystem.C:2408-2456)
+ * on skipped rows, whose hypre graph row holds only the diagonal: the
+ * diagonal becomes 1, rhs = bc - sol; the row's other entries (zero: the
+ * scatter skipped them) are left. */
+void orc_csr_apply_dirichlet_hypre(orc_csr* A, int64_t nnodes,
+                                   const int32_t* nodes, const double* sol,
+                                   const double* bc, int ncomp);
 
 /* Graph of self + edge neighbours, sorted and expanded to ndof
  * (TpetraLinearSystem::buildEdgeToNodeGraph / addConnections /
@@ -239,12 +257,17 @@ void orc_csr_free(orc_csr* A);
 /* sum_into / sum_into_row / sum_into_row_vec_3
  * (src/TpetraLinearSystem.C:1293-1459): sort the entity column ids,
  * walk each sorted row once, add; entries with no graph column are
- * dropped.  lhs is [(n*ndof)][(n*ndof)] row-major, rhs [n*ndof]. */
+ * dropped.  lhs is [(n*ndof)][(n*ndof)] row-major, rhs [n*ndof].  With
+ * skipped rows set and their check on, the rows of skipped nodes are left
+ * alone (HypreLinSysCoeffApplier::sum_into_1DoF,
+ * src/HypreLinearSystem.C:2190-2195). */
 void orc_csr_sum_into(orc_csr* A, int nent, const int32_t* nodes,
                       const double* lhs, const double* rhs);
 
 /* reset_rows (src/TpetraLinearSystem.C:1461-1520): row -> diag_value on
- * the diagonal, zero elsewhere; rhs -> rhs_residual. */
+ * the diagonal, zero elsewhere; rhs -> rhs_residual.  Lifts the
+ * skipped-row check (HypreLinSysCoeffApplier::resetRows,
+ * src/HypreLinearSystem.C:2326). */
 void orc_csr_reset_rows(orc_csr* A, int64_t nnodes, const int32_t* nodes,
                         int begin, int end, double diag_value,
                         double rhs_residual);

@@ -57,7 +57,8 @@ class Csr(C.Structure):
     _fields_ = [("nnode", C.c_int64), ("ndof", C.c_int), ("nrows", C.c_int64),
                 ("nnz", C.c_int64), ("row_ptr", C.POINTER(C.c_int64)),
                 ("cols", C.POINTER(C.c_int32)), ("vals", C.POINTER(C.c_double)),
-                ("rhs", C.POINTER(C.c_double))]
+                ("rhs", C.POINTER(C.c_double)), ("skip", C.c_void_p),
+                ("skip_check", C.c_int)]
 
 
 class GmresOpts(C.Structure):
@@ -115,6 +116,9 @@ lib.orc_csr_apply_dirichlet.argtypes = [C.POINTER(Csr), C.c_int64, _i32p,
                                         C.c_int, C.c_int, _f64p, _f64p, C.c_int]
 lib.orc_csr_fix_node.argtypes = [C.POINTER(Csr), C.c_int32, C.c_double,
                                  C.c_double]
+lib.orc_csr_zero.argtypes = [C.POINTER(Csr)]
+lib.orc_csr_apply_dirichlet_hypre.argtypes = [C.POINTER(Csr), C.c_int64, _i32p,
+                                              _f64p, _f64p, C.c_int]
 _dense = C.c_void_p
 lib.orc_assemble_scalar.argtypes = [C.POINTER(ScalarOpts), C.c_int64,
                                     C.c_int64, _i32p] + [_f64p] * 8 + [
@@ -368,6 +372,22 @@ class CsrSystem:
     def reset_rows(self, nodes, begin, end, diag=0.0, rhs_res=0.0):
         lib.orc_csr_reset_rows(self.ref, len(nodes), i32(nodes), begin, end,
                                diag, rhs_res)
+
+    def set_skipped_rows(self, nodes):
+        """Skipped rows of the hypre backend (orc_csr.skip)."""
+        self._skip = np.zeros(self.c.nnode, np.uint8)
+        self._skip[np.asarray(nodes, np.int64)] = 1
+        self.c.skip = self._skip.ctypes.data_as(C.c_void_p)
+        self.c.skip_check = 1
+
+    def zero(self):
+        lib.orc_csr_zero(self.ref)
+
+    def apply_dirichlet_hypre(self, nodes, sol, bc):
+        sol = f64(sol)
+        ncomp = 1 if sol.ndim == 1 else sol.shape[1]
+        lib.orc_csr_apply_dirichlet_hypre(self.ref, len(nodes), i32(nodes),
+                                          sol.ravel(), f64(bc).ravel(), ncomp)
 
     def apply_dirichlet(self, nodes, begin, end, sol, bc):
         sol = f64(sol)

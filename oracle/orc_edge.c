@@ -420,6 +420,8 @@ int orc_csr_build_edge_graph(int64_t nnode, int64_t nedge,
           cmp_i32);
   A->nnode = nnode;
   A->ndof = ndof;
+  A->skip = NULL;
+  A->skip_check = 1;
   A->nrows = nnode * ndof;
   A->nnz = nptr[nnode] * (int64_t)ndof * ndof;
   A->row_ptr = (int64_t*)malloc(sizeof(int64_t) * ((size_t)A->nrows + 1));
@@ -511,6 +513,7 @@ void orc_csr_sum_into(orc_csr* A, int nent, const int32_t* nodes,
   }
   for (int r = 0; r < numRows; ++r) {
     const int i = perm[r] / ndof;
+    if (A->skip && A->skip_check && A->skip[nodes[i]]) continue;
     const int64_t rowLid = (int64_t)nodes[i] * ndof + perm[r] % ndof;
     const double* cur_lhs = lhs + (int64_t)perm[r] * numRows;
     sum_into_row(A, rowLid, nent, ndof, localIds, perm, cur_lhs);
@@ -523,6 +526,7 @@ void orc_csr_reset_rows(orc_csr* A, int64_t nnodes, const int32_t* nodes,
                         double rhs_residual)
 {
   /* reset_rows / reset_row, src/TpetraLinearSystem.C:1461-1520 */
+  A->skip_check = 0;
   for (int64_t nn = 0; nn < nnodes; ++nn)
     for (int d = begin; d < end; ++d) {
       const int64_t row = (int64_t)nodes[nn] * A->ndof + d;
@@ -530,6 +534,30 @@ void orc_csr_reset_rows(orc_csr* A, int64_t nnodes, const int32_t* nodes,
         A->vals[k] = (A->cols[k] == row) ? diag_value : 0.0;
       A->rhs[row] = rhs_residual;
     }
+}
+
+void orc_csr_zero(orc_csr* A)
+{
+  memset(A->vals, 0, sizeof(double) * (size_t)A->nnz);
+  memset(A->rhs, 0, sizeof(double) * (size_t)A->nrows);
+  A->skip_check = 1;
+}
+
+void orc_csr_apply_dirichlet_hypre(orc_csr* A, int64_t nnodes,
+                                   const int32_t* nodes, const double* sol,
+                                   const double* bc, int ncomp)
+{
+  /* src/HypreLinearSystem.C:2446-2455: vals(row start) = 1, the row start
+   * of a skipped row being its diagonal entry */
+  for (int64_t nn = 0; nn < nnodes; ++nn) {
+    const int32_t n = nodes[nn];
+    for (int d = 0; d < A->ndof; ++d) {
+      const int64_t row = (int64_t)n * A->ndof + d;
+      for (int64_t k = A->row_ptr[row]; k < A->row_ptr[row + 1]; ++k)
+        if (A->cols[k] == row) A->vals[k] = 1.0;
+      A->rhs[row] = bc[(int64_t)n * ncomp + d] - sol[(int64_t)n * ncomp + d];
+    }
+  }
 }
 
 void orc_csr_apply_dirichlet(orc_csr* A, int64_t nnodes,

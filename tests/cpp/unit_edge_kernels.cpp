@@ -294,7 +294,154 @@ void test_geometry_unit_hex()
   for (size_t i = 0; i < dv.size(); ++i) std::printf("%s%.17g", i ? "," : "", dv[i]);
   std::printf("], \"area_vector\": [");
   for (size_t i = 0; i < av.size(); ++i) std::printf("%s%.17g", i ? "," : "", av[i]);
-  std::printf("]}\n");
+  std::printf("]},\n");
+}
+
+void print_array(const char* name, const std::vector<double>& a, bool comma = true)
+{
+  std::printf("\"%s\": [", name);
+  for (size_t i = 0; i < a.size(); ++i) std::printf("%s%.17g", i ? "," : "", a[i]);
+  std::printf("]%s\n", comma ? "," : "");
+}
+
+void print_solve(const char* name, const GpuLinearSystem& ls, const nw_solve_stats& st,
+                 double timerSolve)
+{
+  std::printf("\"%s\": {\"iters\": %d, \"converged\": %d, \"final_rel\": %.17g, "
+              "\"final_abs\": %.17g, \"bnorm\": %.17g, \"linearResidual\": %.17g, "
+              "\"nonLinearResidual\": %.17g, \"linearSolveIterations\": %d, "
+              "\"timerSolve\": %.17g},\n",
+              name, st.iters, st.converged, st.final_rel, st.final_abs, st.bnorm,
+              ls.linearResidual(), ls.nonLinearResidual(), ls.linearSolveIterations(),
+              timerSolve);
+}
+
+// Two equation systems on one Realm (continuity with a Dirichlet pressure
+// on the top face, solved through EquationSystem::assemble_and_solve with a
+// GMRES + Jacobi configuration; and a scalar adv-diff system), which take
+// turns on the Realm's context, then the continuity system again on the
+// hypre backend.  Box 5 x 4 x 6 stretched 1.1, jitter 0.15, seed 3; the
+// fields are printed for the host-side checks.
+void test_two_systems_one_realm()
+{
+  const int nx = 5, ny = 4, nz = 6;
+  int64_t nn = 0, nel = 0;
+  check(nw_hex_box_sizes(nx, ny, nz, &nn, &nel), "sizes");
+  std::vector<double> coords(3 * nn);
+  std::vector<int32_t> conn(8 * nel);
+  check(nw_hex_box(nx, ny, nz, 1.1, 0.15, 3, coords.data(), conn.data()), "box");
+  const int64_t cap = 3 * nn + 12 * nel;
+  std::vector<int32_t> edges(2 * cap), ee(12 * nel);
+  int64_t ne = 0;
+  check(nw_hex_edges(nel, conn.data(), nn, edges.data(), cap, &ne, ee.data()), "edges");
+  edges.resize(2 * ne);
+  std::vector<double> areav(3 * ne);
+  check(nw_hex_edge_area_vector(nel, conn.data(), coords.data(), ee.data(),
+                                edges.data(), ne, areav.data()),
+        "area");
+  std::vector<double> vel(3 * nn), p(nn), dp(3 * nn), q(nn), rho(nn), gam(nn);
+  for (int64_t n = 0; n < nn; ++n) {
+    const double x = coords[3 * n], y = coords[3 * n + 1], z = coords[3 * n + 2];
+    vel[3 * n] = std::sin(1.3 * x + 0.2) * std::cos(0.7 * y);
+    vel[3 * n + 1] = std::cos(0.9 * x) * std::sin(1.1 * z + 0.4);
+    vel[3 * n + 2] = 0.3 * std::sin(2.0 * y + z);
+    p[n] = std::cos(1.7 * x + 0.5 * y) * (1.0 + z);
+    dp[3 * n] = -1.7 * std::sin(1.7 * x + 0.5 * y) * (1.0 + z);
+    dp[3 * n + 1] = -0.5 * std::sin(1.7 * x + 0.5 * y) * (1.0 + z);
+    dp[3 * n + 2] = std::cos(1.7 * x + 0.5 * y);
+    q[n] = std::sin(3.0 * x) * std::cos(2.0 * y) + z;
+    rho[n] = 1.0 + 0.2 * std::sin(x + 2.0 * z);
+    gam[n] = 1.0e-2 * (1.0 + x * y);
+  }
+  std::vector<double> mdot(ne);
+  check(nw_mdot_edge(ne, edges.data(), areav.data(), rho.data(), vel.data(),
+                     mdot.data()),
+        "mdot");
+  std::vector<int32_t> top;
+  for (int64_t n = 0; n < nn; ++n)
+    if (coords[3 * n + 2] >= 1.0 - 1e-12) top.push_back((int32_t)n);
+  Realm realm(0);
+  realm.set_mesh(nn, edges);
+  realm.set_field(NW_F_COORDINATES, coords);
+  realm.set_field(NW_E_AREA_VECTOR, areav);
+  realm.set_field(NW_F_VELOCITY, vel);
+  realm.set_field(NW_F_PRESSURE, p);
+  realm.set_field(NW_F_DPDX, dp);
+  realm.set_field(NW_F_DENSITY, rho);
+  realm.set_field(NW_F_UDIAG, std::vector<double>(nn, 2.0));
+  realm.set_field(NW_F_PRESSURE_BC, std::vector<double>(nn, 0.0));
+  realm.set_field(NW_F_SCALAR_Q, q);
+  realm.set_field(NW_F_DQDX, std::vector<double>(3 * nn, 0.0));
+  realm.set_field(NW_F_DIFF_FLUX_COEFF, gam);
+  realm.set_field(NW_E_MASS_FLOW_RATE, mdot);
+  realm.timeIntegrator.timeStepN = 0.5;
+  realm.timeIntegrator.gamma1 = 1.0;
+  realm.solutionOptions.hybridMap["mixture_fraction"] = 1.0;
+  realm.solutionOptions.alphaMap["mixture_fraction"] = 0.3;
+
+  auto make_continuity = [&](EquationSystem& eq) {
+    eq.add_algorithm(std::unique_ptr<SolverAlgorithm>(new ContinuityEdgeSolverAlg(realm, &eq)));
+    eq.add_algorithm(std::unique_ptr<SolverAlgorithm>(
+      new DirichletBC(realm, &eq, NW_F_PRESSURE, NW_F_PRESSURE_BC, top, 0, 1)));
+  };
+  EquationSystem cont(realm, 1);
+  make_continuity(cont);
+  EquationSystem scal(realm, 1);
+  scal.add_algorithm(std::unique_ptr<SolverAlgorithm>(
+    new ScalarEdgeSolverAlg(realm, &scal, "mixture_fraction")));
+  cont.initialize();
+  scal.initialize();  // now the scalar system holds the context
+
+  nw_solver_cfg cfg{};
+  cfg.kspace = 50;
+  cfg.max_iters = 2000;
+  cfg.tol = 1e-10;
+  cfg.precond = NW_PREC_JACOBI;
+  cfg.mode = NW_SOLVE_FP64;
+  cfg.x0 = NW_X0_ZERO;
+
+  std::printf("\"two_systems\": {\n");
+  print_array("coords", coords);
+  print_array("velocity", vel);
+  print_array("pressure", p);
+  print_array("dpdx", dp);
+  print_array("density", rho);
+  print_array("scalar", q);
+  print_array("diff_flux_coeff", gam);
+  print_array("mdot", mdot);
+  std::printf("\"edges\": [");
+  for (size_t i = 0; i < edges.size(); ++i) std::printf("%s%d", i ? "," : "", edges[i]);
+  std::printf("],\n\"top\": [");
+  for (size_t i = 0; i < top.size(); ++i) std::printf("%s%d", i ? "," : "", top[i]);
+  std::printf("],\n");
+  // the continuity system takes the context back (zeroSystem), assembles,
+  // applies its Dirichlet rows and solves
+  cont.assemble_and_solve(&cfg);
+  print_system("continuity", cont.linsys(), false);
+  print_array("continuity_solution", cont.linsys().solution());
+  print_solve("continuity_solve", cont.linsys(), cont.linsys().lastStats(),
+              cont.timerSolve_);
+  // the scalar system next; the continuity system is then inactive
+  scal.assemble_and_solve(nullptr);
+  print_system("scalar", scal.linsys(), false);
+  int inactive_throws = 0;
+  try {
+    cont.linsys().solve(cfg);
+  } catch (const std::runtime_error&) {
+    inactive_throws = 1;
+  }
+  std::printf("\"inactive_throws\": %d,\n", inactive_throws);
+  // the same continuity equation system on the hypre backend
+  EquationSystem hyp(realm, 1, LinearSystemType::Hypre);
+  make_continuity(hyp);
+  hyp.initialize();
+  hyp.assemble_and_solve(&cfg);
+  print_system("hypre_continuity", hyp.linsys(), false);
+  print_array("hypre_solution", hyp.linsys().solution());
+  print_solve("hypre_solve", hyp.linsys(), hyp.linsys().lastStats(), hyp.timerSolve_);
+  std::printf("\"timers\": [%.17g, %.17g, %.17g]\n", cont.timerAssemble_,
+              cont.timerLoadComplete_, cont.timerSolve_);
+  std::printf("}\n");
 }
 
 }  // namespace
@@ -308,6 +455,7 @@ int main()
     test_scalar_adv_diff();
     test_mdot_calc_edge();
     test_geometry_unit_hex();
+    test_two_systems_one_realm();
     std::printf("}\n");
   } catch (const std::exception& e) {
     std::fprintf(stderr, "FAILED: %s\n", e.what());

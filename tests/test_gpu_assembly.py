@@ -461,3 +461,124 @@ def test_edge_order_and_orientation_do_not_change_the_system(nw, skewed):
                 P.pecfac[perm], ndof, fn)
         ref = types.SimpleNamespace(row_ptr=a[0], cols=a[1], vals=a[2], rhs=a[3])
         pb.assert_csr_close(b, ref, rtol=1e-13)
+
+
+# ------------------------------------------------------------ hypre backend
+
+@pytest.mark.parametrize("ndof", [1, 3])
+def test_skipped_rows_match_oracle(nw, ctx, skewed, ndof):
+    """nw_linsys_set_skipped_rows: the hypre applier's skipped rows
+    (HypreLinearSystem.C:2165-2239, 2326, 2408-2456) through the edge
+    assembly (overwrite and accumulate), sumInto, applyDirichletBCs, a row
+    reset that lifts the check and zeroSystem that restores it, against the
+    oracle's restatement."""
+    P = skewed
+    P.upload(ctx)
+    ctx.build_graph(ndof)
+    rng = np.random.default_rng(31)
+    skipped = np.unique(rng.integers(0, P.n, 25)).astype(np.int32)
+    ctx.set_skipped_rows(skipped)
+    if ndof == 1:
+        o = nw.continuity_defaults(accumulate=1)
+        run = lambda: ctx.assemble_continuity(o)
+        oracle = lambda A: orc.assemble_continuity(
+            pb.to_orc_continuity(o), P.n, P.edges, P.coords, P.vel, P.dpdx, P.rho,
+            P.p, P.udiag, P.areav, csr=A)
+        sol_f, bc_f = nw.F_PRESSURE, nw.F_PRESSURE_BC
+        sol, bc = P.p, rng.normal(size=P.n)
+    else:
+        o = nw.momentum_defaults(accumulate=1, alpha=0.3)
+        run = lambda: ctx.assemble_momentum(o)
+        oracle = lambda A: orc.assemble_momentum(
+            pb.to_orc_momentum(o), P.n, P.edges, P.coords, P.vel, P.dudx, P.visc,
+            P.rho, P.areav, P.mdot, P.pecfac, P.wall, csr=A)
+        sol_f, bc_f = nw.F_VELOCITY, nw.F_BC_VALUE
+        sol, bc = P.vel, rng.normal(size=(P.n, 3))
+    ctx.field_upload(bc_f, bc)
+    A = orc.CsrSystem(P.n, P.edges, ndof)
+    A.set_skipped_rows(skipped)
+    ctx.zero()
+    run()
+    oracle(A)
+    pb.assert_csr_close(ctx.download(), A)
+    # overwrite mode writes the skipped rows as zeros
+    o.accumulate = 0
+    ctx.field_upload(sol_f, sol)
+    run()
+    pb.assert_csr_close(ctx.download(), A)
+    o.accumulate = 1
+    ctx.apply_dirichlet(skipped, sol_f, bc_f, 0, ndof)
+    A.apply_dirichlet_hypre(skipped, sol, bc)
+    pb.assert_csr_close(ctx.download(), A)
+    # a second assembly adds to the other rows only
+    run()
+    oracle(A)
+    pb.assert_csr_close(ctx.download(), A)
+    e = P.edges[np.where(np.isin(P.edges, skipped).any(1))[0][0]]
+    lhs = rng.normal(size=(2 * ndof, 2 * ndof))
+    rhs = rng.normal(size=2 * ndof)
+    ctx.sum_into(e, lhs, rhs)
+    A.sum_into(e, lhs, rhs)
+    pb.assert_csr_close(ctx.download(), A)
+    # a row reset lifts the check: the skipped rows take contributions
+    nodes = np.array([int(np.setdiff1d(np.arange(P.n), skipped)[0])], np.int32)
+    ctx.reset_rows(nodes, 0, ndof, 2.0, 0.5)
+    A.reset_rows(nodes, 0, ndof, 2.0, 0.5)
+    ctx.sum_into(e, lhs, rhs)
+    A.sum_into(e, lhs, rhs)
+    run()
+    oracle(A)
+    pb.assert_csr_close(ctx.download(), A)
+    # zeroSystem turns it back on
+    ctx.zero()
+    A.zero()
+    run()
+    oracle(A)
+    pb.assert_csr_close(ctx.download(), A)
+    ctx.set_skipped_rows(np.zeros(0, np.int32))   # cleared: the Tpetra system
+    ctx.zero()
+    run()
+    pb.assert_csr_close(ctx.download(), _tpetra(P, ndof, oracle))
+
+
+def _tpetra(P, ndof, oracle):
+    A = orc.CsrSystem(P.n, P.edges, ndof)
+    oracle(A)
+    return A
+
+
+def test_momentum_extract_diagonal(nw, ctx, skewed):
+    """NGPApplyCoeff::extract_diagonal (src/SolverAlgorithm.C:89-105): the
+    (0, 0) entry of each node's (node, node) block of this assembly is added
+    to momentum_diag, in overwrite and accumulate modes and in the one-pass
+    form; other calls leave the field alone."""
+    P = skewed
+    P.upload(ctx)
+    ctx.build_graph(3)
+    u0 = np.random.default_rng(2).uniform(1.0, 2.0, P.n)
+    ctx.field_upload(nw.F_UDIAG, u0)
+    o = nw.momentum_defaults(alpha=0.2, relax=0.7, extract_diagonal=1)
+    A = P.oracle_momentum(pb.to_orc_momentum(o))
+    D = A.dense()
+    d00 = np.diag(D)[0::3]
+    ctx.assemble_momentum(o)
+    pb.assert_csr_close(ctx.download(), A)
+    u1 = ctx.field_download(nw.F_UDIAG)
+    np.testing.assert_allclose(u1, u0 + d00, rtol=1e-13, atol=0)
+    o.accumulate = 1
+    ctx.assemble_momentum(o)
+    np.testing.assert_allclose(ctx.field_download(nw.F_UDIAG), u0 + 2 * d00,
+                               rtol=1e-13, atol=0)
+    o.extract_diagonal = 0
+    o.accumulate = 0
+    ctx.field_upload(nw.F_UDIAG, u0)
+    ctx.assemble_momentum(o)
+    np.testing.assert_array_equal(ctx.field_download(nw.F_UDIAG), u0)
+    o.extract_diagonal = 1
+    pf = nw.Peclet(0, 1.0, 2.0, 4.0)
+    ctx.momentum_edge_peclet(pf)
+    ctx.assemble_momentum(o)
+    u2 = ctx.field_download(nw.F_UDIAG)
+    ctx.field_upload(nw.F_UDIAG, u0)
+    ctx.assemble_momentum_with_peclet(o, pf)
+    np.testing.assert_allclose(ctx.field_download(nw.F_UDIAG), u2, rtol=1e-13, atol=0)

@@ -286,6 +286,61 @@ def test_learned_guess_needs_a_network(nw, ctx):
         net.params = np.zeros(net.nparams - 1)
 
 
+def _rebuild_permuted(nw, ctx, P, top, seed):
+    """The same mesh with its edge list permuted (same node pairs, same
+    orientation): every edge id changes, the sizes and the matrix do not."""
+    perm = np.random.default_rng(seed).permutation(P.ne)
+    P.edges = np.ascontiguousarray(P.edges[perm])
+    P.areav = np.ascontiguousarray(P.areav[perm])
+    P.mdot = np.ascontiguousarray(P.mdot[perm])
+    P.pecfac = np.ascontiguousarray(P.pecfac[perm])
+    P.upload(ctx)
+    ctx.build_graph(1)
+    ctx.assemble_continuity(nw.continuity_defaults())
+    ctx.field_upload(nw.F_PRESSURE, np.zeros(P.n))
+    ctx.field_upload(nw.F_DIFF_FLUX_COEFF, np.zeros(P.n))
+    ctx.apply_dirichlet(top, nw.F_PRESSURE, nw.F_DIFF_FLUX_COEFF, 0, 1)
+
+
+def test_graph_rebuild_between_network_calls(nw, ctx):
+    """A network whose context rebuilds its graph between record, loss and
+    predict: once with the same edges, once from a permuted edge list of
+    the same sizes (every edge id changes; buffers may come back at the
+    same addresses).  The network rebuilds its per-edge slots and then
+    gives bitwise what a freshly created network gives."""
+    P, A, top = _system(nw, ctx, seed=21)
+    p = _params(nw, 22, scale=0.1)
+    net = nw.Mpnn(ctx, H, L, seed=3)
+    net.params = p
+    _solved_pair(nw, ctx, A, 23)
+    net.record()
+    l0, g0 = net.loss_grad(0)
+    x_a, s_a = net.predict()
+    for k in range(2):
+        if k == 0:  # the same edge list, the graph built again
+            ctx.build_graph(1)
+            ctx.assemble_continuity(nw.continuity_defaults())
+            ctx.apply_dirichlet(top, nw.F_PRESSURE, nw.F_DIFF_FLUX_COEFF, 0, 1)
+        else:
+            _rebuild_permuted(nw, ctx, P, top, 24)
+        # the sample recorded before the rebuild, on the same matrix
+        l1, g1 = net.loss_grad(0)
+        assert l1 == l0 and np.array_equal(g1, g0)
+        _solved_pair(nw, ctx, A, 23)
+        net.record()
+        fresh = nw.Mpnn(ctx, H, L, seed=3)
+        fresh.params = p
+        fresh.record()
+        lf, gf = fresh.loss_grad(0)
+        ln, gn = net.loss_grad(net.samples - 1)
+        assert ln == lf and np.array_equal(gn, gf)
+        x_n, s_n = net.predict()
+        x_f, s_f = fresh.predict()
+        np.testing.assert_array_equal(x_n, x_f)
+        np.testing.assert_array_equal(x_n, x_a)
+        fresh.close()
+
+
 def test_learned_guess_beats_previous_step_guess(nw, ctx):
     """Train on solves of steps 0..29 of a time-evolving flow, then compare
     steps 30..37 started from the previous-step solution with and without

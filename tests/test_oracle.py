@@ -697,3 +697,51 @@ def test_mdot_alg_reference_gold():
                            np.full(n, g["pressure"]), np.ones(n), areav, 1.0,
                            g["interp_together"])
     np.testing.assert_allclose(md, g["mdot"], rtol=0, atol=g["tol"])
+
+
+def test_hypre_skipped_rows_semantics():
+    """The hypre backend's skipped rows (HypreLinearSystem.C:2190-2195,
+    2326, 1392, 2446-2455): with the check on, the edge assembly leaves the
+    rows of skipped nodes at zero and every other row equal to the Tpetra
+    assembly; applyDirichletBCs writes diagonal 1 and rhs = bc - sol there;
+    a row reset lifts the check, zeroSystem turns it back on."""
+    coords, conn, edges, areav, dv = _skewed_box()
+    n = coords.shape[0]
+    rng = np.random.default_rng(5)
+    vel = rng.normal(size=(n, 3))
+    args = (n, edges, coords, vel, rng.normal(size=(n, 3)), rng.uniform(0.5, 2, n),
+            rng.normal(size=n), rng.uniform(0.5, 2, n), areav)
+    skipped = np.array([0, 3, n - 1], np.int32)
+    full = orc.CsrSystem(n, edges, 1)
+    orc.assemble_continuity(CONT_OPTS, *args, csr=full)
+    H = orc.CsrSystem(n, edges, 1)
+    H.set_skipped_rows(skipped)
+    orc.assemble_continuity(CONT_OPTS, *args, csr=H)
+    rows = np.repeat(np.arange(n), np.diff(H.row_ptr))
+    sk = np.isin(rows, skipped)
+    assert np.all(H.vals[sk] == 0) and np.all(H.rhs[skipped] == 0)
+    np.testing.assert_array_equal(H.vals[~sk], full.vals[~sk])
+    keep = np.ones(n, bool)
+    keep[skipped] = False
+    np.testing.assert_array_equal(H.rhs[keep], full.rhs[keep])
+    sol, bc = rng.normal(size=n), rng.normal(size=n)
+    H.apply_dirichlet_hypre(skipped, sol, bc)
+    diag = sk & (H.cols == rows)
+    assert np.all(H.vals[diag] == 1.0) and np.all(H.vals[sk & ~diag] == 0)
+    np.testing.assert_array_equal(H.rhs[skipped], bc[skipped] - sol[skipped])
+    # a block touching a skipped node: its row is left alone, the other row
+    # takes its share
+    e = edges[np.where((edges[:, 0] == 3) | (edges[:, 1] == 3))[0][0]]
+    before = H.vals.copy(), H.rhs.copy()
+    H.sum_into(e, np.array([[1.0, 2.0], [3.0, 4.0]]), np.array([5.0, 6.0]))
+    i3 = 0 if e[0] == 3 else 1
+    assert H.rhs[3] == before[1][3] and H.rhs[e[1 - i3]] == before[1][e[1 - i3]] + [5.0, 6.0][1 - i3]
+    # after a reset the check is off: the skipped row takes contributions
+    H.reset_rows(np.array([10], np.int32), 0, 1, 1.0, 0.0)
+    r3 = H.rhs[3]
+    H.sum_into(e, np.array([[1.0, 2.0], [3.0, 4.0]]), np.array([5.0, 6.0]))
+    assert H.rhs[3] == r3 + [5.0, 6.0][i3]
+    # zeroSystem: check on again
+    H.zero()
+    orc.assemble_continuity(CONT_OPTS, *args, csr=H)
+    assert np.all(H.vals[sk] == 0)
