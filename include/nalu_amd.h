@@ -62,6 +62,7 @@ enum {
   NW_E_PECLET_FACTOR,   /* 1  "peclet_factor" (edge field) */
   NW_E_PECLET_NUMBER,   /* 1  "peclet_number" (edge field) */
   NW_F_PRESSURE_BC,     /* 1  "pressure_bc" (Dirichlet values of pressure) */
+  NW_F_UTMP,            /* 3  "uTmp" (the dp/dx saved by the projection) */
   NW_NUM_FIELDS
 };
 
@@ -129,6 +130,12 @@ int nw_field_fill(nw_ctx* ctx, int field, double value);
 /* Device pointer of the field (allocated on first use), [n][ncomp] like
  * the host arrays. */
 int nw_field_device_ptr(nw_ctx* ctx, int field, double** dptr);
+/* The coordinates were written through their device pointer (a moving
+ * mesh): the row kernels' locality order is rebuilt from them on the next
+ * call (an upload of NW_F_COORDINATES does this by itself).  Results never
+ * depend on that order, only the speed of the row kernels does.  No
+ * counterpart in the reference, whose edge loops keep no locality order. */
+int nw_mesh_coordinates_changed(nw_ctx* ctx);
 int nw_field_components(int field);
 
 /* ---- linear system (LinearSystem / TpetraLinearSystem) -------------------
@@ -240,11 +247,35 @@ int nw_compute_mdot(nw_ctx* ctx, const nw_mdot_opts* opts);
  * (+ for n = L, - for n = R), q_ip = (q_L + q_R)/2, V the dual volume
  * (NW_F_DUAL_VOLUME).  field_in with c components (PRESSURE, VELOCITY,
  * SCALAR_Q), field_out with 3c (DPDX, DUDX, DQDX), component 3 j + i =
- * d q_j / d x_i; overwritten.  One pass over the rows of the context's
+ * d q_j / d x_i; overwritten (accumulate = 0) or added to (accumulate = 1:
+ * the reference's driver zeroes the field and every interior and boundary
+ * algorithm adds its part, so an edge gradient that runs after boundary
+ * contributions must accumulate).  One pass over the rows of the context's
  * edge graph (built by nw_linsys_build_edge_graph): each node's sum is
  * formed by one lane in slot order, so the result is bitwise reproducible.
  * Boundary-face contributions (NodalGradBndryElemAlg) are not built. */
-int nw_nodal_grad(nw_ctx* ctx, int field_in, int field_out);
+int nw_nodal_grad(nw_ctx* ctx, int field_in, int field_out, int accumulate);
+
+/* ---- the projection step around the pressure solve ----------------------
+ * EquationSystem::solution_update (src/EquationSystem.C:556-573):
+ *   field = delta_frac * delta + field_frac * field
+ * over all nodes (or edges) and components of `field`; delta is a field of
+ * the same shape, or NW_LINSYS_SOLUTION for the context's linear-system
+ * solution (the reference's pTmp / uTmp increment fields), which needs a
+ * system of as many dofs as `field` has components.  The pressure update
+ * after the continuity solve (src/LowMachEquationSystem.C:800-804) is
+ * nw_solution_update(ctx, 1, NW_LINSYS_SOLUTION, 1, NW_F_PRESSURE). */
+enum { NW_LINSYS_SOLUTION = -1 };
+int nw_solution_update(nw_ctx* ctx, double delta_frac, int delta,
+                       double field_frac, int field);
+/* LowMachEquationSystem::project_nodal_velocity (src/LowMachEquationSystem.C:
+ * 882-939): saves dp/dx into NW_F_UTMP, recomputes NW_F_DPDX from
+ * NW_F_PRESSURE by the edge nodal gradient (nw_nodal_grad, which needs the
+ * edge graph and NW_F_DUAL_VOLUME), then projects the velocity at every
+ * node: u -= (dp/dx_new - dp/dx_old) / (rho * Udiag) with NW_F_DENSITY and
+ * NW_F_UDIAG.  (The reference's not-projected parts and directions are
+ * boundary options, not built.) */
+int nw_project_nodal_velocity(nw_ctx* ctx);
 
 /* ---- geometry (GeometryInteriorAlg, src/ngp_algorithms/
  * GeometryInteriorAlg.C:70-225) ---------------------------------------------

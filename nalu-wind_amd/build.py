@@ -17,6 +17,7 @@ ROOT = os.path.dirname(HERE)
 INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(HERE, "build")
 LIB = os.path.join(HERE, "lib", "libnaluwind_amd.so")
+STREAM_LIB = os.path.join(HERE, "lib", "libnw_stream.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -61,6 +62,11 @@ def build(jobs=None, verbose=False):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] +
          objs + ["-lgomp"])
     os.replace(tmp, LIB)
+    # the streaming-copy probe the benchmark measures the HBM rate with
+    stmp = STREAM_LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", stmp,
+                                os.path.join(HERE, "stream", "nw_stream.hip")])
+    os.replace(stmp, STREAM_LIB)
     # C++ adapter test driver (mirrors the reference unit tests)
     test_src = os.path.join(ROOT, "tests", "cpp", "unit_edge_kernels.cpp")
     if os.path.exists(test_src):

@@ -32,6 +32,7 @@ int field_components(int f)
   case NW_F_DPDX:
   case NW_F_DQDX:
   case NW_F_BC_VALUE:
+  case NW_F_UTMP:
   case NW_E_AREA_VECTOR:
     return 3;
   case NW_F_DUDX:
@@ -256,8 +257,14 @@ int nw_mesh_set(nw_ctx* c, int64_t nnode, int64_t nedge,
 int nw_field_device_ptr(nw_ctx* c, int field, double** dptr)
 {
   NW_REQUIRE(c && dptr, NW_E_ARG, "null argument");
-  if (field == NW_F_COORDINATES) c->g.sched_valid = false;
   return ensure_field(c, field, dptr);
+}
+
+int nw_mesh_coordinates_changed(nw_ctx* c)
+{
+  NW_REQUIRE(c, NW_E_ARG, "null context");
+  c->g.sched_valid = false;
+  return NW_OK;
 }
 
 int nw_field_upload(nw_ctx* c, int field, const double* host, int64_t count)

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_nodal_grad(
   const int32_t* __restrict__ cols, const SchedChunk* __restrict__ sched,
   int nchunks, const double* __restrict__ q, const double* __restrict__ areav,
   const double* __restrict__ dualvol, double* __restrict__ grad, int64_t N,
-  int64_t E)
+  int64_t E, int accumulate)
 {
   constexpr int WPB = kBlock / kWave;
   constexpr int NG = 3 * C;  // gradient components per node
@@ -141,7 +141,10 @@ __global__ __launch_bounds__(kBlock) void k_nodal_grad(
   wave_lds_sync();
   const int nv = (ch.r1 - ch.r0) * NG;
   double* dst = grad + (int64_t)ch.r0 * NG;
-  for (int i = lane; i < nv; i += kWave) dst[i] = out_s[wv][i];
+  if (accumulate)
+    for (int i = lane; i < nv; i += kWave) dst[i] += out_s[wv][i];
+  else
+    for (int i = lane; i < nv; i += kWave) dst[i] = out_s[wv][i];
 }
 
 }  // namespace
@@ -187,7 +190,7 @@ int nw_compute_mdot(nw_ctx* c, const nw_mdot_opts* o)
   return NW_OK;
 }
 
-int nw_nodal_grad(nw_ctx* c, int fin, int fout)
+int nw_nodal_grad(nw_ctx* c, int fin, int fout, int accumulate)
 {
   NW_REQUIRE(c, NW_E_ARG, "null context");
   NW_REQUIRE(fin >= 0 && fin < NW_NUM_FIELDS && fout >= 0 && fout < NW_NUM_FIELDS &&
@@ -216,7 +219,7 @@ int nw_nodal_grad(nw_ctx* c, int fin, int fout)
 #define NW_GRAD(CC, SC)                                                       \
   k_nodal_grad<CC, 4, SC><<<nb, kBlock, 0, c->stream>>>(                      \
     g.row_ptr, g.code, g.cols, g.sched, (int)g.nchunks, q, areav, dualvol,    \
-    grad, c->nnode, c->nedge)
+    grad, c->nnode, c->nedge, accumulate)
   if (C == 1 && small)
     NW_GRAD(1, kGradSlotsSmall);
   else if (C == 1)

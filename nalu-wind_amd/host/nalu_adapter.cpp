@@ -315,7 +315,8 @@ void MdotEdgeAlg::execute()
 
 void NodalGradEdgeAlg::execute()
 {
-  check(nw_nodal_grad(realm_.ctx(), phi_, gradPhi_), "NodalGradEdgeAlg");
+  check(nw_nodal_grad(realm_.ctx(), phi_, gradPhi_, accumulate_ ? 1 : 0),
+        "NodalGradEdgeAlg");
 }
 
 void DirichletBC::initialize_connectivity()

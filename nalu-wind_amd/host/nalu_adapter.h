@@ -304,11 +304,14 @@ class MdotEdgeAlg {
 // NodalGradEdgeAlg (src/ngp_algorithms/NodalGradEdgeAlg.C:58-111): the
 // interior-edge nodal gradient of node field `phi` into `gradPhi`
 // (NW_F_PRESSURE -> NW_F_DPDX, NW_F_VELOCITY -> NW_F_DUDX, ...); needs the
-// context's edge graph (built by the linear system).
+// context's edge graph (built by the linear system).  It overwrites
+// gradPhi unless `accumulate`: with boundary-face contributions added
+// before it (the reference driver zeroes the field and every algorithm
+// adds), construct it with accumulate = true.
 class NodalGradEdgeAlg {
  public:
-  NodalGradEdgeAlg(Realm& realm, int phi, int gradPhi)
-    : realm_(realm), phi_(phi), gradPhi_(gradPhi)
+  NodalGradEdgeAlg(Realm& realm, int phi, int gradPhi, bool accumulate = false)
+    : realm_(realm), phi_(phi), gradPhi_(gradPhi), accumulate_(accumulate)
   {
   }
   void execute();
@@ -316,6 +319,7 @@ class NodalGradEdgeAlg {
  private:
   Realm& realm_;
   int phi_, gradPhi_;
+  bool accumulate_;
 };
 
 // DirichletBC (src/DirichletBC.C): its nodes' rows become

@@ -30,7 +30,8 @@ OK, E_ARG, E_STATE, E_HIP, E_NOGRAPHCOL, E_NONFINITE, E_NOCONV, E_ALLOC = (
 (F_COORDINATES, F_VELOCITY, F_DENSITY, F_PRESSURE, F_DPDX, F_UDIAG,
  F_VISCOSITY, F_DUDX, F_WALL_MASK, F_SCALAR_Q, F_DQDX, F_DIFF_FLUX_COEFF,
  F_DUAL_VOLUME, F_BC_VALUE, E_AREA_VECTOR, E_MASS_FLOW_RATE, E_PECLET_FACTOR,
- E_PECLET_NUMBER, F_PRESSURE_BC) = range(19)
+ E_PECLET_NUMBER, F_PRESSURE_BC, F_UTMP) = range(20)
+LINSYS_SOLUTION = -1  # nw_solution_update: the linear system's solution
 
 PREC_NONE, PREC_JACOBI, PREC_CHEBYSHEV, PREC_AMG = 0, 1, 2, 3
 SOLVE_FP64, SOLVE_FP32_REFINE = 0, 1
@@ -148,6 +149,7 @@ _sig = {
     "nw_field_download": ([_P, C.c_int, _f64p, C.c_int64], C.c_int),
     "nw_field_fill": ([_P, C.c_int, C.c_double], C.c_int),
     "nw_field_device_ptr": ([_P, C.c_int, C.POINTER(_P)], C.c_int),
+    "nw_mesh_coordinates_changed": ([_P], C.c_int),
     "nw_field_components": ([C.c_int], C.c_int),
     "nw_linsys_build_edge_graph": ([_P, C.c_int], C.c_int),
     "nw_linsys_zero": ([_P], C.c_int),
@@ -170,7 +172,9 @@ _sig = {
     "nw_assemble_momentum_with_peclet": ([_P, C.POINTER(MomentumOpts),
                                           C.POINTER(Peclet)], C.c_int),
     "nw_compute_mdot": ([_P, C.POINTER(MdotOpts)], C.c_int),
-    "nw_nodal_grad": ([_P, C.c_int, C.c_int], C.c_int),
+    "nw_nodal_grad": ([_P, C.c_int, C.c_int, C.c_int], C.c_int),
+    "nw_solution_update": ([_P, C.c_double, C.c_int, C.c_double, C.c_int], C.c_int),
+    "nw_project_nodal_velocity": ([_P], C.c_int),
     "nw_mesh_elements": ([_P, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
     "nw_compute_geometry": ([_P], C.c_int),
     "nw_linsys_solve": ([_P, C.POINTER(SolverCfg), C.POINTER(SolveStats)], C.c_int),
@@ -470,9 +474,19 @@ class Context:
         """MdotEdgeAlg: NW_E_MASS_FLOW_RATE from the node fields."""
         check(lib.nw_compute_mdot(self.h, C.byref(MdotOpts(noc, interp_together))))
 
-    def nodal_grad(self, field_in, field_out):
-        """NodalGradEdgeAlg: field_out = edge-based gradient of field_in."""
-        check(lib.nw_nodal_grad(self.h, field_in, field_out))
+    def nodal_grad(self, field_in, field_out, accumulate=0):
+        """NodalGradEdgeAlg: field_out = (or +=, with accumulate) the
+        edge-based gradient of field_in."""
+        check(lib.nw_nodal_grad(self.h, field_in, field_out, int(accumulate)))
+
+    def solution_update(self, delta_frac, delta, field_frac, field):
+        """EquationSystem::solution_update: field = delta_frac * delta +
+        field_frac * field (delta = LINSYS_SOLUTION: the solved increment)."""
+        check(lib.nw_solution_update(self.h, delta_frac, delta, field_frac, field))
+
+    def project_nodal_velocity(self):
+        """LowMachEquationSystem::project_nodal_velocity."""
+        check(lib.nw_project_nodal_velocity(self.h))
 
     def mesh_elements(self, types, conn):
         """The mesh's elements for the device geometry: types [nelem]

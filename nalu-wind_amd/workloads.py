@@ -142,7 +142,7 @@ def continuity_rf(n=256, seed=1234, stretch=None, dt=1.0e-3, t=0.0,
         nw.F_VELOCITY: vel, nw.F_DENSITY: np.ones(W.n),
         nw.F_PRESSURE: np.zeros(W.n), nw.F_DPDX: np.zeros((W.n, 3)),
         nw.F_UDIAG: np.full(W.n, 1.0 / dt),
-        nw.F_DIFF_FLUX_COEFF: np.zeros(W.n),
+        nw.F_PRESSURE_BC: np.zeros(W.n),
     }
     W.opts = nw.continuity_defaults(dt=dt, gamma1=1.0)
     W.top = np.where(W.coords[:, 2] >= 1.0 - 1e-12)[0].astype(np.int32)
@@ -181,7 +181,7 @@ def continuity_mixed(n=160, seed=1234, dt=1.0e-3, f_hex=0.6, f_tet=0.3,
         nw.F_VELOCITY: vel, nw.F_DENSITY: np.ones(W.n),
         nw.F_PRESSURE: np.zeros(W.n), nw.F_DPDX: np.zeros((W.n, 3)),
         nw.F_UDIAG: np.full(W.n, 1.0 / dt),
-        nw.F_DIFF_FLUX_COEFF: np.zeros(W.n),
+        nw.F_PRESSURE_BC: np.zeros(W.n),
     }
     W.opts = nw.continuity_defaults(dt=dt, gamma1=1.0)
     W.top = np.where(W.coords[:, 2] >= 1.0 - 1e-12)[0].astype(np.int32)
@@ -190,16 +190,30 @@ def continuity_mixed(n=160, seed=1234, dt=1.0e-3, f_hex=0.6, f_tet=0.3,
 
 
 def apply_pressure_bc(ctx, W):
-    """Dirichlet p = 0 on the top face (solution field = bc field = 0)."""
-    ctx.apply_dirichlet(W.top, nw.F_PRESSURE, nw.F_DIFF_FLUX_COEFF, 0, 1)
+    """Dirichlet p = 0 on the top face: rows of the top nodes become the
+    identity with rhs = pressure_bc - p (pressure_bc = 0)."""
+    ctx.apply_dirichlet(W.top, nw.F_PRESSURE, nw.F_PRESSURE_BC, 0, 1)
 
 
 class PressureSeries:
-    """Time-evolving pressure systems of the continuity_rf workload: step k
-    assembles the continuity system for the Fourier velocity at
-    t = k * dt_flow (same mesh, same seed, phases advancing with time) and
-    applies the top-face Dirichlet condition.  The matrix is the same at
-    every step (a static mesh); only the right-hand side changes."""
+    """The pressure-projection step of the low-Mach algorithm
+    (LowMachEquationSystem::solve_and_update, src/LowMachEquationSystem.C:
+    793-819) on the continuity_rf box, one outer iteration per time step:
+
+      step(k)   the momentum predictor: the velocity of the seeded random
+                Fourier flow at t = k dt_flow (generated on the device);
+                then the continuity assembly with the current pressure and
+                pressure gradient, and Dirichlet rows on the top face
+                (dp = pressure_bc - p = 0 there);
+      the caller solves for the pressure increment dp;
+      update()  p += dp (solution_update), mdot from the new pressure
+                (MdotEdgeAlg) and the nodal velocity projection, which
+                recomputes dp/dx from the new pressure.
+
+    p and dp/dx start at zero, so step 0 solves for the whole pressure and
+    later steps for the corrections the changing flow and the updated
+    pressure gradient ask for.  The matrix is the same at every step (a
+    static mesh with Udiag = gamma1/dt); the right-hand side is not."""
 
     def __init__(self, ctx, n, seed=1234, stretch=None, dt_flow=0.02):
         self.ctx = ctx
@@ -211,8 +225,14 @@ class PressureSeries:
         self.modes = fourier_modes(seed)
 
     def step(self, k):
-        """Velocity at t = k dt_flow (generated on the device), continuity
-        assembly, Dirichlet rows."""
+        """Predictor velocity at t = k dt_flow, continuity assembly with the
+        current p and dp/dx, Dirichlet rows."""
         self.ctx.fourier_velocity(self.modes, k * self.dt_flow)
         self.ctx.assemble_continuity(self.W.opts)
         apply_pressure_bc(self.ctx, self.W)
+
+    def update(self):
+        """After the solve: p += dp, mdot, velocity projection (dp/dx)."""
+        self.ctx.solution_update(1.0, nw.LINSYS_SOLUTION, 1.0, nw.F_PRESSURE)
+        self.ctx.compute_mdot(1.0, 1.0)
+        self.ctx.project_nodal_velocity()

@@ -282,6 +282,22 @@ def mdot_edge_alg(edges, coords, vel, dpdx, rho, p, udiag, areav, noc=1.0,
     return out
 
 
+def solution_update(delta_frac, delta, field_frac, field):
+    """EquationSystem::solution_update (src/EquationSystem.C:556-573,
+    nalu_ngp::field_axpby): field_frac * field + delta_frac * delta."""
+    return delta_frac * f64(delta) + field_frac * f64(field)
+
+
+def project_nodal_velocity(edges, areav, dualvol, p, dpdx_old, vel, rho, udiag):
+    """LowMachEquationSystem::project_nodal_velocity (src/LowMachEquation
+    System.C:882-939): dp/dx of the new pressure by the edge nodal gradient
+    (:913), then u -= (dp/dx_new - dp/dx_old) / (rho Udiag) at every node
+    (:930-939).  Returns (dpdx_new, vel_new)."""
+    g = nodal_grad_edge(edges, areav, dualvol, p)
+    fac = 1.0 / (f64(rho) * f64(udiag))
+    return g, f64(vel) - fac[:, None] * (g - f64(dpdx_old))
+
+
 def momentum_edge_peclet(edges, coords, vel, rho, visc, peclet,
                          with_number=False):
     """MomentumEdgePecletAlg: the edge Peclet factor, and with with_number

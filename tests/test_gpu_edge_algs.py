@@ -68,6 +68,14 @@ def test_nodal_grad_matches_oracle(nw, ctx, case):
         ref = orc.nodal_grad_edge(P.edges, P.areav, P.dualvol, val)
         np.testing.assert_allclose(got.reshape(ref.shape), ref, rtol=0,
                                    atol=1e-12 * np.abs(ref).max())
+        # accumulate: added to what the field holds (a boundary part, here
+        # random), as the reference's algorithms add into a zeroed field
+        g0 = np.random.default_rng(1).normal(size=ref.shape)
+        ctx.field_upload(fout, g0)
+        ctx.nodal_grad(fin, fout, accumulate=1)
+        np.testing.assert_allclose(ctx.field_download(fout).reshape(ref.shape),
+                                   g0 + ref, rtol=0,
+                                   atol=1e-12 * np.abs(ref).max() + 1e-15)
 
 
 def test_nodal_grad_exact_for_linear_field_and_reproducible(nw, ctx):
