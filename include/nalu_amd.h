@@ -170,6 +170,21 @@ int nw_linsys_device_arrays(nw_ctx* ctx, int32_t** row_ptr, int32_t** cols,
 int nw_linsys_sum_into(nw_ctx* ctx, int nblocks, int nent,
                        const int32_t* nodes, const double* lhs,
                        const double* rhs);
+/* The same scatter on device arrays (the device-resident CoeffApplier,
+ * include/LinearSystem.h:53-70, src/TpetraLinearSystem.C:1381-1459, as the
+ * reference's device algorithms call it per entity): d_nodes
+ * [nblocks][nent], d_lhs [nblocks][(nent*ndof)^2], d_rhs [nblocks][nent*ndof]
+ * on the device, stream-ordered on the context's stream.  Deterministic:
+ * the contributions to each entry are summed in block order and the result
+ * equals nw_linsys_sum_into's bitwise.  A column absent from the graph
+ * (NW_E_NOGRAPHCOL) or a node id out of range (NW_E_ARG) is found on the
+ * device before anything is applied, and nothing of that call is applied
+ * (the call synchronises the stream once to learn this).  Its scratch is a
+ * context buffer that is allocated only when a call needs more than every
+ * call before it; nent <= 64. */
+int nw_linsys_sum_into_device(nw_ctx* ctx, int nblocks, int nent,
+                              const int32_t* d_nodes, const double* d_lhs,
+                              const double* d_rhs);
 /* CoeffApplier::resetRows / LinearSystem::resetRows
  * (src/TpetraLinearSystem.C:1461-1520, 1759-1786). Host node list. */
 int nw_linsys_reset_rows(nw_ctx* ctx, int64_t nnodes, const int32_t* nodes,

@@ -189,6 +189,7 @@ int nw_ctx_destroy(nw_ctx* c)
   (void)hipFree(c->x_hist2);
   free_graph(c->g);
   geom_free(c->geom);
+  free_sum_scratch(c);
   for (auto& f : c->field) (void)hipFree(f);
   (void)hipFree(c->edges);
   (void)hipStreamDestroy(c->stream);

@@ -170,7 +170,11 @@ struct Ctx {
   double* x_hist2 = nullptr; // [nnode] and the one before that
   Mpnn* mpnn = nullptr;      // network for NW_X0_LEARNED (not owned)
   Geom* geom = nullptr;      // elements for the device geometry
+  void* sum_buf = nullptr;   // scratch of nw_linsys_sum_into_device
+  size_t sum_cap = 0;        // its size in bytes (grows, never shrinks)
 };
+
+void free_sum_scratch(Ctx* c);
 
 int field_components(int f);
 bool field_is_edge(int f);

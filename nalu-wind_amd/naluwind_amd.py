@@ -165,6 +165,7 @@ _sig = {
                                    C.c_int, C.c_int], C.c_int),
     "nw_fix_pressure_at_node": ([_P, C.c_int32, C.c_double], C.c_int),
     "nw_linsys_set_skipped_rows": ([_P, C.c_int64, _i32p], C.c_int),
+    "nw_linsys_sum_into_device": ([_P, C.c_int, C.c_int, _P, _P, _P], C.c_int),
     "nw_assemble_scalar": ([_P, C.POINTER(ScalarOpts)], C.c_int),
     "nw_assemble_momentum": ([_P, C.POINTER(MomentumOpts)], C.c_int),
     "nw_assemble_continuity": ([_P, C.POINTER(ContinuityOpts)], C.c_int),
@@ -334,6 +335,55 @@ def mdot_edge(edges, areav, rho, vel):
 
 # ------------------------------------------------------------------ context
 
+_hip = None
+
+
+def _hip_rt():
+    """The HIP runtime the library links (device buffers for the calls that
+    take device arrays)."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so.7")
+        _hip.hipMalloc.argtypes = [C.POINTER(_P), C.c_size_t]
+        _hip.hipFree.argtypes = [_P]
+        _hip.hipMemcpy.argtypes = [_P, _P, C.c_size_t, C.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
+    return _hip
+
+
+class DeviceBuffer:
+    """A device copy of a numpy array (hipMalloc + hipMemcpy)."""
+
+    def __init__(self, arr):
+        self.arr = np.ascontiguousarray(arr)
+        self.ptr = _P()
+        rt = _hip_rt()
+        nb = max(self.arr.nbytes, 1)
+        if rt.hipMalloc(C.byref(self.ptr), nb) != 0:
+            raise NwError(E_ALLOC, "hipMalloc of %d bytes failed" % nb)
+        self.ptr = self.ptr.value
+        if self.arr.nbytes:
+            assert rt.hipMemcpy(self.ptr, self.arr.ctypes.data, self.arr.nbytes, 1) == 0
+
+    def download(self):
+        out = np.empty_like(self.arr)
+        _hip_rt().hipDeviceSynchronize()
+        if out.nbytes:
+            assert _hip_rt().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0
+        return out
+
+    def free(self):
+        if self.ptr:
+            _hip_rt().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Context:
     """One GPU context: mesh, fields, one linear system, solver workspace."""
 
@@ -434,6 +484,12 @@ class Context:
         nent = nodes.shape[-1]
         check(lib.nw_linsys_sum_into(self.h, nb, nent, nodes.ravel(),
                                      _f64(lhs).ravel(), _f64(rhs).ravel()))
+
+    def sum_into_device(self, d_nodes, d_lhs, d_rhs, nblocks, nent):
+        """sumInto of device arrays (DeviceBuffer or raw pointers)."""
+        ptr = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+        check(lib.nw_linsys_sum_into_device(self.h, nblocks, nent, ptr(d_nodes),
+                                            ptr(d_lhs), ptr(d_rhs)))
 
     def reset_rows(self, nodes, begin, end, diag=0.0, rhs_residual=0.0):
         nodes = _i32(nodes)

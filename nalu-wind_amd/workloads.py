@@ -215,14 +215,22 @@ class PressureSeries:
     pressure gradient ask for.  The matrix is the same at every step (a
     static mesh with Udiag = gamma1/dt); the right-hand side is not."""
 
-    def __init__(self, ctx, n, seed=1234, stretch=None, dt_flow=0.02):
+    def __init__(self, ctx, n=256, seed=1234, stretch=None, dt_flow=0.02, W=None):
+        """A new continuity_rf(n, seed, stretch) box loaded into ctx, or a
+        continuity_rf workload W that ctx already holds with its 1-dof
+        graph (its pressure and pressure gradient are reset to zero)."""
         self.ctx = ctx
-        self.W = continuity_rf(n, seed, stretch)
-        self.W.upload(ctx)
-        ctx.build_graph(1)
+        if W is None:
+            W = continuity_rf(n, seed, stretch)
+            W.upload(ctx)
+            ctx.build_graph(1)
+        else:
+            ctx.field_fill(nw.F_PRESSURE, 0.0)
+            ctx.field_fill(nw.F_DPDX, 0.0)
+        self.W = W
         self.dt_flow = dt_flow
-        self.seed = seed
-        self.modes = fourier_modes(seed)
+        self.seed = W.seed
+        self.modes = fourier_modes(W.seed)
 
     def step(self, k):
         """Predictor velocity at t = k dt_flow, continuity assembly with the

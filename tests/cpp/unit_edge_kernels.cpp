@@ -406,7 +406,7 @@ void test_two_systems_one_realm()
   print_array("pressure", p);
   print_array("dpdx", dp);
   print_array("density", rho);
-  print_array("scalar", q);
+  print_array("scalar_q", q);
   print_array("diff_flux_coeff", gam);
   print_array("mdot", mdot);
   std::printf("\"edges\": [");

@@ -115,7 +115,7 @@ def _oracle_two_systems(t):
     A.apply_dirichlet(top, 0, 1, p, np.zeros(n))
     S = orc.CsrSystem(n, edges, 1)
     orc.assemble_scalar(orc.ScalarOpts(0.3, 1.0, 1.0, 1.0, 0, orc.Peclet(0, 1.0, 2.0, 4.0)),
-                        n, edges, coords, vel, np.array(t["scalar"]), np.zeros((n, 3)),
+                        n, edges, coords, vel, np.array(t["scalar_q"]), np.zeros((n, 3)),
                         rho, np.array(t["diff_flux_coeff"]), areav, np.array(t["mdot"]),
                         csr=S)
     return A, S

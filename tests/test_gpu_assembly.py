@@ -582,3 +582,92 @@ def test_momentum_extract_diagonal(nw, ctx, skewed):
     ctx.field_upload(nw.F_UDIAG, u0)
     ctx.assemble_momentum_with_peclet(o, pf)
     np.testing.assert_allclose(ctx.field_download(nw.F_UDIAG), u2, rtol=1e-13, atol=0)
+
+
+# ------------------------------------------------------------ device sumInto
+
+@pytest.mark.parametrize("ndof", [1, 3])
+def test_device_sum_into_matches_host_and_oracle(nw, skewed, ndof):
+    """nw_linsys_sum_into_device (the device-resident CoeffApplier): many
+    edge and self blocks, edges repeated so that entries take several
+    contributions, from device arrays.  Two contexts hold the same
+    assembled system; one takes each round of blocks through the host
+    sumInto, the other through the device one: the systems stay bitwise
+    equal (same block-order sums) and equal to the oracle's.  Rounds of
+    decreasing size reuse the scratch.  A block whose column is not in the
+    graph, or whose node id is out of range, is an error that applies
+    nothing."""
+    P = skewed
+    rng = np.random.default_rng(40 + ndof)
+    o = nw.continuity_defaults() if ndof == 1 else nw.momentum_defaults(alpha=0.2)
+    ctxs = [nw.Context(0), nw.Context(0)]
+    try:
+        for c in ctxs:
+            P.upload(c)
+            c.build_graph(ndof)
+            if ndof == 1:
+                c.assemble_continuity(o)
+            else:
+                c.assemble_momentum(o)
+        A = (P.oracle_continuity(pb.to_orc_continuity(o)) if ndof == 1 else
+             P.oracle_momentum(pb.to_orc_momentum(o)))
+        for nb, nent in ((3000, 2), (500, 1), (40, 2)):
+            if nent == 2:
+                nodes = P.edges[rng.integers(0, P.ne, nb)].copy()
+                nodes[::2] = nodes[::2, ::-1]   # both orientations
+            else:
+                nodes = rng.integers(0, P.n, (nb, 1)).astype(np.int32)
+            nr = nent * ndof
+            lhs = rng.normal(size=(nb, nr, nr))
+            rhs = rng.normal(size=(nb, nr))
+            ctxs[0].sum_into(nodes, lhs, rhs)
+            bufs = [nw.DeviceBuffer(a) for a in (nodes.astype(np.int32), lhs, rhs)]
+            ctxs[1].sum_into_device(*bufs, nb, nent)
+            host, dev = ctxs[0].download(), ctxs[1].download()
+            for k in range(4):
+                assert np.array_equal(dev[k], host[k]), k
+            for b in range(nb):
+                A.sum_into(nodes[b], lhs[b], rhs[b])
+            pb.assert_csr_close(dev, A)
+            for d in bufs:
+                d.free()
+        c = ctxs[1]
+        lhs = nw.DeviceBuffer(np.ones((2, 2 * ndof, 2 * ndof)))
+        rhs = nw.DeviceBuffer(np.ones((2, 2 * ndof)))
+        before = c.download()
+        far = nw.DeviceBuffer(np.array([[0, 1], [0, P.n - 1]], np.int32))
+        with pytest.raises(nw.NwError) as ei:
+            c.sum_into_device(far, lhs, rhs, 2, 2)
+        assert ei.value.code == nw.E_NOGRAPHCOL
+        bad = nw.DeviceBuffer(np.array([[0, P.n]], np.int32))
+        with pytest.raises(nw.NwError) as ei:
+            c.sum_into_device(bad, lhs, rhs, 1, 2)
+        assert ei.value.code == nw.E_ARG
+        after = c.download()
+        assert all(np.array_equal(x, y) for x, y in zip(before, after))
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+def test_device_sum_into_skips_hypre_rows(nw, ctx, skewed):
+    """With skipped rows set (and their check on) the device sumInto drops
+    their contributions, as the host one and the oracle do."""
+    P = skewed
+    P.upload(ctx)
+    ctx.build_graph(1)
+    skipped = np.array([0, 5, 17, P.n - 1], np.int32)
+    ctx.set_skipped_rows(skipped)
+    A = orc.CsrSystem(P.n, P.edges, 1)
+    A.set_skipped_rows(skipped)
+    rng = np.random.default_rng(9)
+    nodes = P.edges[np.isin(P.edges, skipped).any(1)][:50].astype(np.int32)
+    nodes = np.concatenate([nodes, P.edges[:100]]).astype(np.int32)
+    lhs = rng.normal(size=(len(nodes), 2, 2))
+    rhs = rng.normal(size=(len(nodes), 2))
+    ctx.zero()
+    ctx.sum_into_device(nw.DeviceBuffer(nodes), nw.DeviceBuffer(lhs),
+                        nw.DeviceBuffer(rhs), len(nodes), 2)
+    for b in range(len(nodes)):
+        A.sum_into(nodes[b], lhs[b], rhs[b])
+    pb.assert_csr_close(ctx.download(), A)

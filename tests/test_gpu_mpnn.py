@@ -319,7 +319,9 @@ def test_graph_rebuild_between_network_calls(nw, ctx):
     for k in range(2):
         if k == 0:  # the same edge list, the graph built again
             ctx.build_graph(1)
+            ctx.field_upload(nw.F_PRESSURE, P.p)   # the system _system made
             ctx.assemble_continuity(nw.continuity_defaults())
+            ctx.field_upload(nw.F_PRESSURE, np.zeros(P.n))
             ctx.apply_dirichlet(top, nw.F_PRESSURE, nw.F_DIFF_FLUX_COEFF, 0, 1)
         else:
             _rebuild_permuted(nw, ctx, P, top, 24)
@@ -393,3 +395,61 @@ def test_device_fourier_velocity_matches_host(nw, ctx):
     u = ctx.field_download(nw.F_VELOCITY)
     np.testing.assert_allclose(u, wl.fourier_velocity(W.coords, 5, t=0.37),
                                rtol=0, atol=1e-12)
+
+
+def _projection_pairs(nw, ctx, S, net, k0, k1, cfg):
+    """Steps k0..k1-1 of the projection series, each solved from the
+    previous increment with the plain method; pairs recorded from step 3."""
+    for k in range(k0, k1):
+        S.step(k)
+        ctx.solve(nw.solver_cfg(x0=nw.X0_SOLUTION, **cfg))
+        if k > 2:
+            net.record()
+        S.update()
+
+
+def test_trained_network_beats_untrained_prior(nw, ctx):
+    """The learned guess on the pressure-projection loop (workloads.
+    PressureSeries: the increment dp of each step): trained on the pairs of
+    steps 3..39, compared on steps 40..47 with the untrained network (its
+    prior, the quadratic extrapolation of the increments) and with the
+    previous-step increment, each from the same state and to the same
+    tolerance.  The trained network starts closer (x0 residual) and needs
+    fewer iterations than both."""
+    from workloads import PressureSeries
+    S = PressureSeries(ctx, 16, seed=77, stretch=1.05, dt_flow=0.02)
+    cfg = dict(kspace=50, max_iters=20000, tol=1e-8, precond=nw.PREC_JACOBI)
+    net = nw.Mpnn(ctx, H, L, seed=2)
+    prior = nw.Mpnn(ctx, H, L, seed=2)
+    ctx.set_solution(np.zeros(S.W.n))
+    _projection_pairs(nw, ctx, S, net, 0, 40, cfg)
+    loss = net.train(nw.mpnn_train_cfg(epochs=30, batch=1, lr=2e-3, seed=1,
+                                       holdout=4))
+    assert not np.array_equal(net.params, prior.params)
+    res = {"trained": [], "prior": [], "prev": []}
+    for k in range(40, 48):
+        x_before = ctx.solution()
+        S.step(k)
+        sols = {}
+        for name, guess, n_ in (("trained", nw.X0_LEARNED, net),
+                                ("prior", nw.X0_LEARNED, prior),
+                                ("prev", nw.X0_SOLUTION, None)):
+            ctx.set_solution(x_before)
+            ctx.set_learned_guess(n_)
+            st = ctx.solve(nw.solver_cfg(x0=guess, **cfg))
+            assert st.converged and st.final_rel <= 1e-8
+            res[name].append((st.iters, st.x0_rel, st.guess))
+            sols[name] = ctx.solution()
+        for name in ("trained", "prior"):
+            np.testing.assert_allclose(sols[name], sols["prev"], rtol=0,
+                                       atol=1e-6 * np.abs(sols["prev"]).max())
+        ctx.set_learned_guess(None)
+        S.update()   # the state advances with the last (plain) solve
+    it = {k: sum(v[0] for v in res[k]) for k in res}
+    r0 = {k: np.mean([v[1] for v in res[k]]) for k in res}
+    print("loss %.4g iterations %s x0 residual %s" % (loss, it, r0), flush=True)
+    assert all(v[2] == nw.X0_LEARNED for v in res["trained"])
+    assert r0["trained"] < 0.5 * r0["prior"] and r0["trained"] < 0.5 * r0["prev"]
+    assert it["trained"] < it["prior"] and it["trained"] < it["prev"]
+    net.close()
+    prior.close()
