@@ -185,6 +185,18 @@ int nw_linsys_sum_into(nw_ctx* ctx, int nblocks, int nent,
 int nw_linsys_sum_into_device(nw_ctx* ctx, int nblocks, int nent,
                               const int32_t* d_nodes, const double* d_lhs,
                               const double* d_rhs);
+/* No counterpart in the reference: nw_linsys_sum_into_device for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and result (bitwise); its scratch is a fixed 32 MiB device array per GPU
+ * (held until the process ends), so it allocates nothing, and a call that
+ * needs more (about 900 thousand scalar contributions, nblocks * nent *
+ * ndof * (nent * ndof + 1)) fails with NW_E_ARG.  All calls on one GPU share
+ * that array: calls that can overlap on one GPU (contexts on different
+ * streams) race and must be ordered by the caller. */
+int nw_linsys_sum_into_device_noalloc(nw_ctx* ctx, int nblocks, int nent,
+                                      const int32_t* d_nodes,
+                                      const double* d_lhs, const double* d_rhs);
 /* CoeffApplier::resetRows / LinearSystem::resetRows
  * (src/TpetraLinearSystem.C:1461-1520, 1759-1786). Host node list. */
 int nw_linsys_reset_rows(nw_ctx* ctx, int64_t nnodes, const int32_t* nodes,

@@ -165,6 +165,8 @@ int nw_ctx_create(int device, nw_ctx** out)
   NW_REQUIRE(device >= 0 && device < ndev, NW_E_ARG,
              "device %d not present (%d devices)", device, ndev);
   NW_HIP(hipSetDevice(device));
+  int rc = noalloc_scratch_init(device);
+  if (rc) return rc;
   nw_ctx* c = new nw_ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) !=

@@ -175,6 +175,9 @@ struct Ctx {
 };
 
 void free_sum_scratch(Ctx* c);
+// the fixed scratch of nw_linsys_sum_into_device_noalloc: takes its device
+// address on `device` (current) once per process
+int noalloc_scratch_init(int device);
 
 int field_components(int f);
 bool field_is_edge(int f);

@@ -19,7 +19,14 @@
 // Scratch: the pairs, their sorted copies, the sort's temporary storage
 // and a status word, in a context buffer that grows (and is allocated) only
 // when a call needs more than any call before.
+//
+// nw_linsys_sum_into_device_noalloc runs the same kernels with that scratch
+// in a fixed device array (`scratch`, kNoAllocBytes per GPU, held until the
+// process ends) and allocates nothing, for callers that cannot allocate
+// device memory.  Its address on each device is taken once, with
+// hipGetSymbolAddress, when the first context on that device is created.
 #include <cstring>
+#include <mutex>
 
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -29,6 +36,17 @@ namespace nw {
 namespace {
 
 constexpr uint32_t kDropped = 0xFFFFFFFFu;  // sorts last, never applied
+
+// The fixed scratch of the allocation-free entry point: 32 MiB per device
+// (about 900 thousand scalar contributions per call).  Every call on a
+// device shares it; calls that overlap on one device (two contexts on two
+// streams) race, so they must be ordered by the caller.
+constexpr size_t kNoAllocBytes = (size_t)32 << 20;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`
 
 // status bits
 constexpr int kBadNode = 1, kNoColumn = 2;
@@ -193,6 +211,20 @@ int check_args(Ctx* c, int nblocks, int nent, const void* nodes, const void* lhs
 
 }  // namespace
 
+int noalloc_scratch_init(int device)
+{
+  NW_REQUIRE(device >= 0 && device < kMaxDevices, NW_E_ARG,
+             "device %d beyond the fixed-scratch table", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return NW_OK;
+  // the caller made `device` current; the array's address differs per device
+  void* ptr = nullptr;
+  NW_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  NW_REQUIRE(ptr != nullptr, NW_E_HIP, "no device address for the fixed scratch");
+  g_scratch_ptr[device] = ptr;
+  return NW_OK;
+}
+
 void free_sum_scratch(Ctx* c)
 {
   (void)hipFree(c->sum_buf);
@@ -226,6 +258,31 @@ int nw_linsys_sum_into_device(nw_ctx* c, int nblocks, int nent,
   }
   return sum_into_device(c, nblocks, nent, d_nodes, d_lhs, d_rhs,
                          static_cast<char*>(c->sum_buf), l);
+}
+
+int nw_linsys_sum_into_device_noalloc(nw_ctx* c, int nblocks, int nent,
+                                      const int32_t* d_nodes, const double* d_lhs,
+                                      const double* d_rhs)
+{
+  int64_t n = 0;
+  int rc = check_args(c, nblocks, nent, d_nodes, d_lhs, d_rhs, &n);
+  if (rc) return rc;
+  if (n == 0) return NW_OK;
+  SumLayout l;
+  if ((rc = sum_layout(n, c->stream, &l))) return rc;
+  NW_REQUIRE(l.total <= kNoAllocBytes, NW_E_ARG,
+             "sumInto (no allocation): %lld contributions need %zu bytes of "
+             "scratch, more than the fixed %zu",
+             (long long)n, l.total, kNoAllocBytes);
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    base = g_scratch_ptr[c->device];
+  }
+  NW_REQUIRE(base != nullptr, NW_E_STATE,
+             "fixed scratch of device %d not set up", c->device);
+  return sum_into_device(c, nblocks, nent, d_nodes, d_lhs, d_rhs,
+                         static_cast<char*>(base), l);
 }
 
 }  // extern "C"

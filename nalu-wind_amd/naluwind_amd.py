@@ -166,6 +166,8 @@ _sig = {
     "nw_fix_pressure_at_node": ([_P, C.c_int32, C.c_double], C.c_int),
     "nw_linsys_set_skipped_rows": ([_P, C.c_int64, _i32p], C.c_int),
     "nw_linsys_sum_into_device": ([_P, C.c_int, C.c_int, _P, _P, _P], C.c_int),
+    "nw_linsys_sum_into_device_noalloc": ([_P, C.c_int, C.c_int, _P, _P, _P],
+                                          C.c_int),
     "nw_assemble_scalar": ([_P, C.POINTER(ScalarOpts)], C.c_int),
     "nw_assemble_momentum": ([_P, C.POINTER(MomentumOpts)], C.c_int),
     "nw_assemble_continuity": ([_P, C.POINTER(ContinuityOpts)], C.c_int),
@@ -490,6 +492,14 @@ class Context:
         ptr = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
         check(lib.nw_linsys_sum_into_device(self.h, nblocks, nent, ptr(d_nodes),
                                             ptr(d_lhs), ptr(d_rhs)))
+
+    def sum_into_device_noalloc(self, d_nodes, d_lhs, d_rhs, nblocks, nent):
+        """sum_into_device with the library's fixed scratch: allocates
+        nothing (for callers that cannot allocate device memory)."""
+        ptr = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+        check(lib.nw_linsys_sum_into_device_noalloc(self.h, nblocks, nent,
+                                                    ptr(d_nodes), ptr(d_lhs),
+                                                    ptr(d_rhs)))
 
     def reset_rows(self, nodes, begin, end, diag=0.0, rhs_residual=0.0):
         nodes = _i32(nodes)
