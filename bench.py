@@ -32,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "nalu-wind_amd"))
 
 HBM_PEAK_GBS = 8000.0
+LEARN_RATE = 2e-3  # Adam step of the learned guess's training
 METRIC = "Medges/s assembled + %HBM peak; pressure solve ms to 1e-8 rel. residual"
 
 
@@ -386,21 +387,30 @@ def learned_guess_series(ctx, W, ntrain, neval, epochs, log):
     t_gen = time.perf_counter() - t0
     hold = max(3, net.samples // 10)
     t0 = time.perf_counter()
-    # small steps: the head starts at the quadratic extrapolation and
-    # training adjusts it; the most recent pairs validate, and the
+    # the head starts at the quadratic extrapolation (output weights zero)
+    # and training learns how much of the third and fourth history
+    # differences to add, per node; the most recent pairs validate, and the
     # parameters that predict them best (the untrained ones included) are
     # kept
-    loss = net.train(nw.mpnn_train_cfg(epochs=epochs, batch=1, lr=1e-4, seed=3,
-                                       holdout=hold))
+    loss = net.train(nw.mpnn_train_cfg(epochs=epochs, batch=1, lr=LEARN_RATE,
+                                       seed=3, holdout=hold))
     t_train = time.perf_counter() - t0
     prior = nw.Mpnn(ctx, 64, 3, seed=7)  # untrained: quadratic extrapolation
     kept_prior = bool(np.array_equal(net.params, prior.params))
+    # a fixed rule with the same inputs for comparison: the output head set
+    # to (1, 1) by hand, no training (quartic extrapolation in time)
+    quartic = nw.Mpnn(ctx, 64, 3, seed=7)
+    pq = quartic.params
+    pq[-2:] = 1.0
+    quartic.params = pq
+    b_out = [float(v) for v in net.params[-2:]]
     res["learned_guess"] = {
         "network": "3 message-passing layers, 64 hidden; prediction in bfloat16 "
                    "weights and activations on the matrix cores with FP32 "
                    "accumulation, training in FP32",
         "pairs": net.samples, "validation_pairs": hold, "epochs": epochs,
-        "adam_lr": 1e-4, "best_validation_loss": loss,
+        "adam_lr": LEARN_RATE, "best_validation_loss": loss,
+        "trained_b_out": b_out,
         "validation_kept_untrained_parameters": kept_prior,
         "pair_generation_s": round(t_gen, 1), "pair_solve_ms_avg": round(
             sum(gen_ms) / len(gen_ms), 2), "training_s": round(t_train, 1)}
@@ -408,21 +418,24 @@ def learned_guess_series(ctx, W, ntrain, neval, epochs, log):
         (net.samples, t_gen, epochs, t_train, loss, kept_prior))
     runs = {"fp64_x0_zero": [], "fp64_prev_step": [], "fp32_refined_x0_zero": [],
             "fp32_refined_prev_step": [], "fp32_refined_untrained_net": [],
-            "fp32_refined_learned": []}
+            "fp32_refined_quartic_rule": [], "fp32_refined_learned": []}
     for k in range(ntrain + 1, ntrain + 1 + neval):
         x_prev = ctx.solution()
         S.step(k)
         # learned first: its inputs are x0 = dp(k-1) and xp = dp(k-2), the
         # guess of the previous step's last solve
         sol = {}
+        nets = {"fp32_refined_untrained_net": prior,
+                "fp32_refined_quartic_rule": quartic}
         for name, cfg in [("fp32_refined_learned", fp32(nw.X0_LEARNED)),
                           ("fp32_refined_untrained_net", fp32(nw.X0_LEARNED)),
+                          ("fp32_refined_quartic_rule", fp32(nw.X0_LEARNED)),
                           ("fp32_refined_prev_step", fp32(nw.X0_SOLUTION)),
                           ("fp32_refined_x0_zero", fp32(nw.X0_ZERO)),
                           ("fp64_prev_step", fp64(nw.X0_SOLUTION)),
                           ("fp64_x0_zero", fp64(nw.X0_ZERO))]:
             ctx.set_solution(x_prev)
-            ctx.set_learned_guess(prior if name == "fp32_refined_untrained_net" else net)
+            ctx.set_learned_guess(nets.get(name, net))
             st = ctx.solve(cfg)
             sol[name] = ctx.solution()
             runs[name].append(st)
@@ -442,7 +455,8 @@ def learned_guess_series(ctx, W, ntrain, neval, epochs, log):
             "x0_rel_residual": sum(s.x0_rel for s in sts) / len(sts),
             "rel_error_vs_fp64": max(e for _, e in lst),
             "converged": all(bool(s.converged) for s in sts)}
-        if name in ("fp32_refined_learned", "fp32_refined_untrained_net"):
+        if name in ("fp32_refined_learned", "fp32_refined_untrained_net",
+                    "fp32_refined_quartic_rule"):
             res[name]["learned_guess_used"] = sum(s.guess == nw.X0_LEARNED for s in sts)
             res[name]["guess_ms"] = round(sum(s.guess_ms for s in sts) / len(sts), 2)
         log("solve %s: %s" % (name, res[name]))
@@ -459,21 +473,28 @@ def learned_guess_series(ctx, W, ntrain, neval, epochs, log):
     res["trained_vs_prev_step"] = {
         "iters": [tl["iters"], tp["iters"]], "ms": [tl["ms"], tp["ms"]],
         "x0_rel_residual": [tl["x0_rel_residual"], tp["x0_rel_residual"]]}
+    tq = res["fp32_refined_quartic_rule"]
+    res["trained_vs_quartic_rule"] = {
+        "iters": [tl["iters"], tq["iters"]], "ms": [tl["ms"], tq["ms"]],
+        "x0_rel_residual": [tl["x0_rel_residual"], tq["x0_rel_residual"]]}
     res["note"] = ("fp32_refined_untrained_net starts from the network before "
-                   "training, whose output head is the quadratic extrapolation "
-                   "3 x(n-1) - 3 x(n-2) + x(n-3) of the increments; "
-                   "fp32_refined_learned from the trained network; pairs "
-                   "x0/xp entries: [trained, other]")
+                   "training, whose output head (zero) gives the quadratic "
+                   "extrapolation 3 x(n-1) - 3 x(n-2) + x(n-3) of the increments; "
+                   "fp32_refined_learned from the trained network, which adds "
+                   "learned per-node multiples of the third and fourth history "
+                   "differences; fp32_refined_quartic_rule from the untrained "
+                   "network with its head set by hand to (1, 1), the fixed "
+                   "quartic extrapolation; pairs entries: [trained, other]")
     net.close()
     prior.close()
+    quartic.close()
     return res
 
 
 def device_geometry(hip, ctx, W, types, steps, warmup):
     """GeometryInteriorAlg on the device for a workload that kept its
     elements: the host setup of the incidence lists (once per mesh), the
-    device time of one nw_compute_geometry call (scratch allocation
-    included), and the largest deviation from the host-computed area vectors
+    device time of one nw_compute_geometry call, and the largest deviation from the host-computed area vectors
     and dual volumes.  Overwrites the context's area vectors and dual
     volumes with the device values."""
     import numpy as np
@@ -738,7 +759,7 @@ def main():
                     help="pressure-projection steps whose pairs train the network "
                          "(pairs = steps - 2)")
     ap.add_argument("--eval-steps", type=int, default=3)
-    ap.add_argument("--epochs", type=int, default=2)
+    ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--only", choices=["continuity", "momentum", "mixed"],
                     help="run one part (profiling); the JSON line is then partial")
     ap.add_argument("--no-cpu", action="store_true")

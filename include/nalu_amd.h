@@ -63,6 +63,8 @@ enum {
   NW_E_PECLET_NUMBER,   /* 1  "peclet_number" (edge field) */
   NW_F_PRESSURE_BC,     /* 1  "pressure_bc" (Dirichlet values of pressure) */
   NW_F_UTMP,            /* 3  "uTmp" (the dp/dx saved by the projection) */
+  NW_F_BUOYANCY_SOURCE, /* 3  "buoyancy_source" (balanced buoyancy force) */
+  NW_F_BUOYANCY_SOURCE_MASK, /* 1  "buoyancy_source_mask" */
   NW_NUM_FIELDS
 };
 
@@ -102,6 +104,12 @@ typedef struct {
   double interp_together; /* mdot interpolates rho*U together (default 1) */
   double incompressible;  /* incompressible continuity (default 0) */
   int accumulate;
+  /* use_balanced_buoyancy_force_ (ContinuityEdgeSolverAlg.C:58-88,147-152,
+   * 168-173): tmdot += tau g.A rho_ip (m_L + m_R)/2 and
+   * G_ip -= (m_R s_R/Udiag_R + m_L s_L/Udiag_L)/2, with s the node field
+   * NW_F_BUOYANCY_SOURCE and m NW_F_BUOYANCY_SOURCE_MASK; 0 = off */
+  int balanced_buoyancy;
+  double gravity[3];
 } nw_continuity_opts;
 
 /* ---- lifecycle ----------------------------------------------------------- */
@@ -112,6 +120,14 @@ int nw_ctx_destroy(nw_ctx* ctx);
 /* The context's hipStream_t, for callers that time or order work on it. */
 int nw_ctx_stream(nw_ctx* ctx, void** stream);
 int nw_ctx_synchronize(nw_ctx* ctx);
+/* Input check of the edge assemblies (the reference checks nothing; its
+ * device asserts, STK_NGP_ThrowRequireMsg, catch only structural errors):
+ * with `on`, every nw_assemble_* call scans the assembled values and rhs
+ * afterwards and returns NW_E_NONFINITE, naming the first row, when an
+ * entry is NaN or Inf (a NaN or Inf in any input field the kernel reads
+ * reaches the rows of that node and its neighbours).  The scan reads the
+ * system once more and synchronises the stream, so it is off by default. */
+int nw_ctx_set_check_finite(nw_ctx* ctx, int on);
 const char* nw_last_error(void);
 
 /* ---- mesh ---------------------------------------------------------------
@@ -262,11 +278,16 @@ int nw_assemble_momentum_with_peclet(nw_ctx* ctx, const nw_momentum_opts* opts,
  * with (rho u)_ip interpolated as rho*U together (interp_together = 1) or
  * as rho_ip u_ip.  Reads COORDINATES, VELOCITY, DPDX, DENSITY, PRESSURE,
  * UDIAG and AREA_VECTOR; writes NW_E_MASS_FLOW_RATE (one value per edge,
- * one launch over the edges, no graph needed).  The balanced-buoyancy and
- * mesh-motion (GCL) terms of the reference are not built. */
+ * one launch over the edges, no graph needed).  With balanced_buoyancy set
+ * it also reads NW_F_BUOYANCY_SOURCE and NW_F_BUOYANCY_SOURCE_MASK.  The
+ * mesh-motion (GCL) term of the reference is not built (static meshes). */
 typedef struct {
   double noc;             /* non-orthogonal correction on (1) / off (0) */
   double interp_together; /* 1: (rho u)_ip = avg(rho u); 0: avg(rho) avg(u) */
+  /* the balanced buoyancy force (MdotEdgeAlg.C:81-102,147-154,163-169), as
+   * in nw_continuity_opts; 0 = off */
+  int balanced_buoyancy;
+  double gravity[3];
 } nw_mdot_opts;
 int nw_compute_mdot(nw_ctx* ctx, const nw_mdot_opts* opts);
 /* NodalGradEdgeAlg::execute (src/ngp_algorithms/NodalGradEdgeAlg.C:58-111),
@@ -312,7 +333,8 @@ int nw_project_nodal_velocity(nw_ctx* ctx);
  *   host, the lists of element sub-control surfaces per edge and
  *   sub-control volumes per node (element order) that the device sums
  *   through.  NW_E_ARG when a node is out of range, an element edge is not
- *   in the edge list, or nelem > 2^26.  Once per mesh.
+ *   in the edge list, or nelem > 2^26.  Once per mesh; it also allocates
+ *   the per-element scratch of nw_compute_geometry (44 f64 per element).
  * nw_compute_geometry: impl_compute_edge_area_vector and
  *   impl_compute_dual_nodal_volume on the device from NW_F_COORDINATES:
  *   writes NW_E_AREA_VECTOR (sum of the element sub-control-surface area
@@ -320,7 +342,8 @@ int nw_project_nodal_velocity(nw_ctx* ctx);
  *   from node[0] to node[1]) and NW_F_DUAL_VOLUME (sum of the sub-control
  *   volumes of each node, HexSCV/TetSCV/WedSCV::determinant_scv).  Sums run
  *   in element order (the reference adds with atomics), so the result is
- *   reproducible.  Call again after the coordinates move. */
+ *   reproducible.  Call again after the coordinates move: stream-ordered,
+ *   allocates nothing. */
 int nw_mesh_elements(nw_ctx* ctx, int64_t nelem, const int32_t* type,
                      const int32_t* conn);
 int nw_compute_geometry(nw_ctx* ctx);

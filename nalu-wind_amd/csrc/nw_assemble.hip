@@ -76,25 +76,34 @@ __device__ inline double van_leer(double dqm, double dqp, double eps)
 //   off[NB]  the (n, other) block of the edge LHS, row-major ndof x ndof
 //   acc[NV]  the (n, n) block contribution (NB values) then rhs(n) (NDOF)
 
+// s, m: buoyancy_source and its mask, loaded only with the balanced
+// buoyancy force (unused registers are dropped otherwise)
 struct ContNode {
-  double x[3], u[3], g[3], rho, p, ud;
+  double x[3], u[3], g[3], rho, p, ud, s[3], m;
 };
 
 // Launch shape of each physics (measured on the 256^3 continuity, 128^3
 // momentum and 160^3 mixed-mesh workloads): kLanes lanes per row (the 1-dof
 // systems choose 1 or 2 from the graph, see launch_rows), kUnroll slots in
-// flight per lane, non-temporal streaming, the LDS slot capacities per wave,
-// the occupancy the register allocation must allow, and whether the
-// persistent kernel (k_assemble_persistent) runs it.
-struct ContinuityPhys {
+// flight per lane with one lane per row and kUnroll2 with two, non-temporal
+// streaming, the LDS slot capacities per wave, the occupancy the register
+// allocation must allow, and whether the persistent kernel
+// (k_assemble_persistent) runs it.
+// BUOY: the balanced buoyancy force (ContinuityEdgeSolverAlg.C:147-152,
+// 168-173), a separate instantiation so that the plain kernel reads no
+// extra field.
+template <bool BUOY>
+struct ContinuityPhysT {
   static constexpr int NDOF = 1;
-  static constexpr int kLanes = 1, kUnroll = 1, kMinWaves = 2;
+  static constexpr int kLanes = 1, kUnroll = 2, kUnroll2 = 1, kMinWaves = 2;
   static constexpr bool kNonTemporal = false, kPersistent = false;
   static constexpr int kSlotsSmall = 512, kSlotsLarge = 1024;
   using Node = ContNode;
   const double *coords, *vel, *dpdx, *rho, *p, *udiag, *areav;
   int64_t N, E;
   double nocFac, tauScale, interpTogether, incompressible;
+  const double *src, *mask;
+  double grav[3];
 
   __device__ void load(int n, Node& r) const
   {
@@ -106,6 +115,10 @@ struct ContinuityPhys {
     r.rho = rho[n];
     r.p = p[n];
     r.ud = udiag[n];
+    if constexpr (BUOY) {
+      for (int d = 0; d < 3; ++d) r.s[d] = src[3 * (int64_t)n + d];
+      r.m = mask[n];
+    }
   }
 
   struct Edge {
@@ -135,11 +148,18 @@ struct ContinuityPhys {
     }
     const double inv_axdx = 1.0 / axdx;
     double tmdot = -projTimeScale * (R.p - L.p) * asq * inv_axdx;
+    if constexpr (BUOY) {
+      const double masked_weights = 0.5 * (L.m + R.m);
+      for (int d = 0; d < 3; ++d)
+        tmdot += projTimeScale * av[d] * grav[d] * rhoIp * masked_weights;
+    }
     for (int d = 0; d < 3; ++d) {
       const double kxj = av[d] - asq * inv_axdx * dx[d];
       const double rhoUjIp = 0.5 * (R.rho * R.u[d] + L.rho * L.u[d]);
       const double ujIp = 0.5 * (R.u[d] + L.u[d]);
-      const double GjIp = 0.5 * (R.g[d] / R.ud + L.g[d] / L.ud);
+      double GjIp = 0.5 * (R.g[d] / R.ud + L.g[d] / L.ud);
+      if constexpr (BUOY)
+        GjIp -= 0.5 * ((R.m * R.s[d]) / R.ud + (L.m * L.s[d]) / L.ud);
       tmdot += (interpTogether * rhoUjIp + om_interpTogether * rhoIp * ujIp +
                 GjIp) * av[d] -
                kxj * GjIp * nocFac;
@@ -159,7 +179,7 @@ struct ScalNode {
 
 struct ScalarPhys {
   static constexpr int NDOF = 1;
-  static constexpr int kLanes = 1, kUnroll = 1, kMinWaves = 2;
+  static constexpr int kLanes = 1, kUnroll = 1, kUnroll2 = 1, kMinWaves = 2;
   static constexpr bool kNonTemporal = false, kPersistent = false;
   static constexpr int kSlotsSmall = 512, kSlotsLarge = 1024;
   using Node = ScalNode;
@@ -285,7 +305,7 @@ struct MomEdgeT<true> {
 template <bool PEC>
 struct MomentumPhysT {
   static constexpr int NDOF = 3;
-  static constexpr int kLanes = 2, kUnroll = 2, kMinWaves = 2;
+  static constexpr int kLanes = 2, kUnroll = 2, kUnroll2 = 2, kMinWaves = 2;
   // persistent: 0.577 vs 0.643 ms on the 128^3 box; with the Peclet factor
   // in the kernel (18 dwords of spill, 33 persistent) it is slower (0.95 vs
   // 0.76 ms per step)
@@ -474,34 +494,46 @@ struct MomentumPhysT {
 // before any of them is evaluated, and they are evaluated and summed in
 // slot order, so U changes only the overlap of the loads.
 template <class P, int G, int U, class Code, class Col, class Put>
-__device__ inline void walk_row(const P& phys, int row, int s0, int s1, int sd,
-                                int g, Code code_at, Col col_at, Put put,
-                                double* acc)
+__device__ inline void walk_row(const P& phys, const typename P::Node& me,
+                                int s0, int s1, int sd, int g, Code code_at,
+                                Col col_at, Put put, double* acc)
 {
   constexpr int ND = P::NDOF, NB = ND * ND, NV = NB + ND;
-  typename P::Node me, ot[U];
+  constexpr bool kCoRead = !(ND == 1 && G == 2);
+  typename P::Node ot[U];
   typename P::Edge er[U];
-  phys.load(row, me);
   const int noff = s1 - s0 - 1;
 #pragma clang loop unroll(disable)
   for (int k = g; k < noff; k += G * U) {
     // the u loops are unrolled so that ot[] and er[] stay in registers
-    int sl[U], c[U];
+    int sl[U], c[U], m[U];
+    bool ok[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int kk = k + u * G;
       sl[u] = s0 + kk + (s0 + kk >= sd ? 1 : 0);
-      c[u] = kk < noff ? code_at(sl[u]) : -1;
+      if constexpr (kCoRead) {
+        // the column read with the code (neither waits for the other)
+        ok[u] = kk < noff;  // else past the row's end (true for u = 0)
+        c[u] = ok[u] ? code_at(sl[u]) : 0;
+        m[u] = ok[u] ? col_at(sl[u]) : 0;
+      } else {
+        // two lanes per 1-dof row (uneven mixed-mesh rows) measured faster
+        // reading the column after the code, behind a test of the code
+        c[u] = kk < noff ? code_at(sl[u]) : -1;
+        ok[u] = c[u] >= 0;
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (c[u] < 0) continue;  // past the row's end
-      phys.load(col_at(sl[u]), ot[u]);
+      if (!ok[u]) continue;
+      if constexpr (!kCoRead) m[u] = col_at(sl[u]);
+      phys.load(m[u], ot[u]);
       phys.load_edge(c[u] >> 1, er[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (c[u] < 0) continue;
+      if (!ok[u]) continue;
       double off[NB], con[NV];
       if (c[u] & 1)
         phys.edge(er[u], 1, ot[u], me, off, con);
@@ -535,6 +567,41 @@ __device__ inline void put_diag_rhs(double* __restrict__ dp,
   } else {
     for (int k = 0; k < NB; ++k) dp[k] = acc[k];
     for (int d = 0; d < ND; ++d) rp[d] = acc[NB + d];
+  }
+}
+
+// Stages the ns <= SCAP codes and columns of a wave's slot range into LDS:
+// every lane issues all of its (up to SCAP/64) loads of each array before it
+// writes any of them, so the staging costs one memory round trip, not one
+// per 64 slots (a loop that stores each loaded pair before loading the
+// next waits for every load in turn).
+template <int SCAP, bool NT>
+__device__ inline void stage_slots(const int32_t* __restrict__ code,
+                                   const int32_t* __restrict__ cols, int ns,
+                                   int lane, int* code_s, int* col_s)
+{
+  constexpr int KS = (SCAP + kWave - 1) / kWave;
+  int cv[KS], mv[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int i = lane + k * kWave;
+    if (i < ns) {
+      if (NT) {
+        cv[k] = __builtin_nontemporal_load(code + i);
+        mv[k] = __builtin_nontemporal_load(cols + i);
+      } else {
+        cv[k] = code[i];
+        mv[k] = cols[i];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int i = lane + k * kWave;
+    if (i < ns) {
+      code_s[i] = cv[k];
+      col_s[i] = mv[k];
+    }
   }
 }
 
@@ -584,22 +651,25 @@ __device__ inline void assemble_task(
   const int s_lo = G == 1 ? chunk.s0 : (half ? chunk.smid : chunk.s0);
   const int s_hi = G == 1 ? chunk.s1 : (half ? chunk.s1 : chunk.smid);
   const int ns = s_hi - s_lo;
-  // the row's slot range and diagonal slot do not depend on the staged
-  // codes: load them now, so that they arrive with the staging loads
+  // the row's slot range, diagonal slot and own node fields do not depend
+  // on the staged codes: load them now, so that they arrive with the
+  // staging loads
   int r_s0 = 0, r_s1 = 0, r_diag = 0;
   bool skipped = false;
+  typename P::Node me;
   if (active) {
     r_s0 = row_ptr[row];
     r_s1 = row_ptr[row + 1];
     r_diag = diag[row];
     if (skip) skipped = skip[row] != 0;
+    phys.load(row, me);
   }
   double acc[NV];
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
   if (ns > SCAP) {
     if (active)
       walk_row<P, G, U>(
-        phys, row, r_s0, r_s1, r_diag, g, [&](int s) { return code[s]; },
+        phys, me, r_s0, r_s1, r_diag, g, [&](int s) { return code[s]; },
         [&](int s) { return cols[s]; },
         [&](int s, const double* off) {
           double* vp = vals + (int64_t)s * NB;
@@ -621,19 +691,11 @@ __device__ inline void assemble_task(
     }
     return;
   }
-  for (int i = lane; i < ns; i += kWave) {
-    if (NT) {
-      code_s[i] = __builtin_nontemporal_load(code + s_lo + i);
-      col_s[i] = __builtin_nontemporal_load(cols + s_lo + i);
-    } else {
-      code_s[i] = code[s_lo + i];
-      col_s[i] = cols[s_lo + i];
-    }
-  }
+  stage_slots<SCAP, NT>(code + s_lo, cols + s_lo, ns, lane, code_s, col_s);
   wave_lds_sync();
   if (active)
     walk_row<P, G, U>(
-      phys, row, r_s0, r_s1, r_diag, g, [&](int s) { return code_s[s - s_lo]; },
+      phys, me, r_s0, r_s1, r_diag, g, [&](int s) { return code_s[s - s_lo]; },
       [&](int s) { return col_s[s - s_lo]; },
       [&](int s, const double* off) {
         double* vp = &val_s[(s - s_lo) * NB];
@@ -1021,6 +1083,7 @@ int build_schedule(Ctx* c)
 template <class P, int G, int SCAP>
 void launch_assemble(Ctx* c, const P& phys, int accumulate, double* xdiag)
 {
+  constexpr int kU = G == 1 ? P::kUnroll : P::kUnroll2;
   const Graph& g = c->g;
   const uint8_t* skip = active_skip_mask(g);
   const int64_t wpb = kBlock / kWave;
@@ -1035,7 +1098,7 @@ void launch_assemble(Ctx* c, const P& phys, int accumulate, double* xdiag)
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                                 c->device) != hipSuccess ||
           hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_assemble_persistent<P, G, P::kUnroll, SCAP, P::kNonTemporal>,
+            &per_cu, k_assemble_persistent<P, G, kU, SCAP, P::kNonTemporal>,
             kBlock, 0) != hipSuccess)
         ncu = per_cu = 0;
       resident = ncu * per_cu > 0 ? ncu * per_cu : -1;
@@ -1046,7 +1109,7 @@ void launch_assemble(Ctx* c, const P& phys, int accumulate, double* xdiag)
       // each other, so the grid's size is a matter of speed only)
       blocks = std::min<int64_t>(blocks, std::max(8, resident / 8 * 8));
       blocks = (blocks + 7) / 8 * 8;
-      k_assemble_persistent<P, G, P::kUnroll, SCAP, P::kNonTemporal>
+      k_assemble_persistent<P, G, kU, SCAP, P::kNonTemporal>
         <<<(unsigned)blocks, kBlock, 0, c->stream>>>(
           phys, g.row_ptr, g.code, g.cols, g.diag, g.sched, (int)g.nchunks,
           g.vals, g.rhs, accumulate, skip, xdiag);
@@ -1054,7 +1117,7 @@ void launch_assemble(Ctx* c, const P& phys, int accumulate, double* xdiag)
     }
   }
   blocks = (blocks + 7) / 8 * 8;  // a multiple of the XCD count
-  k_assemble<P, G, P::kUnroll, SCAP, P::kNonTemporal>
+  k_assemble<P, G, kU, SCAP, P::kNonTemporal>
     <<<(unsigned)blocks, kBlock, 0, c->stream>>>(phys, g.row_ptr, g.code, g.cols,
                                                  g.diag, g.sched, (int)g.nchunks,
                                                  g.vals, g.rhs, accumulate, skip,
@@ -1116,10 +1179,22 @@ int nw_assemble_continuity(nw_ctx* c, const nw_continuity_opts* o)
   NW_GET(NW_F_PRESSURE, p);
   NW_GET(NW_F_UDIAG, udiag);
   NW_GET(NW_E_AREA_VECTOR, areav);
-  ContinuityPhys P{coords, vel, dpdx, rho, p, udiag, areav, c->nnode, c->nedge,
-                   o->noc, o->dt / o->gamma1, o->interp_together,
-                   o->incompressible};
-  return launch_rows(c, P, o->accumulate);
+  if (o->balanced_buoyancy) {
+    NW_GET(NW_F_BUOYANCY_SOURCE, src);
+    NW_GET(NW_F_BUOYANCY_SOURCE_MASK, mask);
+    ContinuityPhysT<true> P{coords, vel, dpdx, rho, p, udiag, areav, c->nnode,
+                            c->nedge, o->noc, o->dt / o->gamma1,
+                            o->interp_together, o->incompressible, src, mask,
+                            {o->gravity[0], o->gravity[1], o->gravity[2]}};
+    return check_system_finite(c, launch_rows(c, P, o->accumulate),
+                               "ContinuityEdgeSolverAlg");
+  }
+  ContinuityPhysT<false> P{coords, vel, dpdx, rho, p, udiag, areav, c->nnode,
+                           c->nedge, o->noc, o->dt / o->gamma1,
+                           o->interp_together, o->incompressible, nullptr,
+                           nullptr, {0.0, 0.0, 0.0}};
+  return check_system_finite(c, launch_rows(c, P, o->accumulate),
+                             "ContinuityEdgeSolverAlg");
 }
 
 int nw_assemble_scalar(nw_ctx* c, const nw_scalar_opts* o)
@@ -1138,7 +1213,8 @@ int nw_assemble_scalar(nw_ctx* c, const nw_scalar_opts* o)
                o->alpha, o->alpha_upw, o->upw, o->relax, o->use_limiter,
                PecletDev{o->peclet.form, o->peclet.hybrid, o->peclet.c1,
                          o->peclet.c2}};
-  return launch_rows(c, P, o->accumulate);
+  return check_system_finite(c, launch_rows(c, P, o->accumulate),
+                             "ScalarEdgeSolverAlg");
 }
 
 int nw_assemble_momentum(nw_ctx* c, const nw_momentum_opts* o)
@@ -1162,7 +1238,8 @@ int nw_assemble_momentum(nw_ctx* c, const nw_momentum_opts* o)
     int rc = ensure_field(c, NW_F_UDIAG, &xdiag);
     if (rc) return rc;
   }
-  return launch_rows(c, P, o->accumulate, xdiag);
+  return check_system_finite(c, launch_rows(c, P, o->accumulate, xdiag),
+                             "MomentumEdgeSolverAlg");
 }
 
 int nw_assemble_momentum_with_peclet(nw_ctx* c, const nw_momentum_opts* o,
@@ -1193,7 +1270,8 @@ int nw_assemble_momentum_with_peclet(nw_ctx* c, const nw_momentum_opts* o,
     rc = ensure_field(c, NW_F_UDIAG, &xdiag);
     if (rc) return rc;
   }
-  return launch_rows(c, P, o->accumulate, xdiag);
+  return check_system_finite(c, launch_rows(c, P, o->accumulate, xdiag),
+                             "MomentumEdgeSolverAlg");
 }
 
 int nw_momentum_edge_peclet(nw_ctx* c, const nw_peclet* pf)

@@ -33,6 +33,7 @@ int field_components(int f)
   case NW_F_DQDX:
   case NW_F_BC_VALUE:
   case NW_F_UTMP:
+  case NW_F_BUOYANCY_SOURCE:
   case NW_E_AREA_VECTOR:
     return 3;
   case NW_F_DUDX:
@@ -74,6 +75,17 @@ int require_field(Ctx* c, int f, const double** out)
   return rc;
 }
 
+int check_system_finite(Ctx* c, int rc, const char* who);
+
+void free_history(Ctx* c)
+{
+  for (auto& p : c->xh) {
+    (void)hipFree(p);
+    p = nullptr;
+  }
+  c->nh = 0;
+}
+
 static void free_graph(Graph& g)
 {
   (void)hipFree(g.row_ptr);
@@ -98,6 +110,16 @@ __global__ void k_fill(double* __restrict__ p, double v, int64_t n)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Smallest index i < n with a non-finite p[i], as min into *first (which
+// starts at n): one atomicMin per wave that found one.
+__global__ void k_first_nonfinite(const double* __restrict__ p, int64_t n,
+                                  unsigned long long* first)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool bad = i < n && !__builtin_isfinite(p[i]);
+  if (bad) atomicMin(first, (unsigned long long)i);
 }
 
 // Point-row reset for the nodes listed: every entry of point row
@@ -146,6 +168,50 @@ __global__ void k_add_segments(const int64_t* __restrict__ seg,
     rhs[-t - 1] += s;
 }
 
+__global__ void k_set_u64x2(unsigned long long* p, unsigned long long a,
+                            unsigned long long b)
+{
+  p[0] = a;
+  p[1] = b;
+}
+
+int check_system_finite(Ctx* c, int rc, const char* who)
+{
+  if (rc || !c->check_finite || !c->g.vals) return rc;
+  const Graph& g = c->g;
+  const int64_t nv = g.nnzb * g.ndof * g.ndof, nr = g.nnode * g.ndof;
+  auto* first = reinterpret_cast<unsigned long long*>(c->d_bad);
+  // first[0]: first non-finite value (nv if none), first[1]: rhs (nr)
+  k_set_u64x2<<<1, 1, 0, c->stream>>>(first, (unsigned long long)nv,
+                                      (unsigned long long)nr);
+  if (nv > 0)
+    k_first_nonfinite<<<grid_for(nv, 256), 256, 0, c->stream>>>(g.vals, nv, first);
+  if (nr > 0)
+    k_first_nonfinite<<<grid_for(nr, 256), 256, 0, c->stream>>>(g.rhs, nr, first + 1);
+  NW_LAUNCH_CHECK();
+  unsigned long long h[2] = {0, 0};
+  NW_HIP(hipMemcpyAsync(h, first, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  NW_HIP(hipStreamSynchronize(c->stream));
+  if (h[0] < (unsigned long long)nv) {
+    // the slot's row: the last row whose first slot is <= slot
+    const int64_t slot = (int64_t)h[0] / (g.ndof * g.ndof);
+    std::vector<int32_t> rp(g.nnode + 1);
+    NW_HIP(hipMemcpy(rp.data(), g.row_ptr, sizeof(int32_t) * rp.size(),
+                     hipMemcpyDeviceToHost));
+    const int64_t row = std::upper_bound(rp.begin(), rp.end(), (int32_t)slot) -
+                        rp.begin() - 1;
+    set_error("%s: non-finite matrix entry in the row of node %lld (an input "
+              "field holds NaN or Inf)", who, (long long)row);
+    return NW_E_NONFINITE;
+  }
+  if (h[1] < (unsigned long long)nr) {
+    set_error("%s: non-finite rhs entry at node %lld (an input field holds NaN "
+              "or Inf)", who, (long long)(h[1] / g.ndof));
+    return NW_E_NONFINITE;
+  }
+  return NW_OK;
+}
+
 }  // namespace nw
 
 using namespace nw;
@@ -186,16 +252,26 @@ int nw_ctx_destroy(nw_ctx* c)
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c->kry);
   amg_free(c->amg);
-  (void)hipFree(c->x_init);
-  (void)hipFree(c->x_hist);
-  (void)hipFree(c->x_hist2);
+  free_history(c);
   free_graph(c->g);
   geom_free(c->geom);
   free_sum_scratch(c);
+  (void)hipFree(c->d_bad);
   for (auto& f : c->field) (void)hipFree(f);
   (void)hipFree(c->edges);
   (void)hipStreamDestroy(c->stream);
   delete c;
+  return NW_OK;
+}
+
+int nw_ctx_set_check_finite(nw_ctx* c, int on)
+{
+  NW_REQUIRE(c, NW_E_ARG, "null context");
+  if (on && !c->d_bad) {
+    NW_HIP(hipSetDevice(c->device));
+    NW_HIP(hipMalloc(&c->d_bad, 2 * sizeof(int64_t)));
+  }
+  c->check_finite = on != 0;
   return NW_OK;
 }
 
@@ -232,12 +308,7 @@ int nw_mesh_set(nw_ctx* c, int64_t nnode, int64_t nedge,
   c->kry = nullptr;
   amg_free(c->amg);
   c->amg = nullptr;
-  (void)hipFree(c->x_init);
-  c->x_init = nullptr;
-  (void)hipFree(c->x_hist);
-  c->x_hist = nullptr;
-  (void)hipFree(c->x_hist2);
-  c->x_hist2 = nullptr;
+  free_history(c);
   free_graph(c->g);
   geom_free(c->geom);
   c->geom = nullptr;
@@ -326,12 +397,7 @@ int nw_linsys_build_edge_graph(nw_ctx* c, int ndof)
   c->kry = nullptr;
   amg_free(c->amg);
   c->amg = nullptr;
-  (void)hipFree(c->x_init);
-  c->x_init = nullptr;
-  (void)hipFree(c->x_hist);
-  c->x_hist = nullptr;
-  (void)hipFree(c->x_hist2);
-  c->x_hist2 = nullptr;
+  free_history(c);
   free_graph(c->g);
   Graph& g = c->g;
   const int64_t N = c->nnode, E = c->nedge;

@@ -1,7 +1,8 @@
 // nw_edge_algs.hip -- the edge algorithms on either side of the pressure
 // solve, on gfx950:
 //
-//   k_mdot       MdotEdgeAlg (src/ngp_algorithms/MdotEdgeAlg.C:121-186):
+//   k_mdot       MdotEdgeAlg (src/ngp_algorithms/MdotEdgeAlg.C:121-186,
+//                with or without the balanced buoyancy force):
 //                one lane per edge, edge-ordered (coalesced) area-vector
 //                reads and mass-flow-rate writes, the two nodes' fields
 //                gathered as node-major records;
@@ -28,8 +29,13 @@ struct MdotArgs {
   const double *coords, *vel, *dpdx, *rho, *p, *udiag, *areav;
   int64_t N, E;
   double noc, interp;
+  // the balanced buoyancy force: buoyancy_source, its mask, gravity
+  const double *src, *mask;
+  double grav[3];
 };
 
+// BUOY: the balanced buoyancy force (MdotEdgeAlg.C:147-154,163-169)
+template <bool BUOY>
 __global__ __launch_bounds__(kBlock) void k_mdot(MdotArgs a,
                                                  const int2* __restrict__ edges,
                                                  const int32_t* __restrict__ order,
@@ -55,12 +61,22 @@ __global__ __launch_bounds__(kBlock) void k_mdot(MdotArgs a,
   }
   const double inv_axdx = 1.0 / axdx;
   double tmdot = -projTimeScale * (a.p[R] - a.p[L]) * asq * inv_axdx;
+  double mL = 0.0, mR = 0.0;
+  if constexpr (BUOY) {
+    mL = a.mask[L];
+    mR = a.mask[R];
+    const double masked_weights = 0.5 * (mL + mR);
+    for (int d = 0; d < 3; ++d)
+      tmdot += projTimeScale * av[d] * a.grav[d] * rhoIp * masked_weights;
+  }
   for (int d = 0; d < 3; ++d) {
     const double kxj = av[d] - asq * inv_axdx * dx[d];
     const double uR = a.vel[3 * R + d], uL = a.vel[3 * L + d];
     const double rhoUjIp = 0.5 * (rhoR * uR + rhoL * uL);
     const double ujIp = 0.5 * (uR + uL);
-    const double GjIp = 0.5 * (a.dpdx[3 * R + d] / udR + a.dpdx[3 * L + d] / udL);
+    double GjIp = 0.5 * (a.dpdx[3 * R + d] / udR + a.dpdx[3 * L + d] / udL);
+    if constexpr (BUOY)
+      GjIp -= 0.5 * ((mR * a.src[3 * R + d]) / udR + (mL * a.src[3 * L + d]) / udL);
     tmdot += (a.interp * rhoUjIp + om_interp * rhoIp * ujIp + GjIp) * av[d] -
              kxj * GjIp * a.noc;
   }
@@ -97,11 +113,27 @@ __global__ __launch_bounds__(kBlock) void k_nodal_grad(
   const SchedChunk ch = sched[chunk_id];
   const int s_lo = ch.s0, ns = ch.s1 - ch.s0;
   const bool staged = ns <= SCAP;
-  if (staged)
-    for (int i = lane; i < ns; i += kWave) {
-      code_s[wv][i] = code[s_lo + i];
-      col_s[wv][i] = cols[s_lo + i];
+  if (staged) {
+    // all of a lane's loads issued before any is stored: one round trip
+    constexpr int KS = SCAP / kWave;
+    int cv[KS], mv[KS];
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int i = lane + k * kWave;
+      if (i < ns) {
+        cv[k] = code[s_lo + i];
+        mv[k] = cols[s_lo + i];
+      }
     }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int i = lane + k * kWave;
+      if (i < ns) {
+        code_s[wv][i] = cv[k];
+        col_s[wv][i] = mv[k];
+      }
+    }
+  }
   wave_lds_sync();
   const int n = ch.r0 + lane;
   const bool active = n < ch.r1;
@@ -172,8 +204,14 @@ int nw_compute_mdot(nw_ctx* c, const nw_mdot_opts* o)
   double* mdot = nullptr;
   if ((rc = ensure_field(c, NW_E_MASS_FLOW_RATE, &mdot))) return rc;
   if (c->nedge == 0) return NW_OK;
-  const MdotArgs a{coords, vel, dpdx, rho, p, udiag, areav, c->nnode, c->nedge,
-                   o->noc, o->interp_together};
+  MdotArgs a{coords, vel, dpdx, rho, p, udiag, areav, c->nnode, c->nedge,
+             o->noc, o->interp_together, nullptr, nullptr, {0.0, 0.0, 0.0}};
+  if (o->balanced_buoyancy) {
+    if ((rc = require_field(c, NW_F_BUOYANCY_SOURCE, &a.src)) ||
+        (rc = require_field(c, NW_F_BUOYANCY_SOURCE_MASK, &a.mask)))
+      return rc;
+    for (int d = 0; d < 3; ++d) a.grav[d] = o->gravity[d];
+  }
   // with the edge graph built, the edges go in its schedule's order (rows
   // of a chunk are neighbours in space, so the two ends' node data is
   // shared by nearby lanes); otherwise in their own order
@@ -184,8 +222,12 @@ int nw_compute_mdot(nw_ctx* c, const nw_mdot_opts* o)
   }
   // XCD-aware block order: each XCD takes one contiguous stretch
   const unsigned nb = (grid_for(c->nedge, kBlock) + 7) / 8 * 8;
-  k_mdot<<<nb, kBlock, 0, c->stream>>>(
-    a, reinterpret_cast<const int2*>(c->edges), order, mdot);
+  if (o->balanced_buoyancy)
+    k_mdot<true><<<nb, kBlock, 0, c->stream>>>(
+      a, reinterpret_cast<const int2*>(c->edges), order, mdot);
+  else
+    k_mdot<false><<<nb, kBlock, 0, c->stream>>>(
+      a, reinterpret_cast<const int2*>(c->edges), order, mdot);
   NW_LAUNCH_CHECK();
   return NW_OK;
 }

@@ -294,6 +294,8 @@ void geom_free(Geom* g)
   (void)hipFree(g->e_inc);
   (void)hipFree(g->n_ptr);
   (void)hipFree(g->n_inc);
+  (void)hipFree(g->scs);
+  (void)hipFree(g->scv);
   delete g;
 }
 
@@ -343,6 +345,9 @@ int nw_mesh_elements(nw_ctx* c, int64_t nelem, const int32_t* type,
       (r = upload(&g->n_ptr, n_ptr.data(), (int64_t)n_ptr.size(), c->stream)) ||
       (r = upload(&g->n_inc, n_inc.data(), (int64_t)n_inc.size(), c->stream)))
     return r;
+  const int64_t ne = std::max<int64_t>(nelem, 1);
+  NW_HIP(hipMalloc(&g->scs, sizeof(double) * kMaxScs * 3 * ne));
+  NW_HIP(hipMalloc(&g->scv, sizeof(double) * kMaxScv * ne));
   NW_HIP(hipStreamSynchronize(c->stream));
   return NW_OK;
 }
@@ -360,9 +365,7 @@ int nw_compute_geometry(nw_ctx* c)
       (rc = ensure_field(c, NW_F_DUAL_VOLUME, &dualvol)))
     return rc;
   const int64_t ne = g->nelem;
-  double *scs = nullptr, *scv = nullptr;
-  NW_HIP(hipMalloc(&scs, sizeof(double) * kMaxScs * 3 * std::max<int64_t>(ne, 1)));
-  NW_HIP(hipMalloc(&scv, sizeof(double) * kMaxScv * std::max<int64_t>(ne, 1)));
+  double *scs = g->scs, *scv = g->scv;
   if (ne > 0) {
     k_elem_scs<<<grid_for(ne, kBlock), kBlock, 0, c->stream>>>(
       ne, g->type, g->conn, coords, c->nnode, scs);
@@ -374,12 +377,7 @@ int nw_compute_geometry(nw_ctx* c)
       c->nedge, ne, g->e_ptr, g->e_inc, scs, areav);
   k_dual_volume<<<grid_for(c->nnode, kBlock), kBlock, 0, c->stream>>>(
     c->nnode, ne, g->n_ptr, g->n_inc, scv, dualvol);
-  const hipError_t le = hipGetLastError();
-  const hipError_t se = hipStreamSynchronize(c->stream);
-  (void)hipFree(scs);
-  (void)hipFree(scv);
-  NW_HIP(le);
-  NW_HIP(se);
+  NW_LAUNCH_CHECK();
   return NW_OK;
 }
 

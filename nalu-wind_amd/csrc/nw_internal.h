@@ -137,6 +137,8 @@ inline const uint8_t* active_skip_mask(const Graph& g)
   return (g.skip && g.nskip > 0 && g.skip_check) ? g.skip : nullptr;
 }
 
+constexpr int kHist = 5;  // guesses kept for the learned initial guess
+
 struct Krylov;     // solver workspace (nw_krylov.hip)
 struct AmgDevice;  // AMG hierarchy (nw_amg.hip)
 struct Mpnn;       // learned initial guess (nw_mpnn.hip)
@@ -151,6 +153,11 @@ struct Geom {
   int32_t* e_inc = nullptr;  // 2 (12 el + s) + flip, element order per edge
   int32_t* n_ptr = nullptr;  // [nnode + 1]
   int32_t* n_inc = nullptr;  // 8 el + a, element order per node
+  // per-element scratch of nw_compute_geometry (sub-control-surface area
+  // vectors and sub-control volumes), allocated with the elements so that a
+  // call (e.g. after mesh motion) allocates nothing
+  double* scs = nullptr;
+  double* scv = nullptr;
 };
 void geom_free(Geom* g);
 
@@ -165,14 +172,22 @@ struct Ctx {
   Graph g;
   Krylov* kry = nullptr;
   AmgDevice* amg = nullptr;
-  double* x_init = nullptr;  // [nnode] guess the last solve started from
-  double* x_hist = nullptr;  // [nnode] guess the solve before it started from
-  double* x_hist2 = nullptr; // [nnode] and the one before that
+  // [kHist][nnode]: the guesses the latest solves started from, newest
+  // first (xh[0]: the last solve's), nh of them valid
+  double* xh[kHist] = {};
+  int nh = 0;
   Mpnn* mpnn = nullptr;      // network for NW_X0_LEARNED (not owned)
   Geom* geom = nullptr;      // elements for the device geometry
   void* sum_buf = nullptr;   // scratch of nw_linsys_sum_into_device
   size_t sum_cap = 0;        // its size in bytes (grows, never shrinks)
+  bool check_finite = false; // nw_ctx_set_check_finite
+  int64_t* d_bad = nullptr;  // first non-finite value index found
 };
+
+// After an assembly (`who` names it): when the context's finite check is
+// on, scans the system's values and rhs and returns NW_E_NONFINITE, naming
+// the first non-finite point row, if any entry is NaN or Inf; else rc.
+int check_system_finite(Ctx* c, int rc, const char* who);
 
 void free_sum_scratch(Ctx* c);
 // the fixed scratch of nw_linsys_sum_into_device_noalloc: takes its device
@@ -190,11 +205,12 @@ void amg_free(AmgDevice* a);
 // Builds the graph's locality-ordered chunk schedule (g.sched, g.nchunks)
 // if it is not current.
 int ensure_schedule(Ctx* c);
-// Learned guess predicted from the context's system, x0 and the guesses
-// before it xp, xpp (device vectors of nnode doubles); wall time in
-// mpnn_last_ms.
-int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp,
-                        const double* xpp, double* x);
+// Learned guess predicted from the context's system and the history of
+// guesses hist[0..nh-1] (newest first, device vectors of nnode doubles);
+// wall time in mpnn_last_ms.
+int mpnn_predict_device(Mpnn* m, const double* const* hist, int nh, double* x);
+// Frees the context's history of guesses.
+void free_history(Ctx* c);
 double mpnn_last_ms(const Mpnn* m);
 
 inline unsigned grid_for(int64_t n, int block) {

@@ -909,15 +909,16 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
   double* x = c->g.sln;
   if (cfg->x0 == NW_X0_ZERO) NW_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
   // History of starting guesses (the learned guess's inputs and recorded
-  // training pairs): x_init, x_hist, x_hist2 are the guesses of the last
-  // three solves that started from the solution vector with distinct
-  // guesses.  A solve from x0 = 0, or from the same guess as the last one
-  // (another method on the same step), leaves the history as it is.
+  // training pairs): xh[0..nh-1] are the guesses of the latest (up to
+  // kHist) solves that started from the solution vector with distinct
+  // guesses, newest first.  A solve from x0 = 0, or from the same guess as
+  // the last one (another method on the same step), leaves the history as
+  // it is.  The buffers rotate; only the new guess is copied.
   if (cfg->x0 != NW_X0_ZERO) {
     bool shift = true;
-    if (c->x_init) {
+    if (c->nh > 0) {
       NW_HIP(hipMemsetAsync(k->bad, 0, sizeof(int), st));
-      k_differs<<<grid_for(n, kBlock), kBlock, 0, st>>>(x, c->x_init, n, k->bad);
+      k_differs<<<grid_for(n, kBlock), kBlock, 0, st>>>(x, c->xh[0], n, k->bad);
       NW_LAUNCH_CHECK();
       int differs = 0;
       NW_HIP(hipMemcpyAsync(&differs, k->bad, sizeof(int),
@@ -926,16 +927,13 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
       shift = differs != 0;
     }
     if (shift) {
-      if (!c->x_hist2) NW_HIP(hipMalloc(&c->x_hist2, sizeof(double) * n));
-      const double* older = c->x_hist ? c->x_hist : (c->x_init ? c->x_init : x);
-      NW_HIP(hipMemcpyAsync(c->x_hist2, older, sizeof(double) * n,
+      double* last = c->xh[kHist - 1];
+      for (int j = kHist - 1; j > 0; --j) c->xh[j] = c->xh[j - 1];
+      if (!last) NW_HIP(hipMalloc(&last, sizeof(double) * n));
+      c->xh[0] = last;
+      NW_HIP(hipMemcpyAsync(c->xh[0], x, sizeof(double) * n,
                             hipMemcpyDeviceToDevice, st));
-      if (!c->x_hist) NW_HIP(hipMalloc(&c->x_hist, sizeof(double) * n));
-      NW_HIP(hipMemcpyAsync(c->x_hist, c->x_init ? c->x_init : x,
-                            sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-      if (!c->x_init) NW_HIP(hipMalloc(&c->x_init, sizeof(double) * n));
-      NW_HIP(hipMemcpyAsync(c->x_init, x, sizeof(double) * n,
-                            hipMemcpyDeviceToDevice, st));
+      c->nh = std::min(c->nh + 1, kHist);
     }
   }
   double bb = 0;
@@ -951,7 +949,7 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
     // the network's prediction from (A, b, x0); kept only if it lowers the
     // residual, otherwise GMRES starts from x0 (the previous-step guess)
     if (!k->xg) NW_HIP(hipMalloc(&k->xg, sizeof(double) * n));
-    rc = mpnn_predict_device(c->mpnn, c->x_init, c->x_hist, c->x_hist2, k->xg);
+    rc = mpnn_predict_device(c->mpnn, c->xh, c->nh, k->xg);
     out->guess_ms = mpnn_last_ms(c->mpnn);
     double rl = 0;
     if (rc == NW_OK) rc = S.residual(k->xg, &rl);

@@ -3,26 +3,33 @@
 // A message-passing network on the graph of the assembled 1-dof matrix
 // predicts the change of the solution from the previous-step guess x0:
 //
-//   features f_n = (r/s_r, log|a_nn| - mean, dirichlet, (x0 - xp)/s_d,
-//                   (xp - xpp)/s_d),  r = b - A x0, xp and xpp the guesses
-//                   of the two solves before (two and three steps back),
-//                   s_* the RMS of each vector, s_d of x0 - xp; differences
-//                   only, so what is learned on earlier steps carries over
+//   history h_0 .. h_{nh-1}: the guesses of the latest nh <= 5 solves,
+//                   newest first (h_0 = x0, the previous step's solution)
+//   xb      = polynomial extrapolation in time of the newest min(nh, 3):
+//             3 h_0 - 3 h_1 + h_2 (2 h_0 - h_1, h_0 for shorter histories)
+//   features f_n = (r/s_r, log|a_nn| - mean, dirichlet,
+//                   (h_0 - 3 h_1 + 3 h_2 - h_3)/s_d,
+//                   (h_0 - 4 h_1 + 6 h_2 - 4 h_3 + h_4)/s_d),  r = b - A xb,
+//                   the third and fourth backward differences of the history
+//                   (0 when it is shorter), s_* the RMS of each vector, s_d
+//                   of h_0 - h_1; differences only, so what is learned on
+//                   earlier steps carries over
 //   h0      = relu(W_in f + b_in)
 //   m_l     = sum_{m != n} w_nm h_{l-1,m},  w_nm = sqrt|a_nm a_mn| / sqrt|a_nn a_mm|
 //   h_l     = relu(W_self,l h_{l-1} + W_nbr,l m_l + b_l),   l = 1..L
 //   a_k,n   = w_out,k . h_L + b_out,k                   k = 1, 2
 //   delta_n = a_1,n f_3,n + a_2,n f_4,n   (the two history features)
-//   x       = xb + s_o delta,  xb = x0 + (x0 - xp) (linear extrapolation
-//             in time; xb = x0 without history), s_o = s_d, or
-//             s_e = rms(r / diag) without history
-// so the network learns per-node coefficients of the history differences
-// on top of the linear extrapolation: a multi-step predictor whose output
-// vanishes with the history and has no offset of its own (an offset would be
-// the smoothest, slowest error for GMRES).  The output head starts at
-// coefficients (1, -1): an untrained network gives the quadratic
-// extrapolation 3 x0 - 3 xp + xpp, and training learns departures from it
-// (per node, from the local features).  Rows without
+//   x       = xb + s_o delta,  s_o = s_d, or s_e = rms(r / diag) without
+//             history
+// so the network learns per-node coefficients of the higher history
+// differences on top of the quadratic extrapolation: a = (1, 0) is the cubic
+// and a = (1, 1) the quartic extrapolation, other values damped or per-node
+// blends.  The predictor's output vanishes with the history and has no
+// offset of its own (an offset would be the smoothest, slowest error for
+// GMRES).  The output head starts at zero: an untrained network gives the
+// quadratic extrapolation 3 x0 - 3 xp + xpp, and training learns from the
+// recorded solves how far to extrapolate (per node, from the local
+// features).  Rows without
 // off-diagonal entries (Dirichlet rows) are decoupled: their prediction is
 // b_n / a_nn exactly, so no boundary error enters the guess.
 //
@@ -134,30 +141,47 @@ __global__ void k_edge_weights(const int2* __restrict__ edges,
   w[sl.y] = we;
 }
 
-// r = b - A x0 and the sums of b^2, x0^2, r^2, (r/|d|)^2, log|d|,
-// (x0 - xp)^2.
+// The history of guesses, newest first (h[j] valid for j < nh).
+struct Hist {
+  const double* h[kHist];
+  int nh;
+};
+
+// xb = polynomial extrapolation in time of the newest min(nh, 3) guesses.
+__global__ void k_base_guess(Hist H, double* __restrict__ xb, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double h0 = H.h[0][i];
+  xb[i] = H.nh >= 3 ? 3.0 * h0 - 3.0 * H.h[1][i] + H.h[2][i]
+        : H.nh == 2 ? 2.0 * h0 - H.h[1][i]
+                    : h0;
+}
+
+// r = b - A xb and the sums of b^2, h_0^2, r^2, (r/|d|)^2, log|d|,
+// (h_0 - h_1)^2.
 __global__ __launch_bounds__(kBlock) void k_feat_stats(
   const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
   const double* __restrict__ va, const int32_t* __restrict__ diag,
-  const double* __restrict__ b, const double* __restrict__ x0,
-  const double* __restrict__ xp, double* __restrict__ r, int n,
-  double* __restrict__ partial)
+  const double* __restrict__ b, const double* __restrict__ xb, Hist H,
+  double* __restrict__ r, int n, double* __restrict__ partial)
 {
   double acc[kStats] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
     double ax = 0.0;
-    for (int s = rp[i]; s < rp[i + 1]; ++s) ax += va[s] * x0[ci[s]];
+    for (int s = rp[i]; s < rp[i + 1]; ++s) ax += va[s] * xb[ci[s]];
     const double ri = b[i] - ax;
     r[i] = ri;
     const double d = fabs(va[diag[i]]);
     const double e = d > 0 ? ri / d : 0.0;
+    const double h0 = H.h[0][i];
     acc[0] += b[i] * b[i];
-    acc[1] += x0[i] * x0[i];
+    acc[1] += h0 * h0;
     acc[2] += ri * ri;
     acc[3] += e * e;
     acc[4] += log(d > 0 ? d : 1.0);
-    const double dx = x0[i] - xp[i];
+    const double dx = H.nh >= 2 ? h0 - H.h[1][i] : 0.0;
     acc[5] += dx * dx;
   }
   mp_block_store<kStats>(acc, partial + (int64_t)blockIdx.x * kStats);
@@ -165,7 +189,6 @@ __global__ __launch_bounds__(kBlock) void k_feat_stats(
 
 struct FeatScale {
   double isr, mlog, isd, iso;
-  double cb;  // 1 with history (xb = 2 x0 - xp), 0 without (xb = x0)
 };
 
 // Features (SoA) and, when xs is given, the target (xs - xb)/s_o.
@@ -173,10 +196,8 @@ __global__ void k_features(const int32_t* __restrict__ rp,
                            const double* __restrict__ va,
                            const int32_t* __restrict__ ci,
                            const int32_t* __restrict__ diag,
-                           const double* __restrict__ b,
-                           const double* __restrict__ x0,
-                           const double* __restrict__ xp,
-                           const double* __restrict__ xpp,
+                           const double* __restrict__ b, Hist H,
+                           const double* __restrict__ xb,
                            const double* __restrict__ r,
                            const double* __restrict__ xs, FeatScale sc,
                            float* __restrict__ feat, float* __restrict__ target,
@@ -191,12 +212,16 @@ __global__ void k_features(const int32_t* __restrict__ rp,
   feat[i] = (float)(r[i] * sc.isr);
   feat[(int64_t)n + i] = (float)(log(d > 0 ? d : 1.0) - sc.mlog);
   feat[2 * (int64_t)n + i] = (float)dir;
-  feat[3 * (int64_t)n + i] = (float)((x0[i] - xp[i]) * sc.isd);
-  feat[4 * (int64_t)n + i] = (float)((xp[i] - xpp[i]) * sc.isd);
-  if (xs) {
-    const double xb = x0[i] + sc.cb * (x0[i] - xp[i]);
-    target[i] = (float)((xs[i] - xb) * sc.iso);
+  double f3 = 0.0, f4 = 0.0;
+  if (H.nh >= 4) {
+    const double h0 = H.h[0][i], h1 = H.h[1][i], h2 = H.h[2][i], h3 = H.h[3][i];
+    f3 = (h0 - 3.0 * h1 + 3.0 * h2 - h3) * sc.isd;
+    if (H.nh >= 5)
+      f4 = (h0 - 4.0 * h1 + 6.0 * h2 - 4.0 * h3 + H.h[4][i]) * sc.isd;
   }
+  feat[3 * (int64_t)n + i] = (float)f3;
+  feat[4 * (int64_t)n + i] = (float)f4;
+  if (xs) target[i] = (float)((xs[i] - xb[i]) * sc.iso);
 }
 
 template <int H>
@@ -542,11 +567,10 @@ __global__ __launch_bounds__(kBlock) void k_layer_bf16(
 }
 
 // x = xb + s_o delta; Dirichlet rows (feature 2 set) get b / a_nn
-__global__ void k_apply_guess(const double* __restrict__ x0,
-                              const double* __restrict__ xp,
+__global__ void k_apply_guess(const double* __restrict__ xb,
                               const float* __restrict__ delta,
                               const float* __restrict__ dir, double so,
-                              double cb, const double* __restrict__ b,
+                              const double* __restrict__ b,
                               const double* __restrict__ va,
                               const int32_t* __restrict__ diag,
                               double* __restrict__ x, int n)
@@ -556,7 +580,7 @@ __global__ void k_apply_guess(const double* __restrict__ x0,
   if (dir[i] != 0.f) {
     x[i] = b[i] / va[diag[i]];
   } else {
-    x[i] = x0[i] + cb * (x0[i] - xp[i]) + so * (double)delta[i];
+    x[i] = xb[i] + so * (double)delta[i];
   }
 }
 
@@ -908,7 +932,7 @@ struct Mpnn {
   std::vector<int64_t> gmap;  // reduced value -> parameter index (-1: loss)
   std::vector<Sample> samples;
   double so = 1.0;         // output scale of the last features
-  double cb = 0.0;         // 1 when the last features had history
+  double* xb = nullptr;    // [n] base guess of the last features
   double last_ms = 0.0;
   int precision = NW_MPNN_BF16;  // arithmetic of the prediction
   int loss_kind = NW_MPNN_LOSS_RESIDUAL;  // training loss
@@ -1010,6 +1034,7 @@ int ensure_buffers(Mpnn* m, bool training)
     NW_HIP(hipMalloc(&m->delta, sizeof(float) * n));
     NW_HIP(hipMalloc(&m->dd, sizeof(float) * 2 * n));
     NW_HIP(hipMalloc(&m->r, sizeof(double) * n));
+    NW_HIP(hipMalloc(&m->xb, sizeof(double) * n));
     NW_HIP(hipMalloc(&m->partial, sizeof(double) * (size_t)kGradBlocks * m->nv));
     NW_HIP(hipMalloc(&m->red, sizeof(double) * m->nv));
     m->h.assign(2, nullptr);
@@ -1071,16 +1096,23 @@ int slot_weights(Mpnn* m)
   return NW_OK;
 }
 
-// features of (A, b, x0, xp, xpp) into feat; target from xs when given
-int features(Mpnn* m, const double* x0, const double* xp, const double* xpp,
-             const double* xs, float* feat, float* target)
+// features of (A, b, history) into feat; target from xs when given; the
+// base guess in m->xb
+int features(Mpnn* m, const double* const* hist, int nh, const double* xs,
+             float* feat, float* target)
 {
   Ctx* c = m->c;
   const Graph& g = c->g;
   const int n = (int)m->n;
   hipStream_t st = c->stream;
+  NW_REQUIRE(nh >= 1 && nh <= kHist && hist[0], NW_E_STATE,
+             "the learned guess needs at least one earlier guess");
+  Hist H{};
+  H.nh = nh;
+  for (int j = 0; j < kHist; ++j) H.h[j] = j < nh ? hist[j] : hist[0];
+  k_base_guess<<<grid_for(n, kBlock), kBlock, 0, st>>>(H, m->xb, n);
   k_feat_stats<<<kGradBlocks, kBlock, 0, st>>>(g.row_ptr, g.cols, g.vals,
-                                              g.diag, g.rhs, x0, xp, m->r, n,
+                                              g.diag, g.rhs, m->xb, H, m->r, n,
                                               m->partial);
   NW_LAUNCH_CHECK();
   k_mp_reduce<<<kStats, kBlock, 0, st>>>(m->partial, kGradBlocks, kStats, m->red);
@@ -1097,13 +1129,11 @@ int features(Mpnn* m, const double* x0, const double* xp, const double* xpp,
   sc.mlog = s[4] / (double)std::max(n, 1);
   sc.isd = 1.0 / rms(s[5]);
   m->so = s[5] > 0 ? rms(s[5]) : rms(s[3]);
-  m->cb = s[5] > 0 ? 1.0 : 0.0;
   sc.iso = 1.0 / m->so;
-  sc.cb = m->cb;
   NW_REQUIRE(std::isfinite(s[0] + s[1] + s[2] + s[3] + s[4] + s[5]),
              NW_E_NONFINITE, "non-finite input to the learned guess");
   k_features<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.row_ptr, g.vals, g.cols,
-                                                    g.diag, g.rhs, x0, xp, xpp,
+                                                    g.diag, g.rhs, H, m->xb,
                                                     m->r, xs, sc, feat, target, n);
   NW_LAUNCH_CHECK();
   return NW_OK;
@@ -1309,10 +1339,9 @@ void mpnn_detach(Ctx* c, Mpnn* m)
   if (c && c->mpnn == m) c->mpnn = nullptr;
 }
 
-// Prediction for the solver: x = xb + s_o delta from the current system,
-// x0 and the guesses before it xp, xpp (device vectors).
-int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp,
-                        const double* xpp, double* x)
+// Prediction for the solver: x = xb + s_o delta from the current system
+// and the history of guesses (device vectors, newest first).
+int mpnn_predict_device(Mpnn* m, const double* const* hist, int nh, double* x)
 {
   const auto t0 = std::chrono::steady_clock::now();
   int rc = check_graph(m);
@@ -1321,14 +1350,14 @@ int mpnn_predict_device(Mpnn* m, const double* x0, const double* xp,
   if (rc) return rc;
   rc = slot_weights(m);
   if (rc) return rc;
-  rc = features(m, x0, xp, xpp, nullptr, m->feat, nullptr);
+  rc = features(m, hist, nh, nullptr, m->feat, nullptr);
   if (rc) return rc;
   rc = forward_predict(m, m->feat);
   if (rc) return rc;
   const Graph& g = m->c->g;
   k_apply_guess<<<grid_for(m->n, kBlock), kBlock, 0, m->c->stream>>>(
-    x0, xp, m->delta, m->feat + 2 * m->n, m->so, m->cb, g.rhs, g.vals, g.diag,
-    x, (int)m->n);
+    m->xb, m->delta, m->feat + 2 * m->n, m->so, g.rhs, g.vals, g.diag, x,
+    (int)m->n);
   NW_LAUNCH_CHECK();
   NW_HIP(hipStreamSynchronize(m->c->stream));
   m->last_ms = 1e3 * std::chrono::duration<double>(
@@ -1384,11 +1413,10 @@ int nw_mpnn_create(nw_ctx* c, int hidden, int layers, uint64_t seed,
     fill(o.ws(l), (int64_t)hidden * hidden, 2 * hidden);
     fill(o.wn(l, hidden), (int64_t)hidden * hidden, 2 * hidden);
   }
-  // output head: w_out = 0 and b_out = (1, -1), so the untrained network
-  // predicts the quadratic extrapolation 3 x0 - 3 xp + xpp (xb plus the
-  // second difference); training learns per-node departures from it
-  m->P[o.bout] = 1.0;
-  m->P[o.bout + 1] = -1.0;
+  // output head: w_out = 0 and b_out = 0, so the untrained network
+  // predicts the base guess, the quadratic extrapolation 3 x0 - 3 xp + xpp;
+  // training learns (per node) how much of the third and fourth history
+  // differences to add
   if (hidden == 64) build_gmap<64>(m);
   else build_gmap<32>(m);
   int rc = NW_OK;
@@ -1428,6 +1456,7 @@ void nw_mpnn_destroy(nw_mpnn* m)
   (void)hipFree(m->delta);
   (void)hipFree(m->dd);
   (void)hipFree(m->r);
+  (void)hipFree(m->xb);
   (void)hipFree(m->partial);
   (void)hipFree(m->red);
   for (auto p : m->hb) (void)hipFree(p);
@@ -1467,7 +1496,7 @@ int nw_mpnn_record(nw_mpnn* m)
   NW_REQUIRE(m, NW_E_ARG, "null argument");
   int rc = check_graph(m);
   if (rc) return rc;
-  NW_REQUIRE(m->c->x_init, NW_E_STATE,
+  NW_REQUIRE(m->c->nh > 0, NW_E_STATE,
              "no solve has run on this system yet (nothing to record)");
   NW_HIP(hipSetDevice(m->c->device));
   rc = ensure_buffers(m, false);
@@ -1479,8 +1508,7 @@ int nw_mpnn_record(nw_mpnn* m)
     set_error("device allocation of a training sample failed");
     return NW_E_ALLOC;
   }
-  rc = features(m, m->c->x_init, m->c->x_hist, m->c->x_hist2, m->c->g.sln,
-                s.feat, s.target);
+  rc = features(m, m->c->xh, m->c->nh, m->c->g.sln, s.feat, s.target);
   if (rc) {
     free_sample(s);
     return rc;
@@ -1618,11 +1646,13 @@ int nw_mpnn_predict(nw_mpnn* m, double* x_host, double* scale)
   const int64_t n = m->n;
   double* x = nullptr;
   NW_HIP(hipMalloc(&x, sizeof(double) * n));
-  // x0 = the solution vector, xp / xpp = the guesses the last two solves
-  // started from
-  const double* xp = m->c->x_init ? m->c->x_init : m->c->g.sln;
-  const double* xpp = m->c->x_hist ? m->c->x_hist : xp;
-  int rc = mpnn_predict_device(m, m->c->g.sln, xp, xpp, x);
+  // the next step's guess: the solution vector, then the guesses the
+  // latest solves started from
+  const double* hist[kHist];
+  hist[0] = m->c->g.sln;
+  int nh = 1;
+  for (int j = 0; j < m->c->nh && nh < kHist; ++j) hist[nh++] = m->c->xh[j];
+  int rc = mpnn_predict_device(m, hist, nh, x);
   if (!rc) {
     if (hipMemcpy(x_host, x, sizeof(double) * n, hipMemcpyDeviceToHost) !=
         hipSuccess) {

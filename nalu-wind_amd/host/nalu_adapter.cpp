@@ -282,6 +282,8 @@ void ContinuityEdgeSolverAlg::execute()
   o.interp_together = so.mdotInterpRhoUTogether ? 1.0 : 0.0;
   o.incompressible = so.solveIncompressibleContinuity ? 1.0 : 0.0;
   o.accumulate = 1;
+  o.balanced_buoyancy = so.use_balanced_buoyancy_force_ ? 1 : 0;
+  for (int d = 0; d < 3; ++d) o.gravity[d] = so.gravity[d];
   require_system(eqSystem_, "ContinuityEdgeSolverAlg");
   check(nw_assemble_continuity(realm_.ctx(), &o), "ContinuityEdgeSolverAlg");
 }
@@ -309,7 +311,11 @@ void MdotEdgeAlg::execute()
   const SolutionOptions& so = realm_.solutionOptions;
   auto it = so.nocMap.find("pressure");
   const bool noc = it == so.nocMap.end() ? so.nocDefault : it->second;
-  const nw_mdot_opts o{noc ? 1.0 : 0.0, so.mdotInterpRhoUTogether ? 1.0 : 0.0};
+  nw_mdot_opts o{};
+  o.noc = noc ? 1.0 : 0.0;
+  o.interp_together = so.mdotInterpRhoUTogether ? 1.0 : 0.0;
+  o.balanced_buoyancy = so.use_balanced_buoyancy_force_ ? 1 : 0;
+  for (int d = 0; d < 3; ++d) o.gravity[d] = so.gravity[d];
   check(nw_compute_mdot(realm_.ctx(), &o), "MdotEdgeAlg");
 }
 

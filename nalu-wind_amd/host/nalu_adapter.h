@@ -49,6 +49,12 @@ struct SolutionOptions {
   double includeDivU = 0.0;
   bool mdotInterpRhoUTogether = true;
   bool solveIncompressibleContinuity = false;
+  // use_balanced_buoyancy_force_ and the gravity vector
+  // (include/SolutionOptions.h; read by ContinuityEdgeSolverAlg and
+  // MdotEdgeAlg, which then also read the "buoyancy_source" and
+  // "buoyancy_source_mask" node fields)
+  bool use_balanced_buoyancy_force_ = false;
+  double gravity[3] = {0.0, 0.0, 0.0};
   std::map<std::string, double> hybridMap, alphaMap, alphaUpwMap, upwMap,
     relaxMap;
   std::map<std::string, bool> limiterMap, nocMap;

@@ -30,7 +30,8 @@ OK, E_ARG, E_STATE, E_HIP, E_NOGRAPHCOL, E_NONFINITE, E_NOCONV, E_ALLOC = (
 (F_COORDINATES, F_VELOCITY, F_DENSITY, F_PRESSURE, F_DPDX, F_UDIAG,
  F_VISCOSITY, F_DUDX, F_WALL_MASK, F_SCALAR_Q, F_DQDX, F_DIFF_FLUX_COEFF,
  F_DUAL_VOLUME, F_BC_VALUE, E_AREA_VECTOR, E_MASS_FLOW_RATE, E_PECLET_FACTOR,
- E_PECLET_NUMBER, F_PRESSURE_BC, F_UTMP) = range(20)
+ E_PECLET_NUMBER, F_PRESSURE_BC, F_UTMP, F_BUOYANCY_SOURCE,
+ F_BUOYANCY_SOURCE_MASK) = range(22)
 LINSYS_SOLUTION = -1  # nw_solution_update: the linear system's solution
 
 PREC_NONE, PREC_JACOBI, PREC_CHEBYSHEV, PREC_AMG = 0, 1, 2, 3
@@ -66,13 +67,15 @@ class MomentumOpts(C.Structure):
 
 
 class MdotOpts(C.Structure):
-    _fields_ = [("noc", C.c_double), ("interp_together", C.c_double)]
+    _fields_ = [("noc", C.c_double), ("interp_together", C.c_double),
+                ("balanced_buoyancy", C.c_int), ("gravity", C.c_double * 3)]
 
 
 class ContinuityOpts(C.Structure):
     _fields_ = [("dt", C.c_double), ("gamma1", C.c_double), ("noc", C.c_double),
                 ("interp_together", C.c_double),
-                ("incompressible", C.c_double), ("accumulate", C.c_int)]
+                ("incompressible", C.c_double), ("accumulate", C.c_int),
+                ("balanced_buoyancy", C.c_int), ("gravity", C.c_double * 3)]
 
 
 class SolverCfg(C.Structure):
@@ -165,6 +168,7 @@ _sig = {
                                    C.c_int, C.c_int], C.c_int),
     "nw_fix_pressure_at_node": ([_P, C.c_int32, C.c_double], C.c_int),
     "nw_linsys_set_skipped_rows": ([_P, C.c_int64, _i32p], C.c_int),
+    "nw_ctx_set_check_finite": ([_P, C.c_int], C.c_int),
     "nw_linsys_sum_into_device": ([_P, C.c_int, C.c_int, _P, _P, _P], C.c_int),
     "nw_linsys_sum_into_device_noalloc": ([_P, C.c_int, C.c_int, _P, _P, _P],
                                           C.c_int),
@@ -487,6 +491,10 @@ class Context:
         check(lib.nw_linsys_sum_into(self.h, nb, nent, nodes.ravel(),
                                      _f64(lhs).ravel(), _f64(rhs).ravel()))
 
+    def set_check_finite(self, on=True):
+        """Assemblies return E_NONFINITE when the system holds NaN/Inf."""
+        check(lib.nw_ctx_set_check_finite(self.h, int(on)))
+
     def sum_into_device(self, d_nodes, d_lhs, d_rhs, nblocks, nent):
         """sumInto of device arrays (DeviceBuffer or raw pointers)."""
         ptr = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
@@ -536,9 +544,15 @@ class Context:
         check(lib.nw_assemble_momentum_with_peclet(self.h, C.byref(opts),
                                                    C.byref(peclet)))
 
-    def compute_mdot(self, noc=1.0, interp_together=1.0):
-        """MdotEdgeAlg: NW_E_MASS_FLOW_RATE from the node fields."""
-        check(lib.nw_compute_mdot(self.h, C.byref(MdotOpts(noc, interp_together))))
+    def compute_mdot(self, noc=1.0, interp_together=1.0, gravity=None):
+        """MdotEdgeAlg: NW_E_MASS_FLOW_RATE from the node fields; with
+        `gravity` (3 values) the balanced buoyancy force is on and the
+        F_BUOYANCY_SOURCE / F_BUOYANCY_SOURCE_MASK fields are read."""
+        o = MdotOpts(noc, interp_together, 0)
+        if gravity is not None:
+            o.balanced_buoyancy = 1
+            o.gravity = (C.c_double * 3)(*[float(g) for g in gravity])
+        check(lib.nw_compute_mdot(self.h, C.byref(o)))
 
     def nodal_grad(self, field_in, field_out, accumulate=0):
         """NodalGradEdgeAlg: field_out = (or +=, with accumulate) the

@@ -30,6 +30,11 @@ def run(kernel, n, reps, copy_mib):
         W = wl.continuity_rf(n)
         ndof = 1
         fn = lambda ctx: ctx.assemble_continuity(W.opts)
+    elif kernel == "mixed":
+        # continuity on the mixed Hex8/Tet4/Wed6 box (RCM-numbered)
+        W = wl.continuity_mixed(n)
+        ndof = 1
+        fn = lambda ctx: ctx.assemble_continuity(W.opts)
     elif kernel == "edge_algs":
         # MdotEdgeAlg and NodalGradEdgeAlg (dp/dx, du/dx) on the continuity box
         W = wl.continuity_rf(n)

@@ -120,13 +120,24 @@ void orc_nodal_grad_edge(int64_t nnode, int64_t nedge, const int32_t* edges,
  * (unit_tests/kernels/UnitTestKernelUtils.C:725-769). */
 void orc_mdot_edge(int64_t nedge, const int32_t* edges, const double* areav,
                    const double* rho, const double* vel, double* mdot);
+/* The balanced buoyancy force option (use_balanced_buoyancy_force_) of
+ * ContinuityEdgeSolverAlg (src/edge_kernels/ContinuityEdgeSolverAlg.C:
+ * 58-88,147-152,168-173) and MdotEdgeAlg (src/ngp_algorithms/MdotEdgeAlg.C:
+ * 81-102,147-154,163-169): gravity vector, node field "buoyancy_source"
+ * [n][3] and node field "buoyancy_source_mask" [n]. */
+typedef struct {
+  double gravity[3];
+  const double* source;
+  const double* mask;
+} orc_buoyancy;
+
 /* MdotEdgeAlg (src/ngp_algorithms/MdotEdgeAlg.C:121-186), interior edges,
- * without the balanced-buoyancy and mesh-motion terms. */
+ * without the mesh-motion (GCL) term; buoy NULL = balanced buoyancy off. */
 void orc_mdot_edge_alg(int64_t nedge, const int32_t* edges,
                        const double* coords, const double* vel,
                        const double* dpdx, const double* rho, const double* p,
                        const double* udiag, const double* areav, double noc,
-                       double interp, double* mdot);
+                       double interp, const orc_buoyancy* buoy, double* mdot);
 
 /* ------------------------------------------------------------------ */
 /* Peclet blending and limiter                                         */
@@ -198,11 +209,12 @@ typedef struct {
   double noc;            /* 1 if non-orthogonal correction is used */
   double interp_together;/* mdot interp rho*U together (1) or not (0) */
   double incompressible; /* solve incompressible continuity (1) or not */
+  const orc_buoyancy* buoy; /* balanced buoyancy force, NULL = off */
 } orc_continuity_opts;
 
 /* One edge of ContinuityEdgeSolverAlg::execute
- * (src/edge_kernels/ContinuityEdgeSolverAlg.C:38-195), no balanced
- * buoyancy forcing and no GCL term. lhs [2][2], rhs [2]. */
+ * (src/edge_kernels/ContinuityEdgeSolverAlg.C:38-195), with the balanced
+ * buoyancy forcing when o->buoy is set; no GCL term. lhs [2][2], rhs [2]. */
 void orc_continuity_edge(const orc_continuity_opts* o, int64_t e,
                          int32_t nL, int32_t nR, const double* coords,
                          const double* vel, const double* dpdx,
@@ -372,9 +384,14 @@ double orc_jacobi_lambda_max(const orc_csr* A, int iters, uint64_t seed);
 int64_t orc_mpnn_nparams(int H, int L);
 /* w[nnz]: sqrt(|a_nm a_mn|)/sqrt(|a_nn a_mm|) off the diagonal, 0 on it */
 void orc_mpnn_slot_weights(const orc_csr* A, double* w);
-/* feat[N][5]; scales = {s_b, s_x, s_r, s_e, s_o} (s_o: output scale) */
-void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
-                       const double* xp, const double* xpp, double* feat,
+/* xb: the base guess, polynomial extrapolation in time of the newest
+ * min(nh, 3) of the history hist[0..nh-1] (hist[0] the newest). */
+void orc_mpnn_base_guess(int64_t n, const double* const* hist, int nh,
+                         double* xb);
+/* feat[N][5]; scales = {s_b, s_x, s_r, s_e, s_o} (s_o: output scale);
+ * hist[0..nh-1] the guesses of the latest solves, newest first, nh 1..5 */
+void orc_mpnn_features(const orc_csr* A, const double* b,
+                       const double* const* hist, int nh, double* feat,
                        double* scales);
 void orc_mpnn_forward(const double* P, int H, int L, const orc_csr* A,
                       const double* w, const double* feat, double* delta,

@@ -47,10 +47,17 @@ class MomentumOpts(C.Structure):
                 ("include_divU", C.c_double), ("use_limiter", C.c_int)]
 
 
+class Buoyancy(C.Structure):
+    """orc_buoyancy: gravity, buoyancy_source [n][3], buoyancy_source_mask."""
+    _fields_ = [("gravity", C.c_double * 3), ("source", C.POINTER(C.c_double)),
+                ("mask", C.POINTER(C.c_double))]
+
+
 class ContinuityOpts(C.Structure):
     _fields_ = [("dt", C.c_double), ("gamma1", C.c_double), ("noc", C.c_double),
                 ("interp_together", C.c_double),
-                ("incompressible", C.c_double)]
+                ("incompressible", C.c_double),
+                ("buoy", C.POINTER(Buoyancy))]
 
 
 class Csr(C.Structure):
@@ -88,7 +95,7 @@ lib.orc_nodal_grad_edge.argtypes = [C.c_int64, C.c_int64, _i32p, _f64p, _f64p,
 lib.orc_mdot_edge.argtypes = [C.c_int64, _i32p, _f64p, _f64p, _f64p, _f64p]
 lib.orc_mdot_edge_alg.argtypes = [C.c_int64, _i32p, _f64p, _f64p, _f64p, _f64p,
                                   _f64p, _f64p, _f64p, C.c_double, C.c_double,
-                                  _f64p]
+                                  C.POINTER(Buoyancy), _f64p]
 lib.orc_peclet_classic.argtypes = [C.c_double, C.c_double]
 lib.orc_peclet_classic.restype = C.c_double
 lib.orc_peclet_tanh.argtypes = [C.c_double, C.c_double, C.c_double]
@@ -272,13 +279,35 @@ def mdot_edge(edges, areav, rho, vel):
     return out
 
 
+def buoyancy(gravity, source, mask):
+    """The balanced buoyancy force inputs (orc_buoyancy): gravity vector,
+    buoyancy_source [n][3], buoyancy_source_mask [n]; the arrays are kept
+    alive by the returned structure."""
+    src = np.ascontiguousarray(source, dtype=np.float64).ravel()
+    msk = np.ascontiguousarray(mask, dtype=np.float64).ravel()
+    b = Buoyancy((C.c_double * 3)(*[float(g) for g in gravity]),
+                 src.ctypes.data_as(C.POINTER(C.c_double)),
+                 msk.ctypes.data_as(C.POINTER(C.c_double)))
+    b.keep_alive = (src, msk)
+    return b
+
+
+def with_buoyancy(opts, buoy):
+    """A copy of ContinuityOpts with the balanced buoyancy force on."""
+    o = ContinuityOpts(opts.dt, opts.gamma1, opts.noc, opts.interp_together,
+                       opts.incompressible, C.pointer(buoy))
+    return o
+
+
 def mdot_edge_alg(edges, coords, vel, dpdx, rho, p, udiag, areav, noc=1.0,
-                  interp=1.0):
-    """MdotEdgeAlg interior edges: node arrays [n][3] / [n], areav [e][3]."""
+                  interp=1.0, buoy=None):
+    """MdotEdgeAlg interior edges: node arrays [n][3] / [n], areav [e][3];
+    buoy: a `buoyancy(...)` structure for the balanced buoyancy force."""
     out = np.zeros(edges.shape[0])
     lib.orc_mdot_edge_alg(edges.shape[0], i32(edges).ravel(), f64(coords).ravel(),
                           f64(vel).ravel(), f64(dpdx).ravel(), f64(rho), f64(p),
-                          f64(udiag), f64(areav).ravel(), noc, interp, out)
+                          f64(udiag), f64(areav).ravel(), noc, interp,
+                          C.byref(buoy) if buoy is not None else None, out)
     return out
 
 
@@ -495,8 +524,11 @@ def assemble_continuity(opts, nnode, edges, coords, vel, dpdx, rho, p, udiag,
 lib.orc_mpnn_nparams.argtypes = [C.c_int, C.c_int]
 lib.orc_mpnn_nparams.restype = C.c_int64
 lib.orc_mpnn_slot_weights.argtypes = [C.POINTER(Csr), _f64p]
-lib.orc_mpnn_features.argtypes = [C.POINTER(Csr), _f64p, _f64p, _f64p, _f64p,
+lib.orc_mpnn_features.argtypes = [C.POINTER(Csr), _f64p,
+                                  C.POINTER(C.POINTER(C.c_double)), C.c_int,
                                   _f64p, _f64p]
+lib.orc_mpnn_base_guess.argtypes = [C.c_int64, C.POINTER(C.POINTER(C.c_double)),
+                                    C.c_int, _f64p]
 lib.orc_mpnn_forward.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr), _f64p,
                                  _f64p, _f64p, C.c_void_p]
 lib.orc_mpnn_forward_bf16.argtypes = [_f64p, C.c_int, C.c_int, C.POINTER(Csr),
@@ -524,15 +556,33 @@ def mpnn_slot_weights(A):
 MPNN_NF = 5
 
 
-def mpnn_features(A, b, x0, xp=None, xpp=None):
-    """Node features and scales {s_b, s_x, s_r, s_e, s_o}; xp and xpp (the
-    guesses before x0) default to x0 and xp (no history)."""
+def _hist_ptrs(hist):
+    arrs = [np.ascontiguousarray(h, dtype=np.float64) for h in hist]
+    ptrs = (C.POINTER(C.c_double) * len(arrs))(
+        *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
+    return arrs, ptrs
+
+
+def mpnn_features(A, b, *hist):
+    """Node features and scales {s_b, s_x, s_r, s_e, s_o} from the history
+    of guesses hist = (x0, xp, xpp, ...), newest first, 1 to 5 of them."""
+    hist = [h for h in hist if h is not None]
+    assert 1 <= len(hist) <= 5
     feat = np.zeros((A.c.nrows, MPNN_NF))
     sc = np.zeros(5)
-    xp = x0 if xp is None else xp
-    xpp = xp if xpp is None else xpp
-    lib.orc_mpnn_features(A.ref, f64(b), f64(x0), f64(xp), f64(xpp), feat, sc)
+    arrs, ptrs = _hist_ptrs(hist)
+    lib.orc_mpnn_features(A.ref, f64(b), ptrs, len(arrs), feat, sc)
     return feat, sc
+
+
+def mpnn_base_guess(*hist):
+    """The base guess xb: polynomial extrapolation in time of the newest
+    min(len(hist), 3) guesses."""
+    hist = [h for h in hist if h is not None]
+    arrs, ptrs = _hist_ptrs(hist)
+    out = np.zeros(arrs[0].shape[0])
+    lib.orc_mpnn_base_guess(out.shape[0], ptrs, len(arrs), out)
+    return out
 
 
 def mpnn_forward(P, H, L, A, w, feat):

@@ -357,14 +357,25 @@ void orc_continuity_edge(const orc_continuity_opts* o, int64_t e,
 
   double tmdot = -projTimeScale * (pressureR - pressureL) * asq * inv_axdx;
 
+  /* ContinuityEdgeSolverAlg.C:147-152 */
+  const orc_buoyancy* bf = o->buoy;
+  if (bf) {
+    const double masked_weights = 0.5 * (bf->mask[nL] + bf->mask[nR]);
+    for (int d = 0; d < 3; ++d)
+      tmdot += projTimeScale * av[d] * bf->gravity[d] * rhoIp * masked_weights;
+  }
+
   for (int d = 0; d < 3; ++d) {
     const double dxj = coords[3 * nR + d] - coords[3 * nL + d];
     const double kxj = av[d] - asq * inv_axdx * dxj;
     const double rhoUjIp =
       0.5 * (densityR * vel[3 * nR + d] + densityL * vel[3 * nL + d]);
     const double ujIp = 0.5 * (vel[3 * nR + d] + vel[3 * nL + d]);
-    const double GjIp =
+    double GjIp =
       0.5 * (dpdx[3 * nR + d] / (udiagR) + dpdx[3 * nL + d] / (udiagL));
+    if (bf) /* ContinuityEdgeSolverAlg.C:168-173 */
+      GjIp -= 0.5 * ((bf->mask[nR] * bf->source[3 * nR + d]) / (udiagR) +
+                     (bf->mask[nL] * bf->source[3 * nL + d]) / (udiagL));
     tmdot += (interpTogether * rhoUjIp + om_interpTogether * rhoIp * ujIp +
               GjIp) * av[d] -
              kxj * GjIp * nocFac;

@@ -467,7 +467,7 @@ void orc_mdot_edge_alg(int64_t nedge, const int32_t* edges,
                        const double* coords, const double* vel,
                        const double* dpdx, const double* rho, const double* p,
                        const double* udiag, const double* areav, double noc,
-                       double interp, double* mdot)
+                       double interp, const orc_buoyancy* buoy, double* mdot)
 {
   for (int64_t e = 0; e < nedge; ++e) {
     const int32_t L = edges[2 * e], R = edges[2 * e + 1];
@@ -482,11 +482,19 @@ void orc_mdot_edge_alg(int64_t nedge, const int32_t* edges,
     }
     const double inv_axdx = 1.0 / axdx;
     double t = -projTimeScale * (p[R] - p[L]) * asq * inv_axdx;
+    if (buoy) { /* MdotEdgeAlg.C:147-154 */
+      const double masked_weights = 0.5 * (buoy->mask[L] + buoy->mask[R]);
+      for (int d = 0; d < 3; ++d)
+        t += projTimeScale * av[d] * buoy->gravity[d] * rhoIp * masked_weights;
+    }
     for (int d = 0; d < 3; ++d) {
       const double kxj = av[d] - asq * inv_axdx * dx[d];
       const double rhoUjIp = 0.5 * (rho[R] * vel[3 * R + d] + rho[L] * vel[3 * L + d]);
       const double ujIp = 0.5 * (vel[3 * R + d] + vel[3 * L + d]);
-      const double GjIp = 0.5 * (dpdx[3 * R + d] / udiag[R] + dpdx[3 * L + d] / udiag[L]);
+      double GjIp = 0.5 * (dpdx[3 * R + d] / udiag[R] + dpdx[3 * L + d] / udiag[L]);
+      if (buoy) /* MdotEdgeAlg.C:163-169 */
+        GjIp -= 0.5 * ((buoy->mask[R] * buoy->source[3 * R + d]) / udiag[R] +
+                       (buoy->mask[L] * buoy->source[3 * L + d]) / udiag[L]);
       t += (interp * rhoUjIp + (1.0 - interp) * rhoIp * ujIp + GjIp) * av[d] -
            kxj * GjIp * noc;
     }

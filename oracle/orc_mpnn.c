@@ -13,17 +13,25 @@
  *   a_k,n   = w_out,k . h_L + b_out,k,   k = 1, 2
  *   delta_n = a_1,n f_3,n + a_2,n f_4,n   (f_3, f_4 the history features)
  * Edge weights w_nm = sqrt(|a_nm a_mn|) / sqrt(|a_nn a_mm|) (symmetric).
- * Node features: r/s_r (r = b - A x0), log|a_nn| - mean, Dirichlet flag,
- * (x0 - xp)/s_d and (xp - xpp)/s_d with xp, xpp the guesses of the two
- * solves before (the solutions two and three steps back) and
- * s_d = rms(x0 - xp): differences only, no absolute levels, so that what
- * is learned on earlier steps carries over to later ones.  Prediction x = xb + s_o delta
- * with xb = 2 x0 - xp and s_o = s_d, or xb = x0 and s_o = s_e =
- * rms(r / diag) when x0 == xp (no history); rows without off-diagonal
+ * History: h_0 = x0 (the newest guess, the previous step's solution),
+ * h_1 .. h_{nh-1} the guesses of the solves before (nh <= 5).  Base guess
+ * xb = the polynomial extrapolation in time of the newest min(nh, 3)
+ * (h_0; 2 h_0 - h_1; 3 h_0 - 3 h_1 + h_2).  Node features: r/s_r
+ * (r = b - A xb), log|a_nn| - mean, Dirichlet flag, and the third and
+ * fourth backward differences of the history, f_3 = (h_0 - 3 h_1 + 3 h_2
+ * - h_3)/s_d (0 when nh < 4), f_4 = (h_0 - 4 h_1 + 6 h_2 - 4 h_3 + h_4)/s_d
+ * (0 when nh < 5), with s_d = rms(h_0 - h_1): differences only, no absolute
+ * levels, so that what is learned on earlier steps carries over to later
+ * ones.  Prediction x = xb + s_o delta with s_o = s_d, or s_o = s_e =
+ * rms(r / diag) when h_0 == h_1 (no history); rows without off-diagonal
  * entries take b / a_nn.
- * The network predicts per-node coefficients of the two history
- * differences (a learned multi-step extrapolation in time), so its output
- * vanishes with the history and carries no offset of its own.
+ * The network predicts per-node coefficients of the two higher
+ * differences: a = (1, 0) turns the quadratic extrapolation into the
+ * cubic one, a = (1, 1) into the quartic one, and other values into
+ * damped or per-node blends (a learned multi-step extrapolation in time),
+ * so its output vanishes with the history and carries no offset of its
+ * own.  The untrained head (w_out = 0, b_out = 0) is the quadratic
+ * extrapolation.
  * Parameters, flat: W_in[H][F], b_in[H], then per layer W_self[H][H],
  * W_nbr[H][H], b[H], then w_out[2][H], b_out[2].
  */
@@ -73,26 +81,40 @@ void orc_mpnn_slot_weights(const orc_csr* A, double* w)
   }
 }
 
-void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
-                       const double* xp, const double* xpp, double* feat,
+void orc_mpnn_base_guess(int64_t n, const double* const* hist, int nh,
+                         double* xb)
+{
+  for (int64_t i = 0; i < n; ++i) {
+    const double h0 = hist[0][i];
+    xb[i] = nh >= 3 ? 3.0 * h0 - 3.0 * hist[1][i] + hist[2][i]
+          : nh == 2 ? 2.0 * h0 - hist[1][i]
+                    : h0;
+  }
+}
+
+void orc_mpnn_features(const orc_csr* A, const double* b,
+                       const double* const* hist, int nh, double* feat,
                        double* scales)
 {
   const int64_t n = A->nrows;
+  double* xb = (double*)malloc(sizeof(double) * (size_t)n);
   double* r = (double*)malloc(sizeof(double) * (size_t)n);
   double* e = (double*)malloc(sizeof(double) * (size_t)n);
   double* d = (double*)malloc(sizeof(double) * (size_t)n);
-  orc_spmv(A, x0, r);
+  const double* h0 = hist[0];
+  orc_mpnn_base_guess(n, hist, nh, xb);
+  orc_spmv(A, xb, r);
   double mlog = 0.0, dd = 0.0;
   for (int64_t i = 0; i < n; ++i) {
     r[i] = b[i] - r[i];
     const double a = fabs(entry(A, i, (int32_t)i));
     e[i] = a > 0 ? r[i] / a : 0.0;
-    d[i] = x0[i] - xp[i];
+    d[i] = nh >= 2 ? h0[i] - hist[1][i] : 0.0;
     dd += d[i] * d[i];
     mlog += log(a > 0 ? a : 1.0);
   }
   mlog /= (double)(n > 0 ? n : 1);
-  const double sb = rms(b, n), sx = rms(x0, n), sr = rms(r, n), se = rms(e, n);
+  const double sb = rms(b, n), sx = rms(h0, n), sr = rms(r, n), se = rms(e, n);
   const double sd = rms(d, n);
   for (int64_t i = 0; i < n; ++i) {
     const double a = fabs(entry(A, i, (int32_t)i));
@@ -102,14 +124,20 @@ void orc_mpnn_features(const orc_csr* A, const double* b, const double* x0,
     feat[NF * i + 0] = r[i] / sr;
     feat[NF * i + 1] = log(a > 0 ? a : 1.0) - mlog;
     feat[NF * i + 2] = (double)dir;
-    feat[NF * i + 3] = d[i] / sd;
-    feat[NF * i + 4] = (xp[i] - xpp[i]) / sd;
+    feat[NF * i + 3] =
+      nh >= 4 ? (h0[i] - 3.0 * hist[1][i] + 3.0 * hist[2][i] - hist[3][i]) / sd
+              : 0.0;
+    feat[NF * i + 4] =
+      nh >= 5 ? (h0[i] - 4.0 * hist[1][i] + 6.0 * hist[2][i] -
+                 4.0 * hist[3][i] + hist[4][i]) / sd
+              : 0.0;
   }
   scales[0] = sb;
   scales[1] = sx;
   scales[2] = sr;
   scales[3] = se;
   scales[4] = dd > 0 ? sd : se; /* output scale */
+  free(xb);
   free(r);
   free(e);
   free(d);

@@ -2,6 +2,7 @@
 unit-test golds, then the oracle on seeded skewed meshes over the scheme
 options, determinism, the sumInto / resetRows / Dirichlet / fix-node row
 operations and error paths."""
+import ctypes
 import json
 import os
 
@@ -146,6 +147,82 @@ def test_continuity_matches_oracle(nw, ctx, skewed, kw):
     ctx.build_graph(1)
     ctx.assemble_continuity(o)
     pb.assert_csr_close(ctx.download(), skewed.oracle_continuity(pb.to_orc_continuity(o)))
+
+
+def _upload_buoyancy(nw, ctx, P, seed, mask=None):
+    rng = np.random.default_rng(seed)
+    src = rng.normal(size=(P.n, 3))
+    if mask is None:
+        mask = rng.uniform(0.0, 1.0, P.n)
+    ctx.field_upload(nw.F_BUOYANCY_SOURCE, src)
+    ctx.field_upload(nw.F_BUOYANCY_SOURCE_MASK, mask)
+    return src, mask
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(noc=0.0, interp_together=0.0),
+                                dict(incompressible=1.0, dt=0.3, gamma1=1.5)])
+def test_continuity_balanced_buoyancy_matches_oracle(nw, ctx, skewed, kw):
+    """ContinuityEdgeSolverAlg with use_balanced_buoyancy_force_
+    (ContinuityEdgeSolverAlg.C:147-152,168-173) on the jittered box, random
+    source and mask in [0, 1], against the oracle."""
+    g = (0.3, -0.7, -9.81)
+    o = nw.continuity_defaults(**kw)
+    o.balanced_buoyancy = 1
+    o.gravity = (ctypes.c_double * 3)(*g)
+    skewed.upload(ctx)
+    src, mask = _upload_buoyancy(nw, ctx, skewed, 17)
+    ctx.build_graph(1)
+    ctx.assemble_continuity(o)
+    b = orc.buoyancy(g, src, mask)
+    A = orc.CsrSystem(skewed.n, skewed.edges, 1)
+    orc.assemble_continuity(orc.with_buoyancy(pb.to_orc_continuity(o), b),
+                            skewed.n, skewed.edges, skewed.coords, skewed.vel,
+                            skewed.dpdx, skewed.rho, skewed.p, skewed.udiag,
+                            skewed.areav, csr=A)
+    pb.assert_csr_close(ctx.download(), A)
+    # MdotEdgeAlg with the same force
+    ctx.compute_mdot(kw.get("noc", 1.0), kw.get("interp_together", 1.0), gravity=g)
+    got = ctx.field_download(nw.E_MASS_FLOW_RATE)
+    want = orc.mdot_edge_alg(skewed.edges, skewed.coords, skewed.vel, skewed.dpdx,
+                             skewed.rho, skewed.p, skewed.udiag, skewed.areav,
+                             kw.get("noc", 1.0), kw.get("interp_together", 1.0),
+                             buoy=b)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+def test_balanced_buoyancy_hydrostatic_rest_on_device(nw, ctx):
+    """A fluid at rest in hydrostatic balance on a rectilinear stretched box
+    (p = rho g.x, u = 0, dp/dx = source = rho g, any mask, uniform Udiag):
+    the device continuity rhs and mdot vanish to rounding, as the oracle's
+    closed form fixes, while the same state without the force carries the
+    flux -tau rho g.A."""
+    P = pb.Problem(10, 8, 9, stretch=1.12, seed=5)
+    g = np.array([0.3, -0.2, -9.81])
+    rho0 = 1.3
+    P.vel[:] = 0.0
+    P.rho[:] = rho0
+    P.p = rho0 * P.coords @ g + 0.25
+    P.dpdx = np.tile(rho0 * g, (P.n, 1))
+    P.udiag[:] = 2.0
+    P.upload(ctx)
+    ctx.field_upload(nw.F_BUOYANCY_SOURCE, P.dpdx)
+    ctx.field_upload(nw.F_BUOYANCY_SOURCE_MASK,
+                     np.random.default_rng(3).uniform(0.0, 1.0, P.n))
+    ctx.build_graph(1)
+    o = nw.continuity_defaults(dt=0.5)
+    o.balanced_buoyancy = 1
+    o.gravity = (ctypes.c_double * 3)(*g)
+    ctx.assemble_continuity(o)
+    rhs = ctx.download()[3]
+    scale = np.abs(P.areav).max() * rho0 * np.abs(g).max()
+    assert np.abs(rhs).max() <= 1e-12 * scale
+    ctx.compute_mdot(gravity=g)
+    assert np.abs(ctx.field_download(nw.E_MASS_FLOW_RATE)).max() <= 1e-12 * scale
+    ctx.field_upload(nw.F_DPDX, np.zeros((P.n, 3)))
+    ctx.compute_mdot()
+    np.testing.assert_allclose(ctx.field_download(nw.E_MASS_FLOW_RATE),
+                               -0.5 * rho0 * P.areav @ g, rtol=1e-12,
+                               atol=1e-13 * scale)
 
 
 SCAL_CASES = [dict(), dict(alpha=0.5, alpha_upw=0.3, upw=0.7, relax=0.8),
@@ -375,6 +452,42 @@ def test_bad_calls_are_errors(nw, ctx, skewed):
         ctx.field_upload(nw.F_VELOCITY, np.zeros(5))
     with pytest.raises(nw.NwError):
         ctx.mesh_set(4, np.array([[0, 0]], np.int32))
+
+
+def test_nonfinite_input_is_an_error(nw, ctx, skewed):
+    """With the finite check on, a NaN in an input field (pressure of one
+    node, for continuity; velocity, for momentum) makes the assembly return
+    E_NONFINITE naming a row next to that node; finite inputs pass and the
+    check changes no value."""
+    P = skewed
+    P.upload(ctx)
+    ctx.build_graph(1)
+    ctx.assemble_continuity(nw.continuity_defaults())
+    ref = ctx.download()
+    ctx.set_check_finite(True)
+    ctx.assemble_continuity(nw.continuity_defaults())
+    got = ctx.download()
+    assert all(np.array_equal(a, b) for a, b in zip(ref, got))
+    bad = 57
+    nbrs = set(P.edges[(P.edges == bad).any(1)].ravel().tolist())
+    p = P.p.copy()
+    p[bad] = np.nan
+    ctx.field_upload(nw.F_PRESSURE, p)
+    with pytest.raises(nw.NwError) as ei:
+        ctx.assemble_continuity(nw.continuity_defaults())
+    assert ei.value.code == nw.E_NONFINITE
+    row = int(str(ei.value).split("node ")[1].split()[0])
+    assert row in nbrs
+    ctx.field_upload(nw.F_PRESSURE, P.p)
+    ctx.build_graph(3)
+    v = P.vel.copy()
+    v[bad, 1] = np.inf
+    ctx.field_upload(nw.F_VELOCITY, v)
+    with pytest.raises(nw.NwError) as ei:
+        ctx.assemble_momentum(nw.momentum_defaults())
+    assert ei.value.code == nw.E_NONFINITE
+    ctx.set_check_finite(False)
+    ctx.assemble_momentum(nw.momentum_defaults())  # unchecked: no error
 
 
 def test_schedule_order_does_not_change_results(nw, monkeypatch):

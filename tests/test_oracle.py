@@ -465,9 +465,7 @@ def test_mpnn_gradient_matches_finite_differences():
     assert npar == H * 5 + H + L * (2 * H * H + H) + 2 * H + 2
     P = rng.normal(size=npar) * 0.4
     w = orc.mpnn_slot_weights(A)
-    feat, _ = orc.mpnn_features(A, A.rhs, rng.normal(size=A.c.nrows),
-                                rng.normal(size=A.c.nrows),
-                                rng.normal(size=A.c.nrows))
+    feat, _ = orc.mpnn_features(A, A.rhs, *rng.normal(size=(5, A.c.nrows)))
     t = rng.normal(size=A.c.nrows)
     loss, g = orc.mpnn_loss_grad(P, H, L, A, w, feat, t)
     d = orc.mpnn_forward(P, H, L, A, w, feat)
@@ -493,16 +491,15 @@ def test_mpnn_permutation_equivariance():
     H, L = 5, 3
     P = rng.normal(size=orc.mpnn_nparams(H, L)) * 0.5
     D = A.dense()
-    x0 = rng.normal(size=n)
+    hist = rng.normal(size=(5, n))
     w = orc.mpnn_slot_weights(A)
-    xp, xpp = rng.normal(size=n), rng.normal(size=n)
-    feat, _ = orc.mpnn_features(A, A.rhs, x0, xp, xpp)
+    feat, _ = orc.mpnn_features(A, A.rhs, *hist)
     d = orc.mpnn_forward(P, H, L, A, w, feat)
     perm = rng.permutation(n)
     Dp = D[np.ix_(perm, perm)]
     B = orc.CsrSystem.from_dense(Dp, A.rhs[perm])
     wp = orc.mpnn_slot_weights(B)
-    featp, _ = orc.mpnn_features(B, A.rhs[perm], x0[perm], xp[perm], xpp[perm])
+    featp, _ = orc.mpnn_features(B, A.rhs[perm], *hist[:, perm])
     dp = orc.mpnn_forward(P, H, L, B, wp, featp)
     np.testing.assert_allclose(dp, d[perm], rtol=1e-12, atol=1e-12)
 
@@ -527,21 +524,34 @@ def test_mpnn_slot_weights_and_features():
     assert np.all(W[top] == 0) and np.all(W[:, top] == 0)
     x0 = rng.normal(size=n)
     b = A.rhs.copy()
-    xp = x0 + 0.1 * rng.normal(size=n)
-    xpp = xp + 0.1 * rng.normal(size=n)
-    feat, sc = orc.mpnn_features(A, b, x0, xp, xpp)
-    r = b - D @ x0
+    h = [x0]
+    for _ in range(4):
+        h.append(h[-1] + 0.1 * rng.normal(size=n))
+    feat, sc = orc.mpnn_features(A, b, *h)
+    xb = 3 * h[0] - 3 * h[1] + h[2]
+    r = b - D @ xb
     rms = lambda v: np.sqrt(np.mean(v * v))
-    np.testing.assert_allclose(sc, [rms(b), rms(x0), rms(r), rms(r / dg),
-                                    rms(x0 - xp)], rtol=1e-13)
-    np.testing.assert_allclose(feat[:, 3], (x0 - xp) / rms(x0 - xp),
+    sd = rms(h[0] - h[1])
+    np.testing.assert_allclose(sc, [rms(b), rms(x0), rms(r), rms(r / dg), sd],
+                               rtol=1e-13)
+    np.testing.assert_allclose(feat[:, 3], (h[0] - 3 * h[1] + 3 * h[2] - h[3]) / sd,
                                rtol=1e-12, atol=1e-14)
-    np.testing.assert_allclose(feat[:, 4], (xp - xpp) / rms(x0 - xp),
-                               rtol=1e-12, atol=1e-14)
-    # without history the last feature is zero and the output scale is s_e
+    np.testing.assert_allclose(feat[:, 4], (h[0] - 4 * h[1] + 6 * h[2] - 4 * h[3] +
+                                            h[4]) / sd, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(orc.mpnn_base_guess(*h), xb, rtol=1e-15, atol=1e-15)
+    # shorter histories: the higher differences vanish and the base guess
+    # drops to the linear extrapolation / x0; without history the output
+    # scale is s_e
+    f3h, _ = orc.mpnn_features(A, b, *h[:3])
+    assert np.all(f3h[:, 3:] == 0)
+    f4h, _ = orc.mpnn_features(A, b, *h[:4])
+    assert np.all(f4h[:, 4] == 0) and np.any(f4h[:, 3] != 0)
+    np.testing.assert_allclose(orc.mpnn_base_guess(*h[:2]), 2 * h[0] - h[1],
+                               rtol=1e-15, atol=1e-15)
     f0, s0 = orc.mpnn_features(A, b, x0)
     assert np.all(f0[:, 3:] == 0) and s0[4] == s0[3]
-    np.testing.assert_allclose(feat[:, 0], r / rms(r), rtol=1e-12, atol=1e-14)
+    assert np.array_equal(orc.mpnn_base_guess(x0), x0)
+    np.testing.assert_allclose(feat[:, 0], r / rms(r), rtol=1e-11, atol=1e-13)
     np.testing.assert_allclose(feat[:, 1], np.log(dg) - np.log(dg).mean(),
                                rtol=1e-12, atol=1e-14)
     flag = np.zeros(n)
@@ -603,8 +613,7 @@ def test_mpnn_bf16_forward_stays_near_fp64():
     H, L = 32, 3
     P = rng.normal(size=orc.mpnn_nparams(H, L)) * 0.3
     w = orc.mpnn_slot_weights(A)
-    feat, _ = orc.mpnn_features(A, A.rhs, rng.normal(size=n),
-                                rng.normal(size=n), rng.normal(size=n))
+    feat, _ = orc.mpnn_features(A, A.rhs, *rng.normal(size=(5, n)))
     d = orc.mpnn_forward(P, H, L, A, w, feat)
     db = orc.mpnn_forward_bf16(P, H, L, A, w, feat)
     rel = np.linalg.norm(db - d) / np.linalg.norm(d)
@@ -659,8 +668,7 @@ def test_mpnn_residual_loss_gradient_matches_finite_differences():
     n = A.c.nrows
     P = rng.normal(size=orc.mpnn_nparams(H, L)) * 0.4
     w = orc.mpnn_slot_weights(A)
-    feat, _ = orc.mpnn_features(A, A.rhs, rng.normal(size=n),
-                                rng.normal(size=n), rng.normal(size=n))
+    feat, _ = orc.mpnn_features(A, A.rhs, *rng.normal(size=(5, n)))
     t = rng.normal(size=n)
     loss, g = orc.mpnn_loss_grad_residual(P, H, L, A, w, feat, t)
     d = orc.mpnn_forward(P, H, L, A, w, feat)
@@ -745,3 +753,118 @@ def test_hypre_skipped_rows_semantics():
     H.zero()
     orc.assemble_continuity(CONT_OPTS, *args, csr=H)
     assert np.all(H.vals[sk] == 0)
+
+
+def _hydrostatic_state(n, coords, rng, rho0=1.7, g=(0.3, -0.2, -9.81)):
+    """Fluid at rest in hydrostatic balance: p = rho g.x + const, u = 0,
+    and dp/dx and the buoyancy source both equal to rho g."""
+    g = np.asarray(g)
+    p = rho0 * coords @ g + 0.4
+    rho = np.full(n, rho0)
+    src = np.tile(rho0 * g, (n, 1))
+    return g, p, rho, src
+
+
+def test_balanced_buoyancy_hydrostatic_closed_form():
+    """ContinuityEdgeSolverAlg.C:147-152,168-173 and MdotEdgeAlg.C:147-169:
+    the balanced buoyancy force is built so that a fluid at rest in
+    hydrostatic balance has no mass flux.  On a rectilinear stretched box
+    (A parallel to dx on every edge, so the non-orthogonal correction
+    vanishes) with p = rho g.x, u = 0 and dp/dx = source = rho g:
+      pressure term  -tau (p_R - p_L)|A|^2/(A.dx) = -tau rho g.A,
+      gravity term   +tau rho g.A (m_L + m_R)/2,
+      G_ip . A       = rho g.A ((1 - m_R)/Ud_R + (1 - m_L)/Ud_L)/2,
+    which sum to zero (i) for any masks when Udiag is uniform and (ii) for
+    any Udiag when the masks are all 0 or all 1.  A flipped gravity sign, a
+    one-sided mask weight, a swapped L/R source or a missing 1/Udiag leaves
+    a non-zero flux on some edge.  The continuity rhs and MdotEdgeAlg's
+    mdot are both checked."""
+    coords, conn = fx.hex_box(4, 3, 5, stretch_z=1.15)
+    edges, elem_edges = orc.create_edges_hex(conn, coords.shape[0])
+    areav = orc.edge_area_vec_hex(conn, coords, elem_edges, edges)
+    n = coords.shape[0]
+    rng = np.random.default_rng(41)
+    g, p, rho, src = _hydrostatic_state(n, coords, rng)
+    vel = np.zeros((n, 3))
+    scale = np.abs(areav).max() * np.abs(rho[0] * g).max() * 2.0
+    cases = [(np.full(n, 2.5), rng.uniform(0.0, 1.0, n)),     # (i)
+             (rng.uniform(1.0, 4.0, n), np.ones(n)),           # (ii)
+             (rng.uniform(1.0, 4.0, n), np.zeros(n))]
+    for udiag, mask in cases:
+        b = orc.buoyancy(g, src, mask)
+        for noc in (1.0, 0.0):
+            o = orc.with_buoyancy(orc.ContinuityOpts(0.7, 1.0, noc, 1.0, 0.0), b)
+            _, R = orc.assemble_continuity(o, n, edges, coords, vel, src, rho, p,
+                                           udiag, areav)
+            assert np.abs(R).max() <= 1e-13 * scale / (0.7 * udiag.min())
+            md = orc.mdot_edge_alg(edges, coords, vel, src, rho, p, udiag, areav,
+                                   noc, 1.0, buoy=b)
+            assert np.abs(md).max() <= 1e-13 * scale / udiag.min()
+    # not vacuous: the same state without the balanced force (and with the
+    # gradient left at zero) carries the hydrostatic flux -tau rho g.A
+    udiag = np.full(n, 2.5)
+    md0 = orc.mdot_edge_alg(edges, coords, vel, np.zeros((n, 3)), rho, p, udiag,
+                            areav, 1.0, 1.0)
+    np.testing.assert_allclose(md0, -(1 / 2.5) * rho[0] * areav @ g,
+                               rtol=1e-12, atol=1e-14 * scale)
+    # and a half-masked state with non-uniform Udiag is not balanced: the
+    # closed form of the residual flux is
+    # rho g.A [tau (m_L+m_R)/2 - (m_R/Ud_R + m_L/Ud_L)/2]
+    udiag = rng.uniform(1.0, 4.0, n)
+    mask = rng.uniform(0.0, 1.0, n)
+    b = orc.buoyancy(g, src, mask)
+    md = orc.mdot_edge_alg(edges, coords, vel, src, rho, p, udiag, areav, 1.0,
+                           1.0, buoy=b)
+    L, R = edges[:, 0], edges[:, 1]
+    tau = 0.5 * (1 / udiag[L] + 1 / udiag[R])
+    want = rho[0] * (areav @ g) * (tau * 0.5 * (mask[L] + mask[R]) -
+                                   0.5 * (mask[R] / udiag[R] + mask[L] / udiag[L]))
+    np.testing.assert_allclose(md, want, rtol=1e-11, atol=1e-13 * scale)
+    assert np.abs(want).max() > 1e-3 * scale
+
+
+def test_balanced_buoyancy_masked_off_is_the_plain_flux():
+    """With buoyancy_source_mask = 0 the balanced force adds nothing: the
+    continuity system and mdot equal those without it, on a jittered mesh
+    with random fields (the source is not read through any other path)."""
+    coords, conn, edges, areav, dv = _skewed_box()
+    n = coords.shape[0]
+    rng = np.random.default_rng(42)
+    vel, dpdx = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    rho, p = rng.uniform(0.5, 2.0, n), rng.normal(size=n)
+    udiag = rng.uniform(1.0, 3.0, n)
+    b = orc.buoyancy((0.1, 0.2, -9.8), rng.normal(size=(n, 3)), np.zeros(n))
+    o = orc.ContinuityOpts(0.3, 1.0, 1.0, 1.0, 0.0)
+    L0, R0 = orc.assemble_continuity(o, n, edges, coords, vel, dpdx, rho, p,
+                                     udiag, areav)
+    L1, R1 = orc.assemble_continuity(orc.with_buoyancy(o, b), n, edges, coords,
+                                     vel, dpdx, rho, p, udiag, areav)
+    assert np.array_equal(L0, L1) and np.array_equal(R0, R1)
+    m0 = orc.mdot_edge_alg(edges, coords, vel, dpdx, rho, p, udiag, areav)
+    m1 = orc.mdot_edge_alg(edges, coords, vel, dpdx, rho, p, udiag, areav, buoy=b)
+    assert np.array_equal(m0, m1)
+
+
+def test_mpnn_history_differences_extrapolate_polynomials_exactly():
+    """The base guess and the two history features pin the stencils by the
+    polynomials they extrapolate exactly: with a history h_j = p(-j) of a
+    polynomial p in time, the base guess is p(1) for quadratics, and the
+    heads a = (1, 0) and a = (1, 1), which add s_d f_3 and s_d (f_3 + f_4),
+    give p(1) for cubics and quartics (a wrong stencil coefficient, a
+    swapped history index or a missing s_d breaks one of these)."""
+    A, top, rng = _mpnn_system(8)
+    n = A.c.nrows
+    c = rng.normal(size=(5, n))
+    t = np.arange(0, -5, -1, dtype=float)
+    for deg in (2, 3, 4):
+        hist = [sum(c[k] * tj ** k for k in range(deg + 1)) for tj in t]
+        p1 = sum(c[k] for k in range(deg + 1))   # p(1)
+        feat, sc = orc.mpnn_features(A, A.rhs, *hist)
+        xb = orc.mpnn_base_guess(*hist)
+        if deg == 2:
+            np.testing.assert_allclose(xb, p1, rtol=1e-12, atol=1e-11)
+            assert np.abs(feat[:, 3:]).max() < 1e-10
+        pred = xb + sc[4] * (feat[:, 3] + (feat[:, 4] if deg == 4 else 0.0))
+        np.testing.assert_allclose(pred, p1, rtol=1e-11, atol=1e-10)
+        if deg >= 3:   # the base guess alone is not exact
+            assert np.abs(xb - p1).max() > 1e-3
