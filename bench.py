@@ -541,7 +541,7 @@ def mixed_mesh(hip, n, steps, warmup, peak, log):
         cb = continuity_bytes(W, nnz)
         res["nnz"] = nnz
         res["assembly"] = {"Medges_per_s": round(W.ne / (kms * 1e-3) / 1e6, 1),
-                           "roofline": roofline("k_assemble<ContinuityPhys>", cb, kms,
+                           "roofline": roofline("k_assemble<ContinuityPhysT<false>>", cb, kms,
                                                 peak, "continuity_mixed_160_assembly.txt")}
         log("mixed assembly: %s" % res["assembly"])
         ctx.assemble_continuity(W.opts)
@@ -613,7 +613,7 @@ def continuity_part(hip, a, n3, quick, log, out, peak):
             "step": "one ContinuityEdgeSolverAlg pass over all edges, inputs "
                     "resident in HBM (rows overwritten: fused zeroSystem)",
             "wall_s_untimed_repeat": round(wall, 4)}
-        out["roofline"] = roofline("k_assemble<ContinuityPhys>", cb, kern_ms, peak,
+        out["roofline"] = roofline("k_assemble<ContinuityPhysT<false>>", cb, kern_ms, peak,
                                    "continuity_rf_256_assembly.txt")
         log("continuity: %.3f ms/step, kernel %.3f ms, %.0f GB/s" %
             (ms_step, kern_ms, cb / (kern_ms * 1e-3) / 1e9))

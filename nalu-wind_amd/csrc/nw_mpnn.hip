@@ -277,52 +277,6 @@ __global__ void k_aggregate(const float* __restrict__ h,
   reinterpret_cast<float4*>(m + (int64_t)i * H)[k4] = a;
 }
 
-// h_next = relu(W_self h + W_nbr m + b): one thread per node with the H
-// accumulators in registers; the block stages the layer's weights (k-major,
-// 2 H x H floats) in LDS once and every thread reads them as broadcasts.
-template <int H>
-__global__ __launch_bounds__(kBlock) void k_dense_layer(
-  const float* __restrict__ hp, const float* __restrict__ mp,
-  const float* __restrict__ WT, const float* __restrict__ bias,
-  float* __restrict__ hn, int n)
-{
-  __shared__ float4 sW[2 * H * H / 4];  // [k][j], k < H: W_self^T, else W_nbr^T
-  for (int q = threadIdx.x; q < 2 * H * H / 4; q += blockDim.x)
-    sW[q] = reinterpret_cast<const float4*>(WT)[q];
-  __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float acc[H];
-#pragma unroll
-  for (int j = 0; j < H; ++j) acc[j] = bias[j];
-  for (int src = 0; src < 2; ++src) {
-    const float4* row =
-      reinterpret_cast<const float4*>((src ? mp : hp) + (int64_t)i * H);
-    const float4* Wk = sW + src * H * H / 4;
-    for (int k4 = 0; k4 < H / 4; ++k4) {
-      const float4 x = row[k4];
-      const float xv[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float4* wr = Wk + (4 * k4 + q) * (H / 4);
-#pragma unroll
-        for (int j4 = 0; j4 < H / 4; ++j4) {
-          const float4 wv = wr[j4];
-          acc[4 * j4] = fmaf(wv.x, xv[q], acc[4 * j4]);
-          acc[4 * j4 + 1] = fmaf(wv.y, xv[q], acc[4 * j4 + 1]);
-          acc[4 * j4 + 2] = fmaf(wv.z, xv[q], acc[4 * j4 + 2]);
-          acc[4 * j4 + 3] = fmaf(wv.w, xv[q], acc[4 * j4 + 3]);
-        }
-      }
-    }
-  }
-  float4* out = reinterpret_cast<float4*>(hn + (int64_t)i * H);
-#pragma unroll
-  for (int j4 = 0; j4 < H / 4; ++j4)
-    out[j4] = make_float4(relu(acc[4 * j4]), relu(acc[4 * j4 + 1]),
-                          relu(acc[4 * j4 + 2]), relu(acc[4 * j4 + 3]));
-}
-
 // delta = a1 f3 + a2 f4 with a_k = w_out,k . h + b_out,k
 template <int H>
 __global__ void k_output(const float* __restrict__ hL,
@@ -682,39 +636,6 @@ __global__ void k_output_back(const float* __restrict__ hL,
   reinterpret_cast<float4*>(gz + (int64_t)i * H)[j4] = g;
 }
 
-// gs = W_self^T gz, q = W_nbr^T gz (weights row-major [j][k]).
-template <int H>
-__global__ __launch_bounds__(kBlock) void k_back_dense(
-  const float* __restrict__ gz, const float* __restrict__ Ws,
-  const float* __restrict__ Wn, float* __restrict__ gs, float* __restrict__ q,
-  int n)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4* g4 = reinterpret_cast<const float4*>(gz + (int64_t)i * H);
-  for (int pass = 0; pass < 2; ++pass) {
-    const float* W = pass ? Wn : Ws;
-    float acc[H];
-#pragma unroll
-    for (int k = 0; k < H; ++k) acc[k] = 0.f;
-    for (int j4 = 0; j4 < H / 4; ++j4) {
-      const float4 g = g4[j4];
-      const float gv[4] = {g.x, g.y, g.z, g.w};
-#pragma unroll
-      for (int qq = 0; qq < 4; ++qq) {
-        const float* wr = W + (4 * j4 + qq) * H;
-#pragma unroll
-        for (int k = 0; k < H; ++k) acc[k] = fmaf(wr[k], gv[qq], acc[k]);
-      }
-    }
-    float4* out = reinterpret_cast<float4*>((pass ? q : gs) + (int64_t)i * H);
-#pragma unroll
-    for (int k4 = 0; k4 < H / 4; ++k4)
-      out[k4] = make_float4(acc[4 * k4], acc[4 * k4 + 1], acc[4 * k4 + 2],
-                            acc[4 * k4 + 3]);
-  }
-}
-
 // gz_prev[i] = mask(h_prev[i] > 0) * (gs[i] + sum_s w_s q[col_s]); the
 // aggregation is the forward's (the edge weights are symmetric), so the
 // transpose needs no scatter.  gz_prev may alias gs.
@@ -759,7 +680,7 @@ struct WgradShape {
 };
 
 // Weight gradients as node sums of outer products A_i B_i^T (the layer
-// gradients, the large ones, have their own kernel k_wgrad_layer):
+// gradients, the large ones, have their own kernel k_wgrad_layer_mfma):
 //   input : A = gz0 [n][H], B = [features | 1]
 //   output: A = dd2 [n][2], B = [h_L | 1]
 // Block b sums a contiguous node range in tiles of kTileN staged in LDS
@@ -793,7 +714,7 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(
       float v = 0.f;
       if (node < i1) {
         if (MODE == kModeLayer) {
-          v = 0.f;  // layer gradients: k_wgrad_layer
+          v = 0.f;  // layer gradients: k_wgrad_layer_mfma
         } else if (MODE == kModeInput) {
           v = k < kF ? B[(int64_t)k * n + node] : 1.f;
         } else {
@@ -824,81 +745,256 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(
   }
 }
 
-// Layer weight gradients G[j][k] = sum_i gz[i][j] [h_prev | m][i][k]
-// (k < 2H) and gb[j] = sum_i gz[i][j], written as partial[b][off + j (2H+1)
-// + k] with the bias at k = 2H.  Thread t owns the 4 x 8 tile
-// j in [4 (t / 16), +4), k in [8 (t % 16), +8) of the 64 x 128 product
-// (H = 64; H = 32 uses the first 4 x 4 tiles of each row group); tiles of
-// kTileN nodes are staged in LDS, FP32 within a tile, FP64 across tiles.
+// ------------------------------------------------- dense parts on MFMA
+// The dense products of the FP32 training pass run on the matrix cores as
+// v_mfma_f32_16x16x4_f32 (FP32 operands and accumulation; operand layout
+// of a 16 x 4 A and 4 x 16 B: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; the 16 x 16 result leaves D[4 (l / 16) + t][l % 16]
+// in item t of lane l).  Tiles of 16 nodes per wave; the weights are
+// staged once per block in LDS in the B-operand layout (one float per
+// lane per fragment, built on the host with the parameters).
+
+// hn[i] = relu(b + W_self h_i + W_nbr m_i): A = the tile's [h | m] rows
+// (LDS, row stride 2H + 2 floats, so that the 32 lanes of a half-wave read
+// 32 different banks), B = [W_self W_nbr]^T fragments
+// wfr[(nt KS + ks) 64 + l] = W[16 nt + l % 16][4 ks + l / 16] over [h, m].
 template <int H>
-__global__ __launch_bounds__(kBlock) void k_wgrad_layer(
+__global__ __launch_bounds__(kBlock) void k_dense_mfma(
+  const float* __restrict__ hp, const float* __restrict__ mp,
+  const float* __restrict__ wfr, const float* __restrict__ bias,
+  float* __restrict__ hn, int n)
+{
+  constexpr int NT = H / 16, KS = 2 * H / 4, AS = 2 * H + 2;
+  constexpr int WPB = kBlock / kWave, C4 = 2 * H / 4;
+  __shared__ float sW[NT * KS * kWave];
+  __shared__ float sA[WPB][16 * AS];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = (int)(threadIdx.x / kWave);
+  const int col = lane & 15, grp = lane >> 4;
+  for (int k = (int)threadIdx.x; k < NT * KS * kWave; k += (int)blockDim.x)
+    sW[k] = wfr[k];
+  __syncthreads();
+  float bs[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) bs[nt] = bias[nt * 16 + col];
+  float* A = sA[wv];
+  const int ntiles = (n + 15) / 16;
+  const int nwaves = (int)gridDim.x * WPB;
+  // the next tile's rows are loaded into registers while the matrix cores
+  // work on the current one
+  constexpr int PF = 16 * C4 / kWave;
+  float4 pv[PF];
+  auto load_tile = [&](int i0) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int q = lane + u * kWave, r = q / C4, c4 = q % C4, i = i0 + r;
+      pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (i < n)
+        pv[u] = c4 < H / 4
+                  ? reinterpret_cast<const float4*>(hp + (int64_t)i * H)[c4]
+                  : reinterpret_cast<const float4*>(mp + (int64_t)i * H)[c4 - H / 4];
+    }
+  };
+  int t = (int)blockIdx.x * WPB + wv;
+  if (t < ntiles) load_tile(t * 16);
+  for (; t < ntiles; t += nwaves) {
+    const int i0 = t * 16;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int q = lane + u * kWave, r = q / C4, c4 = q % C4;
+      float* d = A + r * AS + 4 * c4;
+      d[0] = pv[u].x;
+      d[1] = pv[u].y;
+      d[2] = pv[u].z;
+      d[3] = pv[u].w;
+    }
+    wave_lds_sync();
+    if (t + nwaves < ntiles) load_tile((t + nwaves) * 16);
+    f32x4 acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = (f32x4){bs[nt], bs[nt], bs[nt], bs[nt]};
+    const float* ar = A + col * AS + grp;
+#pragma unroll 8
+    for (int ks = 0; ks < KS; ++ks) {
+      const float a = ar[4 * ks];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+          a, sW[(nt * KS + ks) * kWave + lane], acc[nt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt) {
+        const int i = i0 + 4 * grp + tt;
+        if (i < n) hn[(int64_t)i * H + nt * 16 + col] = relu(acc[nt][tt]);
+      }
+    wave_lds_sync();  // the next tile's staging overwrites A
+  }
+}
+
+// gs[i] = W_self^T gz_i, q[i] = W_nbr^T gz_i: A = the tile's gz rows (LDS,
+// stride H + 2), B fragments wbk[(nt KS + ks) 64 + l] =
+// W[4 ks + l / 16][16 nt + l % 16] over [W_self | W_nbr] (output columns
+// 16 nt + l % 16 < H go to gs, the rest to q).
+template <int H>
+__global__ __launch_bounds__(kBlock) void k_back_dense_mfma(
+  const float* __restrict__ gz, const float* __restrict__ wbk,
+  float* __restrict__ gs, float* __restrict__ q, int n)
+{
+  constexpr int NT = 2 * H / 16, KS = H / 4, AS = H + 2;
+  constexpr int WPB = kBlock / kWave, C4 = H / 4;
+  __shared__ float sW[NT * KS * kWave];
+  __shared__ float sA[WPB][16 * AS];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = (int)(threadIdx.x / kWave);
+  const int col = lane & 15, grp = lane >> 4;
+  for (int k = (int)threadIdx.x; k < NT * KS * kWave; k += (int)blockDim.x)
+    sW[k] = wbk[k];
+  __syncthreads();
+  float* A = sA[wv];
+  const int ntiles = (n + 15) / 16;
+  const int nwaves = (int)gridDim.x * WPB;
+  // the next tile's rows are loaded into registers during the current one
+  constexpr int PF = 16 * C4 / kWave;
+  float4 pv[PF];
+  auto load_tile = [&](int i0) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int p = lane + u * kWave, r = p / C4, c4 = p % C4, i = i0 + r;
+      pv[u] = i < n ? reinterpret_cast<const float4*>(gz + (int64_t)i * H)[c4]
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  int t = (int)blockIdx.x * WPB + wv;
+  if (t < ntiles) load_tile(t * 16);
+  for (; t < ntiles; t += nwaves) {
+    const int i0 = t * 16;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int p = lane + u * kWave, r = p / C4, c4 = p % C4;
+      float* d = A + r * AS + 4 * c4;
+      d[0] = pv[u].x;
+      d[1] = pv[u].y;
+      d[2] = pv[u].z;
+      d[3] = pv[u].w;
+    }
+    wave_lds_sync();
+    if (t + nwaves < ntiles) load_tile((t + nwaves) * 16);
+    f32x4 acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const float* ar = A + col * AS + grp;
+#pragma unroll 4
+    for (int ks = 0; ks < KS; ++ks) {
+      const float a = ar[4 * ks];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+          a, sW[(nt * KS + ks) * kWave + lane], acc[nt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt) {
+        const int i = i0 + 4 * grp + tt;
+        const int k = nt * 16 + col;
+        if (i < n) (k < H ? gs : q)[(int64_t)i * H + (k % H)] = acc[nt][tt];
+      }
+    wave_lds_sync();
+  }
+}
+
+// Layer weight gradients G[j][k] = sum_i gz[i][j] [h_prev | m][i][k] and
+// gb[j] = sum_i gz[i][j] on the matrix cores: wave w takes the rows
+// j in [16 w, 16 w + 16) (H = 32: waves 0, 1) and all 2H / 16 column tiles;
+// a K step is 4 nodes of the block's LDS tile of kTileN nodes (row
+// strides padded to 16 mod 32 floats: a half-wave reads 32 banks).  FP32
+// within a tile, FP64 across tiles (a fixed order: the result is bitwise
+// reproducible); written as partial[b][off + j (2H+1) + k], bias at k = 2H.
+template <int H>
+__global__ __launch_bounds__(kBlock) void k_wgrad_layer_mfma(
   const float* __restrict__ gz, const float* __restrict__ hp,
   const float* __restrict__ mp, int n, double* __restrict__ partial, int nv,
   int off)
 {
-  constexpr int RJ = 4, RK = 2 * H / 16;  // 16 column groups of 2H / 16
-  __shared__ float sA[kTileN][H];
-  __shared__ float sB[kTileN][2 * H];
-  const int jb = threadIdx.x / 16, kb = threadIdx.x % 16;
-  const bool own = jb * RJ < H;  // H = 32: threads with jb >= 8 idle
-  double acc[RJ][RK], accb[RJ];
+  constexpr int NT = 2 * H / 16, SA = H + 16, SB = 2 * H + 16;
+  __shared__ float sA[kTileN][SA];
+  __shared__ float sB[kTileN][SB];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = (int)(threadIdx.x / kWave);
+  const int col = lane & 15, grp = lane >> 4;
+  const bool own = wv * 16 < H;
+  double accd[NT][4], accb = 0.0;
 #pragma unroll
-  for (int a = 0; a < RJ; ++a) {
-    accb[a] = 0.0;
+  for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-    for (int b = 0; b < RK; ++b) acc[a][b] = 0.0;
-  }
+    for (int tt = 0; tt < 4; ++tt) accd[nt][tt] = 0.0;
   const int per_block = ((n + gridDim.x - 1) / gridDim.x + kTileN - 1) / kTileN * kTileN;
   const int i0 = min(n, (int)blockIdx.x * per_block);
   const int i1 = min(n, i0 + per_block);
-  for (int t0 = i0; t0 < i1; t0 += kTileN) {
-    for (int idx = threadIdx.x; idx < kTileN * H; idx += kBlock) {
-      const int t = idx / H, k = idx % H, node = t0 + t;
+  // the next tile's rows are loaded into registers while the matrix cores
+  // work on the current one (PF floats of each array per thread)
+  constexpr int PF = kTileN * H / kBlock;
+  float ra[PF], rh[PF], rm[PF];
+  auto load_tile = [&](int t0) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int idx = (int)threadIdx.x + u * kBlock;
+      const int node = t0 + idx / H, k = idx % H;
       const bool in = node < i1;
-      sA[t][k] = in ? gz[(int64_t)node * H + k] : 0.f;
-      sB[t][k] = in ? hp[(int64_t)node * H + k] : 0.f;
-      sB[t][H + k] = in ? mp[(int64_t)node * H + k] : 0.f;
+      ra[u] = in ? gz[(int64_t)node * H + k] : 0.f;
+      rh[u] = in ? hp[(int64_t)node * H + k] : 0.f;
+      rm[u] = in ? mp[(int64_t)node * H + k] : 0.f;
+    }
+  };
+  if (i0 < i1) load_tile(i0);
+  for (int t0 = i0; t0 < i1; t0 += kTileN) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int idx = (int)threadIdx.x + u * kBlock;
+      const int t = idx / H, k = idx % H;
+      sA[t][k] = ra[u];
+      sB[t][k] = rh[u];
+      sB[t][H + k] = rm[u];
     }
     __syncthreads();
+    if (t0 + kTileN < i1) load_tile(t0 + kTileN);
     if (own) {
-      float ta[RJ][RK], tb[RJ];
+      f32x4 acc[NT];
 #pragma unroll
-      for (int a = 0; a < RJ; ++a) {
-        tb[a] = 0.f;
+      for (int nt = 0; nt < NT; ++nt) acc[nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+      float tb = 0.f;
 #pragma unroll
-        for (int b = 0; b < RK; ++b) ta[a][b] = 0.f;
-      }
-      for (int t = 0; t < kTileN; ++t) {
-        float av[RJ], bv[RK];
+      for (int ks = 0; ks < kTileN / 4; ++ks) {
+        const float a = sA[4 * ks + grp][16 * wv + col];
+        tb += a;
 #pragma unroll
-        for (int a = 0; a < RJ; ++a) av[a] = sA[t][jb * RJ + a];
-#pragma unroll
-        for (int b = 0; b < RK; ++b) bv[b] = sB[t][kb * RK + b];
-#pragma unroll
-        for (int a = 0; a < RJ; ++a) {
-          tb[a] += av[a];
-#pragma unroll
-          for (int b = 0; b < RK; ++b) ta[a][b] = fmaf(av[a], bv[b], ta[a][b]);
-        }
+        for (int nt = 0; nt < NT; ++nt)
+          acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+            a, sB[4 * ks + grp][16 * nt + col], acc[nt], 0, 0, 0);
       }
 #pragma unroll
-      for (int a = 0; a < RJ; ++a) {
-        accb[a] += (double)tb[a];
+      for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-        for (int b = 0; b < RK; ++b) acc[a][b] += (double)ta[a][b];
-      }
+        for (int tt = 0; tt < 4; ++tt) accd[nt][tt] += (double)acc[nt][tt];
+      accb += (double)tb;
     }
     __syncthreads();
   }
   if (!own) return;
+  // the four node groups' bias sums, in a fixed order
+  accb += __shfl_xor(accb, 16);
+  accb += __shfl_xor(accb, 32);
   double* out = partial + (int64_t)blockIdx.x * nv + off;
 #pragma unroll
-  for (int a = 0; a < RJ; ++a) {
-    const int j = jb * RJ + a;
+  for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-    for (int b = 0; b < RK; ++b) out[j * (2 * H + 1) + kb * RK + b] = acc[a][b];
-    if (kb == 0) out[j * (2 * H + 1) + 2 * H] = accb[a];
-  }
+    for (int tt = 0; tt < 4; ++tt) {
+      const int j = 16 * wv + 4 * grp + tt;
+      out[j * (2 * H + 1) + 16 * nt + col] = accd[nt][tt];
+    }
+  if (grp == 0) out[(16 * wv + col) * (2 * H + 1) + 2 * H] = accb;
 }
 
 }  // namespace
@@ -939,6 +1035,8 @@ struct Mpnn {
   float* vd = nullptr;        // [n] residual loss: D^-2 A u
   uint16_t* hb[2] = {nullptr, nullptr};  // bfloat16 activations [n][H]
   uint16_t* wfrag = nullptr;  // per layer [H/16][2H/16][64 lanes][4] bf16
+  float* wf32 = nullptr;      // per layer [H/16][2H/4][64] FP32 B fragments
+  float* wb32 = nullptr;      // per layer [2H/16][H/4][64] (backward)
   int32_t* eslot = nullptr;   // [nedge][2] the slots (L, R), (R, L) of an edge
   uint64_t eslot_gen = 0;     // generation of the graph eslot was built for
   int64_t eslot_nedge = -1;
@@ -1003,6 +1101,36 @@ int upload_params(Mpnn* m)
                                   : f[o.wn(l, H) + (int64_t)j * H + (k - H)];
             wb[((((size_t)l * NT + nt) * KS + ks) * kWave + ln) * 4 + tt] = f2bf(v);
           }
+  // FP32 fragments of the training pass (v_mfma_f32_16x16x4_f32 B
+  // operands, one float per lane): forward wf[l][nt][ks][lane] =
+  // W[16 nt + lane % 16][4 ks + lane / 16] over [W_self | W_nbr] (k < 2H),
+  // backward wk[l][nt][ks][lane] = W[4 ks + lane / 16][16 nt + lane % 16]
+  // over the output columns [W_self | W_nbr] (n < 2H)
+  const size_t per = 2 * (size_t)H * H;
+  std::vector<float> wf((size_t)L * per), wk((size_t)L * per);
+  auto wcat = [&](int l, int j, int k) {  // [W_self | W_nbr][j][k], k < 2H
+    return k < H ? f[o.ws(l) + (int64_t)j * H + k]
+                 : f[o.wn(l, H) + (int64_t)j * H + (k - H)];
+  };
+  for (int l = 0; l < L; ++l) {
+    const int FNT = H / 16, FKS = 2 * H / 4, BNT = 2 * H / 16, BKS = H / 4;
+    for (int nt = 0; nt < FNT; ++nt)
+      for (int ks = 0; ks < FKS; ++ks)
+        for (int ln = 0; ln < kWave; ++ln)
+          wf[l * per + ((size_t)nt * FKS + ks) * kWave + ln] =
+            wcat(l, 16 * nt + ln % 16, 4 * ks + ln / 16);
+    for (int nt = 0; nt < BNT; ++nt)
+      for (int ks = 0; ks < BKS; ++ks)
+        for (int ln = 0; ln < kWave; ++ln)
+          wk[l * per + ((size_t)nt * BKS + ks) * kWave + ln] =
+            wcat(l, 4 * ks + ln / 16, 16 * nt + ln % 16);
+  }
+  if (!m->wf32) NW_HIP(hipMalloc(&m->wf32, sizeof(float) * wf.size()));
+  if (!m->wb32) NW_HIP(hipMalloc(&m->wb32, sizeof(float) * wk.size()));
+  NW_HIP(hipMemcpyAsync(m->wf32, wf.data(), sizeof(float) * wf.size(),
+                        hipMemcpyHostToDevice, m->c->stream));
+  NW_HIP(hipMemcpyAsync(m->wb32, wk.data(), sizeof(float) * wk.size(),
+                        hipMemcpyHostToDevice, m->c->stream));
   if (!m->wfrag) NW_HIP(hipMalloc(&m->wfrag, sizeof(uint16_t) * wb.size()));
   NW_HIP(hipMemcpyAsync(m->dP, f.data(), sizeof(float) * m->np,
                         hipMemcpyHostToDevice, m->c->stream));
@@ -1139,6 +1267,15 @@ int features(Mpnn* m, const double* const* hist, int nh, const double* xs,
   return NW_OK;
 }
 
+// blocks of the tile kernels: one wave per 16-node tile up to 8 resident
+// blocks per CU's worth (the waves then stride over the tiles)
+inline unsigned dense_grid(int n)
+{
+  const int64_t tiles = (n + 15) / 16;
+  const int64_t blocks = (tiles + kBlock / kWave - 1) / (kBlock / kWave);
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks, 256 * 8));
+}
+
 template <int H>
 int forward_h(Mpnn* m, const float* feat, bool keep)
 {
@@ -1154,14 +1291,14 @@ int forward_h(Mpnn* m, const float* feat, bool keep)
   int cur = 0;
   const unsigned gq = (unsigned)(((int64_t)n * (H / 4) + kBlock - 1) / kBlock);
   for (int l = 0; l < L; ++l) {
-    const float* WsT = WT + (size_t)H * kF + 2 * (size_t)l * H * H;
     const int nxt = keep ? l + 1 : 1 - cur;
     float* msg = m->msg[keep ? l : 0];
     k_aggregate<H><<<gq, kBlock, 0, st>>>(m->h[cur], g.row_ptr, g.cols, m->w,
                                           msg, n);
     NW_LAUNCH_CHECK();
-    k_dense_layer<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(
-      m->h[cur], msg, WsT, m->dP + o.b(l, H), m->h[nxt], n);
+    k_dense_mfma<H><<<dense_grid(n), kBlock, 0, st>>>(
+      m->h[cur], msg, m->wf32 + (size_t)l * 2 * H * H, m->dP + o.b(l, H),
+      m->h[nxt], n);
     NW_LAUNCH_CHECK();
     cur = nxt;
   }
@@ -1266,12 +1403,12 @@ int loss_grad_h(Mpnn* m, const Sample& smp, double scale, double* loss,
   float* gz = m->gz;
   float* gs = m->gs;
   for (int l = L - 1; l >= 0; --l) {
-    k_wgrad_layer<H><<<kGradBlocks, kBlock, 0, st>>>(gz, m->h[l], m->msg[l], n,
-                                                     m->partial, nv, off);
+    k_wgrad_layer_mfma<H><<<kGradBlocks, kBlock, 0, st>>>(
+      gz, m->h[l], m->msg[l], n, m->partial, nv, off);
     NW_LAUNCH_CHECK();
     off += WgradShape<H, kModeLayer>::NOUT;
-    k_back_dense<H><<<grid_for(n, kBlock), kBlock, 0, st>>>(
-      gz, m->dP + o.ws(l), m->dP + o.wn(l, H), gs, m->gq, n);
+    k_back_dense_mfma<H><<<dense_grid(n), kBlock, 0, st>>>(
+      gz, m->wb32 + (size_t)l * 2 * H * H, gs, m->gq, n);
     NW_LAUNCH_CHECK();
     k_back_aggregate<H><<<gq, kBlock, 0, st>>>(gs, m->gq, g.row_ptr, g.cols,
                                                m->w, m->h[l], gs, n);
@@ -1461,6 +1598,8 @@ void nw_mpnn_destroy(nw_mpnn* m)
   (void)hipFree(m->red);
   for (auto p : m->hb) (void)hipFree(p);
   (void)hipFree(m->wfrag);
+  (void)hipFree(m->wf32);
+  (void)hipFree(m->wb32);
   (void)hipFree(m->eslot);
   (void)hipFree(m->vd);
   delete m;

@@ -15,8 +15,9 @@ at any size and do not depend on the oracle:
               LHS sums to zero (an edge adds -c to one row and +c to the
               other in the same column), and sum(rhs) = 0 per component
 
-The pressure solves of the 256^3 system are checked by their residual,
-recomputed on the host from the downloaded system; the device geometry,
+The pressure solves of the 256^3 system (SA-AMG, Jacobi and Chebyshev
+preconditioning) are checked by their residual, recomputed on the host from
+the downloaded system; the device geometry,
 NodalGradEdgeAlg and MdotEdgeAlg of the 256^3 box against the oracle, and
 the geometry by the tiling of the box and the closure of the area vectors;
 the mixed Hex8/Tet4/Wed6 box's continuity system and device geometry.
@@ -205,6 +206,55 @@ def test_pressure_solve_256_cubed_full_size(nw, wl):
             assert err < 1e-5, (name, err)
         assert out["learned"][0].guess == nw.X0_LEARNED
         assert out["learned"][0].iters < out["fp32_refined"][0].iters
+
+
+@pytest.mark.timeout(600)
+def test_pressure_solve_256_cubed_jacobi_and_chebyshev(nw, wl):
+    """The configuration's own preconditioners at 256^3: GMRES(50) right-
+    preconditioned by Jacobi and by Chebyshev(8) on D^-1 A, FP64 and
+    FP32-inner / FP64-refined, x0 = 0, tol 1e-8, on the continuity system
+    with its Dirichlet rows.  Chebyshev converges; its residual, recomputed
+    here in FP64 from the downloaded system, meets the tolerance, and the
+    FP32-refined solution agrees with the FP64 one.  Jacobi does not
+    converge in its 2000 iterations on this graded mesh: the solve says so
+    (converged = 0) and the residual it reports is the one recomputed
+    here."""
+    with nw.Context(0) as ctx:
+        W = wl.continuity_rf(256)
+        W.upload(ctx)
+        ctx.build_graph(1)
+        ctx.assemble_continuity(W.opts)
+        wl.apply_pressure_bc(ctx, W)
+        rp, cols, vals, b = ctx.download()
+        A = sp.csr_matrix((vals, cols, rp), shape=(W.n, W.n))
+        bn = np.linalg.norm(b)
+        out = {}
+        for name, pre, maxit in (
+                ("chebyshev8", dict(precond=nw.PREC_CHEBYSHEV, cheb_degree=8,
+                                    cheb_ratio=30.0), 4000),
+                ("jacobi", dict(precond=nw.PREC_JACOBI), 2000)):
+            for mode, mname in ((nw.SOLVE_FP64, "fp64"),
+                                (nw.SOLVE_FP32_REFINE, "fp32_refined")):
+                st = ctx.solve(nw.solver_cfg(kspace=50, max_iters=maxit, tol=1e-8,
+                                             mode=mode, inner_tol=1e-5,
+                                             x0=nw.X0_ZERO, **pre),
+                               raise_on_noconv=False)
+                x = ctx.solution()
+                rel = np.linalg.norm(b - A @ x) / bn
+                print("256^3 %s %s: %d iterations, %.0f ms, converged %d, residual "
+                      "%.3e (host %.3e)" % (name, mname, st.iters, st.ms,
+                                            st.converged, st.final_rel, rel),
+                      flush=True)
+                assert st.final_rel == pytest.approx(rel, rel=1e-3, abs=1e-14)
+                out[(name, mname)] = (st, x)
+        for mname in ("fp64", "fp32_refined"):
+            st, _ = out[("chebyshev8", mname)]
+            assert st.converged and st.final_rel <= 1e-8
+            st, _ = out[("jacobi", mname)]
+            assert not st.converged and st.iters == 2000 and st.final_rel > 1e-8
+        x64 = out[("chebyshev8", "fp64")][1]
+        x32 = out[("chebyshev8", "fp32_refined")][1]
+        assert np.linalg.norm(x32 - x64) / np.linalg.norm(x64) < 1e-5
 
 
 @pytest.mark.timeout(900)
