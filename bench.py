@@ -16,8 +16,8 @@ the momentum (128^3) and scalar (32^3) assemblies with their own rooflines,
 the edge algorithms and device geometry of the 256^3 box, the pressure
 solves of that box (GMRES(50) with SA-AMG, Jacobi and Chebyshev
 preconditioning, FP64 and FP32-inner / FP64-refined), the learned initial
-guess on a 256^3 pressure-projection series, the mixed Hex8/Tet4/Wed6
-mesh, the HBM copy rate measured by a streaming kernel, and the oracle
+guess on a 256^3 pressure-projection series, the conforming mixed
+Hex8/Tet4/Wed6/Pyr5 mesh, the HBM copy rate measured by a streaming kernel, and the oracle
 timed on this host's cores on the same 256^3 continuity assembly
 (`cpu_baseline`) and the 128^3 momentum assembly.
 """
@@ -519,19 +519,23 @@ def device_geometry(hip, ctx, W, types, steps, warmup):
 def mixed_mesh(hip, n, steps, warmup, peak, log):
     """Config of the unstructured mixed mesh: continuity assembly and the
     AMG-preconditioned pressure solve (FP64 and FP32-inner/FP64-refined,
-    x0 = 0) on an n^3 box cut into Hex8/Tet4/Wed6 cells (60/30/10 %),
-    jittered, randomly permuted and RCM-renumbered."""
+    x0 = 0) on an n^3 box cut into Hex8/Tet4/Wed6 cells (60/30/10 %), with
+    transition cells of Pyr5 and Tet4 where faces would not match, jittered,
+    randomly permuted and RCM-renumbered."""
     import naluwind_amd as nw
     import workloads as wl
     t0 = time.perf_counter()
     W = wl.continuity_mixed(n, keep_elements=True)
     log("mixed mesh %d^3: %d nodes %d edges (%.1fs)" % (n, W.n, W.ne,
                                                         time.perf_counter() - t0))
-    res = {"workload": "continuity on a %d^3 box cut into Hex8/Tet4/Wed6 "
-                       "(60/30/10 %% of the cells), +-10 %% jitter, nodes randomly "
+    res = {"workload": "continuity on a conforming %d^3 box cut into "
+                       "Hex8/Tet4/Wed6 (60/30/10 %% of the cells); a Hex8 or Wed6 "
+                       "cell whose whole face meets a cut one becomes Pyr5 + Tet4 "
+                       "about a centre node; +-10 %% jitter, nodes randomly "
                        "permuted then RCM-renumbered" % n,
            "nodes": W.n, "edges": W.ne,
-           "elements_hex_tet_wedge": [int(v) for v in W.elements]}
+           "transition_cells": int(W.n - (n + 1) ** 3),
+           "elements_hex_tet_wedge_pyramid": [int(v) for v in W.elements]}
     with nw.Context(0) as ctx:
         W.upload(ctx)
         ctx.build_graph(1)

@@ -328,8 +328,8 @@ int nw_project_nodal_velocity(nw_ctx* ctx);
 /* ---- geometry (GeometryInteriorAlg, src/ngp_algorithms/
  * GeometryInteriorAlg.C:70-225) ---------------------------------------------
  * nw_mesh_elements: the mesh's elements over the nodes and edges of
- *   nw_mesh_set: type [nelem] (0 Hex8, 1 Tet4, 2 Wed6; NULL = all Hex8),
- *   conn [nelem][8] in exodus node order (unused slots -1).  Builds, on the
+ *   nw_mesh_set: type [nelem] (0 Hex8, 1 Tet4, 2 Wed6, 3 Pyr5; NULL = all
+ *   Hex8), conn [nelem][8] in exodus node order (unused slots -1).  Builds, on the
  *   host, the lists of element sub-control surfaces per edge and
  *   sub-control volumes per node (element order) that the device sums
  *   through.  NW_E_ARG when a node is out of range, an element edge is not
@@ -338,9 +338,9 @@ int nw_project_nodal_velocity(nw_ctx* ctx);
  * nw_compute_geometry: impl_compute_edge_area_vector and
  *   impl_compute_dual_nodal_volume on the device from NW_F_COORDINATES:
  *   writes NW_E_AREA_VECTOR (sum of the element sub-control-surface area
- *   vectors of each edge, HexSCS/TetSCS/WedSCS::determinant_scs, oriented
+ *   vectors of each edge, Hex/Tet/Wed/PyrSCS::determinant_scs, oriented
  *   from node[0] to node[1]) and NW_F_DUAL_VOLUME (sum of the sub-control
- *   volumes of each node, HexSCV/TetSCV/WedSCV::determinant_scv).  Sums run
+ *   volumes of each node, Hex/Tet/Wed/PyrSCV::determinant_scv).  Sums run
  *   in element order (the reference adds with atomics), so the result is
  *   reproducible.  Call again after the coordinates move: stream-ordered,
  *   allocates nothing. */

@@ -61,13 +61,19 @@ int nw_mdot_edge(int64_t nedge, const int32_t* edges, const double* areav,
 
 /* ---- unstructured mixed meshes -------------------------------------------- */
 
-/* Mixed Hex8 / Tet4 / Wed6 mesh of [0,1]^3 on an n^3 cell lattice: the
- * cells with the lowest values of a seeded smooth random field stay Hex8
- * (fraction f_hex), the next f_tet are split into 6 Kuhn Tet4, the rest into
- * 2 Wed6; interior nodes are moved by +-jitter/n; nodes are randomly
- * permuted and then renumbered by reverse Cuthill-McKee.  type [nelem]
- * (0 Hex8, 1 Tet4, 2 Wed6), conn [nelem][8] (exodus node order, unused
- * slots -1).  With coords == NULL only *nnode and *nelem are returned. */
+/* Conforming mixed Hex8 / Tet4 / Wed6 / Pyr5 mesh of [0,1]^3 on an n^3
+ * cell lattice: the cells with the lowest values of a seeded smooth random
+ * field stay Hex8 (fraction f_hex), the next f_tet are split into 6 Kuhn
+ * Tet4, the rest into 2 Wed6 (end faces z); a Hex8 or Wed6 cell with a
+ * whole face against a neighbour that cuts it into triangles becomes a
+ * transition cell: a node at its centre, a Pyr5 on each whole face and two
+ * Tet4 on each cut face (the neighbour's diagonal), so every face of the
+ * mesh is shared by matching elements.  Interior lattice nodes are moved
+ * by +-jitter/n; nodes are randomly permuted and then renumbered by reverse
+ * Cuthill-McKee.  nnode = (n+1)^3 + the number of transition cells.  type
+ * [nelem] (0 Hex8, 1 Tet4, 2 Wed6, 3 Pyr5), conn [nelem][8] (exodus node
+ * order, the apex of a Pyr5 last, unused slots -1).  With coords == NULL
+ * only *nnode and *nelem are returned. */
 int nw_mixed_box(int n, double f_hex, double f_tet, double jitter,
                  uint64_t seed, int64_t* nnode, int64_t* nelem,
                  double* coords, int32_t* type, int32_t* conn);
@@ -79,8 +85,9 @@ int nw_mixed_edges(int64_t nelem, const int32_t* type, const int32_t* conn,
 
 /* GeometryInteriorAlg::impl_compute_edge_area_vector
  * (src/ngp_algorithms/GeometryInteriorAlg.C:163-225) over Hex8, Tet4
- * (src/master_element/Tet4CVFEM.C:522-618) and Wed6
- * (src/master_element/Wed6CVFEM.C:541-646) sub-control surfaces. */
+ * (src/master_element/Tet4CVFEM.C:522-618), Wed6
+ * (src/master_element/Wed6CVFEM.C:541-646) and Pyr5
+ * (src/master_element/Pyr5CVFEM.C:771-900) sub-control surfaces. */
 int nw_mixed_edge_area_vector(int64_t nelem, const int32_t* type,
                               const int32_t* conn, const double* coords,
                               const int32_t* edges, int64_t nedge,

@@ -64,9 +64,14 @@ struct PecletDev {
 
 __device__ inline double van_leer(double dqm, double dqp, double eps)
 {
-  // include/edge_kernels/EdgeKernelUtils.h:18-24
-  return (2.0 * (dqm * dqp + fabs(dqm * dqp))) /
-         ((dqm + dqp) * (dqm + dqp) + eps);
+  // include/edge_kernels/EdgeKernelUtils.h:18-24.  No contraction: fused
+  // into fma(dqm, dqp, |dqm dqp|) the numerator of slopes of opposite sign
+  // is the product's rounding error instead of 0, which the denominator
+  // (dqm + dqp)^2 + eps turns into a limiter of order one when the slopes
+  // nearly cancel.
+#pragma clang fp contract(off)
+  const double p = dqm * dqp;
+  return (2.0 * (p + fabs(p))) / ((dqm + dqp) * (dqm + dqp) + eps);
 }
 
 // ------------------------------------------------------------------ physics

@@ -1,6 +1,7 @@
 // nw_geometry.hip -- GeometryInteriorAlg (src/ngp_algorithms/
 // GeometryInteriorAlg.C:70-225) on gfx950: the edge area vectors and dual
-// nodal volumes of a Hex8 / Tet4 / Wed6 mesh from its node coordinates.
+// nodal volumes of a Hex8 / Tet4 / Wed6 / Pyr5 mesh from its node
+// coordinates.
 //
 //   k_elem_scs,    one lane per element: the element's sub-control-surface
 //   k_elem_scv     (SCS) area vectors and sub-control volumes (SCV) into
@@ -17,10 +18,13 @@
 // same order, as the host producers in host/nw_mesh.cpp and nw_mixed.cpp.
 //
 // Each SCS is a quad of four points of the subdivided element and its area
-// vector is taken over four triangles about the quad's centroid; each SCV is
-// a hexahedron of eight such points whose volume is Grandy's
-// (hex_volume_grandy, include/master_element/Hex8GeometryFunctions.h:85).
-// A point is the average of a set of element vertices.
+// vector is taken over four triangles about the quad's centroid (a Pyr5
+// surface next to the apex is a triangle: a quad with a repeated point);
+// each SCV is a hexahedron of eight such points whose volume is Grandy's
+// (hex_volume_grandy, include/master_element/Hex8GeometryFunctions.h:85),
+// except in Pyr5: bent hexahedra at the base nodes (bhex_volume_grandy,
+// :163) and an octahedron at the apex (src/master_element/Pyr5CVFEM.C:
+// 395-433).  A point is the average of a set of element vertices.
 #include <algorithm>
 #include <vector>
 
@@ -93,6 +97,29 @@ struct WedTopo {
     {20, 16, 14, 18, 13, 10, 4, 11}, {19, 20, 18, 17, 12, 13, 11, 5}};
 };
 
+// Pyr5 (src/master_element/Pyr5CVFEM.C:437-571, 771-900): base edge
+// midpoints, base centroid, apex edge midpoints and triangle centroids,
+// element centroid.
+struct PyrTopo {
+  static constexpr int kNodes = 5, kScs = 12, kScv = 5, kPts = 19;
+  static constexpr int8_t pts[19][9] = {
+    {1, 0},       {1, 1},       {1, 2},          {1, 3},       {1, 4},
+    {2, 0, 1},    {2, 1, 2},    {2, 2, 3},       {2, 3, 0},    {4, 0, 1, 2, 3},
+    {2, 1, 4},    {2, 4, 0},    {3, 0, 1, 4},    {2, 2, 4},    {3, 1, 2, 4},
+    {2, 3, 4},    {3, 3, 4, 2}, {3, 0, 4, 3},    {5, 0, 1, 2, 3, 4}};
+  static constexpr int8_t quad[12][4] = {
+    {5, 9, 18, 12},   {6, 9, 18, 14},   {7, 9, 18, 16},   {8, 17, 18, 9},
+    {12, 12, 18, 17}, {11, 12, 12, 17}, {14, 14, 18, 12}, {10, 14, 14, 12},
+    {16, 16, 18, 14}, {13, 16, 16, 14}, {17, 17, 18, 16}, {15, 17, 17, 16}};
+  // base sub-volumes (bent hexahedra, top face split along 5-7), then the
+  // apex octahedron's ten vertices
+  static constexpr int8_t scv[4][8] = {{0, 5, 9, 8, 11, 12, 18, 17},
+                                       {1, 6, 9, 5, 10, 14, 18, 12},
+                                       {2, 7, 9, 6, 13, 16, 18, 14},
+                                       {3, 8, 9, 7, 15, 17, 18, 16}};
+  static constexpr int8_t apex[10] = {4, 18, 15, 17, 11, 12, 10, 14, 13, 16};
+};
+
 constexpr int kMaxScs = 12, kMaxScv = 8;
 
 // Area vector of the quad (q0, q1, q2, q3): half the sum of the cross
@@ -114,26 +141,11 @@ __device__ inline void quad_area(const double (&q)[4][3], double (&a)[3])
   for (int d = 0; d < 3; ++d) a[d] *= 0.5;
 }
 
-// Volume of the hexahedron h (exodus order) by 24 triangles about its face
-// centroids and the divergence theorem (Grandy 1997).
-__device__ inline double grandy_volume(const double (&h)[8][3])
+// Divergence-theorem volume of a closed surface of 24 triangles over the
+// points p: sum of (a + b + c) . ((b - a) x (c - a)) / 18.
+__device__ inline double facet_volume(const double (&p)[14][3],
+                                      const int8_t (&tri)[24][3])
 {
-  constexpr int8_t face[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4},
-                                 {2, 3, 7, 6}, {1, 2, 6, 5}, {0, 4, 3, 7}};
-  constexpr int8_t tri[24][3] = {
-    {0, 8, 1},  {8, 2, 1},  {3, 2, 8},  {3, 8, 0},  {6, 9, 5},  {7, 9, 6},
-    {4, 9, 7},  {4, 5, 9},  {10, 0, 1}, {5, 10, 1}, {4, 10, 5}, {4, 0, 10},
-    {7, 6, 11}, {6, 2, 11}, {2, 3, 11}, {3, 7, 11}, {6, 12, 2}, {5, 12, 6},
-    {5, 1, 12}, {1, 2, 12}, {0, 4, 13}, {4, 7, 13}, {7, 3, 13}, {3, 0, 13}};
-  double p[14][3];
-#pragma unroll
-  for (int n = 0; n < 8; ++n)
-    for (int d = 0; d < 3; ++d) p[n][d] = h[n][d];
-#pragma unroll
-  for (int f = 0; f < 6; ++f)
-    for (int d = 0; d < 3; ++d)
-      p[8 + f][d] = 0.25 * (p[face[f][0]][d] + p[face[f][1]][d] +
-                            p[face[f][2]][d] + p[face[f][3]][d]);
   double vol = 0.0;
 #pragma unroll
   for (int t = 0; t < 24; ++t) {
@@ -147,6 +159,59 @@ __device__ inline double grandy_volume(const double (&h)[8][3])
     vol += m[0] * n0 + m[1] * n1 + m[2] * n2;
   }
   return vol / 18.0;
+}
+
+// Hexahedron surface: four triangles about a centre point of each face
+// (points 8-13, faces bottom, top, -y, +y, +x, -x of the exodus order).
+__device__ constexpr int8_t kHexTri[24][3] = {
+  {0, 8, 1},  {8, 2, 1},  {3, 2, 8},  {3, 8, 0},  {6, 9, 5},  {7, 9, 6},
+  {4, 9, 7},  {4, 5, 9},  {10, 0, 1}, {5, 10, 1}, {4, 10, 5}, {4, 0, 10},
+  {7, 6, 11}, {6, 2, 11}, {2, 3, 11}, {3, 7, 11}, {6, 12, 2}, {5, 12, 6},
+  {5, 1, 12}, {1, 2, 12}, {0, 4, 13}, {4, 7, 13}, {7, 3, 13}, {3, 0, 13}};
+
+// Volume of the hexahedron h (exodus order) by 24 triangles about its face
+// centroids and the divergence theorem (Grandy 1997); with bent = true the
+// top face is the two triangles on either side of its 5-7 diagonal (its
+// centre point is that diagonal's midpoint), the Pyr5 base sub-volumes.
+template <bool BENT>
+__device__ inline double grandy_volume(const double (&h)[8][3])
+{
+  constexpr int8_t face[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                 {2, 3, 7, 6}, {1, 2, 6, 5}, {0, 4, 3, 7}};
+  double p[14][3];
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+    for (int d = 0; d < 3; ++d) p[n][d] = h[n][d];
+#pragma unroll
+  for (int f = 0; f < 6; ++f)
+    for (int d = 0; d < 3; ++d)
+      p[8 + f][d] = (BENT && f == 1)
+                      ? 0.5 * (p[5][d] + p[7][d])
+                      : 0.25 * (p[face[f][0]][d] + p[face[f][1]][d] +
+                                p[face[f][2]][d] + p[face[f][3]][d]);
+  return facet_volume(p, kHexTri);
+}
+
+// Volume of the Pyr5 apex sub-volume from its ten vertices o (apex,
+// element centroid, then around the apex: edge midpoint, triangle centroid,
+// ...): its four bent faces are split at the midpoints of o3-o9, o3-o5,
+// o5-o7, o7-o9 (points 10-13) into the 24 triangles below.
+__device__ inline double octahedron_volume(const double (&o)[10][3])
+{
+  constexpr int8_t tri[24][3] = {
+    {1, 3, 10}, {2, 10, 3}, {2, 9, 10}, {10, 9, 1}, {4, 3, 11}, {3, 1, 11},
+    {11, 1, 5}, {4, 11, 5}, {1, 12, 5}, {1, 7, 12}, {12, 7, 6}, {5, 12, 6},
+    {9, 8, 13}, {13, 8, 7}, {13, 7, 1}, {9, 13, 1}, {4, 5, 0},  {5, 6, 0},
+    {6, 7, 0},  {7, 8, 0},  {0, 8, 9},  {0, 9, 2},  {0, 2, 3},  {0, 3, 4}};
+  constexpr int8_t mid[4][2] = {{3, 9}, {3, 5}, {5, 7}, {7, 9}};
+  double p[14][3];
+#pragma unroll
+  for (int n = 0; n < 10; ++n)
+    for (int d = 0; d < 3; ++d) p[n][d] = o[n][d];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    for (int d = 0; d < 3; ++d) p[10 + k][d] = 0.5 * (o[mid[k][0]][d] + o[mid[k][1]][d]);
+  return facet_volume(p, tri);
 }
 
 // Point k of the subdivided element of topology T from its corners c.
@@ -204,12 +269,20 @@ __device__ inline void elem_scv(int64_t el, int64_t nelem,
 {
   double c[T::kNodes][3];
   corners<T>(el, conn, coords, N, c);
+  constexpr bool kPyr = T::kNodes == 5;
+  constexpr int kHexScv = kPyr ? 4 : T::kScv;
 #pragma unroll
-  for (int v = 0; v < T::kScv; ++v) {
+  for (int v = 0; v < kHexScv; ++v) {
     double h[8][3];
 #pragma unroll
     for (int i = 0; i < 8; ++i) sub_point<T>(c, T::scv[v][i], h[i]);
-    scv[v * nelem + el] = grandy_volume(h);
+    scv[v * nelem + el] = grandy_volume<kPyr>(h);
+  }
+  if constexpr (kPyr) {
+    double o[10][3];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) sub_point<T>(c, T::apex[i], o[i]);
+    scv[4 * nelem + el] = octahedron_volume(o);
   }
 }
 
@@ -225,8 +298,10 @@ __global__ __launch_bounds__(kBlock) void k_elem_scs(
     elem_scs<HexTopo>(el, nelem, conn, coords, N, scs);
   else if (t == 1)
     elem_scs<TetTopo>(el, nelem, conn, coords, N, scs);
-  else
+  else if (t == 2)
     elem_scs<WedTopo>(el, nelem, conn, coords, N, scs);
+  else
+    elem_scs<PyrTopo>(el, nelem, conn, coords, N, scs);
 }
 
 __global__ __launch_bounds__(kBlock) void k_elem_scv(
@@ -241,8 +316,10 @@ __global__ __launch_bounds__(kBlock) void k_elem_scv(
     elem_scv<HexTopo>(el, nelem, conn, coords, N, scv);
   else if (t == 1)
     elem_scv<TetTopo>(el, nelem, conn, coords, N, scv);
-  else
+  else if (t == 2)
     elem_scv<WedTopo>(el, nelem, conn, coords, N, scv);
+  else
+    elem_scv<PyrTopo>(el, nelem, conn, coords, N, scv);
 }
 
 // areav[3 e + d] = sum over the edge's surfaces (element order) of

@@ -147,7 +147,7 @@ struct Mpnn;       // learned initial guess (nw_mpnn.hip)
 // (nw_mesh_elements, nw_compute_geometry); all device arrays.
 struct Geom {
   int64_t nelem = 0;
-  int32_t* type = nullptr;   // [nelem] 0 Hex8, 1 Tet4, 2 Wed6
+  int32_t* type = nullptr;   // [nelem] 0 Hex8, 1 Tet4, 2 Wed6, 3 Pyr5
   int32_t* conn = nullptr;   // [nelem][8]
   int32_t* e_ptr = nullptr;  // [nedge + 1]
   int32_t* e_inc = nullptr;  // 2 (12 el + s) + flip, element order per edge

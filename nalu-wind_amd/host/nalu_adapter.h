@@ -283,7 +283,7 @@ class MomentumEdgePecletAlg {
 
 // GeometryInteriorAlg (src/ngp_algorithms/GeometryInteriorAlg.C:70-225):
 // the edge area vectors and dual nodal volumes of the realm's elements
-// (type [nelem]: 0 Hex8, 1 Tet4, 2 Wed6, or null for all Hex8; conn
+// (type [nelem]: 0 Hex8, 1 Tet4, 2 Wed6, 3 Pyr5, or null for all Hex8; conn
 // [nelem][8]) from its coordinates, on the device.  The constructor hands
 // the elements over once; execute() recomputes after the nodes move.
 class GeometryInteriorAlg {

@@ -12,7 +12,8 @@ namespace geom {
 // HexSCS::determinant_scs (src/master_element/Hex8CVFEM.C:567-591).
 void hex_scs(const double c[8][3], double a[12][3]);
 
-// Nodes and sub-control surfaces of topology t (0 Hex8, 1 Tet4, 2 Wed6).
+// Nodes and sub-control surfaces of topology t (0 Hex8, 1 Tet4, 2 Wed6,
+// 3 Pyr5).
 int topo_nodes(int t);
 int topo_scs(int t);
 

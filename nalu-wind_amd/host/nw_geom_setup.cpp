@@ -15,20 +15,23 @@ namespace {
 
 // Element-local L/R node of each SCS, in SCS order: Hex8
 // (include/master_element/Hex8CVFEM.h:292-293), Tet4 (Tet4CVFEM.h:266),
-// Wed6 (Wed6CVFEM.h:268).
-const int kLr[3][12][2] = {
+// Wed6 (Wed6CVFEM.h:268), Pyr5 (Pyr5CVFEM.h:300-301: two surfaces on each
+// apex edge).
+const int kLr[4][12][2] = {
   {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {4, 5}, {5, 6}, {6, 7}, {4, 7}, {0, 4},
    {1, 5}, {2, 6}, {3, 7}},
   {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}},
-  {{0, 1}, {1, 2}, {0, 2}, {3, 4}, {4, 5}, {3, 5}, {0, 3}, {1, 4}, {2, 5}}};
+  {{0, 1}, {1, 2}, {0, 2}, {3, 4}, {4, 5}, {3, 5}, {0, 3}, {1, 4}, {2, 5}},
+  {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {0, 4}, {0, 4}, {1, 4}, {1, 4}, {2, 4},
+   {2, 4}, {3, 4}, {3, 4}}};
 
 }  // namespace
 
 namespace nw {
 namespace geom {
 
-int topo_nodes(int t) { return t == 0 ? 8 : (t == 1 ? 4 : 6); }
-int topo_scs(int t) { return t == 0 ? 12 : (t == 1 ? 6 : 9); }
+int topo_nodes(int t) { return t == 0 ? 8 : (t == 1 ? 4 : (t == 2 ? 6 : 5)); }
+int topo_scs(int t) { return t == 0 ? 12 : (t == 1 ? 6 : (t == 2 ? 9 : 12)); }
 
 int build_incidence(int64_t nelem, const int32_t* type, const int32_t* conn,
                     int64_t nnode, int64_t nedge, const int32_t* edges,
@@ -38,7 +41,7 @@ int build_incidence(int64_t nelem, const int32_t* type, const int32_t* conn,
   if (nelem < 0 || nelem >= (int64_t{1} << 26)) return NW_E_ARG;
   for (int64_t el = 0; el < nelem; ++el) {
     const int t = type ? type[el] : 0;
-    if (t < 0 || t > 2) return NW_E_ARG;
+    if (t < 0 || t > 3) return NW_E_ARG;
     for (int a = 0; a < topo_nodes(t); ++a) {
       const int32_t n = conn[8 * el + a];
       if (n < 0 || n >= nnode) return NW_E_ARG;

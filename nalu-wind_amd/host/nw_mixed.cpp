@@ -1,8 +1,10 @@
-// nw_mixed.cpp -- unstructured mixed Hex8 / Tet4 / Wed6 meshes for the edge
-// path: a box generator (cells split into Kuhn tetrahedra or prisms by a
-// seeded smooth random field, nodes randomly permuted and then renumbered by
-// reverse Cuthill-McKee), edge extraction for all three topologies and the
-// CVFEM edge area vectors of Tet4 and Wed6 sub-control surfaces.
+// nw_mixed.cpp -- unstructured conforming Hex8 / Tet4 / Wed6 / Pyr5 meshes
+// for the edge path: a box generator (lattice cells kept as hexahedra, split
+// into Kuhn tetrahedra or prisms by a seeded smooth random field; cells whose
+// faces would not match a neighbour's become transition cells of pyramids and
+// tetrahedra about their centre; nodes randomly permuted and then renumbered
+// by reverse Cuthill-McKee), edge extraction for all four topologies and the
+// CVFEM edge area vectors of their sub-control surfaces.
 //
 // Element geometry runs in parallel in element chunks and is accumulated
 // onto edges serially in element order (thread-count independent results).
@@ -23,9 +25,12 @@ namespace {
 // a set of its vertices; each SCS is a quad of four such points, and its
 // area vector points from the L to the R node of its edge.  The point lists
 // and quads follow TetSCS::determinant_scs
-// (src/master_element/Tet4CVFEM.C:522-618) and WedSCS::determinant_scs
-// (src/master_element/Wed6CVFEM.C:541-646); L/R pairs are the lrscv tables
-// (include/master_element/Tet4CVFEM.h:266, Wed6CVFEM.h:268).
+// (src/master_element/Tet4CVFEM.C:522-618), WedSCS::determinant_scs
+// (src/master_element/Wed6CVFEM.C:541-646) and PyrSCS::determinant_scs
+// (src/master_element/Pyr5CVFEM.C:771-900, whose surfaces between a base
+// node and the apex are pairs of triangles: quads with a repeated point);
+// L/R pairs are the lrscv tables (include/master_element/Tet4CVFEM.h:266,
+// Wed6CVFEM.h:268, Pyr5CVFEM.h:300-301).
 struct Topo {
   int npe, nscs, npts;
   const int (*pts)[7];  // {count, vertices...}
@@ -52,6 +57,17 @@ const int kWedQuad[9][4] = {{6, 9, 20, 16},   {7, 9, 20, 18},  {9, 8, 19, 20},
 const int kWedLr[9][2] = {{0, 1}, {1, 2}, {0, 2}, {3, 4}, {4, 5},
                           {3, 5}, {0, 3}, {1, 4}, {2, 5}};
 
+const int kPyrPts[19][7] = {
+  {1, 0}, {1, 1}, {1, 2}, {1, 3}, {1, 4}, {2, 0, 1}, {2, 1, 2}, {2, 2, 3},
+  {2, 3, 0}, {4, 0, 1, 2, 3}, {2, 1, 4}, {2, 4, 0}, {3, 0, 1, 4}, {2, 2, 4},
+  {3, 1, 2, 4}, {2, 3, 4}, {3, 3, 4, 2}, {3, 0, 4, 3}, {5, 0, 1, 2, 3, 4}};
+const int kPyrQuad[12][4] = {
+  {5, 9, 18, 12},   {6, 9, 18, 14},   {7, 9, 18, 16},   {8, 17, 18, 9},
+  {12, 12, 18, 17}, {11, 12, 12, 17}, {14, 14, 18, 12}, {10, 14, 14, 12},
+  {16, 16, 18, 14}, {13, 16, 16, 14}, {17, 17, 18, 16}, {15, 17, 17, 16}};
+const int kPyrLr[12][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {0, 4}, {0, 4},
+                           {1, 4}, {1, 4}, {2, 4}, {2, 4}, {3, 4}, {3, 4}};
+
 // Hex8 edges in the order of its SCS (include/master_element/Hex8CVFEM.h:
 // 292-293); its area vectors come from the structured path's Hex8 SCS.
 const int kHexLr[12][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {4, 5}, {5, 6},
@@ -59,10 +75,16 @@ const int kHexLr[12][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {4, 5}, {5, 6},
 
 const Topo kTet = {4, 6, 15, kTetPts, kTetQuad, kTetLr};
 const Topo kWed = {6, 9, 21, kWedPts, kWedQuad, kWedLr};
+const Topo kPyr = {5, 12, 19, kPyrPts, kPyrQuad, kPyrLr};
 
-int npe_of(int t) { return t == 0 ? 8 : (t == 1 ? 4 : 6); }
-int nedge_of(int t) { return t == 0 ? 12 : (t == 1 ? 6 : 9); }
-const int (*lr_of(int t))[2] { return t == 0 ? kHexLr : (t == 1 ? kTetLr : kWedLr); }
+// per topology (0 Hex8, 1 Tet4, 2 Wed6, 3 Pyr5): nodes, sub-control
+// surfaces (one per edge, two per apex edge of Pyr5), their L/R nodes
+int npe_of(int t) { return t == 0 ? 8 : (t == 1 ? 4 : (t == 2 ? 6 : 5)); }
+int nscs_of(int t) { return t == 0 ? 12 : (t == 1 ? 6 : (t == 2 ? 9 : 12)); }
+const int (*lr_of(int t))[2]
+{
+  return t == 0 ? kHexLr : (t == 1 ? kTetLr : (t == 2 ? kWedLr : kPyrLr));
+}
 
 void quad_area4(const double q[4][3], double a[3])
 {
@@ -84,7 +106,7 @@ void quad_area4(const double q[4][3], double a[3])
 
 void scs_areas(const Topo& T, const double c[][3], double a[][3])
 {
-  double p[21][3];
+  double p[21][3];  // the largest point set (Wed6)
   for (int k = 0; k < T.npts; ++k) {
     const int cnt = T.pts[k][0];
     for (int d = 0; d < 3; ++d) {
@@ -122,6 +144,19 @@ const int kKuhn[6][4] = {{0, 1, 2, 6}, {0, 2, 3, 6}, {0, 3, 7, 6},
 // two prisms: bottom/top triangles split along the 0-2 / 4-6 diagonals
 const int kPrism[2][6] = {{0, 1, 2, 4, 5, 6}, {0, 2, 3, 4, 6, 7}};
 
+// Faces of a lattice cell (-x, +x, -y, +y, -z, +z; face f and f ^ 1 are
+// opposite) as cycles of hex vertices that start at the face's lowest
+// lattice corner, so q0-q2 is the diagonal the Kuhn tetrahedra and the
+// prisms cut the face along: a triangulated face is (q0 q1 q2), (q0 q2 q3).
+const int kCellFace[6][4] = {{0, 3, 7, 4}, {1, 2, 6, 5}, {0, 1, 5, 4},
+                             {3, 2, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
+
+enum : int8_t { kCellHex = 0, kCellTet = 1, kCellPrism = 2, kCellTransition = 3 };
+
+// Whether a cell of class k cuts face f into two triangles: Kuhn cells all
+// faces, prism cells their end faces (z), hexahedra none.
+bool cuts_face(int k, int f) { return k == kCellTet || (k == kCellPrism && f >= 4); }
+
 double tet_volume(const double c[][3])
 {
   double u[3], v[3], w[3];
@@ -146,9 +181,9 @@ int build_edges(int64_t nelem, const int32_t* type, const int32_t* conn,
   std::vector<int64_t> cnt(nnode + 1, 0);
   for (int64_t el = 0; el < nelem; ++el) {
     const int t = type[el];
-    if (t < 0 || t > 2) return NW_E_ARG;
+    if (t < 0 || t > 3) return NW_E_ARG;
     const auto lr = lr_of(t);
-    for (int k = 0; k < nedge_of(t); ++k) {
+    for (int k = 0; k < nscs_of(t); ++k) {
       const int32_t a = conn[8 * el + lr[k][0]], b = conn[8 * el + lr[k][1]];
       if (a < 0 || b < 0 || a >= nnode || b >= nnode || a == b) return NW_E_ARG;
       cnt[std::min(a, b) + 1]++;
@@ -160,7 +195,7 @@ int build_edges(int64_t nelem, const int32_t* type, const int32_t* conn,
     std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
     for (int64_t el = 0; el < nelem; ++el) {
       const auto lr = lr_of(type[el]);
-      for (int k = 0; k < nedge_of(type[el]); ++k) {
+      for (int k = 0; k < nscs_of(type[el]); ++k) {
         const int32_t a = conn[8 * el + lr[k][0]], b = conn[8 * el + lr[k][1]];
         nb[pos[std::min(a, b)]++] = std::max(a, b);
       }
@@ -248,9 +283,8 @@ int nw_mixed_box(int n, double f_hex, double f_tet, double jitter,
   if (n < 1 || f_hex < 0 || f_tet < 0 || f_hex + f_tet > 1.0 || jitter < 0 ||
       jitter >= 0.5 || !nnode_out || !nelem_out)
     return NW_E_ARG;
-  const int64_t p = n + 1, nn = p * p * p, nc = (int64_t)n * n * n;
-  if (nn >= INT32_MAX) return NW_E_ARG;
-  // cell types: quantiles of a smooth seeded field (4 random Fourier modes)
+  const int64_t p = n + 1, nl = p * p * p, nc = (int64_t)n * n * n;
+  // cell classes: quantiles of a smooth seeded field (4 random Fourier modes)
   double k[4][3], ph[4];
   for (int m = 0; m < 4; ++m) {
     for (int d = 0; d < 3; ++d) k[m][d] = 2.0 * M_PI * (1.0 + 2.0 * uni(seed, 10 * m + d));
@@ -269,20 +303,67 @@ int nw_mixed_box(int n, double f_hex, double f_tet, double jitter,
   std::vector<int64_t> ord(nc);
   std::iota(ord.begin(), ord.end(), 0);
   std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return f[a] < f[b]; });
-  std::vector<int8_t> ct(nc);
+  std::vector<int8_t> cls(nc);
   const int64_t nh = (int64_t)std::llround(f_hex * nc);
   const int64_t nt = (int64_t)std::llround(f_tet * nc);
-  for (int64_t r = 0; r < nc; ++r) ct[ord[r]] = r < nh ? 0 : (r < nh + nt ? 1 : 2);
-  int64_t ne = 0;
-  for (int64_t c = 0; c < nc; ++c) ne += ct[c] == 0 ? 1 : (ct[c] == 1 ? 6 : 2);
+  for (int64_t r = 0; r < nc; ++r)
+    cls[ord[r]] = r < nh ? kCellHex : (r < nh + nt ? kCellTet : kCellPrism);
+  auto neighbour = [&](int64_t c, int fc) -> int64_t {
+    const int64_t i = c % n, j = (c / n) % n, l = c / ((int64_t)n * n);
+    const int64_t idx[3] = {i, j, l}, stride[3] = {1, n, (int64_t)n * n};
+    const int d = fc >> 1;
+    const int64_t v = idx[d] + ((fc & 1) ? 1 : -1);
+    return (v < 0 || v >= n) ? -1 : c + ((fc & 1) ? stride[d] : -stride[d]);
+  };
+  // a cell that keeps a face whole against a neighbour that cuts it becomes
+  // a transition cell: pyramids on its whole faces, tetrahedra on its cut
+  // ones, all about a node at its centre
+  std::vector<int8_t> ct(cls);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < nc; ++c) {
+    if (cls[c] == kCellTet) continue;
+    for (int fc = 0; fc < 6; ++fc) {
+      const int64_t o = neighbour(c, fc);
+      if (o >= 0 && !cuts_face(cls[c], fc) && cuts_face(cls[o], fc ^ 1))
+        ct[c] = kCellTransition;
+    }
+  }
+  // faces a transition cell cuts: those its (kept) neighbour cuts
+  std::vector<uint8_t> cut(nc, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < nc; ++c) {
+    if (ct[c] != kCellTransition) continue;
+    uint8_t m = 0;
+    for (int fc = 0; fc < 6; ++fc) {
+      const int64_t o = neighbour(c, fc);
+      if (o >= 0 && ct[o] != kCellTransition && cuts_face(ct[o], fc ^ 1))
+        m |= (uint8_t)(1u << fc);
+    }
+    cut[c] = m;
+  }
+  std::vector<int64_t> ebase(nc + 1, 0), centre(nc, -1);
+  int64_t nx = 0;
+  for (int64_t c = 0; c < nc; ++c) {
+    int64_t m = 1;
+    if (ct[c] == kCellTet) m = 6;
+    if (ct[c] == kCellPrism) m = 2;
+    if (ct[c] == kCellTransition) {
+      m = 6 + __builtin_popcount(cut[c]);  // a pyramid or two tetrahedra per face
+      centre[c] = nl + nx++;
+    }
+    ebase[c + 1] = ebase[c] + m;
+  }
+  const int64_t nn = nl + nx, ne = ebase[nc];
+  if (nn >= INT32_MAX) return NW_E_ARG;
   *nnode_out = nn;
   *nelem_out = ne;
   if (!coords) return NW_OK;  // sizes only
   if (!type || !conn) return NW_E_ARG;
-  // base numbering x fastest; interior nodes jittered
+  // base numbering: lattice nodes x fastest (interior ones jittered), then
+  // the transition-cell centres (mean of the cell's corners) in cell order
   std::vector<double> xb(3 * nn);
 #pragma omp parallel for schedule(static)
-  for (int64_t v = 0; v < nn; ++v) {
+  for (int64_t v = 0; v < nl; ++v) {
     const int64_t i = v % p, j = (v / p) % p, l = v / (p * p);
     const int64_t idx[3] = {i, j, l};
     for (int d = 0; d < 3; ++d) {
@@ -292,35 +373,77 @@ int nw_mixed_box(int n, double f_hex, double f_tet, double jitter,
       xb[3 * v + d] = x;
     }
   }
+  auto corners = [&](int64_t c, int64_t hv[8]) {
+    const int64_t i = c % n, j = (c / n) % n, l = c / ((int64_t)n * n);
+    const int64_t v0 = i + p * (j + p * l);
+    const int64_t h[8] = {v0, v0 + 1, v0 + 1 + p, v0 + p, v0 + p * p,
+                          v0 + 1 + p * p, v0 + 1 + p + p * p, v0 + p + p * p};
+    for (int a = 0; a < 8; ++a) hv[a] = h[a];
+  };
+  auto X = [&](int64_t v, int d) { return xb[3 * v + d]; };
   std::vector<int32_t> tb(ne), cb(8 * ne, -1);
-  {
-    int64_t e = 0;
-    for (int64_t c = 0; c < nc; ++c) {
-      const int64_t i = c % n, j = (c / n) % n, l = c / ((int64_t)n * n);
-      const int64_t v0 = i + p * (j + p * l);
-      const int64_t hv[8] = {v0, v0 + 1, v0 + 1 + p, v0 + p, v0 + p * p,
-                             v0 + 1 + p * p, v0 + 1 + p + p * p, v0 + p + p * p};
-      if (ct[c] == 0) {
-        tb[e] = 0;
-        for (int a = 0; a < 8; ++a) cb[8 * e + a] = (int32_t)hv[a];
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < nc; ++c) {
+    int64_t hv[8];
+    corners(c, hv);
+    int64_t e = ebase[c];
+    // tetrahedron (q0 q1 q2 q3) with positive volume
+    auto put_tet = [&](int64_t q0, int64_t q1, int64_t q2, int64_t q3) {
+      double cc[4][3];
+      const int64_t q[4] = {q0, q1, q2, q3};
+      for (int a = 0; a < 4; ++a)
+        for (int d = 0; d < 3; ++d) cc[a][d] = X(q[a], d);
+      const bool flip = tet_volume(cc) < 0;
+      tb[e] = 1;
+      cb[8 * e] = (int32_t)q0;
+      cb[8 * e + 1] = (int32_t)(flip ? q2 : q1);
+      cb[8 * e + 2] = (int32_t)(flip ? q1 : q2);
+      cb[8 * e + 3] = (int32_t)q3;
+      ++e;
+    };
+    if (ct[c] == kCellHex) {
+      tb[e] = 0;
+      for (int a = 0; a < 8; ++a) cb[8 * e + a] = (int32_t)hv[a];
+    } else if (ct[c] == kCellTet) {
+      for (int t = 0; t < 6; ++t)
+        put_tet(hv[kKuhn[t][0]], hv[kKuhn[t][1]], hv[kKuhn[t][2]], hv[kKuhn[t][3]]);
+    } else if (ct[c] == kCellPrism) {
+      for (int t = 0; t < 2; ++t, ++e) {
+        tb[e] = 2;
+        for (int a = 0; a < 6; ++a) cb[8 * e + a] = (int32_t)hv[kPrism[t][a]];
+      }
+    } else {
+      const int64_t ctr = centre[c];
+      for (int d = 0; d < 3; ++d) {
+        double s = 0.0;
+        for (int a = 0; a < 8; ++a) s += X(hv[a], d);
+        xb[3 * ctr + d] = 0.125 * s;
+      }
+      for (int fc = 0; fc < 6; ++fc) {
+        int64_t q[4];
+        for (int a = 0; a < 4; ++a) q[a] = hv[kCellFace[fc][a]];
+        if (cut[c] >> fc & 1) {
+          put_tet(q[0], q[1], q[2], ctr);
+          put_tet(q[0], q[2], q[3], ctr);
+          continue;
+        }
+        // pyramid: base ordered so that its right-hand normal points to the
+        // apex (the quad's vector area (q2 - q0) x (q3 - q1) / 2)
+        double u[3], w[3], m[3], nrm[3];
+        for (int d = 0; d < 3; ++d) {
+          u[d] = X(q[2], d) - X(q[0], d);
+          w[d] = X(q[3], d) - X(q[1], d);
+          m[d] = xb[3 * ctr + d] -
+                 0.25 * (X(q[0], d) + X(q[1], d) + X(q[2], d) + X(q[3], d));
+        }
+        nrm[0] = u[1] * w[2] - u[2] * w[1];
+        nrm[1] = u[2] * w[0] - u[0] * w[2];
+        nrm[2] = u[0] * w[1] - u[1] * w[0];
+        if (nrm[0] * m[0] + nrm[1] * m[1] + nrm[2] * m[2] < 0) std::swap(q[1], q[3]);
+        tb[e] = 3;
+        for (int a = 0; a < 4; ++a) cb[8 * e + a] = (int32_t)q[a];
+        cb[8 * e + 4] = (int32_t)ctr;
         ++e;
-      } else if (ct[c] == 1) {
-        for (int t = 0; t < 6; ++t, ++e) {
-          tb[e] = 1;
-          int32_t q[4];
-          double cc[4][3];
-          for (int a = 0; a < 4; ++a) {
-            q[a] = (int32_t)hv[kKuhn[t][a]];
-            for (int d = 0; d < 3; ++d) cc[a][d] = xb[3 * (int64_t)q[a] + d];
-          }
-          if (tet_volume(cc) < 0) std::swap(q[1], q[2]);
-          for (int a = 0; a < 4; ++a) cb[8 * e + a] = q[a];
-        }
-      } else {
-        for (int t = 0; t < 2; ++t, ++e) {
-          tb[e] = 2;
-          for (int a = 0; a < 6; ++a) cb[8 * e + a] = (int32_t)hv[kPrism[t][a]];
-        }
       }
     }
   }
@@ -397,6 +520,10 @@ int nw_mixed_edge_area_vector(int64_t nelem, const int32_t* type,
 #pragma omp parallel for schedule(static) reduction(| : bad)
     for (int64_t el = e0; el < e1; ++el) {
       const int t = type[el];
+      if (t < 0 || t > 3) {
+        bad = 1;
+        continue;
+      }
       double c[8][3];
       for (int a = 0; a < npe_of(t); ++a)
         for (int d = 0; d < 3; ++d) c[a][d] = coords[3 * (int64_t)conn[8 * el + a] + d];
@@ -404,10 +531,10 @@ int nw_mixed_edge_area_vector(int64_t nelem, const int32_t* type,
       if (t == 0) {
         nw::geom::hex_scs(c, a);
       } else {
-        scs_areas(t == 1 ? kTet : kWed, c, a);
+        scs_areas(t == 1 ? kTet : (t == 2 ? kWed : kPyr), c, a);
       }
       const auto lr = lr_of(t);
-      for (int s = 0; s < nedge_of(t); ++s) {
+      for (int s = 0; s < nscs_of(t); ++s) {
         const int32_t nl = conn[8 * el + lr[s][0]], nr = conn[8 * el + lr[s][1]];
         const int64_t e = find(std::min(nl, nr), std::max(nl, nr));
         if (e < 0) bad = 1;
@@ -417,7 +544,7 @@ int nw_mixed_edge_area_vector(int64_t nelem, const int32_t* type,
     if (bad) return NW_E_ARG;
     // element-ordered accumulation, +1 when L is the edge's node[0]
     for (int64_t el = e0; el < e1; ++el)
-      for (int s = 0; s < nedge_of(type[el]); ++s) {
+      for (int s = 0; s < nscs_of(type[el]); ++s) {
         const int64_t code = eid[(el - e0) * 12 + s];
         const int64_t e = code >= 0 ? code : -code - 1;
         const double sgn = code >= 0 ? 1.0 : -1.0;

@@ -265,7 +265,9 @@ def hex_box(nx, ny, nz, stretch_z=1.0, jitter=0.0, seed=0):
 
 
 def mixed_box(n, f_hex=0.6, f_tet=0.3, jitter=0.0, seed=0):
-    """Mixed Hex8/Tet4/Wed6 box: (coords [N][3], type [E], conn [E][8])."""
+    """Conforming mixed Hex8/Tet4/Wed6/Pyr5 box (transition cells of Pyr5
+    and Tet4 about a centre node where Hex8 or Wed6 faces meet cut ones):
+    (coords [N][3], type [E], conn [E][8])."""
     nn, ne = C.c_int64(), C.c_int64()
     check(lib.nw_mixed_box(n, f_hex, f_tet, jitter, seed, C.byref(nn),
                            C.byref(ne), None, None, None))
@@ -570,7 +572,7 @@ class Context:
 
     def mesh_elements(self, types, conn):
         """The mesh's elements for the device geometry: types [nelem]
-        (0 Hex8, 1 Tet4, 2 Wed6; None = all Hex8), conn [nelem][8]."""
+        (0 Hex8, 1 Tet4, 2 Wed6, 3 Pyr5; None = all Hex8), conn [nelem][8]."""
         conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 8)
         t = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
         check(lib.nw_mesh_elements(

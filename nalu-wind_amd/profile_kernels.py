@@ -31,7 +31,7 @@ def run(kernel, n, reps, copy_mib):
         ndof = 1
         fn = lambda ctx: ctx.assemble_continuity(W.opts)
     elif kernel == "mixed":
-        # continuity on the mixed Hex8/Tet4/Wed6 box (RCM-numbered)
+        # continuity on the conforming mixed Hex8/Tet4/Wed6/Pyr5 box (RCM-numbered)
         W = wl.continuity_mixed(n)
         ndof = 1
         fn = lambda ctx: ctx.assemble_continuity(W.opts)

@@ -18,8 +18,10 @@ edges and geometry, and the seeded node/edge fields of one configuration:
                  face z = 1
   continuity_mixed  the continuity fields on the unstructured mixed mesh: an
                  n^3 box cut into Hex8/Tet4/Wed6 cells (60/30/10 % of the
-                 cells, seeded), +-10 % node jitter, nodes randomly permuted
-                 and then RCM-renumbered by the library's generator
+                 cells, seeded), conforming through Pyr5 + Tet4 transition
+                 cells about a centre node, +-10 % node jitter, nodes
+                 randomly permuted and then RCM-renumbered by the library's
+                 generator
 """
 import math
 import os
@@ -153,8 +155,8 @@ def continuity_rf(n=256, seed=1234, stretch=None, dt=1.0e-3, t=0.0,
 
 
 class MixedWorkload(Workload):
-    """Mixed Hex8/Tet4/Wed6 box from the library's generator (RCM node
-    order), its edges and edge area vectors."""
+    """Conforming mixed Hex8/Tet4/Wed6/Pyr5 box from the library's
+    generator (RCM node order), its edges and edge area vectors."""
 
     def __init__(self, name, n, f_hex, f_tet, jitter, seed, keep_elements=False):
         self.name = name
@@ -165,7 +167,7 @@ class MixedWorkload(Workload):
         self.ne = self.edges.shape[0]
         self.areav = nw.mixed_edge_area_vector(types, conn, self.coords,
                                                self.edges)
-        self.elements = np.bincount(types, minlength=3)  # hex, tet, wedge
+        self.elements = np.bincount(types, minlength=4)  # hex, tet, wedge, pyramid
         self.dualvol = None
         self.fields = {}
         self.top = None

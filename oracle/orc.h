@@ -71,14 +71,24 @@ extern const int orc_wed_lrscv[18];
 void orc_tet_scv_volume(const double coords[12], double vol[4]);
 void orc_wed_scv_volume(const double coords[18], double vol[6]);
 
-/* Dual nodal volumes of a mixed mesh (type 0 Hex8, 1 Tet4, 2 Wed6; conn
- * [nelem][8]), src/ngp_algorithms/GeometryInteriorAlg.C:70-112. */
+/* Pyr5 SCS area vectors (12 sub-control surfaces, L/R in orc_pyr_lrscv:
+ * the base edges one each, the apex edges two each) and SCV volumes (4
+ * bent hexahedra at the base nodes, an octahedron at the apex), restating
+ * PyrSCS::determinant_scs and PyrSCV::determinant_scv
+ * (src/master_element/Pyr5CVFEM.C:437-571, 771-900) and lrscv_
+ * (include/master_element/Pyr5CVFEM.h:300-301). */
+void orc_pyr_scs_areav(const double coords[15], double areav[36]);
+void orc_pyr_scv_volume(const double coords[15], double vol[5]);
+extern const int orc_pyr_lrscv[24];
+
+/* Dual nodal volumes of a mixed mesh (type 0 Hex8, 1 Tet4, 2 Wed6, 3 Pyr5;
+ * conn [nelem][8]), src/ngp_algorithms/GeometryInteriorAlg.C:70-112. */
 void orc_dual_volume_mixed(int64_t nelem, const int32_t* type,
                            const int32_t* conn, const double* coords,
                            int64_t nnode, double* dualvol);
 
-/* Edge area vectors of a mixed mesh (type 0 Hex8, 1 Tet4, 2 Wed6; conn
- * [nelem][8]); edges sorted (node[0] < node[1]).  Returns -1 if an element
+/* Edge area vectors of a mixed mesh (type 0 Hex8, 1 Tet4, 2 Wed6, 3 Pyr5;
+ * conn [nelem][8]); edges sorted (node[0] < node[1]).  Returns -1 if an element
  * edge is missing from the list. */
 int orc_edge_area_vec_mixed(int64_t nelem, const int32_t* type,
                             const int32_t* conn, const double* coords,

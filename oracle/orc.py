@@ -142,6 +142,8 @@ lib.orc_wed_scs_areav.argtypes = [_f64p, _f64p]
 lib.orc_edge_area_vec_mixed.argtypes = [C.c_int64, _i32p, _i32p, _f64p, _i32p,
                                         C.c_int64, _f64p]
 lib.orc_edge_area_vec_mixed.restype = C.c_int
+lib.orc_pyr_scs_areav.argtypes = [_f64p, _f64p]
+lib.orc_pyr_scv_volume.argtypes = [_f64p, _f64p]
 lib.orc_tet_scv_volume.argtypes = [_f64p, _f64p]
 lib.orc_wed_scv_volume.argtypes = [_f64p, _f64p]
 lib.orc_dual_volume_mixed.argtypes = [C.c_int64, _i32p, _i32p, _f64p, C.c_int64,
@@ -212,6 +214,22 @@ def wed_scv_volume(coords6):
     out = np.zeros(6)
     lib.orc_wed_scv_volume(f64(coords6).ravel(), out)
     return out
+
+
+def pyr_scs_areav(coords5):
+    out = np.zeros(36)
+    lib.orc_pyr_scs_areav(f64(coords5).ravel(), out)
+    return out.reshape(12, 3)
+
+
+def pyr_scv_volume(coords5):
+    out = np.zeros(5)
+    lib.orc_pyr_scv_volume(f64(coords5).ravel(), out)
+    return out
+
+
+PYR_LRSCV = np.array([0, 1, 1, 2, 2, 3, 0, 3, 0, 4, 0, 4, 1, 4, 1, 4, 2, 4,
+                      2, 4, 3, 4, 3, 4]).reshape(12, 2)
 
 
 def dual_volume_mixed(types, conn, coords):

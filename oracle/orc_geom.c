@@ -173,6 +173,61 @@ void orc_wed_scs_areav(const double c[18], double areav[27])
   }
 }
 
+/* Pyr5 points, PyrSCS::determinant_scs / PyrSCV::determinant_scv
+ * (src/master_element/Pyr5CVFEM.C:437-571, 771-900): the 5 vertices, the
+ * base edge midpoints (5-8), the base centroid (9), apex edge midpoints and
+ * triangle centroids (10-17) and the element centroid (18), formed in the
+ * reference's order of operations. */
+static void pyr_points(const double c[15], double v[19][3])
+{
+#define C(n, d) c[3 * (n) + (d)]
+  const double one3rd = 1.0 / 3.0;
+  for (int d = 0; d < 3; ++d) {
+    for (int n = 0; n < 5; ++n) v[n][d] = C(n, d);
+    v[5][d] = 0.5 * (C(0, d) + C(1, d));
+    v[6][d] = 0.5 * (C(1, d) + C(2, d));
+    v[7][d] = 0.5 * (C(2, d) + C(3, d));
+    v[8][d] = 0.5 * (C(3, d) + C(0, d));
+    v[9][d] = 0.25 * (C(0, d) + C(1, d) + C(2, d) + C(3, d));
+    v[10][d] = 0.5 * (C(1, d) + C(4, d));
+    v[11][d] = 0.5 * (C(4, d) + C(0, d));
+    v[12][d] = one3rd * (C(0, d) + C(1, d) + C(4, d));
+    v[13][d] = 0.5 * (C(2, d) + C(4, d));
+    v[14][d] = one3rd * (C(1, d) + C(2, d) + C(4, d));
+    v[15][d] = 0.5 * (C(3, d) + C(4, d));
+    v[16][d] = one3rd * (C(3, d) + C(4, d) + C(2, d));
+    v[17][d] = one3rd * (C(0, d) + C(4, d) + C(3, d));
+    v[18][d] = 0.0;
+    for (int n = 0; n < 5; ++n) v[18][d] += 0.2 * C(n, d);
+  }
+#undef C
+}
+
+/* PyrSCS::determinant_scs (Pyr5CVFEM.C:771-900): 12 surfaces, each a quad
+ * of the points above; the surfaces between a base node and the apex are
+ * two triangles each (written as quads with a repeated point), the inner
+ * one towards the element centroid first. */
+void orc_pyr_scs_areav(const double c[15], double areav[36])
+{
+  static const int facet[12][4] = {
+    {5, 9, 18, 12},   {6, 9, 18, 14},   {7, 9, 18, 16},   {8, 17, 18, 9},
+    {12, 12, 18, 17}, {11, 12, 12, 17}, {14, 14, 18, 12}, {10, 14, 14, 12},
+    {16, 16, 18, 14}, {13, 16, 16, 14}, {17, 17, 18, 16}, {15, 17, 17, 16}};
+  double v[19][3];
+  pyr_points(c, v);
+  for (int ics = 0; ics < 12; ++ics) {
+    double q[4][3];
+    for (int i = 0; i < 4; ++i)
+      for (int d = 0; d < 3; ++d) q[i][d] = v[facet[ics][i]][d];
+    quad_area(q, &areav[3 * ics]);
+  }
+}
+
+/* include/master_element/Pyr5CVFEM.h:300-305 (lrscv_): surfaces 4-11 come
+ * in pairs on the four apex edges. */
+const int orc_pyr_lrscv[24] = {0, 1, 1, 2, 2, 3, 0, 3, 0, 4, 0, 4,
+                               1, 4, 1, 4, 2, 4, 2, 4, 3, 4, 3, 4};
+
 /* include/master_element/Tet4CVFEM.h:266, Wed6CVFEM.h:268 */
 const int orc_tet_lrscv[12] = {0, 1, 1, 2, 0, 2, 0, 3, 1, 3, 2, 3};
 const int orc_wed_lrscv[18] = {0, 1, 1, 2, 0, 2, 3, 4, 4, 5,
@@ -195,11 +250,12 @@ static int64_t find_edge(const int32_t* edges, int64_t nedge, int32_t a,
   return -1;
 }
 
-/* Edge area vectors of a mixed Hex8/Tet4/Wed6 mesh: the element loop of
- * GeometryInteriorAlg::impl_compute_edge_area_vector
+/* Edge area vectors of a mixed Hex8/Tet4/Wed6/Pyr5 mesh: the element loop
+ * of GeometryInteriorAlg::impl_compute_edge_area_vector
  * (src/ngp_algorithms/GeometryInteriorAlg.C:163-225) with the SCS of each
  * topology; edges sorted by (node[0], node[1]) with node[0] < node[1].
- * type: 0 Hex8, 1 Tet4, 2 Wed6; conn [nelem][8] (unused slots ignored).
+ * type: 0 Hex8, 1 Tet4, 2 Wed6, 3 Pyr5; conn [nelem][8] (unused slots
+ * ignored).
  * Returns -1 if an element edge is not in the list. */
 int orc_edge_area_vec_mixed(int64_t nelem, const int32_t* type,
                             const int32_t* conn, const double* coords,
@@ -208,9 +264,11 @@ int orc_edge_area_vec_mixed(int64_t nelem, const int32_t* type,
   memset(areav, 0, sizeof(double) * 3 * (size_t)nedge);
   for (int64_t el = 0; el < nelem; ++el) {
     const int t = type[el];
-    const int npe = t == 0 ? 8 : (t == 1 ? 4 : 6);
-    const int nscs = t == 0 ? 12 : (t == 1 ? 6 : 9);
-    const int* lr = t == 0 ? orc_hex_lrscv : (t == 1 ? orc_tet_lrscv : orc_wed_lrscv);
+    const int npe = t == 0 ? 8 : (t == 1 ? 4 : (t == 2 ? 6 : 5));
+    const int nscs = t == 0 ? 12 : (t == 1 ? 6 : (t == 2 ? 9 : 12));
+    const int* lr = t == 0 ? orc_hex_lrscv
+                           : (t == 1 ? orc_tet_lrscv
+                                     : (t == 2 ? orc_wed_lrscv : orc_pyr_lrscv));
     double c[24], a[36];
     for (int n = 0; n < npe; ++n)
       for (int d = 0; d < 3; ++d) c[3 * n + d] = coords[3 * conn[8 * el + n] + d];
@@ -218,8 +276,10 @@ int orc_edge_area_vec_mixed(int64_t nelem, const int32_t* type,
       orc_hex_scs_areav(c, a);
     else if (t == 1)
       orc_tet_scs_areav(c, a);
-    else
+    else if (t == 2)
       orc_wed_scs_areav(c, a);
+    else
+      orc_pyr_scs_areav(c, a);
     for (int ip = 0; ip < nscs; ++ip) {
       const int32_t nl = conn[8 * el + lr[2 * ip]], nr = conn[8 * el + lr[2 * ip + 1]];
       const int64_t e = find_edge(edges, nedge, nl < nr ? nl : nr, nl < nr ? nr : nl);
@@ -321,11 +381,109 @@ void orc_wed_scv_volume(const double c[18], double vol[6])
   }
 }
 
-/* Dual nodal volumes of a mixed Hex8/Tet4/Wed6 mesh: the element loop of
- * GeometryInteriorAlg::impl_compute_dual_nodal_volume
+/* bhex_volume_grandy (Hex8GeometryFunctions.h:163-254): Grandy's volume
+ * of a hexahedron whose top face (4, 5, 6, 7) is two triangles split along
+ * 5-7 (its "centroid" is the midpoint of that diagonal). */
+static double bhex_volume_grandy(const double h[8][3])
+{
+  static const int tri[24][3] = {
+    {0, 8, 1},  {8, 2, 1},  {3, 2, 8},  {3, 8, 0},  {6, 9, 5},  {7, 9, 6},
+    {4, 9, 7},  {4, 5, 9},  {10, 0, 1}, {5, 10, 1}, {4, 10, 5}, {4, 0, 10},
+    {7, 6, 11}, {6, 2, 11}, {2, 3, 11}, {3, 7, 11}, {6, 12, 2}, {5, 12, 6},
+    {5, 1, 12}, {1, 2, 12}, {0, 4, 13}, {4, 7, 13}, {7, 3, 13}, {3, 0, 13}};
+  double cv[14][3];
+  for (int n = 0; n < 8; ++n)
+    for (int d = 0; d < 3; ++d) cv[n][d] = h[n][d];
+  for (int d = 0; d < 3; ++d) {
+    cv[8][d] = 0.25 * (h[0][d] + h[1][d] + h[2][d] + h[3][d]);
+    cv[9][d] = 0.5 * (h[5][d] + h[7][d]);
+    cv[10][d] = 0.25 * (h[0][d] + h[1][d] + h[5][d] + h[4][d]);
+    cv[11][d] = 0.25 * (h[3][d] + h[2][d] + h[6][d] + h[7][d]);
+    cv[12][d] = 0.25 * (h[1][d] + h[2][d] + h[6][d] + h[5][d]);
+    cv[13][d] = 0.25 * (h[0][d] + h[3][d] + h[7][d] + h[4][d]);
+  }
+  double vol = 0.0;
+  for (int k = 0; k < 24; ++k) {
+    const int p = tri[k][0], q = tri[k][1], r = tri[k][2];
+    const double m[3] = {cv[p][0] + cv[q][0] + cv[r][0],
+                         cv[p][1] + cv[q][1] + cv[r][1],
+                         cv[p][2] + cv[q][2] + cv[r][2]};
+    double dx[3];
+    dx[0] = (cv[q][1] - cv[p][1]) * (cv[r][2] - cv[p][2]) -
+            (cv[r][1] - cv[p][1]) * (cv[q][2] - cv[p][2]);
+    dx[1] = (cv[r][0] - cv[p][0]) * (cv[q][2] - cv[p][2]) -
+            (cv[q][0] - cv[p][0]) * (cv[r][2] - cv[p][2]);
+    dx[2] = (cv[q][0] - cv[p][0]) * (cv[r][1] - cv[p][1]) -
+            (cv[r][0] - cv[p][0]) * (cv[q][1] - cv[p][1]);
+    vol += m[0] * dx[0] + m[1] * dx[1] + m[2] * dx[2];
+  }
+  return vol / 18.0;
+}
+
+/* octohedron_volume_by_triangle_facets + polyhedral_volume_by_faces
+ * (Pyr5CVFEM.C:349-433): the apex sub-volume from its 10 vertices, the four
+ * non-planar faces split at the midpoints of (3, 9), (3, 5), (5, 7), (7, 9),
+ * as 24 triangles and the divergence theorem. */
+static double octahedron_volume(const double v10[10][3])
+{
+  static const int tri[24][3] = {
+    {1, 3, 10}, {2, 10, 3}, {2, 9, 10}, {10, 9, 1}, {4, 3, 11}, {3, 1, 11},
+    {11, 1, 5}, {4, 11, 5}, {1, 12, 5}, {1, 7, 12}, {12, 7, 6}, {5, 12, 6},
+    {9, 8, 13}, {13, 8, 7}, {13, 7, 1}, {9, 13, 1}, {4, 5, 0},  {5, 6, 0},
+    {6, 7, 0},  {7, 8, 0},  {0, 8, 9},  {0, 9, 2},  {0, 2, 3},  {0, 3, 4}};
+  double cv[14][3];
+  for (int n = 0; n < 10; ++n)
+    for (int d = 0; d < 3; ++d) cv[n][d] = v10[n][d];
+  for (int d = 0; d < 3; ++d) {
+    cv[10][d] = 0.5 * (v10[3][d] + v10[9][d]);
+    cv[11][d] = 0.5 * (v10[3][d] + v10[5][d]);
+    cv[12][d] = 0.5 * (v10[5][d] + v10[7][d]);
+    cv[13][d] = 0.5 * (v10[7][d] + v10[9][d]);
+  }
+  double vol = 0.0;
+  for (int k = 0; k < 24; ++k) {
+    const int p = tri[k][0], q = tri[k][1], r = tri[k][2];
+    double xf[3];
+    for (int d = 0; d < 3; ++d) xf[d] = cv[p][d] + cv[q][d] + cv[r][d];
+    vol = vol +
+          xf[0] * ((cv[q][1] - cv[p][1]) * (cv[r][2] - cv[p][2]) -
+                   (cv[r][1] - cv[p][1]) * (cv[q][2] - cv[p][2])) -
+          xf[1] * ((cv[q][0] - cv[p][0]) * (cv[r][2] - cv[p][2]) -
+                   (cv[r][0] - cv[p][0]) * (cv[q][2] - cv[p][2])) +
+          xf[2] * ((cv[q][0] - cv[p][0]) * (cv[r][1] - cv[p][1]) -
+                   (cv[r][0] - cv[p][0]) * (cv[q][1] - cv[p][1]));
+  }
+  return vol / 18.0;
+}
+
+/* PyrSCV::determinant_scv (Pyr5CVFEM.C:437-571): one bent hexahedron per
+ * base node, the octahedron about the apex. */
+void orc_pyr_scv_volume(const double c[15], double vol[5])
+{
+  static const int sub[5][10] = {{0, 5, 9, 8, 11, 12, 18, 17, -1, -1},
+                                 {1, 6, 9, 5, 10, 14, 18, 12, -1, -1},
+                                 {2, 7, 9, 6, 13, 16, 18, 14, -1, -1},
+                                 {3, 8, 9, 7, 15, 17, 18, 16, -1, -1},
+                                 {4, 18, 15, 17, 11, 12, 10, 14, 13, 16}};
+  double v[19][3];
+  pyr_points(c, v);
+  for (int s = 0; s < 4; ++s) {
+    double h[8][3];
+    for (int n = 0; n < 8; ++n)
+      for (int d = 0; d < 3; ++d) h[n][d] = v[sub[s][n]][d];
+    vol[s] = bhex_volume_grandy(h);
+  }
+  double o[10][3];
+  for (int n = 0; n < 10; ++n)
+    for (int d = 0; d < 3; ++d) o[n][d] = v[sub[4][n]][d];
+  vol[4] = octahedron_volume(o);
+}
+
+/* Dual nodal volumes of a mixed Hex8/Tet4/Wed6/Pyr5 mesh: the element loop
+ * of GeometryInteriorAlg::impl_compute_dual_nodal_volume
  * (src/ngp_algorithms/GeometryInteriorAlg.C:70-112); sub-volume ip of an
  * element belongs to its local node ip (ipNodeMap is the identity for all
- * three, include/master_element/{Hex8,Tet4,Wed6}CVFEM.h). */
+ * four, include/master_element/{Hex8,Tet4,Wed6,Pyr5}CVFEM.h). */
 void orc_dual_volume_mixed(int64_t nelem, const int32_t* type,
                            const int32_t* conn, const double* coords,
                            int64_t nnode, double* dualvol)
@@ -333,7 +491,7 @@ void orc_dual_volume_mixed(int64_t nelem, const int32_t* type,
   memset(dualvol, 0, sizeof(double) * (size_t)nnode);
   for (int64_t el = 0; el < nelem; ++el) {
     const int t = type[el];
-    const int npe = t == 0 ? 8 : (t == 1 ? 4 : 6);
+    const int npe = t == 0 ? 8 : (t == 1 ? 4 : (t == 2 ? 6 : 5));
     double c[24], v[8];
     for (int n = 0; n < npe; ++n)
       for (int d = 0; d < 3; ++d) c[3 * n + d] = coords[3 * conn[8 * el + n] + d];
@@ -341,8 +499,10 @@ void orc_dual_volume_mixed(int64_t nelem, const int32_t* type,
       orc_hex_scv_volume(c, v);
     else if (t == 1)
       orc_tet_scv_volume(c, v);
-    else
+    else if (t == 2)
       orc_wed_scv_volume(c, v);
+    else
+      orc_pyr_scv_volume(c, v);
     for (int ip = 0; ip < npe; ++ip) dualvol[conn[8 * el + ip]] += v[ip];
   }
 }

@@ -265,8 +265,9 @@ def test_momentum_matches_oracle(nw, ctx, skewed, kw):
 
 @pytest.fixture(scope="module")
 def mixed():
-    """Mixed Hex8/Tet4/Wed6 box (60/30/10 % of the cells), jittered, nodes
-    permuted then RCM-numbered: irregular rows."""
+    """Conforming mixed Hex8/Tet4/Wed6/Pyr5 box (60/30/10 % of the cells cut
+    Hex8/Tet4/Wed6, transition cells between), jittered, nodes permuted then
+    RCM-numbered: irregular rows."""
     return pb.Problem(9, 0, 0, jitter=0.15, seed=6, mixed=(0.6, 0.3))
 
 
@@ -289,6 +290,27 @@ def test_mixed_mesh_scalar_and_momentum_match_oracle(nw, ctx, mixed):
     m = nw.momentum_defaults(alpha=0.5)
     ctx.assemble_momentum(m)
     pb.assert_csr_close(ctx.download(), mixed.oracle_momentum(pb.to_orc_momentum(m)))
+
+
+def test_limiter_of_nearly_flat_slopes_matches_oracle(nw, ctx, skewed):
+    """The van Leer limiter (EdgeKernelUtils.h:18-24) of slopes of opposite
+    sign is exactly 0; with the gradients scaled to 1e-9 the projected
+    slopes nearly cancel the jump, so the limiter's denominator is near its
+    eps and any residue in the numerator (the rounding error of the product,
+    left by a fused multiply-add) would become a limiter of order one."""
+    import copy
+    P = copy.copy(skewed)
+    P.dqdx = skewed.dqdx * 1e-9
+    P.dudx = skewed.dudx * 1e-9
+    P.upload(ctx)
+    ctx.build_graph(1)
+    o = _scal_opts(nw, dict(alpha=0.5, use_limiter=1))
+    ctx.assemble_scalar(o)
+    pb.assert_csr_close(ctx.download(), P.oracle_scalar(pb.to_orc_scalar(o)))
+    ctx.build_graph(3)
+    m = nw.momentum_defaults(alpha=0.5, use_limiter=1)
+    ctx.assemble_momentum(m)
+    pb.assert_csr_close(ctx.download(), P.oracle_momentum(pb.to_orc_momentum(m)))
 
 
 def test_uniform_32_cubed_scalar_config(nw, ctx):
@@ -398,8 +420,14 @@ def test_momentum_with_peclet_equals_the_two_algorithms(nw, ctx, skewed, mixed,
                                           orc.Peclet(pf.form, pf.hybrid, pf.c1, pf.c2),
                                           with_number=True)
     np.testing.assert_allclose(pf1, ref, rtol=1e-13, atol=1e-15)
-    np.testing.assert_allclose(pn1, ref_n, rtol=1e-13, atol=0)
-    np.testing.assert_allclose(pn2, ref_n, rtol=1e-13, atol=0)
+    # u.dx can cancel: bound each edge by the rounding of its terms
+    # sum |dx_d (uL_d + uR_d) / 2| / diffIp
+    L, R = P.edges[:, 0], P.edges[:, 1]
+    dx = P.coords[R] - P.coords[L]
+    diff_ip = 0.5 * (P.visc[L] / P.rho[L] + P.visc[R] / P.rho[R])
+    mag = np.abs(0.5 * dx * (P.vel[L] + P.vel[R])).sum(1) / diff_ip
+    for pn in (pn1, pn2):
+        assert (np.abs(pn - ref_n) <= 1e-13 * np.abs(ref_n) + 1e-15 * mag).all()
 
 
 # ------------------------------------------------------------ row operations

@@ -20,7 +20,8 @@ preconditioning) are checked by their residual, recomputed on the host from
 the downloaded system; the device geometry,
 NodalGradEdgeAlg and MdotEdgeAlg of the 256^3 box against the oracle, and
 the geometry by the tiling of the box and the closure of the area vectors;
-the mixed Hex8/Tet4/Wed6 box's continuity system and device geometry.
+the conforming mixed Hex8/Tet4/Wed6/Pyr5 box's continuity system and device
+geometry.
 
 The oracle runs on up to 16 host threads (its atomic scatter only reorders
 the sums); each test takes 5-30 s on a 16-core host, most of it host mesh
@@ -310,17 +311,17 @@ def test_geometry_and_edge_algorithms_256_cubed_full_size(nw, wl, host_threads):
 
 @pytest.mark.timeout(900)
 def test_mixed_160_cubed_full_size(nw, wl, host_threads):
-    """The mixed Hex8/Tet4/Wed6 box of the benchmark (160^3 cells cut 60/30/10
-    %, +-10 % jitter, RCM-numbered: 4,173,281 nodes, 17,969,515 edges): the
-    continuity system against the oracle over every entry, symmetric with
-    zero row sums and sum(rhs) = 0; the device geometry against the oracle
-    (to 1e-13 of the largest area vector: the two sum an edge's
-    sub-control-surface terms in different element orders).  The box's
-    hex-tet interfaces are not conforming (a hex's quad face meets two tet
-    triangles), so its dual volumes do not tile the box exactly and are not
-    checked for it."""
+    """The conforming mixed box of the benchmark (160^3 cells cut 60/30/10 %
+    into Hex8 / Kuhn Tet4 / Wed6, transition cells of Pyr5 and Tet4 where
+    faces would not match, +-10 % jitter, RCM-numbered: 4,328,769 nodes,
+    19,200,110 edges): the continuity system against the oracle over every
+    entry, symmetric with zero row sums and sum(rhs) = 0; the device
+    geometry against the oracle (to 1e-13 of the largest area vector: the
+    two sum an edge's sub-control-surface terms in different element
+    orders), its dual volumes tiling the unit box and its area vectors
+    closing around every interior node."""
     W = wl.continuity_mixed(160, keep_elements=True)
-    assert (W.n, W.ne) == (4173281, 17969515)
+    assert (W.n, W.ne) == (4328769, 19200110)
     f = W.fields
     x = W.coords
     A = orc.CsrSystem(W.n, W.edges, 1)
@@ -348,3 +349,9 @@ def test_mixed_160_cubed_full_size(nw, wl, host_threads):
         np.testing.assert_allclose(a, ra, rtol=0, atol=1e-13 * np.abs(ra).max())
         np.testing.assert_allclose(v, rv, rtol=1e-12, atol=0)
         assert (v > 0).all()
+        assert abs(v.sum() - 1.0) < 1e-11
+        clo = np.zeros_like(x)
+        np.add.at(clo, W.edges[:, 0], a)
+        np.add.at(clo, W.edges[:, 1], -a)
+        inner = np.all((x > 1e-9) & (x < 1 - 1e-9), axis=1)
+        assert np.abs(clo[inner]).max() < 1e-12 * np.abs(a).max()

@@ -1,7 +1,8 @@
 """GeometryInteriorAlg on the device (nw_mesh_elements +
-nw_compute_geometry): edge area vectors and dual volumes of Hex8 boxes and
-mixed Hex8/Tet4/Wed6 meshes against the oracle and the reference unit-cube
-golds, for any edge order and orientation, and after the nodes move."""
+nw_compute_geometry): edge area vectors and dual volumes of Hex8 boxes,
+single pyramids and conforming mixed Hex8/Tet4/Wed6/Pyr5 meshes against the
+oracle, the reference unit-cube golds and the reference pyramid's closed
+forms, for any edge order and orientation, and after the nodes move."""
 import json
 import os
 
@@ -67,6 +68,32 @@ def test_geometry_hex_box_matches_oracle(nw, ctx, shape):
     np.testing.assert_allclose(v, nw.hex_dual_volume(conn, coords), rtol=1e-14, atol=0)
 
 
+def test_geometry_pyramids_match_closed_forms_and_oracle(nw, ctx):
+    """The reference pyramid (UnitTestUtils.C:226-235) and a distorted one
+    sharing its base: dual volumes 32/135 at the base nodes and 52/135 at
+    the apex of the first, each surface pair of an apex edge summed onto
+    that edge, all against the oracle."""
+    ref = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0], [0, 0, 1],
+                    [0.3, -0.2, -1.1]], float)
+    conn = np.full((2, 8), -1, np.int32)
+    conn[0, :5] = [0, 1, 2, 3, 4]
+    conn[1, :5] = [0, 3, 2, 1, 5]  # below the base: base reversed
+    t = np.array([3, 3], np.int32)
+    edges = nw.mixed_edges(t, conn, 6)
+    assert edges.shape[0] == 12
+    a, v = _device_geometry(nw, ctx, ref, t, conn, edges)
+    ra = orc.edge_area_vec_mixed(t, conn, ref, edges)
+    rv = orc.dual_volume_mixed(t, conn, ref)
+    np.testing.assert_allclose(a, ra, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(v, rv, rtol=1e-14, atol=0)
+    own = orc.pyr_scv_volume(ref[:5])
+    np.testing.assert_allclose(own, [32 / 135] * 4 + [52 / 135], rtol=1e-14)
+    np.testing.assert_allclose(v[4], 52 / 135, rtol=1e-14)
+    np.testing.assert_allclose(v[5], orc.pyr_scv_volume(ref[[0, 3, 2, 1, 5]])[4],
+                               rtol=1e-14)
+    assert abs(v.sum() - (4 / 3 + 4 * 1.1 / 3)) < 1e-13
+
+
 @pytest.mark.parametrize("mix", [(0.6, 0.3), (0.0, 1.0), (0.0, 0.0)])
 def test_geometry_mixed_matches_oracle(nw, ctx, mix):
     coords, t, conn = nw.mixed_box(7, mix[0], mix[1], 0.1, seed=5)
@@ -128,4 +155,4 @@ def test_geometry_errors(nw, ctx):
     with pytest.raises(RuntimeError):
         ctx.mesh_elements(None, bad)
     with pytest.raises(RuntimeError):
-        ctx.mesh_elements(np.full(conn.shape[0], 3, np.int32), conn)
+        ctx.mesh_elements(np.full(conn.shape[0], 4, np.int32), conn)  # no type 4
