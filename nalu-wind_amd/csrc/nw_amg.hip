@@ -67,8 +67,9 @@ struct DevMat {
 
 // Rows up to this many entries on average are applied in sliced ELL form
 // (measured at 256^3: the 7-entry fine matrix and the prolongators gain
-// 1.5-1.9x over CSR; the 27-60-entry coarse operators lose 20 %).
-constexpr double kSellMaxAvgRow = 12.0;
+// 1.5-1.9x over CSR, and the 20-24-entry operators of levels 1-2 cut the
+// whole solve by 5-10 %; the 36-200-entry coarser operators lose).
+constexpr double kSellMaxAvgRow = 25.0;
 
 template <typename T>
 static void mat_spmv(const DevMat<T>& M, const T* x, T* y, T alpha,
@@ -102,17 +103,38 @@ static void mat_resid(const DevMat<T>& M, const T* x, const T* b, T* r,
     NW_ROW_LAUNCH(k_resid, T, M.G, M.n, st, M.rp, M.ci, M.v, x, b, r, M.n);
 }
 
+// zo = z + d after the step, or with add zo += z + d
 template <typename T>
 static void mat_cheb_step(const DevMat<T>& M, const T* z, const T* v,
-                          const T* dinv, T* d, T* zo, T c1, T c2,
+                          const T* dinv, T* d, T* zo, T c1, T c2, bool add,
                           hipStream_t st)
 {
-  if (M.sp)
+  if (M.sp && add)
+    k_sell_cheb_step<T, true><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(
+      M.sp, M.ci, M.v, z, v, dinv, d, zo, M.n, c1, c2);
+  else if (M.sp)
     k_sell_cheb_step<T><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(
       M.sp, M.ci, M.v, z, v, dinv, d, zo, M.n, c1, c2);
+  else if (add)
+    NW_ROW_LAUNCH_ADD(k_cheb_step, T, M.G, M.n, st, M.rp, M.ci, M.v, z, v,
+                      dinv, d, zo, M.n, c1, c2);
   else
     NW_ROW_LAUNCH(k_cheb_step, T, M.G, M.n, st, M.rp, M.ci, M.v, z, v, dinv,
                   d, zo, M.n, c1, c2);
+}
+
+// r = b - A x, then d = z = dinv r inv_theta
+template <typename T>
+static void mat_resid_cheb_init(const DevMat<T>& M, const T* x, const T* b,
+                                T* r, const T* dinv, T* d, T* z, T inv_theta,
+                                hipStream_t st)
+{
+  if (M.sp)
+    k_sell_resid_cheb_init<T><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(
+      M.sp, M.ci, M.v, x, b, r, dinv, d, z, M.n, inv_theta);
+  else
+    NW_ROW_LAUNCH(k_resid_cheb_init, T, M.G, M.n, st, M.rp, M.ci, M.v, x, b,
+                  r, dinv, d, z, M.n, inv_theta);
 }
 
 template <typename T>
@@ -310,7 +332,7 @@ int amg_setup(Ctx* c, AmgDevice** out)
   NW_HIP(hipStreamSynchronize(c->stream));
   amg::Options o;
   amg::Hierarchy h;
-  const int rc = amg::build(A, o, h);
+  const int rc = amg::build(std::move(A), o, h);
   NW_REQUIRE(rc == 0, NW_E_ARG, "AMG setup failed (%d): coarse level too "
              "large or singular", rc);
   AmgDevice* a = new AmgDevice();
@@ -375,29 +397,53 @@ static void level_resid(DevLevel<T>& L, const T* x, const T* b, hipStream_t st)
   mat_resid<T>(L.A, x, b, L.r, st);
 }
 
-// z = p(D^-1 A) D^-1 b on level L (zero initial guess, degree k)
+// Chebyshev interval centre theta = (lmax + lmin) / 2 of level L
+static double cheb_theta(const AmgDevice* a, double lmax_est)
+{
+  const double lmax = 1.1 * lmax_est, lmin = lmax / a->cheb_ratio;
+  return 0.5 * (lmax + lmin);
+}
+
+// z = p(D^-1 A) D^-1 b on level L (zero initial guess, degree k), written
+// to out, or with add added to out.  With init_done the first term
+// d = z = D^-1 b / theta is already in L.cd and L.cz (computed with the
+// residual b, mat_resid_cheb_init).
 template <typename T>
-static int cheb0(AmgDevice* a, DevLevel<T>& L, const T* b, T** zout,
-                 hipStream_t st)
+static int cheb0(AmgDevice* a, DevLevel<T>& L, const T* b, T* out, bool add,
+                 bool init_done, hipStream_t st)
 {
   const double lmax = 1.1 * L.lmax, lmin = lmax / a->cheb_ratio;
-  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double theta = cheb_theta(a, L.lmax), delta = 0.5 * (lmax - lmin);
   const double sigma = theta / delta;
   double rho = 1.0 / sigma;
-  k_cheb_init<T><<<grid_for(L.n, kBlock), kBlock, 0, st>>>(b, L.dinv, L.cd, L.cz,
-                                                          L.n, (T)(1.0 / theta));
-  NW_LAUNCH_CHECK();
+  const int k = a->cheb_degree;
+  if (!init_done) {
+    T* z0 = (k == 1 && !add) ? out : L.cz;
+    k_cheb_init<T><<<grid_for(L.n, kBlock), kBlock, 0, st>>>(
+      b, L.dinv, L.cd, z0, L.n, (T)(1.0 / theta));
+    NW_LAUNCH_CHECK();
+  }
+  NW_REQUIRE(k > 1 || !(add == false && init_done), NW_E_STATE,
+             "degree-1 smoother with a fused start");
+  if (k == 1) {
+    if (add) {
+      k_axpy_t<T><<<grid_for(L.n, kBlock), kBlock, 0, st>>>(out, L.cz, L.n);
+      NW_LAUNCH_CHECK();
+    }
+    return NW_OK;
+  }
   T* z = L.cz;
   T* z2 = L.cz2;
-  for (int it = 1; it < a->cheb_degree; ++it) {
+  for (int it = 1; it < k; ++it) {
     const double rho_new = 1.0 / (2.0 * sigma - rho);
-    mat_cheb_step<T>(L.A, z, b, L.dinv, L.cd, z2, (T)(rho_new * rho),
-                     (T)(2.0 * rho_new / delta), st);
+    const bool last = it == k - 1;
+    mat_cheb_step<T>(L.A, z, b, L.dinv, L.cd, last ? out : z2,
+                     (T)(rho_new * rho), (T)(2.0 * rho_new / delta),
+                     last && add, st);
     NW_LAUNCH_CHECK();
     std::swap(z, z2);
     rho = rho_new;
   }
-  *zout = z;
   return NW_OK;
 }
 
@@ -416,10 +462,8 @@ static int vcycle(AmgDevice* a, DevHier<T>& H, int l, const T* b, T* x,
     return NW_OK;
   }
   DevLevel<T>& C = H.lv[l + 1];
-  T* z = nullptr;
-  int rc = cheb0<T>(a, L, b, &z, st);  // pre-smooth from zero
+  int rc = cheb0<T>(a, L, b, x, false, false, st);  // pre-smooth from zero
   if (rc) return rc;
-  NW_HIP(hipMemcpyAsync(x, z, sizeof(T) * L.n, hipMemcpyDeviceToDevice, st));
   level_resid<T>(L, x, b, st);
   NW_LAUNCH_CHECK();
   mat_spmv<T>(L.R, L.r, C.b, (T)1, st);
@@ -428,13 +472,11 @@ static int vcycle(AmgDevice* a, DevHier<T>& H, int l, const T* b, T* x,
   if (rc) return rc;
   mat_spmv_add<T>(L.P, C.x, x, st);
   NW_LAUNCH_CHECK();
-  level_resid<T>(L, x, b, st);
+  // post-smooth the correction of the new residual, added to x
+  mat_resid_cheb_init<T>(L.A, x, b, L.r, L.dinv, L.cd, L.cz,
+                         (T)(1.0 / cheb_theta(a, L.lmax)), st);
   NW_LAUNCH_CHECK();
-  rc = cheb0<T>(a, L, L.r, &z, st);  // post-smooth the correction
-  if (rc) return rc;
-  k_axpy_t<T><<<grid_for(L.n, kBlock), kBlock, 0, st>>>(x, z, L.n);
-  NW_LAUNCH_CHECK();
-  return NW_OK;
+  return cheb0<T>(a, L, L.r, x, true, true, st);
 }
 
 template <typename T>

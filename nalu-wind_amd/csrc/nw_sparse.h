@@ -78,8 +78,9 @@ __global__ void k_cheb_init(const T* __restrict__ v, const T* __restrict__ dinv,
   z[i] = di;
 }
 
-// Chebyshev step: r = v - A z; d = c1 d + c2 dinv r; zo = z + d.
-template <typename T, int G>
+// Chebyshev step: r = v - A z; d = c1 d + c2 dinv r; zo = z + d (with ADD,
+// zo += z + d: the last step of a smoother that corrects zo).
+template <typename T, int G, bool ADD = false>
 __global__ __launch_bounds__(kBlock) void k_cheb_step(
   const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
   const T* __restrict__ va, const T* __restrict__ z, const T* __restrict__ v,
@@ -94,7 +95,10 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(
   if (row < n && g == 0) {
     const T dn = c1 * d[row] + c2 * dinv[row] * (v[row] - s);
     d[row] = dn;
-    zo[row] = z[row] + dn;
+    if (ADD)
+      zo[row] += z[row] + dn;
+    else
+      zo[row] = z[row] + dn;
   }
 }
 
@@ -125,6 +129,29 @@ __global__ __launch_bounds__(kBlock) void k_resid(
   T s = row < n ? csr_dot<T, G>(rp, ci, va, x, row, g) : (T)0;
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   if (row < n && g == 0) r[row] = b[row] - s;
+}
+
+// r = b - A x, and the start of a Chebyshev smoother on r:
+// d = z = dinv r inv_theta (k_resid followed by k_cheb_init).
+template <typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_resid_cheb_init(
+  const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+  const T* __restrict__ va, const T* __restrict__ x, const T* __restrict__ b,
+  T* __restrict__ r, const T* __restrict__ dinv, T* __restrict__ d,
+  T* __restrict__ z, int n, T inv_theta)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = (int)(t / G);
+  const int g = threadIdx.x & (G - 1);
+  T s = row < n ? csr_dot<T, G>(rp, ci, va, x, row, g) : (T)0;
+  for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+  if (row < n && g == 0) {
+    const T ri = b[row] - s;
+    r[row] = ri;
+    const T di = dinv[row] * ri * inv_theta;
+    d[row] = di;
+    z[row] = di;
+  }
 }
 
 // ------------------------------------------------------------ sliced ELL
@@ -181,8 +208,9 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmv_add(
   y[row] += sell_dot(sp, ci, va, x, row);
 }
 
-// Chebyshev step: r = v - A z; d = c1 d + c2 dinv r; zo = z + d.
-template <typename T>
+// Chebyshev step: r = v - A z; d = c1 d + c2 dinv r; zo = z + d (with ADD,
+// zo += z + d).
+template <typename T, bool ADD = false>
 __global__ __launch_bounds__(kBlock) void k_sell_cheb_step(
   const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
   const T* __restrict__ va, const T* __restrict__ z, const T* __restrict__ v,
@@ -194,7 +222,10 @@ __global__ __launch_bounds__(kBlock) void k_sell_cheb_step(
   const T s = sell_dot(sp, ci, va, z, row);
   const T dn = c1 * d[row] + c2 * dinv[row] * (v[row] - s);
   d[row] = dn;
-  zo[row] = z[row] + dn;
+  if (ADD)
+    zo[row] += z[row] + dn;
+  else
+    zo[row] = z[row] + dn;
 }
 
 // r = b - A x
@@ -207,6 +238,23 @@ __global__ __launch_bounds__(kBlock) void k_sell_resid(
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
   r[row] = b[row] - sell_dot(sp, ci, va, x, row);
+}
+
+// r = b - A x and d = z = dinv r inv_theta (k_sell_resid + k_cheb_init)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sell_resid_cheb_init(
+  const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
+  const T* __restrict__ va, const T* __restrict__ x, const T* __restrict__ b,
+  T* __restrict__ r, const T* __restrict__ dinv, T* __restrict__ d,
+  T* __restrict__ z, int n, T inv_theta)
+{
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const T ri = b[row] - sell_dot(sp, ci, va, x, row);
+  r[row] = ri;
+  const T di = dinv[row] * ri * inv_theta;
+  d[row] = di;
+  z[row] = di;
 }
 
 // out[p] = T(a[slot[p]] * (colscale ? colscale[ci[p]] : 1)), 0 for padding
@@ -228,6 +276,31 @@ int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
                hipStream_t st);
 
 // launch helper: dispatch a G-templated row kernel by lanes-per-row
+// (kernel<T, G>; NW_ROW_LAUNCH_ADD: kernel<T, G, true>)
+#define NW_ROW_LAUNCH_ADD(kernel, T, G, n, stream, ...)                      \
+  do {                                                                       \
+    const unsigned nw_blocks_ = grid_for((int64_t)(n) * (G), kBlock);        \
+    switch (G) {                                                             \
+    case 1:                                                                  \
+      kernel<T, 1, true><<<nw_blocks_, kBlock, 0, stream>>>(__VA_ARGS__);    \
+      break;                                                                 \
+    case 2:                                                                  \
+      kernel<T, 2, true><<<nw_blocks_, kBlock, 0, stream>>>(__VA_ARGS__);    \
+      break;                                                                 \
+    case 4:                                                                  \
+      kernel<T, 4, true><<<nw_blocks_, kBlock, 0, stream>>>(__VA_ARGS__);    \
+      break;                                                                 \
+    case 8:                                                                  \
+      kernel<T, 8, true><<<nw_blocks_, kBlock, 0, stream>>>(__VA_ARGS__);    \
+      break;                                                                 \
+    case 16:                                                                 \
+      kernel<T, 16, true><<<nw_blocks_, kBlock, 0, stream>>>(__VA_ARGS__);   \
+      break;                                                                 \
+    default:                                                                 \
+      kernel<T, 32, true><<<nw_blocks_, kBlock, 0, stream>>>(__VA_ARGS__);   \
+    }                                                                        \
+  } while (0)
+
 #define NW_ROW_LAUNCH(kernel, T, G, n, stream, ...)                          \
   do {                                                                       \
     const unsigned nw_blocks_ = grid_for((int64_t)(n) * (G), kBlock);        \

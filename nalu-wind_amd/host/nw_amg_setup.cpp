@@ -28,20 +28,23 @@ namespace amg {
 
 namespace {
 
-void spmv(const Csr& A, const std::vector<double>& x, std::vector<double>& y)
-{
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < A.nrows; ++i) {
-    double s = 0.0;
-    for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) s += A.v[k] * x[A.ci[k]];
-    y[i] = s;
-  }
-}
+// Sums run over fixed chunks of kChunk elements in parallel and then over
+// the chunks in order, so a reduction gives the same bits for any thread
+// count.
+constexpr int64_t kChunk = 1 << 14;
 
 double dot(const std::vector<double>& a, const std::vector<double>& b)
 {
+  const int64_t n = (int64_t)a.size(), nc = (n + kChunk - 1) / kChunk;
+  std::vector<double> part(nc, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < nc; ++c) {
+    double s = 0.0;
+    for (int64_t i = c * kChunk; i < std::min(n, (c + 1) * kChunk); ++i) s += a[i] * b[i];
+    part[c] = s;
+  }
   double s = 0.0;
-  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  for (double v : part) s += v;
   return s;
 }
 
@@ -61,13 +64,29 @@ double power_lmax(const Csr& A, const std::vector<double>& dinv, int iters)
   for (int it = 0; it < iters; ++it) {
     const double nx = std::sqrt(dot(x, x));
     if (!(nx > 0)) break;
-    for (auto& v : x) v /= nx;
-    spmv(A, x, y);
-    for (int64_t i = 0; i < n; ++i) y[i] *= dinv[i];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) x[i] /= nx;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {  // y = D^-1 A x
+      double s = 0.0;
+      for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) s += A.v[k] * x[A.ci[k]];
+      y[i] = s * dinv[i];
+    }
     lam = dot(x, y);
     x.swap(y);
   }
   return lam;
+}
+
+// 1 / a_ii per row (0 where the diagonal is 0 or absent)
+std::vector<double> inv_diagonal(const Csr& A)
+{
+  std::vector<double> d(A.nrows, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < A.nrows; ++i)
+    for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+      if (A.ci[k] == i && A.v[k] != 0.0) d[i] = 1.0 / A.v[k];
+  return d;
 }
 
 Csr transpose(const Csr& P)
@@ -194,6 +213,7 @@ void coarsen(const Csr& A, const Options& o, Csr& P)
   const int64_t n = A.nrows;
   std::vector<double> diag(n, 0.0);
   std::vector<char> dirichlet(n, 1);
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i)
     for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
       if (A.ci[k] == i)
@@ -201,33 +221,41 @@ void coarsen(const Csr& A, const Options& o, Csr& P)
       else if (A.v[k] != 0.0)
         dirichlet[i] = 0;
     }
-  // strength and filtered matrix (weak entries lumped to the diagonal)
+  // strength and filtered matrix (weak entries lumped to the diagonal):
+  // rows counted, then filled, in parallel
+  const double th2 = o.drop_tol * o.drop_tol;
+  auto keep = [&](int64_t i, int64_t k) {
+    const int32_t j = A.ci[k];
+    const double a = A.v[k];
+    return j == i || (!dirichlet[i] && !dirichlet[j] &&
+                      a * a > th2 * std::fabs(diag[i] * diag[j]));
+  };
   Csr F;
   F.nrows = F.ncols = n;
   F.rp.assign(n + 1, 0);
-  F.ci.reserve(A.nnz());
-  F.v.reserve(A.nnz());
-  const double th2 = o.drop_tol * o.drop_tol;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t c = 0;
+    for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) c += keep(i, k);
+    F.rp[i + 1] = c;
+  }
+  for (int64_t i = 0; i < n; ++i) F.rp[i + 1] += F.rp[i];
+  F.ci.resize(F.rp[n]);
+  F.v.resize(F.rp[n]);
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     double lump = 0.0;
-    int64_t dpos = -1;
+    int64_t dpos = -1, p = F.rp[i];
     for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-      const int32_t j = A.ci[k];
-      const double a = A.v[k];
-      if (j == i) {
-        dpos = (int64_t)F.ci.size();
-        F.ci.push_back(j);
-        F.v.push_back(a);
-      } else if (!dirichlet[i] && !dirichlet[j] &&
-                 a * a > th2 * std::fabs(diag[i] * diag[j])) {
-        F.ci.push_back(j);
-        F.v.push_back(a);
+      if (keep(i, k)) {
+        if (A.ci[k] == i) dpos = p;
+        F.ci[p] = A.ci[k];
+        F.v[p++] = A.v[k];
       } else {
-        lump += a;
+        lump += A.v[k];
       }
     }
     if (dpos >= 0) F.v[dpos] += lump;
-    F.rp[i + 1] = (int64_t)F.ci.size();
   }
   // aggregation
   std::vector<int32_t> agg(n, -1);
@@ -269,70 +297,66 @@ void coarsen(const Csr& A, const Options& o, Csr& P)
   for (int64_t i = 0; i < n; ++i)
     if (agg[i] >= 0) size[agg[i]]++;
   // omega from lambda_max(D_F^-1 A_F)
-  std::vector<double> dfinv(n, 0.0);
-  for (int64_t i = 0; i < n; ++i)
-    for (int64_t k = F.rp[i]; k < F.rp[i + 1]; ++k)
-      if (F.ci[k] == i && F.v[k] != 0.0) dfinv[i] = 1.0 / F.v[k];
+  const std::vector<double> dfinv = inv_diagonal(F);
   const double lf = power_lmax(F, dfinv, o.power_iters);
   const double omega = (4.0 / 3.0) / (lf > 0 ? lf : 1.0);
-  // P = P0 - omega D_F^-1 A_F P0, one row per node
+  // P = P0 - omega D_F^-1 A_F P0, one row per node: each row's terms (at
+  // most 1 + its F entries) are merged by aggregate into a staging layout
+  // with room for that many, then packed
   P.nrows = n;
   P.ncols = na;
+  std::vector<int64_t> ub(n + 1, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) ub[i + 1] = 1 + F.rp[i + 1] - F.rp[i];
+  for (int64_t i = 0; i < n; ++i) ub[i + 1] += ub[i];
+  std::vector<std::pair<int32_t, double>> stage(ub[n]);
   P.rp.assign(n + 1, 0);
-  std::vector<int64_t> cnt(n, 0);
-  std::vector<std::vector<std::pair<int32_t, double>>> rows(n);
 #pragma omp parallel for schedule(dynamic, 4096)
   for (int64_t i = 0; i < n; ++i) {
-    auto& r = rows[i];
     if (agg[i] < 0) continue;
-    r.push_back({agg[i], 1.0 / std::sqrt((double)size[agg[i]])});
+    auto* r = stage.data() + ub[i];
+    size_t m = 0;
+    r[m++] = {agg[i], 1.0 / std::sqrt((double)size[agg[i]])};
     for (int64_t k = F.rp[i]; k < F.rp[i + 1]; ++k) {
       const int32_t j = F.ci[k];
       if (agg[j] < 0) continue;
-      r.push_back({agg[j], -omega * dfinv[i] * F.v[k] /
-                             std::sqrt((double)size[agg[j]])});
+      r[m++] = {agg[j], -omega * dfinv[i] * F.v[k] / std::sqrt((double)size[agg[j]])};
     }
-    std::sort(r.begin(), r.end(), [](const auto& a, const auto& b) {
-      return a.first < b.first;
-    });
+    std::sort(r, r + m, [](const auto& a, const auto& b) { return a.first < b.first; });
     size_t w = 0;
-    for (size_t q = 0; q < r.size(); ++q) {
+    for (size_t q = 0; q < m; ++q) {
       if (w > 0 && r[w - 1].first == r[q].first)
         r[w - 1].second += r[q].second;
       else
         r[w++] = r[q];
     }
-    r.resize(w);
-    cnt[i] = (int64_t)w;
+    P.rp[i + 1] = (int64_t)w;
   }
-  for (int64_t i = 0; i < n; ++i) P.rp[i + 1] = P.rp[i] + cnt[i];
+  for (int64_t i = 0; i < n; ++i) P.rp[i + 1] += P.rp[i];
   P.ci.resize(P.rp[n]);
   P.v.resize(P.rp[n]);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) {
-    int64_t p = P.rp[i];
-    for (const auto& e : rows[i]) {
-      P.ci[p] = e.first;
-      P.v[p++] = e.second;
+    const auto* r = stage.data() + ub[i];
+    for (int64_t p = P.rp[i]; p < P.rp[i + 1]; ++p) {
+      P.ci[p] = r[p - P.rp[i]].first;
+      P.v[p] = r[p - P.rp[i]].second;
     }
   }
 }
 
 }  // namespace
 
-int build(const Csr& A0, const Options& o, Hierarchy& h)
+int build(Csr A0, const Options& o, Hierarchy& h)
 {
   const auto t0 = std::chrono::steady_clock::now();
   h.levels.clear();
   h.levels.push_back(Level());
-  h.levels[0].A = A0;
+  h.levels[0].A = std::move(A0);
   for (int l = 0;; ++l) {
     Level& L = h.levels[l];
     const int64_t n = L.A.nrows;
-    L.dinv.assign(n, 0.0);
-    for (int64_t i = 0; i < n; ++i)
-      for (int64_t k = L.A.rp[i]; k < L.A.rp[i + 1]; ++k)
-        if (L.A.ci[k] == i && L.A.v[k] != 0.0) L.dinv[i] = 1.0 / L.A.v[k];
+    L.dinv = inv_diagonal(L.A);
     L.lmax = power_lmax(L.A, L.dinv, o.power_iters);
     if (n <= o.coarse_max || l + 1 >= o.max_levels) break;
     Csr P;

@@ -42,8 +42,9 @@ struct Hierarchy {
 };
 
 // Builds the hierarchy from the fine matrix (CSR with sorted columns and a
-// diagonal entry in every row).  Returns 0 or a negative error.
-int build(const Csr& A, const Options& o, Hierarchy& h);
+// diagonal entry in every row), which becomes levels[0].A (pass it with
+// std::move to avoid a copy).  Returns 0 or a negative error.
+int build(Csr A, const Options& o, Hierarchy& h);
 
 }  // namespace amg
 }  // namespace nw
