@@ -201,12 +201,13 @@ int check_system_finite(Ctx* c, int rc, const char* who)
     const int64_t row = std::upper_bound(rp.begin(), rp.end(), (int32_t)slot) -
                         rp.begin() - 1;
     set_error("%s: non-finite matrix entry in the row of node %lld (an input "
-              "field holds NaN or Inf)", who, (long long)row);
+              "field holds NaN or Inf, or an edge has a . dx = 0)", who,
+              (long long)row);
     return NW_E_NONFINITE;
   }
   if (h[1] < (unsigned long long)nr) {
     set_error("%s: non-finite rhs entry at node %lld (an input field holds NaN "
-              "or Inf)", who, (long long)(h[1] / g.ndof));
+              "or Inf, or an edge has a . dx = 0)", who, (long long)(h[1] / g.ndof));
     return NW_E_NONFINITE;
   }
   return NW_OK;

@@ -37,6 +37,10 @@ class Problem:
             self.areav = m.mixed_edge_area_vector(self.types, self.conn,
                                                   self.coords, self.edges)
             self.dualvol = np.ones(self.n)  # not read by the edge kernels
+        self._random_fields(seed)
+
+    def _random_fields(self, seed):
+        m = nw()
         self.ne = self.edges.shape[0]
         rng = np.random.default_rng(seed + 1000)
         n, ne = self.n, self.ne
@@ -53,6 +57,19 @@ class Problem:
         self.dflux = rng.uniform(1e-3, 1e-1, n)
         self.mdot = m.mdot_edge(self.edges, self.areav, self.rho, self.vel)
         self.pecfac = rng.uniform(0.0, 1.0, ne)
+
+    @classmethod
+    def from_graph(cls, coords, edges, areav, seed=0):
+        """Any edge graph (coords [n][3], edges [e][2], areav [e][3]) with the
+        same seeded random fields as the meshes above."""
+        P = cls.__new__(cls)
+        P.coords = np.ascontiguousarray(coords, dtype=np.float64)
+        P.n = P.coords.shape[0]
+        P.edges = np.ascontiguousarray(edges, dtype=np.int32).reshape(-1, 2)
+        P.areav = np.ascontiguousarray(areav, dtype=np.float64).reshape(-1, 3)
+        P.dualvol = np.ones(P.n)
+        P._random_fields(seed)
+        return P
 
     def upload(self, ctx):
         m = nw()
