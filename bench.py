@@ -340,7 +340,7 @@ def pressure_solves_c3(ctx, W, quick, log):
     return res
 
 
-def learned_guess_series(ctx, W, ntrain, neval, epochs, log):
+def learned_guess_series(ctx, W, ntrain, neval, epochs, log, dt_flow=0.02):
     """The learned initial guess on the pressure-projection loop
     (workloads.PressureSeries): steps 1..ntrain solve for the pressure
     increment with the plain FP64 method (from the previous increment) and
@@ -355,12 +355,14 @@ def learned_guess_series(ctx, W, ntrain, neval, epochs, log):
     import workloads as wl
     res = {}
     t0 = time.perf_counter()
-    S = wl.PressureSeries(ctx, W=W, dt_flow=0.02)
+    S = wl.PressureSeries(ctx, W=W, dt_flow=dt_flow)
     res["workload"] = (
         "pressure-projection loop on the headline box: per step the random-Fourier "
-        "velocity at t = 0.02 k as the momentum predictor, continuity assembly with "
+        "velocity at t = %g k as the momentum predictor, continuity assembly with "
         "the current p and dp/dx, Dirichlet dp = 0 on z = 1, solve for dp (tol 1e-8, "
-        "GMRES(50) + SA-AMG), p += dp, MdotEdgeAlg, nodal velocity projection")
+        "GMRES(50) + SA-AMG), p += dp, MdotEdgeAlg, nodal velocity projection"
+        % dt_flow)
+    res["dt_flow"] = dt_flow
     res["nodes"] = W.n
     base = dict(kspace=50, max_iters=2000, tol=1e-8, precond=nw.PREC_AMG,
                 reuse_preconditioner=1)
