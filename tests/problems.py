@@ -20,7 +20,8 @@ def nw():
 class Problem:
     def __init__(self, nx, ny, nz, stretch=1.0, jitter=0.0, seed=0, mixed=None):
         """Hex8 box (nx, ny, nz, stretch, jitter), or with mixed =
-        (f_hex, f_tet) an nx^3 mixed Hex8/Tet4/Wed6 box (RCM-numbered)."""
+        (f_hex, f_tet) an nx^3 conforming mixed Hex8/Tet4/Wed6/Pyr5 box
+        (RCM-numbered)."""
         m = nw()
         if mixed is None:
             self.coords, self.conn = m.hex_box(nx, ny, nz, stretch, jitter, seed)

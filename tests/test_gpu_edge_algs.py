@@ -1,6 +1,6 @@
 """The edge algorithms around the pressure solve (MdotEdgeAlg,
 NodalGradEdgeAlg) through the C ABI against the oracle, on stretched and
-jittered Hex8 boxes and a mixed Hex8/Tet4/Wed6 mesh."""
+jittered Hex8 boxes and a conforming mixed Hex8/Tet4/Wed6/Pyr5 mesh."""
 import numpy as np
 import pytest
 

@@ -152,9 +152,9 @@ def test_amg_beats_jacobi_and_is_reused(nw, ctx):
 
 @pytest.mark.parametrize("mode", [0, 1])
 def test_mixed_mesh_pressure_solve(nw, ctx, mode):
-    """Pressure system of a mixed Hex8/Tet4/Wed6 mesh (RCM-numbered): AMG-
-    preconditioned GMRES, FP64 and FP32-inner/FP64-refined, against the
-    dense solve."""
+    """Pressure system of a conforming mixed Hex8/Tet4/Wed6/Pyr5 mesh
+    (RCM-numbered): AMG-preconditioned GMRES, FP64 and FP32-inner/FP64-
+    refined, against the dense solve."""
     P = pb.Problem(10, 0, 0, jitter=0.1, seed=8, mixed=(0.6, 0.3))
     P.upload(ctx)
     ctx.build_graph(1)
