@@ -43,7 +43,10 @@ namespace {
 
 constexpr int kMaxBasis = 64;  // kspace + 1 <= 64
 constexpr int kTile = 8;       // basis vectors per dot tile
-constexpr int kRowBlocks = 512;
+// row blocks of the fused dot kernels: 2048 keeps 8 waves per CU busy even
+// for a short tile (512 measured 2 % / 4 % slower for the FP64 / FP32
+// solves at 256^3, whose dots were latency-bound; 4096-8192 no better)
+constexpr int kRowBlocks = 2048;
 constexpr int kDotVals = 2 * kTile + 3;  // per tile: Q.y, Q.z, y.y, y.z, z.z
 
 __device__ inline double wave_sum(double v)
