@@ -239,18 +239,6 @@ __global__ void k_diag_inv(const int32_t* __restrict__ diag,
   dinv[i] = 1.0 / d;
 }
 
-// out = T(a_ij * (dinv ? dinv_j : 1)) (column scaling for right Jacobi)
-template <typename T>
-__global__ void k_scale_copy(const int32_t* __restrict__ ci,
-                             const double* __restrict__ va,
-                             const double* __restrict__ dinv,
-                             T* __restrict__ out, int64_t nnz)
-{
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nnz) return;
-  out[k] = (T)(dinv ? va[k] * dinv[ci[k]] : va[k]);
-}
-
 template <typename T>
 __global__ void k_convert(const double* __restrict__ a, T* __restrict__ b,
                           double scale, int64_t n)

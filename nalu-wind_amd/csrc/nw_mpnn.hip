@@ -1015,7 +1015,7 @@ struct Mpnn {
   std::vector<double> am, av;  // Adam moments
   int64_t adam_t = 0;
   float* dP = nullptr;     // device parameters, flat layout
-  float* dPT = nullptr;    // W_in^T, then per layer W_self^T, W_nbr^T
+  float* dPT = nullptr;    // W_in^T (the input layers read it k-major)
   float* w = nullptr;      // [nnz] slot weights
   std::vector<float*> h;   // L+1 activations [n][H]
   std::vector<float*> msg; // messages m_l [n][H] (L when training, else 1)
@@ -1072,19 +1072,10 @@ int upload_params(Mpnn* m)
 {
   const int H = m->H, L = m->L;
   Off o(H, L);
-  std::vector<float> f(m->np), t((size_t)H * kF + 2 * (size_t)L * H * H);
+  std::vector<float> f(m->np), t((size_t)H * kF);
   for (int64_t i = 0; i < m->np; ++i) f[i] = (float)m->P[i];
   for (int j = 0; j < H; ++j)
     for (int k = 0; k < kF; ++k) t[(size_t)k * H + j] = f[o.win + j * kF + k];
-  for (int l = 0; l < L; ++l) {
-    float* ws = t.data() + (size_t)H * kF + 2 * (size_t)l * H * H;
-    float* wn = ws + (size_t)H * H;
-    for (int j = 0; j < H; ++j)
-      for (int k = 0; k < H; ++k) {
-        ws[(size_t)k * H + j] = f[o.ws(l) + (int64_t)j * H + k];
-        wn[(size_t)k * H + j] = f[o.wn(l, H) + (int64_t)j * H + k];
-      }
-  }
   // bfloat16 weights of the fused layers in the MFMA B-operand layout:
   // fragment (nt, ks), lane l, item t holds W^T[k][j] with
   // k = 16 ks + 4 (l / 16) + t over [h, m] (W_self then W_nbr) and
@@ -1558,9 +1549,7 @@ int nw_mpnn_create(nw_ctx* c, int hidden, int layers, uint64_t seed,
   else build_gmap<32>(m);
   int rc = NW_OK;
   if (hipMalloc(&m->dP, sizeof(float) * m->np) != hipSuccess ||
-      hipMalloc(&m->dPT, sizeof(float) * ((size_t)hidden * kF +
-                                          2 * (size_t)layers * hidden * hidden)) !=
-        hipSuccess) {
+      hipMalloc(&m->dPT, sizeof(float) * (size_t)hidden * kF) != hipSuccess) {
     set_error("device allocation of the network parameters failed");
     rc = NW_E_ALLOC;
   }
