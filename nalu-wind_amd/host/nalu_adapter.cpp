@@ -1,7 +1,9 @@
 // nalu_adapter.cpp -- the reference-shaped host classes over the C ABI.
 #include "nalu_adapter.h"
 
+#include <algorithm>
 #include <chrono>
+#include <memory>
 #include <string>
 
 namespace nalu_amd {
@@ -36,6 +38,94 @@ void Realm::set_mesh(int64_t nnode, const std::vector<int32_t>& edge_nodes)
   nnode_ = nnode;
   nedge_ = (int64_t)edge_nodes.size() / 2;
   active_ = nullptr;  // the new mesh has no graph yet
+}
+
+void Realm::create_mesh(const std::string& path)
+{
+  nw_exo* exo = nullptr;
+  check(nw_exo_open(path.c_str(), &exo), "Realm::create_mesh");
+  std::unique_ptr<nw_exo, void (*)(nw_exo*)> hold(exo, nw_exo_close);
+  nw_exo_info info{};
+  check(nw_exo_sizes(exo, &info), "Realm::create_mesh");
+  std::vector<double> coords(3 * info.nnode);
+  std::vector<int32_t> type(info.nelem), conn(8 * info.nelem);
+  check(nw_exo_read_mesh(exo, coords.data(), type.data(), conn.data(), nullptr),
+        "Realm::populate_mesh");
+  int64_t ne = 0;
+  check(nw_mixed_edges(info.nelem, type.data(), conn.data(), info.nnode, nullptr, 0, &ne),
+        "Realm::create_edges");
+  std::vector<int32_t> edges(2 * ne);
+  check(nw_mixed_edges(info.nelem, type.data(), conn.data(), info.nnode, edges.data(), ne,
+                       &ne),
+        "Realm::create_edges");
+  set_mesh(info.nnode, edges);
+  set_field(NW_F_COORDINATES, coords);
+  meshPath_ = path;
+  elemType_ = std::move(type);
+  elemConn_ = std::move(conn);
+}
+
+std::vector<int32_t> Realm::bc_nodes(const std::string& name) const
+{
+  if (meshPath_.empty()) throw std::runtime_error("Realm::bc_nodes: no mesh file");
+  nw_exo* exo = nullptr;
+  check(nw_exo_open(meshPath_.c_str(), &exo), "Realm::bc_nodes");
+  std::unique_ptr<nw_exo, void (*)(nw_exo*)> hold(exo, nw_exo_close);
+  int32_t idx = -1;
+  int64_t count = 0;
+  std::vector<int32_t> nodes;
+  if (nw_exo_find(exo, NW_EXO_NODE_SET, name.c_str(), &idx) == NW_OK) {
+    check(nw_exo_entity(exo, NW_EXO_NODE_SET, idx, nullptr, &count, nullptr, nullptr, 0),
+          "Realm::bc_nodes");
+    nodes.resize(count);
+    check(nw_exo_node_set(exo, idx, nodes.data()), "Realm::bc_nodes");
+    std::sort(nodes.begin(), nodes.end());
+    nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+    return nodes;
+  }
+  check(nw_exo_find(exo, NW_EXO_SIDE_SET, name.c_str(), &idx), "Realm::bc_nodes");
+  check(nw_exo_entity(exo, NW_EXO_SIDE_SET, idx, nullptr, &count, nullptr, nullptr, 0),
+        "Realm::bc_nodes");
+  std::vector<int32_t> elem(count), side(count);
+  check(nw_exo_side_set(exo, idx, elem.data(), side.data()), "Realm::bc_nodes");
+  int64_t nn = 0;
+  check(nw_sides_to_nodes(nelem(), elemType_.data(), elemConn_.data(), nnode_, count,
+                          elem.data(), side.data(), nullptr, &nn),
+        "Realm::bc_nodes");
+  nodes.resize(nn);
+  check(nw_sides_to_nodes(nelem(), elemType_.data(), elemConn_.data(), nnode_, count,
+                          elem.data(), side.data(), nodes.data(), &nn),
+        "Realm::bc_nodes");
+  return nodes;
+}
+
+void Realm::write_output(const std::string& path,
+                         const std::vector<std::pair<std::string, int>>& fields,
+                         double time) const
+{
+  std::vector<std::string> names;
+  std::vector<double> values;
+  for (const auto& f : fields) {
+    const int id = f.second, nc = nw_field_components(id);
+    if (nc <= 0 || id == NW_E_AREA_VECTOR || id == NW_E_MASS_FLOW_RATE ||
+        id == NW_E_PECLET_FACTOR || id == NW_E_PECLET_NUMBER)
+      throw std::runtime_error("Realm::write_output: " + f.first + " is not a node field");
+    const std::vector<double> v = get_field(id);
+    for (int c = 0; c < nc; ++c) {
+      std::string name = f.first;
+      if (nc == 3) name += std::string("_") + "xyz"[c];
+      if (nc == 9) name += std::string("_") + "xyz"[c / 3] + "xyz"[c % 3];
+      names.push_back(name);
+      for (int64_t n = 0; n < nnode_; ++n) values.push_back(v[n * nc + c]);
+    }
+  }
+  std::vector<const char*> cn;
+  for (const auto& n : names) cn.push_back(n.c_str());
+  const std::vector<double> coords = get_field(NW_F_COORDINATES);
+  check(nw_exo_write(path.c_str(), "nalu-wind edge realm", nnode_, coords.data(), nelem(),
+                     elemType_.data(), elemConn_.data(), nullptr, nullptr,
+                     (int32_t)names.size(), cn.data(), values.data(), time, nullptr),
+        "Realm::write_output");
 }
 
 void Realm::set_field(int field, const std::vector<double>& data)
@@ -299,6 +389,12 @@ GeometryInteriorAlg::GeometryInteriorAlg(Realm& realm, int64_t nelem,
   : realm_(realm)
 {
   check(nw_mesh_elements(realm_.ctx(), nelem, type, conn), "GeometryInteriorAlg");
+}
+
+GeometryInteriorAlg::GeometryInteriorAlg(Realm& realm)
+  : GeometryInteriorAlg(realm, realm.nelem(), realm.elemTypes().data(),
+                        realm.elemConn().data())
+{
 }
 
 void GeometryInteriorAlg::execute()

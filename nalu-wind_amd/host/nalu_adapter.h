@@ -20,6 +20,11 @@
 //   CoeffApplier (include/LinearSystem.h:44-74)         CoeffApplier
 //   EquationSystem::assemble_and_solve                  EquationSystem::
 //     (src/EquationSystem.C:301-350)                    assemble_and_solve
+//   Realm::create_mesh / populate_mesh / create_edges   Realm::create_mesh
+//     (src/Realm.C:2048-2094, 450-466), the mesh file   (Exodus II, netCDF
+//     of a boundary condition's target_name             classic), bc_nodes
+//   Realm::create_output_mesh / output              Realm::write_output
+//     (src/Realm.C:2100-2215)
 //
 // Errors from the library become std::runtime_error, as the reference's
 // host code throws; nothing is thrown across the C ABI itself.
@@ -33,6 +38,7 @@
 #include <vector>
 
 #include "nalu_amd.h"
+#include "nalu_amd_mesh.h"
 
 namespace nalu_amd {
 
@@ -95,6 +101,23 @@ class Realm {
   Realm& operator=(const Realm&) = delete;
 
   void set_mesh(int64_t nnode, const std::vector<int32_t>& edge_nodes);
+  // create_mesh + populate_mesh + create_edges for an Exodus II file
+  // (netCDF classic or 64-bit-offset form): the nodes, the edges of its
+  // Hex8/Tet4/Wed6/Pyr5 elements and the coordinates go to the context;
+  // the elements stay with the realm for GeometryInteriorAlg(realm).
+  void create_mesh(const std::string& path);
+  // The nodes of the node set, or else of the side set, named `name` in
+  // the mesh file (a boundary condition's target_name), sorted and unique.
+  std::vector<int32_t> bc_nodes(const std::string& name) const;
+  // The mesh and node fields at `time` to an Exodus II file: `fields`
+  // pairs an output name with a context field; a 3-component field is
+  // written as name_x, name_y, name_z.
+  void write_output(const std::string& path,
+                    const std::vector<std::pair<std::string, int>>& fields,
+                    double time) const;
+  int64_t nelem() const { return (int64_t)elemType_.size(); }
+  const std::vector<int32_t>& elemTypes() const { return elemType_; }
+  const std::vector<int32_t>& elemConn() const { return elemConn_; }
   void set_field(int field, const std::vector<double>& data);
   std::vector<double> get_field(int field) const;
   int64_t nnode() const { return nnode_; }
@@ -111,6 +134,8 @@ class Realm {
   nw_ctx* ctx_ = nullptr;
   int64_t nnode_ = 0, nedge_ = 0;
   const LinearSystem* active_ = nullptr;
+  std::string meshPath_;
+  std::vector<int32_t> elemType_, elemConn_;  // [nelem], [nelem][8]
 };
 
 // LinearSystem::CoeffApplier: additive block scatter + row reset.
@@ -290,6 +315,8 @@ class GeometryInteriorAlg {
  public:
   GeometryInteriorAlg(Realm& realm, int64_t nelem, const int32_t* type,
                       const int32_t* conn);
+  // the elements of the realm's mesh file (Realm::create_mesh)
+  explicit GeometryInteriorAlg(Realm& realm);
   void execute();
 
  private:

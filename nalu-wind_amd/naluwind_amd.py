@@ -104,6 +104,21 @@ class MpnnTrainCfg(C.Structure):
                 ("eps", C.c_double), ("seed", C.c_uint64), ("holdout", C.c_int)]
 
 
+class ExoInfo(C.Structure):
+    _fields_ = [("nnode", C.c_int64), ("nelem", C.c_int64), ("nblock", C.c_int32),
+                ("nnodeset", C.c_int32), ("nsideset", C.c_int32),
+                ("nnodvar", C.c_int32), ("ntime", C.c_int32), ("format", C.c_int32)]
+
+
+class ExoSets(C.Structure):
+    _fields_ = [("count", C.c_int32), ("ids", C.c_void_p), ("names", C.c_void_p),
+                ("offsets", C.c_void_p), ("entries", C.c_void_p),
+                ("sides", C.c_void_p)]
+
+
+EXO_BLOCK, EXO_NODE_SET, EXO_SIDE_SET, EXO_NODAL_VAR = range(4)
+
+
 def mpnn_train_cfg(**kw):
     o = MpnnTrainCfg(10, 4, 1e-3, 0.9, 0.999, 1e-8, 0, 0)
     for k, v in kw.items():
@@ -226,6 +241,24 @@ _sig = {
     "nw_nodal_grad_edge": ([C.c_int64, C.c_int64, _i32p, _f64p, _f64p, C.c_int,
                             _f64p, _f64p], C.c_int),
     "nw_mdot_edge": ([C.c_int64, _i32p, _f64p, _f64p, _f64p, _f64p], C.c_int),
+    "nw_exo_open": ([C.c_char_p, C.POINTER(_P)], C.c_int),
+    "nw_exo_close": ([_P], None),
+    "nw_exo_sizes": ([_P, C.POINTER(ExoInfo)], C.c_int),
+    "nw_exo_entity": ([_P, C.c_int, C.c_int32, C.POINTER(C.c_int32),
+                       C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_char_p,
+                       C.c_int32], C.c_int),
+    "nw_exo_find": ([_P, C.c_int, C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
+    "nw_exo_read_mesh": ([_P, _f64p, _i32p, _i32p, _P], C.c_int),
+    "nw_exo_node_set": ([_P, C.c_int32, _i32p], C.c_int),
+    "nw_exo_side_set": ([_P, C.c_int32, _i32p, _i32p], C.c_int),
+    "nw_exo_read_nodal_var": ([_P, C.c_int32, C.c_int32, _f64p,
+                               C.POINTER(C.c_double)], C.c_int),
+    "nw_side_nodes": ([C.c_int32, C.c_int32, _i32p], C.c_int),
+    "nw_sides_to_nodes": ([C.c_int64, _i32p, _i32p, C.c_int64, C.c_int64, _i32p,
+                           _i32p, _P, C.POINTER(C.c_int64)], C.c_int),
+    "nw_exo_write": ([C.c_char_p, C.c_char_p, C.c_int64, _f64p, C.c_int64, _i32p,
+                      _i32p, C.POINTER(ExoSets), C.POINTER(ExoSets), C.c_int32,
+                      _P, _P, C.c_double, _P], C.c_int),
 }
 for _name, (_a, _r) in _sig.items():
     _f = getattr(lib, _name)
@@ -339,6 +372,154 @@ def mdot_edge(edges, areav, rho, vel):
     check(lib.nw_mdot_edge(edges.shape[0], _i32(edges), _f64(areav), _f64(rho),
                            _f64(vel), out))
     return out
+
+
+# ------------------------------------------------------------ Exodus II I/O
+
+def side_nodes(topo, side):
+    """Element-local nodes of side `side` of topology `topo`, outward by the
+    right-hand rule."""
+    out = np.zeros(4, np.int32)
+    n = lib.nw_side_nodes(topo, side, out)
+    check(min(n, 0))
+    return out[:n]
+
+
+def sides_to_nodes(types, conn, nnode, elem, side):
+    """Sorted unique nodes of a list of element sides (a side set)."""
+    types, conn = _i32(types), _i32(conn)
+    elem, side = _i32(elem), _i32(side)
+    cnt = C.c_int64()
+    check(lib.nw_sides_to_nodes(types.size, types, conn, nnode, elem.size, elem,
+                                side, None, C.byref(cnt)))
+    out = np.zeros(cnt.value, np.int32)
+    check(lib.nw_sides_to_nodes(types.size, types, conn, nnode, elem.size, elem,
+                                side, out.ctypes.data_as(_P), C.byref(cnt)))
+    return out
+
+
+class ExodusMesh:
+    """An Exodus II file (netCDF classic or 64-bit-offset) opened for
+    reading: header on open, arrays on request."""
+
+    def __init__(self, path):
+        self._h = _P()
+        check(lib.nw_exo_open(os.fsencode(path), C.byref(self._h)))
+        info = ExoInfo()
+        check(lib.nw_exo_sizes(self._h, C.byref(info)))
+        self.nnode, self.nelem = info.nnode, info.nelem
+        self.format, self.ntime = info.format, info.ntime
+        self.blocks = [self._entity(EXO_BLOCK, i) for i in range(info.nblock)]
+        self.node_sets = [self._entity(EXO_NODE_SET, i) for i in range(info.nnodeset)]
+        self.side_sets = [self._entity(EXO_SIDE_SET, i) for i in range(info.nsideset)]
+        self.nodal_vars = [self._entity(EXO_NODAL_VAR, i)["name"]
+                           for i in range(info.nnodvar)]
+
+    def _entity(self, kind, i):
+        eid, cnt, topo = C.c_int32(), C.c_int64(), C.c_int32()
+        name = C.create_string_buffer(256)
+        check(lib.nw_exo_entity(self._h, kind, i, C.byref(eid), C.byref(cnt),
+                                C.byref(topo), name, 256))
+        return {"id": eid.value, "count": cnt.value, "topo": topo.value,
+                "name": name.value.decode()}
+
+    def _index(self, kind, key):
+        if isinstance(key, str):
+            idx = C.c_int32()
+            check(lib.nw_exo_find(self._h, kind, key.encode(), C.byref(idx)))
+            return idx.value
+        return int(key)
+
+    def read_mesh(self):
+        """(coords [N][3], type [E], conn [E][8], block [E])"""
+        coords = np.zeros((self.nnode, 3))
+        types = np.zeros(self.nelem, np.int32)
+        conn = np.zeros((self.nelem, 8), np.int32)
+        block = np.zeros(self.nelem, np.int32)
+        check(lib.nw_exo_read_mesh(self._h, coords, types, conn,
+                                   block.ctypes.data_as(_P)))
+        return coords, types, conn, block
+
+    def node_set(self, key):
+        i = self._index(EXO_NODE_SET, key)
+        out = np.zeros(self.node_sets[i]["count"], np.int32)
+        check(lib.nw_exo_node_set(self._h, i, out))
+        return out
+
+    def side_set(self, key):
+        """(elem, side) of side set `key` (name or index)"""
+        i = self._index(EXO_SIDE_SET, key)
+        n = self.side_sets[i]["count"]
+        elem, side = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        check(lib.nw_exo_side_set(self._h, i, elem, side))
+        return elem, side
+
+    def nodal_var(self, key, step=0):
+        """(values [N], time) of nodal variable `key` at time step `step`"""
+        i = self._index(EXO_NODAL_VAR, key)
+        out = np.zeros(self.nnode)
+        t = C.c_double()
+        check(lib.nw_exo_read_nodal_var(self._h, i, step, out, C.byref(t)))
+        return out, t.value
+
+    def close(self):
+        if self._h:
+            lib.nw_exo_close(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _exo_sets(sets, sides):
+    """{name: nodes} or {name: (elem, side)} -> (ExoSets with ids 1.., the
+    arrays it points into)"""
+    if not sets:
+        return None, ()
+    names = (C.c_char_p * len(sets))(*[k.encode() for k in sets])
+    ents = [_i32(v[0] if sides else v) for v in sets.values()]
+    offsets = np.zeros(len(sets) + 1, np.int64)
+    offsets[1:] = np.cumsum([e.size for e in ents])
+    entries = _i32(np.concatenate(ents))
+    sd = _i32(np.concatenate([_i32(v[1]) for v in sets.values()])) if sides else None
+    s = ExoSets(len(sets), None, C.cast(names, C.c_void_p).value,
+                offsets.ctypes.data, entries.ctypes.data,
+                sd.ctypes.data if sides else None)
+    return s, (names, offsets, entries, sd)
+
+
+def exo_write(path, coords, types, conn, node_sets=None, side_sets=None,
+              fields=None, time=0.0, title="mesh"):
+    """Write a CDF-2 Exodus II file.  node_sets {name: nodes}, side_sets
+    {name: (elem, side)}, fields {name: values [N]} at one time step.
+    Returns the file's element order (file element j is input element
+    order[j]: elements are grouped by topology)."""
+    coords, types, conn = _f64(coords), _i32(types), _i32(conn)
+    ns, keep_ns = _exo_sets(node_sets, False)
+    ss, keep_ss = _exo_sets(side_sets, True)
+    fields = fields or {}
+    vn = (C.c_char_p * max(len(fields), 1))(*[k.encode() for k in fields])
+    vals = (_f64(np.stack([np.asarray(v, np.float64) for v in fields.values()]))
+            if fields else None)
+    order = np.zeros(types.size, np.int64)
+    check(lib.nw_exo_write(os.fsencode(path), title.encode(), coords.shape[0],
+                           coords, types.size, types, conn,
+                           C.byref(ns) if ns is not None else None,
+                           C.byref(ss) if ss is not None else None, len(fields),
+                           C.cast(vn, C.c_void_p) if fields else None,
+                           vals.ctypes.data_as(_P) if fields else None, time,
+                           order.ctypes.data_as(_P)))
+    del keep_ns, keep_ss
+    return order
 
 
 # ------------------------------------------------------------------ context

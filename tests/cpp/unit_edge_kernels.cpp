@@ -4,7 +4,10 @@
 // the reference algorithms: one Hex8 ("generated:1x1x1"), the trigonometric
 // test fields, helper objects that create the algorithm and execute it.
 // Prints one JSON object per test; tests/test_gpu_adapter.py compares the
-// numbers with the reference golds.
+// numbers with the reference golds.  Given a mesh file and an output path it
+// instead runs a realm read from that Exodus II file (create_mesh, device
+// geometry, boundary-condition node and side sets, a continuity solve,
+// results output), which the same test module checks against the oracle.
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -444,11 +447,93 @@ void test_two_systems_one_realm()
   std::printf("}\n");
 }
 
+// A realm read from an Exodus II file (Realm::create_mesh): the device
+// geometry of its elements, a continuity system with a Dirichlet pressure on
+// the node set "top" solved through EquationSystem::assemble_and_solve, the
+// nodes of the side set "bottom", and the results written to `out`.
+void test_exodus_realm(const char* mesh, const char* out)
+{
+  Realm realm(0);
+  realm.create_mesh(mesh);
+  GeometryInteriorAlg geom(realm);
+  geom.execute();
+  const int64_t nn = realm.nnode();
+  const std::vector<double> coords = realm.get_field(NW_F_COORDINATES);
+  std::vector<double> vel(3 * nn), p(nn), dp(3 * nn), rho(nn);
+  for (int64_t n = 0; n < nn; ++n) {
+    const double x = coords[3 * n], y = coords[3 * n + 1], z = coords[3 * n + 2];
+    vel[3 * n] = std::sin(1.3 * x + 0.2) * std::cos(0.7 * y);
+    vel[3 * n + 1] = std::cos(0.9 * x) * std::sin(1.1 * z + 0.4);
+    vel[3 * n + 2] = 0.3 * std::sin(2.0 * y + z);
+    p[n] = std::cos(1.7 * x + 0.5 * y) * (1.0 + z);
+    dp[3 * n] = -1.7 * std::sin(1.7 * x + 0.5 * y) * (1.0 + z);
+    dp[3 * n + 1] = -0.5 * std::sin(1.7 * x + 0.5 * y) * (1.0 + z);
+    dp[3 * n + 2] = std::cos(1.7 * x + 0.5 * y);
+    rho[n] = 1.0 + 0.2 * std::sin(x + 2.0 * z);
+  }
+  realm.set_field(NW_F_VELOCITY, vel);
+  realm.set_field(NW_F_PRESSURE, p);
+  realm.set_field(NW_F_DPDX, dp);
+  realm.set_field(NW_F_DENSITY, rho);
+  realm.set_field(NW_F_UDIAG, std::vector<double>(nn, 2.0));
+  realm.set_field(NW_F_PRESSURE_BC, std::vector<double>(nn, 0.0));
+  realm.timeIntegrator.timeStepN = 0.5;
+  realm.timeIntegrator.gamma1 = 1.0;
+  const std::vector<int32_t> top = realm.bc_nodes("top"), bottom = realm.bc_nodes("bottom");
+  EquationSystem cont(realm, 1);
+  cont.add_algorithm(std::unique_ptr<SolverAlgorithm>(new ContinuityEdgeSolverAlg(realm, &cont)));
+  cont.add_algorithm(std::unique_ptr<SolverAlgorithm>(
+    new DirichletBC(realm, &cont, NW_F_PRESSURE, NW_F_PRESSURE_BC, top, 0, 1)));
+  cont.initialize();
+  nw_solver_cfg cfg{};
+  cfg.kspace = 50;
+  cfg.max_iters = 2000;
+  cfg.tol = 1e-10;
+  cfg.precond = NW_PREC_AMG;
+  cfg.mode = NW_SOLVE_FP64;
+  cfg.x0 = NW_X0_ZERO;
+  cont.assemble_and_solve(&cfg);
+  realm.write_output(out, {{"pressure", NW_F_PRESSURE}, {"velocity", NW_F_VELOCITY},
+                           {"dual_nodal_volume", NW_F_DUAL_VOLUME}}, 2.5);
+  std::printf("\"exodus_realm\": {\n");
+  std::printf("\"nnode\": %lld, \"nelem\": %lld, \"nedge\": %lld,\n", (long long)nn,
+              (long long)realm.nelem(), (long long)realm.nedge());
+  print_array("velocity", vel);
+  print_array("pressure", p);
+  print_array("dpdx", dp);
+  print_array("density", rho);
+  print_array("area_vector", realm.get_field(NW_E_AREA_VECTOR));
+  print_array("dual_volume", realm.get_field(NW_F_DUAL_VOLUME));
+  std::printf("\"top\": [");
+  for (size_t i = 0; i < top.size(); ++i) std::printf("%s%d", i ? "," : "", top[i]);
+  std::printf("],\n\"bottom\": [");
+  for (size_t i = 0; i < bottom.size(); ++i) std::printf("%s%d", i ? "," : "", bottom[i]);
+  std::printf("],\n");
+  print_system("continuity", cont.linsys(), false);
+  print_array("solution", cont.linsys().solution());
+  print_solve("solve", cont.linsys(), cont.linsys().lastStats(), cont.timerSolve_);
+  int missing_throws = 0;
+  try {
+    realm.bc_nodes("no_such_surface");
+  } catch (const std::runtime_error&) {
+    missing_throws = 1;
+  }
+  std::printf("\"missing_throws\": %d\n}\n", missing_throws);
+}
+
 }  // namespace
 
-int main()
+// With no arguments: the reference unit-test fixtures.  With a mesh file
+// and an output path: the realm read from that Exodus II file.
+int main(int argc, char** argv)
 {
   try {
+    if (argc == 3) {
+      std::printf("{\n");
+      test_exodus_realm(argv[1], argv[2]);
+      std::printf("}\n");
+      return 0;
+    }
     std::printf("{\n");
     test_continuity_advection();
     test_momentum_advection_diffusion();

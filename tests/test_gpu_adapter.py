@@ -161,3 +161,80 @@ def test_two_systems_one_realm_and_solve_leg(out):
                                atol=1e-9 * np.abs(x).max())
     hs = t["hypre_solve"]
     assert hs["linearResidual"] == pytest.approx(hs["final_rel"] * hs["bnorm"], rel=1e-15)
+
+
+def _boundary_sides(nw, coords, types, conn, axis, value):
+    """(elem, side) of the faces owned by one element whose nodes all lie
+    on the plane coords[:, axis] == value"""
+    nsides = {0: 6, 1: 4, 2: 5, 3: 5}
+    owner = {}
+    for e in range(types.size):
+        for s in range(nsides[types[e]]):
+            key = tuple(sorted(conn[e, nw.side_nodes(types[e], s)]))
+            owner.setdefault(key, []).append((e, s))
+    sel = [v[0] for k, v in owner.items() if len(v) == 1 and
+           np.all(np.abs(coords[list(k), axis] - value) < 1e-12)]
+    return (np.array([e for e, _ in sel], np.int32), np.array([s for _, s in sel], np.int32))
+
+
+def test_realm_from_exodus_file(tmp_path):
+    """Realm::create_mesh on an Exodus II file of a conforming mixed
+    Hex8/Tet4/Wed6/Pyr5 box (node set "top", side set "bottom"): the
+    device geometry and the continuity system equal the oracle's on the
+    mesh read back, the boundary-condition nodes are the plane's nodes, the
+    AMG-preconditioned solve reaches its tolerance, and the results file
+    holds the realm's fields."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "nalu-wind_amd"))
+    import naluwind_amd as nw
+    from oracle import orc
+    from tests import problems as pb
+    coords, types, conn = nw.mixed_box(10, 0.6, 0.3, jitter=0.15, seed=4)
+    top = np.where(coords[:, 2] >= 1 - 1e-12)[0].astype(np.int32)
+    bottom = _boundary_sides(nw, coords, types, conn, 2, 0.0)
+    mesh, res = str(tmp_path / "box.exo"), str(tmp_path / "results.exo")
+    order = nw.exo_write(mesh, coords, types, conn, node_sets={"top": top},
+                         side_sets={"bottom": bottom})
+    r = subprocess.run([EXE, mesh, res], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    t = json.loads(r.stdout)["exodus_realm"]
+    types, conn = types[order], conn[order]  # the file's element order
+    n = coords.shape[0]
+    edges = nw.mixed_edges(types, conn, n)
+    assert (t["nnode"], t["nelem"], t["nedge"]) == (n, types.size, edges.shape[0])
+    ra = orc.edge_area_vec_mixed(types, conn, coords, edges)
+    rv = orc.dual_volume_mixed(types, conn, coords)
+    a = np.array(t["area_vector"]).reshape(-1, 3)
+    np.testing.assert_allclose(a, ra, rtol=0, atol=1e-14 * np.abs(ra).max())
+    np.testing.assert_allclose(t["dual_volume"], rv, rtol=1e-13, atol=0)
+    np.testing.assert_array_equal(t["top"], top)
+    np.testing.assert_array_equal(t["bottom"], np.where(np.abs(coords[:, 2]) < 1e-12)[0])
+    assert t["missing_throws"] == 1
+    vel = np.array(t["velocity"]).reshape(-1, 3)
+    p = np.array(t["pressure"])
+    A = orc.CsrSystem(n, edges, 1)
+    orc.assemble_continuity(orc.ContinuityOpts(0.5, 1.0, 1.0, 1.0, 0.0), n, edges, coords,
+                            vel, np.array(t["dpdx"]).reshape(-1, 3), np.array(t["density"]),
+                            p, np.full(n, 2.0), ra, csr=A)
+    A.apply_dirichlet(top, 0, 1, p, np.zeros(n))
+    got = _got(t["continuity"])
+    pb.assert_csr_close(got, A, rtol=1e-11)
+    st = t["solve"]
+    assert st["converged"] == 1 and st["iters"] > 0
+    x = np.array(t["solution"])
+    res_norm = np.linalg.norm(A.rhs - A.spmv(x))
+    assert res_norm <= 1.001e-10 * np.linalg.norm(A.rhs) + 1e-12 * np.abs(A.vals).max()
+    with nw.ExodusMesh(res) as out:
+        c2, t2, k2, _ = out.read_mesh()
+        np.testing.assert_array_equal(c2, coords)
+        o2 = np.argsort(types, kind="stable")
+        np.testing.assert_array_equal(t2, types[o2])
+        np.testing.assert_array_equal(k2, conn[o2])
+        assert out.nodal_vars == ["pressure", "velocity_x", "velocity_y", "velocity_z",
+                                  "dual_nodal_volume"]
+        v, tm = out.nodal_var("velocity_y")
+        assert tm == 2.5
+        np.testing.assert_array_equal(v, vel[:, 1])
+        np.testing.assert_array_equal(out.nodal_var("pressure")[0], p)
+        np.testing.assert_array_equal(out.nodal_var("dual_nodal_volume")[0], t["dual_volume"])
