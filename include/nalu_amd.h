@@ -390,7 +390,10 @@ typedef struct {
   double bnorm;
   double ms;            /* device time of the solve on the context stream,
                            preconditioner setup excluded */
-  double setup_ms;      /* preconditioner setup of this call (0 if reused) */
+  double setup_ms;      /* preconditioner setup of this call, wall time (0
+                           if reused): for AMG the matrix download, the host
+                           build and the upload of the hierarchy, or the
+                           FP32 copy made on a first FP32-inner solve */
   int precond_levels;   /* AMG levels (0 for other preconditioners) */
   int guess;            /* NW_X0_* of the guess GMRES started from (a learned
                            guess that did not lower the residual falls back

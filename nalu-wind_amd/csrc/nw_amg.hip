@@ -5,8 +5,10 @@
 // as SpMVs, and a dense inverse on the coarsest level.  The V-cycle is a
 // fixed linear operator, so it serves as a right preconditioner for GMRES.
 // It runs in the solver's storage type: double for the FP64 method, float
-// for the FP32-inner method (the FP32 copies are made at upload).
+// for the FP32-inner method (the FP32 copy is narrowed from the FP64 one on
+// the device and shares its structure arrays).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -60,6 +62,7 @@ struct DevMat {
   const int32_t* rp = nullptr;
   const int32_t* ci = nullptr;
   T* v = nullptr;
+  int64_t nv = 0;  // stored values (padding included)
   int64_t* own_sp = nullptr;
   int32_t* own_rp = nullptr;
   int32_t* own_ci = nullptr;
@@ -157,14 +160,12 @@ struct DevHier {
 
 struct AmgDevice {
   DevHier<double> d;
-  DevHier<float> f;
-  std::vector<amg::Level> hostlv;  // kept for the FP32 upload on demand
-  std::vector<double> coarse_inv;
+  DevHier<float> f;  // made from d on the device when first asked for
+  std::vector<int64_t> rows, nnz;  // per level
   int nlev = 0;
   double setup_seconds = 0;
   int cheb_degree = 2;
   double cheb_ratio = 20.0;
-  int64_t fine_nnz = 0;
 };
 
 template <typename T>
@@ -195,124 +196,201 @@ static void free_hier(DevHier<T>& h)
 void amg_free(AmgDevice* a)
 {
   if (!a) return;
+  free_hier(a->f);  // shares structure arrays with d
   free_hier(a->d);
-  free_hier(a->f);
   delete a;
 }
 
-// Uploads a host CSR matrix in sliced ELL form; padding entries repeat the
-// row's last column (or column 0 for an empty row) with value 0.
+namespace {
+
+// device buffers released when the scope ends
+struct DevScratch {
+  std::vector<void*> p;
+  template <typename T>
+  int alloc(T** q, int64_t n)
+  {
+    *q = nullptr;
+    if (hipMalloc(q, sizeof(T) * std::max<int64_t>(n, 1)) != hipSuccess) return NW_E_ALLOC;
+    p.push_back(*q);
+    return NW_OK;
+  }
+  ~DevScratch()
+  {
+    for (void* q : p) (void)hipFree(q);
+  }
+};
+
 template <typename T>
-static int upload_mat(const amg::Csr& h, DevMat<T>& d, hipStream_t st)
+__global__ void k_narrow(const double* __restrict__ in, T* __restrict__ out, int64_t n)
 {
-  NW_REQUIRE(h.nrows < INT32_MAX && h.ncols >= 1 && h.nnz() < INT32_MAX,
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (T)in[i];
+}
+
+}  // namespace
+
+// Uploads a host CSR matrix.  Rows of more than kSellMaxAvgRow entries on
+// average stay in CSR (G lanes per row); shorter ones are packed into sliced
+// ELL on the device from the uploaded CSR, with padding positions repeating
+// the row's last column (value 0), so that they stay inside the columns of
+// a rectangular P or R.
+static int upload_mat(const amg::Csr& h, DevMat<double>& d, hipStream_t st)
+{
+  NW_REQUIRE(h.nrows >= 1 && h.nrows < INT32_MAX && h.ncols >= 1 && h.nnz() < INT32_MAX,
              NW_E_ARG, "AMG level too large or empty");
-  const double avg = (double)h.nnz() / std::max<int64_t>(h.nrows, 1);
+  const int64_t n = h.nrows, nnz = h.nnz();
+  const double avg = (double)nnz / (double)n;
+  const std::vector<int32_t> rp(h.rp.begin(), h.rp.end());
+  d.n = (int)n;
   if (avg > kSellMaxAvgRow) {
-    // CSR, G lanes per row
-    std::vector<int32_t> rp(h.rp.begin(), h.rp.end());
-    std::vector<T> v(h.v.begin(), h.v.end());
-    d.n = (int)h.nrows;
     d.G = lanes_per_row(avg);
+    d.nv = nnz;
     NW_HIP(hipMalloc(&d.own_rp, sizeof(int32_t) * rp.size()));
-    NW_HIP(hipMalloc(&d.own_ci, sizeof(int32_t) * std::max<int64_t>(h.nnz(), 1)));
-    NW_HIP(hipMalloc(&d.v, sizeof(T) * std::max<int64_t>(h.nnz(), 1)));
+    NW_HIP(hipMalloc(&d.own_ci, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+    NW_HIP(hipMalloc(&d.v, sizeof(double) * std::max<int64_t>(nnz, 1)));
     d.rp = d.own_rp;
     d.ci = d.own_ci;
     NW_HIP(hipMemcpyAsync(d.own_rp, rp.data(), sizeof(int32_t) * rp.size(),
                           hipMemcpyHostToDevice, st));
-    NW_HIP(hipMemcpyAsync(d.own_ci, h.ci.data(), sizeof(int32_t) * h.nnz(),
+    NW_HIP(hipMemcpyAsync(d.own_ci, h.ci.data(), sizeof(int32_t) * nnz,
                           hipMemcpyHostToDevice, st));
-    NW_HIP(hipMemcpyAsync(d.v, v.data(), sizeof(T) * h.nnz(),
-                          hipMemcpyHostToDevice, st));
+    NW_HIP(hipMemcpyAsync(d.v, h.v.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, st));
     NW_HIP(hipStreamSynchronize(st));
     return NW_OK;
   }
-  const int64_t n = h.nrows, ns = (n + kWave - 1) / kWave;
-  std::vector<int64_t> sp(ns + 1, 0);
-  for (int64_t k = 0; k < ns; ++k) {
-    int64_t w = 0;
-    for (int64_t r = k * kWave; r < std::min(n, (k + 1) * kWave); ++r)
-      w = std::max(w, h.rp[r + 1] - h.rp[r]);
-    sp[k + 1] = sp[k] + kWave * w;
+  DevScratch tmp;
+  int32_t *drp = nullptr, *dci = nullptr;
+  double* dv = nullptr;
+  int rc = tmp.alloc(&drp, n + 1);
+  if (!rc) rc = tmp.alloc(&dci, nnz);
+  if (!rc) rc = tmp.alloc(&dv, nnz);
+  NW_REQUIRE(rc == NW_OK, NW_E_ALLOC, "AMG level upload: device allocation failed");
+  NW_HIP(hipMemcpyAsync(drp, rp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
+  NW_HIP(hipMemcpyAsync(dci, h.ci.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
+  NW_HIP(hipMemcpyAsync(dv, h.v.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+  Sell s;
+  rc = sell_build(drp, dci, (int)n, s, st, true);
+  d.own_sp = s.sp;  // owned by d from here on, also on failure
+  d.own_ci = s.ci;
+  tmp.p.push_back(s.slot);
+  if (rc) return rc;
+  d.sp = s.sp;
+  d.ci = s.ci;
+  d.nv = s.len;
+  NW_HIP(hipMalloc(&d.v, sizeof(double) * std::max<int64_t>(s.len, 1)));
+  if (s.len > 0) {
+    k_sell_values<double><<<grid_for(s.len, kBlock), kBlock, 0, st>>>(s.slot, s.ci, dv,
+                                                                      nullptr, d.v, s.len);
+    NW_LAUNCH_CHECK();
   }
-  const int64_t len = sp[ns];
-  std::vector<int32_t> ci(std::max<int64_t>(len, 1), 0);
-  std::vector<T> v(std::max<int64_t>(len, 1), (T)0);
-  for (int64_t k = 0; k < ns; ++k) {
-    const int64_t w = (sp[k + 1] - sp[k]) / kWave;
-    for (int64_t l = 0; l < kWave; ++l) {
-      const int64_t r = k * kWave + l;
-      const int64_t r0 = r < n ? h.rp[r] : 0, m = r < n ? h.rp[r + 1] - r0 : 0;
-      for (int64_t j = 0; j < w; ++j) {
-        const int64_t p = sp[k] + kWave * j + l;
-        if (j < m) {
-          ci[p] = h.ci[r0 + j];
-          v[p] = (T)h.v[r0 + j];
-        } else {
-          ci[p] = m > 0 ? h.ci[r0 + m - 1] : 0;
-        }
-      }
-    }
-  }
-  d.n = (int)n;
-  NW_HIP(hipMalloc(&d.own_sp, sizeof(int64_t) * (ns + 1)));
-  NW_HIP(hipMalloc(&d.own_ci, sizeof(int32_t) * ci.size()));
-  NW_HIP(hipMalloc(&d.v, sizeof(T) * v.size()));
-  d.sp = d.own_sp;
-  d.ci = d.own_ci;
-  NW_HIP(hipMemcpyAsync(d.own_sp, sp.data(), sizeof(int64_t) * (ns + 1),
-                        hipMemcpyHostToDevice, st));
-  NW_HIP(hipMemcpyAsync(d.own_ci, ci.data(), sizeof(int32_t) * ci.size(),
-                        hipMemcpyHostToDevice, st));
-  NW_HIP(hipMemcpyAsync(d.v, v.data(), sizeof(T) * v.size(),
-                        hipMemcpyHostToDevice, st));
   NW_HIP(hipStreamSynchronize(st));
   return NW_OK;
 }
 
+// The FP32 copy of an FP64 level matrix: the same structure arrays (not
+// owned), values narrowed on the device.
+static int narrow_mat(const DevMat<double>& s, DevMat<float>& d, hipStream_t st)
+{
+  d.n = s.n;
+  d.G = s.G;
+  d.sp = s.sp;
+  d.rp = s.rp;
+  d.ci = s.ci;
+  d.nv = s.nv;
+  NW_HIP(hipMalloc(&d.v, sizeof(float) * std::max<int64_t>(s.nv, 1)));
+  if (s.nv > 0) {
+    k_narrow<float><<<grid_for(s.nv, kBlock), kBlock, 0, st>>>(s.v, d.v, s.nv);
+    NW_LAUNCH_CHECK();
+  }
+  return NW_OK;
+}
+
 template <typename T>
-static int upload_hier(Ctx* c, AmgDevice* a, DevHier<T>& H)
+static int alloc_level_vectors(DevLevel<T>& L)
+{
+  for (T** p : {&L.x, &L.b, &L.r, &L.t, &L.cd, &L.cz, &L.cz2})
+    NW_HIP(hipMalloc(p, sizeof(T) * std::max(L.n, 1)));
+  return NW_OK;
+}
+
+// The FP64 hierarchy on the device from the host one.
+static int upload_hier(Ctx* c, const amg::Hierarchy& h, DevHier<double>& H)
 {
   hipStream_t st = c->stream;
-  const int nl = (int)a->hostlv.size();
+  const int nl = (int)h.levels.size();
   H.lv.resize(nl);
   for (int l = 0; l < nl; ++l) {
-    const amg::Level& h = a->hostlv[l];
-    DevLevel<T>& L = H.lv[l];
-    L.n = (int)(l == 0 ? c->g.nnode : h.A.nrows);
-    L.lmax = h.lmax;
+    const amg::Level& hl = h.levels[l];
+    DevLevel<double>& L = H.lv[l];
+    L.n = (int)hl.A.nrows;
+    L.lmax = hl.lmax;
     if (l == 0) {
       // the fine operator is the linear system's own matrix, in the
-      // graph's sliced ELL order
-      NW_REQUIRE(c->g.sell.ci, NW_E_STATE, "sliced ELL form not built");
+      // graph's sliced ELL order (values refreshed at every solve)
+      NW_REQUIRE(c->g.sell.ci && L.n == c->g.nnode, NW_E_STATE,
+                 "sliced ELL form not built");
       L.A.n = L.n;
       L.A.sp = c->g.sell.sp;
       L.A.ci = c->g.sell.ci;
-      NW_HIP(hipMalloc(&L.A.v, sizeof(T) * std::max<int64_t>(c->g.sell.len, 1)));
+      L.A.nv = c->g.sell.len;
+      NW_HIP(hipMalloc(&L.A.v, sizeof(double) * std::max<int64_t>(c->g.sell.len, 1)));
     } else {
-      int rc = upload_mat(h.A, L.A, st);
+      int rc = upload_mat(hl.A, L.A, st);
       if (rc) return rc;
     }
-    std::vector<T> dinv(h.dinv.begin(), h.dinv.end());
-    NW_HIP(hipMalloc(&L.dinv, sizeof(T) * L.n));
-    NW_HIP(hipMemcpyAsync(L.dinv, dinv.data(), sizeof(T) * L.n,
+    NW_HIP(hipMalloc(&L.dinv, sizeof(double) * std::max(L.n, 1)));
+    NW_HIP(hipMemcpyAsync(L.dinv, hl.dinv.data(), sizeof(double) * L.n,
                           hipMemcpyHostToDevice, st));
-    for (T** p : {&L.x, &L.b, &L.r, &L.t, &L.cd, &L.cz, &L.cz2})
-      NW_HIP(hipMalloc(p, sizeof(T) * std::max(L.n, 1)));
-    if (l + 1 < nl) {
-      int rc = upload_mat(h.P, L.P, st);
-      if (rc) return rc;
-      rc = upload_mat(h.R, L.R, st);
+    int rc = alloc_level_vectors(L);
+    if (!rc && l + 1 < nl) rc = upload_mat(hl.P, L.P, st);
+    if (!rc && l + 1 < nl) rc = upload_mat(hl.R, L.R, st);
+    if (rc) return rc;
+  }
+  H.nc = (int)h.levels.back().A.nrows;
+  NW_HIP(hipMalloc(&H.cinv, sizeof(double) * std::max<size_t>(h.coarse_inv.size(), 1)));
+  NW_HIP(hipMemcpyAsync(H.cinv, h.coarse_inv.data(), sizeof(double) * h.coarse_inv.size(),
+                        hipMemcpyHostToDevice, st));
+  NW_HIP(hipStreamSynchronize(st));
+  H.ready = true;
+  return NW_OK;
+}
+
+// The FP32 hierarchy from the FP64 one, on the device.
+static int narrow_hier(Ctx* c, const DevHier<double>& D, DevHier<float>& H)
+{
+  hipStream_t st = c->stream;
+  const int nl = (int)D.lv.size();
+  H.lv.resize(nl);
+  for (int l = 0; l < nl; ++l) {
+    const DevLevel<double>& S = D.lv[l];
+    DevLevel<float>& L = H.lv[l];
+    L.n = S.n;
+    L.lmax = S.lmax;
+    if (l == 0) {
+      L.A.n = S.A.n;
+      L.A.sp = S.A.sp;
+      L.A.ci = S.A.ci;
+      L.A.nv = S.A.nv;
+      NW_HIP(hipMalloc(&L.A.v, sizeof(float) * std::max<int64_t>(S.A.nv, 1)));
+    } else {
+      int rc = narrow_mat(S.A, L.A, st);
       if (rc) return rc;
     }
-    NW_HIP(hipStreamSynchronize(st));
+    NW_HIP(hipMalloc(&L.dinv, sizeof(float) * std::max(L.n, 1)));
+    k_narrow<float><<<grid_for(L.n, kBlock), kBlock, 0, st>>>(S.dinv, L.dinv, L.n);
+    NW_LAUNCH_CHECK();
+    int rc = alloc_level_vectors(L);
+    if (!rc && l + 1 < nl) rc = narrow_mat(S.P, L.P, st);
+    if (!rc && l + 1 < nl) rc = narrow_mat(S.R, L.R, st);
+    if (rc) return rc;
   }
-  H.nc = (int)a->hostlv.back().A.nrows;
-  std::vector<T> ci(a->coarse_inv.begin(), a->coarse_inv.end());
-  NW_HIP(hipMalloc(&H.cinv, sizeof(T) * std::max<size_t>(ci.size(), 1)));
-  NW_HIP(hipMemcpyAsync(H.cinv, ci.data(), sizeof(T) * ci.size(),
-                        hipMemcpyHostToDevice, st));
+  H.nc = D.nc;
+  const int64_t nci = (int64_t)D.nc * D.nc;
+  NW_HIP(hipMalloc(&H.cinv, sizeof(float) * std::max<int64_t>(nci, 1)));
+  if (nci > 0) {
+    k_narrow<float><<<grid_for(nci, kBlock), kBlock, 0, st>>>(D.cinv, H.cinv, nci);
+    NW_LAUNCH_CHECK();
+  }
   NW_HIP(hipStreamSynchronize(st));
   H.ready = true;
   return NW_OK;
@@ -320,13 +398,10 @@ static int upload_hier(Ctx* c, AmgDevice* a, DevHier<T>& H)
 
 int amg_setup(Ctx* c, AmgDevice** out)
 {
+  const auto t0 = std::chrono::steady_clock::now();
   Graph& g = c->g;
   NW_REQUIRE(g.ndof == 1, NW_E_STATE, "AMG needs a 1-dof system");
-  amg::Csr A;
-  A.nrows = A.ncols = g.nnode;
-  A.rp.assign(g.h_row_ptr.begin(), g.h_row_ptr.end());
-  A.ci = g.h_cols;
-  A.v.resize(g.nnzb);
+  amg::Csr A = amg::csr_with_structure(g.nnode, g.h_row_ptr.data(), g.h_cols.data());
   NW_HIP(hipMemcpyAsync(A.v.data(), g.vals, sizeof(double) * g.nnzb,
                         hipMemcpyDeviceToHost, c->stream));
   NW_HIP(hipStreamSynchronize(c->stream));
@@ -336,14 +411,20 @@ int amg_setup(Ctx* c, AmgDevice** out)
   NW_REQUIRE(rc == 0, NW_E_ARG, "AMG setup failed (%d): coarse level too "
              "large or singular", rc);
   AmgDevice* a = new AmgDevice();
-  a->setup_seconds = h.setup_seconds;
   a->cheb_degree = o.cheb_degree;
   a->cheb_ratio = o.cheb_ratio;
   a->nlev = (int)h.levels.size();
-  a->fine_nnz = g.nnzb;
-  a->coarse_inv = std::move(h.coarse_inv);
-  a->hostlv = std::move(h.levels);
-  a->hostlv[0].A = amg::Csr();  // the fine matrix lives on the device
+  for (const auto& L : h.levels) {
+    a->rows.push_back(L.A.nrows);
+    a->nnz.push_back(L.A.nnz());
+  }
+  const int urc = upload_hier(c, h, a->d);
+  if (urc) {
+    amg_free(a);
+    return urc;
+  }
+  a->setup_seconds =
+    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   *out = a;
   return NW_OK;
 }
@@ -353,9 +434,8 @@ double amg_setup_seconds(const AmgDevice* a) { return a ? a->setup_seconds : 0; 
 int amg_level_info(const AmgDevice* a, int l, int64_t* rows, int64_t* nnz)
 {
   if (!a || l < 0 || l >= a->nlev) return NW_E_ARG;
-  const amg::Level& h = a->hostlv[l];
-  *rows = l == 0 ? (int64_t)h.dinv.size() : h.A.nrows;
-  *nnz = l == 0 ? a->fine_nnz : h.A.nnz();
+  *rows = a->rows[l];
+  *nnz = a->nnz[l];
   return NW_OK;
 }
 
@@ -373,12 +453,19 @@ DevHier<float>& hier<float>(AmgDevice* a)
 }
 
 template <typename T>
-int amg_prepare(Ctx* c, AmgDevice* a)
+int amg_prepare(Ctx* c, AmgDevice* a, double* seconds)
 {
   DevHier<T>& H = hier<T>(a);
+  NW_REQUIRE(a->d.ready, NW_E_STATE, "AMG hierarchy not set up");
+  if (seconds) *seconds = 0.0;
   if (!H.ready) {
-    int rc = upload_hier<T>(c, a, H);
-    if (rc) return rc;
+    if constexpr (sizeof(T) == 4) {
+      const auto t0 = std::chrono::steady_clock::now();
+      int rc = narrow_hier(c, a->d, a->f);
+      if (rc) return rc;
+      if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
   }
   // the fine level follows the current matrix values (sliced ELL order)
   const Sell& sl = c->g.sell;
@@ -485,8 +572,8 @@ int amg_apply(AmgDevice* a, const T* b, T* x, hipStream_t st)
   return vcycle<T>(a, hier<T>(a), 0, b, x, st);
 }
 
-template int amg_prepare<double>(Ctx*, AmgDevice*);
-template int amg_prepare<float>(Ctx*, AmgDevice*);
+template int amg_prepare<double>(Ctx*, AmgDevice*, double*);
+template int amg_prepare<float>(Ctx*, AmgDevice*, double*);
 template int amg_apply<double>(AmgDevice*, const double*, double*, hipStream_t);
 template int amg_apply<float>(AmgDevice*, const float*, float*, hipStream_t);
 

@@ -290,11 +290,14 @@ __global__ __launch_bounds__(kBlock) void k_sell_width(
   if ((threadIdx.x & (kWave - 1)) == 0 && (t >> 6) < nslices) width[t >> 6] = len;
 }
 
-// columns and CSR slots of every sliced-ELL position (a lane per row)
+// columns and CSR slots of every sliced-ELL position (a lane per row);
+// padding positions take the row's own index as their column (square
+// matrices) or, with pad_last, the row's last column (column 0 for an
+// empty row), which stays inside any matrix's columns
 __global__ __launch_bounds__(kBlock) void k_sell_struct(
   const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, int n,
   const int64_t* __restrict__ sp, int nslices, int32_t* __restrict__ sci,
-  int32_t* __restrict__ sslot)
+  int32_t* __restrict__ sslot, bool pad_last)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = t >> 6;
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_struct(
       sci[p] = ci[r0 + j];
       sslot[p] = r0 + j;
     } else {
-      sci[p] = t < n ? t : 0;
+      sci[p] = pad_last ? (len > 0 ? ci[r0 + len - 1] : 0) : (t < n ? t : 0);
       sslot[p] = -1;
     }
   }
@@ -318,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_struct(
 }  // namespace
 
 int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
-               hipStream_t st)
+               hipStream_t st, bool pad_last)
 {
   s.n = n;
   s.nslices = (n + kWave - 1) / kWave;
@@ -341,7 +344,7 @@ int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
   NW_HIP(hipMemcpyAsync(s.sp, sp.data(), sizeof(int64_t) * (ns + 1),
                         hipMemcpyHostToDevice, st));
   k_sell_struct<<<grid_for((int64_t)ns * kWave, kBlock), kBlock, 0, st>>>(
-    rp, ci, n, s.sp, ns, s.ci, s.slot);
+    rp, ci, n, s.sp, ns, s.ci, s.slot, pad_last);
   NW_LAUNCH_CHECK();
   NW_HIP(hipStreamSynchronize(st));
   return NW_OK;
@@ -860,8 +863,10 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
     out->setup_ms = 1e3 * amg_setup_seconds(c->amg);
   }
   if (amgp) {
-    rc = f32 ? amg_prepare<float>(c, c->amg) : amg_prepare<double>(c, c->amg);
+    double narrow_s = 0.0;  // the FP32 hierarchy, made on first use
+    rc = f32 ? amg_prepare<float>(c, c->amg, &narrow_s) : amg_prepare<double>(c, c->amg);
     if (rc) return rc;
+    out->setup_ms += 1e3 * narrow_s;
     out->precond_levels = amg_levels(c->amg);
   }
   NW_HIP(hipEventRecord(k->ev0, st));

@@ -271,9 +271,12 @@ __global__ void k_sell_values(const int32_t* __restrict__ slot,
   out[p] = k < 0 ? (T)0 : (T)(colscale ? a[k] * colscale[ci[p]] : a[k]);
 }
 
-// Builds the sliced ELL structure of the CSR graph (rp, ci) with n rows.
+// Builds the sliced ELL structure of the CSR graph (rp, ci; device arrays)
+// with n rows.  Padding positions read the row's own index as column
+// (square matrices), or with pad_last the row's last column, which keeps
+// them inside the columns of a rectangular matrix.
 int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
-               hipStream_t st);
+               hipStream_t st, bool pad_last = false);
 
 // launch helper: dispatch a G-templated row kernel by lanes-per-row
 // (kernel<T, G>; NW_ROW_LAUNCH_ADD: kernel<T, G, true>)

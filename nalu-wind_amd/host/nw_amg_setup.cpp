@@ -48,10 +48,13 @@ double dot(const std::vector<double>& a, const std::vector<double>& b)
   return s;
 }
 
-// lambda_max(D^-1 A) by power iteration from a fixed start vector
+// lambda_max(D^-1 A) by power iteration from a fixed start vector.  Each
+// step normalises x, then forms y = D^-1 A x together with the chunked
+// partial sums of x.y (the Rayleigh quotient) and y.y (the next norm), in
+// the chunks of dot(), so the result equals the unfused sequence bitwise.
 double power_lmax(const Csr& A, const std::vector<double>& dinv, int iters)
 {
-  const int64_t n = A.nrows;
+  const int64_t n = A.nrows, nc = (n + kChunk - 1) / kChunk;
   std::vector<double> x(n), y(n);
   uint64_t z = 0x2545f4914f6cdd1dULL;
   for (int64_t i = 0; i < n; ++i) {
@@ -60,19 +63,32 @@ double power_lmax(const Csr& A, const std::vector<double>& dinv, int iters)
     z ^= z << 17;
     x[i] = 0.5 + (double)(z >> 11) * (1.0 / 9007199254740992.0);
   }
-  double lam = 1.0;
+  std::vector<double> pxy(nc), pyy(nc);
+  double lam = 1.0, nx2 = dot(x, x);
   for (int it = 0; it < iters; ++it) {
-    const double nx = std::sqrt(dot(x, x));
+    const double nx = std::sqrt(nx2);
     if (!(nx > 0)) break;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) x[i] /= nx;
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < n; ++i) {  // y = D^-1 A x
-      double s = 0.0;
-      for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) s += A.v[k] * x[A.ci[k]];
-      y[i] = s * dinv[i];
+    for (int64_t c = 0; c < nc; ++c) {
+      double sxy = 0.0, syy = 0.0;
+      for (int64_t i = c * kChunk; i < std::min(n, (c + 1) * kChunk); ++i) {
+        double s = 0.0;
+        for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) s += A.v[k] * x[A.ci[k]];
+        y[i] = s * dinv[i];
+        sxy += x[i] * y[i];
+        syy += y[i] * y[i];
+      }
+      pxy[c] = sxy;
+      pyy[c] = syy;
     }
-    lam = dot(x, y);
+    lam = 0.0;
+    nx2 = 0.0;
+    for (int64_t c = 0; c < nc; ++c) {
+      lam += pxy[c];
+      nx2 += pyy[c];
+    }
     x.swap(y);
   }
   return lam;
@@ -89,22 +105,46 @@ std::vector<double> inv_diagonal(const Csr& A)
   return d;
 }
 
+// R = P^T with sorted columns: entries counted and placed per column in
+// parallel, then each row of R sorted by column (the rows of P), which
+// gives the same arrays as a serial row-order fill.
 Csr transpose(const Csr& P)
 {
   Csr R;
   R.nrows = P.ncols;
   R.ncols = P.nrows;
   R.rp.assign(R.nrows + 1, 0);
-  for (int64_t k = 0; k < P.nnz(); ++k) R.rp[P.ci[k] + 1]++;
+  const int64_t nnz = P.nnz();
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < nnz; ++k) {
+#pragma omp atomic
+    R.rp[P.ci[k] + 1]++;
+  }
   for (int64_t i = 0; i < R.nrows; ++i) R.rp[i + 1] += R.rp[i];
-  R.ci.resize(P.nnz());
-  R.v.resize(P.nnz());
-  std::vector<int64_t> pos(R.rp.begin(), R.rp.end() - 1);
-  for (int64_t i = 0; i < P.nrows; ++i)  // row order -> sorted columns of R
+  R.ci.resize(nnz);
+  R.v.resize(nnz);
+  uvec<int64_t> pos(R.rp.begin(), R.rp.end() - 1);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t i = 0; i < P.nrows; ++i)
     for (int64_t k = P.rp[i]; k < P.rp[i + 1]; ++k) {
-      const int64_t p = pos[P.ci[k]]++;
+      int64_t p;
+#pragma omp atomic capture
+      p = pos[P.ci[k]]++;
       R.ci[p] = (int32_t)i;
       R.v[p] = P.v[k];
+    }
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t r = 0; r < R.nrows; ++r)  // insertion sort: rows are short
+    for (int64_t k = R.rp[r] + 1; k < R.rp[r + 1]; ++k) {
+      const int32_t c = R.ci[k];
+      const double v = R.v[k];
+      int64_t q = k;
+      for (; q > R.rp[r] && R.ci[q - 1] > c; --q) {
+        R.ci[q] = R.ci[q - 1];
+        R.v[q] = R.v[q - 1];
+      }
+      R.ci[q] = c;
+      R.v[q] = v;
     }
   return R;
 }
@@ -296,6 +336,8 @@ void coarsen(const Csr& A, const Options& o, Csr& P)
   std::vector<int32_t> size(na, 0);
   for (int64_t i = 0; i < n; ++i)
     if (agg[i] >= 0) size[agg[i]]++;
+  std::vector<double> sq(na);  // the aggregate indicator's norm
+  for (int32_t a = 0; a < na; ++a) sq[a] = std::sqrt((double)size[a]);
   // omega from lambda_max(D_F^-1 A_F)
   const std::vector<double> dfinv = inv_diagonal(F);
   const double lf = power_lmax(F, dfinv, o.power_iters);
@@ -305,22 +347,22 @@ void coarsen(const Csr& A, const Options& o, Csr& P)
   // with room for that many, then packed
   P.nrows = n;
   P.ncols = na;
-  std::vector<int64_t> ub(n + 1, 0);
+  uvec<int64_t> ub(n + 1, 0);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) ub[i + 1] = 1 + F.rp[i + 1] - F.rp[i];
   for (int64_t i = 0; i < n; ++i) ub[i + 1] += ub[i];
-  std::vector<std::pair<int32_t, double>> stage(ub[n]);
+  uvec<std::pair<int32_t, double>> stage(ub[n]);
   P.rp.assign(n + 1, 0);
 #pragma omp parallel for schedule(dynamic, 4096)
   for (int64_t i = 0; i < n; ++i) {
     if (agg[i] < 0) continue;
     auto* r = stage.data() + ub[i];
     size_t m = 0;
-    r[m++] = {agg[i], 1.0 / std::sqrt((double)size[agg[i]])};
+    r[m++] = {agg[i], 1.0 / sq[agg[i]]};
     for (int64_t k = F.rp[i]; k < F.rp[i + 1]; ++k) {
       const int32_t j = F.ci[k];
       if (agg[j] < 0) continue;
-      r[m++] = {agg[j], -omega * dfinv[i] * F.v[k] / std::sqrt((double)size[agg[j]])};
+      r[m++] = {agg[j], -omega * dfinv[i] * F.v[k] / sq[agg[j]]};
     }
     std::sort(r, r + m, [](const auto& a, const auto& b) { return a.first < b.first; });
     size_t w = 0;
@@ -346,6 +388,21 @@ void coarsen(const Csr& A, const Options& o, Csr& P)
 }
 
 }  // namespace
+
+Csr csr_with_structure(int64_t n, const int32_t* rp, const int32_t* ci)
+{
+  Csr A;
+  A.nrows = A.ncols = n;
+  A.rp.resize(n + 1);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i <= n; ++i) A.rp[i] = rp[i];
+  const int64_t nnz = rp[n];
+  A.ci.resize(nnz);
+  A.v.resize(nnz);
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < nnz; ++k) A.ci[k] = ci[k];
+  return A;
+}
 
 int build(Csr A0, const Options& o, Hierarchy& h)
 {

@@ -6,18 +6,50 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
+#include <utility>
 #include <vector>
 
 namespace nw {
 namespace amg {
 
+// Storage whose resize leaves new elements uninitialised: the setup's
+// arrays are filled by parallel loops right after they are sized, so a
+// value-initialising resize would only add a serial pass over memory.
+template <class T>
+struct DefaultInit : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = DefaultInit<U>;
+  };
+  DefaultInit() = default;
+  template <class U>
+  DefaultInit(const DefaultInit<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept
+  {
+    ::new ((void*)p) U;
+  }
+  template <class U, class... Args>
+  void construct(U* p, Args&&... args)
+  {
+    ::new ((void*)p) U(std::forward<Args>(args)...);
+  }
+};
+template <class T>
+using uvec = std::vector<T, DefaultInit<T>>;
+
 struct Csr {
   int64_t nrows = 0, ncols = 0;
-  std::vector<int64_t> rp;
-  std::vector<int32_t> ci;
-  std::vector<double> v;
+  uvec<int64_t> rp;
+  uvec<int32_t> ci;
+  uvec<double> v;
   int64_t nnz() const { return rp.empty() ? 0 : rp.back(); }
 };
+
+// A CSR matrix of n rows with the given structure (rp [n + 1], ci); its
+// values are left for the caller to fill.  Copies in parallel.
+Csr csr_with_structure(int64_t n, const int32_t* rp, const int32_t* ci);
 
 struct Level {
   Csr A;                    // operator on this level

@@ -132,6 +132,46 @@ def test_amg_gmres_matches_direct_solve(nw, ctx, mode):
     np.testing.assert_allclose(x, xd, rtol=0, atol=1e-7 * np.abs(xd).max())
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_amg_on_a_system_within_the_coarse_size(nw, ctx, mode):
+    """A system of at most 1,000 rows is its own coarsest level: the
+    preconditioner is its dense inverse, so GMRES converges at once (in
+    FP32-inner form, one FP32 cycle per refinement)."""
+    P, A = _pressure_system(nw, ctx, nx=6)  # 343 rows
+    xd = orc.dense_solve(A.dense(), A.rhs)
+    st = ctx.solve(nw.solver_cfg(kspace=20, max_iters=50, tol=1e-10, precond=nw.PREC_AMG,
+                                 mode=mode, reuse_preconditioner=0))
+    x = ctx.solution()
+    assert st.converged and st.precond_levels == 1 and st.setup_ms > 0
+    assert st.iters <= (2 if mode == 0 else 6), st.iters
+    r = A.rhs - A.spmv(x)
+    assert np.linalg.norm(r) <= 1.0001e-10 * np.linalg.norm(A.rhs)
+    np.testing.assert_allclose(x, xd, rtol=0, atol=1e-7 * np.abs(xd).max())
+
+
+def test_amg_levels_and_fp32_copy(nw, ctx):
+    """The hierarchy's level sizes shrink and the FP32 solve made from the
+    same setup (its copy narrowed on the device, counted as setup of that
+    call) reaches the FP64 tolerance by refinement."""
+    P, A = _pressure_system(nw, ctx, nx=24, stretch=1.05, jitter=0.1)
+    s64 = ctx.solve(nw.solver_cfg(kspace=50, max_iters=300, tol=1e-9,
+                                  precond=nw.PREC_AMG, reuse_preconditioner=0))
+    lv = ctx.amg_levels()
+    assert len(lv) == s64.precond_levels >= 2
+    assert lv[0] == (P.n, A.vals.size)
+    assert all(b[0] < a[0] for a, b in zip(lv, lv[1:])) and lv[-1][0] <= 1000
+    s32 = ctx.solve(nw.solver_cfg(kspace=50, max_iters=300, tol=1e-9, precond=nw.PREC_AMG,
+                                  mode=nw.SOLVE_FP32_REFINE, inner_tol=1e-4,
+                                  reuse_preconditioner=1))
+    assert s32.converged and 0 < s32.setup_ms < s64.setup_ms
+    r = A.rhs - A.spmv(ctx.solution())
+    assert np.linalg.norm(r) <= 1.0001e-9 * np.linalg.norm(A.rhs)
+    s32b = ctx.solve(nw.solver_cfg(kspace=50, max_iters=300, tol=1e-9, precond=nw.PREC_AMG,
+                                   mode=nw.SOLVE_FP32_REFINE, inner_tol=1e-4,
+                                   reuse_preconditioner=1))
+    assert s32b.setup_ms == 0 and s32b.iters == s32.iters
+
+
 def test_amg_beats_jacobi_and_is_reused(nw, ctx):
     """On a 32^3 box stretched 1.05 the AMG V-cycle needs a small, bounded
     number of GMRES iterations; a second solve reuses the hierarchy."""
