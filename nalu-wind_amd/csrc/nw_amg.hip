@@ -227,7 +227,110 @@ __global__ void k_narrow(const double* __restrict__ in, T* __restrict__ out, int
   if (i < n) out[i] = (T)in[i];
 }
 
+// The power iteration's start vector: x_i = 0.5 + u_i, u_i in [0, 1) a
+// splitmix64 hash of i (independent of the launch)
+__global__ void k_pow_start(double* __restrict__ x, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = (uint64_t)i + 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  z ^= z >> 31;
+  x[i] = 0.5 + (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void k_pow_div(double* __restrict__ x, double d, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] /= d;
+}
+
+// y = D^-1 y (y holding A x) and, per block in a fixed tree order, the
+// partial sums of x.y and y.y; with dinv == nullptr only y.y of y as it is
+__global__ __launch_bounds__(kBlock) void k_pow_scale(double* __restrict__ y,
+                                                     const double* __restrict__ dinv,
+                                                     const double* __restrict__ x, int n,
+                                                     double* __restrict__ part)
+{
+  __shared__ double sxy[kBlock], syy[kBlock];
+  const int t = threadIdx.x, i = blockIdx.x * kBlock + t;
+  double a = 0.0, b = 0.0;
+  if (i < n) {
+    double v = y[i];
+    if (dinv) {
+      v *= dinv[i];
+      y[i] = v;
+      a = x[i] * v;
+    }
+    b = v * v;
+  }
+  sxy[t] = a;
+  syy[t] = b;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      sxy[t] += sxy[t + o];
+      syy[t] += syy[t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[2 * blockIdx.x] = sxy[0];
+    part[2 * blockIdx.x + 1] = syy[0];
+  }
+}
+
 }  // namespace
+
+// lambda_max(D^-1 A) of an uploaded level by `iters` power-iteration steps
+// from a fixed start vector: each step normalises x, forms y = D^-1 A x and
+// the Rayleigh quotient x.y with the next norm y.y; block partial sums are
+// added on the host in block order, so the estimate is reproducible.  Uses
+// the level's x and t vectors as work space.
+template <typename M>
+static int device_lmax(DevLevel<double>& L, const M& spmv, int iters, hipStream_t st,
+                       double* out)
+{
+  const int n = L.n;
+  const unsigned nb = grid_for(n, kBlock);
+  DevScratch tmp;
+  double* part = nullptr;
+  NW_REQUIRE(tmp.alloc(&part, 2 * (int64_t)nb) == NW_OK, NW_E_ALLOC,
+             "AMG eigenvalue estimate: device allocation failed");
+  std::vector<double> hp(2 * (size_t)nb);
+  auto sums = [&](double& xy, double& yy) -> int {
+    NW_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(),
+                          hipMemcpyDeviceToHost, st));
+    NW_HIP(hipStreamSynchronize(st));
+    xy = yy = 0.0;
+    for (unsigned b = 0; b < nb; ++b) {
+      xy += hp[2 * b];
+      yy += hp[2 * b + 1];
+    }
+    return NW_OK;
+  };
+  double *x = L.x, *y = L.t;
+  k_pow_start<<<nb, kBlock, 0, st>>>(x, n);
+  k_pow_scale<<<nb, kBlock, 0, st>>>(x, nullptr, nullptr, n, part);
+  NW_LAUNCH_CHECK();
+  double lam = 1.0, nx2 = 0.0, unused = 0.0;
+  int rc = sums(unused, nx2);
+  if (rc) return rc;
+  for (int it = 0; it < iters; ++it) {
+    const double nx = std::sqrt(nx2);
+    if (!(nx > 0)) break;
+    k_pow_div<<<nb, kBlock, 0, st>>>(x, nx, n);
+    spmv(x, y);
+    k_pow_scale<<<nb, kBlock, 0, st>>>(y, L.dinv, x, n, part);
+    NW_LAUNCH_CHECK();
+    rc = sums(lam, nx2);
+    if (rc) return rc;
+    std::swap(x, y);
+  }
+  *out = lam;
+  return NW_OK;
+}
 
 // Uploads a host CSR matrix.  Rows of more than kSellMaxAvgRow entries on
 // average stay in CSR (G lanes per row); shorter ones are packed into sliced
@@ -406,6 +509,7 @@ int amg_setup(Ctx* c, AmgDevice** out)
                         hipMemcpyDeviceToHost, c->stream));
   NW_HIP(hipStreamSynchronize(c->stream));
   amg::Options o;
+  o.smoother_lmax = false;  // estimated on the device after the upload
   amg::Hierarchy h;
   const int rc = amg::build(std::move(A), o, h);
   NW_REQUIRE(rc == 0, NW_E_ARG, "AMG setup failed (%d): coarse level too "
@@ -418,7 +522,22 @@ int amg_setup(Ctx* c, AmgDevice** out)
     a->rows.push_back(L.A.nrows);
     a->nnz.push_back(L.A.nnz());
   }
-  const int urc = upload_hier(c, h, a->d);
+  int urc = upload_hier(c, h, a->d);
+  // the smoothers' lambda_max(D^-1 A) on the device, every level but the
+  // coarsest (solved directly); the fine level's values first
+  if (!urc) {
+    DevLevel<double>& L0 = a->d.lv[0];
+    const Sell& sl = g.sell;
+    k_sell_values<double><<<grid_for(sl.len, kBlock), kBlock, 0, c->stream>>>(
+      sl.slot, sl.ci, g.vals, nullptr, L0.A.v, sl.len);
+    if (hipGetLastError() != hipSuccess) urc = NW_E_HIP;
+  }
+  for (int l = 0; !urc && l + 1 < a->nlev; ++l) {
+    DevLevel<double>& L = a->d.lv[l];
+    urc = device_lmax(L, [&](const double* x, double* y) {
+      mat_spmv<double>(L.A, x, y, 1.0, c->stream);
+    }, o.power_iters, c->stream, &L.lmax);
+  }
   if (urc) {
     amg_free(a);
     return urc;

@@ -342,47 +342,56 @@ void coarsen(const Csr& A, const Options& o, Csr& P)
   const std::vector<double> dfinv = inv_diagonal(F);
   const double lf = power_lmax(F, dfinv, o.power_iters);
   const double omega = (4.0 / 3.0) / (lf > 0 ? lf : 1.0);
-  // P = P0 - omega D_F^-1 A_F P0, one row per node: each row's terms (at
-  // most 1 + its F entries) are merged by aggregate into a staging layout
-  // with room for that many, then packed
+  // P = P0 - omega D_F^-1 A_F P0, one row per node: a row's terms (its
+  // aggregate's indicator and a smoothing term per F entry into an
+  // aggregate) are merged by aggregate; the rows are formed twice, once to
+  // count and once to write, which costs less than staging them
   P.nrows = n;
   P.ncols = na;
-  uvec<int64_t> ub(n + 1, 0);
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < n; ++i) ub[i + 1] = 1 + F.rp[i + 1] - F.rp[i];
-  for (int64_t i = 0; i < n; ++i) ub[i + 1] += ub[i];
-  uvec<std::pair<int32_t, double>> stage(ub[n]);
   P.rp.assign(n + 1, 0);
-#pragma omp parallel for schedule(dynamic, 4096)
-  for (int64_t i = 0; i < n; ++i) {
-    if (agg[i] < 0) continue;
-    auto* r = stage.data() + ub[i];
-    size_t m = 0;
-    r[m++] = {agg[i], 1.0 / sq[agg[i]]};
+  using Term = std::pair<int32_t, double>;
+  auto row_terms = [&](int64_t i, std::vector<Term>& r) {
+    r.clear();
+    r.push_back({agg[i], 1.0 / sq[agg[i]]});
     for (int64_t k = F.rp[i]; k < F.rp[i + 1]; ++k) {
       const int32_t j = F.ci[k];
       if (agg[j] < 0) continue;
-      r[m++] = {agg[j], -omega * dfinv[i] * F.v[k] / sq[agg[j]]};
+      r.push_back({agg[j], -omega * dfinv[i] * F.v[k] / sq[agg[j]]});
     }
-    std::sort(r, r + m, [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::sort(r.begin(), r.end(), [](const Term& a, const Term& b) { return a.first < b.first; });
     size_t w = 0;
-    for (size_t q = 0; q < m; ++q) {
+    for (size_t q = 0; q < r.size(); ++q) {
       if (w > 0 && r[w - 1].first == r[q].first)
         r[w - 1].second += r[q].second;
       else
         r[w++] = r[q];
     }
-    P.rp[i + 1] = (int64_t)w;
+    r.resize(w);
+  };
+#pragma omp parallel
+  {
+    std::vector<Term> r;
+#pragma omp for schedule(dynamic, 4096)
+    for (int64_t i = 0; i < n; ++i) {
+      if (agg[i] < 0) continue;
+      row_terms(i, r);
+      P.rp[i + 1] = (int64_t)r.size();
+    }
   }
   for (int64_t i = 0; i < n; ++i) P.rp[i + 1] += P.rp[i];
   P.ci.resize(P.rp[n]);
   P.v.resize(P.rp[n]);
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < n; ++i) {
-    const auto* r = stage.data() + ub[i];
-    for (int64_t p = P.rp[i]; p < P.rp[i + 1]; ++p) {
-      P.ci[p] = r[p - P.rp[i]].first;
-      P.v[p] = r[p - P.rp[i]].second;
+#pragma omp parallel
+  {
+    std::vector<Term> r;
+#pragma omp for schedule(dynamic, 4096)
+    for (int64_t i = 0; i < n; ++i) {
+      if (agg[i] < 0) continue;
+      row_terms(i, r);
+      for (size_t q = 0; q < r.size(); ++q) {
+        P.ci[P.rp[i] + q] = r[q].first;
+        P.v[P.rp[i] + q] = r[q].second;
+      }
     }
   }
 }
@@ -414,7 +423,7 @@ int build(Csr A0, const Options& o, Hierarchy& h)
     Level& L = h.levels[l];
     const int64_t n = L.A.nrows;
     L.dinv = inv_diagonal(L.A);
-    L.lmax = power_lmax(L.A, L.dinv, o.power_iters);
+    if (o.smoother_lmax) L.lmax = power_lmax(L.A, L.dinv, o.power_iters);
     if (n <= o.coarse_max || l + 1 >= o.max_levels) break;
     Csr P;
     coarsen(L.A, o, P);

@@ -65,6 +65,11 @@ struct Options {
   double cheb_ratio = 20.0;   // smoother interval [lmax/ratio, lmax]
   int cheb_degree = 2;
   int power_iters = 15;
+  // lambda_max(D^-1 A) of each level for its Chebyshev smoother, by power
+  // iteration on the host; false leaves Level::lmax at 0 for the caller to
+  // estimate where the operators live (the prolongator damping still uses
+  // the host's estimate on the filtered operator)
+  bool smoother_lmax = true;
 };
 
 struct Hierarchy {
