@@ -61,8 +61,11 @@ struct DevMat {
   const int64_t* sp = nullptr;
   const int32_t* rp = nullptr;
   const int32_t* ci = nullptr;
+  const int32_t* dia_w = nullptr;  // offset form of the fine level's slices
+  const int32_t* dia_off = nullptr;
   T* v = nullptr;
   int64_t nv = 0;  // stored values (padding included)
+  SellView view() const { return {sp, ci, dia_w, dia_off}; }
   int64_t* own_sp = nullptr;
   int32_t* own_rp = nullptr;
   int32_t* own_ci = nullptr;
@@ -79,8 +82,8 @@ static void mat_spmv(const DevMat<T>& M, const T* x, T* y, T alpha,
                      hipStream_t st)
 {
   if (M.sp)
-    k_sell_spmv<T><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(M.sp, M.ci, M.v,
-                                                             x, y, M.n, alpha);
+    k_sell_spmv<T><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(M.view(), M.v, x, y, M.n,
+                                                             alpha);
   else
     NW_ROW_LAUNCH(k_spmv, T, M.G, M.n, st, M.rp, M.ci, M.v, x, y, M.n, alpha);
 }
@@ -89,8 +92,8 @@ template <typename T>
 static void mat_spmv_add(const DevMat<T>& M, const T* x, T* y, hipStream_t st)
 {
   if (M.sp)
-    k_sell_spmv_add<T><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(M.sp, M.ci,
-                                                                 M.v, x, y, M.n);
+    k_sell_spmv_add<T><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(M.view(), M.v, x, y,
+                                                                 M.n);
   else
     NW_ROW_LAUNCH(k_spmv_add, T, M.G, M.n, st, M.rp, M.ci, M.v, x, y, M.n);
 }
@@ -100,8 +103,8 @@ static void mat_resid(const DevMat<T>& M, const T* x, const T* b, T* r,
                       hipStream_t st)
 {
   if (M.sp)
-    k_sell_resid<T><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(M.sp, M.ci, M.v,
-                                                              x, b, r, M.n);
+    k_sell_resid<T><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(M.view(), M.v, x, b, r,
+                                                              M.n);
   else
     NW_ROW_LAUNCH(k_resid, T, M.G, M.n, st, M.rp, M.ci, M.v, x, b, r, M.n);
 }
@@ -114,10 +117,10 @@ static void mat_cheb_step(const DevMat<T>& M, const T* z, const T* v,
 {
   if (M.sp && add)
     k_sell_cheb_step<T, true><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(
-      M.sp, M.ci, M.v, z, v, dinv, d, zo, M.n, c1, c2);
+      M.view(), M.v, z, v, dinv, d, zo, M.n, c1, c2);
   else if (M.sp)
     k_sell_cheb_step<T><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(
-      M.sp, M.ci, M.v, z, v, dinv, d, zo, M.n, c1, c2);
+      M.view(), M.v, z, v, dinv, d, zo, M.n, c1, c2);
   else if (add)
     NW_ROW_LAUNCH_ADD(k_cheb_step, T, M.G, M.n, st, M.rp, M.ci, M.v, z, v,
                       dinv, d, zo, M.n, c1, c2);
@@ -134,7 +137,7 @@ static void mat_resid_cheb_init(const DevMat<T>& M, const T* x, const T* b,
 {
   if (M.sp)
     k_sell_resid_cheb_init<T><<<grid_for(M.n, kBlock), kBlock, 0, st>>>(
-      M.sp, M.ci, M.v, x, b, r, dinv, d, z, M.n, inv_theta);
+      M.view(), M.v, x, b, r, dinv, d, z, M.n, inv_theta);
   else
     NW_ROW_LAUNCH(k_resid_cheb_init, T, M.G, M.n, st, M.rp, M.ci, M.v, x, b,
                   r, dinv, d, z, M.n, inv_theta);
@@ -435,6 +438,8 @@ static int upload_hier(Ctx* c, const amg::Hierarchy& h, DevHier<double>& H)
       L.A.n = L.n;
       L.A.sp = c->g.sell.sp;
       L.A.ci = c->g.sell.ci;
+      L.A.dia_w = c->g.sell.dia_w;
+      L.A.dia_off = c->g.sell.dia_off;
       L.A.nv = c->g.sell.len;
       NW_HIP(hipMalloc(&L.A.v, sizeof(double) * std::max<int64_t>(c->g.sell.len, 1)));
     } else {
@@ -473,6 +478,8 @@ static int narrow_hier(Ctx* c, const DevHier<double>& D, DevHier<float>& H)
       L.A.n = S.A.n;
       L.A.sp = S.A.sp;
       L.A.ci = S.A.ci;
+      L.A.dia_w = S.A.dia_w;
+      L.A.dia_off = S.A.dia_off;
       L.A.nv = S.A.nv;
       NW_HIP(hipMalloc(&L.A.v, sizeof(float) * std::max<int64_t>(S.A.nv, 1)));
     } else {

@@ -100,6 +100,8 @@ static void free_graph(Graph& g)
   (void)hipFree(g.sell.sp);
   (void)hipFree(g.sell.ci);
   (void)hipFree(g.sell.slot);
+  (void)hipFree(g.sell.dia_w);
+  (void)hipFree(g.sell.dia_off);
   (void)hipFree(g.skip);
   g = Graph();
 }

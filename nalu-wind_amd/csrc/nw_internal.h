@@ -84,12 +84,33 @@ struct alignas(32) SchedChunk {
 // slice's width being its longest row; shorter rows (and the rows past n in
 // the last slice) are padded with column = the row itself (0 past n) and
 // value 0.  slot[p] is the CSR entry stored at position p, -1 for padding.
+//
+// Offset form (square matrices whose slices repeat a stencil, as a
+// structured mesh's do): a slice whose rows' column offsets (column - row)
+// form a union of at most kDiaMax values, all inside the matrix for every
+// row, stores position j of each row at the slice's j-th offset
+// (dia_off[k kDiaMax + j], dia_w[k] of them; value 0 where the row has no
+// entry there), so that a product computes its columns as row + offset
+// instead of loading them.  ci still holds those columns, for readers
+// that do not use the offsets.  dia_w[k] = 0 for slices in the plain form.
+constexpr int kDiaMax = 16;
 struct Sell {
   int n = 0, nslices = 0;
-  int64_t len = 0;          // positions, padding included
-  int64_t* sp = nullptr;    // [nslices + 1]
-  int32_t* ci = nullptr;    // [len]
-  int32_t* slot = nullptr;  // [len]
+  int64_t len = 0;              // positions, padding included
+  int64_t* sp = nullptr;        // [nslices + 1]
+  int32_t* ci = nullptr;        // [len]
+  int32_t* slot = nullptr;      // [len]
+  int32_t* dia_w = nullptr;     // [nslices], null when not built
+  int32_t* dia_off = nullptr;   // [nslices][kDiaMax]
+  int64_t dia_slices = 0;       // slices in offset form
+};
+
+// What the sliced ELL products read of a structure.
+struct SellView {
+  const int64_t* sp;
+  const int32_t* ci;
+  const int32_t* dia_w;    // null: no slice in offset form
+  const int32_t* dia_off;
 };
 
 struct Graph {

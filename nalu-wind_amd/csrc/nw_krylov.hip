@@ -29,6 +29,7 @@
 // [lmax/ratio, lmax] (Saad, Iterative Methods for Sparse Linear Systems,
 // 2nd ed., Alg. 12.1), lmax from a seeded power iteration.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -290,14 +291,54 @@ __global__ __launch_bounds__(kBlock) void k_sell_width(
   if ((threadIdx.x & (kWave - 1)) == 0 && (t >> 6) < nslices) width[t >> 6] = len;
 }
 
+// The offset form of each slice (a wave per slice): the sorted union of
+// its rows' column offsets (column - row), taken one smallest offset at a
+// time over the wave, when it has at most kDiaMax values and row + offset
+// lies in [0, n) for every row of the slice; dia_w[k] = its size, or 0.
+__global__ __launch_bounds__(kBlock) void k_dia_pattern(
+  const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, int n, int nslices,
+  int32_t* __restrict__ dia_w, int32_t* __restrict__ dia_off)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t >> 6;
+  if (k >= nslices) return;  // whole waves
+  const int lane = t & 63;
+  const bool valid = t < n;
+  int pos = valid ? rp[t] : 0;
+  const int end = valid ? rp[t + 1] : 0;
+  int cnt = 0;
+  bool ok = true;
+  for (;;) {
+    const int mine = pos < end ? ci[pos] - t : INT_MAX;
+    int cand = mine;
+    for (int o = kWave / 2; o > 0; o >>= 1) cand = min(cand, __shfl_xor(cand, o, kWave));
+    if (cand == INT_MAX) break;  // the same on every lane
+    if (cnt == kDiaMax) {
+      ok = false;
+      break;
+    }
+    const int64_t col = (int64_t)t + cand;
+    if (__ballot(valid && (col < 0 || col >= n)) != 0) {
+      ok = false;
+      break;
+    }
+    if (lane == 0) dia_off[(int64_t)k * kDiaMax + cnt] = cand;
+    ++cnt;
+    if (mine == cand) ++pos;
+  }
+  if (lane == 0) dia_w[k] = ok ? cnt : 0;
+}
+
 // columns and CSR slots of every sliced-ELL position (a lane per row);
 // padding positions take the row's own index as their column (square
 // matrices) or, with pad_last, the row's last column (column 0 for an
-// empty row), which stays inside any matrix's columns
+// empty row), which stays inside any matrix's columns.  A slice in offset
+// form puts each row's entry with offset dia_off[j] at position j.
 __global__ __launch_bounds__(kBlock) void k_sell_struct(
   const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, int n,
   const int64_t* __restrict__ sp, int nslices, int32_t* __restrict__ sci,
-  int32_t* __restrict__ sslot, bool pad_last)
+  int32_t* __restrict__ sslot, bool pad_last, const int32_t* __restrict__ dia_w,
+  const int32_t* __restrict__ dia_off)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = t >> 6;
@@ -306,6 +347,26 @@ __global__ __launch_bounds__(kBlock) void k_sell_struct(
   const int w = (int)((sp[k + 1] - sp[k]) >> 6);
   const int r0 = t < n ? rp[t] : 0;
   const int len = t < n ? rp[t + 1] - r0 : 0;
+  if (dia_w && dia_w[k] > 0) {
+    int q = r0;  // the row's entries, in column (= offset) order
+    for (int j = 0; j < w; ++j) {
+      const int64_t p = b + 64 * (int64_t)j;
+      const int off = dia_off[(int64_t)k * kDiaMax + j];
+      if (t < n) {
+        sci[p] = t + off;
+        if (q < r0 + len && ci[q] - t == off) {
+          sslot[p] = q;
+          ++q;
+        } else {
+          sslot[p] = -1;
+        }
+      } else {
+        sci[p] = 0;
+        sslot[p] = -1;
+      }
+    }
+    return;
+  }
   for (int j = 0; j < w; ++j) {
     const int64_t p = b + 64 * (int64_t)j;
     if (j < len) {
@@ -321,30 +382,53 @@ __global__ __launch_bounds__(kBlock) void k_sell_struct(
 }  // namespace
 
 int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
-               hipStream_t st, bool pad_last)
+               hipStream_t st, bool pad_last, bool offsets)
 {
   s.n = n;
   s.nslices = (n + kWave - 1) / kWave;
   const int ns = s.nslices;
+  const unsigned grid = grid_for((int64_t)ns * kWave, kBlock);
   int32_t* dw = nullptr;
   NW_HIP(hipMalloc(&dw, sizeof(int32_t) * std::max(ns, 1)));
-  k_sell_width<<<grid_for((int64_t)ns * kWave, kBlock), kBlock, 0, st>>>(rp, n, ns, dw);
-  std::vector<int32_t> w(ns);
-  hipError_t e = hipMemcpyAsync(w.data(), dw, sizeof(int32_t) * ns,
-                                hipMemcpyDeviceToHost, st);
+  k_sell_width<<<grid, kBlock, 0, st>>>(rp, n, ns, dw);
+  std::vector<int32_t> w(ns), dia(offsets ? ns : 0);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && offsets) {
+    e = hipMalloc(&s.dia_w, sizeof(int32_t) * std::max(ns, 1));
+    if (e == hipSuccess)
+      e = hipMalloc(&s.dia_off, sizeof(int32_t) * (size_t)std::max(ns, 1) * kDiaMax);
+    if (e == hipSuccess) {
+      k_dia_pattern<<<grid, kBlock, 0, st>>>(rp, ci, n, ns, s.dia_w, s.dia_off);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dia.data(), s.dia_w, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(w.data(), dw, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   (void)hipFree(dw);
   NW_HIP(e);
   std::vector<int64_t> sp(ns + 1, 0);
-  for (int k = 0; k < ns; ++k) sp[k + 1] = sp[k] + (int64_t)kWave * w[k];
+  s.dia_slices = 0;
+  for (int k = 0; k < ns; ++k) {
+    const int wk = offsets && dia[k] > 0 ? dia[k] : w[k];  // the union holds every row
+    s.dia_slices += offsets && dia[k] > 0;
+    sp[k + 1] = sp[k] + (int64_t)kWave * wk;
+  }
+  if (offsets && s.dia_slices == 0) {  // nothing to gain: the plain form
+    (void)hipFree(s.dia_w);
+    (void)hipFree(s.dia_off);
+    s.dia_w = s.dia_off = nullptr;
+  }
   s.len = sp[ns];
   NW_HIP(hipMalloc(&s.sp, sizeof(int64_t) * (ns + 1)));
   NW_HIP(hipMalloc(&s.ci, sizeof(int32_t) * std::max<int64_t>(s.len, 1)));
   NW_HIP(hipMalloc(&s.slot, sizeof(int32_t) * std::max<int64_t>(s.len, 1)));
   NW_HIP(hipMemcpyAsync(s.sp, sp.data(), sizeof(int64_t) * (ns + 1),
                         hipMemcpyHostToDevice, st));
-  k_sell_struct<<<grid_for((int64_t)ns * kWave, kBlock), kBlock, 0, st>>>(
-    rp, ci, n, s.sp, ns, s.ci, s.slot, pad_last);
+  k_sell_struct<<<grid, kBlock, 0, st>>>(rp, ci, n, s.sp, ns, s.ci, s.slot, pad_last,
+                                         s.dia_w, s.dia_off);
   NW_LAUNCH_CHECK();
   NW_HIP(hipStreamSynchronize(st));
   return NW_OK;
@@ -455,7 +539,7 @@ struct Solve {
   {
     const int n = (int)k->n;
     k_sell_spmv<T><<<grid_for(n, kBlock), kBlock, 0, st>>>(
-      c->g.sell.sp, c->g.sell.ci, va, x, y, n, alpha);
+      sell_view(c->g.sell), va, x, y, n, alpha);
     NW_LAUNCH_CHECK();
     return NW_OK;
   }
@@ -476,7 +560,7 @@ struct Solve {
   {
     const int n = (int)k->n;
     k_sell_cheb_step<T><<<grid_for(n, kBlock), kBlock, 0, st>>>(
-      c->g.sell.sp, c->g.sell.ci, va, z, v, dinv, d, zo, n, c1, c2);
+      sell_view(c->g.sell), va, z, v, dinv, d, zo, n, c1, c2);
     NW_LAUNCH_CHECK();
     return NW_OK;
   }
@@ -845,7 +929,8 @@ int nw_linsys_solve(nw_ctx* c, const nw_solver_cfg* cfg, nw_solve_stats* out)
     NW_HIP(hipEventCreate(&k->ev1));
   }
   if (!c->g.sell.ci) {
-    int rc0 = sell_build(c->g.row_ptr, c->g.cols, (int)n, c->g.sell, c->stream);
+    int rc0 = sell_build(c->g.row_ptr, c->g.cols, (int)n, c->g.sell, c->stream,
+                         false, true);
     if (rc0) return rc0;
   }
   const int64_t slen = c->g.sell.len;

@@ -159,24 +159,42 @@ __global__ __launch_bounds__(kBlock) void k_resid_cheb_init(
 // consecutive values and columns, and all loads of a row are independent
 // (issued eight at a time), so a wave keeps a whole row's gathers in
 // flight instead of walking a dependent chain.  Padding reads x at a valid
-// column with value 0.  The row sum runs over j in order.
+// column with value 0.  The row sum runs over j in order.  A slice in
+// offset form (A.dia_w[k] > 0) takes its columns as row + the slice's
+// offsets, which are the same for the whole wave (scalar loads), and reads
+// no column array.
 template <typename T>
-__device__ inline T sell_dot(const int64_t* __restrict__ sp,
-                             const int32_t* __restrict__ ci,
-                             const T* __restrict__ va, const T* __restrict__ x,
-                             int row)
+__device__ inline T sell_dot(const SellView& A, const T* __restrict__ va,
+                             const T* __restrict__ x, int row)
 {
-  const int k = row >> 6;
-  const int64_t b = sp[k] + (row & 63);
-  const int w = (int)((sp[k + 1] - sp[k]) >> 6);
+  // rows of one slice are one wave: the slice index is wave-uniform
+  const int k = __builtin_amdgcn_readfirstlane(row >> 6);
+  const int64_t b = A.sp[k] + (row & 63);
+  const int w = (int)((A.sp[k + 1] - A.sp[k]) >> 6);
   T s = 0;
+  if (A.dia_w && A.dia_w[k] > 0) {
+    const int32_t* off = A.dia_off + (int64_t)k * kDiaMax;
+    for (int j0 = 0; j0 < w; j0 += 8) {
+      int c[8];
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = min(j0 + u, w - 1);
+        c[u] = row + off[j];
+        v[u] = j0 + u < w ? va[b + 64 * (int64_t)j] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u] * x[c[u]];
+    }
+    return s;
+  }
   for (int j0 = 0; j0 < w; j0 += 8) {
     int c[8];
     T v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int j = min(j0 + u, w - 1);
-      c[u] = ci[b + 64 * (int64_t)j];
+      c[u] = A.ci[b + 64 * (int64_t)j];
       v[u] = j0 + u < w ? va[b + 64 * (int64_t)j] : (T)0;
     }
 #pragma unroll
@@ -188,38 +206,35 @@ __device__ inline T sell_dot(const int64_t* __restrict__ sp,
 // y = alpha * (A x)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sell_spmv(
-  const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
-  const T* __restrict__ va, const T* __restrict__ x, T* __restrict__ y, int n,
+  SellView A, const T* __restrict__ va, const T* __restrict__ x, T* __restrict__ y, int n,
   T alpha)
 {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
-  y[row] = alpha * sell_dot(sp, ci, va, x, row);
+  y[row] = alpha * sell_dot(A, va, x, row);
 }
 
 // y += A x
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sell_spmv_add(
-  const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
-  const T* __restrict__ va, const T* __restrict__ x, T* __restrict__ y, int n)
+  SellView A, const T* __restrict__ va, const T* __restrict__ x, T* __restrict__ y, int n)
 {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
-  y[row] += sell_dot(sp, ci, va, x, row);
+  y[row] += sell_dot(A, va, x, row);
 }
 
 // Chebyshev step: r = v - A z; d = c1 d + c2 dinv r; zo = z + d (with ADD,
 // zo += z + d).
 template <typename T, bool ADD = false>
 __global__ __launch_bounds__(kBlock) void k_sell_cheb_step(
-  const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
-  const T* __restrict__ va, const T* __restrict__ z, const T* __restrict__ v,
+  SellView A, const T* __restrict__ va, const T* __restrict__ z, const T* __restrict__ v,
   const T* __restrict__ dinv, T* __restrict__ d, T* __restrict__ zo, int n,
   T c1, T c2)
 {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
-  const T s = sell_dot(sp, ci, va, z, row);
+  const T s = sell_dot(A, va, z, row);
   const T dn = c1 * d[row] + c2 * dinv[row] * (v[row] - s);
   d[row] = dn;
   if (ADD)
@@ -231,26 +246,24 @@ __global__ __launch_bounds__(kBlock) void k_sell_cheb_step(
 // r = b - A x
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sell_resid(
-  const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
-  const T* __restrict__ va, const T* __restrict__ x, const T* __restrict__ b,
+  SellView A, const T* __restrict__ va, const T* __restrict__ x, const T* __restrict__ b,
   T* __restrict__ r, int n)
 {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
-  r[row] = b[row] - sell_dot(sp, ci, va, x, row);
+  r[row] = b[row] - sell_dot(A, va, x, row);
 }
 
 // r = b - A x and d = z = dinv r inv_theta (k_sell_resid + k_cheb_init)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sell_resid_cheb_init(
-  const int64_t* __restrict__ sp, const int32_t* __restrict__ ci,
-  const T* __restrict__ va, const T* __restrict__ x, const T* __restrict__ b,
+  SellView A, const T* __restrict__ va, const T* __restrict__ x, const T* __restrict__ b,
   T* __restrict__ r, const T* __restrict__ dinv, T* __restrict__ d,
   T* __restrict__ z, int n, T inv_theta)
 {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
-  const T ri = b[row] - sell_dot(sp, ci, va, x, row);
+  const T ri = b[row] - sell_dot(A, va, x, row);
   r[row] = ri;
   const T di = dinv[row] * ri * inv_theta;
   d[row] = di;
@@ -274,9 +287,12 @@ __global__ void k_sell_values(const int32_t* __restrict__ slot,
 // Builds the sliced ELL structure of the CSR graph (rp, ci; device arrays)
 // with n rows.  Padding positions read the row's own index as column
 // (square matrices), or with pad_last the row's last column, which keeps
-// them inside the columns of a rectangular matrix.
+// them inside the columns of a rectangular matrix.  With offsets (square
+// matrices only) the slices that qualify are stored in offset form.
 int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
-               hipStream_t st, bool pad_last = false);
+               hipStream_t st, bool pad_last = false, bool offsets = false);
+
+inline SellView sell_view(const Sell& s) { return {s.sp, s.ci, s.dia_w, s.dia_off}; }
 
 // launch helper: dispatch a G-templated row kernel by lanes-per-row
 // (kernel<T, G>; NW_ROW_LAUNCH_ADD: kernel<T, G, true>)
