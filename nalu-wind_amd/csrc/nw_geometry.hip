@@ -4,10 +4,12 @@
 // coordinates.
 //
 //   k_elem_scs,    one lane per element: the element's sub-control-surface
-//   k_elem_scv     (SCS) area vectors and sub-control volumes (SCV) into
-//                  element-minor buffers (entry k of element el at
+//   k_elem_scv,    (SCS) area vectors and sub-control volumes (SCV) into
+//   k_elem_scv_hex element-minor buffers (entry k of element el at
 //                  k * nelem + el, so a wave's stores are coalesced); each
 //                  sub-point is formed from the corners where it is used;
+//                  the Hex8 SCVs by a kernel of their own that evaluates
+//                  each face of the sub-volumes once;
 //   k_edge_area    one lane per edge: sums +-SCS over the edge's surfaces in
 //                  element order, + when the surface's L node is the edge's
 //                  node[0] (GeometryInteriorAlg.C:209-219);
@@ -59,6 +61,26 @@ struct HexTopo {
     {12, 9, 2, 10, 26, 24, 22, 23},  {11, 12, 10, 3, 25, 26, 23, 21},
     {19, 20, 26, 25, 4, 13, 17, 16}, {20, 18, 24, 26, 13, 5, 14, 17},
     {26, 24, 22, 23, 17, 14, 6, 15}, {25, 26, 23, 21, 16, 17, 15, 7}};
+  // The 36 distinct faces of the eight sub-hexahedra: the 24 quarters of
+  // the element's faces and the 12 sub-control surfaces, each as its four
+  // points in the outward order of the sub-hexahedron listed first, that
+  // sub-hexahedron, and the one on its other side (-1: none).  Derived
+  // from scv[][] and the faces of grandy_volume (each cycle as its
+  // triangles run); a face's flux counts + for the first and - for the
+  // second.
+  static constexpr int8_t scv_face[36][6] = {
+    {0, 11, 12, 8, 0, -1}, {19, 20, 26, 25, 0, 4}, {0, 8, 20, 19, 0, -1},
+    {12, 11, 25, 26, 0, 3}, {8, 12, 26, 20, 0, 1}, {0, 19, 25, 11, 0, -1},
+    {8, 12, 9, 1, 1, -1}, {20, 18, 24, 26, 1, 5}, {8, 1, 18, 20, 1, -1},
+    {9, 12, 26, 24, 1, 2}, {1, 9, 24, 18, 1, -1}, {12, 10, 2, 9, 2, -1},
+    {26, 24, 22, 23, 2, 6}, {2, 10, 23, 22, 2, -1}, {9, 2, 22, 24, 2, -1},
+    {12, 26, 23, 10, 2, 3}, {11, 3, 10, 12, 3, -1}, {25, 26, 23, 21, 3, 7},
+    {10, 3, 21, 23, 3, -1}, {11, 25, 21, 3, 3, -1}, {4, 13, 17, 16, 4, -1},
+    {19, 20, 13, 4, 4, -1}, {26, 25, 16, 17, 4, 7}, {20, 26, 17, 13, 4, 5},
+    {19, 4, 16, 25, 4, -1}, {13, 5, 14, 17, 5, -1}, {20, 18, 5, 13, 5, -1},
+    {24, 26, 17, 14, 5, 6}, {18, 24, 14, 5, 5, -1}, {17, 14, 6, 15, 6, -1},
+    {22, 23, 15, 6, 6, -1}, {24, 22, 6, 14, 6, -1}, {26, 17, 15, 23, 6, 7},
+    {16, 17, 15, 7, 7, -1}, {23, 21, 7, 15, 7, -1}, {25, 16, 7, 21, 7, -1}};
 };
 
 // Tet4 (src/master_element/Tet4CVFEM.C:243-343, 522-618)
@@ -286,6 +308,51 @@ __device__ inline void elem_scv(int64_t el, int64_t nelem,
   }
 }
 
+// The eight Hex8 sub-control volumes by the divergence theorem over their
+// faces, as grandy_volume takes them (each face four triangles about its
+// centroid), but with each of the 36 distinct faces evaluated once: a
+// face between two sub-volumes adds its flux to one and subtracts it from
+// the other: 36 faces instead of 48, about 30 % fewer operations.
+__device__ inline void elem_scv_hex(int64_t el, int64_t nelem,
+                                    const int32_t* __restrict__ conn,
+                                    const double* __restrict__ coords, int64_t N,
+                                    double* __restrict__ scv)
+{
+  double c[8][3];
+  corners<HexTopo>(el, conn, coords, N, c);
+  // the volumes do not depend on the origin: taking it at the element's
+  // centroid keeps the face fluxes near the size of the volumes (absolute
+  // coordinates make them larger by the distance to the origin over the
+  // element's size, and the volumes lose that many digits to cancellation)
+  for (int d = 0; d < 3; ++d) {
+    const double o = 0.125 * (c[0][d] + c[1][d] + c[2][d] + c[3][d] + c[4][d] + c[5][d] +
+                              c[6][d] + c[7][d]);
+    for (int a = 0; a < 8; ++a) c[a][d] -= o;
+  }
+  double vol[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma clang loop unroll(full)
+  for (int k = 0; k < 36; ++k) {
+    double q[4][3], m[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sub_point<HexTopo>(c, HexTopo::scv_face[k][i], q[i]);
+    for (int d = 0; d < 3; ++d) m[d] = 0.25 * (q[0][d] + q[1][d] + q[2][d] + q[3][d]);
+    double phi = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double* a = q[i];
+      const double* b = q[(i + 1) & 3];
+      const double n0 = (b[1] - a[1]) * (m[2] - a[2]) - (m[1] - a[1]) * (b[2] - a[2]);
+      const double n1 = (m[0] - a[0]) * (b[2] - a[2]) - (b[0] - a[0]) * (m[2] - a[2]);
+      const double n2 = (b[0] - a[0]) * (m[1] - a[1]) - (m[0] - a[0]) * (b[1] - a[1]);
+      phi += (a[0] + b[0] + m[0]) * n0 + (a[1] + b[1] + m[1]) * n1 + (a[2] + b[2] + m[2]) * n2;
+    }
+    vol[HexTopo::scv_face[k][4]] += phi;
+    if (HexTopo::scv_face[k][5] >= 0) vol[HexTopo::scv_face[k][5]] -= phi;
+  }
+#pragma unroll
+  for (int v = 0; v < 8; ++v) scv[v * nelem + el] = vol[v] / 18.0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_elem_scs(
   int64_t nelem, const int32_t* __restrict__ type,
   const int32_t* __restrict__ conn, const double* __restrict__ coords,
@@ -313,13 +380,23 @@ __global__ __launch_bounds__(kBlock) void k_elem_scv(
   if (el >= nelem) return;
   const int t = type[el];
   if (t == 0)
-    elem_scv<HexTopo>(el, nelem, conn, coords, N, scv);
+    return;  // k_elem_scv_hex
   else if (t == 1)
     elem_scv<TetTopo>(el, nelem, conn, coords, N, scv);
   else if (t == 2)
     elem_scv<WedTopo>(el, nelem, conn, coords, N, scv);
   else
     elem_scv<PyrTopo>(el, nelem, conn, coords, N, scv);
+}
+
+__global__ __launch_bounds__(kBlock) void k_elem_scv_hex(
+  int64_t nelem, const int32_t* __restrict__ type,
+  const int32_t* __restrict__ conn, const double* __restrict__ coords,
+  int64_t N, double* __restrict__ scv)
+{
+  const int64_t el = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= nelem || type[el] != 0) return;
+  elem_scv_hex(el, nelem, conn, coords, N, scv);
 }
 
 // areav[3 e + d] = sum over the edge's surfaces (element order) of
@@ -414,6 +491,7 @@ int nw_mesh_elements(nw_ctx* c, int64_t nelem, const int32_t* type,
   g->nelem = nelem;
   std::vector<int32_t> t(nelem, 0);
   if (type) t.assign(type, type + nelem);
+  g->nhex = std::count(t.begin(), t.end(), 0);
   int r = 0;
   if ((r = upload(&g->type, t.data(), nelem, c->stream)) ||
       (r = upload(&g->conn, conn, 8 * nelem, c->stream)) ||
@@ -446,8 +524,12 @@ int nw_compute_geometry(nw_ctx* c)
   if (ne > 0) {
     k_elem_scs<<<grid_for(ne, kBlock), kBlock, 0, c->stream>>>(
       ne, g->type, g->conn, coords, c->nnode, scs);
-    k_elem_scv<<<grid_for(ne, kBlock), kBlock, 0, c->stream>>>(
-      ne, g->type, g->conn, coords, c->nnode, scv);
+    if (g->nhex > 0)
+      k_elem_scv_hex<<<grid_for(ne, kBlock), kBlock, 0, c->stream>>>(
+        ne, g->type, g->conn, coords, c->nnode, scv);
+    if (g->nhex < ne)
+      k_elem_scv<<<grid_for(ne, kBlock), kBlock, 0, c->stream>>>(
+        ne, g->type, g->conn, coords, c->nnode, scv);
   }
   if (c->nedge > 0)
     k_edge_area<<<grid_for(c->nedge, kBlock), kBlock, 0, c->stream>>>(

@@ -167,7 +167,7 @@ struct Mpnn;       // learned initial guess (nw_mpnn.hip)
 // Elements of the mesh and the incidence lists of the device geometry
 // (nw_mesh_elements, nw_compute_geometry); all device arrays.
 struct Geom {
-  int64_t nelem = 0;
+  int64_t nelem = 0, nhex = 0;
   int32_t* type = nullptr;   // [nelem] 0 Hex8, 1 Tet4, 2 Wed6, 3 Pyr5
   int32_t* conn = nullptr;   // [nelem][8]
   int32_t* e_ptr = nullptr;  // [nedge + 1]
