@@ -16,8 +16,10 @@
 //   k_dual_volume  one lane per node: sums its SCVs in element order.
 // The reference adds element contributions into the edge and node fields
 // with atomics; here the incidence lists built once per mesh fix the order
-// of every sum, so the result is reproducible and is the same sum, in the
-// same order, as the host producers in host/nw_mesh.cpp and nw_mixed.cpp.
+// of every sum, so the result is reproducible: the area vectors are the
+// same sums, in the same order, as the host producers in host/nw_mesh.cpp
+// and nw_mixed.cpp, and the volumes agree with them to rounding (the Hex8
+// sub-volumes are summed by faces in element-centred coordinates here).
 //
 // Each SCS is a quad of four points of the subdivided element and its area
 // vector is taken over four triangles about the quad's centroid (a Pyr5
