@@ -384,6 +384,10 @@ __global__ __launch_bounds__(kBlock) void k_sell_struct(
 int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
                hipStream_t st, bool pad_last, bool offsets)
 {
+  // the callers pass a new structure or one a failed build left part way
+  for (void* p : {(void*)s.sp, (void*)s.ci, (void*)s.slot, (void*)s.dia_w, (void*)s.dia_off})
+    (void)hipFree(p);
+  s = Sell();
   s.n = n;
   s.nslices = (n + kWave - 1) / kWave;
   const int ns = s.nslices;

@@ -285,7 +285,8 @@ __global__ void k_sell_values(const int32_t* __restrict__ slot,
 }
 
 // Builds the sliced ELL structure of the CSR graph (rp, ci; device arrays)
-// with n rows.  Padding positions read the row's own index as column
+// with n rows into s, which must be new or left by a failed build (its
+// arrays are released first).  Padding positions read the row's own index as column
 // (square matrices), or with pad_last the row's last column, which keeps
 // them inside the columns of a rectangular matrix.  With offsets (square
 // matrices only) the slices that qualify are stored in offset form.
