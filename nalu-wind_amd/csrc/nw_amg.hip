@@ -284,18 +284,85 @@ __global__ __launch_bounds__(kBlock) void k_pow_scale(double* __restrict__ y,
   }
 }
 
+// The fine level's filtered operator F of the smoothed-aggregation setup
+// (host/nw_amg_setup.cpp, coarsen): entry (i, j != i) is kept when
+// a_ij^2 > theta^2 |a_ii a_jj| and neither row is a Dirichlet row (no
+// nonzero off-diagonal); the others are lumped into the diagonal.  Per
+// row: a_ii and the Dirichlet flag.
+__global__ void k_filter_diag(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                              const double* __restrict__ a, int n, double* __restrict__ d,
+                              int8_t* __restrict__ dir)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double di = 0.0;
+  int8_t only_diag = 1;
+  for (int k = rp[i]; k < rp[i + 1]; ++k) {
+    if (ci[k] == i)
+      di = a[k];
+    else if (a[k] != 0.0)
+      only_diag = 0;
+  }
+  d[i] = di;
+  dir[i] = only_diag;
+}
+
+__device__ inline bool filter_keep(int i, int j, double aij, const double* d,
+                                   const int8_t* dir, double th2)
+{
+  return j == i || (!dir[i] && !dir[j] && aij * aij > th2 * fabs(d[i] * d[j]));
+}
+
+// F's diagonal a_ii + (sum of the row's dropped entries, in row order) and
+// its inverse (0 for a zero diagonal)
+__global__ void k_filter_lump(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                              const double* __restrict__ a, int n, const double* __restrict__ d,
+                              const int8_t* __restrict__ dir, double th2,
+                              double* __restrict__ fd, double* __restrict__ fdinv)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double lump = 0.0;
+  for (int k = rp[i]; k < rp[i + 1]; ++k)
+    if (!filter_keep(i, ci[k], a[k], d, dir, th2)) lump += a[k];
+  const double f = d[i] + lump;
+  fd[i] = f;
+  fdinv[i] = f != 0.0 ? 1.0 / f : 0.0;
+}
+
+// y = F x over the kept entries in row order (the diagonal as F's)
+__global__ void k_filter_spmv(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                              const double* __restrict__ a, int n, const double* __restrict__ d,
+                              const int8_t* __restrict__ dir, double th2,
+                              const double* __restrict__ fd, const double* __restrict__ x,
+                              double* __restrict__ y)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = rp[i]; k < rp[i + 1]; ++k) {
+    const int j = ci[k];
+    if (j == i)
+      s += fd[i] * x[j];
+    else if (filter_keep(i, j, a[k], d, dir, th2))
+      s += a[k] * x[j];
+  }
+  y[i] = s;
+}
+
 }  // namespace
 
-// lambda_max(D^-1 A) of an uploaded level by `iters` power-iteration steps
-// from a fixed start vector: each step normalises x, forms y = D^-1 A x and
-// the Rayleigh quotient x.y with the next norm y.y; block partial sums are
-// added on the host in block order, so the estimate is reproducible.  Uses
-// the level's x and t vectors as work space.
+// lambda_max(D^-1 A) by `iters` power-iteration steps from a fixed start
+// vector (the hashed one, or the host array `start`; spmv(x, y) forms
+// y = A x; dinv = 1 / diag): each step normalises
+// x, forms y = D^-1 A x and the Rayleigh quotient x.y with the next norm
+// y.y; block partial sums are added on the host in block order, so the
+// estimate is reproducible.  x and y are n-vectors of work space.
 template <typename M>
-static int device_lmax(DevLevel<double>& L, const M& spmv, int iters, hipStream_t st,
-                       double* out)
+static int device_lmax(int n, double* x, double* y, const double* dinv, const M& spmv,
+                       int iters, hipStream_t st, double* out,
+                       const double* start = nullptr)
 {
-  const int n = L.n;
   const unsigned nb = grid_for(n, kBlock);
   DevScratch tmp;
   double* part = nullptr;
@@ -313,8 +380,10 @@ static int device_lmax(DevLevel<double>& L, const M& spmv, int iters, hipStream_
     }
     return NW_OK;
   };
-  double *x = L.x, *y = L.t;
-  k_pow_start<<<nb, kBlock, 0, st>>>(x, n);
+  if (start)
+    NW_HIP(hipMemcpyAsync(x, start, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  else
+    k_pow_start<<<nb, kBlock, 0, st>>>(x, n);
   k_pow_scale<<<nb, kBlock, 0, st>>>(x, nullptr, nullptr, n, part);
   NW_LAUNCH_CHECK();
   double lam = 1.0, nx2 = 0.0, unused = 0.0;
@@ -325,7 +394,7 @@ static int device_lmax(DevLevel<double>& L, const M& spmv, int iters, hipStream_
     if (!(nx > 0)) break;
     k_pow_div<<<nb, kBlock, 0, st>>>(x, nx, n);
     spmv(x, y);
-    k_pow_scale<<<nb, kBlock, 0, st>>>(y, L.dinv, x, n, part);
+    k_pow_scale<<<nb, kBlock, 0, st>>>(y, dinv, x, n, part);
     NW_LAUNCH_CHECK();
     rc = sums(lam, nx2);
     if (rc) return rc;
@@ -517,6 +586,37 @@ int amg_setup(Ctx* c, AmgDevice** out)
   NW_HIP(hipStreamSynchronize(c->stream));
   amg::Options o;
   o.smoother_lmax = false;  // estimated on the device after the upload
+  {
+    // the fine prolongator's damping estimate, lambda_max(D_F^-1 F), on
+    // the device from the matrix there
+    const int n = (int)g.nnode;
+    const unsigned nb = grid_for(n, kBlock);
+    DevScratch tmp;
+    double *d = nullptr, *fd = nullptr, *fdinv = nullptr, *x = nullptr, *y = nullptr;
+    int8_t* dir = nullptr;
+    int arc = tmp.alloc(&d, n);
+    if (!arc) arc = tmp.alloc(&fd, n);
+    if (!arc) arc = tmp.alloc(&fdinv, n);
+    if (!arc) arc = tmp.alloc(&x, n);
+    if (!arc) arc = tmp.alloc(&y, n);
+    if (!arc) arc = tmp.alloc(&dir, n);
+    NW_REQUIRE(arc == NW_OK, NW_E_ALLOC, "AMG setup: device allocation failed");
+    const double th2 = o.drop_tol * o.drop_tol;
+    k_filter_diag<<<nb, kBlock, 0, c->stream>>>(g.row_ptr, g.cols, g.vals, n, d, dir);
+    k_filter_lump<<<nb, kBlock, 0, c->stream>>>(g.row_ptr, g.cols, g.vals, n, d, dir, th2, fd,
+                                                fdinv);
+    NW_LAUNCH_CHECK();
+    double lf = 0.0;
+    // from the host setup's start vector, so that the estimate is the one
+    // the host would make, up to the order of the dot-product sums
+    const std::vector<double> x0 = amg::power_start(n);
+    const int prc = device_lmax(n, x, y, fdinv, [&](const double* xv, double* yv) {
+      k_filter_spmv<<<nb, kBlock, 0, c->stream>>>(g.row_ptr, g.cols, g.vals, n, d, dir, th2, fd,
+                                                  xv, yv);
+    }, o.power_iters, c->stream, &lf, x0.data());
+    if (prc) return prc;
+    o.filtered_lmax0 = lf;
+  }
   amg::Hierarchy h;
   const int rc = amg::build(std::move(A), o, h);
   NW_REQUIRE(rc == 0, NW_E_ARG, "AMG setup failed (%d): coarse level too "
@@ -541,7 +641,7 @@ int amg_setup(Ctx* c, AmgDevice** out)
   }
   for (int l = 0; !urc && l + 1 < a->nlev; ++l) {
     DevLevel<double>& L = a->d.lv[l];
-    urc = device_lmax(L, [&](const double* x, double* y) {
+    urc = device_lmax(L.n, L.x, L.t, L.dinv, [&](const double* x, double* y) {
       mat_spmv<double>(L.A, x, y, 1.0, c->stream);
     }, o.power_iters, c->stream, &L.lmax);
   }

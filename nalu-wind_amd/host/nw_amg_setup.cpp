@@ -28,6 +28,23 @@ namespace amg {
 
 namespace {
 
+}  // namespace
+
+std::vector<double> power_start(int64_t n)
+{
+  std::vector<double> x(n);
+  uint64_t z = 0x2545f4914f6cdd1dULL;
+  for (int64_t i = 0; i < n; ++i) {
+    z ^= z << 13;
+    z ^= z >> 7;
+    z ^= z << 17;
+    x[i] = 0.5 + (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  }
+  return x;
+}
+
+namespace {
+
 // Sums run over fixed chunks of kChunk elements in parallel and then over
 // the chunks in order, so a reduction gives the same bits for any thread
 // count.
@@ -55,14 +72,7 @@ double dot(const std::vector<double>& a, const std::vector<double>& b)
 double power_lmax(const Csr& A, const std::vector<double>& dinv, int iters)
 {
   const int64_t n = A.nrows, nc = (n + kChunk - 1) / kChunk;
-  std::vector<double> x(n), y(n);
-  uint64_t z = 0x2545f4914f6cdd1dULL;
-  for (int64_t i = 0; i < n; ++i) {
-    z ^= z << 13;
-    z ^= z >> 7;
-    z ^= z << 17;
-    x[i] = 0.5 + (double)(z >> 11) * (1.0 / 9007199254740992.0);
-  }
+  std::vector<double> x = power_start(n), y(n);
   std::vector<double> pxy(nc), pyy(nc);
   double lam = 1.0, nx2 = dot(x, x);
   for (int it = 0; it < iters; ++it) {
@@ -247,8 +257,9 @@ int invert_dense(int64_t n, const Csr& A, std::vector<double>& inv)
   return 0;
 }
 
-// one coarsening step: P and R for level A
-void coarsen(const Csr& A, const Options& o, Csr& P)
+// one coarsening step: P and R for level A (lf_given > 0: the damping's
+// eigenvalue estimate, else estimated here)
+void coarsen(const Csr& A, const Options& o, Csr& P, double lf_given)
 {
   const int64_t n = A.nrows;
   std::vector<double> diag(n, 0.0);
@@ -340,7 +351,7 @@ void coarsen(const Csr& A, const Options& o, Csr& P)
   for (int32_t a = 0; a < na; ++a) sq[a] = std::sqrt((double)size[a]);
   // omega from lambda_max(D_F^-1 A_F)
   const std::vector<double> dfinv = inv_diagonal(F);
-  const double lf = power_lmax(F, dfinv, o.power_iters);
+  const double lf = lf_given > 0 ? lf_given : power_lmax(F, dfinv, o.power_iters);
   const double omega = (4.0 / 3.0) / (lf > 0 ? lf : 1.0);
   // P = P0 - omega D_F^-1 A_F P0, one row per node: a row's terms (its
   // aggregate's indicator and a smoothing term per F entry into an
@@ -426,7 +437,7 @@ int build(Csr A0, const Options& o, Hierarchy& h)
     if (o.smoother_lmax) L.lmax = power_lmax(L.A, L.dinv, o.power_iters);
     if (n <= o.coarse_max || l + 1 >= o.max_levels) break;
     Csr P;
-    coarsen(L.A, o, P);
+    coarsen(L.A, o, P, l == 0 ? o.filtered_lmax0 : 0.0);
     if (P.ncols == 0 || P.ncols > 0.9 * n) break;
     Csr R = transpose(P);
     Csr AP = spgemm(L.A, P);

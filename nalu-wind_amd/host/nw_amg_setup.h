@@ -47,6 +47,10 @@ struct Csr {
   int64_t nnz() const { return rp.empty() ? 0 : rp.back(); }
 };
 
+// The start vector of the setup's power iterations (a fixed xorshift
+// sequence in [0.5, 1.5)).
+std::vector<double> power_start(int64_t n);
+
 // A CSR matrix of n rows with the given structure (rp [n + 1], ci); its
 // values are left for the caller to fill.  Copies in parallel.
 Csr csr_with_structure(int64_t n, const int32_t* rp, const int32_t* ci);
@@ -70,6 +74,9 @@ struct Options {
   // estimate where the operators live (the prolongator damping still uses
   // the host's estimate on the filtered operator)
   bool smoother_lmax = true;
+  // > 0: lambda_max(D_F^-1 F) of the fine level's filtered operator, as
+  // estimated by the caller, used for the fine prolongator's damping
+  double filtered_lmax0 = 0.0;
 };
 
 struct Hierarchy {
