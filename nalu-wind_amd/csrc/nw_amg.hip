@@ -61,11 +61,13 @@ struct DevMat {
   const int64_t* sp = nullptr;
   const int32_t* rp = nullptr;
   const int32_t* ci = nullptr;
-  const int32_t* dia_w = nullptr;  // offset form of the fine level's slices
-  const int32_t* dia_off = nullptr;
+  const int32_t* dia_w = nullptr;  // offset and delta forms of the fine
+  const int32_t* dia_off = nullptr;  // level's slices
+  const int8_t* d16 = nullptr;
+  const int16_t* cd = nullptr;
   T* v = nullptr;
   int64_t nv = 0;  // stored values (padding included)
-  SellView view() const { return {sp, ci, dia_w, dia_off}; }
+  SellView view() const { return {sp, ci, dia_w, dia_off, d16, cd}; }
   int64_t* own_sp = nullptr;
   int32_t* own_rp = nullptr;
   int32_t* own_ci = nullptr;
@@ -509,6 +511,8 @@ static int upload_hier(Ctx* c, const amg::Hierarchy& h, DevHier<double>& H)
       L.A.ci = c->g.sell.ci;
       L.A.dia_w = c->g.sell.dia_w;
       L.A.dia_off = c->g.sell.dia_off;
+      L.A.d16 = c->g.sell.d16;
+      L.A.cd = c->g.sell.cd;
       L.A.nv = c->g.sell.len;
       NW_HIP(hipMalloc(&L.A.v, sizeof(double) * std::max<int64_t>(c->g.sell.len, 1)));
     } else {
@@ -549,6 +553,8 @@ static int narrow_hier(Ctx* c, const DevHier<double>& D, DevHier<float>& H)
       L.A.ci = S.A.ci;
       L.A.dia_w = S.A.dia_w;
       L.A.dia_off = S.A.dia_off;
+      L.A.d16 = S.A.d16;
+      L.A.cd = S.A.cd;
       L.A.nv = S.A.nv;
       NW_HIP(hipMalloc(&L.A.v, sizeof(float) * std::max<int64_t>(S.A.nv, 1)));
     } else {

@@ -102,6 +102,8 @@ static void free_graph(Graph& g)
   (void)hipFree(g.sell.slot);
   (void)hipFree(g.sell.dia_w);
   (void)hipFree(g.sell.dia_off);
+  (void)hipFree(g.sell.d16);
+  (void)hipFree(g.sell.cd);
   (void)hipFree(g.skip);
   g = Graph();
 }

@@ -93,6 +93,11 @@ struct alignas(32) SchedChunk {
 // entry there), so that a product computes its columns as row + offset
 // instead of loading them.  ci still holds those columns, for readers
 // that do not use the offsets.  dia_w[k] = 0 for slices in the plain form.
+//
+// Delta form (square matrices numbered for a narrow band, e.g. by reverse
+// Cuthill-McKee): a slice not in offset form whose columns all lie within
+// 32767 of their rows also keeps cd[p] = column - row as 16-bit integers
+// (d16[k] = 1), read instead of ci.
 constexpr int kDiaMax = 16;
 struct Sell {
   int n = 0, nslices = 0;
@@ -103,6 +108,9 @@ struct Sell {
   int32_t* dia_w = nullptr;     // [nslices], null when not built
   int32_t* dia_off = nullptr;   // [nslices][kDiaMax]
   int64_t dia_slices = 0;       // slices in offset form
+  int8_t* d16 = nullptr;        // [nslices], null when not built
+  int16_t* cd = nullptr;        // [len] column - row on slices with d16[k]
+  int64_t d16_slices = 0;       // slices in delta form
 };
 
 // What the sliced ELL products read of a structure.
@@ -111,6 +119,8 @@ struct SellView {
   const int32_t* ci;
   const int32_t* dia_w;    // null: no slice in offset form
   const int32_t* dia_off;
+  const int8_t* d16;       // null: no slice in delta form
+  const int16_t* cd;
 };
 
 struct Graph {

@@ -379,13 +379,39 @@ __global__ __launch_bounds__(kBlock) void k_sell_struct(
   }
 }
 
+// The delta form of each slice not in offset form (a wave per slice):
+// when every column of its rows lies within 32767 of the row, cd[p] =
+// column - row and d16[k] = 1; else d16[k] = 0.
+__global__ __launch_bounds__(kBlock) void k_sell_delta16(
+  const int64_t* __restrict__ sp, const int32_t* __restrict__ sci, int n, int nslices,
+  const int32_t* __restrict__ dia_w, int16_t* __restrict__ cd, int8_t* __restrict__ d16)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t >> 6;
+  if (k >= nslices) return;  // whole waves
+  const int64_t b = sp[k] + (t & 63);
+  const int w = (int)((sp[k + 1] - sp[k]) >> 6);
+  bool fits = true;
+  for (int j = 0; j < w && t < n; ++j) {
+    const int d = sci[b + 64 * (int64_t)j] - t;
+    fits = fits && d >= -32767 && d <= 32767;
+  }
+  const bool ok = (dia_w == nullptr || dia_w[k] == 0) && __ballot(!fits) == 0;
+  for (int j = 0; j < w; ++j) {
+    const int64_t p = b + 64 * (int64_t)j;
+    cd[p] = ok && t < n ? (int16_t)(sci[p] - t) : (int16_t)0;
+  }
+  if ((t & 63) == 0) d16[k] = ok ? 1 : 0;
+}
+
 }  // namespace
 
 int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
                hipStream_t st, bool pad_last, bool offsets)
 {
   // the callers pass a new structure or one a failed build left part way
-  for (void* p : {(void*)s.sp, (void*)s.ci, (void*)s.slot, (void*)s.dia_w, (void*)s.dia_off})
+  for (void* p : {(void*)s.sp, (void*)s.ci, (void*)s.slot, (void*)s.dia_w, (void*)s.dia_off,
+                  (void*)s.d16, (void*)s.cd})
     (void)hipFree(p);
   s = Sell();
   s.n = n;
@@ -434,6 +460,23 @@ int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
   k_sell_struct<<<grid, kBlock, 0, st>>>(rp, ci, n, s.sp, ns, s.ci, s.slot, pad_last,
                                          s.dia_w, s.dia_off);
   NW_LAUNCH_CHECK();
+  if (offsets) {
+    NW_HIP(hipMalloc(&s.d16, sizeof(int8_t) * std::max(ns, 1)));
+    NW_HIP(hipMalloc(&s.cd, sizeof(int16_t) * std::max<int64_t>(s.len, 1)));
+    k_sell_delta16<<<grid, kBlock, 0, st>>>(s.sp, s.ci, n, ns, s.dia_w, s.cd, s.d16);
+    NW_LAUNCH_CHECK();
+    std::vector<int8_t> d16(ns);
+    NW_HIP(hipMemcpyAsync(d16.data(), s.d16, ns, hipMemcpyDeviceToHost, st));
+    NW_HIP(hipStreamSynchronize(st));
+    s.d16_slices = 0;
+    for (int k = 0; k < ns; ++k) s.d16_slices += d16[k];
+    if (s.d16_slices == 0) {  // nothing to gain
+      (void)hipFree(s.d16);
+      (void)hipFree(s.cd);
+      s.d16 = nullptr;
+      s.cd = nullptr;
+    }
+  }
   NW_HIP(hipStreamSynchronize(st));
   return NW_OK;
 }

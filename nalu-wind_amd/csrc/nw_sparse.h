@@ -162,7 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_resid_cheb_init(
 // column with value 0.  The row sum runs over j in order.  A slice in
 // offset form (A.dia_w[k] > 0) takes its columns as row + the slice's
 // offsets, which are the same for the whole wave (scalar loads), and reads
-// no column array.
+// no column array; a slice in delta form (A.d16[k]) reads 16-bit column -
+// row deltas instead of the 32-bit columns.
 template <typename T>
 __device__ inline T sell_dot(const SellView& A, const T* __restrict__ va,
                              const T* __restrict__ x, int row)
@@ -181,6 +182,21 @@ __device__ inline T sell_dot(const SellView& A, const T* __restrict__ va,
       for (int u = 0; u < 8; ++u) {
         const int j = min(j0 + u, w - 1);
         c[u] = row + off[j];
+        v[u] = j0 + u < w ? va[b + 64 * (int64_t)j] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u] * x[c[u]];
+    }
+    return s;
+  }
+  if (A.d16 && A.d16[k]) {
+    for (int j0 = 0; j0 < w; j0 += 8) {
+      int c[8];
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = min(j0 + u, w - 1);
+        c[u] = row + A.cd[b + 64 * (int64_t)j];
         v[u] = j0 + u < w ? va[b + 64 * (int64_t)j] : (T)0;
       }
 #pragma unroll
@@ -289,11 +305,16 @@ __global__ void k_sell_values(const int32_t* __restrict__ slot,
 // arrays are released first).  Padding positions read the row's own index as column
 // (square matrices), or with pad_last the row's last column, which keeps
 // them inside the columns of a rectangular matrix.  With offsets (square
-// matrices only) the slices that qualify are stored in offset form.
+// matrices only) the slices that qualify are stored in offset form, and
+// the others whose columns lie within 32767 of their rows get 16-bit
+// deltas.
 int sell_build(const int32_t* rp, const int32_t* ci, int n, Sell& s,
                hipStream_t st, bool pad_last = false, bool offsets = false);
 
-inline SellView sell_view(const Sell& s) { return {s.sp, s.ci, s.dia_w, s.dia_off}; }
+inline SellView sell_view(const Sell& s)
+{
+  return {s.sp, s.ci, s.dia_w, s.dia_off, s.d16, s.cd};
+}
 
 // launch helper: dispatch a G-templated row kernel by lanes-per-row
 // (kernel<T, G>; NW_ROW_LAUNCH_ADD: kernel<T, G, true>)
