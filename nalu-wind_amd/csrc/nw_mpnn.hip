@@ -534,7 +534,7 @@ __global__ void k_apply_guess(const double* __restrict__ xb,
   if (dir[i] != 0.f) {
     x[i] = b[i] / va[diag[i]];
   } else {
-    x[i] = xb[i] + so * (double)delta[i];
+    x[i] = delta ? xb[i] + so * (double)delta[i] : xb[i];  // null: a zero head
   }
 }
 
@@ -1030,6 +1030,9 @@ struct Mpnn {
   double so = 1.0;         // output scale of the last features
   double* xb = nullptr;    // [n] base guess of the last features
   double last_ms = 0.0;
+  // the output head (w_out, b_out) is all zero: the prediction is the base
+  // guess exactly, and the message-passing layers need not run
+  bool head_zero = true;
   int precision = NW_MPNN_BF16;  // arithmetic of the prediction
   int loss_kind = NW_MPNN_LOSS_RESIDUAL;  // training loss
   float* vd = nullptr;        // [n] residual loss: D^-2 A u
@@ -1074,6 +1077,8 @@ int upload_params(Mpnn* m)
   Off o(H, L);
   std::vector<float> f(m->np), t((size_t)H * kF);
   for (int64_t i = 0; i < m->np; ++i) f[i] = (float)m->P[i];
+  m->head_zero = true;
+  for (int64_t i = o.wout; i < o.bout + 2; ++i) m->head_zero = m->head_zero && m->P[i] == 0.0;
   for (int j = 0; j < H; ++j)
     for (int k = 0; k < kF; ++k) t[(size_t)k * H + j] = f[o.win + j * kF + k];
   // bfloat16 weights of the fused layers in the MFMA B-operand layout:
@@ -1480,12 +1485,16 @@ int mpnn_predict_device(Mpnn* m, const double* const* hist, int nh, double* x)
   if (rc) return rc;
   rc = features(m, hist, nh, nullptr, m->feat, nullptr);
   if (rc) return rc;
-  rc = forward_predict(m, m->feat);
-  if (rc) return rc;
+  // a zero output head multiplies the layers' output by zero: the guess is
+  // the base guess, without running them
+  if (!m->head_zero) {
+    rc = forward_predict(m, m->feat);
+    if (rc) return rc;
+  }
   const Graph& g = m->c->g;
   k_apply_guess<<<grid_for(m->n, kBlock), kBlock, 0, m->c->stream>>>(
-    m->xb, m->delta, m->feat + 2 * m->n, m->so, g.rhs, g.vals, g.diag, x,
-    (int)m->n);
+    m->xb, m->head_zero ? nullptr : m->delta, m->feat + 2 * m->n, m->so, g.rhs, g.vals,
+    g.diag, x, (int)m->n);
   NW_LAUNCH_CHECK();
   NW_HIP(hipStreamSynchronize(m->c->stream));
   m->last_ms = 1e3 * std::chrono::duration<double>(
