@@ -319,7 +319,8 @@ def test_mixed_160_cubed_full_size(nw, wl, host_threads):
     geometry against the oracle (to 1e-13 of the largest area vector: the
     two sum an edge's sub-control-surface terms in different element
     orders), its dual volumes tiling the unit box and its area vectors
-    closing around every interior node."""
+    closing around every interior node; the AMG pressure solves on it,
+    checked by their residual recomputed on the host."""
     W = wl.continuity_mixed(160, keep_elements=True)
     assert (W.n, W.ne) == (4328769, 19200110)
     f = W.fields
@@ -342,6 +343,23 @@ def test_mixed_160_cubed_full_size(nw, wl, host_threads):
         assert D.nnz == 0 or abs(D).max() <= 1e-14 * abs(S).max()
         _conservation(S, got[3], 1)
         del S, D, got, A
+        # the pressure solve on this mesh (a reverse Cuthill-McKee
+        # numbering: the fine level in 16-bit delta form), FP64 and
+        # FP32-refined, each residual recomputed here from the downloaded
+        # system with its Dirichlet rows
+        wl.apply_pressure_bc(ctx, W)
+        rp, cols, vals, b = ctx.download()
+        M = sp.csr_matrix((vals, cols, rp), shape=(W.n, W.n))
+        bn = np.linalg.norm(b)
+        for mode in (nw.SOLVE_FP64, nw.SOLVE_FP32_REFINE):
+            st = ctx.solve(nw.solver_cfg(kspace=50, max_iters=500, tol=1e-8,
+                                         precond=nw.PREC_AMG, mode=mode, inner_tol=1e-4,
+                                         reuse_preconditioner=1))
+            rel = np.linalg.norm(b - M @ ctx.solution()) / bn
+            print("mixed 160^3 solve mode %d: %d iterations, %.1f ms, host residual %.3e"
+                  % (mode, st.iters, st.ms, rel), flush=True)
+            assert st.converged and rel <= 1.0001e-8
+        del M, rp, cols, vals, b
         ctx.mesh_elements(W.types, W.conn)
         ctx.compute_geometry()
         a = ctx.field_download(nw.E_AREA_VECTOR)
